@@ -1,0 +1,168 @@
+"""Benchmark of the fused MaterKit time step on one MI355X.
+
+Workload: D3Q19 TRT flow with Boussinesq buoyancy + D3Q7 thermal + D3Q19 anisotropic phase
+field on a 256^3 grid, one solid seed of radius 8 at the centre (``cases/
+fused_flow_thermal_dendrite_256.xml``); FP32 deviation storage.  A step is one pass of the
+phase moment kernel and one pass of the fused collide-stream kernel over all cells.  The FP64
+path is timed on the same workload for the FP32/FP64 ratio, and the FP64 CPU oracle (same
+step, OpenMP) on a bounded slab of the same workload as the CPU baseline.
+
+Prints one JSON line (rank 0).  Only one GPU is supported (--gpus > 1 exits).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell update of each kernel (each population / scalar read once and
+# written once; flags read once; phase value read for the gradient stencil once per cell).
+#   fused collide-stream: f 19 r+w, h 7 r+w, g 19 r+w, phi^n r, phi^{n-1} r, flag r
+#   phase moment pass:    g 19 r, phi^n w, flag r
+def alg_bytes(esz: int, fq=19, pq=19, th=7) -> tuple[int, int]:
+    fused = 2 * esz * (fq + th + pq) + 2 * esz + 1
+    moments = esz * pq + esz + 1
+    return fused, moments
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--fp64-steps", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def run_device(prec, size, steps, warmup):
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+
+    case = cases.load("fused256", precision=prec, nx=size, ny=size, nz=size)
+    dev = mkg.Solver(case.params)
+    cases.setup(dev, case)
+    dev.step(warmup, mkg.ALL)
+    dev.sync()
+    torch.cuda.synchronize()
+    dev.enable_timing(True)
+    t0 = time.perf_counter()
+    dev.step(steps, mkg.ALL)
+    dev.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    tim = dev.last_timing()
+    nonfinite = dev.reduce(mkg.R_NONFINITE)[0]
+    solid = dev.reduce(mkg.R_SOLID)[0]
+    dev.close()
+    return case.params, wall, tim, nonfinite, solid
+
+
+def cpu_baseline(seconds: float):
+    """FP64 oracle (OpenMP) on a 256 x 256 x 16 slab of the same workload."""
+    import numpy as np
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1))))
+    case = cases.load("fused256", precision=mkg.PREC_FP64, nz=16)
+    p = case.params
+    st = O.State.empty(p)
+    # same initial fields as the device (seed sphere cut by the slab)
+    c, r, _ = case.seeds
+    st.init_seeds(c, r)
+    f = cases.initial_fields(case)
+    st.init_equilibrium(T=f["T"])
+    N = p.ncell
+    st.step(1, 15, O.OP_TRT)  # warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        st.step(1, 15, O.OP_TRT)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": round(N * n / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle, fused flow+thermal+phase step, {p.nx}x{p.ny}x{p.nz} slab "
+                      f"of the 256^3 workload, {n} steps in {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
+
+    size = a.size
+    N = size ** 3
+    p32, wall32, tim32, nf32, solid32 = run_device(mkg.PREC_FP32, size, a.steps, a.warmup)
+    p64, wall64, tim64, nf64, _ = run_device(mkg.PREC_FP64, size, a.fp64_steps, 3)
+
+    b32, m32 = alg_bytes(4)
+    b64, m64 = alg_bytes(8)
+    mlups32 = N * a.steps / wall32 / 1e6
+    mlups64 = N * a.fp64_steps / wall64 / 1e6
+    k32 = tim32["fused_ms"] / tim32["steps"] * 1e-3  # s per fused launch
+    k64 = tim64["fused_ms"] / tim64["steps"] * 1e-3
+    km32 = tim32["phase_moments_ms"] / tim32["steps"] * 1e-3
+    ach32 = b32 * N / k32 / 1e9
+    ach64 = b64 * N / k64 / 1e9
+    step_bytes32 = (b32 + m32) * N
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_fused256.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("fused_bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "MLUPS (fused D3Q19-TRT + D3Q7 + D3Q19 phase-field step, 256^3)",
+        "value": round(mlups32, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall32 / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "fused flow(D3Q19 TRT)+thermal(D3Q7)+phase field(D3Q19), "
+                               f"{size}^3, one seed r=8, FP32 deviation storage",
+                   "cells": N},
+        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "k_step<float,19,19,thermal> (fused collide-stream)",
+                     "alg_bytes_per_cell": b32},
+        "step_hbm_frac": round(step_bytes32 / (wall32 / a.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {"fused": round(k32 * 1e3, 4), "phase_moments": round(km32 * 1e3, 4)},
+        "fp64": {"value": round(mlups64, 1), "unit": "MLUPS",
+                 "fused_kernel_gbs": round(ach64, 1),
+                 "frac": round(ach64 / HBM_PEAK_GBS, 4), "alg_bytes_per_cell": b64},
+        "fp32_over_fp64": round(mlups32 / mlups64, 3),
+        "adjoint_over_forward": None,
+        "nonfinite": nf32 + nf64,
+        "solid_fraction": solid32,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

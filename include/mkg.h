@@ -1,0 +1,214 @@
+/*
+ * mkg.h -- C ABI of the MI355X (gfx950) lattice-Boltzmann / phase-field time step.
+ *
+ * This is the drop-in boundary for MaterKit's per-step solver calls.  Every entry point is
+ * plain C (extern "C"), takes plain pointers and sizes, returns an int status (0 = ok) and
+ * never throws; the text of the last error is available from mkg_last_error().
+ *
+ * The handle owns all device memory (structure-of-arrays populations, phase field, flags,
+ * per-cell conductivity).  Calls are issued on one HIP stream per handle; get/reduce calls
+ * synchronise that stream.  A handle is not thread-safe; separate handles are independent.
+ *
+ * Reference interfaces replaced (MaterKit, all bodies of which live in a closed DLL):
+ *   mkg_params_from_xml   <- mkFluidsLBM::read_parameters / mkFluidsMRT::initialize(xml)
+ *                            (MaterKit/inc/mkFluidsLBM.h:41-43, mkFluidsMRT.h:39),
+ *                            readParameters (MaterKit/inc/mkFunction.h:62)
+ *   mkg_create            <- mkFluidsMRT(const mkSpace&) + node[] allocation
+ *                            (MaterKit/inc/mkFluidsMRT.h:37, mkFluidsLBM.h:95)
+ *   mkg_init_geometry     <- initialize_geometry / initialize_geometry_Poiseuille /
+ *                            initialize_seeds (mkFluidsMRT.h:40, mkFlowField.h:33,
+ *                            mkMultiphaseField.h:52, MaterKit/src/mkFluidsUser.cpp:44-66)
+ *   mkg_step (mask)       <- per-step call sequence propagate()/propagate_PBB()/relaxation()
+ *                            of every field (MaterKit/MaterKit.cpp:277-280, 500-518,
+ *                            775-781; mkFluidsLBM.h:47-49; mkPhaseField.h:41-48;
+ *                            mkThermalField.h:33-35; mkSolutalField.h:35-37)
+ *   face boundary codes   <- bdyVelocity / bdy_RBC / bdy_flowin / bdy_constT / bdy_LBflowin
+ *                            (mkFluidsLBM.h:58-65, mkThermalField.h:37-40,
+ *                            mkSolutalField.h:51-53), applied inside the step
+ *   mkg_get_field         <- node[index].{density, vel, phi, temperature, flag, ...}
+ *                            (MaterKit/inc/mkNode.h:134-198)
+ *   mkg_reduce            <- Thermal_NusseltRBC / Solutal_NusseltRBC / check_variables
+ *                            (mkFluidsLBM.h:55,67-68)
+ *   mkg_checkpoint_*      <- (no counterpart; the reference writes VTK/Tecplot frames only)
+ *   mkg_adjoint_*         <- (no counterpart; discrete adjoint of the step)
+ */
+#ifndef MKG_H
+#define MKG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MKG_ABI_VERSION 1
+
+/* ---- node flags: identical codes to MaterKit/inc/mkGlobalDefs.h:140-145 ---- */
+#define MKG_IS_OUTLET (-2)
+#define MKG_IS_INLET (-1)
+#define MKG_IS_FLUID 0
+#define MKG_IS_INTER 1
+#define MKG_IS_SOLID 2
+#define MKG_IS_WALLS 3
+
+/* ---- lattices / operators / precision ---- */
+#define MKG_OP_BGK 0
+#define MKG_OP_TRT 1
+#define MKG_OP_MRT 2
+#define MKG_PREC_FP64 0
+#define MKG_PREC_FP32 1 /* FP32 populations stored as deviations p_i - w_i * ref */
+
+/* ---- face boundary codes (faces ordered x-, x+, y-, y+, z-, z+) ---- */
+#define MKG_BC_PERIODIC 0
+#define MKG_BC_BOUNCE 1    /* flow: half-way bounce-back, moving wall / inlet with u_face;
+                              scalar: zero flux */
+#define MKG_BC_DIRICHLET 2 /* scalar only: anti-bounce-back to value s_face */
+#define MKG_BC_OUTFLOW 3   /* unknown incoming populations copied from the cell itself */
+
+/* ---- lattice masks for mkg_step ---- */
+#define MKG_FLOW 1
+#define MKG_PHASE 2
+#define MKG_THERMAL 4
+#define MKG_SOLUTE 8
+
+/* ---- field ids for get/set ---- */
+#define MKG_F_POP_FLOW 0    /* Q x N  pre-collision f_i at the current time level */
+#define MKG_F_POP_PHASE 1   /* Qg x N */
+#define MKG_F_POP_THERMAL 2 /* 7 x N */
+#define MKG_F_POP_SOLUTE 3  /* 7 x N */
+#define MKG_F_RHO 4         /* N */
+#define MKG_F_VEL 5         /* 3 x N, Guo half-force velocity */
+#define MKG_F_TEMP 6        /* N */
+#define MKG_F_CONC 7        /* N, the conserved solute variable c = A(phi) U */
+#define MKG_F_PHI 8         /* N, phase field at the last phase moment pass */
+#define MKG_F_PHI_PREV 9    /* N */
+#define MKG_F_FLAG 10       /* N int8 */
+#define MKG_F_GRAIN 11      /* N int16 */
+#define MKG_F_KAPPA 12      /* N, per-cell conductivity multiplier */
+
+#define MKG_DT_F64 0
+#define MKG_DT_F32 1
+#define MKG_DT_I8 2
+#define MKG_DT_I16 3
+
+/* ---- reductions (FP64 on the device) ---- */
+#define MKG_R_MASS 0      /* sum rho */
+#define MKG_R_ENERGY 1    /* sum T */
+#define MKG_R_SOLUTE 2    /* sum c */
+#define MKG_R_SOLID 3     /* mean solid fraction (1+phi)/2 */
+#define MKG_R_PHI 4       /* sum phi */
+#define MKG_R_NUSSELT 5   /* 1 + <u_z T>/(alpha dT / H) for z-vertical Rayleigh-Benard */
+#define MKG_R_MOMENTUM 6  /* out[0..2] = sum rho u */
+#define MKG_R_UMAX 7      /* max |u| */
+#define MKG_R_NONFINITE 8 /* number of non-finite populations */
+#define MKG_R_TIP 9       /* out[0..2]: furthest phi>=0 extent from the domain centre, +x,+y,+z */
+
+#define MKG_MAX_GRAINS 64
+
+typedef struct mkg_params {
+    int32_t nx, ny, nz;   /* cells; linear index (x*ny + y)*nz + z, z contiguous */
+    int32_t flow_q;       /* 0 (off), 19 or 27 */
+    int32_t phase_q;      /* 0 (off), 7 or 19 */
+    int32_t thermal;      /* 0/1: D3Q7 thermal ADE */
+    int32_t solute;       /* 0/1: D3Q7 solutal ADE */
+    int32_t precision;    /* MKG_PREC_* */
+    int32_t flow_op;      /* MKG_OP_* (informational; the rates below define it) */
+    int32_t pf_coupling;  /* phase-field driving variable: 0 none, 1 temperature, 2 solute U */
+    int32_t ngrains;      /* 0: no rotation; else per-cell grain ids index rot[] (1-based) */
+    int32_t has_kappa;    /* per-cell thermal conductivity multiplier present */
+    int32_t bc_f[6], bc_g[6], bc_h[6], bc_c[6]; /* per lattice, per face */
+    int32_t pad0;
+    /* flow */
+    double s_nu;          /* shear rate = 1/tau */
+    double s_bulk;        /* bulk (trace) rate */
+    double s_odd;         /* odd ghost-moment rate (TRT: omega-) */
+    double s_even;        /* even ghost-moment rate */
+    double rho0;
+    double force[3];      /* constant body force density */
+    double buoy[3];       /* buoyancy: F += rho0 * buoy * (T - T_ref) */
+    double T_ref;
+    double u_face[6][3];  /* wall / inlet velocity per face */
+    /* thermal */
+    double alpha_l, alpha_s; /* lattice diffusivities in liquid / solid */
+    double latent_T;         /* source w_i * latent_T * (phi^n - phi^{n-1}) */
+    double T_face[6];
+    /* solute */
+    double D_l, D_s, k_part, latent_C, q_min;
+    double C_face[6];
+    /* phase field (Karma-Rappel units W0 = tau0 = 1 scaled by dx, dt) */
+    double pf_dt;         /* dt / tau0 */
+    double pf_wdx2;       /* W0^2 / dx^2 */
+    double eps4;
+    double lambda;
+    double phi_s;         /* flag threshold: phi >= phi_s solid, > -phi_s interface */
+    /* deviation references for FP32 storage */
+    double ref_f, ref_g, ref_h, ref_c;
+    double rot[MKG_MAX_GRAINS][9]; /* row-major lab<-crystal rotation per grain */
+} mkg_params;
+
+typedef struct mkg_ctx* mkg_handle;
+
+int mkg_abi_version(void);
+size_t mkg_params_size(void);
+const char* mkg_last_error(void);
+
+/* Fill p with neutral defaults (everything off, periodic, rho0 = 1, phi_s = 0.99). */
+int mkg_params_default(mkg_params* p);
+
+/* Read a BoltzTechs XML case (the reference schema plus an optional <solver> block) and derive
+ * lattice-unit parameters.  run_info (optional, 4 ints): numLoops, numFrames, seed count, 0. */
+int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info);
+
+/* Seeds of a case file: explicit <phasechange><seed> entries (ZYX Euler angles) or
+ * <solver><seedsRandom> count radius rngseed (uniform quaternions).  Writes up to max
+ * seeds -- centres (3n), radii (n), row-major rotations (9n, may be NULL) -- and returns the
+ * count, or -1. */
+int mkg_seeds_from_xml(const char* path, int max, double* centers, double* radii, double* rot);
+
+int mkg_create(const mkg_params* p, mkg_handle* out);
+int mkg_destroy(mkg_handle h);
+int mkg_get_params(mkg_handle h, mkg_params* p);
+/* Update scalar parameters (rates, forces, eps4, lambda...) of a live handle; grid, lattices
+ * and precision must not change. */
+int mkg_set_params(mkg_handle h, const mkg_params* p);
+
+/* Initial state from macroscopic fields (host arrays of N doubles, NULL = uniform default):
+ * equilibrium populations, phase buffers, flags from phi (walls kept). */
+int mkg_init_equilibrium(mkg_handle h, const double* rho, const double* vel3, const double* T,
+                         const double* C, const double* phi);
+
+/* Seeds: centres (3*n, lattice units), radii (n), grain per seed = index+1.  Sets phi
+ * (tanh profile, interface width W0/dx), grain ids (nearest seed surface), flags. */
+int mkg_init_seeds(mkg_handle h, int n, const double* centers, const double* radii);
+
+/* Random seed placement for multi-dendrite cases: uniform centres in [r, dim-1-r] and uniform
+ * unit quaternions (Shoemake), both drawn from a splitmix64 stream of `seed`. */
+int mkg_random_seeds(uint64_t seed, int n, double radius, int nx, int ny, int nz,
+                     double* centers, double* quats);
+/* Quaternion (w,x,y,z) to row-major rotation matrix. */
+int mkg_quat_to_rot(const double* q, double* R);
+
+/* Run nsteps time steps advancing the lattices in mask (others are read for coupling only).
+ * A single fused kernel pair per step is used when all advanced lattices share a parity. */
+int mkg_step(mkg_handle h, int nsteps, int mask);
+int mkg_sync(mkg_handle h);
+int64_t mkg_step_count(mkg_handle h, int lattice_bit);
+
+int mkg_get_field(mkg_handle h, int field, void* host, int dtype, size_t count);
+int mkg_set_field(mkg_handle h, int field, const void* host, int dtype, size_t count);
+
+int mkg_reduce(mkg_handle h, int which, double* out);
+
+int mkg_checkpoint_save(mkg_handle h, const char* path);
+int mkg_checkpoint_load(mkg_handle h, const char* path);
+
+/* Device time (ms) of the last mkg_step call, per kernel class: out[0] = phase moment pass,
+ * out[1] = fused collide-stream, out[2] = total; out[3] = number of steps timed. */
+int mkg_last_timing(mkg_handle h, double* out);
+int mkg_enable_timing(mkg_handle h, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MKG_H */

@@ -1,0 +1,758 @@
+// C-ABI layer of the MI355X time step (see include/mkg.h for the contract and the reference
+// interfaces each entry point replaces).  Owns device memory and the HIP stream of a handle,
+// tracks the AA layout of every lattice, and dispatches the fused kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mkg.h"
+#include "kernels.h"
+#include "lattice.h"
+
+using namespace mkg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+    g_err = msg;
+    return -1;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+enum { LF = 0, LG = 1, LH = 2, LC = 3 };
+
+}  // namespace
+
+namespace mkg {
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace mkg
+
+struct mkg_ctx {
+    mkg_params p;
+    int fp32 = 0;
+    size_t esz = 8;
+    uint32_t N = 0, S = 0, nblk = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    void* pop[4] = {nullptr, nullptr, nullptr, nullptr};
+    int q[4] = {0, 0, 0, 0};
+    int lay[4] = {0, 0, 0, 0};
+    int64_t steps[4] = {0, 0, 0, 0};
+    void* phi[2] = {nullptr, nullptr};
+    int64_t pf_count = 0;
+    int8_t* flag = nullptr;
+    int16_t* grain = nullptr;
+    void* kappa = nullptr;
+    double* rot = nullptr;
+    double* scratch = nullptr;  // 27*N doubles, allocated on first use
+    double* partials = nullptr;
+    double* red_out = nullptr;
+    int has_walls = 0;
+    int timing = 0;
+    std::vector<hipEvent_t> events;
+    double t_last[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int alloc_zero(void** p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) {
+        *p = nullptr;
+        return 0;
+    }
+    HIPCHK(hipMalloc(p, bytes));
+    HIPCHK(hipMemsetAsync(*p, 0, bytes, s));
+    return 0;
+}
+
+int ensure_scratch(mkg_ctx* h) {
+    if (h->scratch) return 0;
+    HIPCHK(hipMalloc((void**)&h->scratch, (size_t)27 * h->N * sizeof(double)));
+    return 0;
+}
+
+bool combo_supported(int fq, int pq, int th, int so) {
+    static const int table[][4] = {{19, 0, 0, 0},  {27, 0, 0, 0}, {19, 0, 1, 0}, {27, 0, 1, 0},
+                                   {0, 19, 1, 0},  {0, 7, 1, 0},  {19, 19, 1, 0}, {19, 7, 1, 0},
+                                   {27, 19, 1, 0}, {19, 19, 0, 1}, {19, 7, 0, 1}, {0, 19, 0, 1},
+                                   {0, 0, 1, 0},   {0, 0, 0, 1},  {0, 19, 0, 0},  {0, 7, 0, 0}};
+    for (auto& t : table)
+        if (t[0] == fq && t[1] == pq && t[2] == th && t[3] == so) return true;
+    return false;
+}
+
+void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const double* fval,
+              LatArgs& out) {
+    std::memset(&out, 0, sizeof(out));
+    out.pop = h->pop[L];
+    int np = 0;
+    for (int f = 0; f < 6; f++) {
+        out.bc[f] = bc[f];
+        if (bc[f] != MKG_BC_PERIODIC) np |= 1 << f;
+        out.fval[f] = fval ? fval[f] : 0.0;
+    }
+    out.npmask = np;
+    out.lay = h->lay[L];
+    out.adv = 0;
+    out.ref = ref;
+}
+
+StepArgs make_args(const mkg_ctx* h) {
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const mkg_params& p = h->p;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.N = h->N;
+    a.S = h->S;
+    a.nblk = h->nblk;
+    a.has_walls = h->has_walls;
+    a.has_kappa = (p.has_kappa && h->kappa) ? 1 : 0;
+    a.ngrains = p.ngrains;
+    a.pf_coupling = p.pf_coupling;
+    fill_lat(h, LF, p.bc_f, p.ref_f, nullptr, a.f);
+    fill_lat(h, LG, p.bc_g, p.ref_g, nullptr, a.g);
+    fill_lat(h, LH, p.bc_h, p.ref_h, p.T_face, a.h);
+    fill_lat(h, LC, p.bc_c, p.ref_c, p.C_face, a.c);
+    a.phi_cur = h->phi[h->pf_count & 1];
+    a.phi_prev = h->phi[(h->pf_count + 1) & 1];
+    a.flag = h->flag;
+    a.grain = h->grain;
+    a.kappa = h->kappa;
+    a.rot = h->rot;
+    a.s_nu = p.s_nu;
+    a.s_bulk = p.s_bulk;
+    a.s_odd = p.s_odd;
+    a.s_even = p.s_even;
+    a.rho0 = p.rho0;
+    for (int k = 0; k < 3; k++) {
+        a.force[k] = p.force[k];
+        a.buoy[k] = p.buoy[k];
+    }
+    a.T_ref = p.T_ref;
+    std::memcpy(a.u_face, p.u_face, sizeof(a.u_face));
+    a.alpha_l = p.alpha_l;
+    a.alpha_s = p.alpha_s;
+    a.latent_T = p.latent_T;
+    a.D_l = p.D_l;
+    a.D_s = p.D_s;
+    a.k_part = p.k_part;
+    a.latent_C = p.latent_C;
+    a.q_min = p.q_min;
+    a.pf_dt = p.pf_dt;
+    a.pf_wdx2 = p.pf_wdx2;
+    a.eps4 = p.eps4;
+    a.lambda = p.lambda;
+    a.phi_s = p.phi_s;
+    const double cs2g = (p.phase_q == 7) ? 0.25 : 1.0 / 3.0;
+    a.omega_g = 1.0 / (0.5 + p.pf_dt * p.pf_wdx2 / cs2g);
+    return a;
+}
+
+int kind_of(int L) { return L == LF ? 0 : 1; }
+
+const int32_t* bc_of(const mkg_params& p, int L) {
+    return L == LF ? p.bc_f : (L == LG ? p.bc_g : (L == LH ? p.bc_h : p.bc_c));
+}
+
+// Rewrite one lattice into the own-slot layout (lay 0).
+int normalize(mkg_ctx* h, int L) {
+    if (!h->pop[L] || h->lay[L] == 0) return 0;
+    if (ensure_scratch(h)) return -1;
+    StepArgs a = make_args(h);
+    LatArgs* la = L == LF ? &a.f : (L == LG ? &a.g : (L == LH ? &a.h : &a.c));
+    if (launch_canonical(h->q[L], h->fp32, a, *la, kind_of(L), h->scratch, h->stream))
+        return fail("normalize: bad lattice");
+    HIPCHK(hipGetLastError());
+    if (launch_store_pops(h->q[L], h->fp32, a, *la, h->scratch, h->stream))
+        return fail("normalize: bad lattice");
+    HIPCHK(hipGetLastError());
+    h->lay[L] = 0;
+    return 0;
+}
+
+int validate(const mkg_params* p) {
+    if (!p) return fail("params is null");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail("grid dimensions must be >= 1");
+    double n = (double)p->nx * p->ny * p->nz;
+    if (n > 2.0e9) return fail("grid too large for 32-bit cell indices");
+    if (p->flow_q != 0 && p->flow_q != 19 && p->flow_q != 27)
+        return fail("flow_q must be 0, 19 or 27");
+    if (p->phase_q != 0 && p->phase_q != 7 && p->phase_q != 19)
+        return fail("phase_q must be 0, 7 or 19");
+    if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
+        return fail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
+    if (p->ngrains < 0 || p->ngrains > MKG_MAX_GRAINS) return fail("ngrains out of range");
+    if (!combo_supported(p->flow_q, p->phase_q, p->thermal ? 1 : 0, p->solute ? 1 : 0))
+        return fail("unsupported combination of lattices (flow_q, phase_q, thermal, solute)");
+    const int32_t* bcs[4] = {p->bc_f, p->bc_g, p->bc_h, p->bc_c};
+    for (int L = 0; L < 4; L++)
+        for (int f = 0; f < 6; f++)
+            if (bcs[L][f] < 0 || bcs[L][f] > 3) return fail("face boundary code out of range");
+    for (int a = 0; a < 3; a++) {
+        // a face pair must be periodic on both sides or on neither
+        for (int L = 0; L < 4; L++)
+            if ((bcs[L][2 * a] == 0) != (bcs[L][2 * a + 1] == 0))
+                return fail("periodic faces must come in pairs");
+    }
+    return 0;
+}
+
+int copy_out(const double* dsrc, void* host, int dtype, size_t count, hipStream_t s) {
+    if (dtype == MKG_DT_F64) {
+        HIPCHK(hipMemcpyAsync(host, dsrc, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    }
+    if (dtype == MKG_DT_F32) {
+        std::vector<double> tmp(count);
+        HIPCHK(hipMemcpyAsync(tmp.data(), dsrc, count * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        float* o = (float*)host;
+        for (size_t i = 0; i < count; i++) o[i] = (float)tmp[i];
+        return 0;
+    }
+    return fail("dtype must be MKG_DT_F64 or MKG_DT_F32 for this field");
+}
+
+int copy_in(double* ddst, const void* host, int dtype, size_t count, hipStream_t s) {
+    if (dtype == MKG_DT_F64) {
+        HIPCHK(hipMemcpyAsync(ddst, host, count * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    }
+    if (dtype == MKG_DT_F32) {
+        std::vector<double> tmp(count);
+        const float* i = (const float*)host;
+        for (size_t k = 0; k < count; k++) tmp[k] = (double)i[k];
+        HIPCHK(hipMemcpyAsync(ddst, tmp.data(), count * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return 0;
+    }
+    return fail("dtype must be MKG_DT_F64 or MKG_DT_F32 for this field");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mkg_abi_version(void) { return MKG_ABI_VERSION; }
+size_t mkg_params_size(void) { return sizeof(mkg_params); }
+const char* mkg_last_error(void) { return g_err.c_str(); }
+
+int mkg_params_default(mkg_params* p) {
+    if (!p) return fail("params is null");
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 1;
+    p->rho0 = 1.0;
+    p->s_nu = p->s_bulk = p->s_odd = p->s_even = 1.0;
+    p->phi_s = 0.99;
+    p->q_min = 1e-3;
+    p->latent_T = 0.5;
+    p->latent_C = 0.5;
+    p->k_part = 1.0;
+    p->ref_f = 1.0;
+    for (int k = 0; k < MKG_MAX_GRAINS; k++)
+        for (int i = 0; i < 3; i++) p->rot[k][4 * i] = 1.0;
+    return 0;
+}
+
+int mkg_create(const mkg_params* p, mkg_handle* out) {
+    if (!out) return fail("out is null");
+    *out = nullptr;
+    if (validate(p)) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail("no HIP device available");
+    mkg_ctx* h = new mkg_ctx();
+    h->p = *p;
+    h->fp32 = p->precision == MKG_PREC_FP32;
+    h->esz = h->fp32 ? 4 : 8;
+    h->N = (uint32_t)((size_t)p->nx * p->ny * p->nz);
+    h->S = (h->N + 127u) & ~127u;
+    h->nblk = (h->N + 255u) / 256u;
+    h->q[LF] = p->flow_q;
+    h->q[LG] = p->phase_q;
+    h->q[LH] = p->thermal ? 7 : 0;
+    h->q[LC] = p->solute ? 7 : 0;
+    auto cleanup = [&](int rc) {
+        mkg_destroy(h);
+        return rc;
+    };
+    if (hipGetDevice(&h->dev) != hipSuccess) return cleanup(fail("hipGetDevice failed"));
+    if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail("hipStreamCreate failed"));
+    for (int L = 0; L < 4; L++)
+        if (alloc_zero(&h->pop[L], (size_t)h->q[L] * h->S * h->esz, h->stream)) return cleanup(-1);
+    if (p->phase_q) {
+        for (int b = 0; b < 2; b++)
+            if (alloc_zero(&h->phi[b], (size_t)h->S * h->esz, h->stream)) return cleanup(-1);
+    }
+    if (alloc_zero((void**)&h->flag, h->S, h->stream)) return cleanup(-1);
+    if (alloc_zero((void**)&h->grain, (size_t)h->S * 2, h->stream)) return cleanup(-1);
+    if (p->has_kappa) {
+        if (alloc_zero(&h->kappa, (size_t)h->S * h->esz, h->stream)) return cleanup(-1);
+        // default conductivity multiplier 1
+        std::vector<double> ones(h->N, 1.0);
+        double* tmp = nullptr;
+        if (hipMalloc((void**)&tmp, h->N * sizeof(double)) != hipSuccess)
+            return cleanup(fail("hipMalloc failed"));
+        hipMemcpyAsync(tmp, ones.data(), h->N * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        launch_from_double(h->fp32, tmp, h->kappa, h->N, h->stream);
+        hipStreamSynchronize(h->stream);
+        hipFree(tmp);
+    }
+    if (alloc_zero((void**)&h->rot, sizeof(p->rot), h->stream)) return cleanup(-1);
+    if (hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream) !=
+        hipSuccess)
+        return cleanup(fail("rotation upload failed"));
+    if (alloc_zero((void**)&h->partials, (size_t)(h->nblk + 1024) * 4 * sizeof(double),
+                   h->stream))
+        return cleanup(-1);
+    if (alloc_zero((void**)&h->red_out, 16 * sizeof(double), h->stream)) return cleanup(-1);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail("init sync failed"));
+    *out = h;
+    return 0;
+}
+
+int mkg_destroy(mkg_handle h) {
+    if (!h) return 0;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (int L = 0; L < 4; L++) hipFree(h->pop[L]);
+    hipFree(h->phi[0]);
+    hipFree(h->phi[1]);
+    hipFree(h->flag);
+    hipFree(h->grain);
+    hipFree(h->kappa);
+    hipFree(h->rot);
+    hipFree(h->scratch);
+    hipFree(h->partials);
+    hipFree(h->red_out);
+    for (auto e : h->events) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int mkg_get_params(mkg_handle h, mkg_params* p) {
+    if (!h || !p) return fail("null argument");
+    *p = h->p;
+    return 0;
+}
+
+int mkg_set_params(mkg_handle h, const mkg_params* p) {
+    if (!h || !p) return fail("null argument");
+    if (validate(p)) return -1;
+    const mkg_params& o = h->p;
+    if (p->nx != o.nx || p->ny != o.ny || p->nz != o.nz || p->flow_q != o.flow_q ||
+        p->phase_q != o.phase_q || p->thermal != o.thermal || p->solute != o.solute ||
+        p->precision != o.precision || (p->has_kappa && !h->kappa))
+        return fail("set_params cannot change grid, lattices, precision or kappa storage");
+    if (p->ref_f != o.ref_f || p->ref_g != o.ref_g || p->ref_h != o.ref_h || p->ref_c != o.ref_c)
+        return fail("set_params cannot change deviation references of live populations");
+    h->p = *p;
+    HIPCHK(hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_init_equilibrium(mkg_handle h, const double* rho, const double* vel3, const double* T,
+                         const double* C, const double* phi) {
+    if (!h) return fail("null handle");
+    if (ensure_scratch(h)) return -1;
+    const size_t N = h->N;
+    double* d = h->scratch;
+    double *drho = nullptr, *du = nullptr, *dT = nullptr, *dC = nullptr, *dphi = nullptr;
+    if (rho) { drho = d; HIPCHK(hipMemcpyAsync(drho, rho, N * 8, hipMemcpyHostToDevice, h->stream)); }
+    if (vel3) { du = d + N; HIPCHK(hipMemcpyAsync(du, vel3, 3 * N * 8, hipMemcpyHostToDevice, h->stream)); }
+    if (T) { dT = d + 4 * N; HIPCHK(hipMemcpyAsync(dT, T, N * 8, hipMemcpyHostToDevice, h->stream)); }
+    if (C) { dC = d + 5 * N; HIPCHK(hipMemcpyAsync(dC, C, N * 8, hipMemcpyHostToDevice, h->stream)); }
+    if (phi) { dphi = d + 6 * N; HIPCHK(hipMemcpyAsync(dphi, phi, N * 8, hipMemcpyHostToDevice, h->stream)); }
+    if (phi && !h->p.phase_q) return fail("phi given but the phase lattice is off");
+    StepArgs a = make_args(h);
+    launch_init_eq(h->fp32, h->q[LF], h->q[LG], a, drho, du, dT, dC, dphi, h->phi[0], h->phi[1],
+                   h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->lay[LF] = h->lay[LH] = h->lay[LC] = 0;
+    if (phi) h->lay[LG] = 0;
+    return 0;
+}
+
+int mkg_init_seeds(mkg_handle h, int n, const double* centers, const double* radii) {
+    if (!h) return fail("null handle");
+    if (!h->p.phase_q) return fail("seeds need the phase lattice");
+    if (n < 1 || !centers || !radii) return fail("need at least one seed");
+    if (ensure_scratch(h)) return -1;
+    double* dc = h->scratch + (size_t)h->N;
+    double* dr = dc + 3 * (size_t)n;
+    HIPCHK(hipMemcpyAsync(dc, centers, 3 * n * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dr, radii, n * 8, hipMemcpyHostToDevice, h->stream));
+    const double width = std::sqrt(2.0) * std::sqrt(h->p.pf_wdx2);
+    const double ts = std::atanh(h->p.phi_s) * width;
+    StepArgs a = make_args(h);
+    launch_init_seeds(h->fp32, h->q[LG], a, n, dc, dr, width, ts, h->phi[0], h->phi[1],
+                      h->scratch, h->grain, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->lay[LG] = 0;
+    return 0;
+}
+
+int mkg_step(mkg_handle h, int nsteps, int mask) {
+    if (!h) return fail("null handle");
+    if (nsteps < 0) return fail("nsteps must be >= 0");
+    int adv[4] = {(mask & MKG_FLOW) && h->q[LF], (mask & MKG_PHASE) && h->q[LG],
+                  (mask & MKG_THERMAL) && h->q[LH], (mask & MKG_SOLUTE) && h->q[LC]};
+    if (!(adv[0] || adv[1] || adv[2] || adv[3])) return 0;
+    Combo k{h->q[LF], h->q[LG], h->q[LH] ? 1 : 0, h->q[LC] ? 1 : 0};
+    double tk[2] = {0, 0};
+    size_t need_ev = h->timing ? (size_t)3 * nsteps : 0;
+    while (h->events.size() < need_ev) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        h->events.push_back(e);
+    }
+    for (int n = 0; n < nsteps; n++) {
+        // advanced lattices must share one layout
+        int lay0 = -1;
+        bool mixed = false;
+        for (int L = 0; L < 4; L++)
+            if (adv[L]) {
+                if (lay0 < 0) lay0 = h->lay[L];
+                else if (h->lay[L] != lay0) mixed = true;
+            }
+        if (mixed) {
+            for (int L = 0; L < 4; L++)
+                if (adv[L] && normalize(h, L)) return -1;
+            lay0 = 0;
+        }
+        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n], h->stream));
+        if (adv[LG]) {
+            h->pf_count++;
+            StepArgs a = make_args(h);
+            if (launch_phase_moments(h->q[LG], h->fp32, lay0, a, h->phi[h->pf_count & 1],
+                                     h->stream))
+                return fail("phase moment kernel unavailable");
+        }
+        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n + 1], h->stream));
+        StepArgs a = make_args(h);
+        a.f.adv = adv[LF];
+        a.g.adv = adv[LG];
+        a.h.adv = adv[LH];
+        a.c.adv = adv[LC];
+        if (launch_step(k, h->fp32, lay0, a, h->stream))
+            return fail("no kernel instantiation for this lattice combination");
+        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n + 2], h->stream));
+        for (int L = 0; L < 4; L++)
+            if (adv[L]) {
+                h->lay[L] ^= 1;
+                h->steps[L]++;
+            }
+    }
+    HIPCHK(hipGetLastError());
+    if (h->timing && nsteps > 0) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int n = 0; n < nsteps; n++) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, h->events[3 * n], h->events[3 * n + 1]));
+            HIPCHK(hipEventElapsedTime(&b, h->events[3 * n + 1], h->events[3 * n + 2]));
+            tk[0] += a;
+            tk[1] += b;
+        }
+        h->t_last[0] = tk[0];
+        h->t_last[1] = tk[1];
+        h->t_last[2] = tk[0] + tk[1];
+        h->t_last[3] = nsteps;
+    }
+    return 0;
+}
+
+int mkg_sync(mkg_handle h) {
+    if (!h) return fail("null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int64_t mkg_step_count(mkg_handle h, int bit) {
+    if (!h) return -1;
+    int L = bit == MKG_FLOW ? LF : bit == MKG_PHASE ? LG : bit == MKG_THERMAL ? LH : bit == MKG_SOLUTE ? LC : -1;
+    return L < 0 ? -1 : h->steps[L];
+}
+
+int mkg_enable_timing(mkg_handle h, int on) {
+    if (!h) return fail("null handle");
+    h->timing = on ? 1 : 0;
+    return 0;
+}
+
+int mkg_last_timing(mkg_handle h, double* out) {
+    if (!h || !out) return fail("null argument");
+    for (int k = 0; k < 4; k++) out[k] = h->t_last[k];
+    return 0;
+}
+
+int mkg_get_field(mkg_handle h, int field, void* host, int dtype, size_t count) {
+    if (!h || !host) return fail("null argument");
+    const size_t N = h->N;
+    if (field >= MKG_F_POP_FLOW && field <= MKG_F_POP_SOLUTE) {
+        int L = field - MKG_F_POP_FLOW;
+        if (!h->pop[L]) return fail("lattice not present");
+        if (count != (size_t)h->q[L] * N) return fail("count must be Q*N");
+        if (ensure_scratch(h)) return -1;
+        StepArgs a = make_args(h);
+        LatArgs* la = L == LF ? &a.f : (L == LG ? &a.g : (L == LH ? &a.h : &a.c));
+        launch_canonical(h->q[L], h->fp32, a, *la, kind_of(L), h->scratch, h->stream);
+        HIPCHK(hipGetLastError());
+        return copy_out(h->scratch, host, dtype, count, h->stream);
+    }
+    if (field == MKG_F_RHO || field == MKG_F_VEL || field == MKG_F_TEMP || field == MKG_F_CONC) {
+        if (field == MKG_F_RHO && !h->q[LF]) return fail("flow lattice not present");
+        if (field == MKG_F_TEMP && !h->q[LH]) return fail("thermal lattice not present");
+        if (field == MKG_F_CONC && !h->q[LC]) return fail("solute lattice not present");
+        size_t want = field == MKG_F_VEL ? 3 * N : N;
+        if (count != want) return fail("count mismatch");
+        if (ensure_scratch(h)) return -1;
+        double* d = h->scratch;
+        StepArgs a = make_args(h);
+        launch_macro(h->q[LF], h->fp32, a, field == MKG_F_RHO ? d : nullptr,
+                     field == MKG_F_VEL ? d : nullptr, field == MKG_F_TEMP ? d : nullptr,
+                     field == MKG_F_CONC ? d : nullptr, h->stream);
+        HIPCHK(hipGetLastError());
+        return copy_out(d, host, dtype, count, h->stream);
+    }
+    if (field == MKG_F_PHI || field == MKG_F_PHI_PREV || field == MKG_F_KAPPA) {
+        const void* src;
+        if (field == MKG_F_KAPPA) {
+            if (!h->kappa) return fail("kappa not allocated (has_kappa = 0)");
+            src = h->kappa;
+        } else {
+            if (!h->q[LG]) return fail("phase lattice not present");
+            src = field == MKG_F_PHI ? h->phi[h->pf_count & 1] : h->phi[(h->pf_count + 1) & 1];
+        }
+        if (count != N) return fail("count must be N");
+        if (ensure_scratch(h)) return -1;
+        launch_to_double(h->fp32, src, h->scratch, h->N, h->stream);
+        HIPCHK(hipGetLastError());
+        return copy_out(h->scratch, host, dtype, count, h->stream);
+    }
+    if (field == MKG_F_FLAG) {
+        if (dtype != MKG_DT_I8 || count != N) return fail("flags are N int8");
+        HIPCHK(hipMemcpyAsync(host, h->flag, N, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (field == MKG_F_GRAIN) {
+        if (dtype != MKG_DT_I16 || count != N) return fail("grain ids are N int16");
+        HIPCHK(hipMemcpyAsync(host, h->grain, N * 2, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    return fail("unknown field id");
+}
+
+int mkg_set_field(mkg_handle h, int field, const void* host, int dtype, size_t count) {
+    if (!h || !host) return fail("null argument");
+    const size_t N = h->N;
+    if (field >= MKG_F_POP_FLOW && field <= MKG_F_POP_SOLUTE) {
+        int L = field - MKG_F_POP_FLOW;
+        if (!h->pop[L]) return fail("lattice not present");
+        if (count != (size_t)h->q[L] * N) return fail("count must be Q*N");
+        if (ensure_scratch(h)) return -1;
+        if (copy_in(h->scratch, host, dtype, count, h->stream)) return -1;
+        StepArgs a = make_args(h);
+        LatArgs* la = L == LF ? &a.f : (L == LG ? &a.g : (L == LH ? &a.h : &a.c));
+        launch_store_pops(h->q[L], h->fp32, a, *la, h->scratch, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        h->lay[L] = 0;
+        return 0;
+    }
+    if (field == MKG_F_PHI || field == MKG_F_PHI_PREV || field == MKG_F_KAPPA) {
+        void* dst;
+        if (field == MKG_F_KAPPA) {
+            if (!h->kappa) return fail("kappa not allocated (has_kappa = 0)");
+            dst = h->kappa;
+        } else {
+            if (!h->q[LG]) return fail("phase lattice not present");
+            dst = field == MKG_F_PHI ? h->phi[h->pf_count & 1] : h->phi[(h->pf_count + 1) & 1];
+        }
+        if (count != N) return fail("count must be N");
+        if (ensure_scratch(h)) return -1;
+        if (copy_in(h->scratch, host, dtype, count, h->stream)) return -1;
+        launch_from_double(h->fp32, h->scratch, dst, h->N, h->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (field == MKG_F_FLAG) {
+        if (dtype != MKG_DT_I8 || count != N) return fail("flags are N int8");
+        const int8_t* fl = (const int8_t*)host;
+        bool walls = false;
+        for (size_t i = 0; i < N; i++) {
+            if (fl[i] < -2 || fl[i] > 3) return fail("flag value out of range");
+            if (fl[i] == MKG_IS_WALLS) walls = true;
+        }
+        HIPCHK(hipMemcpyAsync(h->flag, host, N, hipMemcpyHostToDevice, h->stream));
+        h->has_walls = walls ? 1 : 0;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (field == MKG_F_GRAIN) {
+        if (dtype != MKG_DT_I16 || count != N) return fail("grain ids are N int16");
+        const int16_t* g = (const int16_t*)host;
+        for (size_t i = 0; i < N; i++)
+            if (g[i] < 0 || g[i] > h->p.ngrains) return fail("grain id out of range");
+        HIPCHK(hipMemcpyAsync(h->grain, host, N * 2, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    return fail("field cannot be set directly (use mkg_init_equilibrium)");
+}
+
+int mkg_reduce(mkg_handle h, int which, double* out) {
+    if (!h || !out) return fail("null argument");
+    if (which < 0 || which > MKG_R_TIP) return fail("unknown reduction");
+    if (ensure_scratch(h)) return -1;
+    const size_t N = h->N;
+    if (which == MKG_R_NONFINITE) {
+        double total = 0;
+        for (int L = 0; L < 4; L++) {
+            if (!h->pop[L]) continue;
+            launch_nonfinite(h->fp32, h->pop[L], (size_t)h->q[L] * h->S, h->partials, h->red_out,
+                             h->stream);
+            HIPCHK(hipGetLastError());
+            double v = 0;
+            HIPCHK(hipMemcpyAsync(&v, h->red_out, 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            total += v;
+        }
+        out[0] = total;
+        return 0;
+    }
+    double* rho = h->scratch;
+    double* u = rho + N;
+    double* T = rho + 4 * N;
+    double* C = rho + 5 * N;
+    double* phi = rho + 6 * N;
+    bool need_flow = which == MKG_R_MASS || which == MKG_R_NUSSELT || which == MKG_R_MOMENTUM ||
+                     which == MKG_R_UMAX;
+    bool need_T = which == MKG_R_ENERGY || which == MKG_R_NUSSELT;
+    bool need_C = which == MKG_R_SOLUTE;
+    bool need_phi = which == MKG_R_SOLID || which == MKG_R_PHI || which == MKG_R_TIP;
+    if (need_flow && !h->q[LF]) return fail("reduction needs the flow lattice");
+    if (need_T && !h->q[LH]) return fail("reduction needs the thermal lattice");
+    if (need_C && !h->q[LC]) return fail("reduction needs the solute lattice");
+    if (need_phi && !h->q[LG]) return fail("reduction needs the phase lattice");
+    StepArgs a = make_args(h);
+    if (need_flow || need_T || need_C) {
+        launch_macro(h->q[LF], h->fp32, a, need_flow ? rho : nullptr, need_flow ? u : nullptr,
+                     need_T ? T : nullptr, need_C ? C : nullptr, h->stream);
+        HIPCHK(hipGetLastError());
+    }
+    if (need_phi) {
+        launch_to_double(h->fp32, h->phi[h->pf_count & 1], phi, h->N, h->stream);
+        HIPCHK(hipGetLastError());
+    }
+    int nout = 1;
+    launch_reduce(which, a, rho, u, T, C, phi, h->partials, h->red_out, &nout, h->stream);
+    HIPCHK(hipGetLastError());
+    double v[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(v, h->red_out, 4 * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (which == MKG_R_SOLID) v[0] /= (double)N;
+    if (which == MKG_R_NUSSELT) {
+        const double dT = h->p.T_face[4] - h->p.T_face[5];
+        const double H = h->p.nz;
+        v[0] = 1.0 + (v[0] / (double)N) * H / (h->p.alpha_l * dT);
+    }
+    for (int k = 0; k < nout; k++) out[k] = v[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------- checkpoints
+static const char kMagic[8] = {'M', 'K', 'G', 'C', 'K', 'P', 'T', '1'};
+
+int mkg_checkpoint_save(mkg_handle h, const char* path) {
+    if (!h || !path) return fail("null argument");
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(std::string("cannot open ") + path);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    auto wr = [&](const void* p, size_t n) { return std::fwrite(p, 1, n, fp) == n; };
+    uint64_t hdr[4] = {MKG_ABI_VERSION, sizeof(mkg_params), h->N, h->esz};
+    bool ok = wr(kMagic, 8) && wr(hdr, sizeof(hdr)) && wr(&h->p, sizeof(h->p)) &&
+              wr(h->lay, sizeof(h->lay)) && wr(h->steps, sizeof(h->steps)) &&
+              wr(&h->pf_count, sizeof(h->pf_count));
+    std::vector<char> buf;
+    auto dump = [&](const void* dptr, size_t bytes) {
+        if (!dptr || bytes == 0) return true;
+        buf.resize(bytes);
+        if (hipMemcpy(buf.data(), dptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        return wr(buf.data(), bytes);
+    };
+    for (int L = 0; L < 4 && ok; L++) ok = dump(h->pop[L], (size_t)h->q[L] * h->S * h->esz);
+    if (ok && h->q[LG]) ok = dump(h->phi[0], h->S * h->esz) && dump(h->phi[1], h->S * h->esz);
+    if (ok) ok = dump(h->flag, h->S) && dump(h->grain, (size_t)h->S * 2);
+    if (ok && h->kappa) ok = dump(h->kappa, h->S * h->esz);
+    std::fclose(fp);
+    if (!ok) return fail("checkpoint write failed");
+    return 0;
+}
+
+int mkg_checkpoint_load(mkg_handle h, const char* path) {
+    if (!h || !path) return fail("null argument");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(std::string("cannot open ") + path);
+    auto rd = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
+    char magic[8];
+    uint64_t hdr[4];
+    mkg_params p;
+    int lay[4];
+    int64_t steps[4], pfc;
+    bool ok = rd(magic, 8) && std::memcmp(magic, kMagic, 8) == 0 && rd(hdr, sizeof(hdr)) &&
+              hdr[0] == MKG_ABI_VERSION && hdr[1] == sizeof(mkg_params) && hdr[2] == h->N &&
+              hdr[3] == h->esz && rd(&p, sizeof(p)) && rd(lay, sizeof(lay)) &&
+              rd(steps, sizeof(steps)) && rd(&pfc, sizeof(pfc));
+    if (!ok) {
+        std::fclose(fp);
+        return fail("checkpoint header does not match this handle");
+    }
+    if (p.flow_q != h->p.flow_q || p.phase_q != h->p.phase_q || p.thermal != h->p.thermal ||
+        p.solute != h->p.solute || (p.has_kappa != 0) != (h->kappa != nullptr)) {
+        std::fclose(fp);
+        return fail("checkpoint lattices do not match this handle");
+    }
+    std::vector<char> buf;
+    auto load = [&](void* dptr, size_t bytes) {
+        if (!dptr || bytes == 0) return true;
+        buf.resize(bytes);
+        if (!rd(buf.data(), bytes)) return false;
+        return hipMemcpy(dptr, buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    for (int L = 0; L < 4 && ok; L++) ok = load(h->pop[L], (size_t)h->q[L] * h->S * h->esz);
+    if (ok && h->q[LG]) ok = load(h->phi[0], h->S * h->esz) && load(h->phi[1], h->S * h->esz);
+    if (ok) ok = load(h->flag, h->S) && load(h->grain, (size_t)h->S * 2);
+    if (ok && h->kappa) ok = load(h->kappa, h->S * h->esz);
+    std::fclose(fp);
+    if (!ok) return fail("checkpoint body truncated");
+    h->p = p;
+    std::memcpy(h->lay, lay, sizeof(lay));
+    std::memcpy(h->steps, steps, sizeof(steps));
+    h->pf_count = pfc;
+    HIPCHK(hipMemcpy(h->rot, p.rot, sizeof(p.rot), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,433 @@
+// Auxiliary kernels: layout-independent population read/write, macroscopic fields,
+// initial states, seed geometry and the FP64 diagnostic reductions.
+#include "device.h"
+
+namespace mkg {
+
+int launch_step_f32(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
+int launch_step_f64(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
+int launch_phase_moments_f32(int pq, int odd, const StepArgs& a, void* out, hipStream_t s);
+int launch_phase_moments_f64(int pq, int odd, const StepArgs& a, void* out, hipStream_t s);
+
+int launch_step(const Combo& k, int fp32, int odd, const StepArgs& a, hipStream_t s) {
+    return fp32 ? launch_step_f32(k, odd, a, s) : launch_step_f64(k, odd, a, s);
+}
+int launch_phase_moments(int pq, int fp32, int odd, const StepArgs& a, void* phi_out,
+                         hipStream_t s) {
+    return fp32 ? launch_phase_moments_f32(pq, odd, a, phi_out, s)
+                : launch_phase_moments_f64(pq, odd, a, phi_out, s);
+}
+
+static inline dim3 grid_of(const StepArgs& a) { return dim3(a.nblk); }
+
+// ------------------------------------------------------------------ canonical populations
+// Wall cells are never touched by a step, so their populations stay in their own slots.
+template <class R, int Q>
+__global__ void __launch_bounds__(256) k_canonical(const StepArgs a, const LatArgs L, int kind,
+                                                   double* __restrict__ out) {
+    Cell c;
+    if (!locate(a, c)) return;
+    R p[Q];
+    if (a.flag[c.id] == kIsWalls)
+        read_pops<Q, false, R>(p, L, a.S, c, a, kind);
+    else
+        read_pops_rt<Q, R>(p, L, a.S, c, a, kind);
+#pragma unroll
+    for (int i = 0; i < Q; i++) out[(size_t)i * a.N + c.id] = (double)p[i] + wgt<Q>(i) * L.ref;
+}
+
+template <class R, int Q>
+__global__ void __launch_bounds__(256) k_store(const StepArgs a, const LatArgs L,
+                                               const double* __restrict__ in) {
+    Cell c;
+    if (!locate(a, c)) return;
+    R* base = (R*)L.pop;
+#pragma unroll
+    for (int i = 0; i < Q; i++)
+        base[(size_t)i * a.S + c.id] = (R)(in[(size_t)i * a.N + c.id] - wgt<Q>(i) * L.ref);
+}
+
+template <class R>
+static int canonical_q(int q, const StepArgs& a, const LatArgs& L, int kind, double* out,
+                       hipStream_t s) {
+    if (q == 7) k_canonical<R, 7><<<grid_of(a), 256, 0, s>>>(a, L, kind, out);
+    else if (q == 19) k_canonical<R, 19><<<grid_of(a), 256, 0, s>>>(a, L, kind, out);
+    else if (q == 27) k_canonical<R, 27><<<grid_of(a), 256, 0, s>>>(a, L, kind, out);
+    else return -1;
+    return 0;
+}
+template <class R>
+static int store_q(int q, const StepArgs& a, const LatArgs& L, const double* in, hipStream_t s) {
+    if (q == 7) k_store<R, 7><<<grid_of(a), 256, 0, s>>>(a, L, in);
+    else if (q == 19) k_store<R, 19><<<grid_of(a), 256, 0, s>>>(a, L, in);
+    else if (q == 27) k_store<R, 27><<<grid_of(a), 256, 0, s>>>(a, L, in);
+    else return -1;
+    return 0;
+}
+
+int launch_canonical(int q, int fp32, const StepArgs& a, const LatArgs& L, int kind,
+                     double* out, hipStream_t s) {
+    return fp32 ? canonical_q<float>(q, a, L, kind, out, s)
+                : canonical_q<double>(q, a, L, kind, out, s);
+}
+int launch_store_pops(int q, int fp32, const StepArgs& a, const LatArgs& L, const double* in,
+                      hipStream_t s) {
+    return fp32 ? store_q<float>(q, a, L, in, s) : store_q<double>(q, a, L, in, s);
+}
+
+// ------------------------------------------------------------------ macroscopic fields
+template <class R, int Q>
+__device__ inline void read_any(R* p, const LatArgs& L, const StepArgs& a, const Cell& c,
+                                int kind, bool wall) {
+    if (wall)
+        read_pops<Q, false, R>(p, L, a.S, c, a, kind);
+    else
+        read_pops_rt<Q, R>(p, L, a.S, c, a, kind);
+}
+
+template <class R, int FQ>
+__global__ void __launch_bounds__(256) k_macro(const StepArgs a, double* rho, double* u,
+                                               double* T, double* C) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const bool wall = a.flag[c.id] == kIsWalls;
+    double Tv = 0;
+    if (a.h.pop) {
+        R h[7];
+        read_any<R, 7>(h, a.h, a, c, kKindScalar, wall);
+        R s = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) s += h[i];
+        Tv = a.h.ref + (double)s;
+        if (T) T[c.id] = Tv;
+    }
+    if (a.c.pop && C) {
+        R p[7];
+        read_any<R, 7>(p, a.c, a, c, kKindScalar, wall);
+        R s = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) s += p[i];
+        C[c.id] = a.c.ref + (double)s;
+    }
+    if constexpr (FQ > 0) {
+        R f[FQ];
+        read_any<R, FQ>(f, a.f, a, c, kKindFlow, wall);
+        R dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < FQ; i++) {
+            dr += f[i];
+            if (cx(i)) jx += (R)cx(i) * f[i];
+            if (cy(i)) jy += (R)cy(i) * f[i];
+            if (cz(i)) jz += (R)cz(i) * f[i];
+        }
+        const R rh = (R)a.f.ref + dr;
+        double F[3] = {a.force[0], a.force[1], a.force[2]};
+        if (a.h.pop)
+            for (int k = 0; k < 3; k++) F[k] += a.rho0 * a.buoy[k] * (Tv - a.T_ref);
+        if (rho) rho[c.id] = (double)rh;
+        if (u) {
+            u[3 * (size_t)c.id + 0] = ((double)jx + 0.5 * F[0]) / (double)rh;
+            u[3 * (size_t)c.id + 1] = ((double)jy + 0.5 * F[1]) / (double)rh;
+            u[3 * (size_t)c.id + 2] = ((double)jz + 0.5 * F[2]) / (double)rh;
+        }
+    } else if (u) {
+        u[3 * (size_t)c.id + 0] = 0;
+        u[3 * (size_t)c.id + 1] = 0;
+        u[3 * (size_t)c.id + 2] = 0;
+    }
+}
+
+template <class R>
+static int macro_r(int fq, const StepArgs& a, double* rho, double* u, double* T, double* C,
+                   hipStream_t s) {
+    if (fq == 0) k_macro<R, 0><<<grid_of(a), 256, 0, s>>>(a, rho, u, T, C);
+    else if (fq == 19) k_macro<R, 19><<<grid_of(a), 256, 0, s>>>(a, rho, u, T, C);
+    else if (fq == 27) k_macro<R, 27><<<grid_of(a), 256, 0, s>>>(a, rho, u, T, C);
+    else return -1;
+    return 0;
+}
+int launch_macro(int fq, int fp32, const StepArgs& a, double* rho, double* u, double* T,
+                 double* C, hipStream_t s) {
+    return fp32 ? macro_r<float>(fq, a, rho, u, T, C, s) : macro_r<double>(fq, a, rho, u, T, C, s);
+}
+
+// ------------------------------------------------------------------ conversions
+template <class R>
+__global__ void k_to_double(const R* __restrict__ src, double* __restrict__ dst, uint32_t n) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+template <class R>
+__global__ void k_from_double(const double* __restrict__ src, R* __restrict__ dst, uint32_t n) {
+    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) dst[i] = (R)src[i];
+}
+int launch_to_double(int fp32, const void* src, double* dst, uint32_t n, hipStream_t s) {
+    dim3 g((n + 255) / 256);
+    if (fp32) k_to_double<float><<<g, 256, 0, s>>>((const float*)src, dst, n);
+    else k_to_double<double><<<g, 256, 0, s>>>((const double*)src, dst, n);
+    return 0;
+}
+int launch_from_double(int fp32, const double* src, void* dst, uint32_t n, hipStream_t s) {
+    dim3 g((n + 255) / 256);
+    if (fp32) k_from_double<float><<<g, 256, 0, s>>>(src, (float*)dst, n);
+    else k_from_double<double><<<g, 256, 0, s>>>(src, (double*)dst, n);
+    return 0;
+}
+
+// ------------------------------------------------------------------ initial states
+// Scalar lattices at rest equilibrium w_i * value; phase buffers and flags from phi.
+template <class R>
+__global__ void __launch_bounds__(256) k_init_eq(const StepArgs a, const double* T,
+                                                 const double* C, const double* phi, R* phi0,
+                                                 R* phi1) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const uint32_t id = c.id;
+    if (a.g.pop && phi) {
+        const double ph = phi[id];
+        phi0[id] = (R)ph;
+        phi1[id] = (R)ph;
+        const int8_t fl = a.flag[id];
+        const int8_t nf = classify(fl, ph, a.phi_s);
+        if (nf != fl) a.flag[id] = nf;
+    }
+    if (a.h.pop) {
+        const double t = T ? T[id] : a.T_ref;
+        R* h = (R*)a.h.pop;
+#pragma unroll
+        for (int i = 0; i < 7; i++) h[(size_t)i * a.S + id] = (R)(wgt<7>(i) * (t - a.h.ref));
+    }
+    if (a.c.pop) {
+        const double cv = C ? C[id] : 0.0;
+        R* p = (R*)a.c.pop;
+#pragma unroll
+        for (int i = 0; i < 7; i++) p[(size_t)i * a.S + id] = (R)(wgt<7>(i) * (cv - a.c.ref));
+    }
+}
+
+// Equilibrium populations of a Q lattice from (rho, u): own-slot layout, deviations.
+template <class R, int Q>
+__global__ void __launch_bounds__(256) k_init_flow(const StepArgs a, const double* rho,
+                                                   const double* u) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const uint32_t id = c.id;
+    double ux = 0, uy = 0, uz = 0;
+    if (u) {
+        ux = u[3 * (size_t)id];
+        uy = u[3 * (size_t)id + 1];
+        uz = u[3 * (size_t)id + 2];
+    }
+    const double r = rho ? rho[id] : a.rho0;
+    const double uu = ux * ux + uy * uy + uz * uz;
+    R* f = (R*)a.f.pop;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const double cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+        const double feq = wgt<Q>(i) * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+        f[(size_t)i * a.S + id] = (R)(feq - wgt<Q>(i) * a.f.ref);
+    }
+}
+
+template <class R, int Q>
+__global__ void __launch_bounds__(256) k_init_phase(const StepArgs a, const double* phi) {
+    Cell c;
+    if (!locate(a, c)) return;
+    R* g = (R*)a.g.pop;
+    const double ph = phi[c.id];
+#pragma unroll
+    for (int i = 0; i < Q; i++) g[(size_t)i * a.S + c.id] = (R)(wgt<Q>(i) * (ph - a.g.ref));
+}
+
+template <class R>
+static int init_eq_r(const StepArgs& a, int fq, int pq, const double* rho, const double* u,
+                     const double* T, const double* C, const double* phi, R* phi0, R* phi1,
+                     hipStream_t s) {
+    k_init_eq<R><<<grid_of(a), 256, 0, s>>>(a, T, C, phi, phi0, phi1);
+    if (fq == 19) k_init_flow<R, 19><<<grid_of(a), 256, 0, s>>>(a, rho, u);
+    if (fq == 27) k_init_flow<R, 27><<<grid_of(a), 256, 0, s>>>(a, rho, u);
+    if (phi && pq == 19) k_init_phase<R, 19><<<grid_of(a), 256, 0, s>>>(a, phi);
+    if (phi && pq == 7) k_init_phase<R, 7><<<grid_of(a), 256, 0, s>>>(a, phi);
+    return 0;
+}
+
+// ------------------------------------------------------------------ seeds
+// phi = -tanh(d / width) with d the distance to the nearest seed surface; grain = that seed;
+// flags from d against +-ts = atanh(phi_s) * width so that they are exact integers of the
+// geometry (mkMultiphaseField::initialize_seeds, MaterKit/inc/mkMultiphaseField.h:52).
+template <class R>
+__global__ void __launch_bounds__(256) k_seeds(const StepArgs a, int n, const double* centers,
+                                               const double* radii, double width, double ts,
+                                               R* phi0, R* phi1, double* phid, int16_t* grain) {
+    Cell c;
+    if (!locate(a, c)) return;
+    double best = 1e300;
+    int gbest = 0;
+    for (int k = 0; k < n; k++) {
+        const double dx = c.x - centers[3 * k], dy = c.y - centers[3 * k + 1],
+                     dz = c.z - centers[3 * k + 2];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz) - radii[k];
+        if (d < best) {
+            best = d;
+            gbest = k + 1;
+        }
+    }
+    const double ph = -tanh(best / width);
+    phi0[c.id] = (R)ph;
+    phi1[c.id] = (R)ph;
+    phid[c.id] = ph;
+    if (grain) grain[c.id] = (int16_t)gbest;
+    const int8_t fl = a.flag[c.id];
+    if (fl != kIsWalls && fl >= 0)
+        a.flag[c.id] = (int8_t)((best <= -ts) ? kIsSolid : (best < ts ? kIsInter : kIsFluid));
+}
+
+int launch_init_eq(int fp32, int fq, int pq, const StepArgs& a, const double* rho,
+                   const double* u, const double* T, const double* C, const double* phi,
+                   void* phi0, void* phi1, hipStream_t s) {
+    if (fp32)
+        return init_eq_r<float>(a, fq, pq, rho, u, T, C, phi, (float*)phi0, (float*)phi1, s);
+    return init_eq_r<double>(a, fq, pq, rho, u, T, C, phi, (double*)phi0, (double*)phi1, s);
+}
+
+int launch_init_seeds(int fp32, int pq, const StepArgs& a, int n, const double* centers,
+                      const double* radii, double width, double ts, void* phi0, void* phi1,
+                      double* phid, int16_t* grain, hipStream_t s) {
+    if (fp32)
+        k_seeds<float><<<grid_of(a), 256, 0, s>>>(a, n, centers, radii, width, ts, (float*)phi0,
+                                                  (float*)phi1, phid, grain);
+    else
+        k_seeds<double><<<grid_of(a), 256, 0, s>>>(a, n, centers, radii, width, ts,
+                                                   (double*)phi0, (double*)phi1, phid, grain);
+    if (fp32) {
+        if (pq == 19) k_init_phase<float, 19><<<grid_of(a), 256, 0, s>>>(a, phid);
+        if (pq == 7) k_init_phase<float, 7><<<grid_of(a), 256, 0, s>>>(a, phid);
+    } else {
+        if (pq == 19) k_init_phase<double, 19><<<grid_of(a), 256, 0, s>>>(a, phid);
+        if (pq == 7) k_init_phase<double, 7><<<grid_of(a), 256, 0, s>>>(a, phid);
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ reductions
+// Wavefront (64-lane) butterfly, then LDS across the four wavefronts of a block; per-block
+// partials are combined by a single block in fixed order, so results are deterministic.
+constexpr int kRedMax = 4;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline void block_reduce(double* v, int nv, bool is_max, double* out_block) {
+    __shared__ double sm[4][kRedMax];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int k = 0; k < nv; k++) {
+        double r = is_max ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) sm[wv][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < nv; k++) {
+            double r = sm[0][k];
+            for (int w = 1; w < 4; w++) r = is_max ? fmax(r, sm[w][k]) : r + sm[w][k];
+            out_block[k] = r;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reduce(const StepArgs a, int which, const double* rho,
+                                                const double* u, const double* T,
+                                                const double* C, const double* phi,
+                                                double* partials) {
+    double v[kRedMax] = {0, 0, 0, 0};
+    uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    bool is_max = (which == 7 || which == 9);
+    if (is_max)
+        for (int k = 0; k < kRedMax; k++) v[k] = -1e300;
+    if (t < a.N && a.flag[t] != kIsWalls) {
+        switch (which) {
+            case 0: v[0] = rho[t]; break;
+            case 1: v[0] = T[t]; break;
+            case 2: v[0] = C[t]; break;
+            case 3: {
+                double fs = 0.5 * (1.0 + phi[t]);
+                v[0] = fs < 0 ? 0 : (fs > 1 ? 1 : fs);
+                break;
+            }
+            case 4: v[0] = phi[t]; break;
+            case 5: v[0] = u[3 * (size_t)t + 2] * T[t]; break;
+            case 6:
+                for (int k = 0; k < 3; k++) v[k] = rho[t] * u[3 * (size_t)t + k];
+                break;
+            case 7: {
+                double ux = u[3 * (size_t)t], uy = u[3 * (size_t)t + 1], uz = u[3 * (size_t)t + 2];
+                v[0] = sqrt(ux * ux + uy * uy + uz * uz);
+                break;
+            }
+            case 8: v[0] = 0; break;
+            case 9: {
+                if (phi[t] >= 0.0) {
+                    uint32_t z = t % a.nz, r = t / a.nz, y = r % a.ny, x = r / a.ny;
+                    v[0] = x - 0.5 * (a.nx - 1);
+                    v[1] = y - 0.5 * (a.ny - 1);
+                    v[2] = z - 0.5 * (a.nz - 1);
+                }
+                break;
+            }
+        }
+    }
+    block_reduce(v, kRedMax, is_max, partials + (size_t)blockIdx.x * kRedMax);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const double* partials, uint32_t nb,
+                                                      int is_max, double* out) {
+    double v[kRedMax];
+    for (int k = 0; k < kRedMax; k++) v[k] = is_max ? -1e300 : 0.0;
+    // fixed assignment of partials to threads and fixed combination order
+    for (uint32_t b = threadIdx.x; b < nb; b += 256)
+        for (int k = 0; k < kRedMax; k++) {
+            double p = partials[(size_t)b * kRedMax + k];
+            v[k] = is_max ? fmax(v[k], p) : v[k] + p;
+        }
+    block_reduce(v, kRedMax, is_max != 0, out);
+}
+
+// Non-finite population count over one lattice buffer (storage type).
+template <class R>
+__global__ void __launch_bounds__(256) k_nonfinite(const R* p, size_t n, double* partials) {
+    double v[kRedMax] = {0, 0, 0, 0};
+    size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * 256u;
+    for (size_t i = t; i < n; i += stride)
+        if (!isfinite((double)p[i])) v[0] += 1.0;
+    block_reduce(v, kRedMax, false, partials + (size_t)blockIdx.x * kRedMax);
+}
+
+int launch_reduce(int which, const StepArgs& a, const double* rho, const double* u,
+                  const double* T, const double* C, const double* phi, double* partials,
+                  double* out, int* nout, hipStream_t s) {
+    int is_max = (which == 7 || which == 9);
+    k_reduce<<<grid_of(a), 256, 0, s>>>(a, which, rho, u, T, C, phi, partials);
+    k_reduce_final<<<1, 256, 0, s>>>(partials, a.nblk, is_max, out);
+    *nout = (which == 6 || which == 9) ? 3 : 1;
+    return 0;
+}
+
+int launch_nonfinite(int fp32, const void* p, size_t n, double* partials, double* out,
+                     hipStream_t s) {
+    const uint32_t nb = 1024;
+    if (fp32) k_nonfinite<float><<<nb, 256, 0, s>>>((const float*)p, n, partials);
+    else k_nonfinite<double><<<nb, 256, 0, s>>>((const double*)p, n, partials);
+    k_reduce_final<<<1, 256, 0, s>>>(partials, nb, 0, out);
+    return 0;
+}
+
+}  // namespace mkg

@@ -1,0 +1,360 @@
+// Device-side building blocks of the time step: cell location, AA-pattern population access
+// with face rules, and the per-cell collision / source functions.  Included by step.hip and
+// aux.hip only.
+//
+// AA pattern (in-place streaming, one population array per lattice):
+//   layout E (lay 0): own slot i of cell x holds the pre-collision f_i(x).  An E step reads
+//     its own slots and writes post-collision f*_i to its own slot inv(i): purely local.
+//   layout O (lay 1): f_i(x) sits in slot inv(i) of cell x - c_i.  An O step gathers from
+//     there and scatters f*_i to slot i of cell x + c_i, which restores layout E.
+// Every (cell, slot) is read and written by the same thread within a step, so the update
+// needs no second array and no inter-thread ordering.  Domain faces and wall cells are only
+// touched by O steps: a population whose source (read) or target (write) lies outside a
+// non-periodic face or in a wall cell is redirected to the cell's own slots with the face
+// rule (half-way bounce-back with the moving-wall term, anti-bounce-back for a Dirichlet
+// value, or zero-gradient outflow).  The same rules in pull form are the streaming of
+// mkFluidsUser::propagate (MaterKit/src/mkFluidsUser.cpp:77-97).
+#pragma once
+#include "kernels.h"
+#include "lattice.h"
+
+namespace mkg {
+
+enum { kKindFlow = 0, kKindScalar = 1 };
+enum : uint32_t { XLO = 1u, XHI = 2u, YLO = 4u, YHI = 8u, ZLO = 16u, ZHI = 32u };
+
+struct Cell {
+    uint32_t id;
+    uint32_t ox[3], oy[3], oz[3];  // linear offsets of coordinate -1, 0, +1 (periodic wrap)
+    uint32_t bits;                 // which domain faces the cell touches
+    int x, y, z;
+};
+
+// Thread -> cell.  Blocks of 256 threads walk the linear index (z contiguous).  When the block
+// count is a multiple of 8, consecutive hardware block ids (dealt round-robin to the eight
+// XCDs) are remapped so that each XCD works on one contiguous eighth of the domain and the
+// neighbour rows it reads stay in its own L2.
+__device__ inline bool locate(const StepArgs& a, Cell& c) {
+    uint32_t b = blockIdx.x;
+    if ((a.nblk & 7u) == 0u) b = (b & 7u) * (a.nblk >> 3) + (b >> 3);
+    uint32_t t = b * 256u + threadIdx.x;
+    if (t >= a.N) return false;
+    uint32_t nz = (uint32_t)a.nz, ny = (uint32_t)a.ny, nx = (uint32_t)a.nx;
+    uint32_t z = t % nz;
+    uint32_t r = t / nz;
+    uint32_t y = r % ny;
+    uint32_t x = r / ny;
+    c.id = t;
+    c.x = (int)x;
+    c.y = (int)y;
+    c.z = (int)z;
+    uint32_t pl = ny * nz;
+    c.ox[0] = (x == 0 ? nx - 1 : x - 1) * pl;
+    c.ox[1] = x * pl;
+    c.ox[2] = (x + 1 == nx ? 0 : x + 1) * pl;
+    c.oy[0] = (y == 0 ? ny - 1 : y - 1) * nz;
+    c.oy[1] = y * nz;
+    c.oy[2] = (y + 1 == ny ? 0 : y + 1) * nz;
+    c.oz[0] = (z == 0 ? nz - 1 : z - 1);
+    c.oz[1] = z;
+    c.oz[2] = (z + 1 == nz ? 0 : z + 1);
+    c.bits = (x == 0 ? XLO : 0u) | (x + 1 == nx ? XHI : 0u) | (y == 0 ? YLO : 0u) |
+             (y + 1 == ny ? YHI : 0u) | (z == 0 ? ZLO : 0u) | (z + 1 == nz ? ZHI : 0u);
+    return true;
+}
+
+// Faces crossed when stepping from a cell by (dx, dy, dz).
+__host__ __device__ constexpr uint32_t faces_of(int dx, int dy, int dz) {
+    return (dx < 0 ? XLO : dx > 0 ? XHI : 0u) | (dy < 0 ? YLO : dy > 0 ? YHI : 0u) |
+           (dz < 0 ? ZLO : dz > 0 ? ZHI : 0u);
+}
+
+// Select among six per-face values without indexing the kernel-argument block dynamically
+// (a runtime index into the by-value argument struct would copy it to scratch memory).
+template <class T>
+__device__ inline T pick6(int f, const T* v) {
+    T r = v[5];
+    r = f == 4 ? v[4] : r;
+    r = f == 3 ? v[3] : r;
+    r = f == 2 ? v[2] : r;
+    r = f == 1 ? v[1] : r;
+    r = f == 0 ? v[0] : r;
+    return r;
+}
+__device__ inline double face_cu(int f, int X, int Y, int Z, const double (*uf)[3]) {
+    double r = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        if (f == k) r = X * uf[k][0] + Y * uf[k][1] + Z * uf[k][2];
+    return r;
+}
+
+__device__ inline uint32_t nbr(const Cell& c, int dx, int dy, int dz) {
+    return c.ox[dx + 1] + c.oy[dy + 1] + c.oz[dz + 1];
+}
+
+// ------------------------------------------------------------------ population access
+template <int Q, bool ODD, class R>
+__device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                                 const StepArgs& a, int kind) {
+    const R* __restrict__ base = (const R*)L.pop;
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int i = 0; i < Q; i++) p[i] = base[(size_t)i * S + c.id];
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
+            uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+            if (hit == 0u) {
+                uint32_t src = nbr(c, dx, dy, dz);
+                if (a.has_walls && i != 0 && a.flag[src] == kIsWalls)
+                    p[i] = base[(size_t)i * S + c.id];
+                else
+                    p[i] = base[(size_t)inv(i) * S + src];
+            } else {
+                int face = __builtin_ctz(hit);
+                int t = pick6(face, L.bc);
+                if (t == kBcOutflow) {
+                    p[i] = base[(size_t)inv(i) * S + c.id];
+                } else if (t == kBcDirichlet && kind == kKindScalar) {
+                    p[i] = -base[(size_t)i * S + c.id] +
+                           (R)(2.0 * wgt<Q>(i) * (pick6(face, L.fval) - L.ref));
+                } else {
+                    R v = base[(size_t)i * S + c.id];
+                    if (kind == kKindFlow) {
+                        double cu = face_cu(face, cx(i), cy(i), cz(i), a.u_face);
+                        v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
+                    }
+                    p[i] = v;
+                }
+            }
+        }
+    }
+}
+
+template <int Q, bool ODD, class R>
+__device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                                  const StepArgs& a, int kind) {
+    R* __restrict__ base = (R*)L.pop;
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int i = 0; i < Q; i++) base[(size_t)inv(i) * S + c.id] = p[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const int dx = cx(i), dy = cy(i), dz = cz(i);
+            uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+            if (hit == 0u) {
+                uint32_t tgt = nbr(c, dx, dy, dz);
+                if (a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
+                    base[(size_t)inv(i) * S + c.id] = p[i];
+                else
+                    base[(size_t)i * S + tgt] = p[i];
+            } else {
+                int face = __builtin_ctz(hit);
+                int t = pick6(face, L.bc);
+                R v;
+                if (t == kBcOutflow) {
+                    v = p[inv(i)];
+                } else if (t == kBcDirichlet && kind == kKindScalar) {
+                    v = -p[i] + (R)(2.0 * wgt<Q>(i) * (pick6(face, L.fval) - L.ref));
+                } else {
+                    v = p[i];
+                    if (kind == kKindFlow) {
+                        double cu = face_cu(face, cx(i), cy(i), cz(i), a.u_face);
+                        v -= (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
+                    }
+                }
+                base[(size_t)inv(i) * S + c.id] = v;
+            }
+        }
+    }
+}
+
+// Read with a runtime layout (lattices used only for coupling in this launch).
+template <int Q, class R>
+__device__ inline void read_pops_rt(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                                    const StepArgs& a, int kind) {
+    if (L.lay)
+        read_pops<Q, true, R>(p, L, S, c, a, kind);
+    else
+        read_pops<Q, false, R>(p, L, S, c, a, kind);
+}
+
+// ------------------------------------------------------------------ flow collision
+// Generalised moment relaxation on the w-orthogonal Hermite decomposition of f - f_eq:
+// momentum part (rate irrelevant under Guo forcing), traceless second-order "shear" part
+// (s_nu), trace "bulk" part (s_bulk) and the remaining ghost moments split into odd (s_odd)
+// and even (s_even) halves.  BGK is all four rates equal; TRT is s_bulk = s_even = s_nu,
+// s_odd = omega-; MRT uses independent rates.  Forcing is Guo et al. (2002) projected onto
+// the same parts with (1 - s/2) factors.  Populations are deviations f - w_i*ref; dr is their
+// sum (rho - ref).  Counterparts: mkNode::collision_BGK / collision_TRT / collision_MRT0-2
+// (MaterKit/inc/mkNode.h:72-87).
+template <int Q, class R>
+__device__ inline void flow_collide(R* f, R dr, R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
+                                    R s_nu, R s_b, R s_o, R s_e, R ns) {
+    const R uu = ux * ux + uy * uy + uz * uz;
+    R feq[Q], neq[Q];
+    R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const R w = (R)wgt<Q>(i);
+        const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+        feq[i] = w * (dr + rho * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu));
+        const R n = f[i] - feq[i];
+        neq[i] = n;
+        if (cx(i)) { pxx += n; jx += cx(i) * n; }
+        if (cy(i)) { pyy += n; jy += cy(i) * n; }
+        if (cz(i)) { pzz += n; jz += cz(i) * n; }
+        if (cx(i) * cy(i)) pxy += (R)(cx(i) * cy(i)) * n;
+        if (cx(i) * cz(i)) pxz += (R)(cx(i) * cz(i)) * n;
+        if (cy(i) * cz(i)) pyz += (R)(cy(i) * cz(i)) * n;
+    }
+    const R tr = pxx + pyy + pzz;
+    const R t3 = tr * (R)(1.0 / 3.0);
+    const R txx = pxx - t3, tyy = pyy - t3, tzz = pzz - t3;
+    // second-order forcing tensor uF + Fu
+    const R fxx = (R)2 * ux * Fx, fyy = (R)2 * uy * Fy, fzz = (R)2 * uz * Fz;
+    const R fxy = ux * Fy + uy * Fx, fxz = ux * Fz + uz * Fx, fyz = uy * Fz + uz * Fy;
+    const R ftr = fxx + fyy + fzz;
+    const R f3 = ftr * (R)(1.0 / 3.0);
+    const R gxx = fxx - f3, gyy = fyy - f3, gzz = fzz - f3;
+    const R a_nu = (R)1 - s_nu, a_b = (R)1 - s_b, a_o = (R)1 - s_o, a_e = (R)1 - s_e;
+    const R b_nu = (R)1 - (R)0.5 * s_nu, b_b = (R)1 - (R)0.5 * s_b;
+    R gh[Q], base[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const R w = (R)wgt<Q>(i);
+        const int X = cx(i), Y = cy(i), Z = cz(i);
+        const R cTc = (R)(X * X) * txx + (R)(Y * Y) * tyy + (R)(Z * Z) * tzz +
+                      (R)(2 * X * Y) * pxy + (R)(2 * X * Z) * pxz + (R)(2 * Y * Z) * pyz;
+        const R cGc = (R)(X * X) * gxx + (R)(Y * Y) * gyy + (R)(Z * Z) * gzz +
+                      (R)(2 * X * Y) * fxy + (R)(2 * X * Z) * fxz + (R)(2 * Y * Z) * fyz;
+        const R sh = (R)4.5 * w * cTc;
+        const R bk = (R)1.5 * w * (R)(csq(i) - 1) * tr;
+        const R mo = (R)3 * w * (X * jx + Y * jy + Z * jz);
+        gh[i] = neq[i] - mo - sh - bk;
+        const R cF = X * Fx + Y * Fy + Z * Fz;
+        base[i] = feq[i] + (R)1.5 * w * cF + a_nu * sh + a_b * bk +
+                  b_nu * ((R)4.5 * w * cGc) + b_b * ((R)1.5 * w * (R)(csq(i) - 1) * ftr);
+    }
+    R out[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const R go = (R)0.5 * (gh[i] - gh[inv(i)]);
+        const R ge = (R)0.5 * (gh[i] + gh[inv(i)]);
+        out[i] = base[i] + a_o * go + a_e * ge;
+    }
+    if (ns != (R)0) {
+#pragma unroll
+        for (int i = 0; i < Q; i++) out[i] = ((R)1 - ns) * out[i] + ns * f[inv(i)];
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = out[i];
+}
+
+// ------------------------------------------------------------------ phase-field source
+template <class R>
+__device__ inline R pow4(R v) {
+    R s = v * v;
+    return s * s;
+}
+
+// Normal component of J = a_s^2 g + |g|^2 a_s da_s/dg on a face with gradient g (lab frame);
+// M (lab <- crystal, row-major) may be null.
+template <class R>
+__device__ inline R face_flux(R g0, R g1, R g2, int axis, R eps4, const R (&M)[9], bool rot) {
+    R c0 = g0, c1 = g1, c2 = g2;
+    if (rot) {
+        c0 = M[0] * g0 + M[3] * g1 + M[6] * g2;
+        c1 = M[1] * g0 + M[4] * g1 + M[7] * g2;
+        c2 = M[2] * g0 + M[5] * g1 + M[8] * g2;
+    }
+    const R g2s = c0 * c0 + c1 * c1 + c2 * c2;
+    const R ga = axis == 0 ? g0 : (axis == 1 ? g1 : g2);
+    if (!(g2s > (R)1e-30)) return ga;
+    const R gn = sqrt(g2s);
+    const R n0 = c0 / gn, n1 = c1 / gn, n2 = c2 / gn;
+    const R s4 = pow4(n0) + pow4(n1) + pow4(n2);
+    const R as = (R)1 - (R)3 * eps4 + (R)4 * eps4 * s4;
+    const R k = (R)16 * eps4 * as * gn;
+    const R N0 = k * (n0 * n0 * n0 - n0 * s4);
+    const R N1 = k * (n1 * n1 * n1 - n1 * s4);
+    const R N2 = k * (n2 * n2 * n2 - n2 * s4);
+    R Nl;
+    if (rot)
+        Nl = M[axis * 3 + 0] * N0 + M[axis * 3 + 1] * N1 + M[axis * 3 + 2] * N2;
+    else
+        Nl = axis == 0 ? N0 : (axis == 1 ? N1 : N2);
+    return as * as * ga + Nl;
+}
+
+// P holds the 3x3x3 neighbourhood, index (dx+1)*9 + (dy+1)*3 + (dz+1) (corners unused).
+// Returns S of the phase-field LB update and the cell anisotropy a_s.
+template <class R>
+__device__ inline R phase_source(const R (&P)[27], R upf, R eps4, R lambda, R pf_dt, R pf_wdx2,
+                                 const R (&M)[9], bool rot, R& a_out) {
+#define PP(dx, dy, dz) P[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
+    R divJ = 0, lap = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int side = sd == 0 ? 1 : -1;
+            const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
+            R g[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b == ax) {
+                    g[b] = (R)side * (PP(o0, o1, o2) - PP(0, 0, 0));
+                } else {
+                    const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                    g[b] = (R)0.25 * (PP(e0, e1, e2) - PP(-e0, -e1, -e2) +
+                                      PP(o0 + e0, o1 + e1, o2 + e2) -
+                                      PP(o0 - e0, o1 - e1, o2 - e2));
+                }
+            }
+            const R J = face_flux<R>(g[0], g[1], g[2], ax, eps4, M, rot);
+            divJ += (R)side * J;
+            lap += (R)side * g[ax];
+        }
+    }
+    // isotropic LB-weighted cell gradient (D3Q19 weights over cs^2 = 1/3)
+    R gr0 = 0, gr1 = 0, gr2 = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)(3.0 * wgt<19>(i)) * PP(cx(i), cy(i), cz(i));
+        if (cx(i)) gr0 += (R)cx(i) * v;
+        if (cy(i)) gr1 += (R)cy(i) * v;
+        if (cz(i)) gr2 += (R)cz(i) * v;
+    }
+    R c0 = gr0, c1 = gr1, c2 = gr2;
+    if (rot) {
+        c0 = M[0] * gr0 + M[3] * gr1 + M[6] * gr2;
+        c1 = M[1] * gr0 + M[4] * gr1 + M[7] * gr2;
+        c2 = M[2] * gr0 + M[5] * gr1 + M[8] * gr2;
+    }
+    R as = 1;
+    const R gg = c0 * c0 + c1 * c1 + c2 * c2;
+    if (gg > (R)1e-30) {
+        const R gn = sqrt(gg);
+        as = (R)1 - (R)3 * eps4 + (R)4 * eps4 * (pow4(c0 / gn) + pow4(c1 / gn) + pow4(c2 / gn));
+    }
+    const R ph = PP(0, 0, 0);
+    const R d = (R)1 - ph * ph;
+    const R Qs = (ph - lambda * upf * d) * d;
+    const R a2 = as * as;
+    a_out = as;
+    return pf_dt / a2 * ((divJ - a2 * lap) * pf_wdx2 + Qs);
+#undef PP
+}
+
+__device__ inline int8_t classify(int8_t old, double phi, double phi_s) {
+    if (old == kIsWalls || old < 0) return old;
+    if (phi >= phi_s) return (int8_t)kIsSolid;
+    if (phi > -phi_s) return (int8_t)kIsInter;
+    return (int8_t)kIsFluid;
+}
+
+}  // namespace mkg

@@ -1,0 +1,91 @@
+// Kernel argument blocks and launcher declarations shared by the C-ABI layer (api.cpp) and
+// the HIP kernels (step.hip, aux.hip).  Plain structs passed by value to the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mkg {
+
+// One population lattice as seen by a kernel.
+struct LatArgs {
+    void* pop;        // Q * S elements of the storage type, slot i at pop + i*S
+    int32_t bc[6];    // face codes x-, x+, y-, y+, z-, z+
+    int32_t npmask;   // bit f set when face f is not periodic
+    int32_t lay;      // current AA layout: 0 = own slots hold f_i, 1 = shifted
+    int32_t adv;      // advance this lattice in this launch
+    int32_t pad;
+    double ref;       // deviation reference: stored value = f_i - w_i * ref
+    double fval[6];   // Dirichlet face values (scalar lattices)
+};
+
+struct StepArgs {
+    int32_t nx, ny, nz;
+    uint32_t N;       // number of cells
+    uint32_t S;       // slot stride (N rounded up)
+    uint32_t nblk;    // blocks in the launch (XCD-aware remap when divisible by 8)
+    int32_t has_walls, has_kappa, ngrains, pf_coupling;
+    LatArgs f, g, h, c;
+    const void* phi_cur;   // storage type, phase field at time level n
+    const void* phi_prev;  // level n-1
+    int8_t* flag;
+    const int16_t* grain;
+    const void* kappa;     // storage type
+    const double* rot;     // ngrains x 9, lab <- crystal
+    double s_nu, s_bulk, s_odd, s_even, rho0;
+    double force[3], buoy[3], T_ref;
+    double u_face[6][3];
+    double alpha_l, alpha_s, latent_T;
+    double D_l, D_s, k_part, latent_C, q_min;
+    double pf_dt, pf_wdx2, eps4, lambda, phi_s, omega_g;
+};
+
+// Selector of a fused-step instantiation: presence of each lattice.
+struct Combo {
+    int fq, pq, th, so;
+};
+
+// Returns 0 on success, -1 when the combination has no instantiation.
+int launch_step(const Combo& k, int fp32, int odd, const StepArgs& a, hipStream_t s);
+int launch_phase_moments(int pq, int fp32, int odd, const StepArgs& a, void* phi_out,
+                         hipStream_t s);
+
+// Canonical (layout-independent) pre-collision populations of one lattice into a dense
+// Q x N buffer of doubles (full values, deviation reference added back).
+// kind: 0 flow, 1 scalar.
+int launch_canonical(int q, int fp32, const StepArgs& a, const LatArgs& L, int kind,
+                     double* out, hipStream_t s);
+// Write full-value populations (Q x N doubles) into own-slot layout.
+int launch_store_pops(int q, int fp32, const StepArgs& a, const LatArgs& L, const double* in,
+                      hipStream_t s);
+
+// Macroscopic fields read straight from the AA storage (each lattice at its own layout):
+// rho, u (3xN, Guo half-force velocity), T, C; any output may be null.
+int launch_macro(int fq, int fp32, const StepArgs& a, double* rho, double* u, double* T,
+                 double* C, hipStream_t s);
+
+// Storage-type <-> double conversions of N-element scalar arrays.
+int launch_to_double(int fp32, const void* src, double* dst, uint32_t n, hipStream_t s);
+int launch_from_double(int fp32, const double* src, void* dst, uint32_t n, hipStream_t s);
+
+// Equilibrium populations from macroscopic double arrays (own-slot layout); phase buffers
+// phi0 / phi1 and flags from phi when given.
+int launch_init_eq(int fp32, int fq, int pq, const StepArgs& a, const double* rho,
+                   const double* u, const double* T, const double* C, const double* phi,
+                   void* phi0, void* phi1, hipStream_t s);
+
+// Seed geometry: phi (both buffers and the double scratch phid), grain ids, flags, phase
+// populations.
+int launch_init_seeds(int fp32, int pq, const StepArgs& a, int n, const double* centers,
+                      const double* radii, double width, double ts, void* phi0, void* phi1,
+                      double* phid, int16_t* grain, hipStream_t s);
+
+// FP64 block reductions over double fields (null when absent); deterministic two-pass
+// tree (per-block partials, then one block in fixed order).  out receives nout doubles.
+int launch_reduce(int which, const StepArgs& a, const double* rho, const double* u,
+                  const double* T, const double* C, const double* phi, double* partials,
+                  double* out, int* nout, hipStream_t s);
+// Count of non-finite values in a storage-type buffer of n elements.
+int launch_nonfinite(int fp32, const void* p, size_t n, double* partials, double* out,
+                     hipStream_t s);
+
+}  // namespace mkg

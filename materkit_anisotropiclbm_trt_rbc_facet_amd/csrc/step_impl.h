@@ -1,0 +1,241 @@
+// Fused time-step kernels.  Included by step_f32.hip / step_f64.hip, which instantiate them
+// for one storage precision each.
+//
+// One time step of the coupled solver is two launches:
+//   k_phase_moments  phi^n = sum_i g_i for every cell (needed by the neighbours' gradient
+//                    stencils before any phase collision), flags re-classified from phi^n
+//                    (mkFluidsUser::relaxation first sweep, MaterKit/src/mkFluidsUser.cpp:
+//                    147-172, and mkPhaseField::reset_flags, MaterKit/inc/mkPhaseField.h:37)
+//   k_step           per cell: thermal / solute moments, flow collide-stream with Guo forcing,
+//                    Boussinesq buoyancy and partial bounce-back in solid, thermal and
+//                    solute collide-stream with the latent / rejection source, phase-field
+//                    collide-stream with the anisotropic source -- every lattice in one pass
+//                    (the per-field propagate()/relaxation() pairs of MaterKit.cpp:500-518)
+// Without a phase lattice only k_step runs.
+#pragma once
+#include "device.h"
+
+namespace mkg {
+
+template <class R>
+__device__ inline R clamp01(R v) {
+    return v < (R)0 ? (R)0 : (v > (R)1 ? (R)1 : v);
+}
+
+template <class R, int PQ, bool ODD>
+__global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __restrict__ phi_out) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const int8_t fl = a.flag[c.id];
+    if (fl == kIsWalls) return;
+    R g[PQ];
+    read_pops<PQ, ODD, R>(g, a.g, a.S, c, a, kKindScalar);
+    R s = 0;
+#pragma unroll
+    for (int i = 0; i < PQ; i++) s += g[i];
+    const R phi = (R)a.g.ref + s;
+    phi_out[c.id] = phi;
+    const int8_t nf = classify(fl, (double)phi, a.phi_s);
+    if (nf != fl) a.flag[c.id] = nf;
+}
+
+template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD>
+__global__ void __launch_bounds__(256) k_step(const StepArgs a) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const int8_t fl = a.flag[c.id];
+    if (fl == kIsWalls) return;
+    const uint32_t S = a.S;
+
+    R phi = (R)-1, dphi = (R)0, ns = (R)0;
+    if constexpr (PQ > 0) {
+        phi = ((const R*)a.phi_cur)[c.id];
+        dphi = phi - ((const R*)a.phi_prev)[c.id];
+        if (fl == kIsSolid)
+            ns = (R)1;
+        else if (fl == kIsInter)
+            ns = clamp01((R)0.5 * ((R)1 + phi));
+    }
+
+    // ---- scalar moments at time level n
+    R h[TH ? 7 : 1];
+    R sh = 0, T = 0;
+    if constexpr (TH) {
+        if (a.h.adv)
+            read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+        else
+            read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
+#pragma unroll
+        for (int i = 0; i < 7; i++) sh += h[i];
+        T = (R)a.h.ref + sh;
+    }
+    R cc[SO ? 7 : 1];
+    R sc = 0, C = 0, U = 0;
+    if constexpr (SO) {
+        if (a.c.adv)
+            read_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
+        else
+            read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
+#pragma unroll
+        for (int i = 0; i < 7; i++) sc += cc[i];
+        C = (R)a.c.ref + sc;
+        const R A = (R)0.5 * (((R)1 + (R)a.k_part) - ((R)1 - (R)a.k_part) * phi);
+        U = C / A;
+    }
+
+    // ---- flow
+    R ux = 0, uy = 0, uz = 0;
+    if constexpr (FQ > 0) {
+        R f[FQ];
+        if (a.f.adv)
+            read_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow);
+        else
+            read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
+        R dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < FQ; i++) {
+            dr += f[i];
+            if (cx(i)) jx += (R)cx(i) * f[i];
+            if (cy(i)) jy += (R)cy(i) * f[i];
+            if (cz(i)) jz += (R)cz(i) * f[i];
+        }
+        const R rho = (R)a.f.ref + dr;
+        R Fx = (R)a.force[0], Fy = (R)a.force[1], Fz = (R)a.force[2];
+        if constexpr (TH) {
+            const R dT = T - (R)a.T_ref;
+            Fx += (R)(a.rho0 * a.buoy[0]) * dT;
+            Fy += (R)(a.rho0 * a.buoy[1]) * dT;
+            Fz += (R)(a.rho0 * a.buoy[2]) * dT;
+        }
+        ux = (jx + (R)0.5 * Fx) / rho;
+        uy = (jy + (R)0.5 * Fy) / rho;
+        uz = (jz + (R)0.5 * Fz) / rho;
+        if (a.f.adv) {
+            flow_collide<FQ, R>(f, dr, rho, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
+                                (R)a.s_odd, (R)a.s_even, ns);
+            write_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow);
+        }
+        const R om = (R)1 - ns;
+        ux *= om;
+        uy *= om;
+        uz *= om;
+    }
+
+    // ---- thermal collide-stream (mkThermalField::propagate + relaxation)
+    if constexpr (TH) {
+        if (a.h.adv) {
+            const R fs = PQ > 0 ? clamp01((R)0.5 * ((R)1 + phi)) : (R)0;
+            R alpha = (R)a.alpha_l * ((R)1 - fs) + (R)a.alpha_s * fs;
+            if (a.has_kappa) alpha *= ((const R*)a.kappa)[c.id];
+            const R om = (R)1 / ((R)0.5 + alpha / (R)0.25);
+            const R src = (R)a.latent_T * dphi;
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                const R w = (R)wgt<7>(i);
+                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+                const R heq = w * (sh + T * cu / (R)0.25);
+                h[i] = h[i] - om * (h[i] - heq) + w * src;
+            }
+            write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+        }
+    }
+
+    // ---- solute collide-stream, c = A(phi) U (mkSolutalField::propagate + relaxation)
+    if constexpr (SO) {
+        if (a.c.adv) {
+            R q = (R)0.5 * ((R)1 - phi);
+            if (a.D_l > 0) q += (R)(a.D_s / a.D_l) * (R)0.5 * ((R)1 + phi);
+            if (q < (R)a.q_min) q = (R)a.q_min;
+            const R om = (R)1 / ((R)0.5 + (R)a.D_l * q / (R)0.25);
+            const R src = (R)a.latent_C * dphi;
+            const R ref = (R)a.c.ref;
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                const R w = (R)wgt<7>(i);
+                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+                const R ceq = (i == 0) ? (C - (R)(1.0 - wgt<7>(0)) * U) - w * ref
+                                       : w * (U * ((R)1 + cu / (R)0.25) - ref);
+                cc[i] = cc[i] - om * (cc[i] - ceq) + w * src;
+            }
+            write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
+        }
+    }
+
+    // ---- phase field collide-stream (mkPhaseField::propagate +
+    //      relaxation_anisotropy_thermal / _solute)
+    if constexpr (PQ > 0) {
+        if (a.g.adv) {
+            R g[PQ];
+            read_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
+            const R* __restrict__ ph = (const R*)a.phi_cur;
+            uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+            uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+            uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+            const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+            if (np & XLO) px[0] = px[1];
+            if (np & XHI) px[2] = px[1];
+            if (np & YLO) py[0] = py[1];
+            if (np & YHI) py[2] = py[1];
+            if (np & ZLO) pz[0] = pz[1];
+            if (np & ZHI) pz[2] = pz[1];
+            R P[27];
+#pragma unroll
+            for (int i = 0; i < 19; i++)
+                P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
+                    ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+            R upf = 0;
+            if (a.pf_coupling == 1) upf = T;
+            if (a.pf_coupling == 2) upf = U;
+            R Mrot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            bool rot = false;
+            if (a.ngrains > 0) {
+                const int gid = a.grain[c.id];
+                if (gid > 0) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) Mrot[k] = (R)a.rot[(gid - 1) * 9 + k];
+                    rot = true;
+                }
+            }
+            R as;
+            const R Sg = phase_source<R>(P, upf, (R)a.eps4, (R)a.lambda, (R)a.pf_dt,
+                                         (R)a.pf_wdx2, Mrot, rot, as);
+            const R om = (R)a.omega_g;
+            const R dp = phi - (R)a.g.ref;
+#pragma unroll
+            for (int i = 0; i < PQ; i++) {
+                const R w = (R)wgt<PQ>(i);
+                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+                const R geq = w * (dp + phi * cu / (R)cs2<PQ>());
+                g[i] = g[i] - om * (g[i] - geq) + w * Sg;
+            }
+            write_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
+        }
+    }
+}
+
+// Host-side launch helpers for one storage type.
+template <class R, int FQ, int PQ, bool TH, bool SO>
+inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
+    dim3 grid(a.nblk), block(256);
+    if (odd)
+        k_step<R, FQ, PQ, TH, SO, true><<<grid, block, 0, s>>>(a);
+    else
+        k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);
+}
+
+template <class R, int PQ>
+inline void run_phase_moments(int odd, const StepArgs& a, void* phi_out, hipStream_t s) {
+    dim3 grid(a.nblk), block(256);
+    if (odd)
+        k_phase_moments<R, PQ, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+    else
+        k_phase_moments<R, PQ, false><<<grid, block, 0, s>>>(a, (R*)phi_out);
+}
+
+// Lattice combinations with an instantiation: (flow Q, phase Q, thermal, solute).
+#define MKG_COMBOS(X)                                                                       \
+    X(19, 0, 0, 0) X(27, 0, 0, 0) X(19, 0, 1, 0) X(27, 0, 1, 0) X(0, 19, 1, 0) X(0, 7, 1, 0) \
+    X(19, 19, 1, 0) X(19, 7, 1, 0) X(27, 19, 1, 0) X(19, 19, 0, 1) X(19, 7, 0, 1)           \
+    X(0, 19, 0, 1) X(0, 0, 1, 0) X(0, 0, 0, 1) X(0, 19, 0, 0) X(0, 7, 0, 0)
+
+}  // namespace mkg

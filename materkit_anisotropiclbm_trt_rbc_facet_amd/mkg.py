@@ -1,0 +1,292 @@
+"""ctypes binding of the C ABI in ``include/mkg.h`` (``libmkg.so``).
+
+The library is loaded from this package directory.  There is no CPU fallback: if the
+library is missing or no GPU is present, the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libmkg.so"
+
+MAX_GRAINS = 64
+
+# flags (MaterKit/inc/mkGlobalDefs.h:140-145)
+IS_OUTLET, IS_INLET, IS_FLUID, IS_INTER, IS_SOLID, IS_WALLS = -2, -1, 0, 1, 2, 3
+OP_BGK, OP_TRT, OP_MRT = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW = 0, 1, 2, 3
+FLOW, PHASE, THERMAL, SOLUTE = 1, 2, 4, 8
+ALL = FLOW | PHASE | THERMAL | SOLUTE
+F_POP_FLOW, F_POP_PHASE, F_POP_THERMAL, F_POP_SOLUTE = 0, 1, 2, 3
+F_RHO, F_VEL, F_TEMP, F_CONC, F_PHI, F_PHI_PREV, F_FLAG, F_GRAIN, F_KAPPA = range(4, 13)
+DT_F64, DT_F32, DT_I8, DT_I16 = 0, 1, 2, 3
+(R_MASS, R_ENERGY, R_SOLUTE, R_SOLID, R_PHI, R_NUSSELT, R_MOMENTUM, R_UMAX, R_NONFINITE,
+ R_TIP) = range(10)
+
+PARAM_FIELDS = [
+    ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+    ("flow_q", C.c_int32), ("phase_q", C.c_int32), ("thermal", C.c_int32),
+    ("solute", C.c_int32), ("precision", C.c_int32), ("flow_op", C.c_int32),
+    ("pf_coupling", C.c_int32), ("ngrains", C.c_int32), ("has_kappa", C.c_int32),
+    ("bc_f", C.c_int32 * 6), ("bc_g", C.c_int32 * 6), ("bc_h", C.c_int32 * 6),
+    ("bc_c", C.c_int32 * 6), ("pad0", C.c_int32),
+    ("s_nu", C.c_double), ("s_bulk", C.c_double), ("s_odd", C.c_double),
+    ("s_even", C.c_double), ("rho0", C.c_double), ("force", C.c_double * 3),
+    ("buoy", C.c_double * 3), ("T_ref", C.c_double), ("u_face", (C.c_double * 3) * 6),
+    ("alpha_l", C.c_double), ("alpha_s", C.c_double), ("latent_T", C.c_double),
+    ("T_face", C.c_double * 6),
+    ("D_l", C.c_double), ("D_s", C.c_double), ("k_part", C.c_double),
+    ("latent_C", C.c_double), ("q_min", C.c_double), ("C_face", C.c_double * 6),
+    ("pf_dt", C.c_double), ("pf_wdx2", C.c_double), ("eps4", C.c_double),
+    ("lambda_", C.c_double), ("phi_s", C.c_double),
+    ("ref_f", C.c_double), ("ref_g", C.c_double), ("ref_h", C.c_double),
+    ("ref_c", C.c_double),
+    ("rot", (C.c_double * 9) * MAX_GRAINS),
+]
+
+
+class MkgParams(C.Structure):
+    """Layout-identical mirror of ``mkg_params``."""
+
+    _fields_ = PARAM_FIELDS
+
+    def copy(self) -> "MkgParams":
+        out = MkgParams()
+        C.memmove(C.byref(out), C.byref(self), C.sizeof(self))
+        return out
+
+    def set(self, **kw) -> "MkgParams":
+        """Assign fields by name; arrays may be partial (leading elements are filled)."""
+        for k, v in kw.items():
+            if k == "lambda":
+                k = "lambda_"
+            cur = getattr(self, k)
+            if isinstance(cur, C.Array):
+                flat = np.asarray(v).reshape(-1)
+                if isinstance(cur[0], C.Array):
+                    inner = len(cur[0])
+                    for n in range(flat.size):
+                        cur[n // inner][n % inner] = float(flat[n])
+                else:
+                    conv = float if isinstance(cur[0], float) else int
+                    for n in range(flat.size):
+                        cur[n] = conv(flat[n])
+            else:
+                setattr(self, k, v)
+        return self
+
+    @property
+    def ncell(self) -> int:
+        return int(self.nx) * int(self.ny) * int(self.nz)
+
+
+_lib = None
+
+
+class MkgError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load ``libmkg.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise MkgError(f"{LIB_PATH} not built; run materkit_anisotropiclbm_trt_rbc_facet_amd.build")
+    L = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "mkg_abi_version": (C.c_int, []),
+        "mkg_params_size": (C.c_size_t, []),
+        "mkg_last_error": (C.c_char_p, []),
+        "mkg_params_default": (C.c_int, [P(MkgParams)]),
+        "mkg_params_from_xml": (C.c_int, [C.c_char_p, P(MkgParams), P(C.c_int32)]),
+        "mkg_seeds_from_xml": (C.c_int, [C.c_char_p, C.c_int, vp, vp, vp]),
+        "mkg_create": (C.c_int, [P(MkgParams), P(vp)]),
+        "mkg_destroy": (C.c_int, [vp]),
+        "mkg_get_params": (C.c_int, [vp, P(MkgParams)]),
+        "mkg_set_params": (C.c_int, [vp, P(MkgParams)]),
+        "mkg_init_equilibrium": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "mkg_init_seeds": (C.c_int, [vp, C.c_int, vp, vp]),
+        "mkg_random_seeds": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, vp, vp]),
+        "mkg_quat_to_rot": (C.c_int, [vp, vp]),
+        "mkg_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mkg_sync": (C.c_int, [vp]),
+        "mkg_step_count": (C.c_int64, [vp, C.c_int]),
+        "mkg_get_field": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_size_t]),
+        "mkg_set_field": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_size_t]),
+        "mkg_reduce": (C.c_int, [vp, C.c_int, vp]),
+        "mkg_checkpoint_save": (C.c_int, [vp, C.c_char_p]),
+        "mkg_checkpoint_load": (C.c_int, [vp, C.c_char_p]),
+        "mkg_last_timing": (C.c_int, [vp, vp]),
+        "mkg_enable_timing": (C.c_int, [vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.mkg_params_size() != C.sizeof(MkgParams):
+        raise MkgError("mkg_params layout mismatch between libmkg.so and mkg.py")
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise MkgError(lib().mkg_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_params(**kw) -> MkgParams:
+    p = MkgParams()
+    _check(lib().mkg_params_default(C.byref(p)))
+    p.set(**kw)
+    return p
+
+
+def params_from_xml(path: str | Path) -> tuple[MkgParams, dict]:
+    p = MkgParams()
+    info = (C.c_int32 * 4)()
+    _check(lib().mkg_params_from_xml(str(path).encode(), C.byref(p), info))
+    return p, {"numLoops": info[0], "numFrames": info[1], "seeds": info[2]}
+
+
+def seeds_from_xml(path: str | Path, max_seeds: int = MAX_GRAINS):
+    centers = np.zeros((max_seeds, 3))
+    radii = np.zeros(max_seeds)
+    rot = np.zeros((max_seeds, 9))
+    n = lib().mkg_seeds_from_xml(str(path).encode(), max_seeds, _ptr(centers), _ptr(radii),
+                                  _ptr(rot))
+    if n < 0:
+        raise MkgError(lib().mkg_last_error().decode())
+    return centers[:n].copy(), radii[:n].copy(), rot[:n].copy()
+
+
+def random_seeds(seed: int, n: int, radius: float, dims) -> tuple[np.ndarray, np.ndarray]:
+    centers = np.zeros((n, 3))
+    quats = np.zeros((n, 4))
+    _check(lib().mkg_random_seeds(seed, n, radius, int(dims[0]), int(dims[1]), int(dims[2]),
+                                  _ptr(centers), _ptr(quats)))
+    return centers, quats
+
+
+def quat_to_rot(q) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    R = np.zeros(9)
+    _check(lib().mkg_quat_to_rot(_ptr(q), _ptr(R)))
+    return R.reshape(3, 3)
+
+
+class Solver:
+    """One device-resident case (wraps an ``mkg_handle``)."""
+
+    def __init__(self, params: MkgParams):
+        self.params = params.copy()
+        h = C.c_void_p()
+        _check(lib().mkg_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.N = self.params.ncell
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mkg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state
+    def init_equilibrium(self, rho=None, vel=None, T=None, C=None, phi=None) -> None:
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                for v in (rho, vel, T, C, phi)]
+        _check(lib().mkg_init_equilibrium(self._h, *[_ptr(a) for a in arrs]))
+
+    def init_seeds(self, centers, radii) -> None:
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
+        r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        _check(lib().mkg_init_seeds(self._h, len(r), _ptr(c), _ptr(r)))
+
+    def set_params(self, params: MkgParams) -> None:
+        _check(lib().mkg_set_params(self._h, C.byref(params)))
+        self.params = params.copy()
+
+    def step(self, nsteps: int = 1, mask: int = ALL) -> None:
+        _check(lib().mkg_step(self._h, int(nsteps), int(mask)))
+
+    def sync(self) -> None:
+        _check(lib().mkg_sync(self._h))
+
+    def step_count(self, bit: int) -> int:
+        return int(lib().mkg_step_count(self._h, bit))
+
+    def _count(self, field: int) -> int:
+        p = self.params
+        N = self.N
+        return {F_POP_FLOW: p.flow_q * N, F_POP_PHASE: p.phase_q * N, F_POP_THERMAL: 7 * N,
+                F_POP_SOLUTE: 7 * N, F_VEL: 3 * N}.get(field, N)
+
+    def get(self, field: int, dtype=np.float64) -> np.ndarray:
+        n = self._count(field)
+        if field == F_FLAG:
+            out = np.empty(n, dtype=np.int8)
+            dt = DT_I8
+        elif field == F_GRAIN:
+            out = np.empty(n, dtype=np.int16)
+            dt = DT_I16
+        else:
+            out = np.empty(n, dtype=dtype)
+            dt = DT_F64 if out.dtype == np.float64 else DT_F32
+        _check(lib().mkg_get_field(self._h, field, _ptr(out), dt, n))
+        return out
+
+    def set(self, field: int, arr) -> None:
+        if field == F_FLAG:
+            a = np.ascontiguousarray(arr, dtype=np.int8).reshape(-1)
+            dt = DT_I8
+        elif field == F_GRAIN:
+            a = np.ascontiguousarray(arr, dtype=np.int16).reshape(-1)
+            dt = DT_I16
+        else:
+            a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+            dt = DT_F64
+        _check(lib().mkg_set_field(self._h, field, _ptr(a), dt, a.size))
+
+    def reduce(self, which: int) -> np.ndarray:
+        out = np.zeros(4)
+        _check(lib().mkg_reduce(self._h, which, _ptr(out)))
+        return out[:3] if which in (R_MOMENTUM, R_TIP) else out[:1]
+
+    def checkpoint_save(self, path) -> None:
+        _check(lib().mkg_checkpoint_save(self._h, str(path).encode()))
+
+    def checkpoint_load(self, path) -> None:
+        _check(lib().mkg_checkpoint_load(self._h, str(path).encode()))
+
+    def enable_timing(self, on: bool = True) -> None:
+        _check(lib().mkg_enable_timing(self._h, 1 if on else 0))
+
+    def last_timing(self) -> dict:
+        out = np.zeros(4)
+        _check(lib().mkg_last_timing(self._h, _ptr(out)))
+        return {"phase_moments_ms": out[0], "fused_ms": out[1], "total_ms": out[2],
+                "steps": int(out[3])}

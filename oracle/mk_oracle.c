@@ -1,0 +1,762 @@
+/*
+ * mk_oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY; see mk_oracle.h).
+ *
+ * Written as a direct, unoptimised restatement: every step
+ *   1. (phase advanced)  phi_prev <- phi, phi <- sum_i g_i, flags re-classified from phi
+ *      (mkFluidsUser::relaxation first loop, MaterKit/src/mkFluidsUser.cpp:147-172, and
+ *       mkPhaseField::reset_flags, MaterKit/inc/mkPhaseField.h:37)
+ *   2. macroscopic moments of every present lattice at time level n
+ *   3. collision of every advanced lattice (post-collision buffers)
+ *   4. pull streaming with face rules and half-way bounce-back at wall cells
+ *      (pull form and "Plan A" bounce-back of mkFluidsUser::propagate,
+ *       MaterKit/src/mkFluidsUser.cpp:68-134)
+ *
+ * Published methods used (the reference's own bodies are in a closed DLL):
+ *   - second-order equilibrium, Guo-Zheng-Shi (2002) forcing with u = (sum c f + F/2)/rho
+ *   - TRT (Ginzburg 2008) with the magic parameter Lambda
+ *   - MRT as relaxation of w-orthogonal moment subspaces (mass, momentum, traceless shear,
+ *     bulk, odd ghosts, even ghosts), projectors built numerically from the velocity set
+ *   - partial ("porous") bounce-back of Walsh, Burwinkle & Saar (2009) for the solid fraction
+ *   - Karma-Rappel (1998) anisotropic phase field, cubic a_s(n) = 1 - 3 eps4 + 4 eps4 sum n^4,
+ *     solved as a D3Q19/D3Q7 LB diffusion (omega_g = 1/(dt/dx^2/cs^2 + 1/2), the form at
+ *     MaterKit/src/mkFluidsUser.cpp:282) plus the source w_i * S, where S carries the
+ *     anisotropic face-flux divergence and Q = (phi - lambda u (1-phi^2))(1-phi^2)
+ *     (MaterKit/src/mkFluidsUser.cpp:261-264)
+ *   - D3Q7 advection-diffusion for T and for the dilute-alloy variable c = A(phi) U
+ *     (Karma 2001, Echebarria et al. 2004), latent/rejection source w_i L dphi
+ */
+#include "mk_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ lattices */
+/* Direction order: rest, 6 faces (+x,-x,+y,-y,+z,-z), 12 edges, 8 corners; opposite pairs
+ * are (2k-1, 2k).  x is the slowest axis of the linear index (x*ny + y)*nz + z. */
+static const int E27[27][3] = {
+    {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+    {1, 1, 0},   {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 0, 1},   {-1, 0, -1}, {1, 0, -1},
+    {-1, 0, 1},  {0, 1, 1},   {0, -1, -1}, {0, 1, -1},  {0, -1, 1},  {1, 1, 1},  {-1, -1, -1},
+    {1, 1, -1},  {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}};
+
+typedef struct {
+    int q;
+    double w[27];
+    double cs2;
+} lat_t;
+
+static lat_t make_lat(int q) {
+    lat_t L;
+    memset(&L, 0, sizeof(L));
+    L.q = q;
+    for (int i = 0; i < q; i++) {
+        int n = abs(E27[i][0]) + abs(E27[i][1]) + abs(E27[i][2]);
+        if (q == 7) L.w[i] = (n == 0) ? 0.25 : 0.125;
+        if (q == 19) L.w[i] = (n == 0) ? 1.0 / 3.0 : (n == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+        if (q == 27)
+            L.w[i] = (n == 0) ? 8.0 / 27.0
+                              : (n == 1 ? 2.0 / 27.0 : (n == 2 ? 1.0 / 54.0 : 1.0 / 216.0));
+    }
+    L.cs2 = (q == 7) ? 0.25 : 1.0 / 3.0;
+    return L;
+}
+
+static int inv_dir(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+int orc_lattice(int q, int* e, double* w, int* inv, double* cs2) {
+    if (q != 7 && q != 19 && q != 27) return -1;
+    lat_t L = make_lat(q);
+    for (int i = 0; i < q; i++) {
+        for (int d = 0; d < 3; d++) e[3 * i + d] = E27[i][d];
+        w[i] = L.w[i];
+        inv[i] = inv_dir(i);
+    }
+    *cs2 = L.cs2;
+    return 0;
+}
+
+size_t orc_params_size(void) { return sizeof(orc_params); }
+
+/* ------------------------------------------------------------------ RNG */
+uint64_t orc_splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double orc_uniform(uint64_t* s) { return (double)(orc_splitmix64(s) >> 11) * 0x1.0p-53; }
+double orc_cell_uniform(uint64_t seed, uint64_t cell) {
+    uint64_t s = seed ^ (cell * 0xD1B54A32D192ED03ULL);
+    return orc_uniform(&s);
+}
+
+/* ------------------------------------------------------------------ helpers */
+#define NCELL(p) ((long)(p)->nx * (p)->ny * (p)->nz)
+static long lin(const orc_params* p, int x, int y, int z) {
+    return ((long)x * p->ny + y) * p->nz + z;
+}
+
+/* Resolve a coordinate against the domain.  Returns the face (0..5) through which the point
+ * leaves a non-periodic side, or -1 (point inside, periodic sides wrapped in place).  Axes
+ * are examined in the order x, y, z. */
+static int resolve(const orc_params* p, const int* bc, int* X, int* Y, int* Z) {
+    int dims[3] = {p->nx, p->ny, p->nz};
+    int* c[3] = {X, Y, Z};
+    for (int a = 0; a < 3; a++) {
+        int v = *c[a];
+        if (v < 0 || v >= dims[a]) {
+            int face = 2 * a + (v < 0 ? 0 : 1);
+            if (bc[face] == 0) {
+                *c[a] = (v + dims[a]) % dims[a];
+            } else {
+                return face;
+            }
+        }
+    }
+    return -1;
+}
+
+/* Phase-field stencil neighbour: periodic sides wrap, other sides replicate the edge. */
+static int clampwrap(int v, int n, int periodic) {
+    if (v >= 0 && v < n) return v;
+    if (periodic) return (v + n) % n;
+    return v < 0 ? 0 : n - 1;
+}
+
+/* ------------------------------------------------------------------ equilibria */
+void orc_equilibrium(int q, double rho, const double* u, double* feq) {
+    lat_t L = make_lat(q);
+    double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int i = 0; i < q; i++) {
+        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+        feq[i] = L.w[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+    }
+}
+
+/* Guo forcing term without its rate factor: w_i [3 (c-u).F + 9 (c.u)(c.F)] */
+static void guo_term(int q, const double* u, const double* F, double* Fi) {
+    lat_t L = make_lat(q);
+    for (int i = 0; i < q; i++) {
+        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+        double cF = E27[i][0] * F[0] + E27[i][1] * F[1] + E27[i][2] * F[2];
+        double uF = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+        Fi[i] = L.w[i] * (3.0 * (cF - uF) + 9.0 * cu * cF);
+    }
+}
+
+/* ------------------------------------------------------------------ MRT projectors */
+/* Projector onto span(B) in the inner product <a,b> = sum a_i b_i / w_i:
+ * P = B (B^T W^-1 B)^-1 B^T W^-1, B is q x m. */
+static void projector(int q, const double* w, const double* B, int m, double* P) {
+    double G[27 * 27], Gi[27 * 27];
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) {
+            double s = 0;
+            for (int i = 0; i < q; i++) s += B[i * m + a] * B[i * m + b] / w[i];
+            G[a * m + b] = s;
+        }
+    /* Gauss-Jordan inverse of the m x m Gram matrix */
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b < m; b++) Gi[a * m + b] = (a == b) ? 1.0 : 0.0;
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (fabs(G[r * m + col]) > fabs(G[piv * m + col])) piv = r;
+        for (int k = 0; k < m; k++) {
+            double t = G[col * m + k];
+            G[col * m + k] = G[piv * m + k];
+            G[piv * m + k] = t;
+            t = Gi[col * m + k];
+            Gi[col * m + k] = Gi[piv * m + k];
+            Gi[piv * m + k] = t;
+        }
+        double d = G[col * m + col];
+        for (int k = 0; k < m; k++) {
+            G[col * m + k] /= d;
+            Gi[col * m + k] /= d;
+        }
+        for (int r = 0; r < m; r++)
+            if (r != col) {
+                double f = G[r * m + col];
+                for (int k = 0; k < m; k++) {
+                    G[r * m + k] -= f * G[col * m + k];
+                    Gi[r * m + k] -= f * Gi[col * m + k];
+                }
+            }
+    }
+    for (int i = 0; i < q; i++)
+        for (int j = 0; j < q; j++) {
+            double s = 0;
+            for (int a = 0; a < m; a++)
+                for (int b = 0; b < m; b++) s += B[i * m + a] * Gi[a * m + b] * B[j * m + b];
+            P[i * q + j] = s / w[j];
+        }
+}
+
+typedef struct {
+    int built;
+    double mom[27 * 27], shear[27 * 27], bulk[27 * 27], godd[27 * 27], geven[27 * 27];
+} mrt_t;
+static mrt_t MRT19, MRT27;
+
+static void build_mrt(int q, mrt_t* M) {
+    if (M->built) return;
+    lat_t L = make_lat(q);
+    double B[27 * 5] = {0}, Pm[27 * 27];
+    /* mass */
+    for (int i = 0; i < q; i++) B[i] = L.w[i];
+    projector(q, L.w, B, 1, Pm);
+    /* momentum */
+    for (int i = 0; i < q; i++)
+        for (int a = 0; a < 3; a++) B[i * 3 + a] = L.w[i] * E27[i][a];
+    projector(q, L.w, B, 3, M->mom);
+    /* traceless shear: xy, xz, yz, xx-yy, yy-zz */
+    for (int i = 0; i < q; i++) {
+        const int* c = E27[i];
+        B[i * 5 + 0] = L.w[i] * c[0] * c[1];
+        B[i * 5 + 1] = L.w[i] * c[0] * c[2];
+        B[i * 5 + 2] = L.w[i] * c[1] * c[2];
+        B[i * 5 + 3] = L.w[i] * (c[0] * c[0] - c[1] * c[1]);
+        B[i * 5 + 4] = L.w[i] * (c[1] * c[1] - c[2] * c[2]);
+    }
+    projector(q, L.w, B, 5, M->shear);
+    /* bulk */
+    for (int i = 0; i < q; i++) {
+        const int* c = E27[i];
+        B[i] = L.w[i] * (c[0] * c[0] + c[1] * c[1] + c[2] * c[2] - 3.0 * L.cs2);
+    }
+    projector(q, L.w, B, 1, M->bulk);
+    /* ghosts = complement, split by parity under inversion */
+    for (int i = 0; i < q; i++)
+        for (int j = 0; j < q; j++) {
+            double id = (i == j) ? 1.0 : 0.0;
+            double g = id - Pm[i * q + j] - M->mom[i * q + j] - M->shear[i * q + j] -
+                       M->bulk[i * q + j];
+            /* P_ghost * (I -/+ R)/2 : column j of R is e_{inv(j)} */
+            M->godd[i * q + j] = 0.5 * g;
+            M->geven[i * q + j] = 0.5 * g;
+        }
+    double tmp_odd[27 * 27], tmp_even[27 * 27];
+    for (int i = 0; i < q; i++)
+        for (int j = 0; j < q; j++) {
+            int jr = inv_dir(j);
+            tmp_odd[i * q + j] = M->godd[i * q + j] - M->godd[i * q + jr];
+            tmp_even[i * q + j] = M->geven[i * q + j] + M->geven[i * q + jr];
+        }
+    memcpy(M->godd, tmp_odd, sizeof(tmp_odd));
+    memcpy(M->geven, tmp_even, sizeof(tmp_even));
+    M->built = 1;
+}
+
+static void matvec(int q, const double* P, const double* v, double* out) {
+    for (int i = 0; i < q; i++) {
+        double s = 0;
+        for (int j = 0; j < q; j++) s += P[i * q + j] * v[j];
+        out[i] = s;
+    }
+}
+
+/* ------------------------------------------------------------------ flow collision */
+/* f: pre-collision populations of one cell, F: force density.  out: post-collision.
+ * ns: partial bounce-back solid fraction (Walsh et al. 2009). */
+void orc_collide_cell(const orc_params* p, int op, const double* f, const double* F, double* out,
+                      double ns) {
+    int q = p->flow_q;
+    double rho = 0, j[3] = {0, 0, 0};
+    for (int i = 0; i < q; i++) {
+        rho += f[i];
+        for (int a = 0; a < 3; a++) j[a] += E27[i][a] * f[i];
+    }
+    double u[3];
+    for (int a = 0; a < 3; a++) u[a] = (j[a] + 0.5 * F[a]) / rho;
+    double feq[27], Fi[27], fs[27];
+    orc_equilibrium(q, rho, u, feq);
+    guo_term(q, u, F, Fi);
+    if (op == ORC_OP_BGK) {
+        double om = p->s_nu;
+        for (int i = 0; i < q; i++) fs[i] = f[i] - om * (f[i] - feq[i]) + (1.0 - 0.5 * om) * Fi[i];
+    } else if (op == ORC_OP_TRT) {
+        double wp = p->s_nu, wm = p->s_odd;
+        for (int i = 0; i < q; i++) {
+            int k = inv_dir(i);
+            double fp = 0.5 * (f[i] + f[k]), fm = 0.5 * (f[i] - f[k]);
+            double ep = 0.5 * (feq[i] + feq[k]), em = 0.5 * (feq[i] - feq[k]);
+            double Fp = 0.5 * (Fi[i] + Fi[k]), Fm = 0.5 * (Fi[i] - Fi[k]);
+            fs[i] = f[i] - wp * (fp - ep) - wm * (fm - em) + (1.0 - 0.5 * wp) * Fp +
+                    (1.0 - 0.5 * wm) * Fm;
+        }
+    } else {
+        mrt_t* M = (q == 19) ? &MRT19 : &MRT27;
+        build_mrt(q, M);
+        double neq[27], t[27];
+        for (int i = 0; i < q; i++) {
+            neq[i] = f[i] - feq[i];
+            fs[i] = feq[i];
+        }
+        const double* Ps[5] = {M->mom, M->shear, M->bulk, M->godd, M->geven};
+        double rates[5] = {1.0, p->s_nu, p->s_bulk, p->s_odd, p->s_even};
+        for (int k = 0; k < 5; k++) {
+            matvec(q, Ps[k], neq, t);
+            for (int i = 0; i < q; i++) fs[i] += (1.0 - rates[k]) * t[i];
+            matvec(q, Ps[k], Fi, t);
+            for (int i = 0; i < q; i++) fs[i] += (1.0 - 0.5 * rates[k]) * t[i];
+        }
+    }
+    for (int i = 0; i < q; i++) out[i] = (1.0 - ns) * fs[i] + ns * f[inv_dir(i)];
+}
+
+/* ------------------------------------------------------------------ phase field */
+static double aniso_as(double eps4, const double* n) {
+    return 1.0 - 3.0 * eps4 + 4.0 * eps4 * (n[0] * n[0] * n[0] * n[0] + n[1] * n[1] * n[1] * n[1] +
+                                            n[2] * n[2] * n[2] * n[2]);
+}
+
+/* Normal component along `axis` of J = a_s^2 grad phi + |grad phi|^2 a_s d a_s / d grad phi,
+ * evaluated from a face gradient g (lab frame); R (lab <- crystal) may be NULL. */
+static double face_flux(const orc_params* p, const double* R, const double* g, int axis) {
+    double gc[3];
+    if (R) {
+        for (int a = 0; a < 3; a++) gc[a] = R[0 * 3 + a] * g[0] + R[1 * 3 + a] * g[1] + R[2 * 3 + a] * g[2];
+    } else {
+        gc[0] = g[0];
+        gc[1] = g[1];
+        gc[2] = g[2];
+    }
+    double g2 = gc[0] * gc[0] + gc[1] * gc[1] + gc[2] * gc[2];
+    if (g2 <= 1e-30) return g[axis];
+    double gn = sqrt(g2);
+    double n[3] = {gc[0] / gn, gc[1] / gn, gc[2] / gn};
+    double s4 = n[0] * n[0] * n[0] * n[0] + n[1] * n[1] * n[1] * n[1] + n[2] * n[2] * n[2] * n[2];
+    double as = 1.0 - 3.0 * p->eps4 + 4.0 * p->eps4 * s4;
+    double Nc[3];
+    for (int a = 0; a < 3; a++) Nc[a] = 16.0 * p->eps4 * as * gn * (n[a] * n[a] * n[a] - n[a] * s4);
+    double Nl = R ? (R[axis * 3 + 0] * Nc[0] + R[axis * 3 + 1] * Nc[1] + R[axis * 3 + 2] * Nc[2])
+                  : Nc[axis];
+    return as * as * g[axis] + Nl;
+}
+
+#define P27(dx, dy, dz) phi27[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
+
+/* Source S of the phase-field LB equation for one cell from its 3x3x3 phi neighbourhood.
+ * S = dt/a^2 [ (div J - a^2 lap phi) W0^2/dx^2 + Q ],  Q = (phi - lambda u (1-phi^2))(1-phi^2)
+ * with J on the six faces (normal difference across the face, transverse central differences
+ * averaged over the two cells), lap the matching 7-point Laplacian, and a = a_s of the
+ * isotropic LB-weighted cell gradient (compute_normal_LB form, MaterKit/src/mkFluidsUser.cpp:
+ * 211-217). */
+void orc_phase_source(const orc_params* p, const double* phi27, double upf, int grain,
+                      double* S_out, double* a_out) {
+    const double* R = (p->ngrains > 0 && grain > 0) ? p->rot[grain - 1] : NULL;
+    double divJ = 0, lap = 0;
+    for (int ax = 0; ax < 3; ax++) {
+        for (int side = 1; side >= -1; side -= 2) {
+            int o[3] = {0, 0, 0};
+            o[ax] = side;
+            double g[3];
+            for (int b = 0; b < 3; b++) {
+                if (b == ax) {
+                    g[b] = side * (P27(o[0], o[1], o[2]) - P27(0, 0, 0));
+                } else {
+                    int e[3] = {0, 0, 0};
+                    e[b] = 1;
+                    g[b] = 0.25 * (P27(e[0], e[1], e[2]) - P27(-e[0], -e[1], -e[2]) +
+                                   P27(o[0] + e[0], o[1] + e[1], o[2] + e[2]) -
+                                   P27(o[0] - e[0], o[1] - e[1], o[2] - e[2]));
+                }
+            }
+            double J = face_flux(p, R, g, ax);
+            divJ += side * J;
+            lap += side * g[ax];
+        }
+    }
+    /* isotropic LB-weighted cell gradient, D3Q19 weights, / cs^2 */
+    lat_t L = make_lat(19);
+    double gr[3] = {0, 0, 0};
+    for (int i = 1; i < 19; i++)
+        for (int a = 0; a < 3; a++) gr[a] += 3.0 * L.w[i] * E27[i][a] * P27(E27[i][0], E27[i][1], E27[i][2]);
+    double a = 1.0;
+    double grc[3];
+    if (R) {
+        for (int k = 0; k < 3; k++) grc[k] = R[0 * 3 + k] * gr[0] + R[1 * 3 + k] * gr[1] + R[2 * 3 + k] * gr[2];
+    } else {
+        grc[0] = gr[0];
+        grc[1] = gr[1];
+        grc[2] = gr[2];
+    }
+    double gg = grc[0] * grc[0] + grc[1] * grc[1] + grc[2] * grc[2];
+    if (gg > 1e-30) {
+        double gn = sqrt(gg);
+        double n[3] = {grc[0] / gn, grc[1] / gn, grc[2] / gn};
+        a = aniso_as(p->eps4, n);
+    }
+    double ph = P27(0, 0, 0);
+    double d = 1.0 - ph * ph;
+    double Q = (ph - p->lambda * upf * d) * d;
+    double a2 = a * a;
+    *S_out = p->pf_dt / a2 * ((divJ - a2 * lap) * p->pf_wdx2 + Q);
+    if (a_out) *a_out = a;
+}
+
+/* ------------------------------------------------------------------ moments */
+static double solid_fraction(double phi) {
+    double fs = 0.5 * (1.0 + phi);
+    return fs < 0 ? 0 : (fs > 1 ? 1 : fs);
+}
+
+typedef struct {
+    double *rho, *u, *T, *C, *U, *ns;
+} macro_t;
+
+static void compute_macro(const orc_params* p, const orc_state* s, macro_t* m) {
+    long N = NCELL(p);
+    lat_t Lf = make_lat(p->flow_q ? p->flow_q : 19);
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < N; x++) {
+        double phi = p->phase_q ? s->phi[x] : -1.0;
+        if (p->thermal) {
+            double T = 0;
+            for (int i = 0; i < 7; i++) T += s->h[i * N + x];
+            m->T[x] = T;
+        }
+        if (p->solute) {
+            double C = 0;
+            for (int i = 0; i < 7; i++) C += s->c[i * N + x];
+            double A = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
+            m->C[x] = C;
+            m->U[x] = C / A;
+        }
+        double ns = 0;
+        if (p->phase_q) {
+            if (s->flag[x] == 2) ns = 1.0;
+            else if (s->flag[x] == 1) ns = solid_fraction(phi);
+        }
+        m->ns[x] = ns;
+        if (p->flow_q) {
+            double rho = 0, j[3] = {0, 0, 0};
+            for (int i = 0; i < p->flow_q; i++) {
+                double fi = s->f[i * N + x];
+                rho += fi;
+                for (int a = 0; a < 3; a++) j[a] += E27[i][a] * fi;
+            }
+            double F[3];
+            for (int a = 0; a < 3; a++) {
+                F[a] = p->force[a];
+                if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (m->T[x] - p->T_ref);
+            }
+            m->rho[x] = rho;
+            for (int a = 0; a < 3; a++) m->u[3 * x + a] = (j[a] + 0.5 * F[a]) / rho;
+        } else {
+            for (int a = 0; a < 3; a++) m->u[3 * x + a] = 0;
+        }
+        (void)Lf;
+    }
+}
+
+void orc_moments(const orc_params* p, const orc_state* s, double* rho, double* vel3, double* T,
+                 double* C, double* phi_now) {
+    long N = NCELL(p);
+    macro_t m;
+    m.rho = malloc(N * sizeof(double));
+    m.u = malloc(3 * N * sizeof(double));
+    m.T = malloc(N * sizeof(double));
+    m.C = malloc(N * sizeof(double));
+    m.U = malloc(N * sizeof(double));
+    m.ns = malloc(N * sizeof(double));
+    compute_macro(p, s, &m);
+    if (rho && p->flow_q) memcpy(rho, m.rho, N * sizeof(double));
+    if (vel3) memcpy(vel3, m.u, 3 * N * sizeof(double));
+    if (T && p->thermal) memcpy(T, m.T, N * sizeof(double));
+    if (C && p->solute) memcpy(C, m.C, N * sizeof(double));
+    if (phi_now && p->phase_q) {
+        lat_t L = make_lat(p->phase_q);
+        for (long x = 0; x < N; x++) {
+            double ph = 0;
+            for (int i = 0; i < L.q; i++) ph += s->g[i * N + x];
+            phi_now[x] = ph;
+        }
+    }
+    free(m.rho); free(m.u); free(m.T); free(m.C); free(m.U); free(m.ns);
+}
+
+/* ------------------------------------------------------------------ streaming */
+/* post: post-collision populations of the lattice; out: new pre-collision populations.
+ * kind 0 flow (bounce with moving-wall term), 1 scalar (noflux / Dirichlet), value per face. */
+static void stream(const orc_params* p, const orc_state* s, int q, const int* bc, int kind,
+                   const double* fvals, const double* post, const double* old, double* out) {
+    long N = NCELL(p);
+    lat_t L = make_lat(q);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int x = 0; x < p->nx; x++)
+        for (int y = 0; y < p->ny; y++)
+            for (int z = 0; z < p->nz; z++) {
+                long id = lin(p, x, y, z);
+                if (s->flag[id] == 3) {
+                    for (int i = 0; i < q; i++) out[i * N + id] = old[i * N + id];
+                    continue;
+                }
+                for (int i = 0; i < q; i++) {
+                    int X = x - E27[i][0], Y = y - E27[i][1], Z = z - E27[i][2];
+                    int face = resolve(p, bc, &X, &Y, &Z);
+                    int k = inv_dir(i);
+                    double v;
+                    if (face >= 0) {
+                        int t = bc[face];
+                        if (t == 3) {
+                            v = post[i * N + id];
+                        } else if (t == 2 && kind == 1) {
+                            v = -post[k * N + id] + 2.0 * L.w[i] * fvals[face];
+                        } else {
+                            v = post[k * N + id];
+                            if (kind == 0) {
+                                const double* uw = p->u_face[face];
+                                double cu = E27[i][0] * uw[0] + E27[i][1] * uw[1] + E27[i][2] * uw[2];
+                                v += 6.0 * L.w[i] * p->rho0 * cu;
+                            }
+                        }
+                    } else {
+                        long src = lin(p, X, Y, Z);
+                        if (s->flag[src] == 3) v = post[k * N + id];
+                        else v = post[i * N + src];
+                    }
+                    out[i * N + id] = v;
+                }
+            }
+}
+
+/* ------------------------------------------------------------------ flags */
+static int8_t classify(const orc_params* p, int8_t old, double phi) {
+    if (old == 3 || old < 0) return old;
+    if (phi >= p->phi_s) return 2;
+    if (phi > -p->phi_s) return 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ one step */
+static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
+    long N = NCELL(p);
+    int adv_f = (mask & 1) && p->flow_q;
+    int adv_g = (mask & 2) && p->phase_q;
+    int adv_h = (mask & 4) && p->thermal;
+    int adv_c = (mask & 8) && p->solute;
+
+    if (adv_g) {
+        int qg = p->phase_q;
+#pragma omp parallel for schedule(static)
+        for (long x = 0; x < N; x++) {
+            double ph = 0;
+            for (int i = 0; i < qg; i++) ph += s->g[i * N + x];
+            s->phi_prev[x] = s->phi[x];
+            s->phi[x] = ph;
+            s->flag[x] = classify(p, s->flag[x], ph);
+        }
+    }
+
+    macro_t m;
+    m.rho = malloc(N * sizeof(double));
+    m.u = malloc(3 * N * sizeof(double));
+    m.T = malloc(N * sizeof(double));
+    m.C = malloc(N * sizeof(double));
+    m.U = malloc(N * sizeof(double));
+    m.ns = malloc(N * sizeof(double));
+    compute_macro(p, s, &m);
+
+    double* post = NULL;
+    size_t maxq = 27;
+    post = malloc(maxq * N * sizeof(double));
+    double* fresh = malloc(maxq * N * sizeof(double));
+
+    /* flow */
+    if (adv_f) {
+        int q = p->flow_q;
+#pragma omp parallel for schedule(static)
+        for (long x = 0; x < N; x++) {
+            double fin[27], fo[27], F[3];
+            if (s->flag[x] == 3) continue;
+            for (int i = 0; i < q; i++) fin[i] = s->f[i * N + x];
+            for (int a = 0; a < 3; a++) {
+                F[a] = p->force[a];
+                if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (m.T[x] - p->T_ref);
+            }
+            orc_collide_cell(p, op, fin, F, fo, m.ns[x]);
+            for (int i = 0; i < q; i++) post[i * N + x] = fo[i];
+        }
+        stream(p, s, q, p->bc_f, 0, NULL, post, s->f, fresh);
+        memcpy(s->f, fresh, (size_t)q * N * sizeof(double));
+    }
+
+    /* advection velocity seen by the scalar lattices */
+    double* uadv = malloc(3 * N * sizeof(double));
+    for (long x = 0; x < N; x++)
+        for (int a = 0; a < 3; a++) uadv[3 * x + a] = (1.0 - m.ns[x]) * m.u[3 * x + a];
+
+    /* phase field */
+    if (adv_g) {
+        int q = p->phase_q;
+        lat_t L = make_lat(q);
+        double omg = 1.0 / (0.5 + p->pf_dt * p->pf_wdx2 / L.cs2);
+        int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int x = 0; x < p->nx; x++)
+            for (int y = 0; y < p->ny; y++)
+                for (int z = 0; z < p->nz; z++) {
+                    long id = lin(p, x, y, z);
+                    if (s->flag[id] == 3) continue;
+                    double phi27[27];
+                    for (int dx = -1; dx <= 1; dx++)
+                        for (int dy = -1; dy <= 1; dy++)
+                            for (int dz = -1; dz <= 1; dz++) {
+                                int X = clampwrap(x + dx, p->nx, per[0]);
+                                int Y = clampwrap(y + dy, p->ny, per[1]);
+                                int Z = clampwrap(z + dz, p->nz, per[2]);
+                                P27(dx, dy, dz) = s->phi[lin(p, X, Y, Z)];
+                            }
+                    double upf = 0;
+                    if (p->pf_coupling == 1 && p->thermal) upf = m.T[id];
+                    if (p->pf_coupling == 2 && p->solute) upf = m.U[id];
+                    double S, a;
+                    orc_phase_source(p, phi27, upf, s->grain ? s->grain[id] : 0, &S, &a);
+                    double ph = s->phi[id];
+                    const double* u = uadv + 3 * id;
+                    for (int i = 0; i < q; i++) {
+                        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                        double geq = L.w[i] * ph * (1.0 + cu / L.cs2);
+                        double gi = s->g[i * N + id];
+                        post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * S;
+                    }
+                }
+        stream(p, s, q, p->bc_g, 1, NULL, post, s->g, fresh);
+        memcpy(s->g, fresh, (size_t)q * N * sizeof(double));
+    }
+
+    /* thermal */
+    if (adv_h) {
+        lat_t L = make_lat(7);
+#pragma omp parallel for schedule(static)
+        for (long x = 0; x < N; x++) {
+            if (s->flag[x] == 3) continue;
+            double phi = p->phase_q ? s->phi[x] : -1.0;
+            double fs = p->phase_q ? solid_fraction(phi) : 0.0;
+            double alpha = p->alpha_l * (1.0 - fs) + p->alpha_s * fs;
+            if (p->has_kappa && s->kappa) alpha *= s->kappa[x];
+            double om = 1.0 / (0.5 + alpha / L.cs2);
+            double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
+            double T = m.T[x];
+            const double* u = uadv + 3 * x;
+            for (int i = 0; i < 7; i++) {
+                double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                double heq = L.w[i] * T * (1.0 + cu / L.cs2);
+                double hi = s->h[i * N + x];
+                post[i * N + x] = hi - om * (hi - heq) + L.w[i] * p->latent_T * dphi;
+            }
+        }
+        stream(p, s, 7, p->bc_h, 1, p->T_face, post, s->h, fresh);
+        memcpy(s->h, fresh, (size_t)7 * N * sizeof(double));
+    }
+
+    /* solute */
+    if (adv_c) {
+        lat_t L = make_lat(7);
+#pragma omp parallel for schedule(static)
+        for (long x = 0; x < N; x++) {
+            if (s->flag[x] == 3) continue;
+            double phi = p->phase_q ? s->phi[x] : -1.0;
+            double q = 0.5 * (1.0 - phi);
+            if (p->D_l > 0) q += (p->D_s / p->D_l) * 0.5 * (1.0 + phi);
+            if (q < p->q_min) q = p->q_min;
+            double om = 1.0 / (0.5 + p->D_l * q / L.cs2);
+            double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
+            double C = m.C[x], U = m.U[x];
+            const double* u = uadv + 3 * x;
+            for (int i = 0; i < 7; i++) {
+                double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                double ceq = (i == 0) ? C - (1.0 - L.w[0]) * U : L.w[i] * U * (1.0 + cu / L.cs2);
+                double ci = s->c[i * N + x];
+                post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * p->latent_C * dphi;
+            }
+        }
+        stream(p, s, 7, p->bc_c, 1, p->C_face, post, s->c, fresh);
+        memcpy(s->c, fresh, (size_t)7 * N * sizeof(double));
+    }
+
+    free(uadv);
+    free(post);
+    free(fresh);
+    free(m.rho); free(m.u); free(m.T); free(m.C); free(m.U); free(m.ns);
+}
+
+int orc_step(const orc_params* p, orc_state* s, int mask, int op, int nsteps) {
+    if (p->flow_q && p->flow_q != 19 && p->flow_q != 27) return -1;
+    if (p->phase_q && p->phase_q != 7 && p->phase_q != 19) return -1;
+    if (p->flow_q == 19) build_mrt(19, &MRT19);
+    if (p->flow_q == 27) build_mrt(27, &MRT27);
+    for (int n = 0; n < nsteps; n++) one_step(p, s, mask, op);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ initial states */
+void orc_init_equilibrium(const orc_params* p, orc_state* s, const double* rho,
+                          const double* vel3, const double* T, const double* C,
+                          const double* phi) {
+    long N = NCELL(p);
+    for (long x = 0; x < N; x++) {
+        double ph = phi ? phi[x] : 0.0;
+        double u[3] = {0, 0, 0};
+        if (vel3)
+            for (int a = 0; a < 3; a++) u[a] = vel3[3 * x + a];
+        if (p->phase_q && phi) {
+            lat_t L = make_lat(p->phase_q);
+            for (int i = 0; i < L.q; i++) s->g[i * N + x] = L.w[i] * ph;
+            s->phi[x] = ph;
+            s->phi_prev[x] = ph;
+            s->flag[x] = classify(p, s->flag[x], ph);
+        }
+        if (p->flow_q) {
+            double feq[27];
+            orc_equilibrium(p->flow_q, rho ? rho[x] : p->rho0, u, feq);
+            for (int i = 0; i < p->flow_q; i++) s->f[i * N + x] = feq[i];
+        }
+        lat_t L7 = make_lat(7);
+        if (p->thermal) {
+            double t = T ? T[x] : p->T_ref;
+            for (int i = 0; i < 7; i++) s->h[i * N + x] = L7.w[i] * t;
+        }
+        if (p->solute) {
+            double c = C ? C[x] : 0.0;
+            for (int i = 0; i < 7; i++) s->c[i * N + x] = L7.w[i] * c;
+        }
+    }
+}
+
+void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* centers,
+                    const double* radii) {
+    /* interface half-width in cells: sqrt(2) W0 / dx  (W0/dx = sqrt(pf_wdx2)) */
+    double width = sqrt(2.0) * sqrt(p->pf_wdx2);
+    double ts = atanh(p->phi_s) * width;
+    for (int x = 0; x < p->nx; x++)
+        for (int y = 0; y < p->ny; y++)
+            for (int z = 0; z < p->nz; z++) {
+                long id = lin(p, x, y, z);
+                double best = 1e300;
+                int gbest = 0;
+                for (int k = 0; k < n; k++) {
+                    double dx = x - centers[3 * k], dy = y - centers[3 * k + 1],
+                           dz = z - centers[3 * k + 2];
+                    double d = sqrt(dx * dx + dy * dy + dz * dz) - radii[k];
+                    if (d < best) {
+                        best = d;
+                        gbest = k + 1;
+                    }
+                }
+                double ph = -tanh(best / width);
+                s->phi[id] = ph;
+                s->phi_prev[id] = ph;
+                {
+                    lat_t L = make_lat(p->phase_q ? p->phase_q : 19);
+                    if (p->phase_q)
+                        for (int i = 0; i < L.q; i++) s->g[i * NCELL(p) + id] = L.w[i] * ph;
+                }
+                if (s->grain) s->grain[id] = (int16_t)gbest;
+                if (s->flag[id] != 3 && s->flag[id] >= 0)
+                    s->flag[id] = (best <= -ts) ? 2 : (best < ts ? 1 : 0);
+            }
+}

@@ -1,0 +1,95 @@
+/*
+ * mk_oracle.h -- FP64 CPU oracle for the MaterKit lattice-Boltzmann / phase-field step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the step, written for
+ * readability: structure-of-arrays doubles, two-lattice pull streaming, textbook per-cell
+ * formulas.  Only the test-suite, the smoke check and the CPU-baseline leg of the benchmark
+ * may load it; the product library never links it.
+ *
+ * The reference's solver bodies exist only inside a closed Windows DLL
+ * (MaterKit/AiLBFlow.dll), so the formulas follow the reference headers and the visible
+ * user-extension loops (MaterKit/src/mkFluidsUser.cpp) plus the published methods named at
+ * each function in mk_oracle.c.  The parameter block has the same layout as the product's
+ * mkg_params so one Python description drives both.
+ */
+#ifndef MK_ORACLE_H
+#define MK_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#define ORC_MAX_GRAINS 64
+
+typedef struct orc_params {
+    int32_t nx, ny, nz;
+    int32_t flow_q, phase_q, thermal, solute, precision, flow_op, pf_coupling, ngrains,
+        has_kappa;
+    int32_t bc_f[6], bc_g[6], bc_h[6], bc_c[6];
+    int32_t pad0;
+    double s_nu, s_bulk, s_odd, s_even;
+    double rho0;
+    double force[3];
+    double buoy[3];
+    double T_ref;
+    double u_face[6][3];
+    double alpha_l, alpha_s, latent_T;
+    double T_face[6];
+    double D_l, D_s, k_part, latent_C, q_min;
+    double C_face[6];
+    double pf_dt, pf_wdx2, eps4, lambda, phi_s;
+    double ref_f, ref_g, ref_h, ref_c;
+    double rot[ORC_MAX_GRAINS][9];
+} orc_params;
+
+/* Collision operator selector for the flow lattice (the oracle keeps three separate
+ * implementations so each can be checked against the others' special cases). */
+#define ORC_OP_BGK 0
+#define ORC_OP_TRT 1
+#define ORC_OP_MRT 2
+
+typedef struct orc_state {
+    double* f;          /* flow_q  x N, pre-collision populations (full values) */
+    double* g;          /* phase_q x N */
+    double* h;          /* 7 x N */
+    double* c;          /* 7 x N */
+    double* phi;        /* N: phase field of the last phase moment pass */
+    double* phi_prev;   /* N: the pass before */
+    int8_t* flag;       /* N */
+    int16_t* grain;     /* N */
+    const double* kappa;/* N or NULL */
+} orc_state;
+
+size_t orc_params_size(void);
+
+/* One or more steps advancing the lattices in mask (1 flow, 2 phase, 4 thermal, 8 solute);
+ * op selects the flow collision implementation (ORC_OP_*). */
+int orc_step(const orc_params* p, orc_state* s, int mask, int op, int nsteps);
+
+/* Macroscopic fields at the current time level (any pointer may be NULL). */
+void orc_moments(const orc_params* p, const orc_state* s, double* rho, double* vel3,
+                 double* T, double* C, double* phi_now);
+
+/* Equilibrium initial state from macroscopic fields (NULL = uniform defaults). */
+void orc_init_equilibrium(const orc_params* p, orc_state* s, const double* rho,
+                          const double* vel3, const double* T, const double* C,
+                          const double* phi);
+
+/* Seed geometry: phi profile, grain ids (nearest seed surface), flags. */
+void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* centers,
+                    const double* radii);
+
+/* Single-cell helpers exposed for unit tests. */
+void orc_equilibrium(int q, double rho, const double* u, double* feq);
+void orc_collide_cell(const orc_params* p, int op, const double* f, const double* F, double* out,
+                      double ns);
+void orc_phase_source(const orc_params* p, const double* phi27, double upf, int grain,
+                      double* S_out, double* a_out);
+
+/* Lattice tables (for tests). */
+int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
+
+/* splitmix64 helpers shared with the golden-vector fixtures. */
+uint64_t orc_splitmix64(uint64_t* state);
+double orc_uniform(uint64_t* state);
+double orc_cell_uniform(uint64_t seed, uint64_t cell);
+
+#endif

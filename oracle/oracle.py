@@ -1,0 +1,203 @@
+"""Python binding of the FP64 CPU oracle (``liborc.so``).
+
+TEST INFRASTRUCTURE ONLY: used by ``tests/``, the smoke check and the CPU-baseline leg of
+``bench.py`` as the checker.  The product library never imports it.
+
+The oracle is a plain-C restatement of the MaterKit step (see ``mk_oracle.c`` for the
+reference file:line each part follows).  State lives in numpy arrays of full-value FP64
+populations, laid out as the product's canonical get/set format: ``f[i*N + cell]`` with
+``cell = (x*ny + y)*nz + z``.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import subprocess
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liborc.so"
+
+OP_BGK, OP_TRT, OP_MRT = 0, 1, 2
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile ``liborc.so`` with the committed Makefile."""
+    if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "mk_oracle.c").stat().st_mtime:
+        subprocess.run(["make", "-s", "-C", str(HERE)] + (["-B"] if force else []), check=True)
+    return LIB
+
+
+def lib() -> ct.CDLL:
+    global _lib
+    if _lib is None:
+        build()
+        L = ct.CDLL(str(LIB))
+        vp = ct.c_void_p
+        L.orc_params_size.restype = ct.c_size_t
+        L.orc_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.c_int]
+        L.orc_step.restype = ct.c_int
+        L.orc_moments.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_init_equilibrium.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_init_seeds.argtypes = [vp, vp, ct.c_int, vp, vp]
+        L.orc_equilibrium.argtypes = [ct.c_int, ct.c_double, vp, vp]
+        L.orc_collide_cell.argtypes = [vp, ct.c_int, vp, vp, vp, ct.c_double]
+        L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
+        L.orc_lattice.argtypes = [ct.c_int, vp, vp, vp, vp]
+        L.orc_lattice.restype = ct.c_int
+        L.orc_splitmix64.argtypes = [ct.POINTER(ct.c_uint64)]
+        L.orc_splitmix64.restype = ct.c_uint64
+        L.orc_uniform.argtypes = [ct.POINTER(ct.c_uint64)]
+        L.orc_uniform.restype = ct.c_double
+        L.orc_cell_uniform.argtypes = [ct.c_uint64, ct.c_uint64]
+        L.orc_cell_uniform.restype = ct.c_double
+        _lib = L
+    return _lib
+
+
+class _State(ct.Structure):
+    _fields_ = [("f", ct.c_void_p), ("g", ct.c_void_p), ("h", ct.c_void_p), ("c", ct.c_void_p),
+                ("phi", ct.c_void_p), ("phi_prev", ct.c_void_p), ("flag", ct.c_void_p),
+                ("grain", ct.c_void_p), ("kappa", ct.c_void_p)]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+def lattice(q: int):
+    e = np.zeros((q, 3), dtype=np.int32)
+    w = np.zeros(q)
+    inv = np.zeros(q, dtype=np.int32)
+    cs2 = ct.c_double()
+    assert lib().orc_lattice(q, _p(e), _p(w), _p(inv), ct.byref(cs2)) == 0
+    return e, w, inv, cs2.value
+
+
+@dataclass
+class State:
+    """Oracle state for one parameter block (params is an ``MkgParams``-layout struct)."""
+
+    params: object
+    f: np.ndarray | None = None
+    g: np.ndarray | None = None
+    h: np.ndarray | None = None
+    c: np.ndarray | None = None
+    phi: np.ndarray | None = None
+    phi_prev: np.ndarray | None = None
+    flag: np.ndarray = field(default=None)
+    grain: np.ndarray = field(default=None)
+    kappa: np.ndarray | None = None
+
+    @classmethod
+    def empty(cls, params) -> "State":
+        if lib().orc_params_size() != ct.sizeof(params):
+            raise RuntimeError("oracle parameter layout differs from the caller's")
+        N = int(params.nx) * int(params.ny) * int(params.nz)
+        s = cls(params=params)
+        if params.flow_q:
+            s.f = np.zeros(params.flow_q * N)
+        if params.phase_q:
+            s.g = np.zeros(params.phase_q * N)
+            s.phi = np.zeros(N)
+            s.phi_prev = np.zeros(N)
+        if params.thermal:
+            s.h = np.zeros(7 * N)
+        if params.solute:
+            s.c = np.zeros(7 * N)
+        s.flag = np.zeros(N, dtype=np.int8)
+        s.grain = np.zeros(N, dtype=np.int16)
+        if params.has_kappa:
+            s.kappa = np.ones(N)
+        return s
+
+    @property
+    def N(self) -> int:
+        p = self.params
+        return int(p.nx) * int(p.ny) * int(p.nz)
+
+    def _c(self) -> _State:
+        return _State(_p(self.f), _p(self.g), _p(self.h), _p(self.c), _p(self.phi),
+                      _p(self.phi_prev), _p(self.flag), _p(self.grain), _p(self.kappa))
+
+    def init_equilibrium(self, rho=None, vel=None, T=None, C=None, phi=None) -> None:
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                for v in (rho, vel, T, C, phi)]
+        st = self._c()
+        lib().orc_init_equilibrium(ct.byref(self.params), ct.byref(st), *[_p(a) for a in arrs])
+
+    def init_seeds(self, centers, radii) -> None:
+        c = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1)
+        r = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        st = self._c()
+        lib().orc_init_seeds(ct.byref(self.params), ct.byref(st), len(r), _p(c), _p(r))
+
+    def step(self, nsteps: int = 1, mask: int = 15, op: int = OP_BGK) -> None:
+        st = self._c()
+        rc = lib().orc_step(ct.byref(self.params), ct.byref(st), int(mask), int(op), int(nsteps))
+        if rc != 0:
+            raise RuntimeError("oracle rejected the parameter block")
+
+    def moments(self) -> dict:
+        N = self.N
+        p = self.params
+        rho = np.zeros(N)
+        vel = np.zeros(3 * N)
+        T = np.zeros(N)
+        Cc = np.zeros(N)
+        ph = np.zeros(N)
+        st = self._c()
+        lib().orc_moments(ct.byref(p), ct.byref(st), _p(rho), _p(vel), _p(T), _p(Cc), _p(ph))
+        out = {"vel": vel}
+        if p.flow_q:
+            out["rho"] = rho
+        if p.thermal:
+            out["T"] = T
+        if p.solute:
+            out["C"] = Cc
+        if p.phase_q:
+            out["phi_sum"] = ph
+        return out
+
+    def copy(self) -> "State":
+        s = State(params=self.params)
+        for k in ("f", "g", "h", "c", "phi", "phi_prev", "flag", "grain", "kappa"):
+            v = getattr(self, k)
+            setattr(s, k, None if v is None else v.copy())
+        return s
+
+
+def equilibrium(q: int, rho: float, u) -> np.ndarray:
+    out = np.zeros(q)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lib().orc_equilibrium(q, rho, _p(u), _p(out))
+    return out
+
+
+def collide_cell(params, op: int, f, F, ns: float = 0.0) -> np.ndarray:
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    out = np.zeros_like(f)
+    lib().orc_collide_cell(ct.byref(params), op, _p(f), _p(F), _p(out), ns)
+    return out
+
+
+def phase_source(params, phi27, upf: float, grain: int = 0):
+    phi27 = np.ascontiguousarray(phi27, dtype=np.float64).reshape(-1)
+    S = ct.c_double()
+    a = ct.c_double()
+    lib().orc_phase_source(ct.byref(params), _p(phi27), upf, grain, ct.byref(S), ct.byref(a))
+    return S.value, a.value
+
+
+def splitmix_stream(seed: int, n: int) -> np.ndarray:
+    s = ct.c_uint64(seed)
+    return np.array([lib().orc_uniform(ct.byref(s)) for _ in range(n)])
+
+
+def cell_uniform(seed: int, cells) -> np.ndarray:
+    return np.array([lib().orc_cell_uniform(seed, int(c)) for c in cells])

@@ -1,0 +1,169 @@
+"""Device (HIP, AA pattern, in-place) vs FP64 CPU oracle (two-lattice pull) on random states.
+
+FP64 storage must agree to 1e-12 (relative to the field's magnitude); FP32 deviation storage
+to 2e-6 after the same steps.  Flags must be bit-identical.  Both AA layouts are covered by
+running an odd and an even number of steps.
+"""
+import numpy as np
+import pytest
+
+from helpers import compare_pops, load_device, load_oracle, random_state
+from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P, BB, DI, OUT = mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_OUTFLOW
+
+
+def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
+                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0):
+    p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], flow_q=flow_q, phase_q=phase_q,
+                           thermal=thermal, solute=solute, precision=prec, ngrains=ngrains,
+                           has_kappa=kappa)
+    om = 1.0 / 0.8
+    p.s_nu = om
+    if op == "bgk":
+        p.s_bulk = p.s_odd = p.s_even = om
+    elif op == "trt":
+        lam = 3.0 / 16.0
+        p.s_bulk = p.s_even = om
+        p.s_odd = 1.0 / (lam / (1.0 / om - 0.5) + 0.5)
+    else:
+        p.s_bulk, p.s_odd, p.s_even = 1.19, 1.2, 1.4
+    p.force[0], p.force[1], p.force[2] = 1e-5, -2e-6, 3e-6
+    if thermal:
+        p.buoy[2] = 2e-4
+        p.T_ref = 0.1
+        p.ref_h = 0.1
+        p.alpha_l, p.alpha_s = 0.1, 0.05
+    if phase_q:
+        p.eps4, p.lambda_, p.pf_dt, p.pf_wdx2, p.phi_s = 0.05, 3.19, 0.008, 6.25, 0.99
+        p.pf_coupling = 1 if thermal else (2 if solute else 0)
+    if solute:
+        p.D_l, p.D_s, p.k_part, p.q_min = 0.1, 0.01, 0.15, 1e-3
+        p.ref_c = -0.5
+    if ngrains:
+        rng = np.random.default_rng(5)
+        for g in range(ngrains):
+            q = rng.standard_normal(4)
+            R = mkg.quat_to_rot(q / np.linalg.norm(q))
+            for k in range(9):
+                p.rot[g][k] = R.reshape(-1)[k]
+    if bcs == "mixed":
+        p.set(bc_f=[BB, OUT, P, P, BB, BB], bc_g=[BB, BB, P, P, BB, BB],
+              bc_h=[DI, OUT, P, P, DI, BB], bc_c=[DI, OUT, P, P, BB, BB],
+              T_face=[0.3, 0, 0, 0, -0.2, 0], C_face=[-0.4, 0, 0, 0, 0, 0])
+        p.u_face[0][0] = 0.02
+        p.u_face[5][0], p.u_face[5][1] = 0.01, 0.005
+    return p
+
+
+OPS = {"bgk": O.OP_BGK, "trt": O.OP_TRT, "mrt": O.OP_MRT}
+
+CASES = [
+    dict(id="bgk19", flow_q=19, op="bgk"),
+    dict(id="trt19", flow_q=19, op="trt"),
+    dict(id="mrt19", flow_q=19, op="mrt"),
+    dict(id="bgk27", flow_q=27, op="bgk"),
+    dict(id="trt27", flow_q=27, op="trt"),
+    dict(id="mrt27", flow_q=27, op="mrt"),
+    dict(id="flow_thermal", flow_q=19, thermal=1, op="trt"),
+    dict(id="flow27_thermal", flow_q=27, thermal=1, op="bgk"),
+    dict(id="pf19_thermal", phase_q=19, thermal=1),
+    dict(id="pf7_thermal", phase_q=7, thermal=1),
+    dict(id="fused", flow_q=19, phase_q=19, thermal=1, op="trt"),
+    dict(id="fused_pf7", flow_q=19, phase_q=7, thermal=1, op="mrt"),
+    dict(id="fused_q27", flow_q=27, phase_q=19, thermal=1, op="bgk"),
+    dict(id="solutal_grains", flow_q=19, phase_q=19, solute=1, op="mrt", ngrains=3),
+    dict(id="pf_solute", phase_q=19, solute=1),
+    dict(id="thermal_only", thermal=1),
+    dict(id="solute_only", solute=1),
+    dict(id="pf_only", phase_q=19),
+    dict(id="pf_thermal_kappa", phase_q=19, thermal=1, kappa=1),
+]
+
+
+def _run(case, bcs, nsteps, prec=mkg.PREC_FP64, walls=False, seed=1):
+    kw = {k: v for k, v in case.items() if k != "id"}
+    op = kw.pop("op", "bgk")
+    p = make_params(op=op, bcs=bcs, prec=prec, **kw)
+    st = random_state(p, seed=seed, with_walls=walls)
+    orc = load_oracle(p, st)
+    dev = load_device(p, st)
+    try:
+        print(f"[parity] {case['id']} bcs={bcs} steps={nsteps} prec={prec} walls={walls}",
+              flush=True)
+        dev.step(nsteps, mkg.ALL)
+        orc.step(nsteps, 15, OPS[op])
+        errs = compare_pops(dev, orc)
+        flags_dev = dev.get(mkg.F_FLAG)
+        return errs, flags_dev, orc.flag.copy(), dev, orc
+    finally:
+        pass
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+@pytest.mark.parametrize("nsteps", [5, 6])
+def test_fp64_matches_oracle(case, bcs, nsteps):
+    errs, fd, fo, dev, orc = _run(case, bcs, nsteps)
+    dev.close()
+    assert np.array_equal(fd, fo), "flags differ"
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c.get("flow_q")],
+                         ids=[c["id"] for c in CASES if c.get("flow_q")])
+def test_fp64_walls_match_oracle(case):
+    errs, fd, fo, dev, orc = _run(case, "mixed", 7, walls=True)
+    dev.close()
+    assert np.array_equal(fd, fo)
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+def test_fp32_deviation_storage_close_to_oracle(case):
+    errs, fd, fo, dev, orc = _run(case, "mixed", 6, prec=mkg.PREC_FP32)
+    dev.close()
+    for k, e in errs.items():
+        assert e < 2e-6, f"{k}: rel err {e:.3e}"
+
+
+def test_macroscopic_fields_match_oracle():
+    case = dict(id="fused", flow_q=19, phase_q=19, thermal=1, op="trt")
+    errs, fd, fo, dev, orc = _run(case, "mixed", 4)
+    m = orc.moments()
+    rho = dev.get(mkg.F_RHO)
+    vel = dev.get(mkg.F_VEL)
+    T = dev.get(mkg.F_TEMP)
+    assert np.abs(rho - m["rho"]).max() < 1e-13
+    assert np.abs(vel - m["vel"]).max() < 1e-13
+    assert np.abs(T - m["T"]).max() < 1e-13
+    # FP64 reductions against the oracle's sums
+    assert abs(dev.reduce(mkg.R_MASS)[0] - m["rho"].sum()) < 1e-10
+    assert abs(dev.reduce(mkg.R_ENERGY)[0] - m["T"].sum()) < 1e-10
+    mom = dev.reduce(mkg.R_MOMENTUM)
+    ref = (m["rho"][:, None] * m["vel"].reshape(-1, 3)).sum(0)
+    assert np.abs(mom - ref).max() < 1e-12
+    assert dev.reduce(mkg.R_NONFINITE)[0] == 0
+    dev.close()
+
+
+def test_sequential_masks_match_oracle():
+    """Operator-by-operator stepping (the reference's per-field call order) equals the oracle
+    driven with the same masks."""
+    p = make_params(flow_q=19, phase_q=19, thermal=1, op="trt", bcs="mixed")
+    st = random_state(p, seed=3)
+    orc = load_oracle(p, st)
+    dev = load_device(p, st)
+    for _ in range(3):
+        for m in (mkg.FLOW, mkg.PHASE, mkg.THERMAL):
+            dev.step(1, m)
+            orc.step(1, m, O.OP_TRT)
+    errs = compare_pops(dev, orc)
+    dev.close()
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
