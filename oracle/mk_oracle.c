@@ -336,6 +336,12 @@ static double face_flux(const orc_params* p, const double* R, const double* g, i
     return as * as * g[axis] + Nl;
 }
 
+/* Exposed for tests: normal flux component of a face with gradient g, grain-less when R is
+ * NULL. */
+double orc_face_flux(const orc_params* p, const double* R, const double* g, int axis) {
+    return face_flux(p, R, g, axis);
+}
+
 #define P27(dx, dy, dz) phi27[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
 
 /* Source S of the phase-field LB equation for one cell from its 3x3x3 phi neighbourhood.

@@ -84,6 +84,8 @@ void orc_collide_cell(const orc_params* p, int op, const double* f, const double
 void orc_phase_source(const orc_params* p, const double* phi27, double upf, int grain,
                       double* S_out, double* a_out);
 
+double orc_face_flux(const orc_params* p, const double* R, const double* g, int axis);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 

@@ -47,6 +47,8 @@ def lib() -> ct.CDLL:
         L.orc_equilibrium.argtypes = [ct.c_int, ct.c_double, vp, vp]
         L.orc_collide_cell.argtypes = [vp, ct.c_int, vp, vp, vp, ct.c_double]
         L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
+        L.orc_face_flux.argtypes = [vp, vp, vp, ct.c_int]
+        L.orc_face_flux.restype = ct.c_double
         L.orc_lattice.argtypes = [ct.c_int, vp, vp, vp, vp]
         L.orc_lattice.restype = ct.c_int
         L.orc_splitmix64.argtypes = [ct.POINTER(ct.c_uint64)]
@@ -192,6 +194,12 @@ def phase_source(params, phi27, upf: float, grain: int = 0):
     a = ct.c_double()
     lib().orc_phase_source(ct.byref(params), _p(phi27), upf, grain, ct.byref(S), ct.byref(a))
     return S.value, a.value
+
+
+def face_flux(params, g, axis: int, R=None) -> float:
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    Rm = None if R is None else np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+    return lib().orc_face_flux(ct.byref(params), _p(Rm), _p(g), int(axis))
 
 
 def splitmix_stream(seed: int, n: int) -> np.ndarray:
