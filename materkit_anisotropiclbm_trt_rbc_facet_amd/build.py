@@ -32,7 +32,7 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "api.cpp", "case.cpp"]
+SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "adjoint.hip", "api.cpp", "case.cpp"]
 
 
 def _hipcc() -> str:
@@ -54,6 +54,34 @@ def _compile(src: str) -> Path:
     return obj
 
 
+SHIM = HERE / "shim"
+SHIM_LIB = HERE / "libmkshim.so"
+MKRUN = HERE / "mkrun"
+
+
+def build_shim() -> None:
+    """The reference-API host layer (g++, host only) and the MaterKit-style driver."""
+    cxx = shutil.which("g++") or "g++"
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{SHIM}"]
+    tmp = SHIM_LIB.with_suffix(".so.tmp")
+    cmds = [
+        [cxx, *flags, "-shared", "-o", str(tmp), str(SHIM / "mkshim.cpp"), f"-L{HERE}", "-lmkg",
+         "-Wl,-rpath,$ORIGIN"],
+    ]
+    for cmd in cmds:
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"shim build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(tmp, SHIM_LIB)
+    tmp = MKRUN.with_suffix(".tmp")
+    cmd = [cxx, *flags, "-o", str(tmp), str(SHIM / "mkrun.cpp"), f"-L{HERE}", "-lmkshim", "-lmkg",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stderr}")
+    os.replace(tmp, MKRUN)
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every source for gfx950 and link ``libmkg.so``; returns its path."""
     BUILD.mkdir(exist_ok=True)
@@ -67,8 +95,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_shim()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB}, {SHIM_LIB}, {MKRUN}")
     return LIB
 
 

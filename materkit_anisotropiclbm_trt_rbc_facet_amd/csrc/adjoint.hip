@@ -1,0 +1,471 @@
+// Discrete adjoint of the fused phase-field (D3Q19) + thermal (D3Q7) step, the forward model
+// of the inverse problem (no flow lattice).  Hand-written reverse of k_phase_moments + k_step:
+//
+//   forward step n at cell x (canonical pull view):
+//     phi = sum_i g_i,  T = sum_k h_k,  S = S(phi over the 19-point stencil, T; eps4)
+//     g*_i = g_i - w_g (g_i - w_i phi) + w_i S
+//     w_h  = 1 / (1/2 + kappa alpha(phi) / cs^2)
+//     h*_k = h_k - w_h (h_k - w_k T) + w_k L (phi - prev)
+//     g'_i(x + c_i) = g*_i(x), h'_k(x + c_k) = h*_k(x)  (face rules), prev' = phi
+//
+// The reverse of one step is two launches:
+//   k_adj_collide  gathers the streamed adjoints, reverses both collisions and the source
+//                  (reverse mode through the face fluxes and the cell anisotropy, with the
+//                  exact Hessian of the Karma-Rappel energy), writes the adjoint populations,
+//                  the per-neighbour stencil contributions P_k, and accumulates dL/dkappa and
+//                  the FP64 block partials of dL/deps4;
+//   k_adj_stencil  gathers P_k back to the phi values they came from (transpose of the
+//                  stencil), giving the part of phi-adjoint that every population of the cell
+//                  receives; it is folded in when the next (earlier) step gathers.
+// Adjoint populations live in plain two-lattice canonical buffers (Q x N).
+#include "device.h"
+
+namespace mkg {
+
+// ---------------------------------------------------------------- source derivatives
+// Face flux and its derivatives in the crystal frame: with n = G/|G|, s4 = sum n^4,
+//   a = 1 - 3 e + 4 e s4,  da_k = |G| da/dG_k = 16 e (n_k^3 - s4 n_k),
+//   J_k = |G| (a^2 n_k + a da_k)                         (= dE/dG_k, E = |G|^2 a^2 / 2)
+//   H_kl = dJ_k/dG_l = a^2 d_kl + 2a (n_k da_l + n_l da_k) + da_k da_l + a dda_kl,
+//   dda_kl = |G|^2 d2a/dG_k dG_l = 16 e [3 n_k^2 d_kl - 4 n_k^3 n_l - 4 n_l^3 n_k - s4 d_kl
+//                                         + 6 s4 n_k n_l]
+//   dJ_k/de = |G| (2 a ae n_k + ae da_k + a da_k / e),  ae = -3 + 4 s4.
+// Returns J_axis (lab) and fills dJ/dg (lab, 3) and dJ/de.
+template <class R>
+__device__ inline R face_flux_d(const R (&g)[3], int axis, R eps4, const R (&M)[9], bool rot,
+                                R (&dJdg)[3], R& dJde) {
+    R G[3];
+    if (rot) {
+        for (int k = 0; k < 3; k++) G[k] = M[0 * 3 + k] * g[0] + M[1 * 3 + k] * g[1] + M[2 * 3 + k] * g[2];
+    } else {
+        G[0] = g[0];
+        G[1] = g[1];
+        G[2] = g[2];
+    }
+    const R r2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    if (!(r2 > (R)1e-30)) {
+        dJdg[0] = dJdg[1] = dJdg[2] = 0;
+        dJdg[axis] = 1;
+        dJde = 0;
+        return g[axis];
+    }
+    const R r = sqrt(r2);
+    R n[3] = {G[0] / r, G[1] / r, G[2] / r};
+    const R s4 = pow4(n[0]) + pow4(n[1]) + pow4(n[2]);
+    const R a = (R)1 - (R)3 * eps4 + (R)4 * eps4 * s4;
+    const R ae = (R)-3 + (R)4 * s4;
+    R da1[3], da[3], J[3], Je[3];
+    for (int k = 0; k < 3; k++) {
+        da1[k] = (R)16 * (n[k] * n[k] * n[k] - s4 * n[k]);
+        da[k] = eps4 * da1[k];
+        J[k] = r * (a * a * n[k] + a * da[k]);
+        Je[k] = r * ((R)2 * a * ae * n[k] + ae * da[k] + a * da1[k]);
+    }
+    // v = row `axis` of M (lab axis in crystal coordinates)
+    R v[3];
+    for (int k = 0; k < 3; k++) v[k] = rot ? M[axis * 3 + k] : (R)(k == axis);
+    R w[3];
+    for (int k = 0; k < 3; k++) {
+        R s = 0;
+        for (int l = 0; l < 3; l++) {
+            const R dkl = (R)(k == l);
+            const R dda = (R)16 * eps4 *
+                          ((R)3 * n[k] * n[k] * dkl - (R)4 * n[k] * n[k] * n[k] * n[l] -
+                           (R)4 * n[l] * n[l] * n[l] * n[k] - s4 * dkl + (R)6 * s4 * n[k] * n[l]);
+            const R H = a * a * dkl + (R)2 * a * (n[k] * da[l] + n[l] * da[k]) + da[k] * da[l] + a * dda;
+            s += H * v[l];
+        }
+        w[k] = s;
+    }
+    R Jl = 0, Jel = 0;
+    for (int k = 0; k < 3; k++) {
+        Jl += v[k] * J[k];
+        Jel += v[k] * Je[k];
+    }
+    if (rot) {
+        for (int b = 0; b < 3; b++) dJdg[b] = M[b * 3 + 0] * w[0] + M[b * 3 + 1] * w[1] + M[b * 3 + 2] * w[2];
+    } else {
+        dJdg[0] = w[0];
+        dJdg[1] = w[1];
+        dJdg[2] = w[2];
+    }
+    dJde = Jel;
+    return Jl;
+}
+
+// Cell anisotropy a(grad) with da/dgrad (lab) and da/de.
+template <class R>
+__device__ inline R cell_as_d(const R (&gr)[3], R eps4, const R (&M)[9], bool rot, R (&dadg)[3],
+                              R& dade) {
+    R G[3];
+    if (rot) {
+        for (int k = 0; k < 3; k++) G[k] = M[0 * 3 + k] * gr[0] + M[1 * 3 + k] * gr[1] + M[2 * 3 + k] * gr[2];
+    } else {
+        G[0] = gr[0];
+        G[1] = gr[1];
+        G[2] = gr[2];
+    }
+    const R gg = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    if (!(gg > (R)1e-30)) {
+        dadg[0] = dadg[1] = dadg[2] = 0;
+        dade = 0;
+        return 1;
+    }
+    const R r = sqrt(gg);
+    R n[3] = {G[0] / r, G[1] / r, G[2] / r};
+    const R s4 = pow4(n[0]) + pow4(n[1]) + pow4(n[2]);
+    R dc[3];
+    for (int k = 0; k < 3; k++) dc[k] = (R)16 * eps4 * (n[k] * n[k] * n[k] - s4 * n[k]) / r;
+    if (rot) {
+        for (int b = 0; b < 3; b++) dadg[b] = M[b * 3 + 0] * dc[0] + M[b * 3 + 1] * dc[1] + M[b * 3 + 2] * dc[2];
+    } else {
+        dadg[0] = dc[0];
+        dadg[1] = dc[1];
+        dadg[2] = dc[2];
+    }
+    dade = (R)-3 + (R)4 * s4;
+    return (R)1 - (R)3 * eps4 + (R)4 * eps4 * s4;
+}
+
+#define PIX(dx, dy, dz) (((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1))
+
+// Reverse of phase_source: given Sh = dL/dS, accumulates dL/dP (27, corners unused) and
+// returns dL/du; adds dL/deps4 to *eh.
+template <class R>
+__device__ inline R phase_source_adj(const R (&P)[27], R upf, R eps4, R lambda, R pf_dt,
+                                     R pf_wdx2, const R (&M)[9], bool rot, R Sh, R (&Ph)[27],
+                                     R& eh) {
+    // ---- forward values needed by the reverse
+    R gf[6][3], dJ[6][3], dJe[6];
+    R divJ = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int f = 2 * ax + sd;
+            const int side = sd == 0 ? 1 : -1;
+            const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b == ax) {
+                    gf[f][b] = (R)side * (P[PIX(o0, o1, o2)] - P[PIX(0, 0, 0)]);
+                } else {
+                    const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                    gf[f][b] = (R)0.25 * (P[PIX(e0, e1, e2)] - P[PIX(-e0, -e1, -e2)] +
+                                          P[PIX(o0 + e0, o1 + e1, o2 + e2)] -
+                                          P[PIX(o0 - e0, o1 - e1, o2 - e2)]);
+                }
+            }
+            const R J = face_flux_d<R>(gf[f], ax, eps4, M, rot, dJ[f], dJe[f]);
+            divJ += (R)side * J;
+        }
+    }
+    R gr[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)(3.0 * wgt<19>(i)) * P[PIX(cx(i), cy(i), cz(i))];
+        if (cx(i)) gr[0] += (R)cx(i) * v;
+        if (cy(i)) gr[1] += (R)cy(i) * v;
+        if (cz(i)) gr[2] += (R)cz(i) * v;
+    }
+    R dadg[3], dade;
+    const R a = cell_as_d<R>(gr, eps4, M, rot, dadg, dade);
+    const R ph = P[PIX(0, 0, 0)];
+    const R d = (R)1 - ph * ph;
+    const R Qs = (ph - lambda * upf * d) * d;
+    const R a2 = a * a;
+    // ---- reverse
+    const R Jh = Sh * pf_dt * pf_wdx2 / a2;          // dL/d(divJ)
+    const R lh = -Sh * pf_dt * pf_wdx2;              // dL/d(lap)
+    const R Qh = Sh * pf_dt / a2;                    // dL/dQ
+    const R ah = Sh * (R)-2 * pf_dt * (pf_wdx2 * divJ + Qs) / (a2 * a);
+    R e_acc = ah * dade;
+#pragma unroll
+    for (int k = 0; k < 27; k++) Ph[k] = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int f = 2 * ax + sd;
+            const int side = sd == 0 ? 1 : -1;
+            const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
+            const R Jf = (R)side * Jh;
+            e_acc += Jf * dJe[f];
+            R gh[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) gh[b] = Jf * dJ[f][b];
+            gh[ax] += (R)side * lh;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b == ax) {
+                    Ph[PIX(o0, o1, o2)] += (R)side * gh[b];
+                    Ph[PIX(0, 0, 0)] -= (R)side * gh[b];
+                } else {
+                    const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                    const R q = (R)0.25 * gh[b];
+                    Ph[PIX(e0, e1, e2)] += q;
+                    Ph[PIX(-e0, -e1, -e2)] -= q;
+                    Ph[PIX(o0 + e0, o1 + e1, o2 + e2)] += q;
+                    Ph[PIX(o0 - e0, o1 - e1, o2 - e2)] -= q;
+                }
+            }
+        }
+    }
+    const R grh0 = ah * dadg[0], grh1 = ah * dadg[1], grh2 = ah * dadg[2];
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R c = (R)(3.0 * wgt<19>(i));
+        Ph[PIX(cx(i), cy(i), cz(i))] += c * ((R)cx(i) * grh0 + (R)cy(i) * grh1 + (R)cz(i) * grh2);
+    }
+    // Q = (ph - lambda u d) d, d = 1 - ph^2
+    const R dQdph = d - (R)2 * ph * (ph - lambda * upf * d) + (R)2 * lambda * upf * ph * d;
+    Ph[PIX(0, 0, 0)] += Qh * dQdph;
+    eh += e_acc;
+    return Qh * (-lambda * d * d);
+}
+
+// ---------------------------------------------------------------- adjoint streaming
+// dL/d(post_i(y)) from the adjoint of the next level's pre-collision populations (canonical
+// layout, adj_j(z) = A[j*N + z] + cellwide[z]); reverses the pull rules of the forward step.
+template <int Q, class R>
+__device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __restrict__ cw,
+                                  const LatArgs& L, const Cell& c, const StepArgs& a, int kind,
+                                  uint32_t N) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int dx = cx(i), dy = cy(i), dz = cz(i);
+        R v = 0;
+        // (a)/(b): where post_i(y) is pulled to
+        uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+        if (hit == 0u) {
+            const uint32_t t = nbr(c, dx, dy, dz);
+            v = A[(size_t)i * N + t] + (cw ? cw[t] : (R)0);
+        } else {
+            const int face = __builtin_ctz(hit);
+            const int tb = pick6(face, L.bc);
+            const R own = A[(size_t)inv(i) * N + c.id] + (cw ? cw[c.id] : (R)0);
+            if (tb == kBcOutflow) v = 0;
+            else if (tb == kBcDirichlet && kind == kKindScalar) v = -own;
+            else v = own;
+        }
+        // (c): zero-gradient outflow copies post_i(y) into new_i(y)
+        uint32_t hin = c.bits & faces_of(-dx, -dy, -dz) & (uint32_t)L.npmask;
+        if (hin != 0u) {
+            const int face = __builtin_ctz(hin);
+            if (pick6(face, L.bc) == kBcOutflow) v += A[(size_t)i * N + c.id] + (cw ? cw[c.id] : (R)0);
+        }
+        out[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_collide(const StepArgs a, const AdjArgs j) {
+    Cell c;
+    const bool valid = locate(a, c);
+    const uint32_t N = a.N;
+    const uint32_t id = c.id;
+    __shared__ double red[4];
+    double eps_loc = 0;
+    if (valid && a.flag[id] != kIsWalls) {
+        R gs[19], hs[7];
+        adj_gather<19, R>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
+        adj_gather<7, R>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+        // forward values at level n
+        R h[7];
+        read_pops_rt<7, R>(h, j.hsave, a.S, c, a, kKindScalar);
+        R sh = 0;
+#pragma unroll
+        for (int k = 0; k < 7; k++) sh += h[k];
+        const R T = (R)a.h.ref + sh;
+        const R* __restrict__ ph = (const R*)j.phin;
+        uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+        uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+        uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+        const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+        if (np & XLO) px[0] = px[1];
+        if (np & XHI) px[2] = px[1];
+        if (np & YLO) py[0] = py[1];
+        if (np & YHI) py[2] = py[1];
+        if (np & ZLO) pz[0] = pz[1];
+        if (np & ZHI) pz[2] = pz[1];
+        R P[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) P[k] = 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++)
+            P[PIX(cx(i), cy(i), cz(i))] = ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+        const R phi = P[PIX(0, 0, 0)];
+        R Mrot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        bool rot = false;
+        if (a.ngrains > 0) {
+            const int gid = a.grain[id];
+            if (gid > 0) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) Mrot[k] = (R)a.rot[(gid - 1) * 9 + k];
+                rot = true;
+            }
+        }
+        // thermal rate and its derivatives
+        const R fsr = (R)0.5 * ((R)1 + phi);
+        const R fs = fsr < (R)0 ? (R)0 : (fsr > (R)1 ? (R)1 : fsr);
+        const R dalpha = (fsr > (R)0 && fsr < (R)1) ? (R)0.5 * ((R)a.alpha_s - (R)a.alpha_l) : (R)0;
+        const R alpha0 = (R)a.alpha_l * ((R)1 - fs) + (R)a.alpha_s * fs;
+        const R kap = a.has_kappa ? ((const R*)a.kappa)[id] : (R)1;
+        const R om = (R)1 / ((R)0.5 + kap * alpha0 / (R)0.25);
+        const R dom_dphi = -om * om * kap * dalpha / (R)0.25;
+        const R dom_dkap = -om * om * alpha0 / (R)0.25;
+        // adjoint sums
+        R A = 0, B = 0, Om = 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++) A += (R)wgt<19>(i) * gs[i];
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            B += (R)wgt<7>(k) * hs[k];
+            Om -= (h[k] - (R)wgt<7>(k) * sh) * hs[k];
+        }
+        R Ph[27];
+        R eh = 0;
+        const R upf = a.pf_coupling == 1 ? T : (R)0;
+        const R uh = phase_source_adj<R>(P, upf, (R)a.eps4, (R)a.lambda, (R)a.pf_dt,
+                                         (R)a.pf_wdx2, Mrot, rot, A, Ph, eh);
+        const R Th = om * B + (a.pf_coupling == 1 ? uh : (R)0);
+        R* __restrict__ ho = (R*)j.hout;
+#pragma unroll
+        for (int k = 0; k < 7; k++) ho[(size_t)k * N + id] = ((R)1 - om) * hs[k] + Th;
+        const R L = (R)a.latent_T;
+        const R pin = ((const R*)j.pin)[id];
+        const R phl = (R)a.omega_g * A + L * B + pin + Om * dom_dphi;
+        R* __restrict__ go = (R*)j.gout;
+        const R og = (R)1 - (R)a.omega_g;
+#pragma unroll
+        for (int i = 0; i < 19; i++) go[(size_t)i * N + id] = og * gs[i] + phl;
+        ((R*)j.pout)[id] = -L * B;
+        if (a.has_kappa) ((R*)j.kap_grad)[id] += Om * dom_dkap;
+        R* __restrict__ Pb = (R*)j.P;
+#pragma unroll
+        for (int i = 0; i < 19; i++) Pb[(size_t)i * N + id] = Ph[PIX(cx(i), cy(i), cz(i))];
+        eps_loc = (double)eh;
+    }
+    // FP64 block sum of dL/deps4 (wavefront butterfly + LDS), accumulated per block
+    double v = eps_loc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t b = blockIdx.x;
+        j.eps_part[b] += red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// Transpose of the clamped / periodic 19-point stencil: phi(x) was read by cell y through
+// direction k when wrapclamp(y + c_k) == x.
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_stencil(const StepArgs a, const R* __restrict__ P,
+                                                     R* __restrict__ nout) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const uint32_t N = a.N;
+    const uint32_t np = (uint32_t)a.g.npmask;
+    R s = 0;
+#pragma unroll
+    for (int k = 0; k < 19; k++) {
+        // per axis: the regular source (x - d, invalid across a non-periodic face) and, on a
+        // non-periodic face, the cell itself read through the clamp
+        uint32_t ox[2], oy[2], oz[2];
+        bool vx[2], vy[2], vz[2];
+        auto axis = [&](int d, const uint32_t* off, uint32_t lo, uint32_t hi, uint32_t* o,
+                        bool* v) {
+            if (d == 0) {
+                o[0] = off[1];
+                v[0] = true;
+                o[1] = off[1];
+                v[1] = false;
+            } else {
+                const uint32_t back = d > 0 ? lo : hi;
+                const uint32_t fwd = d > 0 ? hi : lo;
+                o[0] = off[1 - d];
+                v[0] = !((c.bits & back) && (np & back));
+                o[1] = off[1];
+                v[1] = (c.bits & fwd) && (np & fwd);
+            }
+        };
+        axis(cx(k), c.ox, XLO, XHI, ox, vx);
+        axis(cy(k), c.oy, YLO, YHI, oy, vy);
+        axis(cz(k), c.oz, ZLO, ZHI, oz, vz);
+#pragma unroll
+        for (int i0 = 0; i0 < 2; i0++)
+#pragma unroll
+            for (int i1 = 0; i1 < 2; i1++)
+#pragma unroll
+                for (int i2 = 0; i2 < 2; i2++)
+                    if (vx[i0] && vy[i1] && vz[i2])
+                        s += P[(size_t)k * N + ox[i0] + oy[i1] + oz[i2]];
+    }
+    nout[c.id] = s;
+}
+
+// Loss L = 1/2 sum (phi - target)^2 (FP64 block partials) and the seed of the reverse sweep:
+// the cell-wide phase adjoint phi - target.
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_loss(const StepArgs a, const double* target,
+                                                  R* __restrict__ nseed, double* part) {
+    Cell c;
+    __shared__ double red[4];
+    double v = 0;
+    if (locate(a, c) && a.flag[c.id] != kIsWalls) {
+        R g[19];
+        read_pops_rt<19, R>(g, a.g, a.S, c, a, kKindScalar);
+        R s = 0;
+#pragma unroll
+        for (int i = 0; i < 19; i++) s += g[i];
+        const double phi = a.g.ref + (double)s;
+        const double d = phi - target[c.id];
+        nseed[c.id] = (R)d;
+        v = 0.5 * d * d;
+    } else if (locate(a, c)) {
+        nseed[c.id] = 0;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_sum_partials(const double* part, uint32_t n, double* out) {
+    __shared__ double red[4];
+    double v = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) v += part[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---------------------------------------------------------------- launchers
+int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
+    if (fp32) k_adj_collide<float><<<a.nblk, 256, 0, s>>>(a, j);
+    else k_adj_collide<double><<<a.nblk, 256, 0, s>>>(a, j);
+    return 0;
+}
+int launch_adj_stencil(int fp32, const StepArgs& a, const void* P, void* nout, hipStream_t s) {
+    if (fp32) k_adj_stencil<float><<<a.nblk, 256, 0, s>>>(a, (const float*)P, (float*)nout);
+    else k_adj_stencil<double><<<a.nblk, 256, 0, s>>>(a, (const double*)P, (double*)nout);
+    return 0;
+}
+int launch_adj_loss(int fp32, const StepArgs& a, const double* target, void* nseed, double* part,
+                    double* out, hipStream_t s) {
+    if (fp32) k_adj_loss<float><<<a.nblk, 256, 0, s>>>(a, target, (float*)nseed, part);
+    else k_adj_loss<double><<<a.nblk, 256, 0, s>>>(a, target, (double*)nseed, part);
+    k_sum_partials<<<1, 256, 0, s>>>(part, a.nblk, out);
+    return 0;
+}
+int launch_sum_partials(const double* part, uint32_t n, double* out, hipStream_t s) {
+    k_sum_partials<<<1, 256, 0, s>>>(part, n, out);
+    return 0;
+}
+
+}  // namespace mkg

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -245,6 +246,53 @@ int copy_in(double* ddst, const void* host, int dtype, size_t count, hipStream_t
     return fail("dtype must be MKG_DT_F64 or MKG_DT_F32 for this field");
 }
 
+// One time step advancing the lattices flagged in adv.  ev (optional) receives three events:
+// before the phase moment pass, between the passes, after the fused pass.  mid (optional)
+// runs between the two passes (used by the adjoint to save phi^n and the thermal state).
+template <class Mid>
+int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid) {
+    Combo k{h->q[LF], h->q[LG], h->q[LH] ? 1 : 0, h->q[LC] ? 1 : 0};
+    int lay0 = -1;
+    bool mixed = false;
+    for (int L = 0; L < 4; L++)
+        if (adv[L]) {
+            if (lay0 < 0) lay0 = h->lay[L];
+            else if (h->lay[L] != lay0) mixed = true;
+        }
+    if (mixed) {
+        for (int L = 0; L < 4; L++)
+            if (adv[L] && normalize(h, L)) return -1;
+        lay0 = 0;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
+    if (adv[LG]) {
+        h->pf_count++;
+        StepArgs a = make_args(h);
+        if (launch_phase_moments(h->q[LG], h->fp32, lay0, a, h->phi[h->pf_count & 1], h->stream))
+            return fail("phase moment kernel unavailable");
+    }
+    if (mid(h)) return -1;
+    if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
+    StepArgs a = make_args(h);
+    a.f.adv = adv[LF];
+    a.g.adv = adv[LG];
+    a.h.adv = adv[LH];
+    a.c.adv = adv[LC];
+    if (launch_step(k, h->fp32, lay0, a, h->stream))
+        return fail("no kernel instantiation for this lattice combination");
+    if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
+    for (int L = 0; L < 4; L++)
+        if (adv[L]) {
+            h->lay[L] ^= 1;
+            h->steps[L]++;
+        }
+    return 0;
+}
+
+int step_once(mkg_ctx* h, const int* adv, hipEvent_t* ev, void*) {
+    return step_once_impl(h, adv, ev, [](mkg_ctx*) { return 0; });
+}
+
 }  // namespace
 
 extern "C" {
@@ -417,8 +465,6 @@ int mkg_step(mkg_handle h, int nsteps, int mask) {
     int adv[4] = {(mask & MKG_FLOW) && h->q[LF], (mask & MKG_PHASE) && h->q[LG],
                   (mask & MKG_THERMAL) && h->q[LH], (mask & MKG_SOLUTE) && h->q[LC]};
     if (!(adv[0] || adv[1] || adv[2] || adv[3])) return 0;
-    Combo k{h->q[LF], h->q[LG], h->q[LH] ? 1 : 0, h->q[LC] ? 1 : 0};
-    double tk[2] = {0, 0};
     size_t need_ev = h->timing ? (size_t)3 * nsteps : 0;
     while (h->events.size() < need_ev) {
         hipEvent_t e;
@@ -426,44 +472,12 @@ int mkg_step(mkg_handle h, int nsteps, int mask) {
         h->events.push_back(e);
     }
     for (int n = 0; n < nsteps; n++) {
-        // advanced lattices must share one layout
-        int lay0 = -1;
-        bool mixed = false;
-        for (int L = 0; L < 4; L++)
-            if (adv[L]) {
-                if (lay0 < 0) lay0 = h->lay[L];
-                else if (h->lay[L] != lay0) mixed = true;
-            }
-        if (mixed) {
-            for (int L = 0; L < 4; L++)
-                if (adv[L] && normalize(h, L)) return -1;
-            lay0 = 0;
-        }
-        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n], h->stream));
-        if (adv[LG]) {
-            h->pf_count++;
-            StepArgs a = make_args(h);
-            if (launch_phase_moments(h->q[LG], h->fp32, lay0, a, h->phi[h->pf_count & 1],
-                                     h->stream))
-                return fail("phase moment kernel unavailable");
-        }
-        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n + 1], h->stream));
-        StepArgs a = make_args(h);
-        a.f.adv = adv[LF];
-        a.g.adv = adv[LG];
-        a.h.adv = adv[LH];
-        a.c.adv = adv[LC];
-        if (launch_step(k, h->fp32, lay0, a, h->stream))
-            return fail("no kernel instantiation for this lattice combination");
-        if (h->timing) HIPCHK(hipEventRecord(h->events[3 * n + 2], h->stream));
-        for (int L = 0; L < 4; L++)
-            if (adv[L]) {
-                h->lay[L] ^= 1;
-                h->steps[L]++;
-            }
+        hipEvent_t* ev = h->timing ? &h->events[3 * n] : nullptr;
+        if (step_once(h, adv, ev, nullptr)) return -1;
     }
     HIPCHK(hipGetLastError());
     if (h->timing && nsteps > 0) {
+        double tk[2] = {0, 0};
         HIPCHK(hipStreamSynchronize(h->stream));
         for (int n = 0; n < nsteps; n++) {
             float a = 0, b = 0;
@@ -752,6 +766,196 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
     std::memcpy(h->steps, steps, sizeof(steps));
     h->pf_count = pfc;
     HIPCHK(hipMemcpy(h->rot, p.rot, sizeof(p.rot), hipMemcpyHostToDevice));
+    return 0;
+}
+
+
+// ---------------------------------------------------------------- discrete adjoint
+int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, double* loss,
+                    double* dL_deps4, double* dL_dkappa, double* times) {
+    if (!h || !target || !loss || !dL_deps4) return fail("null argument");
+    if (h->q[LF] || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
+        return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model only");
+    if (h->has_walls) return fail("the adjoint does not support wall cells");
+    if (nsteps < 1 || ckpt < 1) return fail("nsteps and ckpt must be >= 1");
+    if (dL_dkappa && !h->kappa) return fail("dL/dkappa needs has_kappa = 1");
+    const size_t N = h->N, S = h->S, e = h->esz;
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    const int adv[4] = {0, 1, 1, 0};
+    // ---- device storage
+    struct Ck {
+        int lay_g, lay_h;
+        int64_t pf;
+    };
+    std::vector<Ck> ck(nseg + 1);
+    const size_t ck_elems = (19 + 7 + 2) * S;
+    const size_t seg_elems = (size_t)ckpt * 8 * S;
+    char *ckbuf = nullptr, *segbuf = nullptr, *work = nullptr;
+    double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr;
+    int rc = 0;
+    auto release = [&]() {
+        hipStreamSynchronize(h->stream);
+        hipFree(ckbuf);
+        hipFree(segbuf);
+        hipFree(work);
+        hipFree(dtarget);
+        hipFree(dpart);
+        hipFree(dscal);
+    };
+#define ADJCHK(expr)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            release();                                                           \
+            return fail(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+        }                                                                        \
+    } while (0)
+    ADJCHK(hipMalloc((void**)&ckbuf, (nseg + 1) * ck_elems * e));
+    ADJCHK(hipMalloc((void**)&segbuf, seg_elems * e));
+    // work: Gin, Gout (19N), nin, nout (N), Hin, Hout (7N), pin, pout (N), P (19N), kap (N)
+    const size_t work_elems = (2 * 19 + 2 + 2 * 7 + 2 + 19 + 1) * N;
+    ADJCHK(hipMalloc((void**)&work, work_elems * e));
+    ADJCHK(hipMemsetAsync(work, 0, work_elems * e, h->stream));
+    ADJCHK(hipMalloc((void**)&dtarget, N * sizeof(double)));
+    ADJCHK(hipMemcpyAsync(dtarget, target, N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ADJCHK(hipMalloc((void**)&dpart, (size_t)h->nblk * 2 * sizeof(double)));
+    ADJCHK(hipMemsetAsync(dpart, 0, (size_t)h->nblk * 2 * sizeof(double), h->stream));
+    ADJCHK(hipMalloc((void**)&dscal, 4 * sizeof(double)));
+    char* Gin = work;
+    char* Gout = Gin + 19 * N * e;
+    char* nin = Gout + 19 * N * e;
+    char* nout = nin + N * e;
+    char* Hin = nout + N * e;
+    char* Hout = Hin + 7 * N * e;
+    char* pin = Hout + 7 * N * e;
+    char* pout = pin + N * e;
+    char* Pb = pout + N * e;
+    char* kap = Pb + 19 * N * e;
+    double* eps_part = dpart;
+    double* loss_part = dpart + h->nblk;
+
+    auto save_ck = [&](int k) -> int {
+        char* base = ckbuf + (size_t)k * ck_elems * e;
+        HIPCHK(hipMemcpyAsync(base, h->pop[LG], 19 * S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + 19 * S * e, h->pop[LH], 7 * S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + 26 * S * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + 27 * S * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
+        ck[k] = {h->lay[LG], h->lay[LH], h->pf_count};
+        return 0;
+    };
+    auto load_ck = [&](int k) -> int {
+        char* base = ckbuf + (size_t)k * ck_elems * e;
+        HIPCHK(hipMemcpyAsync(h->pop[LG], base, 19 * S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->pop[LH], base + 19 * S * e, 7 * S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[0], base + 26 * S * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[1], base + 27 * S * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        h->lay[LG] = ck[k].lay_g;
+        h->lay[LH] = ck[k].lay_h;
+        h->pf_count = ck[k].pf;
+        return 0;
+    };
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; k++) ADJCHK(hipEventCreate(&ev[k]));
+    auto cleanup_ev = [&]() {
+        for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
+    };
+    ADJCHK(hipEventRecord(ev[0], h->stream));
+    // ---- forward pass with checkpoints
+    for (int n = 0; n < nsteps; n++) {
+        if (n % ckpt == 0 && save_ck(n / ckpt)) { rc = -1; break; }
+        if (step_once(h, adv, nullptr, nullptr)) { rc = -1; break; }
+    }
+    if (rc == 0 && save_ck(nseg)) rc = -1;  // final forward state
+    if (rc) { cleanup_ev(); release(); return -1; }
+    // ---- loss and seed of the reverse sweep (phi_N - target, cell-wide)
+    {
+        StepArgs a = make_args(h);
+        launch_adj_loss(h->fp32, a, dtarget, nin, loss_part, dscal, h->stream);
+        ADJCHK(hipGetLastError());
+    }
+    ADJCHK(hipEventRecord(ev[1], h->stream));
+    std::vector<int> seg_lay(ckpt);
+    float t_re = 0, t_rev = 0;
+    for (int sgi = nseg - 1; sgi >= 0 && rc == 0; sgi--) {
+        const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+        hipEvent_t ea, eb, ec;
+        hipEventCreate(&ea);
+        hipEventCreate(&eb);
+        hipEventCreate(&ec);
+        hipEventRecord(ea, h->stream);
+        if (load_ck(sgi)) { rc = -1; break; }
+        for (int n = n0; n < n1; n++) {
+            char* slot = segbuf + (size_t)(n - n0) * 8 * S * e;
+            seg_lay[n - n0] = h->lay[LH];
+            auto mid = [&](mkg_ctx* hh) -> int {
+                HIPCHK(hipMemcpyAsync(slot, hh->phi[hh->pf_count & 1], S * e,
+                                      hipMemcpyDeviceToDevice, hh->stream));
+                HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], 7 * S * e,
+                                      hipMemcpyDeviceToDevice, hh->stream));
+                return 0;
+            };
+            if (step_once_impl(h, adv, nullptr, mid)) { rc = -1; break; }
+        }
+        hipEventRecord(eb, h->stream);
+        for (int n = n1 - 1; n >= n0 && rc == 0; n--) {
+            char* slot = segbuf + (size_t)(n - n0) * 8 * S * e;
+            StepArgs a = make_args(h);
+            AdjArgs j;
+            std::memset(&j, 0, sizeof(j));
+            j.gin = Gin; j.nin = nin; j.hin = Hin; j.pin = pin;
+            j.gout = Gout; j.hout = Hout; j.pout = pout;
+            j.P = Pb; j.kap_grad = kap; j.phin = slot;
+            j.hsave = a.h;
+            j.hsave.pop = slot + S * e;
+            j.hsave.lay = seg_lay[n - n0];
+            j.eps_part = eps_part;
+            launch_adj_collide(h->fp32, a, j, h->stream);
+            launch_adj_stencil(h->fp32, a, Pb, nout, h->stream);
+            std::swap(Gin, Gout);
+            std::swap(nin, nout);
+            std::swap(Hin, Hout);
+            std::swap(pin, pout);
+        }
+        hipEventRecord(ec, h->stream);
+        hipEventSynchronize(ec);
+        float x = 0, y = 0;
+        hipEventElapsedTime(&x, ea, eb);
+        hipEventElapsedTime(&y, eb, ec);
+        t_re += x;
+        t_rev += y;
+        hipEventDestroy(ea);
+        hipEventDestroy(eb);
+        hipEventDestroy(ec);
+        if (hipGetLastError() != hipSuccess) rc = fail("adjoint kernel launch failed");
+    }
+    if (rc) { cleanup_ev(); release(); return -1; }
+    // ---- outputs
+    launch_sum_partials(eps_part, h->nblk, dscal + 1, h->stream);
+    ADJCHK(hipGetLastError());
+    double sc[2];
+    ADJCHK(hipMemcpyAsync(sc, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (dL_dkappa) {
+        if (ensure_scratch(h)) { cleanup_ev(); release(); return -1; }
+        launch_to_double(h->fp32, kap, h->scratch, h->N, h->stream);
+        ADJCHK(hipMemcpyAsync(dL_dkappa, h->scratch, N * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
+    }
+    // leave the handle at the final forward state
+    if (load_ck(nseg)) { cleanup_ev(); release(); return -1; }
+    ADJCHK(hipStreamSynchronize(h->stream));
+    float t_fwd = 0;
+    hipEventElapsedTime(&t_fwd, ev[0], ev[1]);
+    *loss = sc[0];
+    *dL_deps4 = sc[1];
+    if (times) {
+        times[0] = t_fwd;
+        times[1] = t_re;
+        times[2] = t_rev;
+        times[3] = t_fwd + t_re + t_rev;
+    }
+    cleanup_ev();
+    release();
+#undef ADJCHK
     return 0;
 }
 

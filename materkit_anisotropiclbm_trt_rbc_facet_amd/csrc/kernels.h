@@ -88,4 +88,26 @@ int launch_reduce(int which, const StepArgs& a, const double* rho, const double*
 int launch_nonfinite(int fp32, const void* p, size_t n, double* partials, double* out,
                      hipStream_t s);
 
+// Arguments of the reverse (adjoint) step of the phase-field + thermal model.
+struct AdjArgs {
+    const void* gin;     // Q x N adjoint of the phase populations at level n+1 (partial)
+    const void* nin;     // N: cell-wide phase adjoint still to be added to every population
+    const void* hin;     // 7 x N adjoint of the thermal populations at level n+1
+    const void* pin;     // N: adjoint of prev at level n+1
+    void* gout;
+    void* hout;
+    void* pout;
+    void* P;             // 19 x N stencil contributions
+    void* kap_grad;      // N accumulated dL/dkappa (storage type)
+    const void* phin;    // N: phi^n of the forward step being reversed
+    LatArgs hsave;       // forward thermal populations at level n (raw AA buffer + layout)
+    double* eps_part;    // per-block FP64 partials of dL/deps4 (accumulated)
+};
+
+int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
+int launch_adj_stencil(int fp32, const StepArgs& a, const void* P, void* nout, hipStream_t s);
+int launch_adj_loss(int fp32, const StepArgs& a, const double* target, void* nseed, double* part,
+                    double* out, hipStream_t s);
+int launch_sum_partials(const double* part, uint32_t n, double* out, hipStream_t s);
+
 }  // namespace mkg

@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
         "mkg_checkpoint_load": (C.c_int, [vp, C.c_char_p]),
         "mkg_last_timing": (C.c_int, [vp, vp]),
         "mkg_enable_timing": (C.c_int, [vp, C.c_int]),
+        "mkg_adjoint_run": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -290,3 +291,25 @@ class Solver:
         _check(lib().mkg_last_timing(self._h, _ptr(out)))
         return {"phase_moments_ms": out[0], "fused_ms": out[1], "total_ms": out[2],
                 "steps": int(out[3])}
+
+    def adjoint_run(self, nsteps: int, ckpt: int, target, want_kappa: bool = True) -> dict:
+        """Forward nsteps with checkpoints, FP64 loss 1/2 sum (phi_N - target)^2 and its
+        gradient w.r.t. eps4 and (when the case has a conductivity field) kappa."""
+        t = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        if t.size != self.N:
+            raise MkgError("target must have N values")
+        loss = C.c_double()
+        deps = C.c_double()
+        dk = np.zeros(self.N) if want_kappa else None
+        times = np.zeros(4)
+        _check(lib().mkg_adjoint_run(self._h, int(nsteps), int(ckpt), _ptr(t), C.byref(loss),
+                                     C.byref(deps), _ptr(dk), _ptr(times)))
+        return {"loss": loss.value, "dL_deps4": deps.value, "dL_dkappa": dk,
+                "forward_ms": times[0], "recompute_ms": times[1], "reverse_ms": times[2],
+                "total_ms": times[3]}
+
+    def phi_now(self) -> np.ndarray:
+        """phi = sum_i g_i of the current pre-collision phase populations."""
+        g = self.get(F_POP_PHASE)
+        return g.reshape(int(self.params.phase_q), self.N).sum(0)
+

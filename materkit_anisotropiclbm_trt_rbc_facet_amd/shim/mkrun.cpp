@@ -1,0 +1,113 @@
+// MaterKit-style driver over the reference-API layer: the run_* loops of
+// MaterKit/MaterKit.cpp (run_flow_Poiseuille :244-288, run_flow_solutal_dendrite :399-465,
+// run_flow_thermal_dendrite :467-527, run_thermal_dendrite :743-788) with their per-step call
+// order, plus a fused loop.  The case file and the loop are chosen on the command line
+// (the reference hard-codes choice = 8, MaterKit.cpp:50).
+//
+//   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
+//   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
+//         rbc | fused
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "mkshim.h"
+
+static void frame_info(mkFluidsMRT& flow, int frame, int loops) {
+    std::printf("%05d\t%18d", frame, loops);
+    const mkg_params& p = flow.params();
+    if (p.phase_q) {
+        double v[4];
+        mkg_reduce(flow.handle(), MKG_R_SOLID, v);
+        std::printf("\tsolid_fraction %.8e", v[0]);
+        mkg_reduce(flow.handle(), MKG_R_TIP, v);
+        std::printf("\ttip_x %.3f", v[0]);
+    }
+    if (p.thermal && p.flow_q && p.T_face[4] != p.T_face[5])
+        std::printf("\tNu %.8f", flow.Thermal_NusseltRBC());
+    std::printf("\n");
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <case.xml> <loop> [maxLoops] [maxFrame] [vtk]\n", argv[0]);
+        return 2;
+    }
+    const std::string xml = argv[1], loop = argv[2];
+    int maxLoops = 0, maxFrame = 0;
+    try {
+        readParameters(maxLoops, maxFrame, xml);
+        if (argc > 3) maxLoops = std::atoi(argv[3]);
+        if (argc > 4) maxFrame = std::atoi(argv[4]);
+        const bool vtk = argc > 5 && std::strcmp(argv[5], "vtk") == 0;
+        if (maxFrame < 1) maxFrame = 1;
+        mkSpace space(xml);
+        mkFluidsMRT flow(space);
+        flow.mkFluidsMRT::initialize(xml);
+        if (loop == "poiseuille")
+            flow.mkFluidsMRT::initialize_geometry_Poiseuille();
+        else
+            flow.mkFluidsMRT::initialize_geometry();
+        std::printf("%20s= %d \n%20s= %d \n\nframe \t\t\t loops\n", "maxLoops ", maxLoops,
+                    "maxFrame ", maxFrame);
+        const int every = std::max(1, maxLoops / maxFrame);
+        int frame = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int loops = 0; loops <= maxLoops; loops++) {
+            if (loops % every == 0) {
+                frame_info(flow, frame, loops);
+                if (vtk) flow.write_vtkfile(loops);
+                frame++;
+            }
+            if (loops == maxLoops) break;
+            if (loop == "poiseuille") {
+                flow.mkFluidsLBM::propagate();
+                flow.mkFluidsLBM::relaxation();
+            } else if (loop == "thermal_dendrite") {
+                flow.mkPhaseField::propagate();
+                flow.mkPhaseField::relaxation_anisotropy_thermal();
+                flow.mkThermalField::propagate();
+                flow.mkThermalField::relaxation();
+            } else if (loop == "flow_thermal_dendrite") {
+                flow.mkFluidsMRT::propagate_PBB();
+                flow.mkFluidsMRT::relaxation();
+                flow.mkFluidsMRT::bdyVelocity();
+                flow.mkPhaseField::propagate();
+                flow.mkPhaseField::relaxation_anisotropy_thermal();
+                flow.mkThermalField::propagate();
+                flow.mkThermalField::relaxation();
+                flow.mkThermalField::bdy_flowin();
+            } else if (loop == "flow_solutal_dendrite") {
+                flow.mkFluidsMRT::propagate_PBB();
+                flow.mkFluidsMRT::relaxation();
+                flow.bdyVelocity();
+                flow.mkPhaseField::propagate();
+                flow.mkPhaseField::relaxation_anisotropy_solute(loops);
+                flow.mkSolutalField::propagate();
+                flow.mkSolutalField::relaxation();
+                flow.mkSolutalField::bdy_LBflowin();
+            } else if (loop == "rbc") {
+                flow.mkFluidsMRT::propagate();
+                flow.mkFluidsMRT::relaxation();
+                flow.mkThermalField::propagate();
+                flow.mkThermalField::relaxation();
+                flow.mkThermalField::bdy_RBC();
+            } else if (loop == "fused") {
+                flow.step(1);
+            } else {
+                std::fprintf(stderr, "unknown loop %s\n", loop.c_str());
+                return 2;
+            }
+        }
+        double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const double cells = (double)space.Mx() * space.My() * space.Mz();
+        std::printf("wall %.3f s, %.1f MLUPS (including frame output)\n", s,
+                    cells * maxLoops / s / 1e6);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,227 @@
+// Implementation of the reference-API host layer (see mkshim.h).
+#include "mkshim.h"
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+namespace {
+
+void throw_io(const std::string& what) { throw mkt::mkIOException(what + ": " + mkg_last_error()); }
+
+uint64_t splitmix(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+// counter-based uniform of one cell: splitmix64 of seed ^ (cell * K)
+double cell_uniform(uint64_t seed, uint64_t cell) {
+    uint64_t s = seed ^ (cell * 0xD1B54A32D192ED03ULL);
+    return (double)(splitmix(&s) >> 11) * 0x1.0p-53;
+}
+
+// <solver><perturbation> amplitude seed </perturbation> of a case file (0 when absent).
+bool read_perturbation(const std::string& xml, double& amp, uint64_t& seed) {
+    std::ifstream f(xml);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string s = ss.str();
+    size_t a = s.find("<perturbation>"), b = s.find("</perturbation>");
+    if (a == std::string::npos || b == std::string::npos) return false;
+    std::istringstream is(s.substr(a + 14, b - a - 14));
+    double sd = 0;
+    if (!(is >> amp >> sd)) return false;
+    seed = (uint64_t)sd;
+    return true;
+}
+
+}  // namespace
+
+void mkSpace::reset(const std::string& xmlFileName) {
+    mkg_params p;
+    if (mkg_params_from_xml(xmlFileName.c_str(), &p, nullptr)) throw_io("mkSpace");
+    mx_ = p.nx;
+    my_ = p.ny;
+    mz_ = p.nz;
+    xml_ = xmlFileName;
+}
+
+void readParameters(int& maxLoops, int& maxFrame, const std::string& xmlFileName) {
+    mkg_params p;
+    int32_t info[4];
+    if (mkg_params_from_xml(xmlFileName.c_str(), &p, info)) throw_io("readParameters");
+    maxLoops = info[0];
+    maxFrame = info[1];
+}
+
+mkFluidsLBM::~mkFluidsLBM() {
+    if (h_) mkg_destroy(h_);
+}
+
+void mkFluidsLBM::check(int rc) const {
+    if (rc) throw mkt::mkLogicException(mkg_last_error());
+}
+
+void mkFluidsLBM::initialize(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_params_from_xml(xmlfname.c_str(), &p_, run_info_)) throw_io("initialize");
+    // seeds carry their rotations into the parameter block before the handle is created
+    if (p_.phase_q && run_info_[2] > 0) {
+        std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS), R(9 * MKG_MAX_GRAINS);
+        int n = mkg_seeds_from_xml(xmlfname.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), R.data());
+        if (n < 0) throw_io("seeds");
+        if (p_.ngrains)
+            for (int k = 0; k < n; k++)
+                for (int i = 0; i < 9; i++) p_.rot[k][i] = R[9 * k + i];
+    }
+    if (h_) mkg_destroy(h_);
+    h_ = nullptr;
+    check(mkg_create(&p_, &h_));
+    // temperature: the case's initial value, or the conductive profile + noise of a
+    // Rayleigh-Benard case
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    std::vector<double> T;
+    if (p_.thermal) {
+        T.assign(N, p_.T_ref);
+        double amp = 0;
+        uint64_t seed = 0;
+        if (read_perturbation(xmlfname, amp, seed)) {
+            const double Th = p_.T_face[4], Tc = p_.T_face[5];
+            for (size_t id = 0; id < N; id++) {
+                const int z = (int)(id % p_.nz);
+                T[id] = Th - (Th - Tc) * (z + 0.5) / p_.nz + amp * (2.0 * cell_uniform(seed, id) - 1.0);
+            }
+        }
+    }
+    check(mkg_init_equilibrium(h_, nullptr, nullptr, p_.thermal ? T.data() : nullptr, nullptr,
+                               nullptr));
+}
+
+void mkFluidsLBM::initialize_geometry() {
+    if (!h_) throw mkt::mkLogicException("initialize_geometry before initialize");
+    if (!(p_.phase_q && run_info_[2] > 0)) return;
+    std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS);
+    int n = mkg_seeds_from_xml(xml_.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), nullptr);
+    if (n < 0) throw_io("seeds");
+    check(mkg_init_seeds(h_, n, c.data(), r.data()));
+    if (p_.solute) {
+        // c = A(phi) U0 in every cell, U0 = the case's initial supersaturation variable
+        const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+        std::vector<double> phi(N), C(N);
+        check(mkg_get_field(h_, MKG_F_PHI, phi.data(), MKG_DT_F64, N));
+        for (size_t i = 0; i < N; i++)
+            C[i] = 0.5 * ((1 + p_.k_part) - (1 - p_.k_part) * phi[i]) * p_.ref_c;
+        std::vector<double> T;
+        check(mkg_init_equilibrium(h_, nullptr, nullptr, nullptr, C.data(), nullptr));
+    }
+}
+
+void mkFluidsLBM::advance(int mask) {
+    if (!h_) throw mkt::mkLogicException("relaxation before initialize");
+    check(mkg_step(h_, 1, mask));
+    pending_ &= ~mask;
+}
+
+void mkFluidsLBM::step(int nsteps) {
+    if (!h_) throw mkt::mkLogicException("step before initialize");
+    check(mkg_step(h_, nsteps, MKG_FLOW | MKG_PHASE | MKG_THERMAL | MKG_SOLUTE));
+    pending_ = 0;
+}
+
+std::vector<double> mkFluidsLBM::field(int id) const {
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    std::vector<double> v(id == MKG_F_VEL ? 3 * N : N);
+    check(mkg_get_field(h_, id, v.data(), MKG_DT_F64, v.size()));
+    return v;
+}
+
+double mkFluidsLBM::Thermal_NusseltRBC() {
+    double v[4];
+    check(mkg_reduce(h_, MKG_R_NUSSELT, v));
+    return v[0];
+}
+
+double mkFluidsLBM::Solutal_NusseltRBC() {
+    // solutal analogue on the same definition: 1 + <u_z c> H / (D dC)
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    std::vector<double> u = field(MKG_F_VEL), c = field(MKG_F_CONC);
+    double s = 0;
+    for (size_t i = 0; i < N; i++) s += u[3 * i + 2] * c[i];
+    const double dC = p_.C_face[4] - p_.C_face[5];
+    if (dC == 0 || p_.D_l == 0) return 1.0;
+    return 1.0 + (s / N) * p_.nz / (p_.D_l * dC);
+}
+
+// Mean velocity in the mid cross-section normal to nPlane (MaterKit/inc/mkFluidsLBM.h:54-55).
+void mkFluidsLBM::check_variables(int loops, const char nPlane) {
+    if (!p_.flow_q) return;
+    std::vector<double> u = field(MKG_F_VEL);
+    double s[3] = {0, 0, 0};
+    long cnt = 0;
+    for (int x = 0; x < p_.nx; x++)
+        for (int y = 0; y < p_.ny; y++)
+            for (int z = 0; z < p_.nz; z++) {
+                bool in = (nPlane == 'x' && x == p_.nx / 2) || (nPlane == 'y' && y == p_.ny / 2) ||
+                          (nPlane == 'z' && z == p_.nz / 2);
+                if (!in) continue;
+                size_t id = ((size_t)x * p_.ny + y) * p_.nz + z;
+                for (int k = 0; k < 3; k++) s[k] += u[3 * id + k];
+                cnt++;
+            }
+    if (cnt)
+        std::printf("loops %d  mean velocity in the %c mid-plane: %.10e %.10e %.10e\n", loops, nPlane,
+                    s[0] / cnt, s[1] / cnt, s[2] / cnt);
+}
+
+void mkFluidsLBM::write_vtkfile(int loops) {
+    const int Mx = p_.nx, My = p_.ny, Mz = p_.nz;
+    const size_t N = (size_t)Mx * My * Mz;
+    std::ostringstream fn;
+    fn << workspace_ << "vtk/fluid_t" << loops << ".vtk";
+    std::ofstream o(fn.str());
+    if (!o) throw mkt::mkIOException("cannot write " + fn.str());
+    o << "# vtk DataFile Version 3.0\nfluid_state\nASCII\nDATASET RECTILINEAR_GRID\n";
+    o << "DIMENSIONS " << Mx << " " << My << " " << Mz << "\n";
+    auto coords = [&](const char* n, int m) {
+        o << n << " " << m << " float\n";
+        for (int i = 0; i < m; i++) o << i + 0.5 << " ";
+        o << "\n";
+    };
+    coords("X_COORDINATES", Mx);
+    coords("Y_COORDINATES", My);
+    coords("Z_COORDINATES", Mz);
+    o << "POINT_DATA " << N << "\n";
+    o << std::setprecision(9);
+    auto scalar = [&](const char* name, const std::vector<double>& v, double off) {
+        o << "SCALARS " << name << " float 1\nLOOKUP_TABLE default\n";
+        for (int z = 0; z < Mz; z++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++) o << v[((size_t)x * My + y) * Mz + z] - off << "\n";
+    };
+    if (p_.flow_q) scalar("density_difference", field(MKG_F_RHO), 1.0);
+    if (p_.phase_q) {
+        std::vector<int16_t> g(N);
+        check(mkg_get_field(h_, MKG_F_GRAIN, g.data(), MKG_DT_I16, N));
+        std::vector<double> gd(g.begin(), g.end());
+        scalar("grain_id", gd, 0.0);
+        std::vector<double> phi = field(MKG_F_PHI), prev = field(MKG_F_PHI_PREV);
+        scalar("order_parameter", phi, 0.0);
+        for (size_t i = 0; i < N; i++) prev[i] = phi[i] - prev[i];
+        scalar("delta_order_parameter", prev, 0.0);
+    }
+    if (p_.solute) scalar("X0_solute", field(MKG_F_CONC), 0.0);
+    if (p_.thermal) scalar("temperature", field(MKG_F_TEMP), 0.0);
+    if (p_.flow_q) {
+        std::vector<double> u = field(MKG_F_VEL);
+        o << "VECTORS velocity_vector float\n";
+        for (int z = 0; z < Mz; z++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++) {
+                    size_t id = ((size_t)x * My + y) * Mz + z;
+                    o << u[3 * id] << " " << u[3 * id + 1] << " " << u[3 * id + 2] << "\n";
+                }
+    }
+}

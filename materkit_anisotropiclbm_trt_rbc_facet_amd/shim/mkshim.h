@@ -1,0 +1,161 @@
+// C++ host layer with the reference's class API (MaterKit/inc/mkFluidsLBM.h, mkFluidsMRT.h,
+// mkFlowField.h, mkThermalField.h, mkSolutalField.h, mkPhaseField.h, mkSpace.h) over the
+// C ABI in include/mkg.h, so that MaterKit-style drivers (MaterKit/MaterKit.cpp run_* loops)
+// keep their call sequence.
+//
+// Semantics of the per-field calls:
+//   X::propagate() / propagate_PBB()   mark field X for streaming (no device work)
+//   X::relaxation*()                   one collide-stream of field X on the device; the other
+//                                      fields are read at their current state for coupling
+//                                      (mkg_step with that field's mask)
+//   bdy*() / bdy_*()                   no device work: the face rules of the case file are
+//                                      applied inside every collide-stream
+//   step(n)                            n fused steps of every field (one kernel pair per step)
+// Errors of the C ABI are thrown as mkt::mkIOException (case files) or
+// mkt::mkLogicException (everything else), as the reference's exception classes
+// (MaterKit/inc/mkRunTimeDiagnostics.h:71-117).
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mkg.h"
+
+namespace mkt {
+struct mkIOException : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct mkLogicException : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+}  // namespace mkt
+
+// Grid of a case file (MaterKit/inc/mkSpace.h:29-73).
+class mkSpace {
+public:
+    mkSpace() = default;
+    explicit mkSpace(const std::string& xmlFileName) { reset(xmlFileName); }
+    void reset(const std::string& xmlFileName);
+    int Mx() const { return mx_; }
+    int My() const { return my_; }
+    int Mz() const { return mz_; }
+    double step() const { return 1.0; }
+    const std::string& xml() const { return xml_; }
+
+private:
+    int mx_ = 0, my_ = 0, mz_ = 0;
+    std::string xml_;
+};
+
+class mkFluidsLBM {
+public:
+    mkFluidsLBM() = default;
+    explicit mkFluidsLBM(const mkSpace& space) : space_(space) {}
+    virtual ~mkFluidsLBM();
+    mkFluidsLBM(const mkFluidsLBM&) = delete;
+    mkFluidsLBM& operator=(const mkFluidsLBM&) = delete;
+
+    void reset(const std::string& xmlFileName) { space_.reset(xmlFileName); }
+    void reset(const mkSpace& space) { space_ = space; }
+
+    // parameters, device allocation and the equilibrium initial state of the case
+    virtual void initialize(const std::string& xmlfname);
+    // flags and seeds of the case (walls from the case's face codes)
+    virtual void initialize_geometry();
+
+    virtual void propagate() { pending_ |= MKG_FLOW; }
+    virtual void propagate_PBB() { pending_ |= MKG_FLOW; }
+    virtual void relaxation() { advance(MKG_FLOW); }
+    virtual void bounceback() {}
+    virtual void boundary() {}
+    virtual void boundary_RBC() {}
+    virtual void boundary_walls() {}
+    virtual void bdyPeriod() {}
+    virtual void bdyVelocity() {}
+
+    virtual double Thermal_NusseltRBC();
+    virtual double Solutal_NusseltRBC();
+    virtual void check_variables(int loops, const char nPlane);
+    // legacy VTK frame (RECTILINEAR_GRID, cell centres x+0.5, x fastest), file
+    // <workspace>vtk/fluid_t<loops>.vtk as read by the reference's SaveContourBatch.py:41
+    virtual void write_vtkfile(int loops);
+
+    // fused steps of every field
+    void step(int nsteps = 1);
+    // direct access for user code
+    mkg_handle handle() const { return h_; }
+    const mkg_params& params() const { return p_; }
+    std::vector<double> field(int id) const;
+    void set_workspace(const std::string& dir) { workspace_ = dir; }
+
+protected:
+    void advance(int mask);
+    void check(int rc) const;
+    mkSpace space_;
+    std::string xml_;
+    std::string workspace_ = "./";
+    mkg_handle h_ = nullptr;
+    mkg_params p_{};
+    int32_t run_info_[4] = {0, 0, 0, 0};
+    int pending_ = 0;
+};
+
+class mkFlowField : virtual public mkFluidsLBM {
+public:
+    mkFlowField() = default;
+    explicit mkFlowField(const mkSpace& space) : mkFluidsLBM(space) {}
+    void initialize_geometry_Poiseuille() { initialize_geometry(); }
+};
+
+class mkThermalField : virtual public mkFluidsLBM {
+public:
+    mkThermalField() = default;
+    explicit mkThermalField(const mkSpace& space) : mkFluidsLBM(space) {}
+    void propagate() override { pending_ |= MKG_THERMAL; }
+    void relaxation() override { advance(MKG_THERMAL); }
+    void bdy_RBC() {}
+    void bdy_noflux() {}
+    void bdy_flowin() {}
+    void bdy_constT() {}
+};
+
+class mkSolutalField : virtual public mkFluidsLBM {
+public:
+    mkSolutalField() = default;
+    explicit mkSolutalField(const mkSpace& space) : mkFluidsLBM(space) {}
+    void propagate() override { pending_ |= MKG_SOLUTE; }
+    void propagate_PBB() override { pending_ |= MKG_SOLUTE; }
+    void relaxation() override { advance(MKG_SOLUTE); }
+    void bdy_constX() {}
+    void bdy_RBC() {}
+    void bdy_LBflowin() {}
+};
+
+class mkPhaseField : virtual public mkFluidsLBM {
+public:
+    mkPhaseField() = default;
+    explicit mkPhaseField(const mkSpace& space) : mkFluidsLBM(space) {}
+    void propagate() override { pending_ |= MKG_PHASE; }
+    void relaxation() override { advance(MKG_PHASE); }
+    void relaxation_anisotropy_thermal() { advance(MKG_PHASE); }
+    void relaxation_anisotropy_solute(const int /*loops*/) { advance(MKG_PHASE); }
+    void reset_flags() {}  // flags are re-classified inside every phase step
+};
+
+class mkFluidsMRT : virtual public mkFluidsLBM,
+                    virtual public mkFlowField,
+                    virtual public mkThermalField,
+                    virtual public mkSolutalField,
+                    virtual public mkPhaseField {
+public:
+    mkFluidsMRT() = default;
+    explicit mkFluidsMRT(const mkSpace& space) : mkFluidsLBM(space) {}
+    void propagate() override { pending_ |= MKG_FLOW; }
+    void propagate_PBB() override { pending_ |= MKG_FLOW; }
+    void relaxation() override { advance(MKG_FLOW); }
+    void initialize(const std::string& xmlfname) override { mkFluidsLBM::initialize(xmlfname); }
+    void initialize_geometry() override { mkFluidsLBM::initialize_geometry(); }
+};
+
+// readParameters of the reference (MaterKit/inc/mkFunction.h:62): loops and frames.
+void readParameters(int& maxLoops, int& maxFrame, const std::string& xmlFileName);
