@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--fp64-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     return ap.parse_args()
 
@@ -109,17 +110,17 @@ def main():
     size = a.size
     N = size ** 3
     p32, wall32, tim32, nf32, solid32 = run_device(mkg.PREC_FP32, size, a.steps, a.warmup)
-    p64, wall64, tim64, nf64, _ = run_device(mkg.PREC_FP64, size, a.fp64_steps, 3)
+    if a.no_fp64:
+        p64, wall64, tim64, nf64 = None, None, None, 0.0
+    else:
+        p64, wall64, tim64, nf64, _ = run_device(mkg.PREC_FP64, size, a.fp64_steps, 3)
 
     b32, m32 = alg_bytes(4)
     b64, m64 = alg_bytes(8)
     mlups32 = N * a.steps / wall32 / 1e6
-    mlups64 = N * a.fp64_steps / wall64 / 1e6
     k32 = tim32["fused_ms"] / tim32["steps"] * 1e-3  # s per fused launch
-    k64 = tim64["fused_ms"] / tim64["steps"] * 1e-3
     km32 = tim32["phase_moments_ms"] / tim32["steps"] * 1e-3
     ach32 = b32 * N / k32 / 1e9
-    ach64 = b64 * N / k64 / 1e9
     step_bytes32 = (b32 + m32) * N
     traffic = None
     tf = ROOT / "profiles" / "traffic_fused256.json"
@@ -151,14 +152,20 @@ def main():
                      "alg_bytes_per_cell": b32},
         "step_hbm_frac": round(step_bytes32 / (wall32 / a.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {"fused": round(k32 * 1e3, 4), "phase_moments": round(km32 * 1e3, 4)},
-        "fp64": {"value": round(mlups64, 1), "unit": "MLUPS",
-                 "fused_kernel_gbs": round(ach64, 1),
-                 "frac": round(ach64 / HBM_PEAK_GBS, 4), "alg_bytes_per_cell": b64},
-        "fp32_over_fp64": round(mlups32 / mlups64, 3),
+        "fp64": None,
+        "fp32_over_fp64": None,
         "adjoint_over_forward": None,
         "nonfinite": nf32 + nf64,
         "solid_fraction": solid32,
     }
+    if tim64 is not None:
+        mlups64 = N * a.fp64_steps / wall64 / 1e6
+        k64 = tim64["fused_ms"] / tim64["steps"] * 1e-3
+        ach64 = b64 * N / k64 / 1e9
+        out["fp64"] = {"value": round(mlups64, 1), "unit": "MLUPS",
+                       "fused_kernel_gbs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                       "alg_bytes_per_cell": b64}
+        out["fp32_over_fp64"] = round(mlups32 / mlups64, 3)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)

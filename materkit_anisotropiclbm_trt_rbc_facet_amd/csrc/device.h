@@ -94,13 +94,41 @@ __device__ inline uint32_t nbr(const Cell& c, int dx, int dy, int dz) {
 }
 
 // ------------------------------------------------------------------ population access
+// Populations are touched exactly once per step, so their loads and stores can bypass cache
+// retention (MKG_NT_POPS) and leave L2 to the phase stencil and the shifted lines of O steps.
+#ifndef MKG_NT_POPS
+#define MKG_NT_POPS 0
+#endif
+#ifndef MKG_NT_LOAD
+#define MKG_NT_LOAD MKG_NT_POPS
+#endif
+#ifndef MKG_NT_STORE
+#define MKG_NT_STORE MKG_NT_POPS
+#endif
+template <class R>
+__device__ inline R ldp(const R* p) {
+#if MKG_NT_LOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <class R>
+__device__ inline void stp(R* p, R v) {
+#if MKG_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 template <int Q, bool ODD, class R>
 __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                  const StepArgs& a, int kind) {
     const R* __restrict__ base = (const R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) p[i] = base[(size_t)i * S + c.id];
+        for (int i = 0; i < Q; i++) p[i] = ldp(&base[(size_t)i * S + c.id]);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -109,19 +137,19 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
             if (hit == 0u) {
                 uint32_t src = nbr(c, dx, dy, dz);
                 if (a.has_walls && i != 0 && a.flag[src] == kIsWalls)
-                    p[i] = base[(size_t)i * S + c.id];
+                    p[i] = ldp(&base[(size_t)i * S + c.id]);
                 else
-                    p[i] = base[(size_t)inv(i) * S + src];
+                    p[i] = ldp(&base[(size_t)inv(i) * S + src]);
             } else {
                 int face = __builtin_ctz(hit);
                 int t = pick6(face, L.bc);
                 if (t == kBcOutflow) {
-                    p[i] = base[(size_t)inv(i) * S + c.id];
+                    p[i] = ldp(&base[(size_t)inv(i) * S + c.id]);
                 } else if (t == kBcDirichlet && kind == kKindScalar) {
-                    p[i] = -base[(size_t)i * S + c.id] +
+                    p[i] = -ldp(&base[(size_t)i * S + c.id]) +
                            (R)(2.0 * wgt<Q>(i) * (pick6(face, L.fval) - L.ref));
                 } else {
-                    R v = base[(size_t)i * S + c.id];
+                    R v = ldp(&base[(size_t)i * S + c.id]);
                     if (kind == kKindFlow) {
                         double cu = face_cu(face, cx(i), cy(i), cz(i), a.u_face);
                         v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
@@ -139,7 +167,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) base[(size_t)inv(i) * S + c.id] = p[i];
+        for (int i = 0; i < Q; i++) stp(&base[(size_t)inv(i) * S + c.id], p[i]);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -148,9 +176,9 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
             if (hit == 0u) {
                 uint32_t tgt = nbr(c, dx, dy, dz);
                 if (a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
-                    base[(size_t)inv(i) * S + c.id] = p[i];
+                    stp(&base[(size_t)inv(i) * S + c.id], p[i]);
                 else
-                    base[(size_t)i * S + tgt] = p[i];
+                    stp(&base[(size_t)i * S + tgt], p[i]);
             } else {
                 int face = __builtin_ctz(hit);
                 int t = pick6(face, L.bc);
@@ -166,7 +194,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
                         v -= (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
                     }
                 }
-                base[(size_t)inv(i) * S + c.id] = v;
+                stp(&base[(size_t)inv(i) * S + c.id], v);
             }
         }
     }

@@ -27,7 +27,11 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
     Cell c;
     if (!locate(a, c)) return;
     const int8_t fl = a.flag[c.id];
-    if (fl == kIsWalls) return;
+    if (fl == kIsWalls) {
+        // wall cells keep their phase value: carry it into the buffer being written
+        phi_out[c.id] = ((const R*)a.phi_prev)[c.id];
+        return;
+    }
     R g[PQ];
     read_pops<PQ, ODD, R>(g, a.g, a.S, c, a, kKindScalar);
     R s = 0;

@@ -549,9 +549,10 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
         int qg = p->phase_q;
 #pragma omp parallel for schedule(static)
         for (long x = 0; x < N; x++) {
+            s->phi_prev[x] = s->phi[x];
+            if (s->flag[x] == 3) continue; /* wall cells keep their phase value */
             double ph = 0;
             for (int i = 0; i < qg; i++) ph += s->g[i * N + x];
-            s->phi_prev[x] = s->phi[x];
             s->phi[x] = ph;
             s->flag[x] = classify(p, s->flag[x], ph);
         }

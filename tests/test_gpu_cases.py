@@ -1,0 +1,207 @@
+"""Configuration-level runs on the device: closed forms, the FP32-vs-FP64 field error, seed
+geometry, checkpoint / resume, determinism and the reference-API driver."""
+import hashlib
+import json
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import compare_pops, load_oracle
+from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+from oracle import oracle as O
+from test_oracle_closed_forms import duct_series
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_config1_poiseuille_duct_startup():
+    """64 x 32 x 32, D3Q19 BGK tau = 0.8, g_x = 1e-6, 1000 steps from rest: the transient
+    rectangular-duct series (walls half-way, H = 32)."""
+    case = cases.load("C1")
+    d = mkg.Solver(case.params)
+    cases.setup(d, case)
+    print("[cases] C1 1000 steps", flush=True)
+    d.step(1000, mkg.ALL)
+    u = d.get(mkg.F_VEL).reshape(64, 32, 32, 3)
+    nu = (1 / 1.25 - 0.5) / 3
+    ref = duct_series(32, 32, nu, 1e-6, 1000)
+    for x in (0, 31, 63):
+        err = np.linalg.norm(u[x, :, :, 0] - ref) / np.linalg.norm(ref)
+        assert err < 0.01, err
+    assert np.abs(u[..., 1:]).max() < 1e-12 * np.abs(u[..., 0]).max() + 1e-18
+    mass = d.reduce(mkg.R_MASS)[0]
+    assert abs(mass - 64 * 32 * 32) < 1e-9
+    d.close()
+
+
+def test_config1_full_size_matches_oracle():
+    case = cases.load("C1")
+    d = mkg.Solver(case.params)
+    cases.setup(d, case)
+    o = O.State.empty(case.params)
+    o.init_equilibrium()
+    d.step(100, mkg.ALL)
+    o.step(100, 1, O.OP_BGK)
+    err = compare_pops(d, o)
+    d.close()
+    assert err["f"] < 1e-13
+
+
+def test_config2_rbc_fp32_field_error_and_convection():
+    """Rayleigh-Benard (128 x 128 x 64, Ra 1e5, TRT + D3Q7): FP32 deviation storage against the
+    FP64 path over a short horizon (pointwise, before chaos amplifies round-off), and onset of
+    convection (Nu > 1) in a longer FP64 run at reduced size."""
+    c64 = cases.load("C2")
+    c32 = cases.load("C2", precision=mkg.PREC_FP32)
+    d64, d32 = mkg.Solver(c64.params), mkg.Solver(c32.params)
+    cases.setup(d64, c64)
+    cases.setup(d32, c32)
+    print("[cases] C2 200 steps fp32 vs fp64", flush=True)
+    d64.step(200, mkg.ALL)
+    d32.step(200, mkg.ALL)
+    T64, T32 = d64.get(mkg.F_TEMP), d32.get(mkg.F_TEMP)
+    u64, u32 = d64.get(mkg.F_VEL), d32.get(mkg.F_VEL)
+    r64, r32 = d64.get(mkg.F_RHO), d32.get(mkg.F_RHO)
+    eT = np.abs(T32 - T64).max() / np.abs(T64).max()
+    eu = np.abs(u32 - u64).max() / max(np.abs(u64).max(), 1e-300)
+    erho = np.abs(r32 - r64).max() / np.abs(r64).max()
+    print(f"[cases] C2 fp32 rel max err T {eT:.2e} u {eu:.2e} rho {erho:.2e}", flush=True)
+    assert eT < 1e-5 and erho < 1e-5
+    assert eu < 1e-3  # velocity is O(1e-4) and grows from noise of amplitude 1e-3
+    d64.close()
+    d32.close()
+    small = cases.load("C2", nx=64, ny=64, nz=32)
+    d = mkg.Solver(small.params)
+    cases.setup(d, small)
+    d.step(6000, mkg.ALL)
+    nu = d.reduce(mkg.R_NUSSELT)[0]
+    print(f"[cases] C2 64x64x32 Nu after 6000 steps {nu:.4f}", flush=True)
+    assert nu > 1.5
+    d.close()
+
+
+def test_config2_conduction_below_onset():
+    """Ra = 1000 < 1708: the noise decays and Nu -> 1."""
+    case = cases.load("C2", nx=32, ny=32, nz=16)
+    p = case.params
+    H, uff = 16, 0.05
+    Ra, Pr = 1000.0, 0.71
+    nu = uff * H * math.sqrt(Pr / Ra)
+    p.s_nu = p.s_bulk = p.s_even = 1 / (3 * nu + 0.5)
+    p.s_odd = 1 / ((3 / 16) / (3 * nu) + 0.5)
+    p.alpha_l = p.alpha_s = nu / Pr
+    p.buoy[2] = uff * uff / H
+    d = mkg.Solver(p)
+    cases.setup(d, case)
+    d.step(8000, mkg.ALL)
+    Nu = d.reduce(mkg.R_NUSSELT)[0]
+    d.close()
+    assert abs(Nu - 1.0) < 1e-3, Nu
+
+
+def test_config3_dendrite_growth_fp32_vs_fp64():
+    """Single seed, eps4 = 0.05, Delta = 0.55 at 64^3: the solid grows, the tips stay cubic
+    (equal along x, y, z), and FP32 deviation storage tracks FP64."""
+    outs = {}
+    for prec in (mkg.PREC_FP64, mkg.PREC_FP32):
+        case = cases.load("C3", nx=64, ny=64, nz=64, precision=prec)
+        d = mkg.Solver(case.params)
+        cases.setup(d, case)
+        fs0 = d.reduce(mkg.R_SOLID)[0]
+        print(f"[cases] C3 64^3 prec={prec} 1500 steps", flush=True)
+        d.step(1500, mkg.ALL)
+        outs[prec] = (d.get(mkg.F_PHI), d.get(mkg.F_TEMP), d.reduce(mkg.R_SOLID)[0],
+                      d.reduce(mkg.R_TIP), fs0, d.reduce(mkg.R_NONFINITE)[0])
+        d.close()
+    phi64, T64, fs64, tip64, fs0, nf = outs[mkg.PREC_FP64]
+    phi32, T32, fs32, tip32, _, nf32 = outs[mkg.PREC_FP32]
+    assert nf == 0 and nf32 == 0
+    assert fs64 > 1.5 * fs0
+    assert np.ptp(tip64) < 1e-9  # cubic symmetry of the three tips
+    ephi = np.abs(phi32 - phi64).max() / np.abs(phi64).max()
+    eT = np.abs(T32 - T64).max() / np.abs(T64).max()
+    print(f"[cases] C3 fp32 rel max err phi {ephi:.2e} T {eT:.2e} solid {fs32:.6f}/{fs64:.6f}",
+          flush=True)
+    assert ephi < 1e-4 and eT < 1e-4
+
+
+def test_seed_geometry_matches_golden_on_device():
+    g = json.loads((ROOT / "tests" / "golden" / "seed_geometry.json").read_text())
+    p = mkg.default_params(nx=g["dims"][0], ny=g["dims"][1], nz=g["dims"][2], phase_q=19,
+                           thermal=1, pf_wdx2=g["pf_wdx2"], phi_s=g["phi_s"])
+    d = mkg.Solver(p)
+    d.init_seeds(np.array(g["centers"]), np.array(g["radii"]))
+    assert hashlib.sha256(d.get(mkg.F_FLAG).tobytes()).hexdigest() == g["flag_sha256"]
+    assert hashlib.sha256(d.get(mkg.F_GRAIN).tobytes()).hexdigest() == g["grain_sha256"]
+    d.close()
+
+
+def test_config4_multiseed_runs_and_matches_oracle_at_start():
+    case = cases.load("C4", nx=48, ny=48, nz=48, precision=mkg.PREC_FP64)
+    d = mkg.Solver(case.params)
+    cases.setup(d, case)
+    o = O.State.empty(case.params)
+    c, r, _ = case.seeds
+    o.init_seeds(c, r)
+    np.testing.assert_array_equal(d.get(mkg.F_FLAG), o.flag)
+    np.testing.assert_array_equal(d.get(mkg.F_GRAIN), o.grain)
+    # same state on both sides, then 10 coupled steps
+    st = {"flag": o.flag.copy(), "grain": o.grain.copy(), "f": d.get(mkg.F_POP_FLOW),
+          "g": d.get(mkg.F_POP_PHASE), "c": d.get(mkg.F_POP_SOLUTE), "phi": d.get(mkg.F_PHI),
+          "phi_prev": d.get(mkg.F_PHI_PREV)}
+    o2 = load_oracle(case.params, st)
+    d.step(10, mkg.ALL)
+    o2.step(10, 15, O.OP_MRT)
+    err = compare_pops(d, o2)
+    assert all(e < 1e-11 for e in err.values()), err
+    d.step(300, mkg.ALL)
+    assert d.reduce(mkg.R_NONFINITE)[0] == 0
+    d.close()
+
+
+def test_checkpoint_resume_is_bit_exact(tmp_path):
+    case = cases.load("fused256", nx=32, ny=32, nz=32, precision=mkg.PREC_FP32)
+    a = mkg.Solver(case.params)
+    cases.setup(a, case)
+    a.step(7, mkg.ALL)
+    path = tmp_path / "ck.bin"
+    a.checkpoint_save(path)
+    a.step(9, mkg.ALL)
+    ref = a.get(mkg.F_POP_PHASE), a.get(mkg.F_POP_FLOW), a.get(mkg.F_FLAG)
+    a.close()
+    b = mkg.Solver(case.params)
+    b.checkpoint_load(path)
+    b.step(9, mkg.ALL)
+    got = b.get(mkg.F_POP_PHASE), b.get(mkg.F_POP_FLOW), b.get(mkg.F_FLAG)
+    b.close()
+    for x, y in zip(ref, got):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_runs_are_deterministic():
+    outs = []
+    for _ in range(2):
+        case = cases.load("fused256", nx=40, ny=36, nz=32, precision=mkg.PREC_FP32)
+        d = mkg.Solver(case.params)
+        cases.setup(d, case)
+        d.step(20, mkg.ALL)
+        outs.append((d.get(mkg.F_POP_PHASE), d.reduce(mkg.R_MASS)[0], d.reduce(mkg.R_SOLID)[0]))
+        d.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1] and outs[0][2] == outs[1][2]
+
+
+def test_reference_api_driver_runs_the_thermal_dendrite_loop(tmp_path):
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    xml = tmp_path / "c3small.xml"
+    src = (ROOT / "cases" / "C3_free_dendrite.xml").read_text()
+    xml.write_text(src.replace("256   256   256", "32 32 32").replace("127.5 127.5 127.5",
+                                                                      "15.5 15.5 15.5"))
+    res = subprocess.run([str(exe), str(xml), "thermal_dendrite", "40", "4"], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "solid_fraction" in res.stdout and "MLUPS" in res.stdout

@@ -1,7 +1,7 @@
 """Device (HIP, AA pattern, in-place) vs FP64 CPU oracle (two-lattice pull) on random states.
 
 FP64 storage must agree to 1e-12 (relative to the field's magnitude); FP32 deviation storage
-to 2e-6 after the same steps.  Flags must be bit-identical.  Both AA layouts are covered by
+to 1e-5 (the stated field-error bound) after the same steps.  Flags must be bit-identical.  Both AA layouts are covered by
 running an odd and an even number of steps.
 """
 import numpy as np
@@ -129,7 +129,7 @@ def test_fp32_deviation_storage_close_to_oracle(case):
     errs, fd, fo, dev, orc = _run(case, "mixed", 6, prec=mkg.PREC_FP32)
     dev.close()
     for k, e in errs.items():
-        assert e < 2e-6, f"{k}: rel err {e:.3e}"
+        assert e < 1e-5, f"{k}: rel err {e:.3e}"
 
 
 def test_macroscopic_fields_match_oracle():
