@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +34,7 @@ int fail(const std::string& msg) {
     } while (0)
 
 enum { LF = 0, LG = 1, LH = 2, LC = 3 };
+constexpr uint32_t kSlotPad = 0;
 
 }  // namespace
 
@@ -330,7 +332,14 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
     h->fp32 = p->precision == MKG_PREC_FP32;
     h->esz = h->fp32 ? 4 : 8;
     h->N = (uint32_t)((size_t)p->nx * p->ny * p->nz);
-    h->S = (h->N + 127u) & ~127u;
+    // Slot stride: the cell count rounded up to 128 elements plus a pad, so that the slots of
+    // one lattice do not all start at the same offset modulo a large power of two (which would
+    // send the Q loads of a cell to the same memory channel).  MKG_SLOT_PAD overrides the pad.
+    {
+        uint32_t pad = kSlotPad;
+        if (const char* env = std::getenv("MKG_SLOT_PAD")) pad = (uint32_t)std::strtoul(env, nullptr, 10);
+        h->S = ((h->N + 127u) & ~127u) + ((pad + 127u) & ~127u);
+    }
     h->nblk = (h->N + 255u) / 256u;
     h->q[LF] = p->flow_q;
     h->q[LG] = p->phase_q;

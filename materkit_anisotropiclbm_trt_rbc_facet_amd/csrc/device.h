@@ -36,7 +36,12 @@ struct Cell {
 // neighbour rows it reads stay in its own L2.
 __device__ inline bool locate(const StepArgs& a, Cell& c) {
     uint32_t b = blockIdx.x;
+#ifndef MKG_XCD_REMAP
+#define MKG_XCD_REMAP 1
+#endif
+#if MKG_XCD_REMAP
     if ((a.nblk & 7u) == 0u) b = (b & 7u) * (a.nblk >> 3) + (b >> 3);
+#endif
     uint32_t t = b * 256u + threadIdx.x;
     if (t >= a.N) return false;
     uint32_t nz = (uint32_t)a.nz, ny = (uint32_t)a.ny, nx = (uint32_t)a.nx;
@@ -222,39 +227,61 @@ __device__ inline void read_pops_rt(R* p, const LatArgs& L, uint32_t S, const Ce
 template <int Q, class R>
 __device__ inline void flow_collide(R* f, R dr, R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,
                                     R s_nu, R s_b, R s_o, R s_e, R ns) {
-    const R uu = ux * ux + uy * uy + uz * uz;
-    R feq[Q], neq[Q];
+    // Second moments of the stored deviations and the non-equilibrium stress:
+    // sum c c f~ = sum c c f - ref cs2 I and sum c c f~eq = dr cs2 I + rho u u.
     R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const R w = (R)wgt<Q>(i);
-        const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
-        feq[i] = w * (dr + rho * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu));
-        const R n = f[i] - feq[i];
-        neq[i] = n;
-        if (cx(i)) { pxx += n; jx += cx(i) * n; }
-        if (cy(i)) { pyy += n; jy += cy(i) * n; }
-        if (cz(i)) { pzz += n; jz += cz(i) * n; }
-        if (cx(i) * cy(i)) pxy += (R)(cx(i) * cy(i)) * n;
-        if (cx(i) * cz(i)) pxz += (R)(cx(i) * cz(i)) * n;
-        if (cy(i) * cz(i)) pyz += (R)(cy(i) * cz(i)) * n;
+    for (int i = 1; i < Q; i++) {
+        const R v = f[i];
+        if (cx(i)) { pxx += v; jx += (R)cx(i) * v; }
+        if (cy(i)) { pyy += v; jy += (R)cy(i) * v; }
+        if (cz(i)) { pzz += v; jz += (R)cz(i) * v; }
+        if (cx(i) * cy(i)) pxy += (R)(cx(i) * cy(i)) * v;
+        if (cx(i) * cz(i)) pxz += (R)(cx(i) * cz(i)) * v;
+        if (cy(i) * cz(i)) pyz += (R)(cy(i) * cz(i)) * v;
     }
+    const R third = (R)(1.0 / 3.0);
+    const R dc = dr * third;
+    pxx -= dc + rho * ux * ux;
+    pyy -= dc + rho * uy * uy;
+    pzz -= dc + rho * uz * uz;
+    pxy -= rho * ux * uy;
+    pxz -= rho * ux * uz;
+    pyz -= rho * uy * uz;
+    // momentum non-equilibrium (-F/2 analytically)
+    jx -= rho * ux;
+    jy -= rho * uy;
+    jz -= rho * uz;
     const R tr = pxx + pyy + pzz;
-    const R t3 = tr * (R)(1.0 / 3.0);
+    const R t3 = tr * third;
     const R txx = pxx - t3, tyy = pyy - t3, tzz = pzz - t3;
-    // second-order forcing tensor uF + Fu
-    const R fxx = (R)2 * ux * Fx, fyy = (R)2 * uy * Fy, fzz = (R)2 * uz * Fz;
+    // second-order forcing tensor uF + Fu, traceless part and trace
+    const R ftr = (R)2 * (ux * Fx + uy * Fy + uz * Fz);
+    const R f3 = ftr * third;
+    const R gxx = (R)2 * ux * Fx - f3, gyy = (R)2 * uy * Fy - f3, gzz = (R)2 * uz * Fz - f3;
     const R fxy = ux * Fy + uy * Fx, fxz = ux * Fz + uz * Fx, fyz = uy * Fz + uz * Fy;
-    const R ftr = fxx + fyy + fzz;
-    const R f3 = ftr * (R)(1.0 / 3.0);
-    const R gxx = fxx - f3, gyy = fyy - f3, gzz = fzz - f3;
     const R a_nu = (R)1 - s_nu, a_b = (R)1 - s_b, a_o = (R)1 - s_o, a_e = (R)1 - s_e;
     const R b_nu = (R)1 - (R)0.5 * s_nu, b_b = (R)1 - (R)0.5 * s_b;
-    R gh[Q], base[Q];
+    const R uu = ux * ux + uy * uy + uz * uz;
+    const R keep = (R)1 - ns;
+    // rest population (even only)
+    {
+        const R w = (R)wgt<Q>(0);
+        const R feq = w * (dr - (R)1.5 * rho * uu);
+        const R bk = (R)-1.5 * w * tr;
+        const R G = f[0] - feq - bk;
+        const R post = feq + a_b * bk + a_e * G + b_b * ((R)-1.5 * w * ftr);
+        f[0] = keep * post + ns * f[0];
+    }
+    // opposite pairs: even part E and odd part O, f*_i = E + O, f*_inv(i) = E - O
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
+    for (int i = 1; i < Q; i += 2) {
+        const int k = i + 1;  // inv(i)
         const R w = (R)wgt<Q>(i);
         const int X = cx(i), Y = cy(i), Z = cz(i);
+        const R cu = X * ux + Y * uy + Z * uz;
+        const R fe = w * (dr + rho * ((R)4.5 * cu * cu - (R)1.5 * uu));  // even part of f~eq
+        const R fo = w * rho * (R)3 * cu;                                 // odd part of f~eq
         const R cTc = (R)(X * X) * txx + (R)(Y * Y) * tyy + (R)(Z * Z) * tzz +
                       (R)(2 * X * Y) * pxy + (R)(2 * X * Z) * pxz + (R)(2 * Y * Z) * pyz;
         const R cGc = (R)(X * X) * gxx + (R)(Y * Y) * gyy + (R)(Z * Z) * gzz +
@@ -262,24 +289,15 @@ __device__ inline void flow_collide(R* f, R dr, R rho, R ux, R uy, R uz, R Fx, R
         const R sh = (R)4.5 * w * cTc;
         const R bk = (R)1.5 * w * (R)(csq(i) - 1) * tr;
         const R mo = (R)3 * w * (X * jx + Y * jy + Z * jz);
-        gh[i] = neq[i] - mo - sh - bk;
-        const R cF = X * Fx + Y * Fy + Z * Fz;
-        base[i] = feq[i] + (R)1.5 * w * cF + a_nu * sh + a_b * bk +
-                  b_nu * ((R)4.5 * w * cGc) + b_b * ((R)1.5 * w * (R)(csq(i) - 1) * ftr);
+        const R fi = f[i], fk = f[k];
+        const R ge = (R)0.5 * (fi + fk) - fe - sh - bk;
+        const R go = (R)0.5 * (fi - fk) - fo - mo;
+        const R E = fe + a_nu * sh + a_b * bk + a_e * ge + b_nu * ((R)4.5 * w * cGc) +
+                    b_b * ((R)1.5 * w * (R)(csq(i) - 1) * ftr);
+        const R O = fo + (R)1.5 * w * (X * Fx + Y * Fy + Z * Fz) + a_o * go;
+        f[i] = keep * (E + O) + ns * fk;
+        f[k] = keep * (E - O) + ns * fi;
     }
-    R out[Q];
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const R go = (R)0.5 * (gh[i] - gh[inv(i)]);
-        const R ge = (R)0.5 * (gh[i] + gh[inv(i)]);
-        out[i] = base[i] + a_o * go + a_e * ge;
-    }
-    if (ns != (R)0) {
-#pragma unroll
-        for (int i = 0; i < Q; i++) out[i] = ((R)1 - ns) * out[i] + ns * f[inv(i)];
-    }
-#pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = out[i];
 }
 
 // ------------------------------------------------------------------ phase-field source

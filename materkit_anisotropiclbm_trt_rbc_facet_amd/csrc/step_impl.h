@@ -43,8 +43,13 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
     if (nf != fl) a.flag[c.id] = nf;
 }
 
+// Minimum waves per SIMD requested from the register allocator for the fused kernel.
+#ifndef MKG_STEP_WAVES
+#define MKG_STEP_WAVES 1
+#endif
+
 template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD>
-__global__ void __launch_bounds__(256) k_step(const StepArgs a) {
+__global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) {
     Cell c;
     if (!locate(a, c)) return;
     const int8_t fl = a.flag[c.id];
@@ -169,8 +174,6 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
     //      relaxation_anisotropy_thermal / _solute)
     if constexpr (PQ > 0) {
         if (a.g.adv) {
-            R g[PQ];
-            read_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
             const R* __restrict__ ph = (const R*)a.phi_cur;
             uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
             uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
@@ -203,6 +206,9 @@ __global__ void __launch_bounds__(256) k_step(const StepArgs a) {
             R as;
             const R Sg = phase_source<R>(P, upf, (R)a.eps4, (R)a.lambda, (R)a.pf_dt,
                                          (R)a.pf_wdx2, Mrot, rot, as);
+            // the populations are read after the source so that the stencil values are dead
+            R g[PQ];
+            read_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
             const R om = (R)a.omega_g;
             const R dp = phi - (R)a.g.ref;
 #pragma unroll
