@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -67,6 +69,42 @@ def run_device(prec, size, steps, warmup):
     solid = dev.reduce(mkg.R_SOLID)[0]
     dev.close()
     return case.params, wall, tim, nonfinite, solid
+
+
+def adjoint_cost(steps: int):
+    """Inverse-problem forward model (phase field + thermal, 128^3, FP32, per-cell kappa):
+    time of the plain forward steps vs the adjoint pass (checkpointed forward, per-segment
+    recompute and reverse sweep), checkpoints every steps/20 steps."""
+    import numpy as np
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+
+    case = cases.load("C5", precision=mkg.PREC_FP32, eps4=0.05)
+    kap = cases.lognormal_field((128, 128, 128), *case.conductivity).reshape(-1)
+    dev = mkg.Solver(case.params)
+    cases.setup(dev, case)
+    dev.set(mkg.F_KAPPA, kap)
+    dev.step(2, mkg.ALL)
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps, mkg.ALL)
+    dev.sync()
+    fwd = time.perf_counter() - t0
+    target = dev.phi_now()
+    dev.close()
+    dev = mkg.Solver(case.params)
+    cases.setup(dev, case)
+    dev.set(mkg.F_KAPPA, kap)
+    out = dev.adjoint_run(steps, max(1, steps // 20), target + 0.01)
+    dev.close()
+    adj = (out["recompute_ms"] + out["reverse_ms"]) * 1e-3
+    return {"forward_s": round(fwd, 4), "adjoint_s": round(adj, 4),
+            "checkpointed_forward_s": round(out["forward_ms"] * 1e-3, 4),
+            "ratio": round(adj / fwd, 3), "steps": steps, "cells": 128 ** 3,
+            "workload": "phase field (D3Q19) + thermal (D3Q7), 128^3, FP32, per-cell kappa, "
+                        f"checkpoint every {max(1, steps // 20)} steps"}
 
 
 def cpu_baseline(seconds: float):
@@ -166,6 +204,10 @@ def main():
                        "fused_kernel_gbs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                        "alg_bytes_per_cell": b64}
         out["fp32_over_fp64"] = round(mlups32 / mlups64, 3)
+    if not a.no_adjoint:
+        adj = adjoint_cost(a.adjoint_steps)
+        out["adjoint_over_forward"] = adj["ratio"]
+        out["adjoint"] = adj
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -22,6 +22,40 @@ __device__ inline R clamp01(R v) {
     return v < (R)0 ? (R)0 : (v > (R)1 ? (R)1 : v);
 }
 
+// 19-point phase stencil around the cell (periodic faces wrap, other faces replicate the
+// edge value), index (dx+1)*9 + (dy+1)*3 + (dz+1).
+template <class R>
+__device__ inline void load_phase_stencil(const StepArgs& a, const Cell& c, R (&P)[27]) {
+    const R* __restrict__ ph = (const R*)a.phi_cur;
+    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & XLO) px[0] = px[1];
+    if (np & XHI) px[2] = px[1];
+    if (np & YLO) py[0] = py[1];
+    if (np & YHI) py[2] = py[1];
+    if (np & ZLO) pz[0] = pz[1];
+    if (np & ZHI) pz[2] = pz[1];
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
+            ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+}
+
+// Isotropic LB-weighted gradient of the stencil (D3Q19 weights over cs^2 = 1/3).
+template <class R>
+__device__ inline void stencil_gradient(const R (&P)[27], R (&gr)[3]) {
+    gr[0] = gr[1] = gr[2] = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)(3.0 * wgt<19>(i)) * P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)];
+        if (cx(i)) gr[0] += (R)cx(i) * v;
+        if (cy(i)) gr[1] += (R)cy(i) * v;
+        if (cz(i)) gr[2] += (R)cz(i) * v;
+    }
+}
+
 template <class R, int PQ, bool ODD>
 __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __restrict__ phi_out) {
     Cell c;
@@ -149,21 +183,30 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
         }
     }
 
-    // ---- solute collide-stream, c = A(phi) U (mkSolutalField::propagate + relaxation)
+    // ---- solute collide-stream (mkSolutalField::propagate + relaxation): the conserved
+    //      c = A(phi) U obeys dc/dt + div(c u) = div(D q grad U), written as an
+    //      advection-diffusion of c with D_c = D q / A and the drift V = D q grad A / A^2
+    R P[27];
+    if constexpr (SO && PQ > 0) load_phase_stencil<R>(a, c, P);
     if constexpr (SO) {
         if (a.c.adv) {
+            R gphi[3] = {0, 0, 0};
+            if constexpr (PQ > 0) stencil_gradient<R>(P, gphi);
             R q = (R)0.5 * ((R)1 - phi);
             if (a.D_l > 0) q += (R)(a.D_s / a.D_l) * (R)0.5 * ((R)1 + phi);
             if (q < (R)a.q_min) q = (R)a.q_min;
-            const R om = (R)1 / ((R)0.5 + (R)a.D_l * q / (R)0.25);
+            const R A = (R)0.5 * (((R)1 + (R)a.k_part) - ((R)1 - (R)a.k_part) * phi);
+            const R Dc = (R)a.D_l * q / A;
+            const R om = (R)1 / ((R)0.5 + Dc / (R)0.25);
+            const R vfac = (R)(-0.5 * (1.0 - a.k_part)) * (R)a.D_l * q / (A * A);
+            const R vx = ux + vfac * gphi[0], vy = uy + vfac * gphi[1], vz = uz + vfac * gphi[2];
             const R src = (R)a.latent_C * dphi;
-            const R ref = (R)a.c.ref;
 #pragma unroll
             for (int i = 0; i < 7; i++) {
                 const R w = (R)wgt<7>(i);
-                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
-                const R ceq = (i == 0) ? (C - (R)(1.0 - wgt<7>(0)) * U) - w * ref
-                                       : w * (U * ((R)1 + cu / (R)0.25) - ref);
+                const R cv = cx(i) * vx + cy(i) * vy + cz(i) * vz;
+                // deviation form: c~eq = w (c - ref) + w c cv / cs^2
+                const R ceq = w * (sc + C * cv / (R)0.25);
                 cc[i] = cc[i] - om * (cc[i] - ceq) + w * src;
             }
             write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
@@ -174,22 +217,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
     //      relaxation_anisotropy_thermal / _solute)
     if constexpr (PQ > 0) {
         if (a.g.adv) {
-            const R* __restrict__ ph = (const R*)a.phi_cur;
-            uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
-            uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
-            uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
-            const uint32_t np = (uint32_t)a.g.npmask & c.bits;
-            if (np & XLO) px[0] = px[1];
-            if (np & XHI) px[2] = px[1];
-            if (np & YLO) py[0] = py[1];
-            if (np & YHI) py[2] = py[1];
-            if (np & ZLO) pz[0] = pz[1];
-            if (np & ZHI) pz[2] = pz[1];
-            R P[27];
-#pragma unroll
-            for (int i = 0; i < 19; i++)
-                P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
-                    ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+            if constexpr (!SO) load_phase_stencil<R>(a, c, P);
             R upf = 0;
             if (a.pf_coupling == 1) upf = T;
             if (a.pf_coupling == 2) upf = U;

@@ -23,7 +23,9 @@
  *     anisotropic face-flux divergence and Q = (phi - lambda u (1-phi^2))(1-phi^2)
  *     (MaterKit/src/mkFluidsUser.cpp:261-264)
  *   - D3Q7 advection-diffusion for T and for the dilute-alloy variable c = A(phi) U
- *     (Karma 2001, Echebarria et al. 2004), latent/rejection source w_i L dphi
+ *     (Karma 2001, Echebarria et al. 2004), latent/rejection source w_i L dphi; the solute
+ *     flux D q grad U is split into diffusion of c with D q / A and a drift velocity
+ *     V = D q grad A / A^2 carried by the equilibrium
  */
 #include "mk_oracle.h"
 
@@ -660,23 +662,43 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
         memcpy(s->h, fresh, (size_t)7 * N * sizeof(double));
     }
 
-    /* solute */
+    /* solute: dc/dt + div(c u) = div(D q grad U), U = c/A(phi), written as
+     * dc/dt + div(c (u + V)) = div(D_c grad c) with D_c = D q / A and V = D q grad A / A^2 */
     if (adv_c) {
         lat_t L = make_lat(7);
-#pragma omp parallel for schedule(static)
-        for (long x = 0; x < N; x++) {
+        lat_t L19 = make_lat(19);
+        int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int X = 0; X < p->nx; X++)
+            for (int Y = 0; Y < p->ny; Y++)
+                for (int Z = 0; Z < p->nz; Z++) {
+            long x = lin(p, X, Y, Z);
             if (s->flag[x] == 3) continue;
             double phi = p->phase_q ? s->phi[x] : -1.0;
+            double gphi[3] = {0, 0, 0};
+            if (p->phase_q) {
+                for (int i = 1; i < 19; i++) {
+                    int XX = clampwrap(X + E27[i][0], p->nx, per[0]);
+                    int YY = clampwrap(Y + E27[i][1], p->ny, per[1]);
+                    int ZZ = clampwrap(Z + E27[i][2], p->nz, per[2]);
+                    double v = 3.0 * L19.w[i] * s->phi[lin(p, XX, YY, ZZ)];
+                    for (int a = 0; a < 3; a++) gphi[a] += E27[i][a] * v;
+                }
+            }
             double q = 0.5 * (1.0 - phi);
             if (p->D_l > 0) q += (p->D_s / p->D_l) * 0.5 * (1.0 + phi);
             if (q < p->q_min) q = p->q_min;
-            double om = 1.0 / (0.5 + p->D_l * q / L.cs2);
+            double A = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
+            double Dc = p->D_l * q / A;
+            double om = 1.0 / (0.5 + Dc / L.cs2);
+            double vfac = -0.5 * (1.0 - p->k_part) * p->D_l * q / (A * A);
             double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
-            double C = m.C[x], U = m.U[x];
+            double C = m.C[x];
             const double* u = uadv + 3 * x;
+            double v[3] = {u[0] + vfac * gphi[0], u[1] + vfac * gphi[1], u[2] + vfac * gphi[2]};
             for (int i = 0; i < 7; i++) {
-                double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
-                double ceq = (i == 0) ? C - (1.0 - L.w[0]) * U : L.w[i] * U * (1.0 + cu / L.cs2);
+                double cv = E27[i][0] * v[0] + E27[i][1] * v[1] + E27[i][2] * v[2];
+                double ceq = L.w[i] * C * (1.0 + cv / L.cs2);
                 double ci = s->c[i * N + x];
                 post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * p->latent_C * dphi;
             }
