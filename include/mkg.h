@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MKG_ABI_VERSION 1
+#define MKG_ABI_VERSION 2
 
 /* ---- node flags: identical codes to MaterKit/inc/mkGlobalDefs.h:140-145 ---- */
 #define MKG_IS_OUTLET (-2)
@@ -135,6 +135,7 @@ typedef struct mkg_params {
     double T_face[6];
     /* solute */
     double D_l, D_s, k_part, latent_C, q_min;
+    double at_coef;       /* anti-trapping coefficient a_t (Karma: 1/(2 sqrt 2)); 0 = off */
     double C_face[6];
     /* phase field (Karma-Rappel units W0 = tau0 = 1 scaled by dx, dt) */
     double pf_dt;         /* dt / tau0 */

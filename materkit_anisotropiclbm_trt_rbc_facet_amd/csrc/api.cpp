@@ -155,6 +155,7 @@ StepArgs make_args(const mkg_ctx* h) {
     a.k_part = p.k_part;
     a.latent_C = p.latent_C;
     a.q_min = p.q_min;
+    a.at_w = p.at_coef * std::sqrt(p.pf_wdx2 > 0 ? p.pf_wdx2 : 0.0);
     a.pf_dt = p.pf_dt;
     a.pf_wdx2 = p.pf_wdx2;
     a.eps4 = p.eps4;

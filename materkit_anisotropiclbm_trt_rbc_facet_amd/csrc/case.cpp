@@ -383,6 +383,9 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
     p->D_s = c.num("composition/diffSolid", 0.0) * lu;
     p->k_part = c.num("composition/partition", 1.0);
     p->latent_C = c.num("solver/rejection", 0.5);
+    // anti-trapping current of the dilute-alloy model (mkSolutalField::ATcurrent,
+    // MaterKit/inc/mkSolutalField.h); Karma's coefficient 1/(2 sqrt 2) unless the case sets one
+    p->at_coef = p->solute ? c.num("solver/antiTrapping", 0.5 / std::sqrt(2.0)) : 0.0;
     p->ref_c = p->solute ? c.num("composition/iniValue", 0.0) : 0.0;
     auto cf = c.nums("solver/faceSolute");
     if (cf.size() == 6)
@@ -412,7 +415,7 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
 }
 
 // Seeds of a case: explicit <phasechange><seed> entries or <solver><seedsRandom> count
-// radius rngseed.  Writes up to max seeds (centres 3n, radii n, rotations 9n row-major) and
+// radius rngseed [margin].  Writes up to max seeds (centres 3n, radii n, rotations 9n row-major) and
 // returns the count, or -1.
 int mkg_seeds_from_xml(const char* path, int max, double* centers, double* radii, double* rot) {
     Case c;
@@ -424,8 +427,10 @@ int mkg_seeds_from_xml(const char* path, int max, double* centers, double* radii
         if (r.size() < 3) { g_case_err = "seedsRandom needs count radius rngseed"; return -1; }
         int n = (int)r[0];
         if (n > max) { g_case_err = "too many seeds"; return -1; }
+        // optional fourth value: distance of the centres from every face (default: radius)
+        const double margin = r.size() > 3 ? r[3] : r[1];
         std::vector<double> q(4 * (size_t)n);
-        mkg_random_seeds((uint64_t)r[2], n, r[1], (int)mesh[0], (int)mesh[1], (int)mesh[2],
+        mkg_random_seeds((uint64_t)r[2], n, margin, (int)mesh[0], (int)mesh[1], (int)mesh[2],
                          centers, q.data());
         for (int k = 0; k < n; k++) {
             radii[k] = r[1];

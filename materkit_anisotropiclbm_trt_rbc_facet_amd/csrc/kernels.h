@@ -36,6 +36,7 @@ struct StepArgs {
     double u_face[6][3];
     double alpha_l, alpha_s, latent_T;
     double D_l, D_s, k_part, latent_C, q_min;
+    double at_w;  // anti-trapping a_t * W0/dx
     double pf_dt, pf_wdx2, eps4, lambda, phi_s, omega_g;
 };
 

@@ -199,14 +199,26 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
             const R Dc = (R)a.D_l * q / A;
             const R om = (R)1 / ((R)0.5 + Dc / (R)0.25);
             const R vfac = (R)(-0.5 * (1.0 - a.k_part)) * (R)a.D_l * q / (A * A);
-            const R vx = ux + vfac * gphi[0], vy = uy + vfac * gphi[1], vz = uz + vfac * gphi[2];
+            // equilibrium flux: c (u + V) plus the anti-trapping current
+            //   j_at = -a_t W (1 + (1-k) U) dphi/dt grad phi / |grad phi|
+            R jx = C * (ux + vfac * gphi[0]), jy = C * (uy + vfac * gphi[1]),
+              jz = C * (uz + vfac * gphi[2]);
+            if constexpr (PQ > 0) {
+                const R gg = gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2];
+                if (a.at_w != 0.0 && gg > (R)1e-20) {
+                    const R jat = -(R)a.at_w * ((R)1 + ((R)1 - (R)a.k_part) * U) * dphi / sqrt(gg);
+                    jx += jat * gphi[0];
+                    jy += jat * gphi[1];
+                    jz += jat * gphi[2];
+                }
+            }
             const R src = (R)a.latent_C * dphi;
 #pragma unroll
             for (int i = 0; i < 7; i++) {
                 const R w = (R)wgt<7>(i);
-                const R cv = cx(i) * vx + cy(i) * vy + cz(i) * vz;
-                // deviation form: c~eq = w (c - ref) + w c cv / cs^2
-                const R ceq = w * (sc + C * cv / (R)0.25);
+                const R cj = cx(i) * jx + cy(i) * jy + cz(i) * jz;
+                // deviation form: c~eq = w (c - ref) + w c_i.j / cs^2
+                const R ceq = w * (sc + cj / (R)0.25);
                 cc[i] = cc[i] - om * (cc[i] - ceq) + w * src;
             }
             write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
@@ -239,11 +251,13 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
             read_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
             const R om = (R)a.omega_g;
             const R dp = phi - (R)a.g.ref;
+            // advective flux (phi + 1) u: the liquid is invariant under any melt velocity
+            const R pa = (phi + (R)1) / (R)cs2<PQ>();
 #pragma unroll
             for (int i = 0; i < PQ; i++) {
                 const R w = (R)wgt<PQ>(i);
                 const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
-                const R geq = w * (dp + phi * cu / (R)cs2<PQ>());
+                const R geq = w * (dp + pa * cu);
                 g[i] = g[i] - om * (g[i] - geq) + w * Sg;
             }
             write_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);

@@ -41,7 +41,8 @@ PARAM_FIELDS = [
     ("alpha_l", C.c_double), ("alpha_s", C.c_double), ("latent_T", C.c_double),
     ("T_face", C.c_double * 6),
     ("D_l", C.c_double), ("D_s", C.c_double), ("k_part", C.c_double),
-    ("latent_C", C.c_double), ("q_min", C.c_double), ("C_face", C.c_double * 6),
+    ("latent_C", C.c_double), ("q_min", C.c_double), ("at_coef", C.c_double),
+    ("C_face", C.c_double * 6),
     ("pf_dt", C.c_double), ("pf_wdx2", C.c_double), ("eps4", C.c_double),
     ("lambda_", C.c_double), ("phi_s", C.c_double),
     ("ref_f", C.c_double), ("ref_g", C.c_double), ("ref_h", C.c_double),

@@ -21,7 +21,8 @@
  *     solved as a D3Q19/D3Q7 LB diffusion (omega_g = 1/(dt/dx^2/cs^2 + 1/2), the form at
  *     MaterKit/src/mkFluidsUser.cpp:282) plus the source w_i * S, where S carries the
  *     anisotropic face-flux divergence and Q = (phi - lambda u (1-phi^2))(1-phi^2)
- *     (MaterKit/src/mkFluidsUser.cpp:261-264)
+ *     (MaterKit/src/mkFluidsUser.cpp:261-264); the melt carries phi with the flux
+ *     (phi + 1) u, so the bulk liquid phi = -1 stays exact where div u != 0
  *   - D3Q7 advection-diffusion for T and for the dilute-alloy variable c = A(phi) U
  *     (Karma 2001, Echebarria et al. 2004), latent/rejection source w_i L dphi; the solute
  *     flux D q grad U is split into diffusion of c with D q / A and a drift velocity
@@ -628,7 +629,9 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                     const double* u = uadv + 3 * id;
                     for (int i = 0; i < q; i++) {
                         double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
-                        double geq = L.w[i] * ph * (1.0 + cu / L.cs2);
+                        /* the advective flux is (phi + 1) u: the liquid (phi = -1) is left
+                         * unchanged by a velocity field with div u != 0 */
+                        double geq = L.w[i] * (ph + (ph + 1.0) * cu / L.cs2);
                         double gi = s->g[i * N + id];
                         post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * S;
                     }
@@ -695,10 +698,21 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
             double C = m.C[x];
             const double* u = uadv + 3 * x;
-            double v[3] = {u[0] + vfac * gphi[0], u[1] + vfac * gphi[1], u[2] + vfac * gphi[2]};
+            /* equilibrium flux c (u + V) + j_at, the anti-trapping current (Karma 2001)
+             * j_at = -a_t W (1 + (1-k) U) dphi/dt n, n = grad phi / |grad phi|, which moves
+             * solute from the growing solid to the liquid (mkSolutalField::ATcurrent,
+             * MaterKit/inc/mkSolutalField.h) */
+            double j[3];
+            for (int a = 0; a < 3; a++) j[a] = C * (u[a] + vfac * gphi[a]);
+            double gg = gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2];
+            double atw = p->at_coef * sqrt(p->pf_wdx2 > 0 ? p->pf_wdx2 : 0.0);
+            if (p->phase_q && atw != 0.0 && gg > 1e-20) {
+                double jat = -atw * (1.0 + (1.0 - p->k_part) * m.U[x]) * dphi / sqrt(gg);
+                for (int a = 0; a < 3; a++) j[a] += jat * gphi[a];
+            }
             for (int i = 0; i < 7; i++) {
-                double cv = E27[i][0] * v[0] + E27[i][1] * v[1] + E27[i][2] * v[2];
-                double ceq = L.w[i] * C * (1.0 + cv / L.cs2);
+                double cj = E27[i][0] * j[0] + E27[i][1] * j[1] + E27[i][2] * j[2];
+                double ceq = L.w[i] * (C + cj / L.cs2);
                 double ci = s->c[i * N + x];
                 post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * p->latent_C * dphi;
             }

@@ -34,6 +34,7 @@ typedef struct orc_params {
     double alpha_l, alpha_s, latent_T;
     double T_face[6];
     double D_l, D_s, k_part, latent_C, q_min;
+    double at_coef;
     double C_face[6];
     double pf_dt, pf_wdx2, eps4, lambda, phi_s;
     double ref_f, ref_g, ref_h, ref_c;

@@ -42,6 +42,7 @@ def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodi
         p.pf_coupling = 1 if thermal else (2 if solute else 0)
     if solute:
         p.D_l, p.D_s, p.k_part, p.q_min = 0.1, 0.01, 0.15, 1e-3
+        p.at_coef = 0.5 / np.sqrt(2.0)
         p.ref_c = -0.5
     if ngrains:
         rng = np.random.default_rng(5)

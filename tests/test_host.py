@@ -28,8 +28,8 @@ def test_library_exports_every_declared_entry_point():
     missing = decl - exported
     assert not missing, f"declared but not exported: {sorted(missing)}"
     L = mkg.lib()
-    assert L.mkg_abi_version() == 1
-    assert L.mkg_params_size() == 5232
+    assert L.mkg_abi_version() == 2
+    assert L.mkg_params_size() == 5240
 
 
 def test_params_layout_matches_oracle():
@@ -117,8 +117,15 @@ def test_case_files_derive_expected_parameters():
     assert c3.pf_wdx2 == 6.25 and c3.eps4 == 0.05 and c3.precision == mkg.PREC_FP32
     c4 = cases.load("C4")
     assert c4.params.ngrains == 32 and len(c4.seeds[1]) == 32
+    # 32 seeds of radius 4 placed at least 24 cells from every face (seedsRandom's fourth
+    # value); the rotations are the golden quaternions (the generator draws them after the
+    # centres, so the margin does not change them)
+    c, _ = mkg.random_seeds(20240917, 32, 24.0, (256, 256, 256))
+    np.testing.assert_array_equal(c4.seeds[0], c)
+    assert c4.seeds[0].min() >= 24.0 and c4.seeds[0].max() <= 255 - 24.0
     g = json.loads((GOLD / "seeds_random.json").read_text())
-    np.testing.assert_array_equal(c4.seeds[0], np.array(g["centers"]))
+    for k in range(32):
+        np.testing.assert_array_equal(c4.seeds[2][k], mkg.quat_to_rot(np.array(g["quats"][k])).reshape(-1))
 
 
 @pytest.mark.skipif(not REF.exists(), reason="reference tree not mounted")

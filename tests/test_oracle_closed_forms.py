@@ -7,7 +7,11 @@
 - anti-bounce-back Dirichlet walls give the exact linear conduction profile;
 - the phase-field source reduces to dt*Q without anisotropy, to dt*Q/a^2 with a = 1+eps4 on an
   axis-aligned plane front, and its face flux is the gradient of the anisotropic energy;
-- the Karma-Rappel 1D equilibrium -tanh(x/sqrt2) is stationary at lambda = 0.
+- the Karma-Rappel 1D equilibrium -tanh(x/sqrt2) is stationary at lambda = 0;
+- the anti-trapping current makes U continuous across a growing planar alloy front (and the
+  solid composition k times the liquid's), against more trapping without it or with its sign
+  flipped;
+- an all-liquid domain stays at phi = -1 exactly under a compressible melt flow.
 """
 import math
 
@@ -195,3 +199,64 @@ def test_flat_interface_is_stationary_without_driving_force():
     assert abs(crossing(phi) - crossing(phi0)) < 0.05
     ref = -np.tanh((np.abs(z - nz / 2) - nz / 4) / (math.sqrt(2) * W * 1.05))
     assert np.abs(phi - ref).max() < 0.01
+
+
+def _planar_alloy_front(at_coef: float, steps: int = 20000):
+    """Solid slab (phi = +1, |z - nz/2| < 16) growing into a supersaturated melt (U0 = -0.5,
+    k = 0.15, lambda = 3.19, D = 2 W0^2/tau0): returns (U averaged over the solid formed during
+    the run, U at the front, composition ratio of that solid to the liquid ahead)."""
+    nz, W, k = 256, 2.5, 0.15
+    p = mkg.default_params(nx=1, ny=1, nz=nz, phase_q=19, solute=1, pf_coupling=2, eps4=0.0,
+                           pf_dt=0.008, pf_wdx2=W * W, D_l=0.1, D_s=0.0, k_part=k, q_min=1e-3,
+                           at_coef=at_coef, ref_c=-0.5, phi_s=0.99, **{"lambda": 3.19})
+    z = np.arange(nz) + 0.5 - nz / 2
+    phi0 = -np.tanh((np.abs(z) - 16) / (math.sqrt(2) * W))
+    A0 = 0.5 * ((1 + k) - (1 - k) * phi0)
+    s = O.State.empty(p)
+    s.init_equilibrium(phi=phi0, C=-0.5 * A0)
+    s.step(steps, 10, O.OP_BGK)
+    phi = s.g.reshape(19, nz).sum(0)
+    C = s.moments()["C"]
+    A = 0.5 * ((1 + k) - (1 - k) * phi)
+    U = C / A
+    i = nz // 2 + np.where((phi[nz // 2:-1] > 0) & (phi[nz // 2 + 1:] <= 0))[0][0]
+    front = z[i] + phi[i] / (phi[i] - phi[i + 1])
+    band = (z > 16 + 3 * W) & (z < front - 3 * W)
+    assert band.sum() >= 15  # the front advanced by more than 6 W
+    comp = C + A / (1 - k)  # proportional to the solute concentration
+    ratio = comp[band].mean() / np.interp(front + 3 * W, z, comp)
+    return U[band].mean(), np.interp(front, z, U), ratio
+
+
+def test_anti_trapping_current_removes_solute_trapping():
+    """Karma's anti-trapping current (coefficient 1/(2 sqrt 2)) makes the solid grow with the
+    interface value of U (U continuous across a moving front, solid composition about k times
+    the liquid's); without it the fast front traps solute, and with the opposite sign it traps
+    more."""
+    at = 0.5 / math.sqrt(2.0)
+    u_at, u_front_at, r_at = _planar_alloy_front(at)
+    u_none, u_front_none, r_none = _planar_alloy_front(0.0)
+    u_neg, _, r_neg = _planar_alloy_front(-at)
+    assert abs(u_at - u_front_at) < 0.05
+    assert u_none - u_front_none > 0.1
+    assert u_at < u_none < u_neg
+    assert abs(r_at - 0.15) < 0.02 and r_none > r_at + 0.01 and r_neg > r_none + 0.01
+
+
+def test_melt_flow_leaves_the_liquid_phase_value_exact():
+    """The melt carries phi with the flux (phi + 1) u: an all-liquid domain (phi = -1) stays
+    exactly liquid under a compressible velocity transient (div u != 0), which a flux phi u
+    would not."""
+    n = 12
+    p = mkg.default_params(nx=n, ny=n, nz=n, flow_q=19, phase_q=19, s_nu=1.0 / 0.8,
+                           s_bulk=1.0 / 0.8, s_odd=1.0 / 0.8, s_even=1.0 / 0.8, eps4=0.05,
+                           pf_dt=0.008, pf_wdx2=6.25, phi_s=0.99, **{"lambda": 3.19})
+    rng = np.random.default_rng(3)
+    rho = 1.0 + 0.02 * rng.standard_normal(n ** 3)
+    vel = 0.03 * rng.standard_normal((n ** 3, 3))
+    s = O.State.empty(p)
+    s.init_equilibrium(rho=rho, vel=vel, phi=np.full(n ** 3, -1.0))
+    s.step(40, 3, O.OP_BGK)
+    assert np.abs(s.moments()["vel"]).max() > 1e-3  # the melt is still moving
+    phi = s.g.reshape(19, -1).sum(0)
+    assert np.abs(phi + 1.0).max() < 1e-13
