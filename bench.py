@@ -44,7 +44,21 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run three plain 1 GiB device copies (the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def run_device(prec, size, steps, warmup):
@@ -147,6 +161,8 @@ def main():
 
     size = a.size
     N = size ** 3
+    if a.copy_calibration:
+        copy_calibration()
     p32, wall32, tim32, nf32, solid32 = run_device(mkg.PREC_FP32, size, a.steps, a.warmup)
     if a.no_fp64:
         p64, wall64, tim64, nf64 = None, None, None, 0.0

@@ -47,6 +47,7 @@ struct mkg_ctx {
     int fp32 = 0;
     size_t esz = 8;
     uint32_t N = 0, S = 0, nblk = 0;
+    uint32_t F = 0;  // cells on the six faces (stride of the outflow face buffer)
     int dev = 0;
     hipStream_t stream = nullptr;
     void* pop[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -66,6 +67,8 @@ struct mkg_ctx {
     int timing = 0;
     std::vector<hipEvent_t> events;
     double t_last[4] = {0, 0, 0, 0};
+    // elements of one lattice's allocation: Q slots of S plus Q face-buffer slots of F
+    size_t pop_elems(int L) const { return (size_t)q[L] * ((size_t)S + F); }
 };
 
 namespace {
@@ -107,6 +110,10 @@ void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const doub
         out.fval[f] = fval ? fval[f] : 0.0;
     }
     out.npmask = np;
+    int om = 0;
+    for (int f = 0; f < 6; f++)
+        if (bc[f] == MKG_BC_OUTFLOW) om |= 1 << f;
+    out.outmask = om;
     out.lay = h->lay[L];
     out.adv = 0;
     out.ref = ref;
@@ -122,6 +129,7 @@ StepArgs make_args(const mkg_ctx* h) {
     a.N = h->N;
     a.S = h->S;
     a.nblk = h->nblk;
+    a.F = h->F;
     a.has_walls = h->has_walls;
     a.has_kappa = (p.has_kappa && h->kappa) ? 1 : 0;
     a.ngrains = p.ngrains;
@@ -342,6 +350,7 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
         h->S = ((h->N + 127u) & ~127u) + ((pad + 127u) & ~127u);
     }
     h->nblk = (h->N + 255u) / 256u;
+    h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
     h->q[LF] = p->flow_q;
     h->q[LG] = p->phase_q;
     h->q[LH] = p->thermal ? 7 : 0;
@@ -353,7 +362,7 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
     if (hipGetDevice(&h->dev) != hipSuccess) return cleanup(fail("hipGetDevice failed"));
     if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail("hipStreamCreate failed"));
     for (int L = 0; L < 4; L++)
-        if (alloc_zero(&h->pop[L], (size_t)h->q[L] * h->S * h->esz, h->stream)) return cleanup(-1);
+        if (alloc_zero(&h->pop[L], h->pop_elems(L) * h->esz, h->stream)) return cleanup(-1);
     if (p->phase_q) {
         for (int b = 0; b < 2; b++)
             if (alloc_zero(&h->phi[b], (size_t)h->S * h->esz, h->stream)) return cleanup(-1);
@@ -420,6 +429,10 @@ int mkg_set_params(mkg_handle h, const mkg_params* p) {
         return fail("set_params cannot change grid, lattices, precision or kappa storage");
     if (p->ref_f != o.ref_f || p->ref_g != o.ref_g || p->ref_h != o.ref_h || p->ref_c != o.ref_c)
         return fail("set_params cannot change deviation references of live populations");
+    // populations still on their way through a face were settled with the old face rules:
+    // bring every lattice to the own-slot layout before the new parameters apply
+    for (int L = 0; L < 4; L++)
+        if (normalize(h, L)) return -1;
     h->p = *p;
     HIPCHK(hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -726,7 +739,7 @@ int mkg_checkpoint_save(mkg_handle h, const char* path) {
         if (hipMemcpy(buf.data(), dptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return false;
         return wr(buf.data(), bytes);
     };
-    for (int L = 0; L < 4 && ok; L++) ok = dump(h->pop[L], (size_t)h->q[L] * h->S * h->esz);
+    for (int L = 0; L < 4 && ok; L++) ok = dump(h->pop[L], h->pop_elems(L) * h->esz);
     if (ok && h->q[LG]) ok = dump(h->phi[0], h->S * h->esz) && dump(h->phi[1], h->S * h->esz);
     if (ok) ok = dump(h->flag, h->S) && dump(h->grain, (size_t)h->S * 2);
     if (ok && h->kappa) ok = dump(h->kappa, h->S * h->esz);
@@ -765,7 +778,7 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
         if (!rd(buf.data(), bytes)) return false;
         return hipMemcpy(dptr, buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    for (int L = 0; L < 4 && ok; L++) ok = load(h->pop[L], (size_t)h->q[L] * h->S * h->esz);
+    for (int L = 0; L < 4 && ok; L++) ok = load(h->pop[L], h->pop_elems(L) * h->esz);
     if (ok && h->q[LG]) ok = load(h->phi[0], h->S * h->esz) && load(h->phi[1], h->S * h->esz);
     if (ok) ok = load(h->flag, h->S) && load(h->grain, (size_t)h->S * 2);
     if (ok && h->kappa) ok = load(h->kappa, h->S * h->esz);
@@ -798,8 +811,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         int64_t pf;
     };
     std::vector<Ck> ck(nseg + 1);
-    const size_t ck_elems = (19 + 7 + 2) * S;
-    const size_t seg_elems = (size_t)ckpt * 8 * S;
+    const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH);
+    const size_t ck_elems = PG + PH + 2 * S;
+    const size_t seg_stride = S + PH;  // phi^n and the h buffer of one step
+    const size_t seg_elems = (size_t)ckpt * seg_stride;
     char *ckbuf = nullptr, *segbuf = nullptr, *work = nullptr;
     double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr;
     int rc = 0;
@@ -846,19 +861,19 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
 
     auto save_ck = [&](int k) -> int {
         char* base = ckbuf + (size_t)k * ck_elems * e;
-        HIPCHK(hipMemcpyAsync(base, h->pop[LG], 19 * S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + 19 * S * e, h->pop[LH], 7 * S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + 26 * S * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + 27 * S * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base, h->pop[LG], PG * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + PG * e, h->pop[LH], PH * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + (PG + PH) * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + (PG + PH + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
         ck[k] = {h->lay[LG], h->lay[LH], h->pf_count};
         return 0;
     };
     auto load_ck = [&](int k) -> int {
         char* base = ckbuf + (size_t)k * ck_elems * e;
-        HIPCHK(hipMemcpyAsync(h->pop[LG], base, 19 * S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->pop[LH], base + 19 * S * e, 7 * S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->phi[0], base + 26 * S * e, S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->phi[1], base + 27 * S * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->pop[LG], base, PG * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->pop[LH], base + PG * e, PH * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[0], base + (PG + PH) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[1], base + (PG + PH + S) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
         h->lay[LG] = ck[k].lay_g;
         h->lay[LH] = ck[k].lay_h;
         h->pf_count = ck[k].pf;
@@ -895,12 +910,12 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         hipEventRecord(ea, h->stream);
         if (load_ck(sgi)) { rc = -1; break; }
         for (int n = n0; n < n1; n++) {
-            char* slot = segbuf + (size_t)(n - n0) * 8 * S * e;
+            char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
             seg_lay[n - n0] = h->lay[LH];
             auto mid = [&](mkg_ctx* hh) -> int {
                 HIPCHK(hipMemcpyAsync(slot, hh->phi[hh->pf_count & 1], S * e,
                                       hipMemcpyDeviceToDevice, hh->stream));
-                HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], 7 * S * e,
+                HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], PH * e,
                                       hipMemcpyDeviceToDevice, hh->stream));
                 return 0;
             };
@@ -908,7 +923,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         }
         hipEventRecord(eb, h->stream);
         for (int n = n1 - 1; n >= n0 && rc == 0; n--) {
-            char* slot = segbuf + (size_t)(n - n0) * 8 * S * e;
+            char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
             StepArgs a = make_args(h);
             AdjArgs j;
             std::memset(&j, 0, sizeof(j));

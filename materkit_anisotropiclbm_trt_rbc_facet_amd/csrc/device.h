@@ -130,6 +130,55 @@ __device__ inline void stp(R* p, R v) {
 #endif
 }
 
+// Position of a face cell in the face buffer: faces x-, x+ (ny*nz cells each, index
+// y*nz + z), then y-, y+ (x*nz + z), then z-, z+ (x*ny + y).
+__device__ inline uint32_t face_slot(const StepArgs& a, const Cell& c, int f) {
+    const uint32_t ny = (uint32_t)a.ny, nz = (uint32_t)a.nz, nx = (uint32_t)a.nx;
+    const uint32_t A = ny * nz, B = nx * nz, C = nx * ny;
+    const uint32_t side = (uint32_t)f & 1u;
+    if (f < 2) return side * A + (uint32_t)c.y * nz + (uint32_t)c.z;
+    if (f < 4) return 2u * A + side * B + (uint32_t)c.x * nz + (uint32_t)c.z;
+    return 2u * (A + B) + side * C + (uint32_t)c.x * ny + (uint32_t)c.y;
+}
+
+// Cell state that enters the face rules: the liquid fraction 1 - n_s scales the inflow
+// (moving-wall) term of the flow, so a solid cell on an inlet receives no mass; the Dirichlet
+// value of a scalar is multiplied by dscale (A(phi) for the solute, whose faces fix U).
+template <class R>
+struct FaceState {
+    R wscale = (R)1;
+    R dscale = (R)1;
+};
+
+// Population entering the cell in direction i through face f, from the cell's post-collision
+// populations p (the pull rule of mkFluidsUser::propagate, MaterKit/src/mkFluidsUser.cpp:77-97,
+// with the face codes of the case):
+//   outflow    zero gradient: the cell's own post-collision population i
+//   Dirichlet  anti-bounce-back to the face value (scalar lattices)
+//   bounce     half-way bounce-back, plus 6 w rho0 c.u_w for a moving wall or inlet (flow)
+template <int Q, class R>
+__device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const StepArgs& a,
+                            int kind, const FaceState<R>& fs) {
+    const int t = pick6(f, L.bc);
+    if (t == kBcOutflow) return p[i];
+    if (t == kBcDirichlet && kind == kKindScalar)
+        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)pick6(f, L.fval) - (R)L.ref);
+    R v = p[inv(i)];
+    if (kind == kKindFlow) {
+        const double cu = face_cu(f, cx(i), cy(i), cz(i), a.u_face);
+        v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu) * fs.wscale;
+    }
+    return v;
+}
+
+// AA access.  E step (ODD false): read own slot i, write own slot inv(i).  O step: read slot
+// inv(i) of x - c_i, write slot i of x + c_i.  A population that crosses a non-periodic face
+// is settled by face_in with the state of the step that collided it:
+//   - crossing out in an O step, it returns at once into the cell's own slot inv(i) (no other
+//     thread writes that slot);
+//   - crossing out in an E step, the E step stores the returning population in the face
+//     buffer, which the face cell alone reads in the next O step.  Its own slots cannot hold
+//     it: the interior neighbours overwrite them during the O step.
 template <int Q, bool ODD, class R>
 __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                  const StepArgs& a, int kind) {
@@ -138,6 +187,7 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
 #pragma unroll
         for (int i = 0; i < Q; i++) p[i] = ldp(&base[(size_t)i * S + c.id]);
     } else {
+        (void)kind;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
@@ -149,21 +199,8 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
                 else
                     p[i] = ldp(&base[(size_t)inv(i) * S + src]);
             } else {
-                int face = __builtin_ctz(hit);
-                int t = pick6(face, L.bc);
-                if (t == kBcOutflow) {
-                    p[i] = ldp(&base[(size_t)inv(i) * S + c.id]);
-                } else if (t == kBcDirichlet && kind == kKindScalar) {
-                    p[i] = -ldp(&base[(size_t)i * S + c.id]) +
-                           (R)(2.0 * wgt<Q>(i) * (pick6(face, L.fval) - L.ref));
-                } else {
-                    R v = ldp(&base[(size_t)i * S + c.id]);
-                    if (kind == kKindFlow) {
-                        double cu = face_cu(face, cx(i), cy(i), cz(i), a.u_face);
-                        v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
-                    }
-                    p[i] = v;
-                }
+                const int face = __builtin_ctz(hit);
+                p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, face)]);
             }
         }
     }
@@ -171,11 +208,24 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
 
 template <int Q, bool ODD, class R>
 __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
-                                  const StepArgs& a, int kind) {
+                                  const StepArgs& a, int kind,
+                                  const FaceState<R>& fs = FaceState<R>()) {
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
         for (int i = 0; i < Q; i++) stp(&base[(size_t)inv(i) * S + c.id], p[i]);
+        if (c.bits & (uint32_t)L.npmask) {
+#pragma unroll
+            for (int i = 1; i < Q; i++) {
+                const uint32_t hit =
+                    c.bits & faces_of(-cx(i), -cy(i), -cz(i)) & (uint32_t)L.npmask;
+                if (hit) {
+                    const int f = __builtin_ctz(hit);
+                    stp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, f)],
+                        face_in<Q, R>(p, i, f, L, a, kind, fs));
+                }
+            }
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < Q; i++) {
@@ -188,21 +238,9 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
                 else
                     stp(&base[(size_t)i * S + tgt], p[i]);
             } else {
-                int face = __builtin_ctz(hit);
-                int t = pick6(face, L.bc);
-                R v;
-                if (t == kBcOutflow) {
-                    v = p[inv(i)];
-                } else if (t == kBcDirichlet && kind == kKindScalar) {
-                    v = -p[i] + (R)(2.0 * wgt<Q>(i) * (pick6(face, L.fval) - L.ref));
-                } else {
-                    v = p[i];
-                    if (kind == kKindFlow) {
-                        double cu = face_cu(face, cx(i), cy(i), cz(i), a.u_face);
-                        v -= (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
-                    }
-                }
-                stp(&base[(size_t)inv(i) * S + c.id], v);
+                const int face = __builtin_ctz(hit);
+                stp(&base[(size_t)inv(i) * S + c.id],
+                    face_in<Q, R>(p, inv(i), face, L, a, kind, fs));
             }
         }
     }

@@ -8,12 +8,13 @@ namespace mkg {
 
 // One population lattice as seen by a kernel.
 struct LatArgs {
-    void* pop;        // Q * S elements of the storage type, slot i at pop + i*S
+    void* pop;        // Q * (S + F) elements of the storage type: slot i at pop + i*S, then
+                      // the outflow face buffer, slot i at pop + Q*S + i*F (see face_slot)
     int32_t bc[6];    // face codes x-, x+, y-, y+, z-, z+
     int32_t npmask;   // bit f set when face f is not periodic
     int32_t lay;      // current AA layout: 0 = own slots hold f_i, 1 = shifted
     int32_t adv;      // advance this lattice in this launch
-    int32_t pad;
+    int32_t outmask;  // bit f set when face f is a zero-gradient outflow
     double ref;       // deviation reference: stored value = f_i - w_i * ref
     double fval[6];   // Dirichlet face values (scalar lattices)
 };
@@ -23,6 +24,7 @@ struct StepArgs {
     uint32_t N;       // number of cells
     uint32_t S;       // slot stride (N rounded up)
     uint32_t nblk;    // blocks in the launch (XCD-aware remap when divisible by 8)
+    uint32_t F;       // face-buffer stride: cells on the six faces, 2 (ny nz + nx nz + nx ny)
     int32_t has_walls, has_kappa, ngrains, pf_coupling;
     LatArgs f, g, h, c;
     const void* phi_cur;   // storage type, phase field at time level n

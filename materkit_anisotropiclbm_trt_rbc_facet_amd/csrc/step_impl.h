@@ -156,7 +156,9 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
         if (a.f.adv) {
             flow_collide<FQ, R>(f, dr, rho, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
                                 (R)a.s_odd, (R)a.s_even, ns);
-            write_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow);
+            FaceState<R> fs;
+            fs.wscale = (R)1 - ns;
+            write_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow, fs);
         }
         const R om = (R)1 - ns;
         ux *= om;
@@ -221,7 +223,9 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
                 const R ceq = w * (sc + cj / (R)0.25);
                 cc[i] = cc[i] - om * (cc[i] - ceq) + w * src;
             }
-            write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
+            FaceState<R> fs;
+            fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
+            write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar, fs);
         }
     }
 

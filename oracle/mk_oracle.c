@@ -489,9 +489,13 @@ void orc_moments(const orc_params* p, const orc_state* s, double* rho, double* v
 
 /* ------------------------------------------------------------------ streaming */
 /* post: post-collision populations of the lattice; out: new pre-collision populations.
- * kind 0 flow (bounce with moving-wall term), 1 scalar (noflux / Dirichlet), value per face. */
+ * kind 0 flow (bounce with moving-wall term), 1 scalar (noflux / Dirichlet), value per face.
+ * wscale (flow, may be NULL = 1): per-cell factor of the moving-wall term, the liquid fraction
+ * 1 - n_s, so that a solid cell on an inlet receives no mass.  dscale (scalar, may be NULL =
+ * 1): per-cell factor of the Dirichlet value (A(phi) for the solute, whose faces fix U). */
 static void stream(const orc_params* p, const orc_state* s, int q, const int* bc, int kind,
-                   const double* fvals, const double* post, const double* old, double* out) {
+                   const double* fvals, const double* wscale, const double* dscale,
+                   const double* post, const double* old, double* out) {
     long N = NCELL(p);
     lat_t L = make_lat(q);
 #pragma omp parallel for collapse(2) schedule(static)
@@ -513,13 +517,14 @@ static void stream(const orc_params* p, const orc_state* s, int q, const int* bc
                         if (t == 3) {
                             v = post[i * N + id];
                         } else if (t == 2 && kind == 1) {
-                            v = -post[k * N + id] + 2.0 * L.w[i] * fvals[face];
+                            double d = dscale ? dscale[id] : 1.0;
+                            v = -post[k * N + id] + 2.0 * L.w[i] * d * fvals[face];
                         } else {
                             v = post[k * N + id];
                             if (kind == 0) {
                                 const double* uw = p->u_face[face];
                                 double cu = E27[i][0] * uw[0] + E27[i][1] * uw[1] + E27[i][2] * uw[2];
-                                v += 6.0 * L.w[i] * p->rho0 * cu;
+                                v += 6.0 * L.w[i] * p->rho0 * cu * (wscale ? wscale[id] : 1.0);
                             }
                         }
                     } else {
@@ -590,7 +595,10 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             orc_collide_cell(p, op, fin, F, fo, m.ns[x]);
             for (int i = 0; i < q; i++) post[i * N + x] = fo[i];
         }
-        stream(p, s, q, p->bc_f, 0, NULL, post, s->f, fresh);
+        double* lf = (double*)malloc((size_t)N * sizeof(double));
+        for (long x = 0; x < N; x++) lf[x] = 1.0 - m.ns[x];
+        stream(p, s, q, p->bc_f, 0, NULL, lf, NULL, post, s->f, fresh);
+        free(lf);
         memcpy(s->f, fresh, (size_t)q * N * sizeof(double));
     }
 
@@ -636,7 +644,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                         post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * S;
                     }
                 }
-        stream(p, s, q, p->bc_g, 1, NULL, post, s->g, fresh);
+        stream(p, s, q, p->bc_g, 1, NULL, NULL, NULL, post, s->g, fresh);
         memcpy(s->g, fresh, (size_t)q * N * sizeof(double));
     }
 
@@ -661,7 +669,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                 post[i * N + x] = hi - om * (hi - heq) + L.w[i] * p->latent_T * dphi;
             }
         }
-        stream(p, s, 7, p->bc_h, 1, p->T_face, post, s->h, fresh);
+        stream(p, s, 7, p->bc_h, 1, p->T_face, NULL, NULL, post, s->h, fresh);
         memcpy(s->h, fresh, (size_t)7 * N * sizeof(double));
     }
 
@@ -717,7 +725,14 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                 post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * p->latent_C * dphi;
             }
         }
-        stream(p, s, 7, p->bc_c, 1, p->C_face, post, s->c, fresh);
+        /* solute Dirichlet faces fix U: c_w = A(phi) U_w */
+        double* Ac = (double*)malloc((size_t)N * sizeof(double));
+        for (long x = 0; x < N; x++) {
+            double ph = p->phase_q ? s->phi[x] : -1.0;
+            Ac[x] = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * ph);
+        }
+        stream(p, s, 7, p->bc_c, 1, p->C_face, NULL, Ac, post, s->c, fresh);
+        free(Ac);
         memcpy(s->c, fresh, (size_t)7 * N * sizeof(double));
     }
 
