@@ -65,6 +65,8 @@ extern "C" {
                               scalar: zero flux */
 #define MKG_BC_DIRICHLET 2 /* scalar only: anti-bounce-back to value s_face */
 #define MKG_BC_OUTFLOW 3   /* unknown incoming populations copied from the cell itself */
+#define MKG_BC_PRESSURE 4  /* flow only: anti-bounce-back to the equilibrium at density rho0 and
+                              the cell's velocity (pressure outlet) */
 
 /* ---- lattice masks for mkg_step ---- */
 #define MKG_FLOW 1

@@ -110,10 +110,6 @@ void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const doub
         out.fval[f] = fval ? fval[f] : 0.0;
     }
     out.npmask = np;
-    int om = 0;
-    for (int f = 0; f < 6; f++)
-        if (bc[f] == MKG_BC_OUTFLOW) om |= 1 << f;
-    out.outmask = om;
     out.lay = h->lay[L];
     out.adv = 0;
     out.ref = ref;
@@ -213,7 +209,10 @@ int validate(const mkg_params* p) {
     const int32_t* bcs[4] = {p->bc_f, p->bc_g, p->bc_h, p->bc_c};
     for (int L = 0; L < 4; L++)
         for (int f = 0; f < 6; f++)
-            if (bcs[L][f] < 0 || bcs[L][f] > 3) return fail("face boundary code out of range");
+            if (bcs[L][f] < 0 || bcs[L][f] > 4) return fail("face boundary code out of range");
+    for (int L = 1; L < 4; L++)
+        for (int f = 0; f < 6; f++)
+            if (bcs[L][f] == MKG_BC_PRESSURE) return fail("pressure faces are for the flow lattice");
     for (int a = 0; a < 3; a++) {
         // a face pair must be periodic on both sides or on neither
         for (int L = 0; L < 4; L++)

@@ -171,6 +171,7 @@ int bc_code(const std::string& w) {
     if (w == "bounce" || w == "wall" || w == "noflux" || w == "velocity") return MKG_BC_BOUNCE;
     if (w == "dirichlet") return MKG_BC_DIRICHLET;
     if (w == "outflow") return MKG_BC_OUTFLOW;
+    if (w == "pressure") return MKG_BC_PRESSURE;
     return -1;
 }
 

@@ -148,6 +148,7 @@ template <class R>
 struct FaceState {
     R wscale = (R)1;
     R dscale = (R)1;
+    R ux = 0, uy = 0, uz = 0;  // the cell's fluid velocity (pressure faces)
 };
 
 // Population entering the cell in direction i through face f, from the cell's post-collision
@@ -156,11 +157,19 @@ struct FaceState {
 //   outflow    zero gradient: the cell's own post-collision population i
 //   Dirichlet  anti-bounce-back to the face value (scalar lattices)
 //   bounce     half-way bounce-back, plus 6 w rho0 c.u_w for a moving wall or inlet (flow)
+//   pressure   anti-bounce-back to the second-order equilibrium at rho0 and the cell's
+//              velocity u: 2 w rho0 (1 + 4.5 (c.u)^2 - 1.5 u.u) - f*_inv(i) (flow outlet)
 template <int Q, class R>
 __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const StepArgs& a,
                             int kind, const FaceState<R>& fs) {
     const int t = pick6(f, L.bc);
     if (t == kBcOutflow) return p[i];
+    if (t == kBcPressure) {
+        const R cu = cx(i) * fs.ux + cy(i) * fs.uy + cz(i) * fs.uz;
+        const R uu = fs.ux * fs.ux + fs.uy * fs.uy + fs.uz * fs.uz;
+        const R e = (R)a.rho0 * ((R)1 + (R)4.5 * cu * cu - (R)1.5 * uu);
+        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (e - (R)L.ref);
+    }
     if (t == kBcDirichlet && kind == kKindScalar)
         return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)pick6(f, L.fval) - (R)L.ref);
     R v = p[inv(i)];

@@ -9,12 +9,12 @@ namespace mkg {
 // One population lattice as seen by a kernel.
 struct LatArgs {
     void* pop;        // Q * (S + F) elements of the storage type: slot i at pop + i*S, then
-                      // the outflow face buffer, slot i at pop + Q*S + i*F (see face_slot)
+                      // the face buffer, slot i at pop + Q*S + i*F (see face_slot)
     int32_t bc[6];    // face codes x-, x+, y-, y+, z-, z+
     int32_t npmask;   // bit f set when face f is not periodic
     int32_t lay;      // current AA layout: 0 = own slots hold f_i, 1 = shifted
     int32_t adv;      // advance this lattice in this launch
-    int32_t outmask;  // bit f set when face f is a zero-gradient outflow
+    int32_t pad;
     double ref;       // deviation reference: stored value = f_i - w_i * ref
     double fval[6];   // Dirichlet face values (scalar lattices)
 };

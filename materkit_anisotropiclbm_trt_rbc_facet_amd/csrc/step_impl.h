@@ -158,6 +158,9 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
                                 (R)a.s_odd, (R)a.s_even, ns);
             FaceState<R> fs;
             fs.wscale = (R)1 - ns;
+            fs.ux = ux;
+            fs.uy = uy;
+            fs.uz = uz;
             write_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow, fs);
         }
         const R om = (R)1 - ns;

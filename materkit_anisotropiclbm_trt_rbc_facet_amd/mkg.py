@@ -19,7 +19,7 @@ MAX_GRAINS = 64
 IS_OUTLET, IS_INLET, IS_FLUID, IS_INTER, IS_SOLID, IS_WALLS = -2, -1, 0, 1, 2, 3
 OP_BGK, OP_TRT, OP_MRT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
-BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW = 0, 1, 2, 3
+BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW, BC_PRESSURE = 0, 1, 2, 3, 4
 FLOW, PHASE, THERMAL, SOLUTE = 1, 2, 4, 8
 ALL = FLOW | PHASE | THERMAL | SOLUTE
 F_POP_FLOW, F_POP_PHASE, F_POP_THERMAL, F_POP_SOLUTE = 0, 1, 2, 3

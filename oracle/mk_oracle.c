@@ -492,10 +492,11 @@ void orc_moments(const orc_params* p, const orc_state* s, double* rho, double* v
  * kind 0 flow (bounce with moving-wall term), 1 scalar (noflux / Dirichlet), value per face.
  * wscale (flow, may be NULL = 1): per-cell factor of the moving-wall term, the liquid fraction
  * 1 - n_s, so that a solid cell on an inlet receives no mass.  dscale (scalar, may be NULL =
- * 1): per-cell factor of the Dirichlet value (A(phi) for the solute, whose faces fix U). */
+ * 1): per-cell factor of the Dirichlet value (A(phi) for the solute, whose faces fix U).
+ * vel (flow): the cells' fluid velocities, for pressure faces (code 4). */
 static void stream(const orc_params* p, const orc_state* s, int q, const int* bc, int kind,
                    const double* fvals, const double* wscale, const double* dscale,
-                   const double* post, const double* old, double* out) {
+                   const double* vel, const double* post, const double* old, double* out) {
     long N = NCELL(p);
     lat_t L = make_lat(q);
 #pragma omp parallel for collapse(2) schedule(static)
@@ -516,6 +517,14 @@ static void stream(const orc_params* p, const orc_state* s, int q, const int* bc
                         int t = bc[face];
                         if (t == 3) {
                             v = post[i * N + id];
+                        } else if (t == 4 && kind == 0) {
+                            /* pressure outlet: anti-bounce-back to the equilibrium at rho0 and
+                             * the cell's velocity */
+                            const double* u = vel + 3 * id;
+                            double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                            double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                            v = -post[k * N + id] +
+                                2.0 * L.w[i] * p->rho0 * (1.0 + 4.5 * cu * cu - 1.5 * uu);
                         } else if (t == 2 && kind == 1) {
                             double d = dscale ? dscale[id] : 1.0;
                             v = -post[k * N + id] + 2.0 * L.w[i] * d * fvals[face];
@@ -597,7 +606,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
         }
         double* lf = (double*)malloc((size_t)N * sizeof(double));
         for (long x = 0; x < N; x++) lf[x] = 1.0 - m.ns[x];
-        stream(p, s, q, p->bc_f, 0, NULL, lf, NULL, post, s->f, fresh);
+        stream(p, s, q, p->bc_f, 0, NULL, lf, NULL, m.u, post, s->f, fresh);
         free(lf);
         memcpy(s->f, fresh, (size_t)q * N * sizeof(double));
     }
@@ -644,7 +653,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                         post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * S;
                     }
                 }
-        stream(p, s, q, p->bc_g, 1, NULL, NULL, NULL, post, s->g, fresh);
+        stream(p, s, q, p->bc_g, 1, NULL, NULL, NULL, NULL, post, s->g, fresh);
         memcpy(s->g, fresh, (size_t)q * N * sizeof(double));
     }
 
@@ -669,7 +678,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                 post[i * N + x] = hi - om * (hi - heq) + L.w[i] * p->latent_T * dphi;
             }
         }
-        stream(p, s, 7, p->bc_h, 1, p->T_face, NULL, NULL, post, s->h, fresh);
+        stream(p, s, 7, p->bc_h, 1, p->T_face, NULL, NULL, NULL, post, s->h, fresh);
         memcpy(s->h, fresh, (size_t)7 * N * sizeof(double));
     }
 
@@ -731,7 +740,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             double ph = p->phase_q ? s->phi[x] : -1.0;
             Ac[x] = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * ph);
         }
-        stream(p, s, 7, p->bc_c, 1, p->C_face, NULL, Ac, post, s->c, fresh);
+        stream(p, s, 7, p->bc_c, 1, p->C_face, NULL, Ac, NULL, post, s->c, fresh);
         free(Ac);
         memcpy(s->c, fresh, (size_t)7 * N * sizeof(double));
     }

@@ -13,7 +13,8 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-P, BB, DI, OUT = mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_OUTFLOW
+P, BB, DI, OUT, PR = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_OUTFLOW,
+                      mkg.BC_PRESSURE)
 
 
 def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
@@ -52,7 +53,7 @@ def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodi
             for k in range(9):
                 p.rot[g][k] = R.reshape(-1)[k]
     if bcs == "mixed":
-        p.set(bc_f=[BB, OUT, P, P, BB, BB], bc_g=[OUT, BB, P, P, BB, OUT],
+        p.set(bc_f=[BB, PR, P, P, OUT, BB], bc_g=[OUT, BB, P, P, BB, OUT],
               bc_h=[DI, OUT, P, P, DI, BB], bc_c=[DI, OUT, P, P, BB, BB],
               T_face=[0.3, 0, 0, 0, -0.2, 0], C_face=[-0.4, 0, 0, 0, 0, 0])
         p.u_face[0][0] = 0.02

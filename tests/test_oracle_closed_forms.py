@@ -11,7 +11,8 @@
 - the anti-trapping current makes U continuous across a growing planar alloy front (and the
   solid composition k times the liquid's), against more trapping without it or with its sign
   flipped;
-- an all-liquid domain stays at phi = -1 exactly under a compressible melt flow.
+- an all-liquid domain stays at phi = -1 exactly under a compressible melt flow;
+- a velocity inlet and a pressure outlet carry the uniform stream at the reference density.
 """
 import math
 
@@ -260,3 +261,27 @@ def test_melt_flow_leaves_the_liquid_phase_value_exact():
     assert np.abs(s.moments()["vel"]).max() > 1e-3  # the melt is still moving
     phi = s.g.reshape(19, -1).sum(0)
     assert np.abs(phi + 1.0).max() < 1e-13
+
+
+def test_velocity_inlet_and_pressure_outlet_carry_a_uniform_stream():
+    """Inlet (bounce-back with the wall velocity u_in) at x-, pressure outlet at x+, periodic
+    y and z: the start-up transient decays to the uniform stream u = u_in, with the outlet cells
+    at the reference density (anti-bounce-back to the equilibrium at rho0).  A zero-gradient
+    outlet settles instead at the density 1 + u_in that the inlet's flux fixes."""
+    nx, u_in = 48, 0.01
+    p = mkg.default_params(nx=nx, ny=2, nz=2, flow_q=19, s_nu=1.0 / 0.8, s_bulk=1.0 / 0.8,
+                           s_odd=1.0 / 0.8, s_even=1.0 / 0.8)
+    p.set(bc_f=[mkg.BC_BOUNCE, mkg.BC_PRESSURE, 0, 0, 0, 0])
+    p.u_face[0][0] = u_in
+    s = O.State.empty(p)
+    s.init_equilibrium()
+    s.step(30000, 1, O.OP_BGK)
+    m = s.moments()
+    rho = m["rho"].reshape(nx, 4)
+    u = m["vel"].reshape(nx, 4, 3)
+    assert np.abs(u[:, :, 0] / u_in - 1).max() < 1e-4
+    assert np.abs(u[:, :, 1:]).max() < 1e-12
+    assert abs(rho[-1].mean() - 1.0) < 1e-6
+    # mass flux rho u_x uniform along the channel and equal to the inflow rho0 u_in
+    flux = (rho * u[:, :, 0]).mean(1)
+    assert np.abs(flux / u_in - 1).max() < 1e-4
