@@ -47,7 +47,8 @@ struct mkg_ctx {
     int fp32 = 0;
     size_t esz = 8;
     uint32_t N = 0, S = 0, nblk = 0;
-    uint32_t F = 0;  // cells on the six faces (stride of the outflow face buffer)
+    uint32_t F = 0;  // cells on the six faces (stride of the face buffer)
+    int zstage = 0;  // O steps take the z shift through LDS (blocks hold whole z rows)
     int dev = 0;
     hipStream_t stream = nullptr;
     void* pop[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -126,6 +127,7 @@ StepArgs make_args(const mkg_ctx* h) {
     a.S = h->S;
     a.nblk = h->nblk;
     a.F = h->F;
+    a.zstage = h->zstage;
     a.has_walls = h->has_walls;
     a.has_kappa = (p.has_kappa && h->kappa) ? 1 : 0;
     a.ngrains = p.ngrains;
@@ -350,6 +352,10 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
     }
     h->nblk = (h->N + 255u) / 256u;
     h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
+    // z stage for O steps when a 256-thread block holds whole z rows of at least one wave
+    // (nz = 64, 128 or 256); MKG_ZSTAGE=0 turns it off
+    h->zstage = (p->nz >= 64 && 256 % p->nz == 0) ? 1 : 0;
+    if (const char* env = std::getenv("MKG_ZSTAGE")) h->zstage = h->zstage && std::atoi(env) != 0;
     h->q[LF] = p->flow_q;
     h->q[LG] = p->phase_q;
     h->q[LH] = p->thermal ? 7 : 0;

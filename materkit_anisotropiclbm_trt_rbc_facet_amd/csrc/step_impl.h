@@ -77,17 +77,75 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
     if (nf != fl) a.flag[c.id] = nf;
 }
 
-// Minimum waves per SIMD requested from the register allocator for the fused kernel.
-#ifndef MKG_STEP_WAVES
-#define MKG_STEP_WAVES 1
+// Minimum waves per SIMD requested from the register allocator for the fused kernel.  FP32:
+// four (at most 128 VGPRs).  Left alone, the allocator gives the z-staged O step 136 VGPRs and
+// three waves; with four it spills 12 bytes per lane and the step runs 7 % faster at 256^3.
+// FP64 kernels need about 230 VGPRs (two waves); a bound there only adds spills.
+#ifndef MKG_STEP_WAVES_F32
+#define MKG_STEP_WAVES_F32 4
 #endif
 
-template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD>
-__global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) {
+// LDS slots of the z stage shared by the flow and phase lattices (one after the other).
+template <int FQ, int PQ>
+constexpr int zstage_slots() {
+    int n = 1;
+    if (FQ == 19 && nzdirs<19>() > n) n = nzdirs<19>();
+    if (FQ == 27 && nzdirs<27>() > n) n = nzdirs<27>();
+    if (PQ == 19 && nzdirs<19>() > n) n = nzdirs<19>();
+    if (PQ == 7 && nzdirs<7>() > n) n = nzdirs<7>();
+    return n;
+}
+
+// Population access of one lattice in k_step: E step, O step, or O step with the z stage.
+template <int Q, bool ODD, bool Z, class R, int ZN>
+__device__ inline void pull(R* p, const LatArgs& L, uint32_t S, const Cell& c, const StepArgs& a,
+                            int kind, R (*zv)[256], uint32_t tid) {
+    if constexpr (Z)
+        read_pops_z<Q, R, ZN>(p, L, S, c, a, zv, tid);
+    else
+        read_pops<Q, ODD, R>(p, L, S, c, a, kind);
+}
+template <int Q, bool ODD, bool Z, class R, int ZN>
+__device__ inline void push(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                            const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
+                            uint32_t tid, bool act, const FaceState<R>& fs) {
+    if constexpr (Z)
+        write_pops_z<Q, R, ZN>(p, L, S, c, a, kind, zv, zm, tid, act, fs);
+    else
+        write_pops<Q, ODD, R>(p, L, S, c, a, kind, fs);
+}
+
+// ZS: O step with the z stage (only instantiated with ODD; the block must hold whole z rows).
+// With it every thread stays to the end (barriers), and act marks the threads that update.
+// Each pull and push has its own LDS region, except that the phase lattice reuses the flow's
+// regions: every thread has passed the flow push's barrier (after its last read of the flow
+// pull region) and the phase pull's barrier (after its last read of the flow push region)
+// before it writes them again.
+template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
+    k_step(const StepArgs a) {
+    constexpr bool Z = ODD && ZS;
+    constexpr int ZN = Z ? zstage_slots<FQ, PQ>() : 1;
+    constexpr int ZH = (Z && TH) ? nzdirs<7>() : 1;
+    constexpr int ZC = (Z && SO) ? nzdirs<7>() : 1;
+    __shared__ R zrf[ZN][256], zwf[ZN][256];  // flow, then phase: pull / push
+    __shared__ R zrh[ZH][256], zwh[ZH][256];  // thermal
+    __shared__ R zrc[ZC][256], zwc[ZC][256];  // solute
+    __shared__ uint32_t zmf[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
+    const uint32_t tid = threadIdx.x;
     Cell c;
-    if (!locate(a, c)) return;
+    bool act = true;
+    if constexpr (Z) {
+        act = locate_clamped(a, c);
+    } else {
+        if (!locate(a, c)) return;
+    }
     const int8_t fl = a.flag[c.id];
-    if (fl == kIsWalls) return;
+    if constexpr (Z) {
+        act = act && fl != kIsWalls;
+    } else {
+        if (fl == kIsWalls) return;
+    }
     const uint32_t S = a.S;
 
     R phi = (R)-1, dphi = (R)0, ns = (R)0;
@@ -105,7 +163,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
     R sh = 0, T = 0;
     if constexpr (TH) {
         if (a.h.adv)
-            read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+            pull<7, ODD, Z, R, ZH>(h, a.h, S, c, a, kKindScalar, zrh, tid);
         else
             read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
 #pragma unroll
@@ -116,7 +174,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
     R sc = 0, C = 0, U = 0;
     if constexpr (SO) {
         if (a.c.adv)
-            read_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar);
+            pull<7, ODD, Z, R, ZC>(cc, a.c, S, c, a, kKindScalar, zrc, tid);
         else
             read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
 #pragma unroll
@@ -131,7 +189,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
     if constexpr (FQ > 0) {
         R f[FQ];
         if (a.f.adv)
-            read_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow);
+            pull<FQ, ODD, Z, R, ZN>(f, a.f, S, c, a, kKindFlow, zrf, tid);
         else
             read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
         R dr = 0, jx = 0, jy = 0, jz = 0;
@@ -161,7 +219,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
             fs.ux = ux;
             fs.uy = uy;
             fs.uz = uz;
-            write_pops<FQ, ODD, R>(f, a.f, S, c, a, kKindFlow, fs);
+            push<FQ, ODD, Z, R, ZN>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
         }
         const R om = (R)1 - ns;
         ux *= om;
@@ -184,7 +242,8 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
                 const R heq = w * (sh + T * cu / (R)0.25);
                 h[i] = h[i] - om * (h[i] - heq) + w * src;
             }
-            write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+            push<7, ODD, Z, R, ZH>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+                                   FaceState<R>());
         }
     }
 
@@ -228,7 +287,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
             }
             FaceState<R> fs;
             fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
-            write_pops<7, ODD, R>(cc, a.c, S, c, a, kKindScalar, fs);
+            push<7, ODD, Z, R, ZC>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
         }
     }
 
@@ -255,7 +314,7 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
                                          (R)a.pf_wdx2, Mrot, rot, as);
             // the populations are read after the source so that the stencil values are dead
             R g[PQ];
-            read_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
+            pull<PQ, ODD, Z, R, ZN>(g, a.g, S, c, a, kKindScalar, zrf, tid);
             const R om = (R)a.omega_g;
             const R dp = phi - (R)a.g.ref;
             // advective flux (phi + 1) u: the liquid is invariant under any melt velocity
@@ -267,7 +326,8 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
                 const R geq = w * (dp + pa * cu);
                 g[i] = g[i] - om * (g[i] - geq) + w * Sg;
             }
-            write_pops<PQ, ODD, R>(g, a.g, S, c, a, kKindScalar);
+            push<PQ, ODD, Z, R, ZN>(g, a.g, S, c, a, kKindScalar, zwf, zmf, tid, act,
+                                    FaceState<R>());
         }
     }
 }
@@ -276,7 +336,9 @@ __global__ void __launch_bounds__(256, MKG_STEP_WAVES) k_step(const StepArgs a) 
 template <class R, int FQ, int PQ, bool TH, bool SO>
 inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
-    if (odd)
+    if (odd && a.zstage)
+        k_step<R, FQ, PQ, TH, SO, true, true><<<grid, block, 0, s>>>(a);
+    else if (odd)
         k_step<R, FQ, PQ, TH, SO, true><<<grid, block, 0, s>>>(a);
     else
         k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);

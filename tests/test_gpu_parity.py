@@ -86,16 +86,16 @@ CASES = [
 ]
 
 
-def _run(case, bcs, nsteps, prec=mkg.PREC_FP64, walls=False, seed=1):
+def _run(case, bcs, nsteps, prec=mkg.PREC_FP64, walls=False, seed=1, dims=(10, 9, 8)):
     kw = {k: v for k, v in case.items() if k != "id"}
     op = kw.pop("op", "bgk")
-    p = make_params(op=op, bcs=bcs, prec=prec, **kw)
+    p = make_params(op=op, bcs=bcs, prec=prec, dims=dims, **kw)
     st = random_state(p, seed=seed, with_walls=walls)
     orc = load_oracle(p, st)
     dev = load_device(p, st)
     try:
-        print(f"[parity] {case['id']} bcs={bcs} steps={nsteps} prec={prec} walls={walls}",
-              flush=True)
+        print(f"[parity] {case['id']} bcs={bcs} steps={nsteps} prec={prec} walls={walls} "
+              f"dims={dims}", flush=True)
         dev.step(nsteps, mkg.ALL)
         orc.step(nsteps, 15, OPS[op])
         errs = compare_pops(dev, orc)
@@ -169,3 +169,52 @@ def test_sequential_masks_match_oracle():
     dev.close()
     for k, e in errs.items():
         assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+# nz = 64: the O steps move the z populations through LDS (a 256-thread block holds four z
+# rows); 5 x 3 x 64 = 960 cells leaves the last block part-filled.
+ZDIMS = (5, 3, 64)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+def test_fp64_z_stage_matches_oracle(case, bcs):
+    errs, fd, fo, dev, orc = _run(case, bcs, 5, dims=ZDIMS)
+    dev.close()
+    assert np.array_equal(fd, fo), "flags differ"
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c.get("flow_q")],
+                         ids=[c["id"] for c in CASES if c.get("flow_q")])
+def test_fp64_z_stage_walls_match_oracle(case):
+    errs, fd, fo, dev, orc = _run(case, "mixed", 7, walls=True, dims=ZDIMS)
+    dev.close()
+    assert np.array_equal(fd, fo)
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+def test_z_stage_agrees_with_direct_o_steps(case, monkeypatch):
+    """The z stage changes only how populations move: FP32 results with and without it agree
+    to FP32 rounding (the two kernels are compiled separately, so the compiler may contract
+    the same arithmetic differently)."""
+    kw = {k: v for k, v in case.items() if k != "id"}
+    op = kw.pop("op", "bgk")
+    p = make_params(op=op, bcs="mixed", prec=mkg.PREC_FP32, dims=ZDIMS, **kw)
+    st = random_state(p, seed=3, with_walls=bool(kw.get("flow_q")))
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MKG_ZSTAGE", flag)
+        dev = load_device(p, st)
+        dev.step(5, mkg.ALL)
+        out.append({f: dev.get(f) for f in (mkg.F_POP_FLOW, mkg.F_POP_PHASE,
+                                             mkg.F_POP_THERMAL, mkg.F_POP_SOLUTE)
+                    if (f != mkg.F_POP_FLOW or p.flow_q) and (f != mkg.F_POP_PHASE or p.phase_q)
+                    and (f != mkg.F_POP_THERMAL or p.thermal) and (f != mkg.F_POP_SOLUTE or p.solute)})
+        dev.close()
+    for f in out[0]:
+        scale = np.abs(out[1][f]).max()
+        assert np.abs(out[0][f] - out[1][f]).max() <= 1e-6 * scale, f
