@@ -87,6 +87,12 @@ void orc_phase_source(const orc_params* p, const double* phi27, double upf, int 
 
 double orc_face_flux(const orc_params* p, const double* R, const double* g, int axis);
 
+/* Reference-structured step (mk_aos.c): the same step as orc_step(mask 15) on AoS node
+ * records of the reference's layout, swept operator by operator; for periodic flow (D3Q19) +
+ * phase + thermal cases without walls.  *seconds = time of the step loop alone.  Returns -1
+ * outside that scope. */
+int orc_aos_step(const orc_params* p, orc_state* s, int op, int nsteps, double* seconds);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 

@@ -27,7 +27,9 @@ _lib = None
 
 def build(force: bool = False) -> Path:
     """Compile ``liborc.so`` with the committed Makefile."""
-    if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "mk_oracle.c").stat().st_mtime:
+    srcs = ("mk_oracle.c", "mk_aos.c", "mk_oracle.h")
+    if force or not LIB.exists() or any(LIB.stat().st_mtime < (HERE / f).stat().st_mtime
+                                        for f in srcs):
         subprocess.run(["make", "-s", "-C", str(HERE)] + (["-B"] if force else []), check=True)
     return LIB
 
@@ -41,6 +43,8 @@ def lib() -> ct.CDLL:
         L.orc_params_size.restype = ct.c_size_t
         L.orc_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.c_int]
         L.orc_step.restype = ct.c_int
+        L.orc_aos_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.POINTER(ct.c_double)]
+        L.orc_aos_step.restype = ct.c_int
         L.orc_moments.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_init_equilibrium.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_init_seeds.argtypes = [vp, vp, ct.c_int, vp, vp]
@@ -143,6 +147,17 @@ class State:
         rc = lib().orc_step(ct.byref(self.params), ct.byref(st), int(mask), int(op), int(nsteps))
         if rc != 0:
             raise RuntimeError("oracle rejected the parameter block")
+
+    def aos_step(self, nsteps: int = 1, op: int = OP_BGK) -> float:
+        """The same steps (all lattices) on the reference-structured AoS path (mk_aos.c);
+        returns the seconds of the step loop."""
+        st = self._c()
+        sec = ct.c_double()
+        rc = lib().orc_aos_step(ct.byref(self.params), ct.byref(st), int(op), int(nsteps),
+                                ct.byref(sec))
+        if rc != 0:
+            raise RuntimeError("the AoS path covers periodic flow + phase + thermal cases only")
+        return sec.value
 
     def moments(self) -> dict:
         N = self.N
