@@ -104,7 +104,9 @@ extern "C" {
 #define MKG_R_MOMENTUM 6  /* out[0..2] = sum rho u */
 #define MKG_R_UMAX 7      /* max |u| */
 #define MKG_R_NONFINITE 8 /* number of non-finite populations */
-#define MKG_R_TIP 9       /* out[0..2]: furthest phi>=0 extent from the domain centre, +x,+y,+z */
+#define MKG_R_TIP 9       /* out[0..2]: furthest phi = 0 crossing (interpolated between a cell
+                             with phi >= 0 and its +x / +y / +z neighbour) from the domain
+                             centre along +x, +y, +z: the dendrite tips; -1e300 if none */
 
 #define MKG_MAX_GRAINS 64
 

@@ -374,11 +374,22 @@ __global__ void __launch_bounds__(256) k_reduce(const StepArgs a, int which, con
             }
             case 8: v[0] = 0; break;
             case 9: {
-                if (phi[t] >= 0.0) {
-                    uint32_t z = t % a.nz, r = t / a.nz, y = r % a.ny, x = r / a.ny;
-                    v[0] = x - 0.5 * (a.nx - 1);
-                    v[1] = y - 0.5 * (a.ny - 1);
-                    v[2] = z - 0.5 * (a.nz - 1);
+                // phi = 0 crossings between a cell with phi >= 0 and its +x / +y / +z
+                // neighbour with phi < 0, linearly interpolated; the largest per axis is the
+                // extreme point of the phi = 0 surface along that axis (the tip)
+                const double p0 = phi[t];
+                if (p0 >= 0.0) {
+                    const uint32_t z = t % a.nz, r = t / a.nz, y = r % a.ny, x = r / a.ny;
+                    const uint32_t c3[3] = {x, y, z};
+                    const uint32_t n3[3] = {(uint32_t)a.nx, (uint32_t)a.ny, (uint32_t)a.nz};
+                    const uint32_t st[3] = {(uint32_t)a.ny * (uint32_t)a.nz, (uint32_t)a.nz, 1u};
+                    for (int k = 0; k < 3; k++) {
+                        if (c3[k] + 1u < n3[k]) {
+                            const double p1 = phi[t + st[k]];
+                            if (p1 < 0.0)
+                                v[k] = c3[k] + p0 / (p0 - p1) - 0.5 * (n3[k] - 1.0);
+                        }
+                    }
                 }
                 break;
             }

@@ -4,9 +4,9 @@
 
 C1  duct start-up: centre-line velocity against the transient rectangular-duct series
 C2  Rayleigh-Benard: Nusselt number per frame
-C3  free dendrite: tip position along +x, +y, +z (phi = 0 crossing on the axes through the
-    seed, linear interpolation between cells), tip velocity (physical units W0 / tau0) and
-    solid fraction per frame
+C3  free dendrite: tip position along +x, +y, +z from the seed centre (the furthest phi = 0
+    crossing along each axis, interpolated between cells, reduced on the GPU), tip velocity
+    (physical units W0 / tau0) and solid fraction per frame
 C4  multi-dendrite growth: solid fraction, solute range and maximum velocity per frame
 C5  forward target run of the inverse problem
 
@@ -25,7 +25,8 @@ from . import cases, mkg
 
 
 def tip_positions(phi: np.ndarray, dims, center) -> np.ndarray:
-    """phi = 0 crossing along +x, +y, +z from the seed centre (cells, from the centre)."""
+    """phi = 0 crossing along +x, +y, +z on the axes through the seed centre (cells, from the
+    centre), from a host copy of phi: the check of the on-GPU tip reduction."""
     nx, ny, nz = dims
     f = phi.reshape(nx, ny, nz)
     cx, cy, cz = (int(round(c)) for c in center)
@@ -67,7 +68,9 @@ def run(name: str, steps: int | None = None, frames: int | None = None, precisio
         if p.phase_q:
             rec["solid_fraction"] = float(d.reduce(mkg.R_SOLID)[0])
             if case.seeds is not None and len(case.seeds[1]) == 1:
-                tip = tip_positions(d.get(mkg.F_PHI), dims, case.seeds[0][0])
+                # on-GPU tip reduction (from the domain centre) moved to the seed centre
+                centre = 0.5 * (np.array(dims, dtype=float) - 1.0)
+                tip = d.reduce(mkg.R_TIP) + centre - np.asarray(case.seeds[0][0], dtype=float)
                 rec["tip_cells"] = [round(float(t), 4) for t in tip]
                 if last_tip is not None and done > last_tip[0]:
                     v = (tip - last_tip[1]) * dx_phys / ((done - last_tip[0]) * dt_phys)

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -180,6 +181,8 @@ void mkFluidsLBM::write_vtkfile(int loops) {
     const int Mx = p_.nx, My = p_.ny, Mz = p_.nz;
     const size_t N = (size_t)Mx * My * Mz;
     std::ostringstream fn;
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "vtk", ec);
     fn << workspace_ << "vtk/fluid_t" << loops << ".vtk";
     std::ofstream o(fn.str());
     if (!o) throw mkt::mkIOException("cannot write " + fn.str());

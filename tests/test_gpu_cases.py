@@ -230,3 +230,71 @@ def test_reference_api_driver_runs_the_thermal_dendrite_loop(tmp_path):
                          text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     assert "solid_fraction" in res.stdout and "MLUPS" in res.stdout
+
+
+def _tip_numpy(phi, dims):
+    """The furthest phi = 0 crossing along +x, +y, +z (interpolated) from the domain centre."""
+    f = phi.reshape(dims)
+    out = []
+    for k in range(3):
+        a = np.moveaxis(f, k, 0)
+        p0, p1 = a[:-1], a[1:]
+        m = (p0 >= 0) & (p1 < 0)
+        if not m.any():
+            out.append(-1e300)
+            continue
+        idx = np.nonzero(m)[0].astype(float)
+        pos = idx + p0[m] / (p0[m] - p1[m]) - 0.5 * (dims[k] - 1)
+        out.append(pos.max())
+    return np.array(out)
+
+
+def test_tip_reduction_tracks_the_phi_zero_surface():
+    """On-GPU tips (MKG_R_TIP) equal the same crossings computed from a host copy of phi, and,
+    for the seed at the centre, the interpolated crossing on the axes through the seed."""
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import runs
+
+    case = cases.load("C3", nx=64, ny=64, nz=64, precision=mkg.PREC_FP64)
+    d = mkg.Solver(case.params)
+    cases.setup(d, case)
+    dims = (64, 64, 64)
+    for n in (0, 300, 300):
+        d.step(n, mkg.ALL)
+        tip = d.reduce(mkg.R_TIP)
+        phi = d.get(mkg.F_PHI)
+        np.testing.assert_allclose(tip, _tip_numpy(phi, dims), rtol=0, atol=1e-12)
+        centre = 0.5 * (np.array(dims) - 1.0)
+        on_axis = runs.tip_positions(phi, dims, case.seeds[0][0])
+        assert np.isfinite(on_axis).all() and on_axis.max() < 25  # inside the domain
+        np.testing.assert_allclose(tip + centre - case.seeds[0][0], on_axis, atol=1e-9)
+    d.close()
+
+
+def test_vtk_frames_of_the_reference_driver(tmp_path):
+    """mkrun with 'vtk' writes fluid_t<loops>.vtk (legacy RECTILINEAR_GRID, cell centres at
+    i + 0.5, x fastest) with the reference writer's fields; frame 0 holds the seed profile."""
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    dims = (12, 10, 8)
+    xml = tmp_path / "small.xml"
+    src = (ROOT / "cases" / "fused_flow_thermal_dendrite_256.xml").read_text()
+    xml.write_text(src.replace("256   256   256", "12 10 8").replace(
+        "127.5 127.5 127.5", "5.5 4.5 3.5").replace("<radius> 8 </radius>", "<radius> 3 </radius>"))
+    res = subprocess.run([str(exe), str(xml), "flow_thermal_dendrite", "4", "2", "vtk"],
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert res.returncode == 0, res.stderr
+    frames = sorted((tmp_path / "vtk").glob("fluid_t*.vtk"))
+    assert [f.name for f in frames] == ["fluid_t0.vtk", "fluid_t2.vtk", "fluid_t4.vtk"]
+    lines = (tmp_path / "vtk" / "fluid_t0.vtk").read_text().split("\n")
+    assert lines[0].startswith("# vtk DataFile") and lines[3] == "DATASET RECTILINEAR_GRID"
+    assert lines[4] == "DIMENSIONS 12 10 8"
+    assert [float(v) for v in lines[6].split()] == [i + 0.5 for i in range(12)]
+    N = 12 * 10 * 8
+    assert f"POINT_DATA {N}" in lines
+    names = [l.split()[1] for l in lines if l.startswith("SCALARS") or l.startswith("VECTORS")]
+    assert names == ["density_difference", "grain_id", "order_parameter",
+                     "delta_order_parameter", "temperature", "velocity_vector"]
+    k = lines.index("SCALARS order_parameter float 1")
+    phi_vtk = np.array([float(v) for v in lines[k + 2:k + 2 + N]])
+    case = cases.load(str(xml), precision=mkg.PREC_FP64)
+    phi0 = cases.seed_phi(case).reshape(dims)  # x, y, z with z fastest
+    np.testing.assert_allclose(phi_vtk, phi0.transpose(2, 1, 0).reshape(-1), atol=2e-7)
