@@ -805,9 +805,13 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     if (h->q[LF] || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
         return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model only");
     if (h->has_walls) return fail("the adjoint does not support wall cells");
-    if (nsteps < 1 || ckpt < 1) return fail("nsteps and ckpt must be >= 1");
+    if (nsteps < 1 || ckpt < 0) return fail("nsteps must be >= 1 and ckpt >= 0");
     if (dL_dkappa && !h->kappa) return fail("dL/dkappa needs has_kappa = 1");
     const size_t N = h->N, S = h->S, e = h->esz;
+    // ckpt == 0: tape -- the forward pass keeps the per-step values the reverse sweep needs,
+    // so there is no recompute (one segment of nsteps steps)
+    const bool tape = ckpt == 0;
+    if (tape) ckpt = nsteps;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
     const int adv[4] = {0, 1, 1, 0};
     // ---- device storage
@@ -815,7 +819,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         int lay_g, lay_h;
         int64_t pf;
     };
-    std::vector<Ck> ck(nseg + 1);
+    std::vector<Ck> ck(tape ? 1 : nseg + 1);
     const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH);
     const size_t ck_elems = PG + PH + 2 * S;
     const size_t seg_stride = S + PH;  // phi^n and the h buffer of one step
@@ -840,7 +844,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
             return fail(std::string(#expr) + ": " + hipGetErrorString(e_));      \
         }                                                                        \
     } while (0)
-    ADJCHK(hipMalloc((void**)&ckbuf, (nseg + 1) * ck_elems * e));
+    if (!tape) ADJCHK(hipMalloc((void**)&ckbuf, (nseg + 1) * ck_elems * e));
     ADJCHK(hipMalloc((void**)&segbuf, seg_elems * e));
     // work: Gin, Gout (19N), nin, nout (N), Hin, Hout (7N), pin, pout (N), P (19N), kap (N)
     const size_t work_elems = (2 * 19 + 2 + 2 * 7 + 2 + 19 + 1) * N;
@@ -889,13 +893,31 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     auto cleanup_ev = [&]() {
         for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
     };
+    std::vector<int> seg_lay(ckpt);
+    // per-step values of a forward step: phi^n and the thermal AA buffer before the step
+    auto stepper = [&](int n, int n0) -> int {
+        char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
+        seg_lay[n - n0] = h->lay[LH];
+        auto mid = [&](mkg_ctx* hh) -> int {
+            HIPCHK(hipMemcpyAsync(slot, hh->phi[hh->pf_count & 1], S * e,
+                                  hipMemcpyDeviceToDevice, hh->stream));
+            HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], PH * e, hipMemcpyDeviceToDevice,
+                                  hh->stream));
+            return 0;
+        };
+        return step_once_impl(h, adv, nullptr, mid);
+    };
     ADJCHK(hipEventRecord(ev[0], h->stream));
-    // ---- forward pass with checkpoints
+    // ---- forward pass: checkpoints, or the tape
     for (int n = 0; n < nsteps; n++) {
+        if (tape) {
+            if (stepper(n, 0)) { rc = -1; break; }
+            continue;
+        }
         if (n % ckpt == 0 && save_ck(n / ckpt)) { rc = -1; break; }
         if (step_once(h, adv, nullptr, nullptr)) { rc = -1; break; }
     }
-    if (rc == 0 && save_ck(nseg)) rc = -1;  // final forward state
+    if (rc == 0 && !tape && save_ck(nseg)) rc = -1;  // final forward state
     if (rc) { cleanup_ev(); release(); return -1; }
     // ---- loss and seed of the reverse sweep (phi_N - target, cell-wide)
     {
@@ -904,7 +926,6 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         ADJCHK(hipGetLastError());
     }
     ADJCHK(hipEventRecord(ev[1], h->stream));
-    std::vector<int> seg_lay(ckpt);
     float t_re = 0, t_rev = 0;
     for (int sgi = nseg - 1; sgi >= 0 && rc == 0; sgi--) {
         const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);
@@ -913,18 +934,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         hipEventCreate(&eb);
         hipEventCreate(&ec);
         hipEventRecord(ea, h->stream);
-        if (load_ck(sgi)) { rc = -1; break; }
-        for (int n = n0; n < n1; n++) {
-            char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
-            seg_lay[n - n0] = h->lay[LH];
-            auto mid = [&](mkg_ctx* hh) -> int {
-                HIPCHK(hipMemcpyAsync(slot, hh->phi[hh->pf_count & 1], S * e,
-                                      hipMemcpyDeviceToDevice, hh->stream));
-                HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], PH * e,
-                                      hipMemcpyDeviceToDevice, hh->stream));
-                return 0;
-            };
-            if (step_once_impl(h, adv, nullptr, mid)) { rc = -1; break; }
+        if (!tape) {
+            if (load_ck(sgi)) { rc = -1; break; }
+            for (int n = n0; n < n1; n++)
+                if (stepper(n, n0)) { rc = -1; break; }
         }
         hipEventRecord(eb, h->stream);
         for (int n = n1 - 1; n >= n0 && rc == 0; n--) {
@@ -970,8 +983,8 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         ADJCHK(hipMemcpyAsync(dL_dkappa, h->scratch, N * sizeof(double), hipMemcpyDeviceToHost,
                               h->stream));
     }
-    // leave the handle at the final forward state
-    if (load_ck(nseg)) { cleanup_ev(); release(); return -1; }
+    // leave the handle at the final forward state (the tape never moved it)
+    if (!tape && load_ck(nseg)) { cleanup_ev(); release(); return -1; }
     ADJCHK(hipStreamSynchronize(h->stream));
     float t_fwd = 0;
     hipEventElapsedTime(&t_fwd, ev[0], ev[1]);

@@ -107,13 +107,19 @@ def test_fp64_adjoint_matches_oracle_finite_differences(target, bcs, rotate):
 
 
 def test_checkpoint_spacing_does_not_change_the_gradient(target):
+    """Checkpoints every 1, 4 or 30 steps, or the tape (ckpt 0, no recompute), give the same
+    gradient; the tape leaves the handle at the final forward state like the others."""
     p = params()
     kap = kappa_field(3)
     outs = []
-    for ck in (1, 4, 30):
+    finals = []
+    for ck in (1, 4, 30, 0):
         d = make_device(p, kap)
         outs.append(d.adjoint_run(NSTEPS, ck, target))
+        finals.append(d.get(mkg.F_POP_PHASE))
         d.close()
+    for fin in finals[1:]:
+        np.testing.assert_array_equal(fin, finals[0])
     for o in outs[1:]:
         assert abs(o["dL_deps4"] - outs[0]["dL_deps4"]) <= 1e-12 * abs(outs[0]["dL_deps4"])
         assert np.abs(o["dL_dkappa"] - outs[0]["dL_dkappa"]).max() <= 1e-12 * np.abs(
