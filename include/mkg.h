@@ -122,7 +122,8 @@ typedef struct mkg_params {
     int32_t ngrains;      /* 0: no rotation; else per-cell grain ids index rot[] (1-based) */
     int32_t has_kappa;    /* per-cell thermal conductivity multiplier present */
     int32_t bc_f[6], bc_g[6], bc_h[6], bc_c[6]; /* per lattice, per face */
-    int32_t pad0;
+    int32_t flow_incomp;  /* 1: He-Luo incompressible equilibrium (rho0 in the u terms,
+                             u = (j + F/2) / rho0; MaterKit/inc/mkNode.h getEquPDFsHe) */
     /* flow */
     double s_nu;          /* shear rate = 1/tau */
     double s_bulk;        /* bulk (trace) rate */

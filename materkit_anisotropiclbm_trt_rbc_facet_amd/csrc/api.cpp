@@ -128,6 +128,7 @@ StepArgs make_args(const mkg_ctx* h) {
     a.nblk = h->nblk;
     a.F = h->F;
     a.zstage = h->zstage;
+    a.incomp = p.flow_incomp ? 1 : 0;
     a.has_walls = h->has_walls;
     a.has_kappa = (p.has_kappa && h->kappa) ? 1 : 0;
     a.ngrains = p.ngrains;

@@ -126,9 +126,10 @@ __global__ void __launch_bounds__(256) k_macro(const StepArgs a, double* rho, do
             for (int k = 0; k < 3; k++) F[k] += a.rho0 * a.buoy[k] * (Tv - a.T_ref);
         if (rho) rho[c.id] = (double)rh;
         if (u) {
-            u[3 * (size_t)c.id + 0] = ((double)jx + 0.5 * F[0]) / (double)rh;
-            u[3 * (size_t)c.id + 1] = ((double)jy + 0.5 * F[1]) / (double)rh;
-            u[3 * (size_t)c.id + 2] = ((double)jz + 0.5 * F[2]) / (double)rh;
+            const double den = a.incomp ? a.rho0 : (double)rh;
+            u[3 * (size_t)c.id + 0] = ((double)jx + 0.5 * F[0]) / den;
+            u[3 * (size_t)c.id + 1] = ((double)jy + 0.5 * F[1]) / den;
+            u[3 * (size_t)c.id + 2] = ((double)jz + 0.5 * F[2]) / den;
         }
     } else if (u) {
         u[3 * (size_t)c.id + 0] = 0;
@@ -225,7 +226,9 @@ __global__ void __launch_bounds__(256) k_init_flow(const StepArgs a, const doubl
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const double cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
-        const double feq = wgt<Q>(i) * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+        const double feq =
+            a.incomp ? wgt<Q>(i) * (r + a.rho0 * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu))
+                     : wgt<Q>(i) * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
         f[(size_t)i * a.S + id] = (R)(feq - wgt<Q>(i) * a.f.ref);
     }
 }

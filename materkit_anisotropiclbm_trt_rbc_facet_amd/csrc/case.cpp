@@ -296,6 +296,13 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
     if (!(sf > 0) && nu_phys > 0) sf = 1.0 / (3.0 * nu_phys * lu + 0.5);
     if (!(sf > 0)) sf = 1.0;
     std::string op = c.word("solver/operator", "bgk");
+    // equilibrium: compressible (default) or the He-Luo incompressible form
+    const std::string eq = c.word("solver/equilibrium", "compressible");
+    if (eq != "compressible" && eq != "incompressible") {
+        g_case_err = "solver/equilibrium must be compressible or incompressible";
+        return -1;
+    }
+    p->flow_incomp = eq == "incompressible" ? 1 : 0;
     p->s_nu = sf;
     p->s_bulk = p->s_odd = p->s_even = sf;
     p->flow_op = MKG_OP_BGK;

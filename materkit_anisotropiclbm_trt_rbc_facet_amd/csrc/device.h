@@ -402,7 +402,8 @@ __device__ inline void read_pops_rt(R* p, const LatArgs& L, uint32_t S, const Ce
 // and even (s_even) halves.  BGK is all four rates equal; TRT is s_bulk = s_even = s_nu,
 // s_odd = omega-; MRT uses independent rates.  Forcing is Guo et al. (2002) projected onto
 // the same parts with (1 - s/2) factors.  Populations are deviations f - w_i*ref; dr is their
-// sum (rho - ref).  Counterparts: mkNode::collision_BGK / collision_TRT / collision_MRT0-2
+// sum (rho - ref); rho is the density of the velocity terms of the equilibrium (rho itself, or
+// rho0 for the He-Luo incompressible form).  Counterparts: mkNode::collision_BGK / collision_TRT / collision_MRT0-2
 // (MaterKit/inc/mkNode.h:72-87).
 template <int Q, class R>
 __device__ inline void flow_collide(R* f, R dr, R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz,

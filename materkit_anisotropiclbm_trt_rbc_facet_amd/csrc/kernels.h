@@ -27,7 +27,7 @@ struct StepArgs {
     uint32_t F;       // face-buffer stride: cells on the six faces, 2 (ny nz + nx nz + nx ny)
     int32_t has_walls, has_kappa, ngrains, pf_coupling;
     int32_t zstage;   // O steps move the z populations through LDS (needs 256 % nz == 0)
-    int32_t pad1;
+    int32_t incomp;   // He-Luo incompressible equilibrium: rho0 in the u terms
     LatArgs f, g, h, c;
     const void* phi_cur;   // storage type, phase field at time level n
     const void* phi_prev;  // level n-1

@@ -208,11 +208,13 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             Fy += (R)(a.rho0 * a.buoy[1]) * dT;
             Fz += (R)(a.rho0 * a.buoy[2]) * dT;
         }
-        ux = (jx + (R)0.5 * Fx) / rho;
-        uy = (jy + (R)0.5 * Fy) / rho;
-        uz = (jz + (R)0.5 * Fz) / rho;
+        // the density in the u terms: rho, or rho0 for the incompressible equilibrium
+        const R den = a.incomp ? (R)a.rho0 : rho;
+        ux = (jx + (R)0.5 * Fx) / den;
+        uy = (jy + (R)0.5 * Fy) / den;
+        uz = (jz + (R)0.5 * Fz) / den;
         if (a.f.adv) {
-            flow_collide<FQ, R>(f, dr, rho, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
+            flow_collide<FQ, R>(f, dr, den, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
                                 (R)a.s_odd, (R)a.s_even, ns);
             FaceState<R> fs;
             fs.wscale = (R)1 - ns;

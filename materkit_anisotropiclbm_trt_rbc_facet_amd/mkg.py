@@ -34,7 +34,7 @@ PARAM_FIELDS = [
     ("solute", C.c_int32), ("precision", C.c_int32), ("flow_op", C.c_int32),
     ("pf_coupling", C.c_int32), ("ngrains", C.c_int32), ("has_kappa", C.c_int32),
     ("bc_f", C.c_int32 * 6), ("bc_g", C.c_int32 * 6), ("bc_h", C.c_int32 * 6),
-    ("bc_c", C.c_int32 * 6), ("pad0", C.c_int32),
+    ("bc_c", C.c_int32 * 6), ("flow_incomp", C.c_int32),
     ("s_nu", C.c_double), ("s_bulk", C.c_double), ("s_odd", C.c_double),
     ("s_even", C.c_double), ("rho0", C.c_double), ("force", C.c_double * 3),
     ("buoy", C.c_double * 3), ("T_ref", C.c_double), ("u_face", (C.c_double * 3) * 6),

@@ -134,7 +134,8 @@ static void aos_one_step(const orc_params* p, aos_node* nd, long N, int op, cons
                 if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (n->T - p->T_ref);
             }
             n->rho = rho;
-            for (int a = 0; a < 3; a++) n->vel[a] = (j[a] + 0.5 * F[a]) / rho;
+            const double den = p->flow_incomp ? p->rho0 : rho;
+            for (int a = 0; a < 3; a++) n->vel[a] = (j[a] + 0.5 * F[a]) / den;
             orc_collide_cell(p, op, n->f_hlp, F, n->f_out, ns);
         }
     }

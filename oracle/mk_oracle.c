@@ -137,6 +137,20 @@ void orc_equilibrium(int q, double rho, const double* u, double* feq) {
     }
 }
 
+/* He & Luo (1997) incompressible equilibrium: w_i [rho + rho0 (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)]
+ * (the getEquPDFsHe variant, MaterKit/inc/mkNode.h:34-50) */
+void orc_equilibrium_he(int q, double rho, double rho0, const double* u, double* feq) {
+    lat_t L = make_lat(q);
+    double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int i = 0; i < q; i++) {
+        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+        feq[i] = L.w[i] * (rho + rho0 * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+    }
+}
+
+/* velocity of a cell: (j + F/2) / rho, or / rho0 for the incompressible equilibrium */
+static double flow_den(const orc_params* p, double rho) { return p->flow_incomp ? p->rho0 : rho; }
+
 /* Guo forcing term without its rate factor: w_i [3 (c-u).F + 9 (c.u)(c.F)] */
 static void guo_term(int q, const double* u, const double* F, double* Fi) {
     lat_t L = make_lat(q);
@@ -272,9 +286,12 @@ void orc_collide_cell(const orc_params* p, int op, const double* f, const double
         for (int a = 0; a < 3; a++) j[a] += E27[i][a] * f[i];
     }
     double u[3];
-    for (int a = 0; a < 3; a++) u[a] = (j[a] + 0.5 * F[a]) / rho;
+    for (int a = 0; a < 3; a++) u[a] = (j[a] + 0.5 * F[a]) / flow_den(p, rho);
     double feq[27], Fi[27], fs[27];
-    orc_equilibrium(q, rho, u, feq);
+    if (p->flow_incomp)
+        orc_equilibrium_he(q, rho, p->rho0, u, feq);
+    else
+        orc_equilibrium(q, rho, u, feq);
     guo_term(q, u, F, Fi);
     if (op == ORC_OP_BGK) {
         double om = p->s_nu;
@@ -453,7 +470,7 @@ static void compute_macro(const orc_params* p, const orc_state* s, macro_t* m) {
                 if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (m->T[x] - p->T_ref);
             }
             m->rho[x] = rho;
-            for (int a = 0; a < 3; a++) m->u[3 * x + a] = (j[a] + 0.5 * F[a]) / rho;
+            for (int a = 0; a < 3; a++) m->u[3 * x + a] = (j[a] + 0.5 * F[a]) / flow_den(p, rho);
         } else {
             for (int a = 0; a < 3; a++) m->u[3 * x + a] = 0;
         }
@@ -779,7 +796,10 @@ void orc_init_equilibrium(const orc_params* p, orc_state* s, const double* rho,
         }
         if (p->flow_q) {
             double feq[27];
-            orc_equilibrium(p->flow_q, rho ? rho[x] : p->rho0, u, feq);
+            if (p->flow_incomp)
+                orc_equilibrium_he(p->flow_q, rho ? rho[x] : p->rho0, p->rho0, u, feq);
+            else
+                orc_equilibrium(p->flow_q, rho ? rho[x] : p->rho0, u, feq);
             for (int i = 0; i < p->flow_q; i++) s->f[i * N + x] = feq[i];
         }
         lat_t L7 = make_lat(7);

@@ -24,7 +24,7 @@ typedef struct orc_params {
     int32_t flow_q, phase_q, thermal, solute, precision, flow_op, pf_coupling, ngrains,
         has_kappa;
     int32_t bc_f[6], bc_g[6], bc_h[6], bc_c[6];
-    int32_t pad0;
+    int32_t flow_incomp;
     double s_nu, s_bulk, s_odd, s_even;
     double rho0;
     double force[3];
@@ -80,6 +80,7 @@ void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* cent
 
 /* Single-cell helpers exposed for unit tests. */
 void orc_equilibrium(int q, double rho, const double* u, double* feq);
+void orc_equilibrium_he(int q, double rho, double rho0, const double* u, double* feq);
 void orc_collide_cell(const orc_params* p, int op, const double* f, const double* F, double* out,
                       double ns);
 void orc_phase_source(const orc_params* p, const double* phi27, double upf, int grain,

@@ -49,6 +49,7 @@ def lib() -> ct.CDLL:
         L.orc_init_equilibrium.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_init_seeds.argtypes = [vp, vp, ct.c_int, vp, vp]
         L.orc_equilibrium.argtypes = [ct.c_int, ct.c_double, vp, vp]
+        L.orc_equilibrium_he.argtypes = [ct.c_int, ct.c_double, ct.c_double, vp, vp]
         L.orc_collide_cell.argtypes = [vp, ct.c_int, vp, vp, vp, ct.c_double]
         L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
         L.orc_face_flux.argtypes = [vp, vp, vp, ct.c_int]
@@ -192,6 +193,13 @@ def equilibrium(q: int, rho: float, u) -> np.ndarray:
     out = np.zeros(q)
     u = np.ascontiguousarray(u, dtype=np.float64)
     lib().orc_equilibrium(q, rho, _p(u), _p(out))
+    return out
+
+
+def equilibrium_he(q: int, rho: float, rho0: float, u) -> np.ndarray:
+    out = np.zeros(q)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lib().orc_equilibrium_he(q, rho, rho0, _p(u), _p(out))
     return out
 
 

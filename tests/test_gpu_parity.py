@@ -18,10 +18,10 @@ P, BB, DI, OUT, PR = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_O
 
 
 def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
-                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0):
+                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0, incomp=0):
     p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], flow_q=flow_q, phase_q=phase_q,
                            thermal=thermal, solute=solute, precision=prec, ngrains=ngrains,
-                           has_kappa=kappa)
+                           has_kappa=kappa, flow_incomp=incomp)
     om = 1.0 / 0.8
     p.s_nu = om
     if op == "bgk":
@@ -83,6 +83,8 @@ CASES = [
     dict(id="solute_only", solute=1),
     dict(id="pf_only", phase_q=19),
     dict(id="pf_thermal_kappa", phase_q=19, thermal=1, kappa=1),
+    dict(id="trt19_incomp", flow_q=19, op="trt", incomp=1),
+    dict(id="fused_incomp", flow_q=19, phase_q=19, thermal=1, op="mrt", incomp=1),
 ]
 
 

@@ -285,3 +285,40 @@ def test_velocity_inlet_and_pressure_outlet_carry_a_uniform_stream():
     # mass flux rho u_x uniform along the channel and equal to the inflow rho0 u_in
     flux = (rho * u[:, :, 0]).mean(1)
     assert np.abs(flux / u_in - 1).max() < 1e-4
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_he_luo_equilibrium_moments(q):
+    """He-Luo incompressible equilibrium: sum f = rho, sum c f = rho0 u,
+    sum c c f = rho cs^2 I + rho0 u u."""
+    e, w, _, cs2 = lat(q)
+    rho, rho0 = 1.03, 0.97
+    u = np.array([0.03, -0.02, 0.015])
+    feq = O.equilibrium_he(q, rho, rho0, u)
+    assert abs(feq.sum() - rho) < 1e-15
+    np.testing.assert_allclose(e.T @ feq, rho0 * u, atol=1e-16)
+    P = np.einsum("ia,ib,i->ab", e, e, feq)
+    np.testing.assert_allclose(P, rho * cs2 * np.eye(3) + rho0 * np.outer(u, u), atol=1e-15)
+
+
+@pytest.mark.parametrize("incomp", [0, 1])
+def test_channel_flux_conservation_of_the_two_equilibria(incomp):
+    """Channel (walls in y) with a velocity inlet and a pressure outlet: in the steady state
+    the compressible equilibrium conserves the mass flux sum(rho u_x) along the channel and
+    lets the volume flux change with the density (4 % here), while the He-Luo incompressible
+    one keeps the volume flux sum(u_x) (div u = 0)."""
+    nx, ny, u_in = 48, 9, 0.02
+    p = mkg.default_params(nx=nx, ny=ny, nz=1, flow_q=19, s_nu=1 / 0.8, s_bulk=1 / 0.8,
+                           s_odd=1 / 0.8, s_even=1 / 0.8, flow_incomp=incomp)
+    p.set(bc_f=[mkg.BC_BOUNCE, mkg.BC_PRESSURE, mkg.BC_BOUNCE, mkg.BC_BOUNCE, 0, 0])
+    p.u_face[0][0] = u_in
+    s = O.State.empty(p)
+    s.init_equilibrium()
+    s.step(10000, 1, O.OP_TRT)
+    m = s.moments()
+    rho = m["rho"].reshape(nx, ny)[2:-2]
+    u = m["vel"].reshape(nx, ny, 3)[2:-2, :, 0]
+    vol, mass = u.sum(1), (rho * u).sum(1)
+    kept, varied = (vol, mass) if incomp else (mass, vol)
+    assert np.ptp(kept) < 1e-10 * kept.mean()
+    assert np.ptp(varied) > 0.02 * varied.mean()
