@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define MKG_ABI_VERSION 2
+#define MKG_ABI_VERSION 3
 
 /* ---- node flags: identical codes to MaterKit/inc/mkGlobalDefs.h:140-145 ---- */
 #define MKG_IS_OUTLET (-2)
@@ -67,6 +67,16 @@ extern "C" {
 #define MKG_BC_OUTFLOW 3   /* unknown incoming populations copied from the cell itself */
 #define MKG_BC_PRESSURE 4  /* flow only: anti-bounce-back to the equilibrium at density rho0 and
                               the cell's velocity (pressure outlet) */
+
+/* ---- anisotropy forms a_s(n), n the unit normal in the crystal frame ---- */
+#define MKG_ANISO_CUBIC 0 /* 1 - 3 eps4 + 4 eps4 sum n^4 (Karma-Rappel), eps4 */
+#define MKG_ANISO_CBC 1   /* 1 + e1 (Q - 3/5) + e2 (3 Q + 66 S - 17/7), Q = sum n^4,
+                             S = nx^2 ny^2 nz^2 (two cubic harmonics: fcc / bcc / cubic);
+                             aniso = {e1, e2} */
+#define MKG_ANISO_HCP 2   /* 1 + e6 Re (nx + i ny)^6 + ez (3 nz^2 - 1) / 2 (six-fold basal
+                             anisotropy and the c axis: hcp, snowflakes); aniso = {e6, ez} */
+#define MKG_ANISO_FACET 3 /* 1 + ef (sum sqrt(n_k^2 + nu^2) - sqrt(1 + nu^2) - 2 nu): cusps at
+                             the {100} facets rounded over nu; aniso = {ef, nu} */
 
 /* ---- lattice masks for mkg_step ---- */
 #define MKG_FLOW 1
@@ -148,6 +158,10 @@ typedef struct mkg_params {
     double eps4;
     double lambda;
     double phi_s;         /* flag threshold: phi >= phi_s solid, > -phi_s interface */
+    int32_t aniso_kind;   /* MKG_ANISO_*: the form of a_s(n) (relaxation_anisotropy_cbc / _hcp,
+                             anisotropy_facet; MaterKit/inc/mkPhaseField.h:49-64) */
+    int32_t pad1;
+    double aniso[2];      /* parameters of the non-cubic forms (see MKG_ANISO_*) */
     /* deviation references for FP32 storage */
     double ref_f, ref_g, ref_h, ref_c;
     double rot[MKG_MAX_GRAINS][9]; /* row-major lab<-crystal rotation per grain */

@@ -168,6 +168,9 @@ StepArgs make_args(const mkg_ctx* h) {
     a.eps4 = p.eps4;
     a.lambda = p.lambda;
     a.phi_s = p.phi_s;
+    a.aniso_kind = p.aniso_kind;
+    a.aniso[0] = p.aniso[0];
+    a.aniso[1] = p.aniso[1];
     const double cs2g = (p.phase_q == 7) ? 0.25 : 1.0 / 3.0;
     a.omega_g = 1.0 / (0.5 + p.pf_dt * p.pf_wdx2 / cs2g);
     return a;
@@ -207,6 +210,10 @@ int validate(const mkg_params* p) {
     if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
         return fail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
     if (p->ngrains < 0 || p->ngrains > MKG_MAX_GRAINS) return fail("ngrains out of range");
+    if (p->aniso_kind < MKG_ANISO_CUBIC || p->aniso_kind > MKG_ANISO_FACET)
+        return fail("aniso_kind must be one of MKG_ANISO_*");
+    if (p->aniso_kind == MKG_ANISO_FACET && !(p->aniso[1] > 0))
+        return fail("the facet anisotropy needs a rounding nu > 0");
     if (!combo_supported(p->flow_q, p->phase_q, p->thermal ? 1 : 0, p->solute ? 1 : 0))
         return fail("unsupported combination of lattices (flow_q, phase_q, thermal, solute)");
     const int32_t* bcs[4] = {p->bc_f, p->bc_g, p->bc_h, p->bc_c};
@@ -806,6 +813,8 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     if (h->q[LF] || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
         return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model only");
     if (h->has_walls) return fail("the adjoint does not support wall cells");
+    if (h->p.aniso_kind != MKG_ANISO_CUBIC)
+        return fail("the adjoint covers the cubic (Karma-Rappel) anisotropy only");
     if (nsteps < 1 || ckpt < 0) return fail("nsteps must be >= 1 and ckpt >= 0");
     if (dL_dkappa && !h->kappa) return fail("dL/dkappa needs has_kappa = 1");
     const size_t N = h->N, S = h->S, e = h->esz;

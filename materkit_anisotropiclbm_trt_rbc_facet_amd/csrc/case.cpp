@@ -303,6 +303,18 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
         return -1;
     }
     p->flow_incomp = eq == "incompressible" ? 1 : 0;
+    // anisotropy form of the phase field and its two parameters
+    const std::string af = c.word("solver/anisotropyForm", "cubic");
+    if (af == "cubic") p->aniso_kind = MKG_ANISO_CUBIC;
+    else if (af == "cbc") p->aniso_kind = MKG_ANISO_CBC;
+    else if (af == "hcp") p->aniso_kind = MKG_ANISO_HCP;
+    else if (af == "facet") p->aniso_kind = MKG_ANISO_FACET;
+    else {
+        g_case_err = "solver/anisotropyForm must be cubic, cbc, hcp or facet";
+        return -1;
+    }
+    auto ap = c.nums("solver/anisotropyParams");
+    for (size_t k = 0; k < ap.size() && k < 2; k++) p->aniso[k] = ap[k];
     p->s_nu = sf;
     p->s_bulk = p->s_odd = p->s_even = sf;
     p->flow_op = MKG_OP_BGK;

@@ -490,8 +490,53 @@ __device__ inline R pow4(R v) {
 
 // Normal component of J = a_s^2 g + |g|^2 a_s da_s/dg on a face with gradient g (lab frame);
 // M (lab <- crystal, row-major) may be null.
+// Anisotropy a_s(n) and its derivative d a_s / d n (n the unit normal, crystal frame) for the
+// forms MKG_ANISO_CUBIC / CBC / HCP / FACET of mkg.h; e4 = eps4, p0 / p1 = the form's two
+// parameters.  Kernel arguments are uniform, so the switch does not diverge.
 template <class R>
-__device__ inline R face_flux(R g0, R g1, R g2, int axis, R eps4, const R (&M)[9], bool rot) {
+struct Aniso {
+    int kind;
+    R e4, p0, p1;
+};
+template <class R>
+__device__ inline R aniso_eval(const Aniso<R>& an, R n0, R n1, R n2, R& d0, R& d1, R& d2) {
+    const R q0 = n0 * n0, q1 = n1 * n1, q2 = n2 * n2;
+    if (an.kind == 1) {  // two cubic harmonics
+        const R Q = q0 * q0 + q1 * q1 + q2 * q2, S = q0 * q1 * q2;
+        const R c4 = (R)4 * an.p0 + (R)12 * an.p1, c6 = (R)132 * an.p1;
+        d0 = c4 * q0 * n0 + c6 * n0 * q1 * q2;
+        d1 = c4 * q1 * n1 + c6 * n1 * q0 * q2;
+        d2 = c4 * q2 * n2 + c6 * n2 * q0 * q1;
+        return (R)1 + an.p0 * (Q - (R)0.6) + an.p1 * ((R)3 * Q + (R)66 * S - (R)(17.0 / 7.0));
+    }
+    if (an.kind == 2) {  // hexagonal: Re (nx + i ny)^6 and the c axis
+        const R H = q0 * q0 * q0 - (R)15 * q0 * q0 * q1 + (R)15 * q0 * q1 * q1 - q1 * q1 * q1;
+        d0 = an.p0 * ((R)6 * q0 * q0 * n0 - (R)60 * q0 * n0 * q1 + (R)30 * n0 * q1 * q1);
+        d1 = an.p0 * ((R)-30 * q0 * q0 * n1 + (R)60 * q0 * q1 * n1 - (R)6 * q1 * q1 * n1);
+        d2 = an.p1 * (R)3 * n2;
+        return (R)1 + an.p0 * H + an.p1 * ((R)1.5 * q2 - (R)0.5);
+    }
+    if (an.kind == 3) {  // facets: |n_k| rounded over nu
+        const R v2 = an.p1 * an.p1;
+        const R r0 = sqrt(q0 + v2), r1 = sqrt(q1 + v2), r2 = sqrt(q2 + v2);
+        d0 = an.p0 * n0 / r0;
+        d1 = an.p0 * n1 / r1;
+        d2 = an.p0 * n2 / r2;
+        return (R)1 + an.p0 * (r0 + r1 + r2 - sqrt((R)1 + v2) - (R)2 * an.p1);
+    }
+    const R k = (R)16 * an.e4;  // Karma-Rappel cubic
+    d0 = k * q0 * n0;
+    d1 = k * q1 * n1;
+    d2 = k * q2 * n2;
+    return (R)1 - (R)3 * an.e4 + (R)4 * an.e4 * (q0 * q0 + q1 * q1 + q2 * q2);
+}
+
+// Normal component along `axis` of J = dE/dG, E = |G|^2 a_s(G/|G|)^2 / 2:
+//   J = a^2 G + |G| a (D - n (n.D)),  D = d a_s / d n,
+// evaluated from a lab-frame face gradient g (crystal frame through M when rot).
+template <class R>
+__device__ inline R face_flux(R g0, R g1, R g2, int axis, const Aniso<R>& an, const R (&M)[9],
+                              bool rot) {
     R c0 = g0, c1 = g1, c2 = g2;
     if (rot) {
         c0 = M[0] * g0 + M[3] * g1 + M[6] * g2;
@@ -503,12 +548,13 @@ __device__ inline R face_flux(R g0, R g1, R g2, int axis, R eps4, const R (&M)[9
     if (!(g2s > (R)1e-30)) return ga;
     const R gn = sqrt(g2s);
     const R n0 = c0 / gn, n1 = c1 / gn, n2 = c2 / gn;
-    const R s4 = pow4(n0) + pow4(n1) + pow4(n2);
-    const R as = (R)1 - (R)3 * eps4 + (R)4 * eps4 * s4;
-    const R k = (R)16 * eps4 * as * gn;
-    const R N0 = k * (n0 * n0 * n0 - n0 * s4);
-    const R N1 = k * (n1 * n1 * n1 - n1 * s4);
-    const R N2 = k * (n2 * n2 * n2 - n2 * s4);
+    R D0, D1, D2;
+    const R as = aniso_eval<R>(an, n0, n1, n2, D0, D1, D2);
+    const R nD = n0 * D0 + n1 * D1 + n2 * D2;
+    const R k = as * gn;
+    const R N0 = k * (D0 - n0 * nD);
+    const R N1 = k * (D1 - n1 * nD);
+    const R N2 = k * (D2 - n2 * nD);
     R Nl;
     if (rot)
         Nl = M[axis * 3 + 0] * N0 + M[axis * 3 + 1] * N1 + M[axis * 3 + 2] * N2;
@@ -517,11 +563,9 @@ __device__ inline R face_flux(R g0, R g1, R g2, int axis, R eps4, const R (&M)[9
     return as * as * ga + Nl;
 }
 
-// P holds the 3x3x3 neighbourhood, index (dx+1)*9 + (dy+1)*3 + (dz+1) (corners unused).
-// Returns S of the phase-field LB update and the cell anisotropy a_s.
 template <class R>
-__device__ inline R phase_source(const R (&P)[27], R upf, R eps4, R lambda, R pf_dt, R pf_wdx2,
-                                 const R (&M)[9], bool rot, R& a_out) {
+__device__ inline R phase_source(const R (&P)[27], R upf, const Aniso<R>& an, R lambda, R pf_dt,
+                                 R pf_wdx2, const R (&M)[9], bool rot, R& a_out) {
 #define PP(dx, dy, dz) P[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
     R divJ = 0, lap = 0;
 #pragma unroll
@@ -542,7 +586,7 @@ __device__ inline R phase_source(const R (&P)[27], R upf, R eps4, R lambda, R pf
                                       PP(o0 - e0, o1 - e1, o2 - e2));
                 }
             }
-            const R J = face_flux<R>(g[0], g[1], g[2], ax, eps4, M, rot);
+            const R J = face_flux<R>(g[0], g[1], g[2], ax, an, M, rot);
             divJ += (R)side * J;
             lap += (R)side * g[ax];
         }
@@ -566,7 +610,8 @@ __device__ inline R phase_source(const R (&P)[27], R upf, R eps4, R lambda, R pf
     const R gg = c0 * c0 + c1 * c1 + c2 * c2;
     if (gg > (R)1e-30) {
         const R gn = sqrt(gg);
-        as = (R)1 - (R)3 * eps4 + (R)4 * eps4 * (pow4(c0 / gn) + pow4(c1 / gn) + pow4(c2 / gn));
+        R D0, D1, D2;
+        as = aniso_eval<R>(an, c0 / gn, c1 / gn, c2 / gn, D0, D1, D2);
     }
     const R ph = PP(0, 0, 0);
     const R d = (R)1 - ph * ph;

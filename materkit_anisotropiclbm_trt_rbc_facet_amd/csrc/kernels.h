@@ -42,6 +42,8 @@ struct StepArgs {
     double D_l, D_s, k_part, latent_C, q_min;
     double at_w;  // anti-trapping a_t * W0/dx
     double pf_dt, pf_wdx2, eps4, lambda, phi_s, omega_g;
+    int32_t aniso_kind, pad2;
+    double aniso[2];
 };
 
 // Selector of a fused-step instantiation: presence of each lattice.

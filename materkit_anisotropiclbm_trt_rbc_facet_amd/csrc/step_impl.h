@@ -312,7 +312,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                 }
             }
             R as;
-            const R Sg = phase_source<R>(P, upf, (R)a.eps4, (R)a.lambda, (R)a.pf_dt,
+            const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
+            const R Sg = phase_source<R>(P, upf, an, (R)a.lambda, (R)a.pf_dt,
                                          (R)a.pf_wdx2, Mrot, rot, as);
             // the populations are read after the source so that the stencil values are dead
             R g[PQ];

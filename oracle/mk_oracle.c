@@ -327,13 +327,50 @@ void orc_collide_cell(const orc_params* p, int op, const double* f, const double
 }
 
 /* ------------------------------------------------------------------ phase field */
-static double aniso_as(double eps4, const double* n) {
-    return 1.0 - 3.0 * eps4 + 4.0 * eps4 * (n[0] * n[0] * n[0] * n[0] + n[1] * n[1] * n[1] * n[1] +
-                                            n[2] * n[2] * n[2] * n[2]);
+/* a_s(n) and D = d a_s / d n for the unit crystal-frame normal n, by form (aniso_kind):
+ *   0 cubic (Karma-Rappel 1998): 1 - 3 e4 + 4 e4 sum n^4
+ *   1 two cubic harmonics (Haxhimali et al. 2006): 1 + e1 (Q - 3/5) + e2 (3Q + 66S - 17/7),
+ *     Q = sum n^4, S = nx^2 ny^2 nz^2
+ *   2 hexagonal: 1 + e6 Re (nx + i ny)^6 + ez (3 nz^2 - 1)/2
+ *   3 faceted: 1 + ef (sum sqrt(n_k^2 + nu^2) - sqrt(1 + nu^2) - 2 nu)
+ * (the cubic, cbc / hcp and facet variants of mkPhaseField, MaterKit/inc/mkPhaseField.h:49-64,
+ * whose bodies are not public: these are the published forms named above) */
+double orc_aniso(const orc_params* p, const double* n, double* D) {
+    double x = n[0], y = n[1], z = n[2];
+    double x2 = x * x, y2 = y * y, z2 = z * z;
+    double e1 = p->aniso[0], e2 = p->aniso[1];
+    switch (p->aniso_kind) {
+        case 1: {
+            double Q = x2 * x2 + y2 * y2 + z2 * z2, S = x2 * y2 * z2;
+            D[0] = 4.0 * e1 * x2 * x + e2 * (12.0 * x2 * x + 132.0 * x * y2 * z2);
+            D[1] = 4.0 * e1 * y2 * y + e2 * (12.0 * y2 * y + 132.0 * y * x2 * z2);
+            D[2] = 4.0 * e1 * z2 * z + e2 * (12.0 * z2 * z + 132.0 * z * x2 * y2);
+            return 1.0 + e1 * (Q - 0.6) + e2 * (3.0 * Q + 66.0 * S - 17.0 / 7.0);
+        }
+        case 2: {
+            double H = x2 * x2 * x2 - 15.0 * x2 * x2 * y2 + 15.0 * x2 * y2 * y2 - y2 * y2 * y2;
+            D[0] = e1 * (6.0 * x2 * x2 * x - 60.0 * x2 * x * y2 + 30.0 * x * y2 * y2);
+            D[1] = e1 * (-30.0 * x2 * x2 * y + 60.0 * x2 * y2 * y - 6.0 * y2 * y2 * y);
+            D[2] = e2 * 3.0 * z;
+            return 1.0 + e1 * H + e2 * (1.5 * z2 - 0.5);
+        }
+        case 3: {
+            double v2 = e2 * e2;
+            double r[3] = {sqrt(x2 + v2), sqrt(y2 + v2), sqrt(z2 + v2)};
+            for (int k = 0; k < 3; k++) D[k] = e1 * n[k] / r[k];
+            return 1.0 + e1 * (r[0] + r[1] + r[2] - sqrt(1.0 + v2) - 2.0 * e2);
+        }
+        default: {
+            double e4 = p->eps4;
+            for (int k = 0; k < 3; k++) D[k] = 16.0 * e4 * n[k] * n[k] * n[k];
+            return 1.0 - 3.0 * e4 + 4.0 * e4 * (x2 * x2 + y2 * y2 + z2 * z2);
+        }
+    }
 }
 
-/* Normal component along `axis` of J = a_s^2 grad phi + |grad phi|^2 a_s d a_s / d grad phi,
- * evaluated from a face gradient g (lab frame); R (lab <- crystal) may be NULL. */
+/* Normal component along `axis` of J = a_s^2 grad phi + |grad phi|^2 a_s d a_s / d grad phi
+ * = a^2 G + |G| a (D - n (n.D)), evaluated from a face gradient g (lab frame); R (lab <- crystal)
+ * may be NULL. */
 static double face_flux(const orc_params* p, const double* R, const double* g, int axis) {
     double gc[3];
     if (R) {
@@ -347,10 +384,11 @@ static double face_flux(const orc_params* p, const double* R, const double* g, i
     if (g2 <= 1e-30) return g[axis];
     double gn = sqrt(g2);
     double n[3] = {gc[0] / gn, gc[1] / gn, gc[2] / gn};
-    double s4 = n[0] * n[0] * n[0] * n[0] + n[1] * n[1] * n[1] * n[1] + n[2] * n[2] * n[2] * n[2];
-    double as = 1.0 - 3.0 * p->eps4 + 4.0 * p->eps4 * s4;
+    double D[3];
+    double as = orc_aniso(p, n, D);
+    double nD = n[0] * D[0] + n[1] * D[1] + n[2] * D[2];
     double Nc[3];
-    for (int a = 0; a < 3; a++) Nc[a] = 16.0 * p->eps4 * as * gn * (n[a] * n[a] * n[a] - n[a] * s4);
+    for (int a = 0; a < 3; a++) Nc[a] = as * gn * (D[a] - n[a] * nD);
     double Nl = R ? (R[axis * 3 + 0] * Nc[0] + R[axis * 3 + 1] * Nc[1] + R[axis * 3 + 2] * Nc[2])
                   : Nc[axis];
     return as * as * g[axis] + Nl;
@@ -413,7 +451,8 @@ void orc_phase_source(const orc_params* p, const double* phi27, double upf, int 
     if (gg > 1e-30) {
         double gn = sqrt(gg);
         double n[3] = {grc[0] / gn, grc[1] / gn, grc[2] / gn};
-        a = aniso_as(p->eps4, n);
+        double D[3];
+        a = orc_aniso(p, n, D);
     }
     double ph = P27(0, 0, 0);
     double d = 1.0 - ph * ph;

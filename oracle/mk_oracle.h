@@ -37,6 +37,8 @@ typedef struct orc_params {
     double at_coef;
     double C_face[6];
     double pf_dt, pf_wdx2, eps4, lambda, phi_s;
+    int32_t aniso_kind, pad1;
+    double aniso[2];
     double ref_f, ref_g, ref_h, ref_c;
     double rot[ORC_MAX_GRAINS][9];
 } orc_params;
@@ -87,6 +89,7 @@ void orc_phase_source(const orc_params* p, const double* phi27, double upf, int 
                       double* S_out, double* a_out);
 
 double orc_face_flux(const orc_params* p, const double* R, const double* g, int axis);
+double orc_aniso(const orc_params* p, const double* n, double* D);
 
 /* Reference-structured step (mk_aos.c): the same step as orc_step(mask 15) on AoS node
  * records of the reference's layout, swept operator by operator; for periodic flow (D3Q19) +

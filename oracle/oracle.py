@@ -54,6 +54,8 @@ def lib() -> ct.CDLL:
         L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
         L.orc_face_flux.argtypes = [vp, vp, vp, ct.c_int]
         L.orc_face_flux.restype = ct.c_double
+        L.orc_aniso.argtypes = [vp, vp, vp]
+        L.orc_aniso.restype = ct.c_double
         L.orc_lattice.argtypes = [ct.c_int, vp, vp, vp, vp]
         L.orc_lattice.restype = ct.c_int
         L.orc_splitmix64.argtypes = [ct.POINTER(ct.c_uint64)]
@@ -224,6 +226,14 @@ def face_flux(params, g, axis: int, R=None) -> float:
     Rm = None if R is None else np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
     return lib().orc_face_flux(ct.byref(params), _p(Rm), _p(g), int(axis))
 
+
+
+def aniso(params, n):
+    """a_s(n) and d a_s / d n of the parameter block's anisotropy form at the crystal-frame n."""
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    D = np.zeros(3)
+    a = lib().orc_aniso(ct.byref(params), _p(n), _p(D))
+    return a, D
 
 def splitmix_stream(seed: int, n: int) -> np.ndarray:
     s = ct.c_uint64(seed)

@@ -18,10 +18,13 @@ P, BB, DI, OUT, PR = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_O
 
 
 def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
-                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0, incomp=0):
+                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0, incomp=0, aniso=None):
     p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], flow_q=flow_q, phase_q=phase_q,
                            thermal=thermal, solute=solute, precision=prec, ngrains=ngrains,
                            has_kappa=kappa, flow_incomp=incomp)
+    if aniso is not None:
+        p.aniso_kind = aniso[0]
+        p.aniso[0], p.aniso[1] = aniso[1]
     om = 1.0 / 0.8
     p.s_nu = om
     if op == "bgk":
@@ -85,6 +88,11 @@ CASES = [
     dict(id="pf_thermal_kappa", phase_q=19, thermal=1, kappa=1),
     dict(id="trt19_incomp", flow_q=19, op="trt", incomp=1),
     dict(id="fused_incomp", flow_q=19, phase_q=19, thermal=1, op="mrt", incomp=1),
+    dict(id="pf_thermal_cbc", phase_q=19, thermal=1, aniso=(mkg.ANISO_CBC, (0.04, -0.02))),
+    dict(id="pf_grains_hcp", phase_q=19, thermal=1, ngrains=3,
+         aniso=(mkg.ANISO_HCP, (0.03, -0.05))),
+    dict(id="fused_facet", flow_q=19, phase_q=19, thermal=1, op="trt",
+         aniso=(mkg.ANISO_FACET, (0.06, 0.1))),
 ]
 
 

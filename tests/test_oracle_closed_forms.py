@@ -322,3 +322,101 @@ def test_channel_flux_conservation_of_the_two_equilibria(incomp):
     kept, varied = (vol, mass) if incomp else (mass, vol)
     assert np.ptp(kept) < 1e-10 * kept.mean()
     assert np.ptp(varied) > 0.02 * varied.mean()
+
+
+ANISO_FORMS = {
+    "cubic": (mkg.ANISO_CUBIC, (0.0, 0.0)),
+    "cbc": (mkg.ANISO_CBC, (0.04, -0.02)),
+    "hcp": (mkg.ANISO_HCP, (0.03, -0.05)),
+    "facet": (mkg.ANISO_FACET, (0.06, 0.1)),
+}
+
+
+def _aniso_params(form):
+    kind, par = ANISO_FORMS[form]
+    p = _pf_params(eps4=0.05)
+    p.aniso_kind = kind
+    p.aniso[0], p.aniso[1] = par
+    return p
+
+
+def _sphere(n=40000, seed=2):
+    v = np.random.default_rng(seed).standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("form", list(ANISO_FORMS))
+def test_anisotropy_derivative_and_face_flux_are_exact_gradients(form):
+    """For every form: D = d a_s / d n equals central differences of a_s, and the face flux
+    equals d/dg of |g|^2 a_s(g/|g|)^2 / 2 in the lab frame of a rotated crystal."""
+    p = _aniso_params(form)
+    rng = np.random.default_rng(5)
+    for _ in range(12):
+        n = rng.standard_normal(3)
+        n /= np.linalg.norm(n)
+        a, D = O.aniso(p, n)
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = 1e-6
+            num = (O.aniso(p, n + e)[0] - O.aniso(p, n - e)[0]) / 2e-6
+            assert abs(D[k] - num) < 1e-8
+        g = rng.standard_normal(3)
+        R = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+
+        def energy(gl):
+            gc = R.T @ gl
+            return 0.5 * (gl @ gl) * O.aniso(p, gc / np.linalg.norm(gc))[0] ** 2
+
+        for ax in range(3):
+            e = np.zeros(3)
+            e[ax] = 1e-6
+            num = (energy(g + e) - energy(g - e)) / 2e-6
+            assert abs(O.face_flux(p, g, ax, R) - num) < 1e-7 * max(1.0, abs(num))
+
+
+def test_cubic_harmonic_and_hexagonal_forms():
+    """The two cubic harmonics and the hexagonal / axial terms average to zero over the sphere
+    (<a_s> = 1); the cbc form has the cubic symmetry, the hcp form the six-fold one about z and
+    the mirror z -> -z; at <100> and <111> the cbc form takes its closed values."""
+    pts = _sphere()
+    for form in ("cbc", "hcp"):
+        p = _aniso_params(form)
+        a = np.array([O.aniso(p, n)[0] for n in pts])
+        assert abs(a.mean() - 1.0) < 4 * a.std() / np.sqrt(len(pts))
+    p = _aniso_params("cbc")
+    e1, e2 = ANISO_FORMS["cbc"][1]
+    assert abs(O.aniso(p, [1, 0, 0])[0] - (1 + 0.4 * e1 + e2 * (3 - 17 / 7))) < 1e-15
+    s = 1 / np.sqrt(3)
+    assert abs(O.aniso(p, [s, s, s])[0] - (1 + e1 * (1 / 3 - 0.6) + e2 * (1 + 66 / 27 - 17 / 7))) < 1e-14
+    rng = np.random.default_rng(3)
+    for n in rng.standard_normal((20, 3)):
+        n /= np.linalg.norm(n)
+        a = O.aniso(p, n)[0]
+        for perm in ([1, 2, 0], [2, 0, 1], [1, 0, 2]):
+            assert abs(O.aniso(p, n[perm] * [1, -1, 1])[0] - a) < 1e-14
+    p = _aniso_params("hcp")
+    c, sn = np.cos(np.pi / 3), np.sin(np.pi / 3)
+    R60 = np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]])
+    for n in rng.standard_normal((20, 3)):
+        n /= np.linalg.norm(n)
+        a = O.aniso(p, n)[0]
+        assert abs(O.aniso(p, R60 @ n)[0] - a) < 1e-14
+        assert abs(O.aniso(p, n * [1, 1, -1])[0] - a) < 1e-15
+    # the six-fold term: cos(6 phi) in the basal plane
+    for phi in np.linspace(0, 1, 7):
+        assert abs(O.aniso(p, [np.cos(phi), np.sin(phi), 0])[0] -
+                   (1 + 0.03 * np.cos(6 * phi) + 0.05 / 2)) < 1e-14
+
+
+def test_facet_form_has_rounded_cusps_at_the_faces():
+    """Facet form: a_s = 1 at <100> (its minimum), and away from the cusp it follows the
+    unrounded 1 + ef (|nx| + |ny| + |nz| - 1) to O(nu)."""
+    p = _aniso_params("facet")
+    ef, nu = ANISO_FORMS["facet"][1]
+    assert abs(O.aniso(p, [1, 0, 0])[0] - 1.0) < 1e-15
+    pts = _sphere(2000)
+    a = np.array([O.aniso(p, n)[0] for n in pts])
+    assert a.min() > 1.0 - 1e-12
+    far = np.abs(pts).min(axis=1) > 0.3
+    sharp = 1 + ef * (np.abs(pts).sum(axis=1) - 1)
+    assert np.abs(a[far] - sharp[far]).max() < 2 * ef * nu
