@@ -203,6 +203,10 @@ int validate(const mkg_params* p) {
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail("grid dimensions must be >= 1");
     double n = (double)p->nx * p->ny * p->nz;
     if (n > 2.0e9) return fail("grid too large for 32-bit cell indices");
+    // kernels address a slot array with 32-bit byte offsets
+    const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
+    if ((n + 65536.0) * esz >= 4294967296.0)
+        return fail("grid too large: one population slot (cells x element size) must stay below 4 GiB");
     if (p->flow_q != 0 && p->flow_q != 19 && p->flow_q != 27)
         return fail("flow_q must be 0, 19 or 27");
     if (p->phase_q != 0 && p->phase_q != 7 && p->phase_q != 19)

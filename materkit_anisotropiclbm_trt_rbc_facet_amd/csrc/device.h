@@ -148,6 +148,19 @@ __device__ inline void stp(R* p, R v) {
 #endif
 }
 
+// Element idx of slot array `slot`.  The slot base is uniform (scalar registers) and the byte
+// offset a 32-bit vector value, which global loads and stores take as their address directly,
+// with no 64-bit address arithmetic per access.  The host keeps every slot array below 4 GiB
+// (cell count times element size < 2^32).
+template <class R>
+__device__ inline R* slot_at(R* slot, uint32_t idx) {
+    return (R*)((char*)slot + idx * (uint32_t)sizeof(R));
+}
+template <class R>
+__device__ inline const R* slot_at(const R* slot, uint32_t idx) {
+    return (const R*)((const char*)slot + idx * (uint32_t)sizeof(R));
+}
+
 // Position of a face cell in the face buffer: faces x-, x+ (ny*nz cells each, index
 // y*nz + z), then y-, y+ (x*nz + z), then z-, z+ (x*ny + y).
 __device__ inline uint32_t face_slot(const StepArgs& a, const Cell& c, int f) {
@@ -206,25 +219,25 @@ __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const St
 //   - crossing out in an E step, the E step stores the returning population in the face
 //     buffer, which the face cell alone reads in the next O step.  Its own slots cannot hold
 //     it: the interior neighbours overwrite them during the O step.
-template <int Q, bool ODD, class R>
+template <int Q, bool ODD, class R, bool FACES = true>
 __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                  const StepArgs& a, int kind) {
     const R* __restrict__ base = (const R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) p[i] = ldp(&base[(size_t)i * S + c.id]);
+        for (int i = 0; i < Q; i++) p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
     } else {
         (void)kind;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
-            uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
             if (hit == 0u) {
                 uint32_t src = nbr(c, dx, dy, dz);
-                if (a.has_walls && i != 0 && a.flag[src] == kIsWalls)
-                    p[i] = ldp(&base[(size_t)i * S + c.id]);
+                if (FACES && a.has_walls && i != 0 && a.flag[src] == kIsWalls)
+                    p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
                 else
-                    p[i] = ldp(&base[(size_t)inv(i) * S + src]);
+                    p[i] = ldp(slot_at(base + (size_t)inv(i) * S, src));
             } else {
                 const int face = __builtin_ctz(hit);
                 p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, face)]);
@@ -233,15 +246,15 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
     }
 }
 
-template <int Q, bool ODD, class R>
+template <int Q, bool ODD, class R, bool FACES = true>
 __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                   const StepArgs& a, int kind,
                                   const FaceState<R>& fs = FaceState<R>()) {
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) stp(&base[(size_t)inv(i) * S + c.id], p[i]);
-        if (c.bits & (uint32_t)L.npmask) {
+        for (int i = 0; i < Q; i++) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
+        if (FACES && (c.bits & (uint32_t)L.npmask)) {
 #pragma unroll
             for (int i = 1; i < Q; i++) {
                 const uint32_t hit =
@@ -257,16 +270,16 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int dx = cx(i), dy = cy(i), dz = cz(i);
-            uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
             if (hit == 0u) {
                 uint32_t tgt = nbr(c, dx, dy, dz);
-                if (a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
-                    stp(&base[(size_t)inv(i) * S + c.id], p[i]);
+                if (FACES && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
+                    stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
                 else
-                    stp(&base[(size_t)i * S + tgt], p[i]);
+                    stp(slot_at(base + (size_t)i * S, tgt), p[i]);
             } else {
                 const int face = __builtin_ctz(hit);
-                stp(&base[(size_t)inv(i) * S + c.id],
+                stp(slot_at(base + (size_t)inv(i) * S, c.id),
                     face_in<Q, R>(p, inv(i), face, L, a, kind, fs));
             }
         }
@@ -294,43 +307,57 @@ __host__ __device__ constexpr int nzdirs() {
     return zslot<Q>(Q);
 }
 
-// Pull of an O step.  Every thread of the block calls it (one barrier inside).  All global
-// loads are issued before the barrier: first the aligned loads that feed the stage, then the
-// direct ones (c_z == 0) and the face / wall values, so that one memory round trip serves the
-// lattice.  zv must not be in use by another stage: the caller gives every pull and push its
-// own region or separates them by a barrier.
-template <int Q, class R, int ZN>
-__device__ inline void read_pops_z(R* p, const LatArgs& L, uint32_t S, const Cell& c,
-                                   const StepArgs& a, R (*zv)[256], uint32_t tid) {
+// The pull and the push of an O step come in two halves each, separated by a block barrier
+// that the caller places, so that several lattices share one barrier:
+//   zpull_issue   issues every global load of the lattice: the aligned loads that feed the
+//                 stage (stored to zv), the direct ones (c_z == 0) and the face / wall values;
+//                 returns the mask of z directions whose value does not come through the stage
+//   zpull_finish  (after a barrier) takes the shifted values from zv, wrapping at the row end
+//   zpush_stage   stores the c_z == 0 populations and the face / wall returns directly, and
+//                 puts the z populations and a per-thread "sent normally" mask in LDS
+//   zpush_finish  (after a barrier) the thread of the target's row position (x+cx, y+cy, z)
+//                 stores each z population aligned, finding it in the sender's stage slot
+// Every thread of the block runs all four (act = the thread holds a live, non-wall cell).  A
+// stage region must not be written again before every thread has passed a barrier that
+// follows its last read.  Global memory needs no ordering: each (cell, slot) is loaded and
+// stored by one thread.
+template <int Q, class R, bool FACES = true>
+__device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                                       const StepArgs& a, R (*zv)[256], uint32_t tid) {
     constexpr int NZ = nzdirs<Q>();
-    static_assert(NZ <= ZN && NZ <= 32, "LDS stage too small");
+    static_assert(NZ <= 32, "z mask too small");
     const R* __restrict__ base = (const R*)L.pop;
     R t[NZ];
 #pragma unroll
     for (int i = 1; i < Q; i++)
         if (cz(i) != 0)
-            t[zslot<Q>(i)] = ldp(&base[(size_t)inv(i) * S + nbr(c, -cx(i), -cy(i), 0)]);
-    uint32_t own = 0;  // z directions whose value does not come through the stage
+            t[zslot<Q>(i)] = ldp(slot_at(base + (size_t)inv(i) * S, nbr(c, -cx(i), -cy(i), 0)));
+    uint32_t own = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
-        const uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+        const uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
         if (hit) {
             p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, __builtin_ctz(hit))]);
             if (dz != 0) own |= 1u << zslot<Q>(i);
         } else {
             const uint32_t src = nbr(c, dx, dy, dz);
-            if (a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
-                p[i] = ldp(&base[(size_t)i * S + c.id]);
+            if (FACES && a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
+                p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
                 if (dz != 0) own |= 1u << zslot<Q>(i);
             } else if (dz == 0) {
-                p[i] = ldp(&base[(size_t)inv(i) * S + src]);
+                p[i] = ldp(slot_at(base + (size_t)inv(i) * S, src));
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < NZ; k++) zv[k][tid] = t[k];
-    __syncthreads();
+    return own;
+}
+
+template <int Q, class R>
+__device__ inline void zpull_finish(R* p, const Cell& c, const StepArgs& a, R (*zv)[256],
+                                    uint32_t tid, uint32_t own) {
     const uint32_t zm1 = (uint32_t)a.nz - 1u;
     const uint32_t rb = tid - (tid & zm1);  // first thread of this z row
 #pragma unroll
@@ -339,39 +366,41 @@ __device__ inline void read_pops_z(R* p, const LatArgs& L, uint32_t S, const Cel
             p[i] = zv[zslot<Q>(i)][rb + ((uint32_t)(c.z - cz(i)) & zm1)];
 }
 
-// Push of an O step.  Every thread of the block calls it (one barrier inside); act = the
-// thread holds a live, non-wall cell.  A population moving along z is stored by the thread
-// of the target's row position (x+cx, y+cy, z), which finds it in the sender's stage slot and
-// its bit in the sender's mask.  zv and zm follow the same rule as in read_pops_z.
-template <int Q, class R, int ZN>
-__device__ inline void write_pops_z(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
-                                    const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
-                                    uint32_t tid, bool act, const FaceState<R>& fs) {
-    static_assert(nzdirs<Q>() <= ZN && nzdirs<Q>() <= 32, "LDS stage too small");
+template <int Q, class R, bool FACES = true>
+__device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
+                                   const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
+                                   uint32_t tid, bool act, const FaceState<R>& fs) {
+    static_assert(nzdirs<Q>() <= 32, "z mask too small");
     R* __restrict__ base = (R*)L.pop;
     uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int dx = cx(i), dy = cy(i), dz = cz(i);
-        const uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+        const uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
         if (dz != 0) zv[zslot<Q>(i)][tid] = p[i];
         if (hit) {
             if (act)
-                stp(&base[(size_t)inv(i) * S + c.id],
+                stp(slot_at(base + (size_t)inv(i) * S, c.id),
                     face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs));
         } else {
             const uint32_t tgt = nbr(c, dx, dy, dz);
-            if (a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
-                if (act) stp(&base[(size_t)inv(i) * S + c.id], p[i]);
+            if (FACES && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
+                if (act) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
             } else if (dz == 0) {
-                if (act) stp(&base[(size_t)i * S + tgt], p[i]);
+                if (act) stp(slot_at(base + (size_t)i * S, tgt), p[i]);
             } else {
                 m |= 1u << zslot<Q>(i);
             }
         }
     }
     zm[tid] = act ? m : 0u;
-    __syncthreads();
+}
+
+template <int Q, class R>
+__device__ inline void zpush_finish(const LatArgs& L, uint32_t S, const Cell& c,
+                                    const StepArgs& a, R (*zv)[256], const uint32_t* zm,
+                                    uint32_t tid) {
+    R* __restrict__ base = (R*)L.pop;
     const uint32_t zm1 = (uint32_t)a.nz - 1u;
     const uint32_t rb = tid - (tid & zm1);
 #pragma unroll
@@ -380,7 +409,7 @@ __device__ inline void write_pops_z(const R* p, const LatArgs& L, uint32_t S, co
             // sender: the cell (x, y, z - c_z) of this thread's row
             const uint32_t s = rb + ((uint32_t)(c.z - cz(i)) & zm1);
             if ((zm[s] >> zslot<Q>(i)) & 1u)
-                stp(&base[(size_t)i * S + nbr(c, cx(i), cy(i), 0)], zv[zslot<Q>(i)][s]);
+                stp(slot_at(base + (size_t)i * S, nbr(c, cx(i), cy(i), 0)), zv[zslot<Q>(i)][s]);
         }
     }
 }

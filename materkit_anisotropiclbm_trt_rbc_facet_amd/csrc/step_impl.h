@@ -13,6 +13,8 @@
 //                    (the per-field propagate()/relaxation() pairs of MaterKit.cpp:500-518)
 // Without a phase lattice only k_step runs.
 #pragma once
+#include <type_traits>
+
 #include "device.h"
 
 namespace mkg {
@@ -24,13 +26,13 @@ __device__ inline R clamp01(R v) {
 
 // 19-point phase stencil around the cell (periodic faces wrap, other faces replicate the
 // edge value), index (dx+1)*9 + (dy+1)*3 + (dz+1).
-template <class R>
+template <class R, bool FACES = true>
 __device__ inline void load_phase_stencil(const StepArgs& a, const Cell& c, R (&P)[27]) {
     const R* __restrict__ ph = (const R*)a.phi_cur;
     uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
     uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
     uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
-    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    const uint32_t np = FACES ? (uint32_t)a.g.npmask & c.bits : 0u;
     if (np & XLO) px[0] = px[1];
     if (np & XHI) px[2] = px[1];
     if (np & YLO) py[0] = py[1];
@@ -40,7 +42,7 @@ __device__ inline void load_phase_stencil(const StepArgs& a, const Cell& c, R (&
 #pragma unroll
     for (int i = 0; i < 19; i++)
         P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
-            ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+            *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
 }
 
 // Isotropic LB-weighted gradient of the stencil (D3Q19 weights over cs^2 = 1/3).
@@ -85,7 +87,8 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
 #define MKG_STEP_WAVES_F32 4
 #endif
 
-// LDS slots of the z stage shared by the flow and phase lattices (one after the other).
+// LDS slots of the z stage: flow pull region (reused by the phase push), flow push region,
+// phase pull region.
 template <int FQ, int PQ>
 constexpr int zstage_slots() {
     int n = 1;
@@ -95,44 +98,50 @@ constexpr int zstage_slots() {
     if (PQ == 7 && nzdirs<7>() > n) n = nzdirs<7>();
     return n;
 }
-
-// Population access of one lattice in k_step: E step, O step, or O step with the z stage.
-template <int Q, bool ODD, bool Z, class R, int ZN>
-__device__ inline void pull(R* p, const LatArgs& L, uint32_t S, const Cell& c, const StepArgs& a,
-                            int kind, R (*zv)[256], uint32_t tid) {
-    if constexpr (Z)
-        read_pops_z<Q, R, ZN>(p, L, S, c, a, zv, tid);
-    else
-        read_pops<Q, ODD, R>(p, L, S, c, a, kind);
-}
-template <int Q, bool ODD, bool Z, class R, int ZN>
-__device__ inline void push(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
-                            const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
-                            uint32_t tid, bool act, const FaceState<R>& fs) {
-    if constexpr (Z)
-        write_pops_z<Q, R, ZN>(p, L, S, c, a, kind, zv, zm, tid, act, fs);
-    else
-        write_pops<Q, ODD, R>(p, L, S, c, a, kind, fs);
+template <int Q>
+constexpr int zslots_of() {
+    return Q == 19 ? nzdirs<19>() : (Q == 27 ? nzdirs<27>() : (Q == 7 ? nzdirs<7>() : 1));
 }
 
 // ZS: O step with the z stage (only instantiated with ODD; the block must hold whole z rows).
 // With it every thread stays to the end (barriers), and act marks the threads that update.
-// Each pull and push has its own LDS region, except that the phase lattice reuses the flow's
-// regions: every thread has passed the flow push's barrier (after its last read of the flow
-// pull region) and the phase pull's barrier (after its last read of the flow push region)
-// before it writes them again.
+// The z stage runs in three block barriers:
+//   B1  after the pull loads of the thermal, solute and flow lattices
+//   B2  after the flow / thermal / solute push stages and the phase pull loads (which are
+//       issued after the phase source, so that the stencil values are dead by then)
+//   B3  after the phase push stage, which reuses the flow pull region (last read before B2)
+// Every other stage region is written once and read once, on either side of one barrier.
 template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     k_step(const StepArgs a) {
     constexpr bool Z = ODD && ZS;
-    constexpr int ZN = Z ? zstage_slots<FQ, PQ>() : 1;
+    constexpr int ZF = Z ? zstage_slots<FQ, PQ>() : 1;
+    constexpr int ZW = (Z && FQ > 0) ? zslots_of<FQ>() : 1;
+    constexpr int ZG = (Z && PQ > 0) ? zslots_of<PQ>() : 1;
     constexpr int ZH = (Z && TH) ? nzdirs<7>() : 1;
     constexpr int ZC = (Z && SO) ? nzdirs<7>() : 1;
-    __shared__ R zrf[ZN][256], zwf[ZN][256];  // flow, then phase: pull / push
+    __shared__ R zrf[ZF][256], zwf[ZW][256], zrg[ZG][256];  // flow pull (phase push), flow push, phase pull
     __shared__ R zrh[ZH][256], zwh[ZH][256];  // thermal
     __shared__ R zrc[ZC][256], zwc[ZC][256];  // solute
-    __shared__ uint32_t zmf[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
+    __shared__ uint32_t zmf[(Z && FQ > 0) ? 256 : 1], zmg[(Z && PQ > 0) ? 256 : 1],
+        zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
     const uint32_t tid = threadIdx.x;
+    // Blocks (z stage) or waves (otherwise) none of whose cells lies on a non-periodic face,
+    // in a domain without wall cells, take the body without face rules (FACES false).
+    bool slow = a.has_walls != 0;
+    if (!slow) {
+        const uint32_t t = thread_cell(a);
+        Cell e;
+        locate_at(a, t < a.N ? t : a.N - 1u, e);
+        const uint32_t np = (uint32_t)(a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask);
+        const int face = (e.bits & np) != 0u;
+        if constexpr (Z)
+            slow = __syncthreads_or(face) != 0;
+        else
+            slow = __ballot(face) != 0ull;
+    }
+    auto body = [&](auto faces_tag) {
+    constexpr bool FACES = decltype(faces_tag)::value;
     Cell c;
     bool act = true;
     if constexpr (Z) {
@@ -150,33 +159,62 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
 
     R phi = (R)-1, dphi = (R)0, ns = (R)0;
     if constexpr (PQ > 0) {
-        phi = ((const R*)a.phi_cur)[c.id];
-        dphi = phi - ((const R*)a.phi_prev)[c.id];
+        phi = *slot_at((const R*)a.phi_cur, c.id);
+        dphi = phi - *slot_at((const R*)a.phi_prev, c.id);
         if (fl == kIsSolid)
             ns = (R)1;
         else if (fl == kIsInter)
             ns = clamp01((R)0.5 * ((R)1 + phi));
     }
 
-    // ---- scalar moments at time level n
+    // ---- population loads at time level n: thermal, solute, flow
     R h[TH ? 7 : 1];
+    R cc[SO ? 7 : 1];
+    R f[FQ > 0 ? FQ : 1];
+    uint32_t own_h = 0, own_c = 0, own_f = 0;
+    if constexpr (TH) {
+        if (!a.h.adv)
+            read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
+        else if constexpr (Z)
+            own_h = zpull_issue<7, R, FACES>(h, a.h, S, c, a, zrh, tid);
+        else
+            read_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
+    }
+    if constexpr (SO) {
+        if (!a.c.adv)
+            read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
+        else if constexpr (Z)
+            own_c = zpull_issue<7, R, FACES>(cc, a.c, S, c, a, zrc, tid);
+        else
+            read_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar);
+    }
+    if constexpr (FQ > 0) {
+        if (!a.f.adv)
+            read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
+        else if constexpr (Z)
+            own_f = zpull_issue<FQ, R, FACES>(f, a.f, S, c, a, zrf, tid);
+        else
+            read_pops<FQ, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow);
+    }
+    if constexpr (Z) {
+        __syncthreads();  // B1
+        if constexpr (TH)
+            if (a.h.adv) zpull_finish<7, R>(h, c, a, zrh, tid, own_h);
+        if constexpr (SO)
+            if (a.c.adv) zpull_finish<7, R>(cc, c, a, zrc, tid, own_c);
+        if constexpr (FQ > 0)
+            if (a.f.adv) zpull_finish<FQ, R>(f, c, a, zrf, tid, own_f);
+    }
+
+    // ---- scalar moments at time level n
     R sh = 0, T = 0;
     if constexpr (TH) {
-        if (a.h.adv)
-            pull<7, ODD, Z, R, ZH>(h, a.h, S, c, a, kKindScalar, zrh, tid);
-        else
-            read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
 #pragma unroll
         for (int i = 0; i < 7; i++) sh += h[i];
         T = (R)a.h.ref + sh;
     }
-    R cc[SO ? 7 : 1];
     R sc = 0, C = 0, U = 0;
     if constexpr (SO) {
-        if (a.c.adv)
-            pull<7, ODD, Z, R, ZC>(cc, a.c, S, c, a, kKindScalar, zrc, tid);
-        else
-            read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
 #pragma unroll
         for (int i = 0; i < 7; i++) sc += cc[i];
         C = (R)a.c.ref + sc;
@@ -187,11 +225,6 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     // ---- flow
     R ux = 0, uy = 0, uz = 0;
     if constexpr (FQ > 0) {
-        R f[FQ];
-        if (a.f.adv)
-            pull<FQ, ODD, Z, R, ZN>(f, a.f, S, c, a, kKindFlow, zrf, tid);
-        else
-            read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
         R dr = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < FQ; i++) {
@@ -221,7 +254,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             fs.ux = ux;
             fs.uy = uy;
             fs.uz = uz;
-            push<FQ, ODD, Z, R, ZN>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+            if constexpr (Z)
+                zpush_stage<FQ, R, FACES>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+            else
+                write_pops<FQ, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow, fs);
         }
         const R om = (R)1 - ns;
         ux *= om;
@@ -244,8 +280,11 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                 const R heq = w * (sh + T * cu / (R)0.25);
                 h[i] = h[i] - om * (h[i] - heq) + w * src;
             }
-            push<7, ODD, Z, R, ZH>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
-                                   FaceState<R>());
+            if constexpr (Z)
+                zpush_stage<7, R, FACES>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+                                  FaceState<R>());
+            else
+                write_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
         }
     }
 
@@ -253,7 +292,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     //      c = A(phi) U obeys dc/dt + div(c u) = div(D q grad U), written as an
     //      advection-diffusion of c with D_c = D q / A and the drift V = D q grad A / A^2
     R P[27];
-    if constexpr (SO && PQ > 0) load_phase_stencil<R>(a, c, P);
+    if constexpr (SO && PQ > 0) load_phase_stencil<R, FACES>(a, c, P);
     if constexpr (SO) {
         if (a.c.adv) {
             R gphi[3] = {0, 0, 0};
@@ -289,15 +328,21 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             }
             FaceState<R> fs;
             fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
-            push<7, ODD, Z, R, ZC>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+            if constexpr (Z)
+                zpush_stage<7, R, FACES>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+            else
+                write_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar, fs);
         }
     }
 
     // ---- phase field collide-stream (mkPhaseField::propagate +
     //      relaxation_anisotropy_thermal / _solute)
+    R g[PQ > 0 ? PQ : 1];
+    R Sg = 0;
+    uint32_t own_g = 0;
     if constexpr (PQ > 0) {
         if (a.g.adv) {
-            if constexpr (!SO) load_phase_stencil<R>(a, c, P);
+            if constexpr (!SO) load_phase_stencil<R, FACES>(a, c, P);
             R upf = 0;
             if (a.pf_coupling == 1) upf = T;
             if (a.pf_coupling == 2) upf = U;
@@ -313,11 +358,29 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             }
             R as;
             const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
-            const R Sg = phase_source<R>(P, upf, an, (R)a.lambda, (R)a.pf_dt,
-                                         (R)a.pf_wdx2, Mrot, rot, as);
+            Sg = phase_source<R>(P, upf, an, (R)a.lambda, (R)a.pf_dt, (R)a.pf_wdx2, Mrot, rot,
+                                 as);
             // the populations are read after the source so that the stencil values are dead
-            R g[PQ];
-            pull<PQ, ODD, Z, R, ZN>(g, a.g, S, c, a, kKindScalar, zrf, tid);
+            // (loaded with the other lattices, they measured 13 % slower at 256^3)
+            if constexpr (Z)
+                own_g = zpull_issue<PQ, R, FACES>(g, a.g, S, c, a, zrg, tid);
+            else
+                read_pops<PQ, ODD, R, FACES>(g, a.g, S, c, a, kKindScalar);
+        }
+    }
+    if constexpr (Z) {
+        __syncthreads();  // B2
+        if constexpr (FQ > 0)
+            if (a.f.adv) zpush_finish<FQ, R>(a.f, S, c, a, zwf, zmf, tid);
+        if constexpr (TH)
+            if (a.h.adv) zpush_finish<7, R>(a.h, S, c, a, zwh, zmh, tid);
+        if constexpr (SO)
+            if (a.c.adv) zpush_finish<7, R>(a.c, S, c, a, zwc, zmc, tid);
+        if constexpr (PQ > 0)
+            if (a.g.adv) zpull_finish<PQ, R>(g, c, a, zrg, tid, own_g);
+    }
+    if constexpr (PQ > 0) {
+        if (a.g.adv) {
             const R om = (R)a.omega_g;
             const R dp = phi - (R)a.g.ref;
             // advective flux (phi + 1) u: the liquid is invariant under any melt velocity
@@ -329,10 +392,22 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                 const R geq = w * (dp + pa * cu);
                 g[i] = g[i] - om * (g[i] - geq) + w * Sg;
             }
-            push<PQ, ODD, Z, R, ZN>(g, a.g, S, c, a, kKindScalar, zwf, zmf, tid, act,
-                                    FaceState<R>());
+            if constexpr (Z)
+                zpush_stage<PQ, R, FACES>(g, a.g, S, c, a, kKindScalar, zrf, zmg, tid, act,
+                                   FaceState<R>());
+            else
+                write_pops<PQ, ODD, R, FACES>(g, a.g, S, c, a, kKindScalar);
+        }
+        if constexpr (Z) {
+            __syncthreads();  // B3
+            if (a.g.adv) zpush_finish<PQ, R>(a.g, S, c, a, zrf, zmg, tid);
         }
     }
+    };
+    if (slow)
+        body(std::true_type{});
+    else
+        body(std::false_type{});
 }
 
 // Host-side launch helpers for one storage type.
