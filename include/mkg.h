@@ -240,6 +240,59 @@ int mkg_enable_timing(mkg_handle h, int on);
 int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* phi_target, double* loss,
                     double* dL_deps4, double* dL_dkappa, double* times);
 
+/* ---- Multi-phase field ---------------------------------------------------------------
+ * n order parameters phi_j in [-1, 1], each with its own D3Q19 population lattice, plus an
+ * optional D3Q7 thermal lattice: the per-grain phase lattices (gVec, phiVec) of
+ * mkFluidsUser::propagate / relaxation (MaterKit/src/mkFluidsUser.cpp:68-134, 136-299; the
+ * node state at MaterKit/inc/mkNode.h:158-164, phase_number at MaterKit/inc/mkMultiphaseField.h:56)
+ * as driven by run_multiPhaseField (MaterKit/MaterKit.cpp:529-576).  Each order parameter
+ * follows a conservative Allen-Cahn LB (Chai et al. 2018, the getAddedPhasePDFs term) with
+ * its own advection velocity and the pairwise interaction term of the template; the combined
+ * field phi = (n - 1) + sum_j phi_j is kept beside them.  One handle owns its device memory
+ * and one HIP stream. */
+#define MKG_MPF_MAX 4
+typedef struct mkg_mpf_params {
+    int32_t nx, ny, nz;
+    int32_t n;                  /* order parameters, 1..MKG_MPF_MAX */
+    int32_t precision;          /* MKG_PREC_FP64 or MKG_PREC_FP32 (deviation storage) */
+    int32_t thermal;            /* 1: D3Q7 thermal lattice */
+    int32_t bc[6], bc_h[6];     /* face codes of the phase lattices / the thermal lattice */
+    double dt;                  /* dt / tau0 */
+    double wdx;                 /* W0 / dx */
+    double u[MKG_MPF_MAX][3];   /* advection velocity of each order parameter (template:
+                                   0.02 in x for j = 1, MaterKit/src/mkFluidsUser.cpp:219-230) */
+    double kappa[MKG_MPF_MAX];  /* interaction strength (template: 0 for j = 0, else 1, :250-255) */
+    double growth;              /* coefficient of the growth term (template: 0, :266-271) */
+    double lambda;              /* coupling constant of the growth term */
+    double eps;                 /* normal regularisation n = s / (|s| + eps) */
+    double alpha, latent_T, T_ref;
+    double T_face[6];
+    double phi_face[6];         /* Dirichlet value of the phase lattices */
+} mkg_mpf_params;
+
+typedef struct mkg_mpf_ctx* mkg_mpf_handle;
+
+#define MKG_MPF_POPS 0      /* n x 19 x N pre-collision populations (canonical, doubles) */
+#define MKG_MPF_PHI 1       /* (n + 1) x N: phi_j at the last moment pass, then combined phi */
+#define MKG_MPF_PHI_PREV 2  /* (n + 1) x N: the pass before */
+#define MKG_MPF_THERMAL 3   /* 7 x N thermal populations (doubles) */
+#define MKG_MPF_FLAG 4      /* N int8: 0 fluid, 3 wall (half-way bounce-back) */
+
+size_t mkg_mpf_params_size(void);
+int mkg_mpf_params_default(mkg_mpf_params* m);
+int mkg_mpf_create(const mkg_mpf_params* m, mkg_mpf_handle* out);
+int mkg_mpf_destroy(mkg_mpf_handle h);
+/* Equilibrium start: phi n x N order parameters, T N (NULL: T_ref), both doubles. */
+int mkg_mpf_init(mkg_mpf_handle h, const double* phi, const double* T);
+/* nsteps steps: phase moment pass, then collide-stream of every lattice (fused). */
+int mkg_mpf_step(mkg_mpf_handle h, int nsteps);
+int mkg_mpf_sync(mkg_mpf_handle h);
+/* Read / write a field (MKG_MPF_*); count is the element count of the host buffer. */
+int mkg_mpf_get(mkg_mpf_handle h, int which, void* host, size_t count);
+int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count);
+/* FP64 sums over the non-wall cells of phi_j (n values) and of the combined phi. */
+int mkg_mpf_sums(mkg_mpf_handle h, double* out);
+
 #ifdef __cplusplus
 }
 #endif

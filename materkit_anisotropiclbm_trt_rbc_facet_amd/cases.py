@@ -164,3 +164,46 @@ def seed_phi(case: Case) -> np.ndarray:
         d = np.sqrt((X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2 + (Z - c[k, 2]) ** 2) - r[k]
         best = np.minimum(best, d)
     return (-np.tanh(best / width)).reshape(-1)
+
+
+# ---------------------------------------------------------------- multi-phase field
+NAMES["MPF"] = "multi_phase_field.xml"
+
+
+def mpf_seed_fields(dims, centers, radii, wdx: float) -> np.ndarray:
+    """One order parameter per seed: phi_j = -tanh((|x - c_j| - r_j) / (sqrt 2 W0/dx)), the
+    equilibrium profile of the conservative Allen-Cahn equation (+1 inside).  Returns
+    n x N values, cell index (x*ny + y)*nz + z."""
+    nx, ny, nz = (int(d) for d in dims)
+    X, Y, Z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    width = math.sqrt(2.0) * wdx
+    out = []
+    for c, r in zip(np.asarray(centers, dtype=float).reshape(-1, 3), np.asarray(radii).reshape(-1)):
+        d = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r
+        out.append(-np.tanh(d / width).reshape(-1))
+    return np.stack(out)
+
+
+def load_mpf(name: str = "MPF", precision: int = mkg.PREC_FP64, **overrides):
+    """Multi-phase-field case from a BoltzTechs XML file (run_multiPhaseField): grid, dx, dt,
+    W0 and the seeds (one order parameter per seed).  Returns (MpfParams, phi n x N).  The
+    order parameters' velocities and interaction strengths are those of the
+    mkFluidsUser::relaxation template (u_1 = 0.02 in x, kappa_0 = 0, kappa_j = 1)."""
+    path = Path(NAMES.get(name, name))
+    if not path.is_absolute():
+        path = CASES_DIR / path
+    p, info = mkg.params_from_xml(path)
+    centers, radii, _ = mkg.seeds_from_xml(path)
+    n = len(radii)
+    if not 1 <= n <= mkg.MPF_MAX:
+        raise mkg.MkgError(f"{path}: {n} seeds; the multi-phase field holds 1..{mkg.MPF_MAX}")
+    m = mkg.default_mpf_params(nx=p.nx, ny=p.ny, nz=p.nz, n=n, precision=precision,
+                               dt=p.pf_dt, wdx=math.sqrt(p.pf_wdx2))
+    m.thermal = 1 if p.thermal else 0
+    if m.thermal:
+        m.alpha = p.alpha_l
+        m.latent_T = p.latent_T
+        m.T_ref = p.T_ref
+    m.set(**overrides)
+    phi = mpf_seed_fields((m.nx, m.ny, m.nz), centers, radii, m.wdx)
+    return m, phi

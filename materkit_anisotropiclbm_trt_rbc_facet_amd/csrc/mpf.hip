@@ -1,0 +1,627 @@
+// Multi-phase field: n order parameters phi_j in [-1, 1], each with its own D3Q19 population
+// lattice, plus an optional D3Q7 thermal lattice -- the per-grain lattices gVec / phiVec of
+// mkFluidsUser::propagate / relaxation (MaterKit/src/mkFluidsUser.cpp:68-134, 136-299) as run
+// by run_multiPhaseField (MaterKit/MaterKit.cpp:529-576).  Same AA-pattern storage, face rules
+// and wall handling as the fused solver step (device.h).
+//
+// One step is two launches:
+//   k_mpf_moments  phi_j = sum_i g_j,i for every cell and the combined phi = (n-1) + sum phi_j
+//                  (the first sweep of mkFluidsUser::relaxation, :147-172); the gradient
+//                  stencils of the collision read the neighbours' phi_j
+//   k_mpf_step     per cell: thermal collide-stream (latent source from the combined dphi), then
+//                  for every order parameter: LB-weighted gradient s_j = sum_i c_i w_i phi_j(x+c_i)
+//                  and normal n_j = s_j / (|s_j| + eps) (:208-217), equilibrium with phi_j as the
+//                  density and the order parameter's own velocity (getEquPDFs, :219-230), the
+//                  interaction P_j = -2 kappa_j (phi_j/2 + 1/2) sum_{k != j} (phi_k/2 + 1/2)^2
+//                  (:236-256), the growth term Q_j (:266-271), the conservative Allen-Cahn
+//                  counter-term (1 - omega/2) w_i (c_i . n_j) (1 - phi_j^2) dx / (sqrt 2 W0)
+//                  (getAddedPhasePDFs, Chai et al. 2018, :273-279), BGK with
+//                  omega_g = 1/(dt/dx^2/cs^2 + 1/2) (:282-293), then streaming.
+// Cells are not re-classified: a cell other than fluid does not collide in the template
+// (:185-188), so classifying interface cells would freeze every grain.  Flag 3 (walls) is the
+// only non-fluid code used here.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/mkg.h"
+#include "device.h"
+
+namespace mkg {
+
+void set_error(const std::string& msg);
+
+struct MpfArgs {
+    StepArgs a;    // geometry, flags and walls; a.g = lattice of order parameter 0, a.h thermal
+    uint64_t lat;  // elements from one order parameter's lattice to the next: 19 (S + F)
+    int32_t n, thermal;
+    double u[MKG_MPF_MAX][3];
+    double kappa[MKG_MPF_MAX];
+    double growth, lambda, eps;
+    double omega_g;  // 1 / (dt / wdx^-2 / cs^2 + 1/2)
+    double zc;       // (1 - omega_g / 2) / (sqrt 2 W0/dx): scale of the Allen-Cahn counter-term
+    double dt, omega_h, latent_T, T_ref;
+};
+
+template <class R>
+__device__ inline LatArgs lattice_of(const MpfArgs& m, int j) {
+    LatArgs L = m.a.g;
+    L.pop = (R*)m.a.g.pop + (size_t)j * m.lat;
+    return L;
+}
+
+template <class R, bool ODD>
+__global__ void __launch_bounds__(256) k_mpf_moments(const MpfArgs m, R* __restrict__ out) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    const uint32_t S = m.a.S;
+    const R* __restrict__ prev = (const R*)m.a.phi_prev;
+    if (m.a.flag[c.id] == kIsWalls) {
+        for (int j = 0; j <= m.n; j++)
+            *slot_at(out + (size_t)j * S, c.id) = *slot_at(prev + (size_t)j * S, c.id);
+        return;
+    }
+    R comb = (R)(m.n - 1);
+#pragma unroll
+    for (int j = 0; j < MKG_MPF_MAX; j++) {
+        if (j < m.n) {
+            const LatArgs L = lattice_of<R>(m, j);
+            R g[19];
+            read_pops<19, ODD, R>(g, L, S, c, m.a, kKindScalar);
+            R s = 0;
+#pragma unroll
+            for (int i = 0; i < 19; i++) s += g[i];
+            const R ph = (R)L.ref + s;
+            *slot_at(out + (size_t)j * S, c.id) = ph;
+            comb += ph;
+        }
+    }
+    *slot_at(out + (size_t)m.n * S, c.id) = comb;
+}
+
+// LB-weighted gradient of order parameter plane ph around the cell (no 1/cs^2, as in the
+// template): periodic faces wrap, other faces replicate the edge value.
+template <class R>
+__device__ inline void mpf_gradient(const StepArgs& a, const Cell& c, const R* ph, R (&s)[3]) {
+    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & XLO) px[0] = px[1];
+    if (np & XHI) px[2] = px[1];
+    if (np & YLO) py[0] = py[1];
+    if (np & YHI) py[2] = py[1];
+    if (np & ZLO) pz[0] = pz[1];
+    if (np & ZHI) pz[2] = pz[1];
+    s[0] = s[1] = s[2] = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)wgt<19>(i) * *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+        if (cx(i)) s[0] += (R)cx(i) * v;
+        if (cy(i)) s[1] += (R)cy(i) * v;
+        if (cz(i)) s[2] += (R)cz(i) * v;
+    }
+}
+
+template <class R, bool TH, bool ODD>
+__global__ void __launch_bounds__(256) k_mpf_step(const MpfArgs m) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    if (m.a.flag[c.id] == kIsWalls) return;
+    const StepArgs& a = m.a;
+    const uint32_t S = a.S;
+    const R* __restrict__ cur = (const R*)a.phi_cur;
+    const R* __restrict__ prev = (const R*)a.phi_prev;
+    R ph[MKG_MPF_MAX];
+#pragma unroll
+    for (int k = 0; k < MKG_MPF_MAX; k++)
+        ph[k] = k < m.n ? *slot_at(cur + (size_t)k * S, c.id) : (R)-1;
+
+    // thermal collide-stream (mkThermalField::propagate / relaxation of run_multiPhaseField)
+    R T = (R)m.T_ref;
+    if constexpr (TH) {
+        R h[7];
+        read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+        R sh = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) sh += h[i];
+        T = (R)a.h.ref + sh;
+        const R dphi = *slot_at(cur + (size_t)m.n * S, c.id) - *slot_at(prev + (size_t)m.n * S, c.id);
+        const R om = (R)m.omega_h, src = (R)m.latent_T * dphi;
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+            const R w = (R)wgt<7>(i);
+            h[i] = h[i] - om * (h[i] - w * sh) + w * src;
+        }
+        write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+    }
+
+    const R om = (R)m.omega_g;
+#pragma unroll
+    for (int j = 0; j < MKG_MPF_MAX; j++) {
+        if (j < m.n) {
+            R s[3];
+            mpf_gradient<R>(a, c, cur + (size_t)j * S, s);
+            const R sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + (R)m.eps;
+            const R n0 = s[0] / sn, n1 = s[1] / sn, n2 = s[2] / sn;
+            const R pj = ph[j];
+            R P = 0;
+#pragma unroll
+            for (int k = 0; k < MKG_MPF_MAX; k++) {
+                if (k != j && k < m.n) {
+                    const R hk = (R)0.5 * ph[k] + (R)0.5;
+                    P += hk * hk;
+                }
+            }
+            P *= (R)(-2.0 * m.kappa[j]) * ((R)0.5 * pj + (R)0.5);
+            const R d = (R)1 - pj * pj;
+            const R Q = (R)m.growth * (pj - (R)m.lambda * T * d) * d;
+            const R src = (R)m.dt * (P + Q);
+            const R ux = (R)m.u[j][0], uy = (R)m.u[j][1], uz = (R)m.u[j][2];
+            const R uu = ux * ux + uy * uy + uz * uz;
+            const LatArgs L = lattice_of<R>(m, j);
+            const R dp = pj - (R)L.ref;
+            const R zd = (R)m.zc * d;
+            R g[19];
+            read_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
+#pragma unroll
+            for (int i = 0; i < 19; i++) {
+                const R w = (R)wgt<19>(i);
+                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+                const R cn = cx(i) * n0 + cy(i) * n1 + cz(i) * n2;
+                // deviation form: g~eq = w (phi_j - ref) + w phi_j (3 cu + 4.5 cu^2 - 1.5 uu)
+                const R geq = w * (dp + pj * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu));
+                g[i] = g[i] - om * (g[i] - geq) + w * (src + zd * cn);
+            }
+            write_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
+        }
+    }
+}
+
+// Equilibrium start in the own-slot layout: phase populations from phi_j (doubles, n x N) with
+// each order parameter's velocity, both phase buffers, thermal populations from T (or T_ref).
+template <class R>
+__global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double* phij,
+                                                  const double* Tin, R* phi0, R* phi1) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    const uint32_t S = m.a.S, N = m.a.N;
+    double comb = m.n - 1.0;
+    for (int j = 0; j < m.n; j++) {
+        const double pj = phij[(size_t)j * N + c.id];
+        R* base = (R*)m.a.g.pop + (size_t)j * m.lat;
+        const double ux = m.u[j][0], uy = m.u[j][1], uz = m.u[j][2];
+        const double uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            const double w = wgt<19>(i);
+            const double cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+            const double geq = w * pj * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+            *slot_at(base + (size_t)i * S, c.id) = (R)(geq - w * m.a.g.ref);
+        }
+        *slot_at(phi0 + (size_t)j * S, c.id) = (R)pj;
+        *slot_at(phi1 + (size_t)j * S, c.id) = (R)pj;
+        comb += pj;
+    }
+    *slot_at(phi0 + (size_t)m.n * S, c.id) = (R)comb;
+    *slot_at(phi1 + (size_t)m.n * S, c.id) = (R)comb;
+    if (m.thermal) {
+        const double t = Tin ? Tin[c.id] : m.T_ref;
+        R* hb = (R*)m.a.h.pop;
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+            *slot_at(hb + (size_t)i * S, c.id) = (R)(wgt<7>(i) * (t - m.a.h.ref));
+    }
+}
+
+template <class R>
+int mpf_launch_moments(const MpfArgs& m, int odd, void* phi_out, hipStream_t s) {
+    dim3 grid(m.a.nblk), block(256);
+    if (odd) {
+        k_mpf_moments<R, true><<<grid, block, 0, s>>>(m, (R*)phi_out);
+    } else {
+        k_mpf_moments<R, false><<<grid, block, 0, s>>>(m, (R*)phi_out);
+    }
+    return 0;
+}
+
+}  // namespace mkg
+
+using namespace mkg;
+
+struct mkg_mpf_ctx {
+    mkg_mpf_params p;
+    int fp32 = 0;
+    size_t esz = 8;
+    uint32_t N = 0, S = 0, F = 0, nblk = 0;
+    hipStream_t stream = nullptr;
+    void* g = nullptr;        // n lattices of 19 (S + F) elements
+    void* h = nullptr;        // 7 (S + F) elements
+    void* phi[2] = {nullptr, nullptr};  // (n + 1) S elements each
+    int8_t* flag = nullptr;
+    double* scratch = nullptr;  // 19 N doubles
+    double* partials = nullptr;
+    double* red_out = nullptr;
+    int lay = 0;              // AA layout of every lattice
+    int64_t pf_count = 0;
+    int has_walls = 0;
+    size_t lat_elems() const { return (size_t)19 * ((size_t)S + F); }
+};
+
+namespace {
+
+int mfail(const std::string& msg) {
+    set_error(msg);
+    return -1;
+}
+
+#define MCHK(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return mfail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t kMpfPad = 1024;
+
+MpfArgs mpf_args(const mkg_mpf_ctx* h) {
+    MpfArgs m;
+    std::memset(&m, 0, sizeof(m));
+    const mkg_mpf_params& p = h->p;
+    StepArgs& a = m.a;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.N = h->N;
+    a.S = h->S;
+    a.nblk = h->nblk;
+    a.F = h->F;
+    a.has_walls = h->has_walls;
+    a.flag = h->flag;
+    a.rho0 = 1.0;
+    auto fill = [&](LatArgs& L, void* pop, const int32_t* bc, double ref, const double* fval) {
+        L.pop = pop;
+        int np = 0;
+        for (int f = 0; f < 6; f++) {
+            L.bc[f] = bc[f];
+            if (bc[f] != MKG_BC_PERIODIC) np |= 1 << f;
+            L.fval[f] = fval[f];
+        }
+        L.npmask = np;
+        L.lay = h->lay;
+        L.adv = 1;
+        L.ref = ref;
+    };
+    fill(a.g, h->g, p.bc, -1.0, p.phi_face);
+    if (h->h) fill(a.h, h->h, p.bc_h, p.T_ref, p.T_face);
+    a.phi_cur = h->phi[h->pf_count & 1];
+    a.phi_prev = h->phi[(h->pf_count + 1) & 1];
+    m.lat = h->lat_elems();
+    m.n = p.n;
+    m.thermal = p.thermal ? 1 : 0;
+    for (int j = 0; j < MKG_MPF_MAX; j++) {
+        for (int k = 0; k < 3; k++) m.u[j][k] = p.u[j][k];
+        m.kappa[j] = p.kappa[j];
+    }
+    m.growth = p.growth;
+    m.lambda = p.lambda;
+    m.eps = p.eps;
+    m.omega_g = 1.0 / (0.5 + p.dt * p.wdx * p.wdx * 3.0);
+    m.zc = (1.0 - 0.5 * m.omega_g) / (std::sqrt(2.0) * p.wdx);
+    m.dt = p.dt;
+    m.omega_h = 1.0 / (0.5 + p.alpha / 0.25);
+    m.latent_T = p.latent_T;
+    m.T_ref = p.T_ref;
+    return m;
+}
+
+// Rewrite every lattice into the own-slot layout.
+int mpf_normalize(mkg_mpf_ctx* h) {
+    if (h->lay == 0) return 0;
+    MpfArgs m = mpf_args(h);
+    for (int j = 0; j < h->p.n; j++) {
+        LatArgs L = m.a.g;
+        L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+        if (launch_canonical(19, h->fp32, m.a, L, 1, h->scratch, h->stream) ||
+            launch_store_pops(19, h->fp32, m.a, L, h->scratch, h->stream))
+            return mfail("normalize: lattice unavailable");
+    }
+    if (h->h) {
+        if (launch_canonical(7, h->fp32, m.a, m.a.h, 1, h->scratch, h->stream) ||
+            launch_store_pops(7, h->fp32, m.a, m.a.h, h->scratch, h->stream))
+            return mfail("normalize: lattice unavailable");
+    }
+    MCHK(hipGetLastError());
+    h->lay = 0;
+    return 0;
+}
+
+template <class R>
+void launch_mpf_step_kernel(const MpfArgs& m, int odd, int th, hipStream_t s) {
+    dim3 grid(m.a.nblk), block(256);
+    if (th) {
+        if (odd) k_mpf_step<R, true, true><<<grid, block, 0, s>>>(m);
+        else k_mpf_step<R, true, false><<<grid, block, 0, s>>>(m);
+    } else {
+        if (odd) k_mpf_step<R, false, true><<<grid, block, 0, s>>>(m);
+        else k_mpf_step<R, false, false><<<grid, block, 0, s>>>(m);
+    }
+}
+
+void mpf_destroy_impl(mkg_mpf_ctx* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* bufs[] = {h->g, h->h, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
+                    h->red_out};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mkg_mpf_params_size(void) { return sizeof(mkg_mpf_params); }
+
+int mkg_mpf_params_default(mkg_mpf_params* m) {
+    if (!m) return mfail("params is null");
+    std::memset(m, 0, sizeof(*m));
+    m->nx = m->ny = m->nz = 1;
+    m->n = 2;
+    m->dt = 0.008;
+    m->wdx = 2.5;
+    m->u[1][0] = 0.02;
+    for (int j = 1; j < MKG_MPF_MAX; j++) m->kappa[j] = 1.0;
+    m->eps = 1e-12;
+    m->alpha = 0.1;
+    m->latent_T = 0.5;
+    for (int f = 0; f < 6; f++) m->phi_face[f] = -1.0;
+    return 0;
+}
+
+int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
+    if (!out) return mfail("out is null");
+    *out = nullptr;
+    if (!p) return mfail("params is null");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return mfail("grid dimensions must be >= 1");
+    if (p->n < 1 || p->n > MKG_MPF_MAX) return mfail("n must be 1..MKG_MPF_MAX");
+    if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
+        return mfail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
+    if (!(p->wdx > 0) || !(p->dt > 0)) return mfail("wdx and dt must be > 0");
+    if (!(p->eps > 0)) return mfail("eps must be > 0");
+    const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
+    const double n = (double)p->nx * p->ny * p->nz;
+    if ((n + 65536.0) * esz * (p->n + 1) >= 4294967296.0)
+        return mfail("grid too large: the phase buffer (cells x (n+1) x element size) must stay below 4 GiB");
+    for (int f = 0; f < 6; f++) {
+        const int32_t codes[2] = {p->bc[f], p->bc_h[f]};
+        for (int32_t v : codes)
+            if (v < MKG_BC_PERIODIC || v > MKG_BC_OUTFLOW)
+                return mfail("face codes of the multi-phase field are periodic, bounce, Dirichlet or outflow");
+    }
+    for (int a = 0; a < 3; a++)
+        if ((p->bc[2 * a] == 0) != (p->bc[2 * a + 1] == 0) ||
+            (p->bc_h[2 * a] == 0) != (p->bc_h[2 * a + 1] == 0))
+            return mfail("a periodic axis needs periodic codes on both faces");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return mfail("no HIP device available");
+    mkg_mpf_ctx* h = new mkg_mpf_ctx();
+    h->p = *p;
+    h->fp32 = p->precision == MKG_PREC_FP32;
+    h->esz = h->fp32 ? 4 : 8;
+    h->N = (uint32_t)n;
+    h->S = ((h->N + 127u) & ~127u) + kMpfPad;
+    h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
+    h->nblk = (h->N + 255u) / 256u;
+    auto bad = [&](hipError_t e, const char* what) {
+        mpf_destroy_impl(h);
+        return mfail(std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad(e, "hipStreamCreate");
+    const size_t gb = (size_t)p->n * h->lat_elems() * h->esz;
+    if ((e = hipMalloc(&h->g, gb)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMemsetAsync(h->g, 0, gb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    if (p->thermal) {
+        const size_t hb = (size_t)7 * ((size_t)h->S + h->F) * h->esz;
+        if ((e = hipMalloc(&h->h, hb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(h->h, 0, hb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    }
+    const size_t pb = (size_t)(p->n + 1) * h->S * h->esz;
+    for (int b = 0; b < 2; b++) {
+        if ((e = hipMalloc(&h->phi[b], pb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(h->phi[b], 0, pb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    }
+    if ((e = hipMalloc((void**)&h->flag, h->S)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMemsetAsync(h->flag, 0, h->S, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    if ((e = hipMalloc((void**)&h->scratch, (size_t)19 * h->N * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->partials, (size_t)(h->nblk + 1) * 4 * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->red_out, 8 * sizeof(double))) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad(e, "hipStreamSynchronize");
+    *out = h;
+    return 0;
+}
+
+int mkg_mpf_destroy(mkg_mpf_handle h) {
+    mpf_destroy_impl(h);
+    return 0;
+}
+
+int mkg_mpf_init(mkg_mpf_handle h, const double* phi, const double* T) {
+    if (!h) return mfail("null handle");
+    if (!phi) return mfail("phi is null");
+    const size_t N = h->N;
+    double* dphi = nullptr;
+    double* dT = nullptr;
+    MCHK(hipMalloc((void**)&dphi, (size_t)h->p.n * N * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(dphi, phi, (size_t)h->p.n * N * sizeof(double),
+                                  hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && T && h->h) {
+        e = hipMalloc((void**)&dT, N * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dT, T, N * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    }
+    if (e == hipSuccess) {
+        h->lay = 0;
+        h->pf_count = 0;
+        MpfArgs m = mpf_args(h);
+        dim3 grid(h->nblk), block(256);
+        if (h->fp32)
+            k_mpf_init<float><<<grid, block, 0, h->stream>>>(m, dphi, dT, (float*)h->phi[0],
+                                                             (float*)h->phi[1]);
+        else
+            k_mpf_init<double><<<grid, block, 0, h->stream>>>(m, dphi, dT, (double*)h->phi[0],
+                                                              (double*)h->phi[1]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dphi);
+    if (dT) (void)hipFree(dT);
+    if (e != hipSuccess) return mfail(std::string("mkg_mpf_init: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int mkg_mpf_step(mkg_mpf_handle h, int nsteps) {
+    if (!h) return mfail("null handle");
+    if (nsteps < 0) return mfail("nsteps must be >= 0");
+    for (int s = 0; s < nsteps; s++) {
+        h->pf_count++;
+        MpfArgs m = mpf_args(h);
+        const int odd = h->lay;
+        if (h->fp32) mpf_launch_moments<float>(m, odd, h->phi[h->pf_count & 1], h->stream);
+        else mpf_launch_moments<double>(m, odd, h->phi[h->pf_count & 1], h->stream);
+        if (h->fp32) launch_mpf_step_kernel<float>(m, odd, h->h != nullptr, h->stream);
+        else launch_mpf_step_kernel<double>(m, odd, h->h != nullptr, h->stream);
+        h->lay ^= 1;
+    }
+    MCHK(hipGetLastError());
+    return 0;
+}
+
+int mkg_mpf_sync(mkg_mpf_handle h) {
+    if (!h) return mfail("null handle");
+    MCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_mpf_get(mkg_mpf_handle h, int which, void* host, size_t count) {
+    if (!h) return mfail("null handle");
+    if (!host) return mfail("host buffer is null");
+    const size_t N = h->N;
+    const int n = h->p.n;
+    MpfArgs m = mpf_args(h);
+    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL) {
+        const int nl = which == MKG_MPF_POPS ? n : 1;
+        const int q = which == MKG_MPF_POPS ? 19 : 7;
+        if (which == MKG_MPF_THERMAL && !h->h) return mfail("thermal lattice not present");
+        if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
+        for (int j = 0; j < nl; j++) {
+            LatArgs L = which == MKG_MPF_POPS ? m.a.g : m.a.h;
+            if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+            if (launch_canonical(q, h->fp32, m.a, L, 1, h->scratch, h->stream))
+                return mfail("lattice unavailable");
+            MCHK(hipMemcpyAsync((double*)host + (size_t)j * q * N, h->scratch,
+                                (size_t)q * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        }
+        MCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_MPF_PHI || which == MKG_MPF_PHI_PREV) {
+        if (count != (size_t)(n + 1) * N) return mfail("count does not match the field size");
+        const void* src = which == MKG_MPF_PHI ? m.a.phi_cur : m.a.phi_prev;
+        for (int j = 0; j <= n; j++) {
+            launch_to_double(h->fp32, (const char*)src + (size_t)j * h->S * h->esz, h->scratch,
+                             h->N, h->stream);
+            MCHK(hipMemcpyAsync((double*)host + (size_t)j * N, h->scratch, N * sizeof(double),
+                                hipMemcpyDeviceToHost, h->stream));
+        }
+        MCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_MPF_FLAG) {
+        if (count != N) return mfail("count does not match the field size");
+        MCHK(hipMemcpyAsync(host, h->flag, N, hipMemcpyDeviceToHost, h->stream));
+        MCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    return mfail("unknown multi-phase field id");
+}
+
+int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count) {
+    if (!h) return mfail("null handle");
+    if (!host) return mfail("host buffer is null");
+    const size_t N = h->N;
+    const int n = h->p.n;
+    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL) {
+        const int nl = which == MKG_MPF_POPS ? n : 1;
+        const int q = which == MKG_MPF_POPS ? 19 : 7;
+        if (which == MKG_MPF_THERMAL && !h->h) return mfail("thermal lattice not present");
+        if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
+        if (mpf_normalize(h)) return -1;
+        MpfArgs m = mpf_args(h);
+        for (int j = 0; j < nl; j++) {
+            LatArgs L = which == MKG_MPF_POPS ? m.a.g : m.a.h;
+            if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+            MCHK(hipMemcpyAsync(h->scratch, (const double*)host + (size_t)j * q * N,
+                                (size_t)q * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            if (launch_store_pops(q, h->fp32, m.a, L, h->scratch, h->stream))
+                return mfail("lattice unavailable");
+        }
+        MCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_MPF_PHI || which == MKG_MPF_PHI_PREV) {
+        if (count != (size_t)(n + 1) * N) return mfail("count does not match the field size");
+        MpfArgs m = mpf_args(h);
+        void* dst = (void*)(which == MKG_MPF_PHI ? m.a.phi_cur : m.a.phi_prev);
+        for (int j = 0; j <= n; j++) {
+            MCHK(hipMemcpyAsync(h->scratch, (const double*)host + (size_t)j * N,
+                                N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            launch_from_double(h->fp32, h->scratch, (char*)dst + (size_t)j * h->S * h->esz,
+                               h->N, h->stream);
+        }
+        MCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_MPF_FLAG) {
+        if (count != N) return mfail("count does not match the field size");
+        const int8_t* f = (const int8_t*)host;
+        int walls = 0;
+        for (size_t k = 0; k < N; k++) {
+            if (f[k] != kIsFluid && f[k] != kIsWalls)
+                return mfail("multi-phase flags are 0 (fluid) or 3 (wall)");
+            walls |= f[k] == kIsWalls;
+        }
+        if (mpf_normalize(h)) return -1;
+        MCHK(hipMemcpyAsync(h->flag, host, N, hipMemcpyHostToDevice, h->stream));
+        MCHK(hipStreamSynchronize(h->stream));
+        h->has_walls = walls;
+        return 0;
+    }
+    return mfail("unknown multi-phase field id");
+}
+
+int mkg_mpf_sums(mkg_mpf_handle h, double* out) {
+    if (!h) return mfail("null handle");
+    if (!out) return mfail("out is null");
+    MpfArgs m = mpf_args(h);
+    for (int j = 0; j <= h->p.n; j++) {
+        launch_to_double(h->fp32, (const char*)m.a.phi_cur + (size_t)j * h->S * h->esz,
+                         h->scratch, h->N, h->stream);
+        int nout = 0;
+        launch_reduce(4, m.a, nullptr, nullptr, nullptr, nullptr, h->scratch, h->partials,
+                      h->red_out, &nout, h->stream);
+        MCHK(hipMemcpyAsync(out + j, h->red_out, sizeof(double), hipMemcpyDeviceToHost,
+                            h->stream));
+    }
+    MCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -59,7 +59,7 @@ class MkgParams(C.Structure):
     _fields_ = PARAM_FIELDS
 
     def copy(self) -> "MkgParams":
-        out = MkgParams()
+        out = type(self)()
         C.memmove(C.byref(out), C.byref(self), C.sizeof(self))
         return out
 
@@ -82,6 +82,34 @@ class MkgParams(C.Structure):
             else:
                 setattr(self, k, v)
         return self
+
+    @property
+    def ncell(self) -> int:
+        return int(self.nx) * int(self.ny) * int(self.nz)
+
+
+MPF_MAX = 4
+MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG = range(5)
+
+MPF_FIELDS = [
+    ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n", C.c_int32),
+    ("precision", C.c_int32), ("thermal", C.c_int32),
+    ("bc", C.c_int32 * 6), ("bc_h", C.c_int32 * 6),
+    ("dt", C.c_double), ("wdx", C.c_double),
+    ("u", (C.c_double * 3) * MPF_MAX), ("kappa", C.c_double * MPF_MAX),
+    ("growth", C.c_double), ("lambda_", C.c_double), ("eps", C.c_double),
+    ("alpha", C.c_double), ("latent_T", C.c_double), ("T_ref", C.c_double),
+    ("T_face", C.c_double * 6), ("phi_face", C.c_double * 6),
+]
+
+
+class MpfParams(C.Structure):
+    """Layout-identical mirror of ``mkg_mpf_params`` (multi-phase field)."""
+
+    _fields_ = MPF_FIELDS
+
+    copy = MkgParams.copy
+    set = MkgParams.set
 
     @property
     def ncell(self) -> int:
@@ -132,6 +160,16 @@ def lib() -> C.CDLL:
         "mkg_last_timing": (C.c_int, [vp, vp]),
         "mkg_enable_timing": (C.c_int, [vp, C.c_int]),
         "mkg_adjoint_run": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]),
+        "mkg_mpf_params_size": (C.c_size_t, []),
+        "mkg_mpf_params_default": (C.c_int, [P(MpfParams)]),
+        "mkg_mpf_create": (C.c_int, [P(MpfParams), P(vp)]),
+        "mkg_mpf_destroy": (C.c_int, [vp]),
+        "mkg_mpf_init": (C.c_int, [vp, vp, vp]),
+        "mkg_mpf_step": (C.c_int, [vp, C.c_int]),
+        "mkg_mpf_sync": (C.c_int, [vp]),
+        "mkg_mpf_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_mpf_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_mpf_sums": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -139,6 +177,8 @@ def lib() -> C.CDLL:
         fn.argtypes = args
     if L.mkg_params_size() != C.sizeof(MkgParams):
         raise MkgError("mkg_params layout mismatch between libmkg.so and mkg.py")
+    if L.mkg_mpf_params_size() != C.sizeof(MpfParams):
+        raise MkgError("mkg_mpf_params layout mismatch between libmkg.so and mkg.py")
     _lib = L
     return L
 
@@ -316,3 +356,74 @@ class Solver:
         g = self.get(F_POP_PHASE)
         return g.reshape(int(self.params.phase_q), self.N).sum(0)
 
+
+
+def default_mpf_params(**kw) -> MpfParams:
+    m = MpfParams()
+    _check(lib().mkg_mpf_params_default(C.byref(m)))
+    m.set(**kw)
+    return m
+
+
+class MpfSolver:
+    """Multi-phase field on the device (wraps an ``mkg_mpf_handle``): n order parameters on
+    D3Q19 plus an optional D3Q7 thermal lattice (mkFluidsUser::propagate / relaxation)."""
+
+    def __init__(self, params: MpfParams):
+        self.params = params.copy()
+        h = C.c_void_p()
+        _check(lib().mkg_mpf_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.N = self.params.ncell
+        self.n = int(self.params.n)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mkg_mpf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, phi, T=None) -> None:
+        ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        if ph.size != self.n * self.N:
+            raise MkgError("phi must hold n x N values")
+        t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
+        _check(lib().mkg_mpf_init(self._h, _ptr(ph), _ptr(t)))
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().mkg_mpf_step(self._h, int(nsteps)))
+
+    def sync(self) -> None:
+        _check(lib().mkg_mpf_sync(self._h))
+
+    def _count(self, which: int) -> int:
+        return {MPF_POPS: self.n * 19 * self.N, MPF_PHI: (self.n + 1) * self.N,
+                MPF_PHI_PREV: (self.n + 1) * self.N, MPF_THERMAL: 7 * self.N,
+                MPF_FLAG: self.N}[which]
+
+    def get(self, which: int) -> np.ndarray:
+        n = self._count(which)
+        out = np.empty(n, dtype=np.int8 if which == MPF_FLAG else np.float64)
+        _check(lib().mkg_mpf_get(self._h, which, _ptr(out), n))
+        return out
+
+    def set(self, which: int, arr) -> None:
+        dt = np.int8 if which == MPF_FLAG else np.float64
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(-1)
+        _check(lib().mkg_mpf_set(self._h, which, _ptr(a), a.size))
+
+    def sums(self) -> np.ndarray:
+        out = np.zeros(self.n + 1)
+        _check(lib().mkg_mpf_sums(self._h, _ptr(out)))
+        return out

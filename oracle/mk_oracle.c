@@ -886,3 +886,176 @@ void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* cent
                     s->flag[id] = (best <= -ts) ? 2 : (best < ts ? 1 : 0);
             }
 }
+
+/* ------------------------------------------------------------------ multi-phase field
+ * n order parameters phi_j in [-1, 1], each with its own D3Q19 population lattice: the gVec
+ * lattices of mkFluidsUser (MaterKit/src/mkFluidsUser.cpp:68-299, phase_number of
+ * MaterKit/inc/mkMultiphaseField.h:56), run by run_multiPhaseField (MaterKit/MaterKit.cpp:
+ * 529-576).  One step, following the visible loops:
+ *   1. phi_j = sum_i g_j,i and the combined phi = (n - 1) + sum_j phi_j (:147-172)
+ *   2. per non-wall cell and order parameter:
+ *        s_j = sum_i c_i w_i phi_j(x + c_i), n_j = s_j / (|s_j| + eps)       (:208-217)
+ *        g_eq = w_i phi_j (1 + 3 c.u_j + 4.5 (c.u_j)^2 - 1.5 u_j^2): getEquPDFs with phi_j
+ *               as the density and the order parameter's own velocity u_j          (:219-230)
+ *        P_j = -2 kappa_j (phi_j/2 + 1/2) sum_{k != j} (phi_k/2 + 1/2)^2           (:236-256)
+ *        Q_j = growth (phi_j - lambda T (1 - phi_j^2)) (1 - phi_j^2)   (growth 0 there, :266-271)
+ *        Z_i = (1 - omega_g/2) w_i (c_i . n_j) (1 - phi_j^2) dx / (sqrt(2) W0): the
+ *              conservative Allen-Cahn counter-term of getAddedPhasePDFs(omega_g, phi_j, n_j)
+ *              in the form of Chai, Sun, Wang & Shi (2018) (:273-279)
+ *        g* = g - omega_g (g - g_eq) + w_i dt (P_j + Q_j) + Z_i                     (:285-293)
+ *      with omega_g = 1 / (dt/dx^2/cs^2 + 1/2) (:282)
+ *   3. pull streaming with the face rules of the lattice (:68-134)
+ * The combined lattice g of the template is not evolved: its density is overwritten by the
+ * combined phi at every step (:171-172), so it feeds nothing.
+ * Optional D3Q7 thermal lattice (run_multiPhaseField's mkThermalField calls, :566-567): BGK
+ * diffusion with alpha, no advection, latent source w_i L dphi of the combined phi; T enters
+ * Q_j. */
+static orc_params mpf_geom(const orc_mpf_params* m, const int* bc) {
+    orc_params p;
+    memset(&p, 0, sizeof(p));
+    p.nx = m->nx;
+    p.ny = m->ny;
+    p.nz = m->nz;
+    for (int f = 0; f < 6; f++) p.bc_g[f] = bc[f];
+    p.rho0 = 1.0;
+    return p;
+}
+
+void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, double* g,
+                  double* phi, double* phi_prev, double* h) {
+    long N = (long)m->nx * m->ny * m->nz;
+    lat_t L = make_lat(19);
+    for (long x = 0; x < N; x++) {
+        double comb = m->n - 1.0;
+        for (int j = 0; j < m->n; j++) {
+            double ph = phij[j * N + x];
+            const double* u = m->u[j];
+            double feq[27];
+            orc_equilibrium(19, ph, u, feq);
+            for (int i = 0; i < 19; i++) g[((long)j * 19 + i) * N + x] = feq[i];
+            phi[j * N + x] = phi_prev[j * N + x] = ph;
+            comb += ph;
+        }
+        phi[(long)m->n * N + x] = phi_prev[(long)m->n * N + x] = comb;
+        if (h) {
+            lat_t L7 = make_lat(7);
+            double t = T ? T[x] : m->T_ref;
+            for (int i = 0; i < 7; i++) h[i * N + x] = L7.w[i] * t;
+        }
+    }
+    (void)L;
+}
+
+int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_prev, double* h,
+                 int8_t* flag, int nsteps) {
+    if (m->n < 1 || m->n > ORC_MPF_MAX) return -1;
+    long N = (long)m->nx * m->ny * m->nz;
+    int n = m->n;
+    lat_t L = make_lat(19);
+    lat_t L7 = make_lat(7);
+    orc_params pg = mpf_geom(m, m->bc);
+    orc_params ph7 = mpf_geom(m, m->bc_h);
+    int8_t* fl = flag;
+    int8_t* own = NULL;
+    if (!fl) {
+        own = calloc((size_t)N, 1);
+        fl = own;
+    }
+    orc_state st;
+    memset(&st, 0, sizeof(st));
+    st.flag = fl;
+    double omg = 1.0 / (0.5 + m->dt * m->wdx * m->wdx / L.cs2);
+    double lam = 1.0 / (sqrt(2.0) * m->wdx);
+    int per[3] = {m->bc[0] == 0, m->bc[2] == 0, m->bc[4] == 0};
+    double* post = malloc((size_t)19 * N * sizeof(double));
+    double* fresh = malloc((size_t)19 * N * sizeof(double));
+    for (int step = 0; step < nsteps; step++) {
+        /* 1. moments */
+        for (long x = 0; x < N; x++) {
+            for (int j = 0; j <= n; j++) phi_prev[j * N + x] = phi[j * N + x];
+            if (fl[x] == 3) continue;
+            double comb = n - 1.0;
+            for (int j = 0; j < n; j++) {
+                double s = 0;
+                for (int i = 0; i < 19; i++) s += g[((long)j * 19 + i) * N + x];
+                phi[j * N + x] = s;
+                comb += s;
+            }
+            phi[(long)n * N + x] = comb;
+        }
+        /* thermal moments at this level (T enters the growth term) */
+        double* Tc = NULL;
+        if (h) {
+            Tc = malloc((size_t)N * sizeof(double));
+            for (long x = 0; x < N; x++) {
+                double t = 0;
+                for (int i = 0; i < 7; i++) t += h[i * N + x];
+                Tc[x] = t;
+            }
+        }
+        /* 2. collision and 3. streaming, one order parameter at a time */
+        for (int j = 0; j < n; j++) {
+            const double* u = m->u[j];
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int X = 0; X < m->nx; X++)
+                for (int Y = 0; Y < m->ny; Y++)
+                    for (int Z = 0; Z < m->nz; Z++) {
+                        long x = lin(&pg, X, Y, Z);
+                        if (fl[x] == 3) continue;
+                        double s3[3] = {0, 0, 0};
+                        for (int i = 1; i < 19; i++) {
+                            int XX = clampwrap(X + E27[i][0], m->nx, per[0]);
+                            int YY = clampwrap(Y + E27[i][1], m->ny, per[1]);
+                            int ZZ = clampwrap(Z + E27[i][2], m->nz, per[2]);
+                            double v = L.w[i] * phi[j * N + lin(&pg, XX, YY, ZZ)];
+                            for (int a = 0; a < 3; a++) s3[a] += E27[i][a] * v;
+                        }
+                        double sn = sqrt(s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]) + m->eps;
+                        double nv[3] = {s3[0] / sn, s3[1] / sn, s3[2] / sn};
+                        double pj = phi[j * N + x];
+                        double P = 0;
+                        for (int k = 0; k < n; k++)
+                            if (k != j) {
+                                double hk = 0.5 * phi[k * N + x] + 0.5;
+                                P += hk * hk;
+                            }
+                        P *= -2.0 * m->kappa[j] * (0.5 * pj + 0.5);
+                        double d = 1.0 - pj * pj;
+                        double T = Tc ? Tc[x] : m->T_ref;
+                        double Q = m->growth * (pj - m->lambda * T * d) * d;
+                        double feq[27];
+                        orc_equilibrium(19, pj, u, feq);
+                        for (int i = 0; i < 19; i++) {
+                            double cn = E27[i][0] * nv[0] + E27[i][1] * nv[1] + E27[i][2] * nv[2];
+                            double Zc = (1.0 - 0.5 * omg) * L.w[i] * cn * d * lam;
+                            double gi = g[((long)j * 19 + i) * N + x];
+                            post[i * N + x] = gi - omg * (gi - feq[i]) +
+                                              L.w[i] * m->dt * (P + Q) + Zc;
+                        }
+                    }
+            double* gj = g + (long)j * 19 * N;
+            stream(&pg, &st, 19, m->bc, 1, m->phi_face, NULL, NULL, NULL, post, gj, fresh);
+            memcpy(gj, fresh, (size_t)19 * N * sizeof(double));
+        }
+        /* thermal */
+        if (h) {
+            for (long x = 0; x < N; x++) {
+                if (fl[x] == 3) continue;
+                double om = 1.0 / (0.5 + m->alpha / L7.cs2);
+                double dphi = phi[(long)n * N + x] - phi_prev[(long)n * N + x];
+                for (int i = 0; i < 7; i++) {
+                    double hi = h[i * N + x];
+                    double heq = L7.w[i] * Tc[x];
+                    post[i * N + x] = hi - om * (hi - heq) + L7.w[i] * m->latent_T * dphi;
+                }
+            }
+            stream(&ph7, &st, 7, m->bc_h, 1, m->T_face, NULL, NULL, NULL, post, h, fresh);
+            memcpy(h, fresh, (size_t)7 * N * sizeof(double));
+            free(Tc);
+        }
+    }
+    free(post);
+    free(fresh);
+    free(own);
+    return 0;
+}

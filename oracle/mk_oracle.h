@@ -97,6 +97,30 @@ double orc_aniso(const orc_params* p, const double* n, double* D);
  * outside that scope. */
 int orc_aos_step(const orc_params* p, orc_state* s, int op, int nsteps, double* seconds);
 
+/* Multi-phase field (mkFluidsUser's phase_number order parameters): n conservative
+ * Allen-Cahn order parameters on D3Q19 with pairwise interaction, optional D3Q7 thermal
+ * lattice.  Same layout as the product's mkg_mpf_params. */
+#define ORC_MPF_MAX 4
+typedef struct orc_mpf_params {
+    int32_t nx, ny, nz, n;
+    int32_t precision, thermal;
+    int32_t bc[6], bc_h[6];
+    double dt, wdx;                 /* pf_dt and W0/dx */
+    double u[ORC_MPF_MAX][3];       /* advection velocity of each order parameter */
+    double kappa[ORC_MPF_MAX];      /* interaction strength of each */
+    double growth, lambda, eps;     /* growth-term coefficient, coupling, normal regulariser */
+    double alpha, latent_T, T_ref;  /* thermal lattice */
+    double T_face[6];
+    double phi_face[6];             /* Dirichlet value of the phase lattices */
+} orc_mpf_params;
+
+/* g: n x 19 x N populations; phi, phi_prev: (n + 1) x N (phi_j, then the combined phi);
+ * h: 7 x N or NULL; flag: N or NULL (no walls). */
+void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, double* g,
+                  double* phi, double* phi_prev, double* h);
+int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_prev, double* h,
+                 int8_t* flag, int nsteps);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 
