@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--no-mpf", action="store_true")
+    ap.add_argument("--mpf-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run three plain 1 GiB device copies (the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -135,6 +137,42 @@ def adjoint_cost(steps: int):
     return {"forward_s": round(fwd, 4), "ratio": runs["tape"]["ratio"], "tape": runs["tape"],
             "checkpointed": runs["checkpointed"], "steps": steps, "cells": 128 ** 3,
             "workload": "phase field (D3Q19) + thermal (D3Q7), 128^3, FP32, per-cell kappa"}
+
+
+def mpf_cost(size: int, steps: int) -> dict:
+    """Multi-phase field (two order parameters on D3Q19 + D3Q7 thermal, FP32) on a size^3 grid:
+    two grains, one carried at 0.02 in x and repelled by the other (the run_multiPhaseField
+    set-up in 3D).  Step time from the wall clock around `steps` steps on a synchronised
+    stream; algorithmic bytes: moment pass 19 n + (n + 1) values + flag, collide-stream
+    2 x 19 n + 2 x 7 + n + 2 values + flag."""
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+
+    n = 2
+    m = mkg.default_mpf_params(nx=size, ny=size, nz=size, n=n, precision=mkg.PREC_FP32,
+                               thermal=1, alpha=0.1, latent_T=0.5)
+    c = size / 2.0
+    phi = cases.mpf_seed_fields((size, size, size), [[c, c, c], [c, c - 16, c]], [8, 8], m.wdx)
+    dev = mkg.MpfSolver(m)
+    dev.init(phi)
+    del phi
+    dev.step(4)
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps)
+    dev.sync()
+    dt = (time.perf_counter() - t0) / steps
+    dev.close()
+    N = size ** 3
+    e = 4
+    bytes_cell = (19 * n + n + 1) * e + 1 + (2 * 19 * n + 2 * 7 + n + 2) * e + 1
+    gbs = bytes_cell * N / dt / 1e9
+    return {"workload": f"multi-phase field, {n} order parameters (D3Q19) + thermal (D3Q7), "
+                        f"{size}^3, FP32", "mlups": round(N / dt / 1e6, 1),
+            "ms_per_step": round(dt * 1e3, 4), "alg_bytes_per_cell": bytes_cell,
+            "step_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(seconds: float):
@@ -245,6 +283,8 @@ def main():
                        "fused_kernel_gbs": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
                        "alg_bytes_per_cell": b64}
         out["fp32_over_fp64"] = round(mlups32 / mlups64, 3)
+    if not a.no_mpf:
+        out["mpf"] = mpf_cost(size, a.mpf_steps)
     if not a.no_adjoint:
         adj = adjoint_cost(a.adjoint_steps)
         out["adjoint_over_forward"] = adj["ratio"]

@@ -160,12 +160,27 @@ typedef struct mkg_params {
     double phi_s;         /* flag threshold: phi >= phi_s solid, > -phi_s interface */
     int32_t aniso_kind;   /* MKG_ANISO_*: the form of a_s(n) (relaxation_anisotropy_cbc / _hcp,
                              anisotropy_facet; MaterKit/inc/mkPhaseField.h:49-64) */
-    int32_t pad1;
+    int32_t pf_model;     /* MKG_PF_KARMA_RAPPEL or MKG_PF_ALLEN_CAHN (see below) */
     double aniso[2];      /* parameters of the non-cubic forms (see MKG_ANISO_*) */
     /* deviation references for FP32 storage */
     double ref_f, ref_g, ref_h, ref_c;
     double rot[MKG_MAX_GRAINS][9]; /* row-major lab<-crystal rotation per grain */
 } mkg_params;
+
+/* Phase-field model of the phase lattice.
+ * MKG_PF_KARMA_RAPPEL  anisotropic dendrite growth (relaxation_anisotropy_thermal / _solute,
+ *                      MaterKit/inc/mkPhaseField.h:41-47); cells are re-classified from phi
+ *                      (reset_flags) and solid cells act on the flow.
+ * MKG_PF_ALLEN_CAHN    conservative Allen-Cahn interface (relaxation_AllenCahn and
+ *                      relaxation_AllenCahn_growth_thermal, MaterKit/inc/mkPhaseField.h:39-40,
+ *                      run by run_flow_AllenCahn, MaterKit/MaterKit.cpp:186-242): the
+ *                      counter-term (1 - omega_g/2) w_i (c_i . n)(1 - phi^2) dx/(sqrt 2 W0) of
+ *                      Chai et al. (2018) with n = s/(|s| + 1e-12), s = sum_i c_i w_i phi(x+c_i),
+ *                      plus the growth source -lambda u (1 - phi^2)^2 dt when pf_coupling
+ *                      selects u; flags are not re-classified (the interface is carried by the
+ *                      flow, not a solid). */
+#define MKG_PF_KARMA_RAPPEL 0
+#define MKG_PF_ALLEN_CAHN 1
 
 typedef struct mkg_ctx* mkg_handle;
 

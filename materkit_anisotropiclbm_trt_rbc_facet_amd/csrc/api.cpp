@@ -97,7 +97,8 @@ bool combo_supported(int fq, int pq, int th, int so) {
     static const int table[][4] = {{19, 0, 0, 0},  {27, 0, 0, 0}, {19, 0, 1, 0}, {27, 0, 1, 0},
                                    {0, 19, 1, 0},  {0, 7, 1, 0},  {19, 19, 1, 0}, {19, 7, 1, 0},
                                    {27, 19, 1, 0}, {19, 19, 0, 1}, {19, 7, 0, 1}, {0, 19, 0, 1},
-                                   {0, 0, 1, 0},   {0, 0, 0, 1},  {0, 19, 0, 0},  {0, 7, 0, 0}};
+                                   {0, 0, 1, 0},   {0, 0, 0, 1},  {0, 19, 0, 0},  {0, 7, 0, 0},
+                                   {19, 19, 0, 0}};
     for (auto& t : table)
         if (t[0] == fq && t[1] == pq && t[2] == th && t[3] == so) return true;
     return false;
@@ -176,6 +177,8 @@ StepArgs make_args(const mkg_ctx* h) {
     a.aniso[1] = p.aniso[1];
     const double cs2g = (p.phase_q == 7) ? 0.25 : 1.0 / 3.0;
     a.omega_g = 1.0 / (0.5 + p.pf_dt * p.pf_wdx2 / cs2g);
+    a.pf_model = p.pf_model;
+    a.ac_zc = p.pf_wdx2 > 0 ? (1.0 - 0.5 * a.omega_g) / (std::sqrt(2.0) * std::sqrt(p.pf_wdx2)) : 0.0;
     return a;
 }
 
@@ -217,6 +220,8 @@ int validate(const mkg_params* p) {
     if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
         return fail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
     if (p->ngrains < 0 || p->ngrains > MKG_MAX_GRAINS) return fail("ngrains out of range");
+    if (p->pf_model != MKG_PF_KARMA_RAPPEL && p->pf_model != MKG_PF_ALLEN_CAHN)
+        return fail("pf_model must be MKG_PF_KARMA_RAPPEL or MKG_PF_ALLEN_CAHN");
     if (p->aniso_kind < MKG_ANISO_CUBIC || p->aniso_kind > MKG_ANISO_FACET)
         return fail("aniso_kind must be one of MKG_ANISO_*");
     if (p->aniso_kind == MKG_ANISO_FACET && !(p->aniso[1] > 0))
@@ -820,7 +825,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     if (h->q[LF] || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
         return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model only");
     if (h->has_walls) return fail("the adjoint does not support wall cells");
-    if (h->p.aniso_kind != MKG_ANISO_CUBIC)
+    if (h->p.aniso_kind != MKG_ANISO_CUBIC || h->p.pf_model != MKG_PF_KARMA_RAPPEL)
         return fail("the adjoint covers the cubic (Karma-Rappel) anisotropy only");
     if (nsteps < 1 || ckpt < 0) return fail("nsteps must be >= 1 and ckpt >= 0");
     if (dL_dkappa && !h->kappa) return fail("dL/dkappa needs has_kappa = 1");

@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256) k_init_eq(const StepArgs a, const double*
         phi0[id] = (R)ph;
         phi1[id] = (R)ph;
         const int8_t fl = a.flag[id];
-        const int8_t nf = classify(fl, ph, a.phi_s);
+        const int8_t nf = classify(fl, ph, a.phi_s, a.pf_model);
         if (nf != fl) a.flag[id] = nf;
     }
     if (a.h.pop) {
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(256) k_seeds(const StepArgs a, int n, const do
     phid[c.id] = ph;
     if (grain) grain[c.id] = (int16_t)gbest;
     const int8_t fl = a.flag[c.id];
-    if (fl != kIsWalls && fl >= 0)
+    if (fl != kIsWalls && fl >= 0 && a.pf_model == 0)
         a.flag[c.id] = (int8_t)((best <= -ts) ? kIsSolid : (best < ts ? kIsInter : kIsFluid));
 }
 

@@ -303,6 +303,14 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
         return -1;
     }
     p->flow_incomp = eq == "incompressible" ? 1 : 0;
+    // phase-field model: Karma-Rappel dendrite growth or the conservative Allen-Cahn interface
+    const std::string pm = c.word("solver/phaseModel", "karmaRappel");
+    if (pm == "karmaRappel") p->pf_model = MKG_PF_KARMA_RAPPEL;
+    else if (pm == "allenCahn") p->pf_model = MKG_PF_ALLEN_CAHN;
+    else {
+        g_case_err = "solver/phaseModel must be karmaRappel or allenCahn";
+        return -1;
+    }
     // anisotropy form of the phase field and its two parameters
     const std::string af = c.word("solver/anisotropyForm", "cubic");
     if (af == "cubic") p->aniso_kind = MKG_ANISO_CUBIC;

@@ -651,8 +651,9 @@ __device__ inline R phase_source(const R (&P)[27], R upf, const Aniso<R>& an, R 
 #undef PP
 }
 
-__device__ inline int8_t classify(int8_t old, double phi, double phi_s) {
-    if (old == kIsWalls || old < 0) return old;
+// reset_flags (MaterKit/inc/mkPhaseField.h:37); the Allen-Cahn model keeps every flag.
+__device__ inline int8_t classify(int8_t old, double phi, double phi_s, int model = 0) {
+    if (old == kIsWalls || old < 0 || model != 0) return old;
     if (phi >= phi_s) return (int8_t)kIsSolid;
     if (phi > -phi_s) return (int8_t)kIsInter;
     return (int8_t)kIsFluid;

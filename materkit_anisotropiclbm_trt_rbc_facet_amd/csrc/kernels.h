@@ -42,8 +42,9 @@ struct StepArgs {
     double D_l, D_s, k_part, latent_C, q_min;
     double at_w;  // anti-trapping a_t * W0/dx
     double pf_dt, pf_wdx2, eps4, lambda, phi_s, omega_g;
-    int32_t aniso_kind, pad2;
+    int32_t aniso_kind, pf_model;
     double aniso[2];
+    double ac_zc;  // Allen-Cahn counter-term scale (1 - omega_g/2) / (sqrt 2 W0/dx)
 };
 
 // Selector of a fused-step instantiation: presence of each lattice.

@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
     for (int i = 0; i < PQ; i++) s += g[i];
     const R phi = (R)a.g.ref + s;
     phi_out[c.id] = phi;
-    const int8_t nf = classify(fl, (double)phi, a.phi_s);
+    const int8_t nf = classify(fl, (double)phi, a.phi_s, a.pf_model);
     if (nf != fl) a.flag[c.id] = nf;
 }
 
@@ -339,6 +339,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     //      relaxation_anisotropy_thermal / _solute)
     R g[PQ > 0 ? PQ : 1];
     R Sg = 0;
+    R acn0 = 0, acn1 = 0, acn2 = 0, acz = 0;  // Allen-Cahn normal and counter-term scale
     uint32_t own_g = 0;
     if constexpr (PQ > 0) {
         if (a.g.adv) {
@@ -356,10 +357,27 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                     rot = true;
                 }
             }
-            R as;
-            const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
-            Sg = phase_source<R>(P, upf, an, (R)a.lambda, (R)a.pf_dt, (R)a.pf_wdx2, Mrot, rot,
-                                 as);
+            if (a.pf_model == 1) {
+                // conservative Allen-Cahn (relaxation_AllenCahn): normal of the LB-weighted
+                // gradient s = sum_i c_i w_i phi(x + c_i), counter-term
+                // (1 - omega/2) w_i (c_i . n)(1 - phi^2) dx / (sqrt 2 W0) of Chai et al.
+                // (2018), growth source -lambda u (1 - phi^2)^2 dt
+                R gr[3];
+                stencil_gradient<R>(P, gr);
+                const R sn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]) +
+                             (R)(3.0 * 1e-12);
+                const R d = (R)1 - phi * phi;
+                acn0 = gr[0] / sn;
+                acn1 = gr[1] / sn;
+                acn2 = gr[2] / sn;
+                acz = (R)a.ac_zc * d;
+                Sg = -(R)a.lambda * upf * d * d * (R)a.pf_dt;
+            } else {
+                R as;
+                const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
+                Sg = phase_source<R>(P, upf, an, (R)a.lambda, (R)a.pf_dt, (R)a.pf_wdx2, Mrot,
+                                     rot, as);
+            }
             // the populations are read after the source so that the stencil values are dead
             // (loaded with the other lattices, they measured 13 % slower at 256^3)
             if constexpr (Z)
@@ -389,8 +407,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             for (int i = 0; i < PQ; i++) {
                 const R w = (R)wgt<PQ>(i);
                 const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+                const R cn = cx(i) * acn0 + cy(i) * acn1 + cz(i) * acn2;
                 const R geq = w * (dp + pa * cu);
-                g[i] = g[i] - om * (g[i] - geq) + w * Sg;
+                g[i] = g[i] - om * (g[i] - geq) + w * (Sg + acz * cn);
             }
             if constexpr (Z)
                 zpush_stage<PQ, R, FACES>(g, a.g, S, c, a, kKindScalar, zrf, zmg, tid, act,
@@ -435,6 +454,6 @@ inline void run_phase_moments(int odd, const StepArgs& a, void* phi_out, hipStre
 #define MKG_COMBOS(X)                                                                       \
     X(19, 0, 0, 0) X(27, 0, 0, 0) X(19, 0, 1, 0) X(27, 0, 1, 0) X(0, 19, 1, 0) X(0, 7, 1, 0) \
     X(19, 19, 1, 0) X(19, 7, 1, 0) X(27, 19, 1, 0) X(19, 19, 0, 1) X(19, 7, 0, 1)           \
-    X(0, 19, 0, 1) X(0, 0, 1, 0) X(0, 0, 0, 1) X(0, 19, 0, 0) X(0, 7, 0, 0)
+    X(0, 19, 0, 1) X(0, 0, 1, 0) X(0, 0, 0, 1) X(0, 19, 0, 0) X(0, 7, 0, 0) X(19, 19, 0, 0)
 
 }  // namespace mkg

@@ -21,6 +21,7 @@ OP_BGK, OP_TRT, OP_MRT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW, BC_PRESSURE = 0, 1, 2, 3, 4
 ANISO_CUBIC, ANISO_CBC, ANISO_HCP, ANISO_FACET = 0, 1, 2, 3
+PF_KARMA_RAPPEL, PF_ALLEN_CAHN = 0, 1
 FLOW, PHASE, THERMAL, SOLUTE = 1, 2, 4, 8
 ALL = FLOW | PHASE | THERMAL | SOLUTE
 F_POP_FLOW, F_POP_PHASE, F_POP_THERMAL, F_POP_SOLUTE = 0, 1, 2, 3
@@ -46,7 +47,7 @@ PARAM_FIELDS = [
     ("C_face", C.c_double * 6),
     ("pf_dt", C.c_double), ("pf_wdx2", C.c_double), ("eps4", C.c_double),
     ("lambda_", C.c_double), ("phi_s", C.c_double),
-    ("aniso_kind", C.c_int32), ("pad1", C.c_int32), ("aniso", C.c_double * 2),
+    ("aniso_kind", C.c_int32), ("pf_model", C.c_int32), ("aniso", C.c_double * 2),
     ("ref_f", C.c_double), ("ref_g", C.c_double), ("ref_h", C.c_double),
     ("ref_c", C.c_double),
     ("rot", (C.c_double * 9) * MAX_GRAINS),

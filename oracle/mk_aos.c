@@ -67,7 +67,7 @@ static long aos_at(const orc_params* p, long id, int dx, int dy, int dz) {
 }
 
 static int aos_classify(const orc_params* p, int old, double phi) {
-    if (old == 3 || old < 0) return old;
+    if (old == 3 || old < 0 || p->pf_model != 0) return old;
     if (phi >= p->phi_s) return 2;
     if (phi > -p->phi_s) return 1;
     return 0;

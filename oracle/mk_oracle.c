@@ -604,7 +604,7 @@ static void stream(const orc_params* p, const orc_state* s, int q, const int* bc
 
 /* ------------------------------------------------------------------ flags */
 static int8_t classify(const orc_params* p, int8_t old, double phi) {
-    if (old == 3 || old < 0) return old;
+    if (old == 3 || old < 0 || p->pf_model != 0) return old; /* Allen-Cahn keeps every flag */
     if (phi >= p->phi_s) return 2;
     if (phi > -p->phi_s) return 1;
     return 0;
@@ -697,16 +697,35 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                     if (p->pf_coupling == 1 && p->thermal) upf = m.T[id];
                     if (p->pf_coupling == 2 && p->solute) upf = m.U[id];
                     double S, a;
-                    orc_phase_source(p, phi27, upf, s->grain ? s->grain[id] : 0, &S, &a);
+                    double nac[3] = {0, 0, 0}, zac = 0;
                     double ph = s->phi[id];
+                    if (p->pf_model == 1) {
+                        /* conservative Allen-Cahn (relaxation_AllenCahn): counter-term
+                         * (1 - omega/2) w_i (c_i . n)(1 - phi^2) dx / (sqrt 2 W0) of Chai et
+                         * al. (2018), n = s / (|s| + 1e-12), s = sum_i c_i w_i phi(x + c_i) on
+                         * D3Q19 weights; growth source -lambda u (1 - phi^2)^2 */
+                        lat_t L19 = make_lat(19);
+                        double s3[3] = {0, 0, 0};
+                        for (int k = 1; k < 19; k++)
+                            for (int d = 0; d < 3; d++)
+                                s3[d] += E27[k][d] * L19.w[k] * P27(E27[k][0], E27[k][1], E27[k][2]);
+                        double sn = sqrt(s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]) + 1e-12;
+                        for (int d = 0; d < 3; d++) nac[d] = s3[d] / sn;
+                        double dd = 1.0 - ph * ph;
+                        S = -p->lambda * upf * dd * dd * p->pf_dt;
+                        zac = (1.0 - 0.5 * omg) * dd / (sqrt(2.0) * sqrt(p->pf_wdx2));
+                    } else {
+                        orc_phase_source(p, phi27, upf, s->grain ? s->grain[id] : 0, &S, &a);
+                    }
                     const double* u = uadv + 3 * id;
                     for (int i = 0; i < q; i++) {
                         double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                        double cn = E27[i][0] * nac[0] + E27[i][1] * nac[1] + E27[i][2] * nac[2];
                         /* the advective flux is (phi + 1) u: the liquid (phi = -1) is left
                          * unchanged by a velocity field with div u != 0 */
                         double geq = L.w[i] * (ph + (ph + 1.0) * cu / L.cs2);
                         double gi = s->g[i * N + id];
-                        post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * S;
+                        post[i * N + id] = gi - omg * (gi - geq) + L.w[i] * (S + zac * cn);
                     }
                 }
         stream(p, s, q, p->bc_g, 1, NULL, NULL, NULL, NULL, post, s->g, fresh);
@@ -882,7 +901,7 @@ void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* cent
                         for (int i = 0; i < L.q; i++) s->g[i * NCELL(p) + id] = L.w[i] * ph;
                 }
                 if (s->grain) s->grain[id] = (int16_t)gbest;
-                if (s->flag[id] != 3 && s->flag[id] >= 0)
+                if (s->flag[id] != 3 && s->flag[id] >= 0 && p->pf_model == 0)
                     s->flag[id] = (best <= -ts) ? 2 : (best < ts ? 1 : 0);
             }
 }

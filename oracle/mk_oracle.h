@@ -37,7 +37,7 @@ typedef struct orc_params {
     double at_coef;
     double C_face[6];
     double pf_dt, pf_wdx2, eps4, lambda, phi_s;
-    int32_t aniso_kind, pad1;
+    int32_t aniso_kind, pf_model;
     double aniso[2];
     double ref_f, ref_g, ref_h, ref_c;
     double rot[ORC_MAX_GRAINS][9];

@@ -18,10 +18,12 @@ P, BB, DI, OUT, PR = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_O
 
 
 def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
-                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0, incomp=0, aniso=None):
+                prec=mkg.PREC_FP64, dims=(10, 9, 8), ngrains=0, kappa=0, incomp=0, aniso=None,
+                model=mkg.PF_KARMA_RAPPEL):
     p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], flow_q=flow_q, phase_q=phase_q,
                            thermal=thermal, solute=solute, precision=prec, ngrains=ngrains,
                            has_kappa=kappa, flow_incomp=incomp)
+    p.pf_model = model
     if aniso is not None:
         p.aniso_kind = aniso[0]
         p.aniso[0], p.aniso[1] = aniso[1]
@@ -93,6 +95,12 @@ CASES = [
          aniso=(mkg.ANISO_HCP, (0.03, -0.05))),
     dict(id="fused_facet", flow_q=19, phase_q=19, thermal=1, op="trt",
          aniso=(mkg.ANISO_FACET, (0.06, 0.1))),
+    # conservative Allen-Cahn interface carried by the flow (run_flow_AllenCahn), with thermal
+    # growth (relaxation_AllenCahn_growth_thermal), and on D3Q7
+    dict(id="ac_flow_mrt", flow_q=19, phase_q=19, op="mrt", model=mkg.PF_ALLEN_CAHN),
+    dict(id="ac_growth_thermal", flow_q=19, phase_q=19, thermal=1, op="trt",
+         model=mkg.PF_ALLEN_CAHN),
+    dict(id="ac_pf7_thermal", phase_q=7, thermal=1, model=mkg.PF_ALLEN_CAHN),
 ]
 
 

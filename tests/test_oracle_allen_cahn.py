@@ -1,0 +1,105 @@
+"""Pins of the oracle's conservative Allen-Cahn phase model (relaxation_AllenCahn /
+relaxation_AllenCahn_growth_thermal, MaterKit/inc/mkPhaseField.h:39-40; run_flow_AllenCahn,
+MaterKit/MaterKit.cpp:186-242):
+
+- with no flow and no growth it is the one-order-parameter multi-phase field at rest: the
+  fused step's phase lattice and the multi-phase path are separate code in the oracle and
+  must agree to round-off;
+- phi is conserved exactly (periodic, no growth) and flags are never re-classified;
+- an interface carried by a uniform flow moves with the flow;
+- the growth source -lambda u (1 - phi^2)^2 makes an undercooled (u < 0) disk grow and a
+  superheated one shrink.
+"""
+import numpy as np
+import pytest
+
+from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+from oracle import oracle as O
+
+
+def ac_params(dims, flow=False, thermal=False, **kw):
+    p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], phase_q=19,
+                           flow_q=19 if flow else 0, thermal=1 if thermal else 0,
+                           precision=mkg.PREC_FP64, pf_model=mkg.PF_ALLEN_CAHN)
+    p.pf_dt, p.pf_wdx2, p.lambda_, p.phi_s = 0.008, 6.25, 3.19, 0.99
+    p.pf_coupling = 1 if thermal else 0
+    if flow:
+        p.s_nu = p.s_bulk = p.s_odd = p.s_even = 1.0 / 0.8
+    if thermal:
+        p.alpha_l = p.alpha_s = 0.1
+        p.latent_T = 0.0
+    p.set(**kw)
+    return p
+
+
+def disk(dims, c, r, wdx=2.5):
+    return cases.mpf_seed_fields(dims, [c], [r], wdx)[0]
+
+
+def start(p, phi, u=None, T=None):
+    s = O.State.empty(p)
+    N = p.ncell
+    vel = None
+    if u is not None:
+        vel = np.tile(np.asarray(u, dtype=float), N)
+    s.init_equilibrium(rho=None, vel=vel, T=T, phi=phi)
+    return s
+
+
+def test_allen_cahn_phase_equals_one_order_parameter_multi_phase_field():
+    dims = (30, 26, 1)
+    p = ac_params(dims)
+    phi0 = disk(dims, [12, 13, 0], 7)
+    s = start(p, phi0)
+    s.step(60, mask=mkg.PHASE)
+    m = mkg.default_mpf_params(nx=dims[0], ny=dims[1], nz=dims[2], n=1, dt=p.pf_dt, wdx=2.5)
+    m.u[0][0] = 0.0
+    m.kappa[0] = 0.0
+    ms = O.MpfState.start(m, phi0[None, :])
+    ms.step(60)
+    N = p.ncell
+    assert np.abs(s.phi - ms.phi.reshape(2, N)[0]).max() < 1e-13
+    assert np.abs(s.g - ms.g).max() < 1e-13
+
+
+def test_allen_cahn_conserves_phi_and_keeps_flags():
+    dims = (24, 22, 6)
+    p = ac_params(dims, flow=True)
+    phi0 = disk(dims, [11, 10, 3], 5)
+    s = start(p, phi0, u=[0.01, -0.02, 0.005])
+    s.init_seeds(np.array([[11, 10, 3]]), np.array([5.0]))
+    assert (s.flag == 0).all()
+    total0 = s.g.reshape(19, -1).sum()
+    s.step(80, mask=mkg.FLOW | mkg.PHASE)
+    assert (s.flag == 0).all()
+    assert abs(s.g.reshape(19, -1).sum() - total0) < 1e-10 * p.ncell
+
+
+def test_allen_cahn_interface_moves_with_a_uniform_flow():
+    dims = (64, 48, 1)
+    p = ac_params(dims, flow=True)
+    u = [0.03, 0.015, 0.0]
+    phi0 = disk(dims, [22, 22, 0], 8)
+    s = start(p, phi0, u=u)
+    steps = 300
+    s.step(steps, mask=mkg.FLOW | mkg.PHASE)
+    X, Y = np.meshgrid(np.arange(64), np.arange(48), indexing="ij")
+
+    def centroid(ph):
+        w = ph.reshape(64, 48) + 1.0
+        return np.array([(X * w).sum(), (Y * w).sum()]) / w.sum()
+
+    d = centroid(s.phi) - centroid(phi0)
+    np.testing.assert_allclose(d, [u[0] * steps, u[1] * steps], rtol=0.03)
+
+
+@pytest.mark.parametrize("T,grows", [(-0.3, True), (0.3, False)])
+def test_allen_cahn_growth_source_sign(T, grows):
+    dims = (40, 40, 1)
+    p = ac_params(dims, thermal=True)
+    phi0 = disk(dims, [20, 20, 0], 8)
+    s = start(p, phi0, T=np.full(p.ncell, T))
+    s.step(200, mask=mkg.PHASE | mkg.THERMAL)
+    vol0 = (phi0 + 1).sum()
+    vol = (s.phi + 1).sum()
+    assert (vol > vol0 * 1.01) if grows else (vol < vol0 * 0.99)
