@@ -36,6 +36,7 @@ void set_error(const std::string& msg);
 
 struct MpfArgs {
     StepArgs a;    // geometry, flags and walls; a.g = lattice of order parameter 0, a.h thermal
+    LatArgs gl[MKG_MPF_MAX];  // lattice of each order parameter (a.g with its own base)
     uint64_t lat;  // elements from one order parameter's lattice to the next: 19 (S + F)
     int32_t n, thermal;
     double u[MKG_MPF_MAX][3];
@@ -45,13 +46,6 @@ struct MpfArgs {
     double zc;       // (1 - omega_g / 2) / (sqrt 2 W0/dx): scale of the Allen-Cahn counter-term
     double dt, omega_h, latent_T, T_ref;
 };
-
-template <class R>
-__device__ inline LatArgs lattice_of(const MpfArgs& m, int j) {
-    LatArgs L = m.a.g;
-    L.pop = (R*)m.a.g.pop + (size_t)j * m.lat;
-    return L;
-}
 
 template <class R, bool ODD>
 __global__ void __launch_bounds__(256) k_mpf_moments(const MpfArgs m, R* __restrict__ out) {
@@ -68,7 +62,7 @@ __global__ void __launch_bounds__(256) k_mpf_moments(const MpfArgs m, R* __restr
 #pragma unroll
     for (int j = 0; j < MKG_MPF_MAX; j++) {
         if (j < m.n) {
-            const LatArgs L = lattice_of<R>(m, j);
+            const LatArgs& L = m.gl[j];  // j is unrolled: a fixed offset, no copy
             R g[19];
             read_pops<19, ODD, R>(g, L, S, c, m.a, kKindScalar);
             R s = 0;
@@ -106,6 +100,10 @@ __device__ inline void mpf_gradient(const StepArgs& a, const Cell& c, const R* p
     }
 }
 
+// No occupancy bound: the FP32 kernel takes 168-199 VGPRs (two or three waves per SIMD);
+// bounded to three waves it spills and measured 7 % slower at 256^3.  O steps access the
+// shifted z populations directly: the z stage of the fused step (LDS, two barriers per order
+// parameter) measured 7-13 % slower here, at two waves per SIMD.
 template <class R, bool TH, bool ODD>
 __global__ void __launch_bounds__(256) k_mpf_step(const MpfArgs m) {
     Cell c;
@@ -162,7 +160,7 @@ __global__ void __launch_bounds__(256) k_mpf_step(const MpfArgs m) {
             const R src = (R)m.dt * (P + Q);
             const R ux = (R)m.u[j][0], uy = (R)m.u[j][1], uz = (R)m.u[j][2];
             const R uu = ux * ux + uy * uy + uz * uz;
-            const LatArgs L = lattice_of<R>(m, j);
+            const LatArgs& L = m.gl[j];  // j is unrolled: a fixed offset, no copy
             const R dp = pj - (R)L.ref;
             const R zd = (R)m.zc * d;
             R g[19];
@@ -295,6 +293,10 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
         L.ref = ref;
     };
     fill(a.g, h->g, p.bc, -1.0, p.phi_face);
+    for (int j = 0; j < MKG_MPF_MAX; j++) {
+        m.gl[j] = a.g;
+        if (j < p.n) m.gl[j].pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+    }
     if (h->h) fill(a.h, h->h, p.bc_h, p.T_ref, p.T_face);
     a.phi_cur = h->phi[h->pf_count & 1];
     a.phi_prev = h->phi[(h->pf_count + 1) & 1];
