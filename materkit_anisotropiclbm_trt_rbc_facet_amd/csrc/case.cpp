@@ -304,9 +304,9 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
     }
     p->flow_incomp = eq == "incompressible" ? 1 : 0;
     // phase-field model: Karma-Rappel dendrite growth or the conservative Allen-Cahn interface
-    const std::string pm = c.word("solver/phaseModel", "karmaRappel");
-    if (pm == "karmaRappel") p->pf_model = MKG_PF_KARMA_RAPPEL;
-    else if (pm == "allenCahn") p->pf_model = MKG_PF_ALLEN_CAHN;
+    const std::string pm = c.word("solver/phaseModel", "karmarappel");
+    if (pm == "karmarappel") p->pf_model = MKG_PF_KARMA_RAPPEL;
+    else if (pm == "allencahn") p->pf_model = MKG_PF_ALLEN_CAHN;
     else {
         g_case_err = "solver/phaseModel must be karmaRappel or allenCahn";
         return -1;

@@ -6,12 +6,16 @@
 //
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
-//         rbc | fused
+//         rbc | allen_cahn | multi_phase_field | fused
+// (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
+// run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser)
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mkshim.h"
 
@@ -30,6 +34,40 @@ static void frame_info(mkFluidsMRT& flow, int frame, int loops) {
     std::printf("\n");
 }
 
+// run_multiPhaseField (MaterKit/MaterKit.cpp:529-576) on the multi-phase user class
+static int run_multi_phase_field(const std::string& xml, int maxLoops, int maxFrame, bool vtk) {
+    mkSpace space(xml);
+    mkFluidsUser flow(space);
+    flow.initialize(xml);
+    flow.initialize_geometry();
+    std::printf("%20s= %d \n%20s= %d \n\nframe \t\t\t loops\n", "maxLoops ", maxLoops,
+                "maxFrame ", maxFrame);
+    const int every = std::max(1, maxLoops / maxFrame);
+    const mkg_mpf_params& m = flow.mpf_params();
+    std::vector<double> sums(m.n + 1);
+    int frame = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int loops = 0; loops <= maxLoops; loops++) {
+        if (loops % every == 0) {
+            mkg_mpf_sums(flow.mpf(), sums.data());
+            std::printf("%05d\t%18d", frame, loops);
+            for (int j = 0; j < m.n; j++) std::printf("\tsum_phi_%d %.8e", j, sums[j]);
+            std::printf("\n");
+            if (vtk) flow.write_vtkfile(loops);
+            frame++;
+        }
+        if (loops == maxLoops) break;
+        flow.propagate();
+        flow.relaxation();
+        flow.mkThermalField::propagate();
+        flow.mkThermalField::relaxation();
+    }
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("wall %.3f s, %.1f MLUPS (including frame output)\n", s,
+                (double)space.Mx() * space.My() * space.Mz() * maxLoops / s / 1e6);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s <case.xml> <loop> [maxLoops] [maxFrame] [vtk]\n", argv[0]);
@@ -43,6 +81,7 @@ int main(int argc, char** argv) {
         if (argc > 4) maxFrame = std::atoi(argv[4]);
         const bool vtk = argc > 5 && std::strcmp(argv[5], "vtk") == 0;
         if (maxFrame < 1) maxFrame = 1;
+        if (loop == "multi_phase_field") return run_multi_phase_field(xml, maxLoops, maxFrame, vtk);
         mkSpace space(xml);
         mkFluidsMRT flow(space);
         flow.mkFluidsMRT::initialize(xml);
@@ -94,6 +133,11 @@ int main(int argc, char** argv) {
                 flow.mkThermalField::propagate();
                 flow.mkThermalField::relaxation();
                 flow.mkThermalField::bdy_RBC();
+            } else if (loop == "allen_cahn") {
+                flow.mkFluidsMRT::propagate();
+                flow.mkFluidsMRT::relaxation();
+                flow.mkPhaseField::propagate();
+                flow.mkPhaseField::relaxation_AllenCahn();
             } else if (loop == "fused") {
                 flow.step(1);
             } else {

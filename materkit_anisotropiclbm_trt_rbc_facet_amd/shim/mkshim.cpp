@@ -228,3 +228,110 @@ void mkFluidsLBM::write_vtkfile(int loops) {
                 }
     }
 }
+
+// ------------------------------------------------------------------ multi-phase field
+mkFluidsUser::~mkFluidsUser() {
+    if (m_) mkg_mpf_destroy(m_);
+}
+
+void mkFluidsUser::initialize(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_params_from_xml(xmlfname.c_str(), &p_, run_info_)) throw_io("initialize");
+    const int n = run_info_[2];
+    if (n < 1 || n > MKG_MPF_MAX)
+        throw mkt::mkLogicException("the multi-phase field needs 1.." + std::to_string(MKG_MPF_MAX) +
+                                    " seeds (one order parameter each)");
+    check(mkg_mpf_params_default(&mp_));
+    mp_.nx = p_.nx;
+    mp_.ny = p_.ny;
+    mp_.nz = p_.nz;
+    mp_.n = n;
+    mp_.precision = p_.precision;
+    mp_.thermal = p_.thermal;
+    mp_.dt = p_.pf_dt;
+    mp_.wdx = std::sqrt(p_.pf_wdx2);
+    mp_.alpha = p_.alpha_l;
+    mp_.latent_T = p_.latent_T;
+    mp_.T_ref = p_.T_ref;
+    for (int f = 0; f < 6; f++) {
+        mp_.bc[f] = p_.bc_g[f];
+        mp_.bc_h[f] = p_.bc_h[f];
+        mp_.T_face[f] = p_.T_face[f];
+    }
+    if (m_) mkg_mpf_destroy(m_);
+    m_ = nullptr;
+    check(mkg_mpf_create(&mp_, &m_));
+}
+
+void mkFluidsUser::initialize_geometry() {
+    if (!m_) throw mkt::mkLogicException("initialize_geometry before initialize");
+    std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS);
+    const int n = mkg_seeds_from_xml(xml_.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), nullptr);
+    if (n < 0) throw_io("seeds");
+    // phi_j = -tanh((|x - c_j| - r_j) / (sqrt 2 W0/dx)): one order parameter per seed
+    const size_t N = (size_t)mp_.nx * mp_.ny * mp_.nz;
+    std::vector<double> phi((size_t)mp_.n * N);
+    const double width = std::sqrt(2.0) * mp_.wdx;
+    for (int j = 0; j < mp_.n; j++)
+        for (int x = 0; x < mp_.nx; x++)
+            for (int y = 0; y < mp_.ny; y++)
+                for (int z = 0; z < mp_.nz; z++) {
+                    const double dx = x - c[3 * j], dy = y - c[3 * j + 1], dz = z - c[3 * j + 2];
+                    const double d = std::sqrt(dx * dx + dy * dy + dz * dz) - r[j];
+                    phi[(size_t)j * N + ((size_t)x * mp_.ny + y) * mp_.nz + z] = -std::tanh(d / width);
+                }
+    check(mkg_mpf_init(m_, phi.data(), nullptr));
+}
+
+void mkFluidsUser::advance(int mask) {
+    if (!m_) throw mkt::mkLogicException("relaxation before initialize");
+    // the thermal lattice steps with the order parameters
+    if (mask & (MKG_PHASE | MKG_FLOW)) check(mkg_mpf_step(m_, 1));
+    pending_ &= ~mask;
+}
+
+std::vector<double> mkFluidsUser::order_parameters() const {
+    const size_t N = (size_t)mp_.nx * mp_.ny * mp_.nz;
+    std::vector<double> v((size_t)(mp_.n + 1) * N);
+    check(mkg_mpf_get(m_, MKG_MPF_PHI, v.data(), v.size()));
+    return v;
+}
+
+void mkFluidsUser::write_vtkfile(int loops) {
+    const int Mx = mp_.nx, My = mp_.ny, Mz = mp_.nz;
+    const size_t N = (size_t)Mx * My * Mz;
+    std::ostringstream fn;
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "vtk", ec);
+    fn << workspace_ << "vtk/fluid_t" << loops << ".vtk";
+    std::ofstream o(fn.str());
+    if (!o) throw mkt::mkIOException("cannot write " + fn.str());
+    o << "# vtk DataFile Version 3.0\nfluid_state\nASCII\nDATASET RECTILINEAR_GRID\n";
+    o << "DIMENSIONS " << Mx << " " << My << " " << Mz << "\n";
+    auto coords = [&](const char* nm, int m) {
+        o << nm << " " << m << " float\n";
+        for (int i = 0; i < m; i++) o << i + 0.5 << " ";
+        o << "\n";
+    };
+    coords("X_COORDINATES", Mx);
+    coords("Y_COORDINATES", My);
+    coords("Z_COORDINATES", Mz);
+    o << "POINT_DATA " << N << "\n" << std::setprecision(9);
+    auto scalar = [&](const std::string& name, const double* v) {
+        o << "SCALARS " << name << " float 1\nLOOKUP_TABLE default\n";
+        for (int z = 0; z < Mz; z++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++) o << v[((size_t)x * My + y) * Mz + z] << "\n";
+    };
+    const std::vector<double> ph = order_parameters();
+    scalar("order_parameter", ph.data() + (size_t)mp_.n * N);
+    for (int j = 0; j < mp_.n; j++)
+        scalar("order_parameter_" + std::to_string(j), ph.data() + (size_t)j * N);
+    if (mp_.thermal) {
+        std::vector<double> h(7 * N), T(N, 0.0);
+        check(mkg_mpf_get(m_, MKG_MPF_THERMAL, h.data(), h.size()));
+        for (int i = 0; i < 7; i++)
+            for (size_t k = 0; k < N; k++) T[k] += h[(size_t)i * N + k];
+        scalar("temperature", T.data());
+    }
+}

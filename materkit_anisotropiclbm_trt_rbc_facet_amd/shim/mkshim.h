@@ -89,7 +89,7 @@ public:
     void set_workspace(const std::string& dir) { workspace_ = dir; }
 
 protected:
-    void advance(int mask);
+    virtual void advance(int mask);
     void check(int rc) const;
     mkSpace space_;
     std::string xml_;
@@ -139,7 +139,20 @@ public:
     void relaxation() override { advance(MKG_PHASE); }
     void relaxation_anisotropy_thermal() { advance(MKG_PHASE); }
     void relaxation_anisotropy_solute(const int /*loops*/) { advance(MKG_PHASE); }
+    // conservative Allen-Cahn interface (MaterKit/inc/mkPhaseField.h:39-40); the case file
+    // must select <solver><phaseModel> allenCahn </phaseModel>
+    void relaxation_AllenCahn() { allen_cahn(); }
+    void relaxation_AllenCahn_growth_thermal() { allen_cahn(); }
     void reset_flags() {}  // flags are re-classified inside every phase step
+
+private:
+    void allen_cahn() {
+        if (p_.pf_model != MKG_PF_ALLEN_CAHN)
+            throw mkt::mkLogicException(
+                "relaxation_AllenCahn: the case file selects the Karma-Rappel phase model "
+                "(<solver><phaseModel> allenCahn </phaseModel> selects Allen-Cahn)");
+        advance(MKG_PHASE);
+    }
 };
 
 class mkFluidsMRT : virtual public mkFluidsLBM,
@@ -155,6 +168,35 @@ public:
     void relaxation() override { advance(MKG_FLOW); }
     void initialize(const std::string& xmlfname) override { mkFluidsLBM::initialize(xmlfname); }
     void initialize_geometry() override { mkFluidsLBM::initialize_geometry(); }
+};
+
+// Multi-phase field of the user-extension class (MaterKit/inc/mkFluidsUser.h;
+// propagate / relaxation of MaterKit/src/mkFluidsUser.cpp:68-299): one order parameter per
+// seed of the case file (at most MKG_MPF_MAX), each on its own D3Q19 lattice, plus the
+// case's D3Q7 thermal lattice, on an mkg_mpf handle.  relaxation() advances every lattice by
+// one step; the thermal calls of run_multiPhaseField (mkThermalField::propagate / relaxation)
+// are then no-ops, because the thermal lattice steps with the order parameters.
+class mkFluidsUser : public mkFluidsMRT {
+public:
+    mkFluidsUser() = default;
+    explicit mkFluidsUser(const mkSpace& space) : mkFluidsLBM(space), mkFluidsMRT(space) {}
+    ~mkFluidsUser() override;
+    void initialize(const std::string& xmlfname) override;
+    void initialize_geometry() override;
+    void propagate() override { pending_ |= MKG_PHASE; }
+    void relaxation() override { advance(MKG_PHASE); }
+    void write_vtkfile(int loops) override;
+    mkg_mpf_handle mpf() const { return m_; }
+    const mkg_mpf_params& mpf_params() const { return mp_; }
+    // phi_j (n x N) and the combined phi (N) at the last moment pass
+    std::vector<double> order_parameters() const;
+
+protected:
+    void advance(int mask) override;
+
+private:
+    mkg_mpf_handle m_ = nullptr;
+    mkg_mpf_params mp_{};
 };
 
 // readParameters of the reference (MaterKit/inc/mkFunction.h:62): loops and frames.

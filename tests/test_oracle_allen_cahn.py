@@ -103,3 +103,15 @@ def test_allen_cahn_growth_source_sign(T, grows):
     vol0 = (phi0 + 1).sum()
     vol = (s.phi + 1).sum()
     assert (vol > vol0 * 1.01) if grows else (vol < vol0 * 0.99)
+
+
+def test_phase_model_tag_of_the_case_file(tmp_path):
+    c = cases.load("allen_cahn_flow.xml")
+    assert c.params.pf_model == mkg.PF_ALLEN_CAHN and c.params.flow_q == 19
+    assert c.params.s_bulk != c.params.s_nu  # MRT rates
+    assert cases.load("C3").params.pf_model == mkg.PF_KARMA_RAPPEL
+    bad = tmp_path / "bad.xml"
+    bad.write_text((cases.CASES_DIR / "allen_cahn_flow.xml").read_text().replace(
+        "allenCahn", "cahnHilliard"))
+    with pytest.raises(mkg.MkgError, match="phaseModel"):
+        cases.load(str(bad))
