@@ -13,10 +13,13 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -45,6 +48,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--no-mpf", action="store_true")
+    ap.add_argument("--no-fvm", action="store_true")
     ap.add_argument("--mpf-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run three plain 1 GiB device copies (the reference kernel for "
@@ -175,6 +179,42 @@ def mpf_cost(size: int, steps: int) -> dict:
             "step_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def fvm_cost(size: int, steps: int) -> dict:
+    """Explicit finite-volume phase field + solute with anti-trapping current (FP32) on a size^3
+    grid, one seed at the centre growing into U = -0.5.  Algorithmic bytes per cell: phase
+    kernel phi, c read and phi' written; solute kernel phi, phi', c read and c' written."""
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
+
+    p = mkg.default_params(nx=size, ny=size, nz=size, phase_q=19, solute=1, pf_coupling=2,
+                           eps4=0.05, pf_dt=0.008, pf_wdx2=6.25, D_l=0.1, D_s=0.0, k_part=0.15,
+                           q_min=1e-3, at_coef=0.5 / math.sqrt(2.0), precision=mkg.PREC_FP32,
+                           **{"lambda": 3.19})
+    x = np.arange(size) - size / 2.0
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    phi = -np.tanh((r - 8.0) / (math.sqrt(2.0) * 2.5)).reshape(-1)
+    dev = mkg.FvmSolver(p)
+    dev.set(mkg.FVM_PHI, phi)
+    dev.set(mkg.FVM_CONC, -0.5 * 0.5 * ((1 + 0.15) - (1 - 0.15) * phi))
+    del phi, r
+    dev.step(4)
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps)
+    dev.sync()
+    dt = (time.perf_counter() - t0) / steps
+    dev.close()
+    N = size ** 3
+    bytes_cell = (3 + 4) * 4
+    gbs = bytes_cell * N / dt / 1e9
+    return {"workload": f"finite-volume phase field + solute (anti-trapping), {size}^3, FP32",
+            "mlups": round(N / dt / 1e6, 1), "ms_per_step": round(dt * 1e3, 4),
+            "alg_bytes_per_cell": bytes_cell, "step_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(seconds: float):
     """The reference-structured FP64 CPU path (oracle/mk_aos.c: AoS node records of the
     reference's 3640 B layout, one OpenMP sweep per operator) on a 256 x 256 x 16 slab of the
@@ -285,6 +325,8 @@ def main():
         out["fp32_over_fp64"] = round(mlups32 / mlups64, 3)
     if not a.no_mpf:
         out["mpf"] = mpf_cost(size, a.mpf_steps)
+    if not a.no_fvm:
+        out["fvm"] = fvm_cost(size, a.mpf_steps)
     if not a.no_adjoint:
         adj = adjoint_cost(a.adjoint_steps)
         out["adjoint_over_forward"] = adj["ratio"]

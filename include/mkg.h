@@ -308,6 +308,33 @@ int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count);
 /* FP64 sums over the non-wall cells of phi_j (n values) and of the combined phi. */
 int mkg_mpf_sums(mkg_mpf_handle h, double* out);
 
+/* ---- Finite-volume phase field and solute ----------------------------------------------
+ * Explicit cell-centred forms of mkPhaseField::evolution_anisotropy
+ * (MaterKit/inc/mkPhaseField.h:51, "on FVM") and of mkSolutalField::evolution_ATC1 /
+ * evolution_ATC2 (MaterKit/inc/mkSolutalField.h:44-46), the solute update of run_solutal_only
+ * (MaterKit/MaterKit.cpp:689-741).  Fields: phi and c = A(phi) U (A = ((1+k) - (1-k) phi)/2),
+ * N values each.  A step: phi' = phi + dt/a^2 (W^2/dx^2 div J + Q) with the face fluxes and
+ * anisotropy of the lattice solver, then c' = c - sum_faces J_out + (phi' - phi)/2 with the
+ * diffusive flux D q(phi) grad U and the anti-trapping current.  The handle takes the phase and
+ * solute parameters of an mkg_params block (pf_dt, pf_wdx2, eps4, lambda, aniso_*, D_l, D_s,
+ * k_part, at_coef, q_min; faces bc_g: periodic or bounce = zero flux; bc_c: periodic, bounce or
+ * Dirichlet with C_face as the face value of U); ngrains must be 0. */
+typedef struct mkg_fvm_ctx* mkg_fvm_handle;
+#define MKG_FVM_PHI 0   /* N, phase field */
+#define MKG_FVM_CONC 1  /* N, c = A(phi) U */
+#define MKG_FVM_U 2     /* N, U = c / A(phi) (get only) */
+
+int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out);
+int mkg_fvm_destroy(mkg_fvm_handle h);
+/* Set a field (MKG_FVM_PHI or MKG_FVM_CONC, N doubles) at the current level. */
+int mkg_fvm_set(mkg_fvm_handle h, int which, const double* host, size_t count);
+/* nsteps explicit steps; evolve_phase 0 holds phi fixed (run_solutal_only). */
+int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase);
+int mkg_fvm_sync(mkg_fvm_handle h);
+int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count);
+/* FP64 sums of phi and of c over every cell (out[0], out[1]). */
+int mkg_fvm_sums(mkg_fvm_handle h, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,377 @@
+// Explicit finite-volume solvers of the phase field (mkPhaseField::evolution_anisotropy,
+// MaterKit/inc/mkPhaseField.h:51, "on FVM") and of the dilute-alloy solute with the
+// anti-trapping current (mkSolutalField::evolution_ATC1 / evolution_ATC2,
+// MaterKit/inc/mkSolutalField.h:44-46; run_solutal_only, MaterKit/MaterKit.cpp:689-741).
+//
+// One step is two launches over cell-centred fields (no populations):
+//   k_fvm_phase   phi' = phi + dt/a^2 (W^2/dx^2 div J + Q): the face fluxes, anisotropy and
+//                 source of the lattice solver's phase_source (device.h) plus the 7-point
+//                 Laplacian that the lattice's diffusion would otherwise carry
+//   k_fvm_solute  c' = c - sum_faces J_out + (phi' - phi)/2 with c = A(phi) U,
+//                 J_out = -D q_f (U_nb - U) + s j_at,a; j_at = -a_t W/dx (1 + (1-k) U_f) dphi_f
+//                 g/|g| on the face (g: normal difference, transverse means of phi)
+// Both read the old fields and write the other buffer of a pair; c - phi/2 is conserved.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/mkg.h"
+#include "device.h"
+
+namespace mkg {
+
+void set_error(const std::string& msg);
+
+template <class R>
+__device__ inline R fvm_q(const StepArgs& a, R phi) {
+    R q = (R)0.5 * ((R)1 - phi);
+    if (a.D_l > 0) q += (R)(a.D_s / a.D_l) * (R)0.5 * ((R)1 + phi);
+    return q < (R)a.q_min ? (R)a.q_min : q;
+}
+template <class R>
+__device__ inline R fvm_A(const StepArgs& a, R phi) {
+    return (R)0.5 * (((R)1 + (R)a.k_part) - ((R)1 - (R)a.k_part) * phi);
+}
+
+// 19-point stencil of phi (periodic phase faces wrap, others replicate the edge).
+template <class R>
+__device__ inline void fvm_stencil(const StepArgs& a, const Cell& c, const R* ph, R (&P)[27]) {
+    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & XLO) px[0] = px[1];
+    if (np & XHI) px[2] = px[1];
+    if (np & YLO) py[0] = py[1];
+    if (np & YHI) py[2] = py[1];
+    if (np & ZLO) pz[0] = pz[1];
+    if (np & ZHI) pz[2] = pz[1];
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
+            *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_fvm_phase(const StepArgs a, const R* __restrict__ phi,
+                                                   const R* __restrict__ cc, R* __restrict__ phin) {
+    Cell c;
+    if (!locate(a, c)) return;
+    R P[27];
+    fvm_stencil<R>(a, c, phi, P);
+    const R p0 = P[13];
+    const R U = *slot_at(cc, c.id) / fvm_A<R>(a, p0);
+    const R M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
+    R as;
+    const R S = phase_source<R>(P, U, an, (R)a.lambda, (R)a.pf_dt, (R)a.pf_wdx2, M, false, as);
+    const R lap = P[22] + P[4] + P[16] + P[10] + P[14] + P[12] - (R)6 * p0;
+    *slot_at(phin, c.id) = p0 + S + (R)(a.pf_dt * a.pf_wdx2) * lap;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_fvm_solute(const StepArgs a, const R* __restrict__ phi,
+                                                    const R* __restrict__ phin,
+                                                    const R* __restrict__ cc,
+                                                    R* __restrict__ cn) {
+    Cell c;
+    if (!locate(a, c)) return;
+    R P[27];
+    fvm_stencil<R>(a, c, phi, P);
+#define PP(dx, dy, dz) P[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
+    const R p0 = P[13];
+    const R U0 = *slot_at(cc, c.id) / fvm_A<R>(a, p0);
+    const R q0 = fvm_q<R>(a, p0);
+    const R d0 = *slot_at(phin, c.id) - p0;
+    const R D = (R)a.D_l, atw = (R)a.at_w, km = (R)1 - (R)a.k_part;
+    const uint32_t npc = (uint32_t)a.c.npmask & c.bits;
+    R out = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int s = sd == 0 ? 1 : -1;
+            const int face = 2 * ax + (s > 0 ? 1 : 0);
+            if (npc & (1u << face)) {
+                // non-periodic solute face: Dirichlet fixes U at the face, others carry no flux
+                if (pick6(face, a.c.bc) == kBcDirichlet)
+                    out += (R)-2 * D * q0 * ((R)pick6(face, a.c.fval) - U0);
+                continue;
+            }
+            const int o0 = ax == 0 ? s : 0, o1 = ax == 1 ? s : 0, o2 = ax == 2 ? s : 0;
+            const uint32_t nb = nbr(c, o0, o1, o2);
+            const R pn = PP(o0, o1, o2);
+            const R Un = *slot_at(cc, nb) / fvm_A<R>(a, pn);
+            const R qf = (R)0.5 * (q0 + fvm_q<R>(a, pn));
+            R J = -D * qf * (Un - U0);
+            if (atw != (R)0) {
+                R g[3];
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    if (b == ax) {
+                        g[b] = (R)s * (pn - p0);
+                    } else {
+                        const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                        g[b] = (R)0.25 * (PP(e0, e1, e2) - PP(-e0, -e1, -e2) +
+                                          PP(o0 + e0, o1 + e1, o2 + e2) -
+                                          PP(o0 - e0, o1 - e1, o2 - e2));
+                    }
+                }
+                const R gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                if (gg > (R)1e-20) {
+                    const R Uf = (R)0.5 * (U0 + Un);
+                    const R df = (R)0.5 * (d0 + (*slot_at(phin, nb) - pn));
+                    const R ja = -atw * ((R)1 + km * Uf) * df * g[ax] / sqrt(gg);
+                    J += (R)s * ja;
+                }
+            }
+            out += J;
+        }
+    }
+#undef PP
+    *slot_at(cn, c.id) = *slot_at(cc, c.id) - out + (R)0.5 * d0;
+}
+
+}  // namespace mkg
+
+using namespace mkg;
+
+struct mkg_fvm_ctx {
+    mkg_params p;
+    int fp32 = 0;
+    size_t esz = 8;
+    uint32_t N = 0, S = 0, nblk = 0;
+    hipStream_t stream = nullptr;
+    void* phi[2] = {nullptr, nullptr};
+    void* c[2] = {nullptr, nullptr};
+    int cur = 0;  // buffer of the current level
+    int8_t* flag = nullptr;   // all fluid: the reductions read flags
+    double* scratch = nullptr;
+    double* partials = nullptr;
+    double* red_out = nullptr;
+};
+
+namespace {
+
+int ffail(const std::string& msg) {
+    set_error(msg);
+    return -1;
+}
+
+#define FCHK(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return ffail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+StepArgs fvm_args(const mkg_fvm_ctx* h) {
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const mkg_params& p = h->p;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.N = h->N;
+    a.S = h->S;
+    a.nblk = h->nblk;
+    a.flag = h->flag;
+    int npg = 0, npc = 0;
+    for (int f = 0; f < 6; f++) {
+        a.g.bc[f] = p.bc_g[f];
+        a.c.bc[f] = p.bc_c[f];
+        a.c.fval[f] = p.C_face[f];
+        if (p.bc_g[f] != MKG_BC_PERIODIC) npg |= 1 << f;
+        if (p.bc_c[f] != MKG_BC_PERIODIC) npc |= 1 << f;
+    }
+    a.g.npmask = npg;
+    a.c.npmask = npc;
+    a.D_l = p.D_l;
+    a.D_s = p.D_s;
+    a.k_part = p.k_part;
+    a.q_min = p.q_min;
+    a.at_w = p.at_coef * std::sqrt(p.pf_wdx2 > 0 ? p.pf_wdx2 : 0.0);
+    a.pf_dt = p.pf_dt;
+    a.pf_wdx2 = p.pf_wdx2;
+    a.eps4 = p.eps4;
+    a.lambda = p.lambda;
+    a.aniso_kind = p.aniso_kind;
+    a.aniso[0] = p.aniso[0];
+    a.aniso[1] = p.aniso[1];
+    return a;
+}
+
+void fvm_destroy_impl(mkg_fvm_ctx* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* bufs[] = {h->phi[0], h->phi[1], h->c[0], h->c[1], h->flag, h->scratch, h->partials,
+                    h->red_out};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
+    if (!out) return ffail("out is null");
+    *out = nullptr;
+    if (!p) return ffail("params is null");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return ffail("grid dimensions must be >= 1");
+    if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
+        return ffail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
+    if (p->ngrains != 0) return ffail("the finite-volume solver has no grain rotations (ngrains 0)");
+    if (p->aniso_kind < MKG_ANISO_CUBIC || p->aniso_kind > MKG_ANISO_FACET)
+        return ffail("aniso_kind out of range");
+    if (p->aniso_kind == MKG_ANISO_FACET && !(p->aniso[1] > 0))
+        return ffail("the facet anisotropy needs a rounding nu > 0");
+    if (!(p->k_part > 0) || !(p->q_min > 0)) return ffail("k_part and q_min must be > 0");
+    for (int f = 0; f < 6; f++) {
+        if (p->bc_g[f] != MKG_BC_PERIODIC && p->bc_g[f] != MKG_BC_BOUNCE)
+            return ffail("finite-volume phase faces are periodic or bounce (zero flux)");
+        if (p->bc_c[f] != MKG_BC_PERIODIC && p->bc_c[f] != MKG_BC_BOUNCE &&
+            p->bc_c[f] != MKG_BC_DIRICHLET)
+            return ffail("finite-volume solute faces are periodic, bounce (zero flux) or Dirichlet");
+    }
+    for (int a = 0; a < 3; a++) {
+        const bool pg = p->bc_g[2 * a] == 0, pc = p->bc_c[2 * a] == 0;
+        if (pg != (p->bc_g[2 * a + 1] == 0) || pc != (p->bc_c[2 * a + 1] == 0))
+            return ffail("a periodic axis needs periodic codes on both faces");
+        if (pg != pc) return ffail("the phase and solute fields must be periodic on the same axes");
+    }
+    const double n = (double)p->nx * p->ny * p->nz;
+    const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
+    if ((n + 65536.0) * esz >= 4294967296.0) return ffail("grid too large for 32-bit offsets");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ffail("no HIP device available");
+    mkg_fvm_ctx* h = new mkg_fvm_ctx();
+    h->p = *p;
+    h->fp32 = p->precision == MKG_PREC_FP32;
+    h->esz = h->fp32 ? 4 : 8;
+    h->N = (uint32_t)n;
+    h->S = ((h->N + 127u) & ~127u) + 1024u;
+    h->nblk = (h->N + 255u) / 256u;
+    auto bad = [&](hipError_t e, const char* what) {
+        fvm_destroy_impl(h);
+        return ffail(std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad(e, "hipStreamCreate");
+    for (int b = 0; b < 2; b++) {
+        if ((e = hipMalloc(&h->phi[b], (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMalloc(&h->c[b], (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
+    }
+    if ((e = hipMalloc((void**)&h->flag, h->S)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMemsetAsync(h->flag, 0, h->S, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    if ((e = hipMalloc((void**)&h->scratch, (size_t)h->N * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->partials, (size_t)(h->nblk + 1) * 4 * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->red_out, 8 * sizeof(double))) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad(e, "hipStreamSynchronize");
+    *out = h;
+    return 0;
+}
+
+int mkg_fvm_destroy(mkg_fvm_handle h) {
+    fvm_destroy_impl(h);
+    return 0;
+}
+
+int mkg_fvm_set(mkg_fvm_handle h, int which, const double* host, size_t count) {
+    if (!h) return ffail("null handle");
+    if (!host) return ffail("host buffer is null");
+    if (which != MKG_FVM_PHI && which != MKG_FVM_CONC) return ffail("set takes MKG_FVM_PHI or MKG_FVM_CONC");
+    if (count != h->N) return ffail("count does not match the field size");
+    void* dst = which == MKG_FVM_PHI ? h->phi[h->cur] : h->c[h->cur];
+    FCHK(hipMemcpyAsync(h->scratch, host, h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    launch_from_double(h->fp32, h->scratch, dst, h->N, h->stream);
+    FCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
+    if (!h) return ffail("null handle");
+    if (nsteps < 0) return ffail("nsteps must be >= 0");
+    const StepArgs a = fvm_args(h);
+    dim3 grid(h->nblk), block(256);
+    for (int s = 0; s < nsteps; s++) {
+        const int o = h->cur, n = h->cur ^ 1;
+        // with the phase held fixed both kernels see phi' = phi
+        void* phin = evolve_phase ? h->phi[n] : h->phi[o];
+        if (h->fp32) {
+            if (evolve_phase)
+                k_fvm_phase<float><<<grid, block, 0, h->stream>>>(
+                    a, (const float*)h->phi[o], (const float*)h->c[o], (float*)phin);
+            k_fvm_solute<float><<<grid, block, 0, h->stream>>>(
+                a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
+                (float*)h->c[n]);
+        } else {
+            if (evolve_phase)
+                k_fvm_phase<double><<<grid, block, 0, h->stream>>>(
+                    a, (const double*)h->phi[o], (const double*)h->c[o], (double*)phin);
+            k_fvm_solute<double><<<grid, block, 0, h->stream>>>(
+                a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
+                (double*)h->c[n]);
+        }
+        if (!evolve_phase)
+            FCHK(hipMemcpyAsync(h->phi[n], h->phi[o], (size_t)h->N * h->esz,
+                                hipMemcpyDeviceToDevice, h->stream));
+        h->cur = n;
+    }
+    FCHK(hipGetLastError());
+    return 0;
+}
+
+int mkg_fvm_sync(mkg_fvm_handle h) {
+    if (!h) return ffail("null handle");
+    FCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count) {
+    if (!h) return ffail("null handle");
+    if (!host) return ffail("host buffer is null");
+    if (count != h->N) return ffail("count does not match the field size");
+    if (which == MKG_FVM_PHI || which == MKG_FVM_CONC) {
+        const void* src = which == MKG_FVM_PHI ? h->phi[h->cur] : h->c[h->cur];
+        launch_to_double(h->fp32, src, h->scratch, h->N, h->stream);
+        FCHK(hipMemcpyAsync(host, h->scratch, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        FCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_FVM_U) {
+        if (mkg_fvm_get(h, MKG_FVM_CONC, host, count)) return -1;
+        double* ph = new double[count];
+        const int rc = mkg_fvm_get(h, MKG_FVM_PHI, ph, count);
+        if (rc == 0) {
+            const double k = h->p.k_part;
+            for (size_t i = 0; i < count; i++) host[i] /= 0.5 * ((1 + k) - (1 - k) * ph[i]);
+        }
+        delete[] ph;
+        return rc;
+    }
+    return ffail("unknown finite-volume field id");
+}
+
+int mkg_fvm_sums(mkg_fvm_handle h, double* out) {
+    if (!h) return ffail("null handle");
+    if (!out) return ffail("out is null");
+    const StepArgs a = fvm_args(h);
+    for (int k = 0; k < 2; k++) {
+        launch_to_double(h->fp32, k == 0 ? h->phi[h->cur] : h->c[h->cur], h->scratch, h->N,
+                         h->stream);
+        int nout = 0;
+        launch_reduce(4, a, nullptr, nullptr, nullptr, nullptr, h->scratch, h->partials,
+                      h->red_out, &nout, h->stream);
+        FCHK(hipMemcpyAsync(out + k, h->red_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    FCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -89,6 +89,7 @@ class MkgParams(C.Structure):
         return int(self.nx) * int(self.ny) * int(self.nz)
 
 
+FVM_PHI, FVM_CONC, FVM_U = 0, 1, 2
 MPF_MAX = 4
 MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG = range(5)
 
@@ -171,6 +172,13 @@ def lib() -> C.CDLL:
         "mkg_mpf_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mpf_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mpf_sums": (C.c_int, [vp, vp]),
+        "mkg_fvm_create": (C.c_int, [P(MkgParams), P(vp)]),
+        "mkg_fvm_destroy": (C.c_int, [vp]),
+        "mkg_fvm_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_fvm_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "mkg_fvm_sync": (C.c_int, [vp]),
+        "mkg_fvm_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_fvm_sums": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -427,4 +435,53 @@ class MpfSolver:
     def sums(self) -> np.ndarray:
         out = np.zeros(self.n + 1)
         _check(lib().mkg_mpf_sums(self._h, _ptr(out)))
+        return out
+
+
+class FvmSolver:
+    """Explicit finite-volume phase field + solute with anti-trapping current on the device
+    (wraps an ``mkg_fvm_handle``; evolution_anisotropy / evolution_ATC1-2)."""
+
+    def __init__(self, params: MkgParams):
+        self.params = params.copy()
+        h = C.c_void_p()
+        _check(lib().mkg_fvm_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.N = self.params.ncell
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mkg_fvm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set(self, which: int, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        _check(lib().mkg_fvm_set(self._h, which, _ptr(a), a.size))
+
+    def get(self, which: int) -> np.ndarray:
+        out = np.empty(self.N)
+        _check(lib().mkg_fvm_get(self._h, which, _ptr(out), self.N))
+        return out
+
+    def step(self, nsteps: int = 1, evolve_phase: bool = True) -> None:
+        _check(lib().mkg_fvm_step(self._h, int(nsteps), 1 if evolve_phase else 0))
+
+    def sync(self) -> None:
+        _check(lib().mkg_fvm_sync(self._h))
+
+    def sums(self) -> np.ndarray:
+        out = np.zeros(2)
+        _check(lib().mkg_fvm_sums(self._h, _ptr(out)))
         return out

@@ -1078,3 +1078,131 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
     free(own);
     return 0;
 }
+
+/* ------------------------------------------------------------------ finite-volume solvers
+ * Explicit cell-centred finite-volume forms of the phase field (evolution_anisotropy,
+ * MaterKit/inc/mkPhaseField.h:51, "on FVM") and of the dilute-alloy solute with the
+ * anti-trapping current (evolution_ATC1 / evolution_ATC2, MaterKit/inc/mkSolutalField.h:44-46;
+ * run_solutal_only, MaterKit/MaterKit.cpp:689-741).  The bodies are in the closed DLL; the
+ * forms are the published Karma-Rappel (1998) and Karma (2001) / Echebarria et al. (2004)
+ * equations on the same stencils as the lattice solver:
+ *   phase:  phi' = phi + dt/a^2 (W^2/dx^2 div J + Q),  J on the six faces as in
+ *           orc_phase_source (so phi' - phi = S_LB + dt W^2/dx^2 lap7 phi, S_LB its source)
+ *   solute: c = A(phi) U,  A = ((1+k) - (1-k) phi)/2,
+ *           c' = c - sum_faces J_out + (phi' - phi)/2,
+ *           J_out = -D q_f (U_nb - U) + s j_at,a   (face with outward side s on axis a)
+ *           q_f = (q(phi) + q(phi_nb))/2, q = (1-phi)/2 + (D_s/D_l)(1+phi)/2 >= q_min
+ *           j_at,a = -a_t W/dx (1 + (1-k) U_f) dphi_f g_a/|g|, U_f and dphi_f face means,
+ *           g the face gradient of phi (normal difference, transverse central means)
+ *   faces:  periodic; bounce = zero flux (phi replicated at the edge); Dirichlet (solute)
+ *           fixes U at the face: J_out = -2 D q (U_w - U)
+ * c - phi/2 is conserved exactly with periodic and zero-flux faces. */
+static double fvm_q(const orc_params* p, double phi) {
+    double q = 0.5 * (1.0 - phi);
+    if (p->D_l > 0) q += (p->D_s / p->D_l) * 0.5 * (1.0 + phi);
+    return q < p->q_min ? p->q_min : q;
+}
+static double fvm_A(const orc_params* p, double phi) {
+    return 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
+}
+
+int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int evolve_phase) {
+    long N = NCELL(p);
+    int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
+    int perc[3] = {p->bc_c[0] == 0, p->bc_c[2] == 0, p->bc_c[4] == 0};
+    double* phin = malloc((size_t)N * sizeof(double));
+    double* cn = malloc((size_t)N * sizeof(double));
+    double* U = malloc((size_t)N * sizeof(double));
+    double atw = p->at_coef * sqrt(p->pf_wdx2 > 0 ? p->pf_wdx2 : 0.0);
+    int dims[3] = {p->nx, p->ny, p->nz};
+    for (int step = 0; step < nsteps; step++) {
+        for (long x = 0; x < N; x++) U[x] = c[x] / fvm_A(p, phi[x]);
+        /* phase */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int X = 0; X < p->nx; X++)
+            for (int Y = 0; Y < p->ny; Y++)
+                for (int Z = 0; Z < p->nz; Z++) {
+                    long id = lin(p, X, Y, Z);
+                    if (!evolve_phase) {
+                        phin[id] = phi[id];
+                        continue;
+                    }
+                    double phi27[27];
+                    for (int dx = -1; dx <= 1; dx++)
+                        for (int dy = -1; dy <= 1; dy++)
+                            for (int dz = -1; dz <= 1; dz++) {
+                                int XX = clampwrap(X + dx, p->nx, per[0]);
+                                int YY = clampwrap(Y + dy, p->ny, per[1]);
+                                int ZZ = clampwrap(Z + dz, p->nz, per[2]);
+                                P27(dx, dy, dz) = phi[lin(p, XX, YY, ZZ)];
+                            }
+                    double S, a;
+                    orc_phase_source(p, phi27, U[id], 0, &S, &a);
+                    double lap = P27(1, 0, 0) + P27(-1, 0, 0) + P27(0, 1, 0) + P27(0, -1, 0) +
+                                 P27(0, 0, 1) + P27(0, 0, -1) - 6.0 * P27(0, 0, 0);
+                    phin[id] = phi[id] + S + p->pf_dt * p->pf_wdx2 * lap;
+                }
+        /* solute */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int X = 0; X < p->nx; X++)
+            for (int Y = 0; Y < p->ny; Y++)
+                for (int Z = 0; Z < p->nz; Z++) {
+                    long id = lin(p, X, Y, Z);
+                    int P3[3] = {X, Y, Z};
+                    double out = 0; /* sum of outward fluxes */
+                    for (int ax = 0; ax < 3; ax++)
+                        for (int s = 1; s >= -1; s -= 2) {
+                            int Q3[3] = {X, Y, Z};
+                            Q3[ax] += s;
+                            int face = 2 * ax + (s > 0 ? 1 : 0);
+                            if (Q3[ax] < 0 || Q3[ax] >= dims[ax]) {
+                                if (perc[ax]) {
+                                    Q3[ax] = (Q3[ax] + dims[ax]) % dims[ax];
+                                } else {
+                                    if (p->bc_c[face] == 2) /* Dirichlet: U fixed at the face */
+                                        out += -2.0 * p->D_l * fvm_q(p, phi[id]) *
+                                               (p->C_face[face] - U[id]);
+                                    continue;
+                                }
+                            }
+                            long nb = lin(p, Q3[0], Q3[1], Q3[2]);
+                            double qf = 0.5 * (fvm_q(p, phi[id]) + fvm_q(p, phi[nb]));
+                            double J = -p->D_l * qf * (U[nb] - U[id]);
+                            if (atw != 0.0) {
+                                /* face gradient of phi^n: normal difference, transverse means */
+                                double g[3];
+                                for (int b = 0; b < 3; b++) {
+                                    if (b == ax) {
+                                        g[b] = s * (phi[nb] - phi[id]);
+                                        continue;
+                                    }
+                                    double v = 0;
+                                    for (int side = 0; side < 2; side++) {
+                                        int* C3 = side ? Q3 : P3;
+                                        int A3[3] = {C3[0], C3[1], C3[2]}, B3[3] = {C3[0], C3[1], C3[2]};
+                                        A3[b] = clampwrap(C3[b] + 1, dims[b], per[b]);
+                                        B3[b] = clampwrap(C3[b] - 1, dims[b], per[b]);
+                                        v += phi[lin(p, A3[0], A3[1], A3[2])] - phi[lin(p, B3[0], B3[1], B3[2])];
+                                    }
+                                    g[b] = 0.25 * v;
+                                }
+                                double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                                if (gg > 1e-20) {
+                                    double Uf = 0.5 * (U[id] + U[nb]);
+                                    double df = 0.5 * ((phin[id] - phi[id]) + (phin[nb] - phi[nb]));
+                                    double ja = -atw * (1.0 + (1.0 - p->k_part) * Uf) * df * g[ax] / sqrt(gg);
+                                    J += s * ja;
+                                }
+                            }
+                            out += J;
+                        }
+                    cn[id] = c[id] - out + 0.5 * (phin[id] - phi[id]);
+                }
+        memcpy(phi, phin, (size_t)N * sizeof(double));
+        memcpy(c, cn, (size_t)N * sizeof(double));
+    }
+    free(phin);
+    free(cn);
+    free(U);
+    return 0;
+}

@@ -121,6 +121,13 @@ void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, 
 int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_prev, double* h,
                  int8_t* flag, int nsteps);
 
+/* Explicit finite-volume phase field (evolution_anisotropy) and solute with anti-trapping
+ * current (evolution_ATC1/2): phi and c (= A(phi) U) are N doubles each, advanced in place;
+ * evolve_phase 0 holds phi fixed (run_solutal_only).  Uses the phase and solute parameters of
+ * p (pf_dt, pf_wdx2, eps4, lambda, aniso_*, D_l, D_s, k_part, at_coef, q_min, bc_g, bc_c,
+ * C_face as U values). */
+int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int evolve_phase);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 

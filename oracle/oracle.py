@@ -64,6 +64,8 @@ def lib() -> ct.CDLL:
         L.orc_uniform.restype = ct.c_double
         L.orc_cell_uniform.argtypes = [ct.c_uint64, ct.c_uint64]
         L.orc_cell_uniform.restype = ct.c_double
+        L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
+        L.orc_fvm_step.restype = ct.c_int
         L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, ct.c_int]
         L.orc_mpf_step.restype = ct.c_int
@@ -278,3 +280,12 @@ class MpfState:
                                 _p(self.phi_prev), _p(self.h), _p(self.flag), int(nsteps))
         if rc != 0:
             raise RuntimeError("oracle rejected the multi-phase parameter block")
+
+
+def fvm_step(params, phi, c, nsteps: int = 1, evolve_phase: bool = True) -> None:
+    """Explicit finite-volume phase field + solute (in place on float64 arrays of N values)."""
+    assert phi.dtype == np.float64 and c.dtype == np.float64
+    assert phi.flags.c_contiguous and c.flags.c_contiguous
+    rc = lib().orc_fvm_step(ct.byref(params), _p(phi), _p(c), int(nsteps), 1 if evolve_phase else 0)
+    if rc != 0:
+        raise RuntimeError("oracle rejected the finite-volume parameter block")
