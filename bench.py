@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--no-mpf", action="store_true")
     ap.add_argument("--no-fvm", action="store_true")
+    ap.add_argument("--no-mp", action="store_true")
     ap.add_argument("--mpf-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run three plain 1 GiB device copies (the reference kernel for "
@@ -215,6 +216,41 @@ def fvm_cost(size: int, steps: int) -> dict:
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def mp_cost(size: int, steps: int) -> dict:
+    """Phase-field two-phase flow (Cahn-Hilliard D3Q19 + velocity-pressure D3Q19, FP32) on a
+    size^3 grid: a drop of radius size/4, density ratio 3, under gravity.  Algorithmic bytes per
+    cell: phi pass 19 r + 1 w; mu pass 1 r + 1 w; collide-stream 2 x 19 r+w, phi and mu r,
+    phi u r+w (3 values)."""
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
+
+    m = mkg.mp_params(nx=size, ny=size, nz=size, precision=mkg.PREC_FP32, rho_l=3.0, rho_g=1.0,
+                      nu_l=0.1, nu_g=0.05, sigma=0.005, width=4.0, mobility=0.05, tau_h=1.0)
+    m.gravity[2] = -1e-6
+    x = np.arange(size) - size / 2.0
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    phi = (0.5 + 0.5 * np.tanh(2 * (size / 4.0 - r) / 4.0)).reshape(-1)
+    dev = mkg.MpSolver(m)
+    dev.init(phi)
+    del phi, r
+    dev.step(4)
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps)
+    dev.sync()
+    dt = (time.perf_counter() - t0) / steps
+    dev.close()
+    N = size ** 3
+    bytes_cell = (19 + 1 + 1 + 1 + 2 * 2 * 19 + 2 + 2 * 3) * 4
+    gbs = bytes_cell * N / dt / 1e9
+    return {"workload": f"two-phase flow (Cahn-Hilliard D3Q19 + velocity-pressure D3Q19), "
+                        f"{size}^3, FP32", "mlups": round(N / dt / 1e6, 1),
+            "ms_per_step": round(dt * 1e3, 4), "alg_bytes_per_cell": bytes_cell,
+            "step_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(seconds: float):
     """The reference-structured FP64 CPU path (oracle/mk_aos.c: AoS node records of the
     reference's 3640 B layout, one OpenMP sweep per operator) on a 256 x 256 x 16 slab of the
@@ -327,6 +363,8 @@ def main():
         out["mpf"] = mpf_cost(size, a.mpf_steps)
     if not a.no_fvm:
         out["fvm"] = fvm_cost(size, a.mpf_steps)
+    if not a.no_mp:
+        out["two_phase"] = mp_cost(size, a.mpf_steps)
     if not a.no_adjoint:
         adj = adjoint_cost(a.adjoint_steps)
         out["adjoint_over_forward"] = adj["ratio"]

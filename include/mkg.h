@@ -335,6 +335,48 @@ int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count);
 /* FP64 sums of phi and of c over every cell (out[0], out[1]). */
 int mkg_fvm_sums(mkg_fvm_handle h, double* out);
 
+/* ---- Two-phase flow ----------------------------------------------------------------------
+ * Phase-field two-phase flow of mkFluidsMultiphase (MaterKit/inc/mkFluidsMultiphase.h:20-99;
+ * run_flow_multiphase, MaterKit/MaterKit.cpp:290-331): a Cahn-Hilliard lattice for the order
+ * parameter phi in [0, 1] and a velocity-pressure flow lattice, both D3Q19, in the model of
+ * Liang et al. (2014) whose parameters the reference class carries (rho_l, rho_g, sigma, W_0,
+ * beta = 12 sigma/W, kappa = 1.5 sigma W, eta; mkFluidsMultiphase.h:81-97), with BGK
+ * collisions: mu = 4 beta phi (phi-1)(phi-1/2) - kappa lap phi, surface force mu grad phi,
+ * rho = rho_g + phi (rho_l - rho_g), nu = nu_g + phi (nu_l - nu_g), mobility
+ * M = eta cs^2 (tau_h - 1/2).  Faces: periodic or bounce (no-slip wall, zero flux). */
+typedef struct mkg_mp_params {
+    int32_t nx, ny, nz, precision;
+    int32_t bc[6];
+    double rho_l, rho_g;      /* densities of phi = 1 and phi = 0 */
+    double nu_l, nu_g;        /* kinematic viscosities (lattice units) */
+    double sigma, width;      /* surface tension, interface width W (cells) */
+    double mobility, tau_h;   /* M and the Cahn-Hilliard relaxation time (> 1/2) */
+    double gravity[3];        /* body acceleration: force rho g */
+    double force[3];          /* uniform body-force density */
+} mkg_mp_params;
+
+typedef struct mkg_mp_ctx* mkg_mp_handle;
+#define MKG_MP_POP_FLOW 0   /* 19 x N flow populations (canonical, doubles) */
+#define MKG_MP_POP_PHASE 1  /* 19 x N Cahn-Hilliard populations */
+#define MKG_MP_PHI 2        /* N order parameter (get) */
+#define MKG_MP_MU 3         /* N chemical potential (get) */
+#define MKG_MP_PRESSURE 4   /* N pressure p (get) */
+#define MKG_MP_VEL 5        /* 3 x N velocity, cell-interleaved (get) */
+#define MKG_MP_PHIU 6       /* 3 x N phi u of the last step, cell-interleaved (set) */
+
+size_t mkg_mp_params_size(void);
+int mkg_mp_params_default(mkg_mp_params* m);
+int mkg_mp_create(const mkg_mp_params* m, mkg_mp_handle* out);
+int mkg_mp_destroy(mkg_mp_handle h);
+/* Equilibrium start from phi (N), u (3 x N, cell-interleaved, or NULL) and p (N or NULL). */
+int mkg_mp_init(mkg_mp_handle h, const double* phi, const double* u, const double* p);
+int mkg_mp_step(mkg_mp_handle h, int nsteps);
+int mkg_mp_sync(mkg_mp_handle h);
+int mkg_mp_get(mkg_mp_handle h, int which, double* host, size_t count);
+int mkg_mp_set(mkg_mp_handle h, int which, const double* host, size_t count);
+/* FP64 sum of phi over every cell. */
+int mkg_mp_sums(mkg_mp_handle h, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,8 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "adjoint.hip", "mpf.hip", "fvm.hip", "api.cpp",
+SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "adjoint.hip", "mpf.hip", "fvm.hip", "mp.hip",
+           "api.cpp",
            "case.cpp"]
 
 # Per-source flags.  The FP32 time step divides and takes square roots with the hardware

@@ -1,0 +1,561 @@
+// Phase-field two-phase flow of mkFluidsMultiphase (MaterKit/inc/mkFluidsMultiphase.h:20-99;
+// run_flow_multiphase, MaterKit/MaterKit.cpp:290-331).  Its parameters (rho_l, rho_g, sigma,
+// W_0, beta = 12 sigma/W, kappa = 1.5 sigma W, eta, mkFluidsMultiphase.h:81-97) are those of the
+// model of Liang et al. (PRE 89, 053320, 2014), restated here with BGK collisions: a
+// Cahn-Hilliard lattice h for the order parameter phi in [0, 1] and a velocity-pressure flow
+// lattice f, both D3Q19, in the AA storage and face rules of the fused solver (device.h).
+//
+// One step is three launches:
+//   k_mp_phi   phi = sum_i h_i
+//   k_mp_mu    mu = 4 beta phi (phi - 1)(phi - 1/2) - kappa lap phi (isotropic 19-point Laplacian)
+//   k_mp_step  per cell: grad phi; rho = rho_g + phi (rho_l - rho_g); F = mu grad phi + rho g +
+//              force; rho u = sum c f + F/2; p = cs^2/(1 - w0) [sum_{i>0} f + u.grad rho/2 +
+//              rho s_0(u)]; flow BGK with tau_f = nu(phi)/cs^2 + 1/2 and the forcing
+//              (1 - 1/(2 tau_f)) w_i [c_i.F + (c_i.u)(c_i.grad rho)]/cs^2 (its zeroth moment
+//              u.grad rho keeps div u = 0 where rho varies);
+//              Cahn-Hilliard BGK with h0eq = phi + (w0 - 1) eta mu, hieq = w_i (eta mu +
+//              phi c_i.u/cs^2) and the source (1 - 1/(2 tau_h)) w_i c_i.d(phi u)/cs^2, where
+//              d(phi u) is the change of phi u over the step (kept per cell)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/mkg.h"
+#include "device.h"
+
+namespace mkg {
+
+void set_error(const std::string& msg);
+
+struct MpArgs {
+    StepArgs a;  // geometry and faces; a.f = flow lattice, a.g = Cahn-Hilliard lattice
+    double rho_l, rho_g, nu_l, nu_g, beta, kappa, eta, tau_h;
+    double gravity[3], force[3];
+    void* phi;   // N (storage type)
+    void* mu;    // N
+    void* phiu;  // 3 planes of S: phi u of the last step
+};
+
+// 19-point stencil of a cell-centred field (periodic faces wrap, others replicate the edge).
+template <class R>
+__device__ inline void mp_stencil(const StepArgs& a, const Cell& c, const R* v, R (&P)[19]) {
+    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & XLO) px[0] = px[1];
+    if (np & XHI) px[2] = px[1];
+    if (np & YLO) py[0] = py[1];
+    if (np & YHI) py[2] = py[1];
+    if (np & ZLO) pz[0] = pz[1];
+    if (np & ZHI) pz[2] = pz[1];
+#pragma unroll
+    for (int i = 0; i < 19; i++) P[i] = *slot_at(v, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+}
+
+template <class R, bool ODD>
+__global__ void __launch_bounds__(256) k_mp_phi(const MpArgs m) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    R h[19];
+    read_pops<19, ODD, R>(h, m.a.g, m.a.S, c, m.a, kKindScalar);
+    R s = 0;
+#pragma unroll
+    for (int i = 0; i < 19; i++) s += h[i];
+    *slot_at((R*)m.phi, c.id) = s;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_mp_mu(const MpArgs m) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    R P[19];
+    mp_stencil<R>(m.a, c, (const R*)m.phi, P);
+    R lap = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) lap += (R)(6.0 * wgt<19>(i)) * (P[i] - P[0]);
+    const R ph = P[0];
+    *slot_at((R*)m.mu, c.id) =
+        (R)(4.0 * m.beta) * ph * (ph - (R)1) * (ph - (R)0.5) - (R)m.kappa * lap;
+}
+
+// Macroscopic state of a cell from its pre-collision f: rho, u, p, F and grad rho.
+template <class R>
+__device__ inline void mp_macro(const MpArgs& m, const R (&f)[19], R ph, R muc, const R (&g)[3],
+                                R& rho, R (&u)[3], R& p, R (&F)[3], R (&gr)[3]) {
+    rho = (R)m.rho_g + ph * (R)(m.rho_l - m.rho_g);
+    R j[3] = {0, 0, 0}, s1 = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        s1 += f[i];
+        if (cx(i)) j[0] += (R)cx(i) * f[i];
+        if (cy(i)) j[1] += (R)cy(i) * f[i];
+        if (cz(i)) j[2] += (R)cz(i) * f[i];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        gr[d] = (R)(m.rho_l - m.rho_g) * g[d];
+        F[d] = muc * g[d] + rho * (R)m.gravity[d] + (R)m.force[d];
+        u[d] = (j[d] + (R)0.5 * F[d]) / rho;
+    }
+    const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const R s0 = (R)wgt<19>(0) * ((R)-1.5 * uu);
+    const R ugr = u[0] * gr[0] + u[1] * gr[1] + u[2] * gr[2];
+    p = (R)(1.0 / 3.0 / (1.0 - wgt<19>(0))) * (s1 + (R)0.5 * ugr + rho * s0);
+}
+
+template <class R>
+__device__ inline void mp_gradient(const R (&P)[19], R (&g)[3]) {
+    g[0] = g[1] = g[2] = 0;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)(3.0 * wgt<19>(i)) * P[i];
+        if (cx(i)) g[0] += (R)cx(i) * v;
+        if (cy(i)) g[1] += (R)cy(i) * v;
+        if (cz(i)) g[2] += (R)cz(i) * v;
+    }
+}
+
+template <class R, bool ODD>
+__global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    const StepArgs& a = m.a;
+    const uint32_t S = a.S;
+    R P[19];
+    mp_stencil<R>(a, c, (const R*)m.phi, P);
+    R g[3];
+    mp_gradient<R>(P, g);
+    const R ph = P[0];
+    const R muc = *slot_at((const R*)m.mu, c.id);
+    R f[19];
+    read_pops<19, ODD, R>(f, a.f, S, c, a, kKindFlow);
+    R rho, u[3], p, F[3], gr[3];
+    mp_macro<R>(m, f, ph, muc, g, rho, u, p, F, gr);
+    const R pc = ph < (R)0 ? (R)0 : (ph > (R)1 ? (R)1 : ph);
+    const R nu = (R)m.nu_g + pc * (R)(m.nu_l - m.nu_g);
+    const R itf = (R)1 / ((R)3 * nu + (R)0.5);  // 1 / tau_f
+    const R kf = (R)1 - (R)0.5 * itf;
+    const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const R w = (R)wgt<19>(i);
+        const R cu = cx(i) * u[0] + cy(i) * u[1] + cz(i) * u[2];
+        const R cg = cx(i) * gr[0] + cy(i) * gr[1] + cz(i) * gr[2];
+        const R cF = cx(i) * F[0] + cy(i) * F[1] + cz(i) * F[2];
+        const R si = w * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu);
+        const R feq = (i == 0 ? (w - (R)1) : w) * (R)3 * p + rho * si;
+        const R Fi = kf * w * (R)3 * (cF + cu * cg);
+        f[i] = f[i] - itf * (f[i] - feq) + Fi;
+    }
+    FaceState<R> fs;
+    write_pops<19, ODD, R>(f, a.f, S, c, a, kKindFlow, fs);
+
+    // Cahn-Hilliard lattice
+    R* pu = (R*)m.phiu;
+    R dpu[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        R* q = slot_at(pu + (size_t)d * S, c.id);
+        const R v = ph * u[d];
+        dpu[d] = v - *q;
+        *q = v;
+    }
+    R h[19];
+    read_pops<19, ODD, R>(h, a.g, S, c, a, kKindScalar);
+    const R ith = (R)(1.0 / m.tau_h), kh = (R)(1.0 - 0.5 / m.tau_h);
+    const R em = (R)m.eta * muc;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const R w = (R)wgt<19>(i);
+        const R cu = cx(i) * u[0] + cy(i) * u[1] + cz(i) * u[2];
+        const R cd = cx(i) * dpu[0] + cy(i) * dpu[1] + cz(i) * dpu[2];
+        const R heq = i == 0 ? ph + (w - (R)1) * em : w * (em + (R)3 * ph * cu);
+        h[i] = h[i] - ith * (h[i] - heq) + kh * w * (R)3 * cd;
+    }
+    write_pops<19, ODD, R>(h, a.g, S, c, a, kKindScalar);
+}
+
+// p and u (3 x N cell-interleaved, doubles) of the current pre-collision state (phi and mu
+// freshly computed from it).
+template <class R>
+__global__ void __launch_bounds__(256) k_mp_macro(const MpArgs m, double* p_out, double* u_out) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    R P[19];
+    mp_stencil<R>(m.a, c, (const R*)m.phi, P);
+    R g[3];
+    mp_gradient<R>(P, g);
+    R f[19];
+    read_pops_rt<19, R>(f, m.a.f, m.a.S, c, m.a, kKindFlow);
+    R rho, u[3], p, F[3], gr[3];
+    mp_macro<R>(m, f, P[0], *slot_at((const R*)m.mu, c.id), g, rho, u, p, F, gr);
+    p_out[c.id] = (double)p;
+    for (int d = 0; d < 3; d++) u_out[3 * (size_t)c.id + d] = (double)u[d];
+}
+
+// Start: h at rest equilibrium w_i phi0 (own-slot layout), for the first phi / mu passes.
+template <class R>
+__global__ void __launch_bounds__(256) k_mp_init_h(const MpArgs m, const double* phi0) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    R* hb = (R*)m.a.g.pop;
+#pragma unroll
+    for (int i = 0; i < 19; i++) *slot_at(hb + (size_t)i * m.a.S, c.id) = (R)(wgt<19>(i) * phi0[c.id]);
+}
+
+// Equilibria of both lattices from phi0, u0 (3N or null), p0 (N or null) and mu of phi0.
+template <class R>
+__global__ void __launch_bounds__(256) k_mp_init_eq(const MpArgs m, const double* phi0,
+                                                    const double* u0, const double* p0) {
+    Cell c;
+    if (!locate(m.a, c)) return;
+    const uint32_t S = m.a.S;
+    double u[3] = {0, 0, 0};
+    if (u0)
+        for (int d = 0; d < 3; d++) u[d] = u0[3 * (size_t)c.id + d];
+    const double ph = phi0[c.id], p = p0 ? p0[c.id] : 0.0;
+    const double rho = m.rho_g + ph * (m.rho_l - m.rho_g);
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double em = m.eta * (double)*slot_at((const R*)m.mu, c.id);
+    R* fb = (R*)m.a.f.pop;
+    R* hb = (R*)m.a.g.pop;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const double w = wgt<19>(i);
+        const double cu = cx(i) * u[0] + cy(i) * u[1] + cz(i) * u[2];
+        const double si = w * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+        *slot_at(fb + (size_t)i * S, c.id) = (R)((i == 0 ? w - 1.0 : w) * 3.0 * p + rho * si);
+        *slot_at(hb + (size_t)i * S, c.id) =
+            (R)(i == 0 ? ph + (w - 1.0) * em : w * (em + 3.0 * ph * cu));
+    }
+    R* pu = (R*)m.phiu;
+    for (int d = 0; d < 3; d++) *slot_at(pu + (size_t)d * S, c.id) = (R)(ph * u[d]);
+}
+
+}  // namespace mkg
+
+using namespace mkg;
+
+struct mkg_mp_ctx {
+    mkg_mp_params p;
+    int fp32 = 0;
+    size_t esz = 8;
+    uint32_t N = 0, S = 0, F = 0, nblk = 0;
+    hipStream_t stream = nullptr;
+    void* f = nullptr;
+    void* h = nullptr;
+    void* phi = nullptr;
+    void* mu = nullptr;
+    void* phiu = nullptr;
+    int8_t* flag = nullptr;
+    double* scratch = nullptr;  // 19 N doubles
+    double* partials = nullptr;
+    double* red_out = nullptr;
+    int lay = 0;
+};
+
+namespace {
+
+int pfail(const std::string& msg) {
+    set_error(msg);
+    return -1;
+}
+
+#define PCHK(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return pfail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+MpArgs mp_args(const mkg_mp_ctx* h) {
+    MpArgs m;
+    std::memset(&m, 0, sizeof(m));
+    const mkg_mp_params& p = h->p;
+    StepArgs& a = m.a;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.N = h->N;
+    a.S = h->S;
+    a.nblk = h->nblk;
+    a.F = h->F;
+    a.flag = h->flag;
+    a.rho0 = 0.0;  // no moving-wall term at bounce-back faces
+    int np = 0;
+    for (int f = 0; f < 6; f++) {
+        a.f.bc[f] = a.g.bc[f] = p.bc[f];
+        if (p.bc[f] != MKG_BC_PERIODIC) np |= 1 << f;
+    }
+    a.f.npmask = a.g.npmask = np;
+    a.f.lay = a.g.lay = h->lay;
+    a.f.adv = a.g.adv = 1;
+    a.f.pop = h->f;
+    a.g.pop = h->h;
+    m.rho_l = p.rho_l;
+    m.rho_g = p.rho_g;
+    m.nu_l = p.nu_l;
+    m.nu_g = p.nu_g;
+    m.beta = 12.0 * p.sigma / p.width;
+    m.kappa = 1.5 * p.sigma * p.width;
+    m.eta = p.mobility / ((1.0 / 3.0) * (p.tau_h - 0.5));
+    m.tau_h = p.tau_h;
+    for (int d = 0; d < 3; d++) {
+        m.gravity[d] = p.gravity[d];
+        m.force[d] = p.force[d];
+    }
+    m.phi = h->phi;
+    m.mu = h->mu;
+    m.phiu = h->phiu;
+    return m;
+}
+
+template <class R>
+void launch_phi_mu(const MpArgs& m, int odd, hipStream_t s) {
+    dim3 grid(m.a.nblk), block(256);
+    if (odd) k_mp_phi<R, true><<<grid, block, 0, s>>>(m);
+    else k_mp_phi<R, false><<<grid, block, 0, s>>>(m);
+    k_mp_mu<R><<<grid, block, 0, s>>>(m);
+}
+
+void mp_destroy_impl(mkg_mp_ctx* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* bufs[] = {h->f, h->h, h->phi, h->mu, h->phiu, h->flag, h->scratch, h->partials,
+                    h->red_out};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mkg_mp_params_size(void) { return sizeof(mkg_mp_params); }
+
+int mkg_mp_params_default(mkg_mp_params* m) {
+    if (!m) return pfail("params is null");
+    std::memset(m, 0, sizeof(*m));
+    m->nx = m->ny = m->nz = 1;
+    m->rho_l = m->rho_g = 1.0;
+    m->nu_l = m->nu_g = 0.1;
+    m->sigma = 0.01;
+    m->width = 4.0;
+    m->mobility = 0.1;
+    m->tau_h = 1.0;
+    return 0;
+}
+
+int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
+    if (!out) return pfail("out is null");
+    *out = nullptr;
+    if (!p) return pfail("params is null");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return pfail("grid dimensions must be >= 1");
+    if (p->precision != MKG_PREC_FP64 && p->precision != MKG_PREC_FP32)
+        return pfail("precision must be MKG_PREC_FP64 or MKG_PREC_FP32");
+    if (!(p->rho_l > 0) || !(p->rho_g > 0)) return pfail("densities must be > 0");
+    if (!(p->nu_l > 0) || !(p->nu_g > 0)) return pfail("viscosities must be > 0");
+    if (!(p->width > 0) || !(p->sigma >= 0)) return pfail("width must be > 0 and sigma >= 0");
+    if (!(p->mobility > 0) || !(p->tau_h > 0.5)) return pfail("mobility > 0 and tau_h > 1/2 required");
+    for (int f = 0; f < 6; f++)
+        if (p->bc[f] != MKG_BC_PERIODIC && p->bc[f] != MKG_BC_BOUNCE)
+            return pfail("two-phase faces are periodic or bounce (no-slip wall, zero flux)");
+    for (int a = 0; a < 3; a++)
+        if ((p->bc[2 * a] == 0) != (p->bc[2 * a + 1] == 0))
+            return pfail("a periodic axis needs periodic codes on both faces");
+    const double n = (double)p->nx * p->ny * p->nz;
+    const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
+    if ((n + 65536.0) * esz >= 4294967296.0) return pfail("grid too large for 32-bit offsets");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return pfail("no HIP device available");
+    mkg_mp_ctx* h = new mkg_mp_ctx();
+    h->p = *p;
+    h->fp32 = p->precision == MKG_PREC_FP32;
+    h->esz = h->fp32 ? 4 : 8;
+    h->N = (uint32_t)n;
+    h->S = ((h->N + 127u) & ~127u) + 1024u;
+    h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
+    h->nblk = (h->N + 255u) / 256u;
+    auto bad = [&](hipError_t e, const char* what) {
+        mp_destroy_impl(h);
+        return pfail(std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad(e, "hipStreamCreate");
+    const size_t lb = (size_t)19 * ((size_t)h->S + h->F) * h->esz;
+    void** bufs[] = {&h->f, &h->h};
+    for (void** b : bufs) {
+        if ((e = hipMalloc(b, lb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(*b, 0, lb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    }
+    if ((e = hipMalloc(&h->phi, (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMalloc(&h->mu, (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMalloc(&h->phiu, (size_t)3 * h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->flag, h->S)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMemsetAsync(h->flag, 0, h->S, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    if ((e = hipMalloc((void**)&h->scratch, (size_t)19 * h->N * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->partials, (size_t)(h->nblk + 1) * 4 * sizeof(double))) != hipSuccess)
+        return bad(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->red_out, 8 * sizeof(double))) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad(e, "hipStreamSynchronize");
+    *out = h;
+    return 0;
+}
+
+int mkg_mp_destroy(mkg_mp_handle h) {
+    mp_destroy_impl(h);
+    return 0;
+}
+
+int mkg_mp_init(mkg_mp_handle h, const double* phi, const double* u, const double* p) {
+    if (!h) return pfail("null handle");
+    if (!phi) return pfail("phi is null");
+    const size_t N = h->N;
+    // device copies of the inputs, in the scratch buffer (19 N doubles >= N + 3 N + N)
+    double* dphi = h->scratch;
+    double* du = u ? h->scratch + N : nullptr;
+    double* dp = p ? h->scratch + 4 * N : nullptr;
+    PCHK(hipMemcpyAsync(dphi, phi, N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (u) PCHK(hipMemcpyAsync(du, u, 3 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (p) PCHK(hipMemcpyAsync(dp, p, N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    h->lay = 0;
+    MpArgs m = mp_args(h);
+    dim3 grid(h->nblk), block(256);
+    if (h->fp32) {
+        k_mp_init_h<float><<<grid, block, 0, h->stream>>>(m, dphi);
+        launch_phi_mu<float>(m, 0, h->stream);
+        k_mp_init_eq<float><<<grid, block, 0, h->stream>>>(m, dphi, du, dp);
+    } else {
+        k_mp_init_h<double><<<grid, block, 0, h->stream>>>(m, dphi);
+        launch_phi_mu<double>(m, 0, h->stream);
+        k_mp_init_eq<double><<<grid, block, 0, h->stream>>>(m, dphi, du, dp);
+    }
+    PCHK(hipGetLastError());
+    PCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_mp_step(mkg_mp_handle h, int nsteps) {
+    if (!h) return pfail("null handle");
+    if (nsteps < 0) return pfail("nsteps must be >= 0");
+    dim3 grid(h->nblk), block(256);
+    for (int s = 0; s < nsteps; s++) {
+        MpArgs m = mp_args(h);
+        const int odd = h->lay;
+        if (h->fp32) {
+            launch_phi_mu<float>(m, odd, h->stream);
+            if (odd) k_mp_step<float, true><<<grid, block, 0, h->stream>>>(m);
+            else k_mp_step<float, false><<<grid, block, 0, h->stream>>>(m);
+        } else {
+            launch_phi_mu<double>(m, odd, h->stream);
+            if (odd) k_mp_step<double, true><<<grid, block, 0, h->stream>>>(m);
+            else k_mp_step<double, false><<<grid, block, 0, h->stream>>>(m);
+        }
+        h->lay ^= 1;
+    }
+    PCHK(hipGetLastError());
+    return 0;
+}
+
+int mkg_mp_sync(mkg_mp_handle h) {
+    if (!h) return pfail("null handle");
+    PCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mkg_mp_get(mkg_mp_handle h, int which, double* host, size_t count) {
+    if (!h) return pfail("null handle");
+    if (!host) return pfail("host buffer is null");
+    const size_t N = h->N;
+    MpArgs m = mp_args(h);
+    dim3 grid(h->nblk), block(256);
+    if (which == MKG_MP_POP_FLOW || which == MKG_MP_POP_PHASE) {
+        if (count != 19 * N) return pfail("count does not match the field size");
+        const LatArgs& L = which == MKG_MP_POP_FLOW ? m.a.f : m.a.g;
+        if (launch_canonical(19, h->fp32, m.a, L, which == MKG_MP_POP_FLOW ? 0 : 1, h->scratch,
+                             h->stream))
+            return pfail("lattice unavailable");
+        PCHK(hipMemcpyAsync(host, h->scratch, 19 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        PCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    // macroscopic fields: phi and mu of the current populations first
+    if (h->fp32) launch_phi_mu<float>(m, h->lay, h->stream);
+    else launch_phi_mu<double>(m, h->lay, h->stream);
+    if (which == MKG_MP_PHI || which == MKG_MP_MU) {
+        if (count != N) return pfail("count does not match the field size");
+        launch_to_double(h->fp32, which == MKG_MP_PHI ? h->phi : h->mu, h->scratch, h->N, h->stream);
+        PCHK(hipMemcpyAsync(host, h->scratch, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        PCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    if (which == MKG_MP_PRESSURE || which == MKG_MP_VEL) {
+        if (count != (which == MKG_MP_VEL ? 3 * N : N)) return pfail("count does not match the field size");
+        if (h->fp32) k_mp_macro<float><<<grid, block, 0, h->stream>>>(m, h->scratch, h->scratch + N);
+        else k_mp_macro<double><<<grid, block, 0, h->stream>>>(m, h->scratch, h->scratch + N);
+        PCHK(hipMemcpyAsync(host, which == MKG_MP_PRESSURE ? h->scratch : h->scratch + N,
+                            count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        PCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    return pfail("unknown two-phase field id");
+}
+
+int mkg_mp_set(mkg_mp_handle h, int which, const double* host, size_t count) {
+    if (!h) return pfail("null handle");
+    if (!host) return pfail("host buffer is null");
+    const size_t N = h->N;
+    if (which == MKG_MP_POP_FLOW || which == MKG_MP_POP_PHASE) {
+        if (count != 19 * N) return pfail("count does not match the field size");
+        // both lattices share one layout: bring the other one to the own-slot layout too
+        MpArgs m = mp_args(h);
+        if (h->lay) {
+            const LatArgs& O = which == MKG_MP_POP_FLOW ? m.a.g : m.a.f;
+            const int kind = which == MKG_MP_POP_FLOW ? 1 : 0;
+            if (launch_canonical(19, h->fp32, m.a, O, kind, h->scratch, h->stream) ||
+                launch_store_pops(19, h->fp32, m.a, O, h->scratch, h->stream))
+                return pfail("lattice unavailable");
+        }
+        PCHK(hipMemcpyAsync(h->scratch, host, 19 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        const LatArgs& L = which == MKG_MP_POP_FLOW ? m.a.f : m.a.g;
+        if (launch_store_pops(19, h->fp32, m.a, L, h->scratch, h->stream)) return pfail("lattice unavailable");
+        PCHK(hipStreamSynchronize(h->stream));
+        h->lay = 0;
+        return 0;
+    }
+    if (which == MKG_MP_PHIU) {
+        if (count != 3 * N) return pfail("count does not match the field size");
+        // cell-interleaved host data to three planes
+        for (int d = 0; d < 3; d++) {
+            PCHK(hipMemcpy2DAsync(h->scratch, sizeof(double), host + d, 3 * sizeof(double),
+                                  sizeof(double), N, hipMemcpyHostToDevice, h->stream));
+            launch_from_double(h->fp32, h->scratch, (char*)h->phiu + (size_t)d * h->S * h->esz,
+                               h->N, h->stream);
+        }
+        PCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    return pfail("set takes MKG_MP_POP_FLOW, MKG_MP_POP_PHASE or MKG_MP_PHIU");
+}
+
+int mkg_mp_sums(mkg_mp_handle h, double* out) {
+    if (!h) return pfail("null handle");
+    if (!out) return pfail("out is null");
+    MpArgs m = mp_args(h);
+    if (h->fp32) launch_phi_mu<float>(m, h->lay, h->stream);
+    else launch_phi_mu<double>(m, h->lay, h->stream);
+    launch_to_double(h->fp32, h->phi, h->scratch, h->N, h->stream);
+    int nout = 0;
+    launch_reduce(4, m.a, nullptr, nullptr, nullptr, nullptr, h->scratch, h->partials, h->red_out,
+                  &nout, h->stream);
+    PCHK(hipMemcpyAsync(out, h->red_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    PCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -90,6 +90,7 @@ class MkgParams(C.Structure):
 
 
 FVM_PHI, FVM_CONC, FVM_U = 0, 1, 2
+MP_POP_FLOW, MP_POP_PHASE, MP_PHI, MP_MU, MP_PRESSURE, MP_VEL, MP_PHIU = range(7)
 MPF_MAX = 4
 MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG = range(5)
 
@@ -103,6 +104,36 @@ MPF_FIELDS = [
     ("alpha", C.c_double), ("latent_T", C.c_double), ("T_ref", C.c_double),
     ("T_face", C.c_double * 6), ("phi_face", C.c_double * 6),
 ]
+
+
+MP_FIELDS = [
+    ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("precision", C.c_int32),
+    ("bc", C.c_int32 * 6),
+    ("rho_l", C.c_double), ("rho_g", C.c_double), ("nu_l", C.c_double), ("nu_g", C.c_double),
+    ("sigma", C.c_double), ("width", C.c_double), ("mobility", C.c_double),
+    ("tau_h", C.c_double), ("gravity", C.c_double * 3), ("force", C.c_double * 3),
+]
+
+
+class MpParams(C.Structure):
+    """Layout-identical mirror of ``mkg_mp_params`` (two-phase flow)."""
+
+    _fields_ = MP_FIELDS
+
+    @property
+    def ncell(self) -> int:
+        return int(self.nx) * int(self.ny) * int(self.nz)
+
+
+def mp_params(**kw) -> "MpParams":
+    """Two-phase flow parameters with the defaults of mkg_mp_params_default (host-side copy,
+    usable without the library)."""
+    m = MpParams()
+    m.nx = m.ny = m.nz = 1
+    m.rho_l, m.rho_g, m.nu_l, m.nu_g = 1.0, 1.0, 0.1, 0.1
+    m.sigma, m.width, m.mobility, m.tau_h = 0.01, 4.0, 0.1, 1.0
+    MkgParams.set(m, **kw)
+    return m
 
 
 class MpfParams(C.Structure):
@@ -172,6 +203,16 @@ def lib() -> C.CDLL:
         "mkg_mpf_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mpf_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mpf_sums": (C.c_int, [vp, vp]),
+        "mkg_mp_params_size": (C.c_size_t, []),
+        "mkg_mp_params_default": (C.c_int, [P(MpParams)]),
+        "mkg_mp_create": (C.c_int, [P(MpParams), P(vp)]),
+        "mkg_mp_destroy": (C.c_int, [vp]),
+        "mkg_mp_init": (C.c_int, [vp, vp, vp, vp]),
+        "mkg_mp_step": (C.c_int, [vp, C.c_int]),
+        "mkg_mp_sync": (C.c_int, [vp]),
+        "mkg_mp_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_mp_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "mkg_mp_sums": (C.c_int, [vp, vp]),
         "mkg_fvm_create": (C.c_int, [P(MkgParams), P(vp)]),
         "mkg_fvm_destroy": (C.c_int, [vp]),
         "mkg_fvm_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
@@ -186,6 +227,8 @@ def lib() -> C.CDLL:
         fn.argtypes = args
     if L.mkg_params_size() != C.sizeof(MkgParams):
         raise MkgError("mkg_params layout mismatch between libmkg.so and mkg.py")
+    if L.mkg_mp_params_size() != C.sizeof(MpParams):
+        raise MkgError("mkg_mp_params layout mismatch between libmkg.so and mkg.py")
     if L.mkg_mpf_params_size() != C.sizeof(MpfParams):
         raise MkgError("mkg_mpf_params layout mismatch between libmkg.so and mkg.py")
     _lib = L
@@ -485,3 +528,59 @@ class FvmSolver:
         out = np.zeros(2)
         _check(lib().mkg_fvm_sums(self._h, _ptr(out)))
         return out
+
+
+class MpSolver:
+    """Phase-field two-phase flow on the device (wraps an ``mkg_mp_handle``;
+    mkFluidsMultiphase, Liang et al. 2014 model)."""
+
+    def __init__(self, params: MpParams):
+        self.params = MpParams()
+        C.memmove(C.byref(self.params), C.byref(params), C.sizeof(params))
+        h = C.c_void_p()
+        _check(lib().mkg_mp_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.N = self.params.ncell
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mkg_mp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, phi, u=None, p=None) -> None:
+        arrs = [None if v is None else np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                for v in (phi, u, p)]
+        _check(lib().mkg_mp_init(self._h, *[_ptr(a) for a in arrs]))
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().mkg_mp_step(self._h, int(nsteps)))
+
+    def sync(self) -> None:
+        _check(lib().mkg_mp_sync(self._h))
+
+    def get(self, which: int) -> np.ndarray:
+        n = {MP_POP_FLOW: 19 * self.N, MP_POP_PHASE: 19 * self.N, MP_VEL: 3 * self.N}.get(which, self.N)
+        out = np.empty(n)
+        _check(lib().mkg_mp_get(self._h, which, _ptr(out), n))
+        return out
+
+    def set(self, which: int, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        _check(lib().mkg_mp_set(self._h, which, _ptr(a), a.size))
+
+    def phi_sum(self) -> float:
+        out = np.zeros(1)
+        _check(lib().mkg_mp_sums(self._h, _ptr(out)))
+        return float(out[0])

@@ -1206,3 +1206,207 @@ int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int ev
     free(U);
     return 0;
 }
+
+/* ------------------------------------------------------------------ two-phase flow
+ * Phase-field two-phase flow of mkFluidsMultiphase (MaterKit/inc/mkFluidsMultiphase.h:20-99;
+ * run_flow_multiphase, MaterKit/MaterKit.cpp:290-331): its parameters are those of Liang et al.
+ * (PRE 89, 053320, 2014) -- rho_l, rho_g, sigma, W_0, beta = 12 sigma / W, kappa =
+ * 1.5 sigma W, eta "controls the mobility" (mkFluidsMultiphase.h:81-97); the bodies are in the
+ * closed DLL, so the published model is restated (BGK collisions, per-cell tau_f):
+ *   order parameter phi in [0, 1] (1: phase of rho_l), rho = rho_g + phi (rho_l - rho_g)
+ *   mu = 4 beta phi (phi - 1)(phi - 1/2) - kappa lap phi
+ *   Cahn-Hilliard lattice h (D3Q19): h0eq = phi + (w0 - 1) eta mu, hieq = w_i eta mu +
+ *     w_i phi c_i.u / cs^2;  source (1 - 1/(2 tau_h)) w_i c_i.[(phi u)^n - (phi u)^(n-1)] / cs^2;
+ *     M = eta cs^2 (tau_h - 1/2)
+ *   flow lattice f (D3Q19): f0eq = (w0 - 1) p / cs^2 + rho s_0(u), fieq = w_i p / cs^2 +
+ *     rho s_i(u), s_i = w_i (c.u/cs^2 + (c.u)^2/(2 cs^4) - u.u/(2 cs^2));
+ *     rho u = sum c f + F/2,  p = cs^2/(1 - w0) [sum_{i>0} f_i + u.grad rho/2 + rho s_0(u)],
+ *     F = mu grad phi + rho g + force;  forcing (1 - 1/(2 tau_f)) w_i [c_i.F +
+ *     (c_i.u)(c_i.grad rho)]/cs^2, whose zeroth moment u.grad rho keeps div u = 0 where rho
+ *     varies (the u.grad rho/2 of p is its half-step part);  nu = nu_g + phi (nu_l - nu_g),
+ *     tau_f = nu/cs^2 + 1/2
+ *   gradients and Laplacian: isotropic D3Q19 forms, grad a = sum w_i c_i a(x+c_i)/cs^2,
+ *     lap a = 2 sum w_i (a(x+c_i) - a(x))/cs^2; non-periodic faces replicate the edge value
+ *   faces: periodic, or bounce (no-slip wall for f, zero flux for h).
+ * One step: phi = sum h; mu; collide both lattices; stream. */
+static void mp_grad_lap(const orc_mp_params* m, const double* a, int X, int Y, int Z,
+                        double* grad, double* lap) {
+    orc_params g = {0};
+    g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
+    int per[3] = {m->bc[0] == 0, m->bc[2] == 0, m->bc[4] == 0};
+    lat_t L = make_lat(19);
+    double a0 = a[lin(&g, X, Y, Z)];
+    double gr[3] = {0, 0, 0}, lp = 0;
+    for (int i = 1; i < 19; i++) {
+        int XX = clampwrap(X + E27[i][0], m->nx, per[0]);
+        int YY = clampwrap(Y + E27[i][1], m->ny, per[1]);
+        int ZZ = clampwrap(Z + E27[i][2], m->nz, per[2]);
+        double v = a[lin(&g, XX, YY, ZZ)];
+        for (int d = 0; d < 3; d++) gr[d] += L.w[i] * E27[i][d] * v / L.cs2;
+        lp += 2.0 * L.w[i] * (v - a0) / L.cs2;
+    }
+    if (grad) for (int d = 0; d < 3; d++) grad[d] = gr[d];
+    if (lap) *lap = lp;
+}
+
+static void mp_phi_mu(const orc_mp_params* m, const double* h, double* phi, double* mu) {
+    long N = (long)m->nx * m->ny * m->nz;
+    for (long x = 0; x < N; x++) {
+        double s = 0;
+        for (int i = 0; i < 19; i++) s += h[i * N + x];
+        phi[x] = s;
+    }
+    double beta = 12.0 * m->sigma / m->width, kap = 1.5 * m->sigma * m->width;
+    orc_params g = {0};
+    g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int X = 0; X < m->nx; X++)
+        for (int Y = 0; Y < m->ny; Y++)
+            for (int Z = 0; Z < m->nz; Z++) {
+                long id = lin(&g, X, Y, Z);
+                double lp;
+                mp_grad_lap(m, phi, X, Y, Z, NULL, &lp);
+                double ph = phi[id];
+                mu[id] = 4.0 * beta * ph * (ph - 1.0) * (ph - 0.5) - kap * lp;
+            }
+}
+
+/* Macroscopic fields of a cell from its pre-collision f (full values), phi and mu. */
+static void mp_macro_cell(const orc_mp_params* m, const double* f, double ph, double muc,
+                          const double* gphi, double* rho_o, double* u, double* p_o, double* F) {
+    lat_t L = make_lat(19);
+    double rho = m->rho_g + ph * (m->rho_l - m->rho_g);
+    double grho[3];
+    for (int d = 0; d < 3; d++) {
+        grho[d] = (m->rho_l - m->rho_g) * gphi[d];
+        F[d] = muc * gphi[d] + rho * m->gravity[d] + m->force[d];
+    }
+    double j[3] = {0, 0, 0}, s1 = 0;
+    for (int i = 1; i < 19; i++) {
+        s1 += f[i];
+        for (int d = 0; d < 3; d++) j[d] += E27[i][d] * f[i];
+    }
+    for (int d = 0; d < 3; d++) u[d] = (j[d] + 0.5 * F[d]) / rho;
+    double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    double s0 = L.w[0] * (-uu / (2.0 * L.cs2));
+    double ugr = u[0] * grho[0] + u[1] * grho[1] + u[2] * grho[2];
+    *p_o = L.cs2 / (1.0 - L.w[0]) * (s1 + 0.5 * ugr + rho * s0);
+    *rho_o = rho;
+}
+
+int orc_mp_step(const orc_mp_params* m, double* f, double* h, double* phiu, int nsteps) {
+    long N = (long)m->nx * m->ny * m->nz;
+    lat_t L = make_lat(19);
+    orc_params g;
+    memset(&g, 0, sizeof(g));
+    g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
+    g.rho0 = 1.0;
+    for (int k = 0; k < 6; k++) g.bc_f[k] = g.bc_g[k] = m->bc[k];
+    orc_state st;
+    memset(&st, 0, sizeof(st));
+    int8_t* fl = calloc((size_t)N, 1);
+    st.flag = fl;
+    double* phi = malloc((size_t)N * sizeof(double));
+    double* mu = malloc((size_t)N * sizeof(double));
+    double* pf = malloc((size_t)19 * N * sizeof(double));
+    double* ph_ = malloc((size_t)19 * N * sizeof(double));
+    double* fresh = malloc((size_t)19 * N * sizeof(double));
+    double eta = m->mobility / (L.cs2 * (m->tau_h - 0.5));
+    for (int step = 0; step < nsteps; step++) {
+        mp_phi_mu(m, h, phi, mu);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int X = 0; X < m->nx; X++)
+            for (int Y = 0; Y < m->ny; Y++)
+                for (int Z = 0; Z < m->nz; Z++) {
+                    long id = lin(&g, X, Y, Z);
+                    double fc[19], gphi[3], u[3], F[3], rho, p;
+                    for (int i = 0; i < 19; i++) fc[i] = f[i * N + id];
+                    mp_grad_lap(m, phi, X, Y, Z, gphi, NULL);
+                    double ph = phi[id];
+                    mp_macro_cell(m, fc, ph, mu[id], gphi, &rho, u, &p, F);
+                    double grho[3];
+                    for (int d = 0; d < 3; d++) grho[d] = (m->rho_l - m->rho_g) * gphi[d];
+                    double pc = ph < 0 ? 0 : (ph > 1 ? 1 : ph);
+                    double nu = m->nu_g + pc * (m->nu_l - m->nu_g);
+                    double tf = nu / L.cs2 + 0.5;
+                    double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                    for (int i = 0; i < 19; i++) {
+                        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                        double cg = E27[i][0] * grho[0] + E27[i][1] * grho[1] + E27[i][2] * grho[2];
+                        double cF = E27[i][0] * F[0] + E27[i][1] * F[1] + E27[i][2] * F[2];
+                        double si = L.w[i] * (cu / L.cs2 + cu * cu / (2 * L.cs2 * L.cs2) - uu / (2 * L.cs2));
+                        double feq = (i == 0 ? (L.w[0] - 1.0) : L.w[i]) * p / L.cs2 + rho * si;
+                        double Fi = (1.0 - 0.5 / tf) * L.w[i] * (cF + cu * cg) / L.cs2;
+                        pf[i * N + id] = fc[i] - (fc[i] - feq) / tf + Fi;
+                    }
+                    /* Cahn-Hilliard lattice */
+                    double nphiu[3] = {ph * u[0], ph * u[1], ph * u[2]};
+                    double dpu[3];
+                    for (int d = 0; d < 3; d++) dpu[d] = nphiu[d] - phiu[3 * id + d];
+                    for (int i = 0; i < 19; i++) {
+                        double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                        double cd = E27[i][0] * dpu[0] + E27[i][1] * dpu[1] + E27[i][2] * dpu[2];
+                        double heq = (i == 0) ? ph + (L.w[0] - 1.0) * eta * mu[id]
+                                              : L.w[i] * eta * mu[id] + L.w[i] * ph * cu / L.cs2;
+                        double Hi = (1.0 - 0.5 / m->tau_h) * L.w[i] * cd / L.cs2;
+                        double hi = h[i * N + id];
+                        ph_[i * N + id] = hi - (hi - heq) / m->tau_h + Hi;
+                    }
+                    for (int d = 0; d < 3; d++) phiu[3 * id + d] = nphiu[d];
+                }
+        stream(&g, &st, 19, m->bc, 0, NULL, NULL, NULL, NULL, pf, f, fresh);
+        memcpy(f, fresh, (size_t)19 * N * sizeof(double));
+        stream(&g, &st, 19, m->bc, 1, NULL, NULL, NULL, NULL, ph_, h, fresh);
+        memcpy(h, fresh, (size_t)19 * N * sizeof(double));
+    }
+    free(fl); free(phi); free(mu); free(pf); free(ph_); free(fresh);
+    return 0;
+}
+
+void orc_mp_macro(const orc_mp_params* m, const double* f, const double* h, double* phi,
+                  double* mu, double* p, double* u3) {
+    long N = (long)m->nx * m->ny * m->nz;
+    orc_params g = {0};
+    g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
+    mp_phi_mu(m, h, phi, mu);
+    for (int X = 0; X < m->nx; X++)
+        for (int Y = 0; Y < m->ny; Y++)
+            for (int Z = 0; Z < m->nz; Z++) {
+                long id = lin(&g, X, Y, Z);
+                double fc[19], gphi[3], F[3], rho;
+                for (int i = 0; i < 19; i++) fc[i] = f[i * N + id];
+                mp_grad_lap(m, phi, X, Y, Z, gphi, NULL);
+                mp_macro_cell(m, fc, phi[id], mu[id], gphi, &rho, u3 + 3 * id, p + id, F);
+            }
+}
+
+/* Equilibrium start from phi (N), u (3N or NULL), p (N or NULL); phiu receives phi u. */
+void orc_mp_init(const orc_mp_params* m, const double* phi0, const double* u0, const double* p0,
+                 double* f, double* h, double* phiu) {
+    long N = (long)m->nx * m->ny * m->nz;
+    lat_t L = make_lat(19);
+    double* phi = malloc((size_t)N * sizeof(double));
+    double* mu = malloc((size_t)N * sizeof(double));
+    /* mu of the initial phi: h at rest equilibrium first, then mu from it */
+    for (long x = 0; x < N; x++)
+        for (int i = 0; i < 19; i++) h[i * N + x] = L.w[i] * phi0[x];
+    mp_phi_mu(m, h, phi, mu);
+    double eta = m->mobility / (L.cs2 * (m->tau_h - 0.5));
+    for (long x = 0; x < N; x++) {
+        double u[3] = {0, 0, 0};
+        if (u0) for (int d = 0; d < 3; d++) u[d] = u0[3 * x + d];
+        double ph = phi0[x], p = p0 ? p0[x] : 0.0;
+        double rho = m->rho_g + ph * (m->rho_l - m->rho_g);
+        double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        for (int i = 0; i < 19; i++) {
+            double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+            double si = L.w[i] * (cu / L.cs2 + cu * cu / (2 * L.cs2 * L.cs2) - uu / (2 * L.cs2));
+            f[i * N + x] = (i == 0 ? (L.w[0] - 1.0) : L.w[i]) * p / L.cs2 + rho * si;
+            h[i * N + x] = (i == 0) ? ph + (L.w[0] - 1.0) * eta * mu[x]
+                                    : L.w[i] * eta * mu[x] + L.w[i] * ph * cu / L.cs2;
+        }
+        for (int d = 0; d < 3; d++) phiu[3 * x + d] = ph * u[d];
+    }
+    free(phi);
+    free(mu);
+}

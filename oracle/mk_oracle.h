@@ -128,6 +128,27 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
  * C_face as U values). */
 int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int evolve_phase);
 
+/* Two-phase flow (mkFluidsMultiphase; Liang et al. 2014 phase-field model): Cahn-Hilliard
+ * lattice h and velocity-pressure flow lattice f, both D3Q19.  Same layout as the product's
+ * mkg_mp_params. */
+typedef struct orc_mp_params {
+    int32_t nx, ny, nz, precision;
+    int32_t bc[6];                 /* periodic (0) or bounce (1): no-slip wall / zero flux */
+    double rho_l, rho_g;           /* densities of phi = 1 and phi = 0 */
+    double nu_l, nu_g;             /* kinematic viscosities */
+    double sigma, width;           /* surface tension, interface width W */
+    double mobility, tau_h;        /* M and the Cahn-Hilliard relaxation time */
+    double gravity[3], force[3];   /* body acceleration and body-force density */
+} orc_mp_params;
+
+/* f, h: 19 x N; phiu: 3 x N (phi u of the last step) */
+void orc_mp_init(const orc_mp_params* m, const double* phi0, const double* u0, const double* p0,
+                 double* f, double* h, double* phiu);
+int orc_mp_step(const orc_mp_params* m, double* f, double* h, double* phiu, int nsteps);
+/* phi, mu, p (N each) and u (3 x N, cell-interleaved) of the current state */
+void orc_mp_macro(const orc_mp_params* m, const double* f, const double* h, double* phi,
+                  double* mu, double* p, double* u3);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 

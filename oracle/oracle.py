@@ -64,6 +64,10 @@ def lib() -> ct.CDLL:
         L.orc_uniform.restype = ct.c_double
         L.orc_cell_uniform.argtypes = [ct.c_uint64, ct.c_uint64]
         L.orc_cell_uniform.restype = ct.c_double
+        L.orc_mp_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_mp_step.argtypes = [vp, vp, vp, vp, ct.c_int]
+        L.orc_mp_step.restype = ct.c_int
+        L.orc_mp_macro.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
         L.orc_fvm_step.restype = ct.c_int
         L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -289,3 +293,38 @@ def fvm_step(params, phi, c, nsteps: int = 1, evolve_phase: bool = True) -> None
     rc = lib().orc_fvm_step(ct.byref(params), _p(phi), _p(c), int(nsteps), 1 if evolve_phase else 0)
     if rc != 0:
         raise RuntimeError("oracle rejected the finite-volume parameter block")
+
+
+@dataclass
+class MpState:
+    """Oracle state of the two-phase flow (params is an ``MpParams``-layout struct):
+    f, h 19 x N full-value populations, phiu 3 x N (phi u of the last step, cell-interleaved)."""
+
+    params: object
+    f: np.ndarray = None
+    h: np.ndarray = None
+    phiu: np.ndarray = None
+
+    @classmethod
+    def start(cls, params, phi, u=None, p=None) -> "MpState":
+        N = int(params.nx) * int(params.ny) * int(params.nz)
+        s = cls(params=params, f=np.zeros(19 * N), h=np.zeros(19 * N), phiu=np.zeros(3 * N))
+        ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        pp = None if p is None else np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        lib().orc_mp_init(ct.byref(params), _p(ph), _p(uu), _p(pp), _p(s.f), _p(s.h),
+                          _p(s.phiu))
+        return s
+
+    def step(self, nsteps: int = 1) -> None:
+        rc = lib().orc_mp_step(ct.byref(self.params), _p(self.f), _p(self.h), _p(self.phiu),
+                               int(nsteps))
+        if rc != 0:
+            raise RuntimeError("oracle rejected the two-phase parameter block")
+
+    def macro(self) -> dict:
+        N = self.f.size // 19
+        phi, mu, p, u = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(3 * N)
+        lib().orc_mp_macro(ct.byref(self.params), _p(self.f), _p(self.h), _p(phi), _p(mu),
+                           _p(p), _p(u))
+        return {"phi": phi, "mu": mu, "p": p, "u": u.reshape(N, 3)}

@@ -1,0 +1,118 @@
+"""Phase-field two-phase flow on the device (HIP, AA pattern) vs the FP64 oracle (two-lattice
+pull): FP64 to 1e-12 and FP32 to 1e-5 of each lattice's magnitude after the same steps, odd
+and even step counts, periodic faces and no-slip walls, density and viscosity contrast,
+gravity."""
+import math
+
+import numpy as np
+import pytest
+
+from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+P, BB = mkg.BC_PERIODIC, mkg.BC_BOUNCE
+
+CASES = {
+    "periodic_3d": dict(dims=(12, 10, 9), bc=[P] * 6),
+    "walls_y": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P]),
+    "walls_xz_2d": dict(dims=(16, 14, 1), bc=[BB, BB, P, P, P, P]),
+}
+
+
+def make(name, prec):
+    cfg = CASES[name]
+    d = cfg["dims"]
+    m = mkg.mp_params(nx=d[0], ny=d[1], nz=d[2], precision=prec, rho_l=3.0, rho_g=1.0,
+                      nu_l=0.12, nu_g=0.05, sigma=0.005, width=4.0, mobility=0.08, tau_h=0.9)
+    m.gravity[1] = -2e-5
+    m.force[0] = 1e-6
+    for f in range(6):
+        m.bc[f] = cfg["bc"][f]
+    return m
+
+
+def start(m, seed=6):
+    rng = np.random.default_rng(seed)
+    d = (int(m.nx), int(m.ny), int(m.nz))
+    X, Y, Z = np.meshgrid(*[np.arange(n) for n in d], indexing="ij")
+    r = np.sqrt((X - d[0] / 2.2) ** 2 + (Y - d[1] / 1.9) ** 2 + (Z - d[2] / 2.0) ** 2)
+    phi = (0.5 + 0.5 * np.tanh(2 * (min(d[0], d[1]) / 4.0 - r) / 4.0)).reshape(-1)
+    u = 0.01 * rng.standard_normal((phi.size, 3))
+    p = 1e-4 * rng.standard_normal(phi.size)
+    return phi, u, p
+
+
+@pytest.mark.parametrize("steps", [5, 6])
+@pytest.mark.parametrize("prec,tol", [(mkg.PREC_FP64, 1e-12), (mkg.PREC_FP32, 1e-5)],
+                         ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_two_phase_matches_oracle(name, prec, tol, steps):
+    m = make(name, prec)
+    phi, u, p = start(m)
+    print(f"two-phase {name} prec {prec} steps {steps}", flush=True)
+    orc = O.MpState.start(make(name, mkg.PREC_FP64), phi, u=u, p=p)
+    dev = mkg.MpSolver(m)
+    # the device starts from the oracle's populations (its own init is checked below)
+    dev.set(mkg.MP_POP_FLOW, orc.f)
+    dev.set(mkg.MP_POP_PHASE, orc.h)
+    dev.set(mkg.MP_PHIU, orc.phiu)
+    dev.step(steps)
+    orc.step(steps)
+    gf, gh = dev.get(mkg.MP_POP_FLOW), dev.get(mkg.MP_POP_PHASE)
+    mac = orc.macro()
+    gp, gu = dev.get(mkg.MP_PRESSURE), dev.get(mkg.MP_VEL).reshape(-1, 3)
+    gphi = dev.get(mkg.MP_PHI)
+    s = dev.phi_sum()
+    dev.close()
+    assert np.abs(gf - orc.f).max() < tol * np.abs(orc.f).max()
+    assert np.abs(gh - orc.h).max() < tol * np.abs(orc.h).max()
+    assert np.abs(gphi - mac["phi"]).max() < tol
+    assert np.abs(gu - mac["u"]).max() < tol * max(np.abs(mac["u"]).max(), 1e-3) * 10
+    assert np.abs(gp - mac["p"]).max() < tol * np.abs(mac["p"]).max() * 10
+    assert abs(s - mac["phi"].sum()) < tol * phi.size
+
+
+@pytest.mark.parametrize("prec,tol", [(mkg.PREC_FP64, 1e-12), (mkg.PREC_FP32, 1e-5)],
+                         ids=["fp64", "fp32"])
+def test_two_phase_device_start_matches_oracle_start(prec, tol):
+    m = make("periodic_3d", prec)
+    phi, u, p = start(m)
+    orc = O.MpState.start(make("periodic_3d", mkg.PREC_FP64), phi, u=u, p=p)
+    dev = mkg.MpSolver(m)
+    dev.init(phi, u, p)
+    gf, gh = dev.get(mkg.MP_POP_FLOW), dev.get(mkg.MP_POP_PHASE)
+    dev.close()
+    assert np.abs(gf - orc.f).max() < tol * np.abs(orc.f).max()
+    assert np.abs(gh - orc.h).max() < tol * np.abs(orc.h).max()
+
+
+def test_two_phase_static_drop_on_device():
+    """The static-drop Laplace check at FP32 on the device: dp R / sigma within 5 %."""
+    n, sigma, R = 48, 0.01, 12.0
+    m = mkg.mp_params(nx=n, ny=n, nz=1, precision=mkg.PREC_FP32, sigma=sigma, width=4.0,
+                      mobility=0.1, tau_h=1.0)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    r = np.sqrt((X - n / 2 + 0.5) ** 2 + (Y - n / 2 + 0.5) ** 2)
+    phi = (0.5 + 0.5 * np.tanh(2 * (R - r) / 4.0)).reshape(-1)
+    dev = mkg.MpSolver(m)
+    dev.init(phi)
+    dev.step(5000)
+    p = dev.get(mkg.MP_PRESSURE)
+    ph = dev.get(mkg.MP_PHI)
+    dev.close()
+    dp = p[ph > 0.99].mean() - p[ph < 0.01].mean()
+    req = math.sqrt(ph.sum() / math.pi)
+    assert abs(dp * req / sigma - 1.0) < 0.05
+
+
+def test_two_phase_rejects_bad_parameters():
+    m = make("periodic_3d", mkg.PREC_FP64)
+    m.bc[0] = mkg.BC_DIRICHLET
+    with pytest.raises(mkg.MkgError, match="periodic or bounce"):
+        mkg.MpSolver(m)
+    m = make("periodic_3d", mkg.PREC_FP64)
+    m.tau_h = 0.4
+    with pytest.raises(mkg.MkgError, match="tau_h"):
+        mkg.MpSolver(m)
