@@ -239,11 +239,11 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
         uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
         if (hit == 0u) {
             const uint32_t t = nbr(c, dx, dy, dz);
-            v = ldp(&A[(size_t)i * N + t]) + (cw ? cw[t] : (R)0);
+            v = ldp(slot_at(A + (size_t)i * N, t)) + (cw ? cw[t] : (R)0);
         } else {
             const int face = __builtin_ctz(hit);
             const int tb = pick6(face, L.bc);
-            const R own = A[(size_t)inv(i) * N + c.id] + (cw ? cw[c.id] : (R)0);
+            const R own = *slot_at(A + (size_t)inv(i) * N, c.id) + (cw ? cw[c.id] : (R)0);
             if (tb == kBcOutflow) v = 0;
             else if (tb == kBcDirichlet && kind == kKindScalar) v = -own;
             else v = own;
@@ -252,7 +252,7 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
         uint32_t hin = c.bits & faces_of(-dx, -dy, -dz) & (uint32_t)L.npmask;
         if (hin != 0u) {
             const int face = __builtin_ctz(hin);
-            if (pick6(face, L.bc) == kBcOutflow) v += A[(size_t)i * N + c.id] + (cw ? cw[c.id] : (R)0);
+            if (pick6(face, L.bc) == kBcOutflow) v += *slot_at(A + (size_t)i * N, c.id) + (cw ? cw[c.id] : (R)0);
         }
         out[i] = v;
     }
@@ -334,19 +334,19 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         const R Th = om * B + (a.pf_coupling == 1 ? uh : (R)0);
         R* __restrict__ ho = (R*)j.hout;
 #pragma unroll
-        for (int k = 0; k < 7; k++) stp(&ho[(size_t)k * N + id], ((R)1 - om) * hs[k] + Th);
+        for (int k = 0; k < 7; k++) stp(slot_at(ho + (size_t)k * N, id), ((R)1 - om) * hs[k] + Th);
         const R L = (R)a.latent_T;
         const R pin = ((const R*)j.pin)[id];
         const R phl = (R)a.omega_g * A + L * B + pin + Om * dom_dphi;
         R* __restrict__ go = (R*)j.gout;
         const R og = (R)1 - (R)a.omega_g;
 #pragma unroll
-        for (int i = 0; i < 19; i++) stp(&go[(size_t)i * N + id], og * gs[i] + phl);
+        for (int i = 0; i < 19; i++) stp(slot_at(go + (size_t)i * N, id), og * gs[i] + phl);
         ((R*)j.pout)[id] = -L * B;
         if (a.has_kappa) ((R*)j.kap_grad)[id] += Om * dom_dkap;
         R* __restrict__ Pb = (R*)j.P;
 #pragma unroll
-        for (int i = 0; i < 19; i++) stp(&Pb[(size_t)i * N + id], Ph[PIX(cx(i), cy(i), cz(i))]);
+        for (int i = 0; i < 19; i++) stp(slot_at(Pb + (size_t)i * N, id), Ph[PIX(cx(i), cy(i), cz(i))]);
         eps_loc = (double)eh;
     }
     // FP64 block sum of dL/deps4 (wavefront butterfly + LDS), accumulated per block
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(256) k_adj_stencil(const StepArgs a, const R* 
 #pragma unroll
                 for (int i2 = 0; i2 < 2; i2++)
                     if (vx[i0] && vy[i1] && vz[i2])
-                        s += ldp(&P[(size_t)k * N + ox[i0] + oy[i1] + oz[i2]]);
+                        s += ldp(slot_at(P + (size_t)k * N, ox[i0] + oy[i1] + oz[i2]));
     }
     nout[c.id] = s;
 }
