@@ -4,19 +4,21 @@
 // by run_multiPhaseField (MaterKit/MaterKit.cpp:529-576).  Same AA-pattern storage, face rules
 // and wall handling as the fused solver step (device.h).
 //
-// One step is two launches:
+// One step is n + 1 launches:
 //   k_mpf_moments  phi_j = sum_i g_j,i for every cell and the combined phi = (n-1) + sum phi_j
 //                  (the first sweep of mkFluidsUser::relaxation, :147-172); the gradient
 //                  stencils of the collision read the neighbours' phi_j
-//   k_mpf_step     per cell: thermal collide-stream (latent source from the combined dphi), then
-//                  for every order parameter: LB-weighted gradient s_j = sum_i c_i w_i phi_j(x+c_i)
+//   k_mpf_one      (one launch per order parameter j) per cell: LB-weighted gradient
+//                  s_j = sum_i c_i w_i phi_j(x+c_i)
 //                  and normal n_j = s_j / (|s_j| + eps) (:208-217), equilibrium with phi_j as the
 //                  density and the order parameter's own velocity (getEquPDFs, :219-230), the
 //                  interaction P_j = -2 kappa_j (phi_j/2 + 1/2) sum_{k != j} (phi_k/2 + 1/2)^2
 //                  (:236-256), the growth term Q_j (:266-271), the conservative Allen-Cahn
 //                  counter-term (1 - omega/2) w_i (c_i . n_j) (1 - phi_j^2) dx / (sqrt 2 W0)
 //                  (getAddedPhasePDFs, Chai et al. 2018, :273-279), BGK with
-//                  omega_g = 1/(dt/dx^2/cs^2 + 1/2) (:282-293), then streaming.
+//                  omega_g = 1/(dt/dx^2/cs^2 + 1/2) (:282-293), then streaming; the launch of
+//                  the last order parameter also collides and streams the thermal lattice
+//                  (latent source from the combined dphi).
 // Cells are not re-classified: a cell other than fluid does not collide in the template
 // (:185-188), so classifying interface cells would freeze every grain.  Flag 3 (walls) is the
 // only non-fluid code used here.
@@ -100,82 +102,126 @@ __device__ inline void mpf_gradient(const StepArgs& a, const Cell& c, const R* p
     }
 }
 
-// No occupancy bound: the FP32 kernel takes 168-199 VGPRs (two or three waves per SIMD);
-// bounded to three waves it spills and measured 7 % slower at 256^3.  O steps access the
-// shifted z populations directly: the z stage of the fused step (LDS, two barriers per order
-// parameter) measured 7-13 % slower here, at two waves per SIMD.
-template <class R, bool TH, bool ODD>
-__global__ void __launch_bounds__(256) k_mpf_step(const MpfArgs m) {
+// Collide-stream of one order parameter j (one launch per order parameter; the thermal
+// lattice rides on the last one).  The populations of order parameter j are updated in place,
+// which no other launch of the step reads; T for the growth term comes from the thermal
+// populations, which only the last launch updates.  O steps take the z shift through LDS when
+// a block holds whole z rows (ZS): one barrier after the pull loads, one after the push stage.
+// Launched per order parameter, the kernel holds one lattice in registers (four waves per
+// SIMD); the earlier single launch over all order parameters took 168-232 VGPRs.
+template <class R, bool TH, bool ODD, bool ZS = false>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
+    k_mpf_one(const MpfArgs m, const LatArgs L, const int j, const double ux_, const double uy_,
+              const double uz_, const double kappa_j) {
+    constexpr bool Z = ODD && ZS;
+    constexpr int ZQ = Z ? nzdirs<19>() : 1;
+    constexpr int ZT = (Z && TH) ? nzdirs<7>() : 1;
+    __shared__ R zr[ZQ][256], zw[ZQ][256], zrh[ZT][256], zwh[ZT][256];
+    __shared__ uint32_t zm[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1];
+    const uint32_t tid = threadIdx.x;
     Cell c;
-    if (!locate(m.a, c)) return;
-    if (m.a.flag[c.id] == kIsWalls) return;
+    bool act = true;
+    if constexpr (Z) {
+        act = locate_clamped(m.a, c);
+    } else {
+        if (!locate(m.a, c)) return;
+    }
+    const bool wall = m.a.flag[c.id] == kIsWalls;
+    if constexpr (Z) {
+        act = act && !wall;
+    } else {
+        if (wall) return;
+    }
     const StepArgs& a = m.a;
     const uint32_t S = a.S;
     const R* __restrict__ cur = (const R*)a.phi_cur;
     const R* __restrict__ prev = (const R*)a.phi_prev;
-    R ph[MKG_MPF_MAX];
-#pragma unroll
-    for (int k = 0; k < MKG_MPF_MAX; k++)
-        ph[k] = k < m.n ? *slot_at(cur + (size_t)k * S, c.id) : (R)-1;
+    const R pj = *slot_at(cur + (size_t)j * S, c.id);
+    R P = 0;
+    for (int k = 0; k < m.n; k++) {
+        if (k != j) {
+            const R hk = (R)0.5 * *slot_at(cur + (size_t)k * S, c.id) + (R)0.5;
+            P += hk * hk;
+        }
+    }
+    P *= (R)(-2.0 * kappa_j) * ((R)0.5 * pj + (R)0.5);
+    R s[3];
+    mpf_gradient<R>(a, c, cur + (size_t)j * S, s);
 
-    // thermal collide-stream (mkThermalField::propagate / relaxation of run_multiPhaseField)
+    // loads: order parameter j, and the thermal populations on the last launch
+    R g[19];
+    uint32_t own = 0;
+    if constexpr (Z)
+        own = zpull_issue<19, R>(g, L, S, c, a, zr, tid);
+    else
+        read_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
+    R h[TH ? 7 : 1];
+    uint32_t own_h = 0;
+    if constexpr (TH) {
+        if constexpr (Z)
+            own_h = zpull_issue<7, R>(h, a.h, S, c, a, zrh, tid);
+        else
+            read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+    }
+    if constexpr (Z) {
+        __syncthreads();
+        zpull_finish<19, R>(g, c, a, zr, tid, own);
+        if constexpr (TH) zpull_finish<7, R>(h, c, a, zrh, tid, own_h);
+    }
     R T = (R)m.T_ref;
     if constexpr (TH) {
-        R h[7];
-        read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+        // thermal collide-stream (mkThermalField::propagate / relaxation of run_multiPhaseField)
         R sh = 0;
 #pragma unroll
         for (int i = 0; i < 7; i++) sh += h[i];
         T = (R)a.h.ref + sh;
         const R dphi = *slot_at(cur + (size_t)m.n * S, c.id) - *slot_at(prev + (size_t)m.n * S, c.id);
-        const R om = (R)m.omega_h, src = (R)m.latent_T * dphi;
+        const R omh = (R)m.omega_h, srh = (R)m.latent_T * dphi;
 #pragma unroll
         for (int i = 0; i < 7; i++) {
             const R w = (R)wgt<7>(i);
-            h[i] = h[i] - om * (h[i] - w * sh) + w * src;
+            h[i] = h[i] - omh * (h[i] - w * sh) + w * srh;
         }
-        write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+        if constexpr (Z)
+            zpush_stage<7, R>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act, FaceState<R>());
+        else
+            write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+    } else {
+        if (m.thermal && m.growth != 0.0) {
+            R hh[7];
+            read_pops_rt<7, R>(hh, a.h, S, c, a, kKindScalar);
+            R sh = 0;
+#pragma unroll
+            for (int i = 0; i < 7; i++) sh += hh[i];
+            T = (R)a.h.ref + sh;
+        }
     }
-
+    const R sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + (R)m.eps;
+    const R n0 = s[0] / sn, n1 = s[1] / sn, n2 = s[2] / sn;
+    const R d = (R)1 - pj * pj;
+    const R Q = (R)m.growth * (pj - (R)m.lambda * T * d) * d;
+    const R src = (R)m.dt * (P + Q);
+    const R ux = (R)ux_, uy = (R)uy_, uz = (R)uz_;
+    const R uu = ux * ux + uy * uy + uz * uz;
+    const R dp = pj - (R)L.ref;
+    const R zd = (R)m.zc * d;
     const R om = (R)m.omega_g;
 #pragma unroll
-    for (int j = 0; j < MKG_MPF_MAX; j++) {
-        if (j < m.n) {
-            R s[3];
-            mpf_gradient<R>(a, c, cur + (size_t)j * S, s);
-            const R sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + (R)m.eps;
-            const R n0 = s[0] / sn, n1 = s[1] / sn, n2 = s[2] / sn;
-            const R pj = ph[j];
-            R P = 0;
-#pragma unroll
-            for (int k = 0; k < MKG_MPF_MAX; k++) {
-                if (k != j && k < m.n) {
-                    const R hk = (R)0.5 * ph[k] + (R)0.5;
-                    P += hk * hk;
-                }
-            }
-            P *= (R)(-2.0 * m.kappa[j]) * ((R)0.5 * pj + (R)0.5);
-            const R d = (R)1 - pj * pj;
-            const R Q = (R)m.growth * (pj - (R)m.lambda * T * d) * d;
-            const R src = (R)m.dt * (P + Q);
-            const R ux = (R)m.u[j][0], uy = (R)m.u[j][1], uz = (R)m.u[j][2];
-            const R uu = ux * ux + uy * uy + uz * uz;
-            const LatArgs& L = m.gl[j];  // j is unrolled: a fixed offset, no copy
-            const R dp = pj - (R)L.ref;
-            const R zd = (R)m.zc * d;
-            R g[19];
-            read_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
-#pragma unroll
-            for (int i = 0; i < 19; i++) {
-                const R w = (R)wgt<19>(i);
-                const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
-                const R cn = cx(i) * n0 + cy(i) * n1 + cz(i) * n2;
-                // deviation form: g~eq = w (phi_j - ref) + w phi_j (3 cu + 4.5 cu^2 - 1.5 uu)
-                const R geq = w * (dp + pj * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu));
-                g[i] = g[i] - om * (g[i] - geq) + w * (src + zd * cn);
-            }
-            write_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
-        }
+    for (int i = 0; i < 19; i++) {
+        const R w = (R)wgt<19>(i);
+        const R cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
+        const R cn = cx(i) * n0 + cy(i) * n1 + cz(i) * n2;
+        // deviation form: g~eq = w (phi_j - ref) + w phi_j (3 cu + 4.5 cu^2 - 1.5 uu)
+        const R geq = w * (dp + pj * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu));
+        g[i] = g[i] - om * (g[i] - geq) + w * (src + zd * cn);
+    }
+    if constexpr (Z) {
+        zpush_stage<19, R>(g, L, S, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
+        __syncthreads();
+        zpush_finish<19, R>(L, S, c, a, zw, zm, tid);
+        if constexpr (TH) zpush_finish<7, R>(a.h, S, c, a, zwh, zmh, tid);
+    } else {
+        write_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
     }
 }
 
@@ -244,6 +290,7 @@ struct mkg_mpf_ctx {
     double* partials = nullptr;
     double* red_out = nullptr;
     int lay = 0;              // AA layout of every lattice
+    int zstage = 0;           // O steps take the z shift through LDS
     int64_t pf_count = 0;
     int has_walls = 0;
     size_t lat_elems() const { return (size_t)19 * ((size_t)S + F); }
@@ -277,6 +324,7 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
     a.nblk = h->nblk;
     a.F = h->F;
     a.has_walls = h->has_walls;
+    a.zstage = h->zstage;
     a.flag = h->flag;
     a.rho0 = 1.0;
     auto fill = [&](LatArgs& L, void* pop, const int32_t* bc, double ref, const double* fval) {
@@ -340,15 +388,23 @@ int mpf_normalize(mkg_mpf_ctx* h) {
     return 0;
 }
 
+template <class R, bool TH>
+void launch_mpf_one(const MpfArgs& m, int j, int odd, int zs, hipStream_t s) {
+    dim3 grid(m.a.nblk), block(256);
+    const double* u = m.u[j];
+    if (odd && zs)
+        k_mpf_one<R, TH, true, true><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+    else if (odd)
+        k_mpf_one<R, TH, true><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+    else
+        k_mpf_one<R, TH, false><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+}
+// Collide-stream of every lattice: one launch per order parameter, thermal on the last.
 template <class R>
 void launch_mpf_step_kernel(const MpfArgs& m, int odd, int th, hipStream_t s) {
-    dim3 grid(m.a.nblk), block(256);
-    if (th) {
-        if (odd) k_mpf_step<R, true, true><<<grid, block, 0, s>>>(m);
-        else k_mpf_step<R, true, false><<<grid, block, 0, s>>>(m);
-    } else {
-        if (odd) k_mpf_step<R, false, true><<<grid, block, 0, s>>>(m);
-        else k_mpf_step<R, false, false><<<grid, block, 0, s>>>(m);
+    for (int j = 0; j < m.n; j++) {
+        if (th && j == m.n - 1) launch_mpf_one<R, true>(m, j, odd, m.a.zstage, s);
+        else launch_mpf_one<R, false>(m, j, odd, m.a.zstage, s);
     }
 }
 
@@ -419,6 +475,10 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     h->S = ((h->N + 127u) & ~127u) + kMpfPad;
     h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
     h->nblk = (h->N + 255u) / 256u;
+    // z stage for O steps of the collide-stream when a 256-thread block holds whole z rows
+    // (nz = 64, 128 or 256); MKG_ZSTAGE=0 turns it off
+    h->zstage = (p->nz >= 64 && 256 % p->nz == 0) ? 1 : 0;
+    if (const char* env = std::getenv("MKG_ZSTAGE")) h->zstage = h->zstage && std::atoi(env) != 0;
     auto bad = [&](hipError_t e, const char* what) {
         mpf_destroy_impl(h);
         return mfail(std::string(what) + ": " + hipGetErrorString(e));

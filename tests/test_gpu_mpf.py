@@ -26,8 +26,8 @@ CASES = {
 }
 
 
-def make(name, prec, growth=0.0):
-    dims, n, th, bc, bch, walls = CASES[name]
+def make(name, prec, growth=0.0, cfg=None):
+    dims, n, th, bc, bch, walls = cfg if cfg is not None else CASES[name]
     m = mkg.default_mpf_params(nx=dims[0], ny=dims[1], nz=dims[2], n=n, precision=prec,
                                thermal=th, growth=growth, lambda_=3.19, alpha=0.12,
                                latent_T=0.5, T_ref=-0.3)
@@ -142,3 +142,36 @@ def test_mpf_rejects_bad_parameters():
     with pytest.raises(mkg.MkgError, match="count"):
         dev.set(mkg.MPF_PHI, np.zeros(10))
     dev.close()
+
+
+@pytest.mark.parametrize("steps", [5, 6])
+@pytest.mark.parametrize("faces", ["periodic", "mixed"])
+def test_mpf_z_stage_matches_oracle(faces, steps):
+    """nz = 64: the O steps of the collide-stream take the z shift through LDS."""
+    cfg = ((6, 5, 64), 2, 1, [P] * 6 if faces == "periodic" else [BB, OUT, P, P, DI, BB],
+           [P] * 6 if faces == "periodic" else [DI, OUT, P, P, DI, BB], False)
+    m = make("z64", mkg.PREC_FP64, growth=0.3, cfg=cfg)
+    dev, orc = random_start(m)
+    print(f"z64 {faces} steps {steps}", flush=True)
+    dev.step(steps)
+    orc.step(steps)
+    e = errors(dev, orc)
+    dev.close()
+    assert all(v < 1e-12 for v in e.values()), e
+
+
+def test_mpf_z_stage_agrees_with_direct_o_steps(monkeypatch):
+    m = mkg.default_mpf_params(nx=6, ny=5, nz=128, n=3, precision=mkg.PREC_FP32, thermal=1,
+                               growth=0.2, lambda_=3.19, alpha=0.1)
+    m.u[1][0], m.u[2][2] = 0.02, -0.01
+    out = []
+    for z in ("1", "0"):
+        monkeypatch.setenv("MKG_ZSTAGE", z)
+        phi = np.stack([np.sin(0.3 * np.arange(m.ncell) + k) * 0.9 for k in range(3)])
+        dev = mkg.MpfSolver(m)
+        dev.init(phi)
+        dev.step(7)
+        out.append((dev.get(mkg.MPF_POPS), dev.get(mkg.MPF_THERMAL)))
+        dev.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
