@@ -376,6 +376,13 @@ int mkg_mp_get(mkg_mp_handle h, int which, double* host, size_t count);
 int mkg_mp_set(mkg_mp_handle h, int which, const double* host, size_t count);
 /* FP64 sum of phi over every cell. */
 int mkg_mp_sums(mkg_mp_handle h, double* out);
+/* Two-phase parameters from the <multiphases> block of a case file (run_flow_multiphase's
+ * case, MaterKit/BoltzTechs - multiPhaseFlow.xml): densities relative to the rare fluid,
+ * kinematic viscosities and mobility scaled by mk_dt/mk_dx^2, W0, sigma, force; the body
+ * acceleration from flow/acceleration; faces from solver/bcFlow.  run_info (4, may be NULL):
+ * numLoops, numFrames, seed count, start geometry (solver/twoPhaseInit: 0 "layered", the
+ * dense fluid in the lower half along y; 1 "drops", one dense drop per seed). */
+int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info);
 
 #ifdef __cplusplus
 }

@@ -484,4 +484,59 @@ int mkg_seeds_from_xml(const char* path, int max, double* centers, double* radii
     return n;
 }
 
+// Two-phase flow parameters from the <multiphases> block of a case file (the block of
+// run_flow_multiphase's case, MaterKit/BoltzTechs - multiPhaseFlow.xml): fluidDense /
+// fluidRare density and kinematicViscosity, W0, Mobility, sigma, force; kinematic viscosity
+// and mobility are scaled by mk_dt / mk_dx^2 into lattice units, densities by the rare
+// fluid's.  flow/acceleration is the body acceleration; solver/bcFlow the faces (periodic or
+// bounce); solver/tauPhase the Cahn-Hilliard relaxation time (default 1).  The contact angle
+// is not read (zero-flux faces give 90 degrees).  run_info: numLoops, numFrames, seeds.
+int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info) {
+    if (!path || !m) { g_case_err = "null argument"; return -1; }
+    Case c;
+    if (!load_case(path, c)) return -1;
+    std::memset(m, 0, sizeof(*m));
+    auto mesh = c.nums("geometry/MeshXYZ");
+    if (mesh.size() != 3) { g_case_err = "geometry/MeshXYZ needs 3 values"; return -1; }
+    m->nx = (int32_t)mesh[0];
+    m->ny = (int32_t)mesh[1];
+    m->nz = (int32_t)mesh[2];
+    if (!c.has("multiphases")) { g_case_err = "the case has no <multiphases> block"; return -1; }
+    const double dx = c.num("numerics/mk_dx", 1.0), dt = c.num("numerics/mk_dt", 1.0);
+    const double lu = dt / (dx * dx);
+    const double rr = c.num("multiphases/fluidRare/density", 1.0);
+    if (!(rr > 0)) { g_case_err = "multiphases/fluidRare/density must be > 0"; return -1; }
+    m->rho_l = c.num("multiphases/fluidDense/density", rr) / rr;
+    m->rho_g = 1.0;
+    m->nu_l = c.num("multiphases/fluidDense/kinematicViscosity", 0.1) * lu;
+    m->nu_g = c.num("multiphases/fluidRare/kinematicViscosity", 0.1) * lu;
+    m->width = c.num("multiphases/W0", 4.0);
+    m->mobility = c.num("multiphases/Mobility", 0.1) * lu;
+    m->sigma = c.num("multiphases/sigma", 0.001);
+    auto f = c.nums("multiphases/force");
+    if (f.size() == 3)
+        for (int k = 0; k < 3; k++) m->force[k] = f[k];
+    auto acc = c.nums("flow/acceleration");
+    if (acc.size() == 3)
+        for (int k = 0; k < 3; k++) m->gravity[k] = acc[k];
+    m->tau_h = c.num("solver/tauPhase", 1.0);
+    m->precision = c.word("solver/precision", "fp64") == "fp32" ? MKG_PREC_FP32 : MKG_PREC_FP64;
+    int32_t bc[6] = {0, 0, 0, 0, 0, 0};
+    if (!read_bc(c, "solver/bcFlow", bc)) return -1;
+    for (int k = 0; k < 6; k++) m->bc[k] = bc[k];
+    if (run_info) {
+        run_info[0] = (int32_t)c.num("output/numLoops", 0);
+        run_info[1] = (int32_t)c.num("output/numFrames", 1);
+        run_info[2] = (int32_t)c.num("phasechange/seed/number", 0);
+        run_info[3] = 0;
+    }
+    const std::string init = c.word("solver/twoPhaseInit", "layered");
+    if (init != "layered" && init != "drops") {
+        g_case_err = "solver/twoPhaseInit is 'layered' or 'drops'";
+        return -1;
+    }
+    if (run_info) run_info[3] = init == "drops" ? 1 : 0;
+    return 0;
+}
+
 }  // extern "C"

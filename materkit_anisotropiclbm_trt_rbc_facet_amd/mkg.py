@@ -213,6 +213,7 @@ def lib() -> C.CDLL:
         "mkg_mp_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mp_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "mkg_mp_sums": (C.c_int, [vp, vp]),
+        "mkg_mp_params_from_xml": (C.c_int, [C.c_char_p, P(MpParams), P(C.c_int32)]),
         "mkg_fvm_create": (C.c_int, [P(MkgParams), P(vp)]),
         "mkg_fvm_destroy": (C.c_int, [vp]),
         "mkg_fvm_set": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
@@ -584,3 +585,12 @@ class MpSolver:
         out = np.zeros(1)
         _check(lib().mkg_mp_sums(self._h, _ptr(out)))
         return float(out[0])
+
+
+def mp_params_from_xml(path) -> tuple:
+    """Two-phase parameters of a case file's <multiphases> block and its run info."""
+    m = MpParams()
+    info = (C.c_int32 * 4)()
+    _check(lib().mkg_mp_params_from_xml(str(path).encode(), C.byref(m), info))
+    return m, {"numLoops": info[0], "numFrames": info[1], "seeds": info[2],
+               "init": "drops" if info[3] else "layered"}

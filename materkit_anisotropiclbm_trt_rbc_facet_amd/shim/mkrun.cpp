@@ -6,11 +6,13 @@
 //
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
-//         rbc | allen_cahn | multi_phase_field | fused
+//         rbc | allen_cahn | multi_phase_field | multiphase | fused
 // (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
-// run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser)
+// run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; multiphase: run_flow_multiphase,
+// MaterKit.cpp:290-331, on mkFluidsMultiphase)
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -68,6 +70,40 @@ static int run_multi_phase_field(const std::string& xml, int maxLoops, int maxFr
     return 0;
 }
 
+// run_flow_multiphase (MaterKit/MaterKit.cpp:290-331) on the two-phase class
+static int run_flow_multiphase(const std::string& xml, int maxLoops, int maxFrame, bool vtk) {
+    mkSpace space(xml);
+    mkFluidsMultiphase flow(space);
+    flow.initialize(xml);
+    flow.initialize_geometry();
+    std::printf("%20s= %d \n%20s= %d \n\nframe \t\t\t loops\n", "maxLoops ", maxLoops,
+                "maxFrame ", maxFrame);
+    const int every = std::max(1, maxLoops / maxFrame);
+    const mkg_mp_params& m = flow.mp_params();
+    const size_t N = (size_t)m.nx * m.ny * m.nz;
+    std::vector<double> u(3 * N);
+    int frame = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int loops = 0; loops <= maxLoops; loops++) {
+        if (loops % every == 0) {
+            double s = 0, umax = 0;
+            mkg_mp_sums(flow.mp(), &s);
+            mkg_mp_get(flow.mp(), MKG_MP_VEL, u.data(), u.size());
+            for (size_t k = 0; k < N; k++) umax = std::max(umax, std::fabs(u[3 * k]));
+            std::printf("%05d\t%18d\tsum_phi %.10e\tumax_x %.8e\n", frame, loops, s, umax);
+            if (vtk) flow.write_vtkfile(loops);
+            frame++;
+        }
+        if (loops == maxLoops) break;
+        flow.propagate();
+        flow.relaxation();
+    }
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("wall %.3f s, %.1f MLUPS (including frame output)\n", s,
+                (double)N * maxLoops / s / 1e6);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s <case.xml> <loop> [maxLoops] [maxFrame] [vtk]\n", argv[0]);
@@ -82,6 +118,7 @@ int main(int argc, char** argv) {
         const bool vtk = argc > 5 && std::strcmp(argv[5], "vtk") == 0;
         if (maxFrame < 1) maxFrame = 1;
         if (loop == "multi_phase_field") return run_multi_phase_field(xml, maxLoops, maxFrame, vtk);
+        if (loop == "multiphase") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk);
         mkSpace space(xml);
         mkFluidsMRT flow(space);
         flow.mkFluidsMRT::initialize(xml);

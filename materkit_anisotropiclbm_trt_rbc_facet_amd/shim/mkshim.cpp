@@ -335,3 +335,93 @@ void mkFluidsUser::write_vtkfile(int loops) {
         scalar("temperature", T.data());
     }
 }
+
+// ------------------------------------------------------------------ two-phase flow
+mkFluidsMultiphase::~mkFluidsMultiphase() {
+    if (m_) mkg_mp_destroy(m_);
+}
+
+void mkFluidsMultiphase::initialize(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_mp_params_from_xml(xmlfname.c_str(), &mp_, run_info_)) throw_io("initialize");
+    if (m_) mkg_mp_destroy(m_);
+    m_ = nullptr;
+    check(mkg_mp_create(&mp_, &m_));
+}
+
+void mkFluidsMultiphase::initialize_geometry() {
+    if (!m_) throw mkt::mkLogicException("initialize_geometry before initialize");
+    const size_t N = (size_t)mp_.nx * mp_.ny * mp_.nz;
+    std::vector<double> phi(N);
+    const bool drops = run_info_[3] == 1;
+    const double W = mp_.width;
+    if (drops) {
+        std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS);
+        const int n = mkg_seeds_from_xml(xml_.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), nullptr);
+        if (n < 0) throw_io("seeds");
+        for (int x = 0; x < mp_.nx; x++)
+            for (int y = 0; y < mp_.ny; y++)
+                for (int z = 0; z < mp_.nz; z++) {
+                    double best = 1e300;
+                    for (int k = 0; k < n; k++) {
+                        const double dx = x - c[3 * k], dy = y - c[3 * k + 1], dz = z - c[3 * k + 2];
+                        best = std::min(best, std::sqrt(dx * dx + dy * dy + dz * dz) - r[k]);
+                    }
+                    phi[((size_t)x * mp_.ny + y) * mp_.nz + z] = 0.5 - 0.5 * std::tanh(2.0 * best / W);
+                }
+    } else {
+        for (int x = 0; x < mp_.nx; x++)
+            for (int y = 0; y < mp_.ny; y++)
+                for (int z = 0; z < mp_.nz; z++)
+                    phi[((size_t)x * mp_.ny + y) * mp_.nz + z] =
+                        0.5 - 0.5 * std::tanh(2.0 * (y + 0.5 - 0.5 * mp_.ny) / W);
+    }
+    check(mkg_mp_init(m_, phi.data(), nullptr, nullptr));
+}
+
+void mkFluidsMultiphase::advance(int mask) {
+    if (!m_) throw mkt::mkLogicException("relaxation before initialize");
+    if (mask & MKG_FLOW) check(mkg_mp_step(m_, 1));
+    pending_ &= ~mask;
+}
+
+void mkFluidsMultiphase::write_vtkfile(int loops) {
+    const int Mx = mp_.nx, My = mp_.ny, Mz = mp_.nz;
+    const size_t N = (size_t)Mx * My * Mz;
+    std::ostringstream fn;
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "vtk", ec);
+    fn << workspace_ << "vtk/fluid_t" << loops << ".vtk";
+    std::ofstream o(fn.str());
+    if (!o) throw mkt::mkIOException("cannot write " + fn.str());
+    o << "# vtk DataFile Version 3.0\nfluid_state\nASCII\nDATASET RECTILINEAR_GRID\n";
+    o << "DIMENSIONS " << Mx << " " << My << " " << Mz << "\n";
+    auto coords = [&](const char* nm, int m) {
+        o << nm << " " << m << " float\n";
+        for (int i = 0; i < m; i++) o << i + 0.5 << " ";
+        o << "\n";
+    };
+    coords("X_COORDINATES", Mx);
+    coords("Y_COORDINATES", My);
+    coords("Z_COORDINATES", Mz);
+    o << "POINT_DATA " << N << "\n" << std::setprecision(9);
+    std::vector<double> phi(N), p(N), u(3 * N);
+    check(mkg_mp_get(m_, MKG_MP_PHI, phi.data(), N));
+    check(mkg_mp_get(m_, MKG_MP_PRESSURE, p.data(), N));
+    check(mkg_mp_get(m_, MKG_MP_VEL, u.data(), 3 * N));
+    auto scalar = [&](const char* name, const std::vector<double>& v) {
+        o << "SCALARS " << name << " float 1\nLOOKUP_TABLE default\n";
+        for (int z = 0; z < Mz; z++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++) o << v[((size_t)x * My + y) * Mz + z] << "\n";
+    };
+    scalar("order_parameter", phi);
+    scalar("pressure", p);
+    o << "VECTORS velocity_vector float\n";
+    for (int z = 0; z < Mz; z++)
+        for (int y = 0; y < My; y++)
+            for (int x = 0; x < Mx; x++) {
+                const size_t id = ((size_t)x * My + y) * Mz + z;
+                o << u[3 * id] << " " << u[3 * id + 1] << " " << u[3 * id + 2] << "\n";
+            }
+}

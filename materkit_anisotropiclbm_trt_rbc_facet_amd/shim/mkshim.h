@@ -199,5 +199,34 @@ private:
     mkg_mpf_params mp_{};
 };
 
+// Phase-field two-phase flow (MaterKit/inc/mkFluidsMultiphase.h:20-99) on an mkg_mp handle:
+// parameters from the case file's <multiphases> block (mkg_mp_params_from_xml).  The
+// geometry is a layered channel (the dense fluid in the lower half along y), or with
+// <solver><twoPhaseInit> drops </twoPhaseInit> one drop of the dense fluid per seed.  The
+// reference ships this class's bodies only in compiled form, so the start geometry is this
+// library's own choice.
+// relaxation() advances both lattices by one step.
+class mkFluidsMultiphase : public mkFluidsMRT {
+public:
+    mkFluidsMultiphase() = default;
+    explicit mkFluidsMultiphase(const mkSpace& space) : mkFluidsLBM(space), mkFluidsMRT(space) {}
+    ~mkFluidsMultiphase() override;
+    void initialize(const std::string& xmlfname) override;
+    void initialize_geometry() override;
+    void propagate() override { pending_ |= MKG_FLOW; }
+    void relaxation() override { advance(MKG_FLOW); }
+    void relaxation_phi() {}  // the Cahn-Hilliard lattice steps with the flow
+    void write_vtkfile(int loops) override;
+    mkg_mp_handle mp() const { return m_; }
+    const mkg_mp_params& mp_params() const { return mp_; }
+
+protected:
+    void advance(int mask) override;
+
+private:
+    mkg_mp_handle m_ = nullptr;
+    mkg_mp_params mp_{};
+};
+
 // readParameters of the reference (MaterKit/inc/mkFunction.h:62): loops and frames.
 void readParameters(int& maxLoops, int& maxFrame, const std::string& xmlFileName);

@@ -334,3 +334,25 @@ def test_reference_api_driver_runs_the_allen_cahn_loop(tmp_path):
     assert (d.get(mkg.F_FLAG) == 0).all()
     assert abs(d.reduce(mkg.R_PHI)[0] - s0) < 1e-9 * case.params.ncell
     d.close()
+
+
+def test_reference_api_driver_runs_the_two_phase_loop(tmp_path):
+    """mkrun multiphase: run_flow_multiphase's call order (propagate, relaxation) on
+    mkFluidsMultiphase over the layered channel case; phi is conserved, the body force drives
+    a growing flow along x, and the frames hold phi, p and u."""
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    res = subprocess.run([str(exe), str(ROOT / "cases" / "two_phase_layered.xml"), "multiphase",
+                          "400", "2", "vtk"], capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path)
+    assert res.returncode == 0, res.stderr
+    rows = [l for l in res.stdout.splitlines() if "sum_phi" in l]
+    assert len(rows) == 3
+    s = [float(l.split("sum_phi")[1].split()[0]) for l in rows]
+    u = [float(l.split("umax_x")[1].split()[0]) for l in rows]
+    assert abs(s[-1] - s[0]) < 1e-9 * 400
+    assert abs(s[0] - 200.0) < 1.0  # the dense half of 4 x 100 cells
+    # at rest the velocity is the half-step force term F / 2 rho, at most 5e-7 here
+    assert u[0] <= 5e-7 + 1e-15 and 10 * u[0] < u[1] < u[2]
+    text = (tmp_path / "vtk" / "fluid_t400.vtk").read_text().split("\n")
+    names = [l.split()[1] for l in text if l.startswith(("SCALARS", "VECTORS"))]
+    assert names == ["order_parameter", "pressure", "velocity_vector"]

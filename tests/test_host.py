@@ -162,3 +162,39 @@ def test_missing_or_malformed_case_file_is_an_error(tmp_path):
     nomesh.write_text("<MaterKit><numerics/></MaterKit>")
     with pytest.raises(mkg.MkgError, match="MeshXYZ"):
         mkg.params_from_xml(nomesh)
+
+
+def test_two_phase_parameters_from_case_file(tmp_path):
+    """<multiphases> of a case file: densities relative to the rare fluid, kinematic viscosity
+    and mobility times mk_dt/mk_dx^2, faces from solver/bcFlow."""
+    m, info = mkg.mp_params_from_xml(ROOT / "cases" / "two_phase_layered.xml")
+    assert (m.nx, m.ny, m.nz) == (4, 100, 1)
+    assert (m.rho_l, m.rho_g, m.nu_l, m.nu_g) == (3.0, 1.0, 0.1, 0.05)
+    assert (m.width, m.mobility, m.sigma, m.tau_h) == (4.0, 0.02, 0.001, 1.0)
+    assert list(m.force) == [1e-6, 0.0, 0.0] and list(m.gravity) == [0.0, 0.0, 0.0]
+    assert list(m.bc) == [mkg.BC_PERIODIC, mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_BOUNCE,
+                          mkg.BC_PERIODIC, mkg.BC_PERIODIC]
+    assert info["init"] == "layered" and info["numLoops"] > 0
+    text = (ROOT / "cases" / "two_phase_layered.xml").read_text()
+    scaled = tmp_path / "scaled.xml"
+    scaled.write_text(text.replace("<solver>", "<solver><twoPhaseInit> drops </twoPhaseInit>")
+                      .replace("<mk_dx> 1.0 </mk_dx>", "<mk_dx> 2.0 </mk_dx>"))
+    m2, info2 = mkg.mp_params_from_xml(scaled)
+    assert info2["init"] == "drops"
+    assert abs(m2.nu_l - 0.1 / 4) < 1e-15 and abs(m2.mobility - 0.02 / 4) < 1e-15
+    bad = tmp_path / "bad.xml"
+    bad.write_text(text.replace("<solver>", "<solver><twoPhaseInit> foam </twoPhaseInit>"))
+    with pytest.raises(mkg.MkgError, match="twoPhaseInit"):
+        mkg.mp_params_from_xml(bad)
+    none = tmp_path / "none.xml"
+    none.write_text("<MaterKit><geometry><MeshXYZ> 4 4 1 </MeshXYZ></geometry></MaterKit>")
+    with pytest.raises(mkg.MkgError, match="multiphases"):
+        mkg.mp_params_from_xml(none)
+
+
+@pytest.mark.skipif(not REF.exists(), reason="reference tree not mounted")
+def test_reference_two_phase_case_parses():
+    m, info = mkg.mp_params_from_xml(REF / "BoltzTechs - multiPhaseFlow.xml")
+    assert (m.nx, m.ny, m.nz) == (5, 101, 1)
+    assert (m.rho_l, m.rho_g, m.width, m.sigma) == (1000.0, 1.0, 5.0, 0.001)
+    assert list(m.force) == [1e-4, 0.0, 0.0] and info["seeds"] == 2
