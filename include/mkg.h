@@ -343,7 +343,13 @@ int mkg_fvm_sums(mkg_fvm_handle h, double* out);
  * beta = 12 sigma/W, kappa = 1.5 sigma W, eta; mkFluidsMultiphase.h:81-97), with BGK
  * collisions: mu = 4 beta phi (phi-1)(phi-1/2) - kappa lap phi, surface force mu grad phi,
  * rho = rho_g + phi (rho_l - rho_g), nu = nu_g + phi (nu_l - nu_g), mobility
- * M = eta cs^2 (tau_h - 1/2).  Faces: periodic or bounce (no-slip wall, zero flux). */
+ * M = eta cs^2 (tau_h - 1/2).  Faces: periodic or bounce (no-slip wall, zero flux).
+ * Wetting (initialize_wetting, mkFluidsMultiphase.h:35; theta, :95): a bounce face whose bit
+ * is set in wet_faces carries the contact angle theta (degrees, measured in the phi = 1
+ * fluid) through the cubic surface energy -sigma cos(theta) phi^2 (3 - 2 phi), i.e.
+ * n.grad phi = (4/W) cos(theta) phi (1 - phi) along the outward normal n; the phi stencils
+ * take the ghost value phi_e + (4/W) cos(theta) phi_e (1 - phi_e) across such a face (phi_e
+ * the edge cell's value).  wet_faces = 0 leaves every wall neutral (90 degrees). */
 typedef struct mkg_mp_params {
     int32_t nx, ny, nz, precision;
     int32_t bc[6];
@@ -353,6 +359,9 @@ typedef struct mkg_mp_params {
     double mobility, tau_h;   /* M and the Cahn-Hilliard relaxation time (> 1/2) */
     double gravity[3];        /* body acceleration: force rho g */
     double force[3];          /* uniform body-force density */
+    int32_t wet_faces;        /* bit f (order of bc[]): face f is a wetting wall */
+    int32_t reserved;
+    double contact_angle;     /* degrees, in (0, 180), on the wetting faces */
 } mkg_mp_params;
 
 typedef struct mkg_mp_ctx* mkg_mp_handle;
@@ -379,7 +388,8 @@ int mkg_mp_sums(mkg_mp_handle h, double* out);
 /* Two-phase parameters from the <multiphases> block of a case file (run_flow_multiphase's
  * case, MaterKit/BoltzTechs - multiPhaseFlow.xml): densities relative to the rare fluid,
  * kinematic viscosities and mobility scaled by mk_dt/mk_dx^2, W0, sigma, force; the body
- * acceleration from flow/acceleration; faces from solver/bcFlow.  run_info (4, may be NULL):
+ * acceleration from flow/acceleration; faces from solver/bcFlow; multiphases/contactAngle
+ * (default 90) on every bounce face (wet_faces) unless it is 90.  run_info (4, may be NULL):
  * numLoops, numFrames, seed count, start geometry (solver/twoPhaseInit: 0 "layered", the
  * dense fluid in the lower half along y; 1 "drops", one dense drop per seed). */
 int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info);

@@ -524,6 +524,15 @@ int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info
     int32_t bc[6] = {0, 0, 0, 0, 0, 0};
     if (!read_bc(c, "solver/bcFlow", bc)) return -1;
     for (int k = 0; k < 6; k++) m->bc[k] = bc[k];
+    // contactAngle (degrees) makes every wall face a wetting face; 90 leaves them neutral
+    m->contact_angle = c.num("multiphases/contactAngle", 90.0);
+    if (!(m->contact_angle > 0.0 && m->contact_angle < 180.0)) {
+        g_case_err = "multiphases/contactAngle must lie in (0, 180) degrees";
+        return -1;
+    }
+    if (m->contact_angle != 90.0)
+        for (int k = 0; k < 6; k++)
+            if (bc[k] == MKG_BC_BOUNCE) m->wet_faces |= 1 << k;
     if (run_info) {
         run_info[0] = (int32_t)c.num("output/numLoops", 0);
         run_info[1] = (int32_t)c.num("output/numFrames", 1);

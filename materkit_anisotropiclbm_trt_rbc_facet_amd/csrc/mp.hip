@@ -16,6 +16,10 @@
 //              Cahn-Hilliard BGK with h0eq = phi + (w0 - 1) eta mu, hieq = w_i (eta mu +
 //              phi c_i.u/cs^2) and the source (1 - 1/(2 tau_h)) w_i c_i.d(phi u)/cs^2, where
 //              d(phi u) is the change of phi u over the step (kept per cell)
+// Wetting walls (initialize_wetting, mkFluidsMultiphase.h:35; theta, :95): the surface energy
+// -sigma cos(theta) phi^2 (3 - 2 phi) gives n.grad phi = (4/W) cos(theta) phi (1 - phi) along
+// the outward normal, imposed through the ghost value phi_e + (4/W) cos(theta) phi_e (1 - phi_e)
+// one cell across the wall in every phi stencil (mu, grad phi, grad rho).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,11 +40,15 @@ struct MpArgs {
     void* phi;   // N (storage type)
     void* mu;    // N
     void* phiu;  // 3 planes of S: phi u of the last step
+    double wet_k;      // (4/W) cos(theta)
+    uint32_t wet_mask;  // face bits (XLO..ZHI) of the wetting walls
 };
 
-// 19-point stencil of a cell-centred field (periodic faces wrap, others replicate the edge).
+// 19-point stencil of phi (periodic faces wrap, others replicate the edge; across a wetting
+// wall the edge value phi_e becomes phi_e + wet_k phi_e (1 - phi_e), once per such face).
 template <class R>
-__device__ inline void mp_stencil(const StepArgs& a, const Cell& c, const R* v, R (&P)[19]) {
+__device__ inline void mp_stencil(const MpArgs& m, const Cell& c, const R* v, R (&P)[19]) {
+    const StepArgs& a = m.a;
     uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
     uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
     uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
@@ -53,6 +61,17 @@ __device__ inline void mp_stencil(const StepArgs& a, const Cell& c, const R* v, 
     if (np & ZHI) pz[2] = pz[1];
 #pragma unroll
     for (int i = 0; i < 19; i++) P[i] = *slot_at(v, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+    const uint32_t wm = m.wet_mask & c.bits;
+    if (wm) {
+        const R k = (R)m.wet_k;
+#pragma unroll
+        for (int i = 1; i < 19; i++) {
+            const int n = ((cx(i) < 0) & !!(wm & XLO)) + ((cx(i) > 0) & !!(wm & XHI)) +
+                          ((cy(i) < 0) & !!(wm & YLO)) + ((cy(i) > 0) & !!(wm & YHI)) +
+                          ((cz(i) < 0) & !!(wm & ZLO)) + ((cz(i) > 0) & !!(wm & ZHI));
+            if (n) P[i] += (R)n * k * P[i] * ((R)1 - P[i]);
+        }
+    }
 }
 
 template <class R, bool ODD>
@@ -72,7 +91,7 @@ __global__ void __launch_bounds__(256) k_mp_mu(const MpArgs m) {
     Cell c;
     if (!locate(m.a, c)) return;
     R P[19];
-    mp_stencil<R>(m.a, c, (const R*)m.phi, P);
+    mp_stencil<R>(m, c, (const R*)m.phi, P);
     R lap = 0;
 #pragma unroll
     for (int i = 1; i < 19; i++) lap += (R)(6.0 * wgt<19>(i)) * (P[i] - P[0]);
@@ -125,7 +144,7 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     const StepArgs& a = m.a;
     const uint32_t S = a.S;
     R P[19];
-    mp_stencil<R>(a, c, (const R*)m.phi, P);
+    mp_stencil<R>(m, c, (const R*)m.phi, P);
     R g[3];
     mp_gradient<R>(P, g);
     const R ph = P[0];
@@ -185,7 +204,7 @@ __global__ void __launch_bounds__(256) k_mp_macro(const MpArgs m, double* p_out,
     Cell c;
     if (!locate(m.a, c)) return;
     R P[19];
-    mp_stencil<R>(m.a, c, (const R*)m.phi, P);
+    mp_stencil<R>(m, c, (const R*)m.phi, P);
     R g[3];
     mp_gradient<R>(P, g);
     R f[19];
@@ -309,6 +328,8 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     m.phi = h->phi;
     m.mu = h->mu;
     m.phiu = h->phiu;
+    m.wet_mask = (uint32_t)p.wet_faces;
+    m.wet_k = p.wet_faces ? 4.0 / p.width * std::cos(p.contact_angle * (M_PI / 180.0)) : 0.0;
     return m;
 }
 
@@ -347,6 +368,7 @@ int mkg_mp_params_default(mkg_mp_params* m) {
     m->width = 4.0;
     m->mobility = 0.1;
     m->tau_h = 1.0;
+    m->contact_angle = 90.0;
     return 0;
 }
 
@@ -367,6 +389,12 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
     for (int a = 0; a < 3; a++)
         if ((p->bc[2 * a] == 0) != (p->bc[2 * a + 1] == 0))
             return pfail("a periodic axis needs periodic codes on both faces");
+    if (p->wet_faces & ~0x3f) return pfail("wet_faces holds face bits 0..5");
+    for (int f = 0; f < 6; f++)
+        if ((p->wet_faces >> f & 1) && p->bc[f] != MKG_BC_BOUNCE)
+            return pfail("a wetting face must be a bounce (wall) face");
+    if (p->wet_faces && !(p->contact_angle > 0.0 && p->contact_angle < 180.0))
+        return pfail("contact_angle must lie in (0, 180) degrees");
     const double n = (double)p->nx * p->ny * p->nz;
     const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
     if ((n + 65536.0) * esz >= 4294967296.0) return pfail("grid too large for 32-bit offsets");

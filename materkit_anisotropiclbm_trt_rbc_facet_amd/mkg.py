@@ -112,6 +112,7 @@ MP_FIELDS = [
     ("rho_l", C.c_double), ("rho_g", C.c_double), ("nu_l", C.c_double), ("nu_g", C.c_double),
     ("sigma", C.c_double), ("width", C.c_double), ("mobility", C.c_double),
     ("tau_h", C.c_double), ("gravity", C.c_double * 3), ("force", C.c_double * 3),
+    ("wet_faces", C.c_int32), ("reserved", C.c_int32), ("contact_angle", C.c_double),
 ]
 
 
@@ -132,6 +133,7 @@ def mp_params(**kw) -> "MpParams":
     m.nx = m.ny = m.nz = 1
     m.rho_l, m.rho_g, m.nu_l, m.nu_g = 1.0, 1.0, 0.1, 0.1
     m.sigma, m.width, m.mobility, m.tau_h = 0.01, 4.0, 0.1, 1.0
+    m.contact_angle = 90.0
     MkgParams.set(m, **kw)
     return m
 

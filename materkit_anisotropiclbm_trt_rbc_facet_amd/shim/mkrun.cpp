@@ -6,10 +6,11 @@
 //
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
-//         rbc | allen_cahn | multi_phase_field | multiphase | fused
+//         rbc | allen_cahn | multi_phase_field | multiphase | wetting | fused
 // (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
 // run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; multiphase: run_flow_multiphase,
-// MaterKit.cpp:290-331, on mkFluidsMultiphase)
+// MaterKit.cpp:290-331, and wetting: run_flow_multiphase_wetting, :333-375, on
+// mkFluidsMultiphase)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -71,11 +72,16 @@ static int run_multi_phase_field(const std::string& xml, int maxLoops, int maxFr
 }
 
 // run_flow_multiphase (MaterKit/MaterKit.cpp:290-331) on the two-phase class
-static int run_flow_multiphase(const std::string& xml, int maxLoops, int maxFrame, bool vtk) {
+static int run_flow_multiphase(const std::string& xml, int maxLoops, int maxFrame, bool vtk,
+                               bool wetting = false) {
     mkSpace space(xml);
     mkFluidsMultiphase flow(space);
-    flow.initialize(xml);
-    flow.initialize_geometry();
+    if (wetting) {
+        flow.initialize_wetting(xml);
+    } else {
+        flow.initialize(xml);
+        flow.initialize_geometry();
+    }
     std::printf("%20s= %d \n%20s= %d \n\nframe \t\t\t loops\n", "maxLoops ", maxLoops,
                 "maxFrame ", maxFrame);
     const int every = std::max(1, maxLoops / maxFrame);
@@ -119,6 +125,7 @@ int main(int argc, char** argv) {
         if (maxFrame < 1) maxFrame = 1;
         if (loop == "multi_phase_field") return run_multi_phase_field(xml, maxLoops, maxFrame, vtk);
         if (loop == "multiphase") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk);
+        if (loop == "wetting") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk, true);
         mkSpace space(xml);
         mkFluidsMRT flow(space);
         flow.mkFluidsMRT::initialize(xml);

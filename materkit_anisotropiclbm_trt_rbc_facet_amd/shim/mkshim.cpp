@@ -379,6 +379,32 @@ void mkFluidsMultiphase::initialize_geometry() {
     check(mkg_mp_init(m_, phi.data(), nullptr, nullptr));
 }
 
+void mkFluidsMultiphase::initialize_wetting(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_mp_params_from_xml(xmlfname.c_str(), &mp_, run_info_)) throw_io("initialize_wetting");
+    bool walls = false;
+    for (int f = 0; f < 6; f++) walls |= mp_.bc[f] == MKG_BC_BOUNCE;
+    if (!walls) mp_.bc[2] = mp_.bc[3] = MKG_BC_BOUNCE;
+    mp_.wet_faces = 0;
+    if (mp_.contact_angle != 90.0)
+        for (int f = 0; f < 6; f++)
+            if (mp_.bc[f] == MKG_BC_BOUNCE) mp_.wet_faces |= 1 << f;
+    if (m_) mkg_mp_destroy(m_);
+    m_ = nullptr;
+    check(mkg_mp_create(&mp_, &m_));
+    const double R = std::max(3.0, std::min(mp_.nx, mp_.ny) / 6.0), W = mp_.width;
+    const double cx = 0.5 * (mp_.nx - 1), cz = 0.5 * (mp_.nz - 1);
+    std::vector<double> phi((size_t)mp_.nx * mp_.ny * mp_.nz);
+    for (int x = 0; x < mp_.nx; x++)
+        for (int y = 0; y < mp_.ny; y++)
+            for (int z = 0; z < mp_.nz; z++) {
+                const double dx = x - cx, dy = y + 0.5, dz = mp_.nz > 1 ? z - cz : 0.0;
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                phi[((size_t)x * mp_.ny + y) * mp_.nz + z] = 0.5 - 0.5 * std::tanh(2.0 * (r - R) / W);
+            }
+    check(mkg_mp_init(m_, phi.data(), nullptr, nullptr));
+}
+
 void mkFluidsMultiphase::advance(int mask) {
     if (!m_) throw mkt::mkLogicException("relaxation before initialize");
     if (mask & MKG_FLOW) check(mkg_mp_step(m_, 1));

@@ -213,6 +213,11 @@ public:
     ~mkFluidsMultiphase() override;
     void initialize(const std::string& xmlfname) override;
     void initialize_geometry() override;
+    // The case's parameters with wetting walls (run_flow_multiphase_wetting,
+    // MaterKit/MaterKit.cpp:333-375): the y faces become walls when the case has none, every
+    // wall carries multiphases/contactAngle, and a half-disc (half-sphere in 3D) drop of the
+    // dense fluid of radius min(nx, ny) / 6 sits on the lower y wall at the centre of x (and z).
+    void initialize_wetting(const std::string& xmlfname);
     void propagate() override { pending_ |= MKG_FLOW; }
     void relaxation() override { advance(MKG_FLOW); }
     void relaxation_phi() {}  // the Cahn-Hilliard lattice steps with the flow

@@ -1228,20 +1228,34 @@ int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int ev
  *   gradients and Laplacian: isotropic D3Q19 forms, grad a = sum w_i c_i a(x+c_i)/cs^2,
  *     lap a = 2 sum w_i (a(x+c_i) - a(x))/cs^2; non-periodic faces replicate the edge value
  *   faces: periodic, or bounce (no-slip wall for f, zero flux for h).
+ *   wetting walls (initialize_wetting, mkFluidsMultiphase.h:35, theta :95): surface energy
+ *     -sigma cos(theta) phi^2 (3 - 2 phi) (Young's law with theta in the phi = 1 fluid);
+ *     minimising the free energy gives kappa n.grad phi = 6 sigma cos(theta) phi (1 - phi),
+ *     n the outward normal, so with kappa = 1.5 sigma W the ghost value one cell across the
+ *     wall is phi_e + (4/W) cos(theta) phi_e (1 - phi_e), once per wetting face crossed.
  * One step: phi = sum h; mu; collide both lattices; stream. */
 static void mp_grad_lap(const orc_mp_params* m, const double* a, int X, int Y, int Z,
                         double* grad, double* lap) {
     orc_params g = {0};
     g.nx = m->nx; g.ny = m->ny; g.nz = m->nz;
     int per[3] = {m->bc[0] == 0, m->bc[2] == 0, m->bc[4] == 0};
+    int n[3] = {m->nx, m->ny, m->nz}, P[3] = {X, Y, Z};
     lat_t L = make_lat(19);
     double a0 = a[lin(&g, X, Y, Z)];
+    double kw = m->wet_faces ? 4.0 / m->width * cos(m->contact_angle * (3.14159265358979323846 / 180.0)) : 0.0;
     double gr[3] = {0, 0, 0}, lp = 0;
     for (int i = 1; i < 19; i++) {
         int XX = clampwrap(X + E27[i][0], m->nx, per[0]);
         int YY = clampwrap(Y + E27[i][1], m->ny, per[1]);
         int ZZ = clampwrap(Z + E27[i][2], m->nz, per[2]);
         double v = a[lin(&g, XX, YY, ZZ)];
+        int crossed = 0;
+        for (int d = 0; d < 3; d++) {
+            int q = P[d] + E27[i][d];
+            if (q < 0 && (m->wet_faces >> (2 * d) & 1)) crossed++;
+            if (q >= n[d] && (m->wet_faces >> (2 * d + 1) & 1)) crossed++;
+        }
+        if (crossed) v += crossed * kw * v * (1.0 - v);
         for (int d = 0; d < 3; d++) gr[d] += L.w[i] * E27[i][d] * v / L.cs2;
         lp += 2.0 * L.w[i] * (v - a0) / L.cs2;
     }

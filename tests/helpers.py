@@ -1,6 +1,8 @@
 """Shared helpers of the test-suite: random initial states, oracle/device pairs."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
@@ -102,3 +104,46 @@ def compare_pops(dev: mkg.Solver, orc: O.State):
         got = dev.get(mkg.F_PHI)
         out["phi"] = float(np.abs(got - orc.phi).max())
     return out
+
+
+def _contour(ph, ymin=2):
+    """phi = 1/2 crossings along columns and rows (rows from ymin up, off the wall layer)."""
+    nx, ny = ph.shape
+    pts = []
+    for x in range(nx):
+        c = ph[x]
+        for j in range(ymin, ny - 1):
+            if (c[j] - 0.5) * (c[j + 1] - 0.5) < 0:
+                pts.append((x, j + (c[j] - 0.5) / (c[j] - c[j + 1])))
+    for y in range(ymin, ny):
+        r = ph[:, y]
+        for i in range(nx - 1):
+            if (r[i] - 0.5) * (r[i + 1] - 0.5) < 0:
+                pts.append((i + (r[i] - 0.5) / (r[i] - r[i + 1]), y))
+    return np.array(pts)
+
+
+def contact_angle(ph):
+    """Contact angle (degrees, in the phi = 1 fluid) of a 2-D drop on the wall at y = -1/2:
+    algebraic circle fit of the contour, cos(theta) = -(height of the centre above the wall)/R."""
+    p = _contour(ph)
+    x, y = p[:, 0], p[:, 1]
+    (a, c, d), *_ = np.linalg.lstsq(np.c_[2 * x, 2 * y, np.ones_like(x)], x * x + y * y,
+                                    rcond=None)
+    R = math.sqrt(d + a * a + c * c)
+    return math.degrees(math.acos(-(c + 0.5) / R)), a
+
+
+def wetting_drop(theta, prec=None, nx=100, ny=40, R=12.0, W=4.0):
+    """A half-disc drop of the dense fluid on the lower y wall (wetting, contact angle theta),
+    neutral upper wall, periodic x."""
+    m = mkg.mp_params(nx=nx, ny=ny, nz=1, rho_l=1.0, rho_g=1.0, nu_l=0.1, nu_g=0.1,
+                      sigma=0.05, width=W, mobility=0.1, tau_h=1.0)
+    if prec is not None:
+        m.precision = prec
+    m.bc[2] = m.bc[3] = mkg.BC_BOUNCE
+    m.wet_faces = 1 << 2
+    m.contact_angle = theta
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    r = np.sqrt((X - (nx - 1) / 2) ** 2 + (Y + 0.5) ** 2)
+    return m, (0.5 - 0.5 * np.tanh(2 * (r - R) / W)).reshape(-1)

@@ -9,7 +9,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import compare_pops, load_oracle
+from helpers import compare_pops, contact_angle, load_oracle
 from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
 from oracle import oracle as O
 from test_oracle_closed_forms import duct_series
@@ -356,3 +356,23 @@ def test_reference_api_driver_runs_the_two_phase_loop(tmp_path):
     text = (tmp_path / "vtk" / "fluid_t400.vtk").read_text().split("\n")
     names = [l.split()[1] for l in text if l.startswith(("SCALARS", "VECTORS"))]
     assert names == ["order_parameter", "pressure", "velocity_vector"]
+
+
+def test_reference_api_driver_runs_the_wetting_loop(tmp_path):
+    """mkrun wetting: run_flow_multiphase_wetting's order (initialize_wetting, then propagate
+    and relaxation) on the 60-degree case; the drop's contact angle in the last frame is
+    within 3 degrees of 60 and phi is conserved."""
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    res = subprocess.run([str(exe), str(ROOT / "cases" / "two_phase_wetting.xml"), "wetting",
+                          "6000", "3", "vtk"], capture_output=True, text=True, timeout=120,
+                         cwd=tmp_path)
+    assert res.returncode == 0, res.stderr
+    s = [float(l.split("sum_phi")[1].split()[0]) for l in res.stdout.splitlines()
+         if "sum_phi" in l]
+    assert len(s) == 4 and abs(s[-1] - s[0]) < 1e-5 * s[0]
+    lines = (tmp_path / "vtk" / "fluid_t6000.vtk").read_text().split("\n")
+    k = lines.index("SCALARS order_parameter float 1") + 2
+    phi = np.array([float(v) for v in lines[k:k + 120 * 60]]).reshape(60, 120).T
+    angle, xc = contact_angle(phi)
+    assert abs(angle - 60.0) < 3.0, angle
+    assert abs(xc - 59.5) < 0.05

@@ -1,12 +1,14 @@
 """Phase-field two-phase flow on the device (HIP, AA pattern) vs the FP64 oracle (two-lattice
 pull): FP64 to 1e-12 and FP32 to 1e-5 of each lattice's magnitude after the same steps, odd
 and even step counts, periodic faces and no-slip walls, density and viscosity contrast,
-gravity."""
+gravity; wetting walls (contact angle) on faces, edges and corners, and the equilibrium
+contact angle of a drop on the device."""
 import math
 
 import numpy as np
 import pytest
 
+from helpers import contact_angle, wetting_drop
 from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
 from oracle import oracle as O
 
@@ -18,6 +20,11 @@ CASES = {
     "periodic_3d": dict(dims=(12, 10, 9), bc=[P] * 6),
     "walls_y": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P]),
     "walls_xz_2d": dict(dims=(16, 14, 1), bc=[BB, BB, P, P, P, P]),
+    # wetting walls: the drop sits on them, so the ghost term phi (1 - phi) is not small
+    "wet_3d": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, BB, BB], wet=0b011100, angle=50.0,
+                   center=(4.5, 1.0, 1.0)),
+    "wet_corner_2d": dict(dims=(16, 14, 1), bc=[BB, BB, BB, BB, P, P], wet=0b001111,
+                          angle=130.0, center=(1.0, 1.0, 0.0)),
 }
 
 
@@ -30,14 +37,17 @@ def make(name, prec):
     m.force[0] = 1e-6
     for f in range(6):
         m.bc[f] = cfg["bc"][f]
+    m.wet_faces = cfg.get("wet", 0)
+    m.contact_angle = cfg.get("angle", 90.0)
     return m
 
 
-def start(m, seed=6):
+def start(m, seed=6, center=None):
     rng = np.random.default_rng(seed)
     d = (int(m.nx), int(m.ny), int(m.nz))
+    c = center or (d[0] / 2.2, d[1] / 1.9, d[2] / 2.0)
     X, Y, Z = np.meshgrid(*[np.arange(n) for n in d], indexing="ij")
-    r = np.sqrt((X - d[0] / 2.2) ** 2 + (Y - d[1] / 1.9) ** 2 + (Z - d[2] / 2.0) ** 2)
+    r = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2)
     phi = (0.5 + 0.5 * np.tanh(2 * (min(d[0], d[1]) / 4.0 - r) / 4.0)).reshape(-1)
     u = 0.01 * rng.standard_normal((phi.size, 3))
     p = 1e-4 * rng.standard_normal(phi.size)
@@ -50,7 +60,7 @@ def start(m, seed=6):
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_two_phase_matches_oracle(name, prec, tol, steps):
     m = make(name, prec)
-    phi, u, p = start(m)
+    phi, u, p = start(m, center=CASES[name].get("center"))
     print(f"two-phase {name} prec {prec} steps {steps}", flush=True)
     orc = O.MpState.start(make(name, mkg.PREC_FP64), phi, u=u, p=p)
     dev = mkg.MpSolver(m)
@@ -116,3 +126,43 @@ def test_two_phase_rejects_bad_parameters():
     m.tau_h = 0.4
     with pytest.raises(mkg.MkgError, match="tau_h"):
         mkg.MpSolver(m)
+
+
+def test_two_phase_rejects_bad_wetting_parameters():
+    m = make("walls_y", mkg.PREC_FP64)
+    m.wet_faces = 1 << 0  # x faces are periodic
+    with pytest.raises(mkg.MkgError, match="wetting face must be a bounce"):
+        mkg.MpSolver(m)
+    m = make("walls_y", mkg.PREC_FP64)
+    m.wet_faces, m.contact_angle = 1 << 2, 180.0
+    with pytest.raises(mkg.MkgError, match="contact_angle"):
+        mkg.MpSolver(m)
+    m.wet_faces, m.contact_angle = 1 << 6, 60.0
+    with pytest.raises(mkg.MkgError, match="bits 0..5"):
+        mkg.MpSolver(m)
+
+
+@pytest.mark.parametrize("theta", [60.0, 120.0])
+def test_wetting_drop_on_device_settles_at_the_contact_angle(theta):
+    """The oracle's contact-angle pin on the device: FP64 fields agree with the oracle after
+    4000 steps and the FP32 drop settles within 3 degrees of theta."""
+    print(f"wetting drop theta {theta}", flush=True)
+    m64, phi = wetting_drop(theta, prec=mkg.PREC_FP64)
+    orc = O.MpState.start(m64, phi)
+    orc.step(4000)
+    ref = orc.macro()["phi"]
+    dev = mkg.MpSolver(m64)
+    dev.init(phi)
+    dev.step(4000)
+    got64 = dev.get(mkg.MP_PHI)
+    dev.close()
+    assert np.abs(got64 - ref).max() < 1e-9
+    m32, _ = wetting_drop(theta, prec=mkg.PREC_FP32)
+    dev = mkg.MpSolver(m32)
+    dev.init(phi)
+    dev.step(4000)
+    got32 = dev.get(mkg.MP_PHI)
+    dev.close()
+    angle, _ = contact_angle(got32.reshape(100, 40))
+    assert abs(angle - theta) < 3.0, angle
+    assert abs(angle - contact_angle(ref.reshape(100, 40))[0]) < 0.05

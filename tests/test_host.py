@@ -182,7 +182,13 @@ def test_two_phase_parameters_from_case_file(tmp_path):
     m2, info2 = mkg.mp_params_from_xml(scaled)
     assert info2["init"] == "drops"
     assert abs(m2.nu_l - 0.1 / 4) < 1e-15 and abs(m2.mobility - 0.02 / 4) < 1e-15
+    assert m.wet_faces == 0 and m.contact_angle == 90.0
+    w, _ = mkg.mp_params_from_xml(ROOT / "cases" / "two_phase_wetting.xml")
+    assert w.contact_angle == 60.0 and w.wet_faces == (1 << 2) | (1 << 3)  # the two y walls
     bad = tmp_path / "bad.xml"
+    bad.write_text(text.replace("<sigma>", "<contactAngle> 180 </contactAngle><sigma>"))
+    with pytest.raises(mkg.MkgError, match="contactAngle"):
+        mkg.mp_params_from_xml(bad)
     bad.write_text(text.replace("<solver>", "<solver><twoPhaseInit> foam </twoPhaseInit>"))
     with pytest.raises(mkg.MkgError, match="twoPhaseInit"):
         mkg.mp_params_from_xml(bad)
@@ -198,3 +204,5 @@ def test_reference_two_phase_case_parses():
     assert (m.nx, m.ny, m.nz) == (5, 101, 1)
     assert (m.rho_l, m.rho_g, m.width, m.sigma) == (1000.0, 1.0, 5.0, 0.001)
     assert list(m.force) == [1e-4, 0.0, 0.0] and info["seeds"] == 2
+    w, _ = mkg.mp_params_from_xml(REF / "BoltzTechs - wetting.xml")
+    assert (w.nx, w.ny, w.contact_angle, w.wet_faces) == (128, 64, 90.0, 0)

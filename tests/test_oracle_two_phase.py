@@ -9,7 +9,11 @@ model; MaterKit/inc/mkFluidsMultiphase.h:20-99):
 - a layered channel flow of two fluids with viscosity ratio 3, driven by a body force between
   no-slip walls, follows the piecewise-parabolic sharp-interface profile within 3 % of the
   peak velocity;
-- a drop three times denser than its surroundings translates with a uniform stream.
+- a drop three times denser than its surroundings translates with a uniform stream;
+- wetting walls: a 2-D drop on a wall of contact angle 60 or 120 degrees settles at that angle
+  within 3 degrees (circle fit of the phi = 1/2 contour); a flipped sign of the wall term
+  gives 120 for 60, a factor 2/W instead of 4/W gives 75.5 for 60; a neutral wall (90
+  degrees) keeps a half-disc drop in place.
 Validated range: density ratios up to 10 for static drops, 3 for moving interfaces (BGK
 collisions; larger ratios in motion show growing velocity errors).
 """
@@ -18,6 +22,7 @@ import math
 import numpy as np
 import pytest
 
+from helpers import contact_angle, wetting_drop
 from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
 from oracle import oracle as O
 
@@ -132,3 +137,23 @@ def test_drop_with_density_contrast_translates_with_the_flow():
     u = s.macro()["u"]
     assert np.abs(u[:, 0] - U).max() < 0.06 * U
     assert np.abs(u[:, 1]).max() < 0.06 * U
+
+
+@pytest.mark.parametrize("theta", [60.0, 120.0])
+def test_wetting_drop_settles_at_the_contact_angle(theta):
+    m, phi = wetting_drop(theta)
+    s = O.MpState.start(m, phi)
+    s.step(4000)
+    ph = s.macro()["phi"]
+    got, xc = contact_angle(ph.reshape(100, 40))
+    assert abs(got - theta) < 3.0, got
+    assert abs(xc - 49.5) < 1e-6  # the drop stays symmetric about its centre
+    assert abs(ph.sum() - phi.sum()) < 1e-10 * phi.size
+
+
+def test_neutral_wall_keeps_a_half_disc():
+    m, phi = wetting_drop(90.0)
+    s = O.MpState.start(m, phi)
+    s.step(2000)
+    got, _ = contact_angle(s.macro()["phi"].reshape(100, 40))
+    assert abs(got - 90.0) < 0.1
