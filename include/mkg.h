@@ -360,9 +360,16 @@ typedef struct mkg_mp_params {
     double gravity[3];        /* body acceleration: force rho g */
     double force[3];          /* uniform body-force density */
     int32_t wet_faces;        /* bit f (order of bc[]): face f is a wetting wall */
-    int32_t reserved;
+    int32_t visc_model;       /* MKG_MP_VISC_*: viscosity across the interface */
     double contact_angle;     /* degrees, in (0, 180), on the wetting faces */
 } mkg_mp_params;
+
+/* Viscosity across the interface, at phi clamped to [0, 1]: nu = nu_g + phi (nu_l - nu_g);
+ * mu = rho nu linear in phi; or 1/mu = phi/mu_l + (1 - phi)/mu_g (continuous shear stress
+ * across a layered interface, the closest to the sharp-interface profile at large ratios). */
+#define MKG_MP_VISC_NU_LINEAR 0
+#define MKG_MP_VISC_MU_LINEAR 1
+#define MKG_MP_VISC_MU_HARMONIC 2
 
 typedef struct mkg_mp_ctx* mkg_mp_handle;
 #define MKG_MP_POP_FLOW 0   /* 19 x N flow populations (canonical, doubles) */
@@ -389,7 +396,8 @@ int mkg_mp_sums(mkg_mp_handle h, double* out);
  * case, MaterKit/BoltzTechs - multiPhaseFlow.xml): densities relative to the rare fluid,
  * kinematic viscosities and mobility scaled by mk_dt/mk_dx^2, W0, sigma, force; the body
  * acceleration from flow/acceleration; faces from solver/bcFlow; multiphases/contactAngle
- * (default 90) on every bounce face (wet_faces) unless it is 90.  run_info (4, may be NULL):
+ * (default 90) on every bounce face (wet_faces) unless it is 90; solver/viscosityModel
+ * (nu-linear, mu-linear, mu-harmonic; default nu-linear).  run_info (4, may be NULL):
  * numLoops, numFrames, seed count, start geometry (solver/twoPhaseInit: 0 "layered", the
  * dense fluid in the lower half along y; 1 "drops", one dense drop per seed). */
 int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info);

@@ -533,6 +533,14 @@ int mkg_mp_params_from_xml(const char* path, mkg_mp_params* m, int32_t* run_info
     if (m->contact_angle != 90.0)
         for (int k = 0; k < 6; k++)
             if (bc[k] == MKG_BC_BOUNCE) m->wet_faces |= 1 << k;
+    const std::string vm = c.word("solver/viscosityModel", "nu-linear");
+    if (vm == "nu-linear") m->visc_model = MKG_MP_VISC_NU_LINEAR;
+    else if (vm == "mu-linear") m->visc_model = MKG_MP_VISC_MU_LINEAR;
+    else if (vm == "mu-harmonic") m->visc_model = MKG_MP_VISC_MU_HARMONIC;
+    else {
+        g_case_err = "solver/viscosityModel is nu-linear, mu-linear or mu-harmonic";
+        return -1;
+    }
     if (run_info) {
         run_info[0] = (int32_t)c.num("output/numLoops", 0);
         run_info[1] = (int32_t)c.num("output/numFrames", 1);

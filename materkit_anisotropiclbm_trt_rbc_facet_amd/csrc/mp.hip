@@ -12,7 +12,7 @@
 //              force; rho u = sum c f + F/2; p = cs^2/(1 - w0) [sum_{i>0} f + u.grad rho/2 +
 //              rho s_0(u)]; flow BGK with tau_f = nu(phi)/cs^2 + 1/2 and the forcing
 //              (1 - 1/(2 tau_f)) w_i [c_i.F + (c_i.u)(c_i.grad rho)]/cs^2 (its zeroth moment
-//              u.grad rho keeps div u = 0 where rho varies);
+//              u.grad rho keeps div u = 0 where rho varies); nu(phi) per visc_model;
 //              Cahn-Hilliard BGK with h0eq = phi + (w0 - 1) eta mu, hieq = w_i (eta mu +
 //              phi c_i.u/cs^2) and the source (1 - 1/(2 tau_h)) w_i c_i.d(phi u)/cs^2, where
 //              d(phi u) is the change of phi u over the step (kept per cell)
@@ -42,7 +42,19 @@ struct MpArgs {
     void* phiu;  // 3 planes of S: phi u of the last step
     double wet_k;      // (4/W) cos(theta)
     uint32_t wet_mask;  // face bits (XLO..ZHI) of the wetting walls
+    int32_t visc;       // MKG_MP_VISC_*
 };
+
+// Kinematic viscosity at a clamped phi (MKG_MP_VISC_* of mkg.h).
+template <class R>
+__device__ inline R mp_nu(const MpArgs& m, R pc) {
+    if (m.visc == MKG_MP_VISC_NU_LINEAR) return (R)m.nu_g + pc * (R)(m.nu_l - m.nu_g);
+    const R rho = (R)m.rho_g + pc * (R)(m.rho_l - m.rho_g);
+    const R ml = (R)(m.rho_l * m.nu_l), mg = (R)(m.rho_g * m.nu_g);
+    const R mu = m.visc == MKG_MP_VISC_MU_LINEAR ? mg + pc * (ml - mg)
+                                                 : (R)1 / (pc / ml + ((R)1 - pc) / mg);
+    return mu / rho;
+}
 
 // 19-point stencil of phi (periodic faces wrap, others replicate the edge; across a wetting
 // wall the edge value phi_e becomes phi_e + wet_k phi_e (1 - phi_e), once per such face).
@@ -154,7 +166,7 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     R rho, u[3], p, F[3], gr[3];
     mp_macro<R>(m, f, ph, muc, g, rho, u, p, F, gr);
     const R pc = ph < (R)0 ? (R)0 : (ph > (R)1 ? (R)1 : ph);
-    const R nu = (R)m.nu_g + pc * (R)(m.nu_l - m.nu_g);
+    const R nu = mp_nu<R>(m, pc);
     const R itf = (R)1 / ((R)3 * nu + (R)0.5);  // 1 / tau_f
     const R kf = (R)1 - (R)0.5 * itf;
     const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
@@ -329,6 +341,7 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     m.mu = h->mu;
     m.phiu = h->phiu;
     m.wet_mask = (uint32_t)p.wet_faces;
+    m.visc = p.visc_model;
     m.wet_k = p.wet_faces ? 4.0 / p.width * std::cos(p.contact_angle * (M_PI / 180.0)) : 0.0;
     return m;
 }
@@ -390,6 +403,8 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
         if ((p->bc[2 * a] == 0) != (p->bc[2 * a + 1] == 0))
             return pfail("a periodic axis needs periodic codes on both faces");
     if (p->wet_faces & ~0x3f) return pfail("wet_faces holds face bits 0..5");
+    if (p->visc_model < MKG_MP_VISC_NU_LINEAR || p->visc_model > MKG_MP_VISC_MU_HARMONIC)
+        return pfail("visc_model must be one of MKG_MP_VISC_*");
     for (int f = 0; f < 6; f++)
         if ((p->wet_faces >> f & 1) && p->bc[f] != MKG_BC_BOUNCE)
             return pfail("a wetting face must be a bounce (wall) face");

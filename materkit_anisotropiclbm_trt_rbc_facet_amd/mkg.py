@@ -91,6 +91,7 @@ class MkgParams(C.Structure):
 
 FVM_PHI, FVM_CONC, FVM_U = 0, 1, 2
 MP_POP_FLOW, MP_POP_PHASE, MP_PHI, MP_MU, MP_PRESSURE, MP_VEL, MP_PHIU = range(7)
+MP_VISC_NU_LINEAR, MP_VISC_MU_LINEAR, MP_VISC_MU_HARMONIC = range(3)
 MPF_MAX = 4
 MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG = range(5)
 
@@ -112,7 +113,7 @@ MP_FIELDS = [
     ("rho_l", C.c_double), ("rho_g", C.c_double), ("nu_l", C.c_double), ("nu_g", C.c_double),
     ("sigma", C.c_double), ("width", C.c_double), ("mobility", C.c_double),
     ("tau_h", C.c_double), ("gravity", C.c_double * 3), ("force", C.c_double * 3),
-    ("wet_faces", C.c_int32), ("reserved", C.c_int32), ("contact_angle", C.c_double),
+    ("wet_faces", C.c_int32), ("visc_model", C.c_int32), ("contact_angle", C.c_double),
 ]
 
 

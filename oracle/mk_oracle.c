@@ -1223,7 +1223,8 @@ int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int ev
  *     rho u = sum c f + F/2,  p = cs^2/(1 - w0) [sum_{i>0} f_i + u.grad rho/2 + rho s_0(u)],
  *     F = mu grad phi + rho g + force;  forcing (1 - 1/(2 tau_f)) w_i [c_i.F +
  *     (c_i.u)(c_i.grad rho)]/cs^2, whose zeroth moment u.grad rho keeps div u = 0 where rho
- *     varies (the u.grad rho/2 of p is its half-step part);  nu = nu_g + phi (nu_l - nu_g),
+ *     varies (the u.grad rho/2 of p is its half-step part);  nu (mp_nu) = nu_g + phi (nu_l - nu_g),
+ *     or from mu = rho nu linear or harmonic in phi,
  *     tau_f = nu/cs^2 + 1/2
  *   gradients and Laplacian: isotropic D3Q19 forms, grad a = sum w_i c_i a(x+c_i)/cs^2,
  *     lap a = 2 sum w_i (a(x+c_i) - a(x))/cs^2; non-periodic faces replicate the edge value
@@ -1285,6 +1286,16 @@ static void mp_phi_mu(const orc_mp_params* m, const double* h, double* phi, doub
             }
 }
 
+/* Kinematic viscosity at a clamped phi: linear in nu (visc_model 0), linear in the dynamic
+ * viscosity mu = rho nu (1), or harmonic in mu, 1/mu = phi/mu_l + (1 - phi)/mu_g (2). */
+static double mp_nu(const orc_mp_params* m, double pc) {
+    if (m->visc_model == 0) return m->nu_g + pc * (m->nu_l - m->nu_g);
+    double rho = m->rho_g + pc * (m->rho_l - m->rho_g);
+    double ml = m->rho_l * m->nu_l, mg = m->rho_g * m->nu_g;
+    double mu = m->visc_model == 1 ? mg + pc * (ml - mg) : 1.0 / (pc / ml + (1.0 - pc) / mg);
+    return mu / rho;
+}
+
 /* Macroscopic fields of a cell from its pre-collision f (full values), phi and mu. */
 static void mp_macro_cell(const orc_mp_params* m, const double* f, double ph, double muc,
                           const double* gphi, double* rho_o, double* u, double* p_o, double* F) {
@@ -1341,7 +1352,7 @@ int orc_mp_step(const orc_mp_params* m, double* f, double* h, double* phiu, int 
                     double grho[3];
                     for (int d = 0; d < 3; d++) grho[d] = (m->rho_l - m->rho_g) * gphi[d];
                     double pc = ph < 0 ? 0 : (ph > 1 ? 1 : ph);
-                    double nu = m->nu_g + pc * (m->nu_l - m->nu_g);
+                    double nu = mp_nu(m, pc);
                     double tf = nu / L.cs2 + 0.5;
                     double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
                     for (int i = 0; i < 19; i++) {

@@ -139,7 +139,8 @@ typedef struct orc_mp_params {
     double sigma, width;           /* surface tension, interface width W */
     double mobility, tau_h;        /* M and the Cahn-Hilliard relaxation time */
     double gravity[3], force[3];   /* body acceleration and body-force density */
-    int32_t wet_faces, reserved;   /* bit f: bounce face f is a wetting wall */
+    int32_t wet_faces;             /* bit f: bounce face f is a wetting wall */
+    int32_t visc_model;            /* 0 linear nu, 1 linear rho nu, 2 harmonic rho nu */
     double contact_angle;          /* degrees, on the wetting faces */
 } orc_mp_params;
 

@@ -2,7 +2,7 @@
 pull): FP64 to 1e-12 and FP32 to 1e-5 of each lattice's magnitude after the same steps, odd
 and even step counts, periodic faces and no-slip walls, density and viscosity contrast,
 gravity; wetting walls (contact angle) on faces, edges and corners, and the equilibrium
-contact angle of a drop on the device."""
+contact angle of a drop on the device; the three viscosity models."""
 import math
 
 import numpy as np
@@ -25,6 +25,9 @@ CASES = {
                    center=(4.5, 1.0, 1.0)),
     "wet_corner_2d": dict(dims=(16, 14, 1), bc=[BB, BB, BB, BB, P, P], wet=0b001111,
                           angle=130.0, center=(1.0, 1.0, 0.0)),
+    # viscosity from mu = rho nu linear or harmonic in phi
+    "visc_mu_linear": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P], visc=1),
+    "visc_mu_harmonic": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P], visc=2),
 }
 
 
@@ -39,6 +42,7 @@ def make(name, prec):
         m.bc[f] = cfg["bc"][f]
     m.wet_faces = cfg.get("wet", 0)
     m.contact_angle = cfg.get("angle", 90.0)
+    m.visc_model = cfg.get("visc", 0)
     return m
 
 
@@ -139,6 +143,9 @@ def test_two_phase_rejects_bad_wetting_parameters():
         mkg.MpSolver(m)
     m.wet_faces, m.contact_angle = 1 << 6, 60.0
     with pytest.raises(mkg.MkgError, match="bits 0..5"):
+        mkg.MpSolver(m)
+    m.wet_faces, m.visc_model = 0, 3
+    with pytest.raises(mkg.MkgError, match="visc_model"):
         mkg.MpSolver(m)
 
 

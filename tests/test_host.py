@@ -2,6 +2,7 @@
 declares, the XML case layer derives the reference-held numbers, seeds and RNG match the
 golden fixtures, and invalid parameter blocks are rejected with a message."""
 import hashlib
+import ctypes as C
 import json
 import math
 import re
@@ -183,6 +184,14 @@ def test_two_phase_parameters_from_case_file(tmp_path):
     assert info2["init"] == "drops"
     assert abs(m2.nu_l - 0.1 / 4) < 1e-15 and abs(m2.mobility - 0.02 / 4) < 1e-15
     assert m.wet_faces == 0 and m.contact_angle == 90.0
+    assert m.visc_model == mkg.MP_VISC_NU_LINEAR
+    assert mkg.lib().mkg_mp_params_size() == C.sizeof(mkg.MpParams)
+    hv = tmp_path / "harmonic.xml"
+    hv.write_text(text.replace("<solver>", "<solver><viscosityModel> mu-harmonic </viscosityModel>"))
+    assert mkg.mp_params_from_xml(hv)[0].visc_model == mkg.MP_VISC_MU_HARMONIC
+    hv.write_text(text.replace("<solver>", "<solver><viscosityModel> cubic </viscosityModel>"))
+    with pytest.raises(mkg.MkgError, match="viscosityModel"):
+        mkg.mp_params_from_xml(hv)
     w, _ = mkg.mp_params_from_xml(ROOT / "cases" / "two_phase_wetting.xml")
     assert w.contact_angle == 60.0 and w.wet_faces == (1 << 2) | (1 << 3)  # the two y walls
     bad = tmp_path / "bad.xml"
