@@ -1,12 +1,14 @@
 // MaterKit-style driver over the reference-API layer: the run_* loops of
 // MaterKit/MaterKit.cpp (run_flow_Poiseuille :244-288, run_flow_solutal_dendrite :399-465,
-// run_flow_thermal_dendrite :467-527, run_thermal_dendrite :743-788) with their per-step call
+// run_flow_thermal_dendrite :467-527, run_solutal_dendrite :633-687, run_solutal_only :689-741,
+// run_thermal_dendrite :743-788) with their per-step call
 // order, plus a fused loop.  The case file and the loop are chosen on the command line
 // (the reference hard-codes choice = 8, MaterKit.cpp:50).
 //
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
-//         rbc | allen_cahn | multi_phase_field | multiphase | wetting | fused
+//         solutal_dendrite | solutal_only | rbc | allen_cahn | multi_phase_field |
+//         multiphase | wetting | fused
 // (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
 // run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; multiphase: run_flow_multiphase,
 // MaterKit.cpp:290-331, and wetting: run_flow_multiphase_wetting, :333-375, on
@@ -171,6 +173,13 @@ int main(int argc, char** argv) {
                 flow.mkSolutalField::propagate();
                 flow.mkSolutalField::relaxation();
                 flow.mkSolutalField::bdy_LBflowin();
+            } else if (loop == "solutal_dendrite") {
+                flow.mkPhaseField::propagate();
+                flow.mkPhaseField::relaxation_anisotropy_solute(loops);
+                flow.mkSolutalField::propagate();
+                flow.mkSolutalField::relaxation();
+            } else if (loop == "solutal_only") {
+                flow.mkSolutalField::evolution_ATC2(flow.node);
             } else if (loop == "rbc") {
                 flow.mkFluidsMRT::propagate();
                 flow.mkFluidsMRT::relaxation();

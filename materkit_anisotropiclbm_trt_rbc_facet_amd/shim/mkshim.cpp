@@ -59,7 +59,59 @@ void readParameters(int& maxLoops, int& maxFrame, const std::string& xmlFileName
 }
 
 mkFluidsLBM::~mkFluidsLBM() {
+    for (mkg_fvm_handle f : fvm_)
+        if (f) mkg_fvm_destroy(f);
     if (h_) mkg_destroy(h_);
+}
+
+void mkFluidsLBM::fvm_evolve(bool anti_trapping) {
+    if (!h_) throw mkt::mkLogicException("evolution before initialize");
+    if (!p_.phase_q || !p_.solute)
+        throw mkt::mkLogicException("the finite-volume updates need the phase and solute fields");
+    const int k = anti_trapping ? 1 : 0;
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    if (fvm_live_ != k) {
+        std::vector<double> phi(N), c(N);
+        if (fvm_live_ >= 0) {  // the other handle holds the newest values
+            check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_PHI, phi.data(), N));
+            check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_CONC, c.data(), N));
+        } else {
+            check(mkg_get_field(h_, MKG_F_PHI, phi.data(), MKG_DT_F64, N));
+            check(mkg_get_field(h_, MKG_F_CONC, c.data(), MKG_DT_F64, N));
+        }
+        if (!fvm_[k]) {
+            mkg_params q = p_;
+            q.ngrains = 0;
+            if (!anti_trapping) q.at_coef = 0.0;
+            check(mkg_fvm_create(&q, &fvm_[k]));
+        }
+        check(mkg_fvm_set(fvm_[k], MKG_FVM_PHI, phi.data(), N));
+        check(mkg_fvm_set(fvm_[k], MKG_FVM_CONC, c.data(), N));
+        fvm_live_ = k;
+    }
+    check(mkg_fvm_step(fvm_[k], 1, fvm_phase_next_ ? 1 : 0));
+    fvm_phase_next_ = false;
+}
+
+void mkFluidsLBM::fvm_flush() const {
+    if (fvm_live_ < 0) return;
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    std::vector<double> phi(N), c(N), rho, u, T;
+    check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_PHI, phi.data(), N));
+    check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_CONC, c.data(), N));
+    if (p_.flow_q) {
+        rho.resize(N);
+        u.resize(3 * N);
+        check(mkg_get_field(h_, MKG_F_RHO, rho.data(), MKG_DT_F64, N));
+        check(mkg_get_field(h_, MKG_F_VEL, u.data(), MKG_DT_F64, 3 * N));
+    }
+    if (p_.thermal) {
+        T.resize(N);
+        check(mkg_get_field(h_, MKG_F_TEMP, T.data(), MKG_DT_F64, N));
+    }
+    check(mkg_init_equilibrium(h_, p_.flow_q ? rho.data() : nullptr, p_.flow_q ? u.data() : nullptr,
+                               p_.thermal ? T.data() : nullptr, c.data(), phi.data()));
+    fvm_live_ = -1;
 }
 
 void mkFluidsLBM::check(int rc) const {
@@ -122,12 +174,14 @@ void mkFluidsLBM::initialize_geometry() {
 
 void mkFluidsLBM::advance(int mask) {
     if (!h_) throw mkt::mkLogicException("relaxation before initialize");
+    fvm_flush();
     check(mkg_step(h_, 1, mask));
     pending_ &= ~mask;
 }
 
 void mkFluidsLBM::step(int nsteps) {
     if (!h_) throw mkt::mkLogicException("step before initialize");
+    fvm_flush();
     check(mkg_step(h_, nsteps, MKG_FLOW | MKG_PHASE | MKG_THERMAL | MKG_SOLUTE));
     pending_ = 0;
 }
@@ -135,11 +189,13 @@ void mkFluidsLBM::step(int nsteps) {
 std::vector<double> mkFluidsLBM::field(int id) const {
     const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
     std::vector<double> v(id == MKG_F_VEL ? 3 * N : N);
+    fvm_flush();
     check(mkg_get_field(h_, id, v.data(), MKG_DT_F64, v.size()));
     return v;
 }
 
 double mkFluidsLBM::Thermal_NusseltRBC() {
+    fvm_flush();
     double v[4];
     check(mkg_reduce(h_, MKG_R_NUSSELT, v));
     return v[0];
@@ -207,6 +263,7 @@ void mkFluidsLBM::write_vtkfile(int loops) {
     if (p_.flow_q) scalar("density_difference", field(MKG_F_RHO), 1.0);
     if (p_.phase_q) {
         std::vector<int16_t> g(N);
+        fvm_flush();
         check(mkg_get_field(h_, MKG_F_GRAIN, g.data(), MKG_DT_I16, N));
         std::vector<double> gd(g.begin(), g.end());
         scalar("grain_id", gd, 0.0);

@@ -47,6 +47,11 @@ private:
     std::string xml_;
 };
 
+// The reference's node record (MaterKit/inc/mkNode.h:134-198).  Node state lives on the device
+// in structure-of-arrays form, so the type is incomplete here: a driver can pass flow.node to
+// the finite-volume calls (evolution_ATC2(flow.node)), but cannot read node[index].
+struct mkNode;
+
 class mkFluidsLBM {
 public:
     mkFluidsLBM() = default;
@@ -83,14 +88,30 @@ public:
     // fused steps of every field
     void step(int nsteps = 1);
     // direct access for user code
-    mkg_handle handle() const { return h_; }
+    mkg_handle handle() const {
+        fvm_flush();
+        return h_;
+    }
     const mkg_params& params() const { return p_; }
     std::vector<double> field(int id) const;
     void set_workspace(const std::string& dir) { workspace_ = dir; }
+    // node array of the reference (MaterKit/inc/mkFluidsLBM.h:95); never dereferenced here
+    mkNode* node = nullptr;
 
 protected:
     virtual void advance(int mask);
     void check(int rc) const;
+    // Finite-volume operators (mkPhaseField::evolution_anisotropy, mkSolutalField::evolution*):
+    // an mkg_fvm handle per anti-trapping choice, created at the first call from the case's
+    // parameters (without grain rotations).  It takes phi and c from the lattices when it
+    // starts and keeps the newest values while only finite-volume calls follow; the next
+    // lattice step or field read hands them back (the lattices restart at the equilibrium of
+    // the current fields).  A phase call marks the next solute call to advance phi with c.
+    void fvm_evolve(bool anti_trapping);
+    void fvm_flush() const;
+    mkg_fvm_handle fvm_[2] = {nullptr, nullptr};
+    mutable int fvm_live_ = -1;  // handle holding the newest phi and c; -1: the lattices
+    bool fvm_phase_next_ = false;
     mkSpace space_;
     std::string xml_;
     std::string workspace_ = "./";
@@ -113,6 +134,9 @@ public:
     explicit mkThermalField(const mkSpace& space) : mkFluidsLBM(space) {}
     void propagate() override { pending_ |= MKG_THERMAL; }
     void relaxation() override { advance(MKG_THERMAL); }
+    // the velocity coupling is read inside every thermal step
+    void relaxation(const mkFluidsLBM& /*flow*/) { advance(MKG_THERMAL); }
+    void update_velocity(const mkFluidsLBM& /*flow*/) {}
     void bdy_RBC() {}
     void bdy_noflux() {}
     void bdy_flowin() {}
@@ -126,6 +150,13 @@ public:
     void propagate() override { pending_ |= MKG_SOLUTE; }
     void propagate_PBB() override { pending_ |= MKG_SOLUTE; }
     void relaxation() override { advance(MKG_SOLUTE); }
+    void relaxation_anisotropy() { advance(MKG_SOLUTE); }
+    // finite-volume solute updates (MaterKit/inc/mkSolutalField.h:41-51): with the
+    // anti-trapping current (ATC1 / ATC2) or without it (evolution)
+    void evolution(const mkNode* /*nptr*/) { fvm_evolve(false); }
+    void evolution_ATC1(const mkNode* /*nptr*/) { fvm_evolve(true); }
+    void evolution_ATC2(const mkNode* /*nptr*/) { fvm_evolve(true); }
+    void updateDCs(const mkNode* /*nptr*/) {}  // D(phi) is evaluated inside every solute update
     void bdy_constX() {}
     void bdy_RBC() {}
     void bdy_LBflowin() {}
@@ -137,8 +168,18 @@ public:
     explicit mkPhaseField(const mkSpace& space) : mkFluidsLBM(space) {}
     void propagate() override { pending_ |= MKG_PHASE; }
     void relaxation() override { advance(MKG_PHASE); }
+    void relaxation_anisotropy() { advance(MKG_PHASE); }
     void relaxation_anisotropy_thermal() { advance(MKG_PHASE); }
+    void relaxation_anisotropy_thermal(const mkFluidsLBM& /*flow*/) { advance(MKG_PHASE); }
     void relaxation_anisotropy_solute(const int /*loops*/) { advance(MKG_PHASE); }
+    // the anisotropy form is the case file's (<solver><anisotropyForm>); these two
+    // check that it is the form their name asks for
+    void relaxation_anisotropy_cbc() { aniso_form(MKG_ANISO_CBC, "cbc"); }
+    void relaxation_anisotropy_hcp() { aniso_form(MKG_ANISO_HCP, "hcp"); }
+    // finite-volume phase update (mkPhaseField.h:41,53): advances phi together with the
+    // solute update that follows it
+    void evolution_anisotropy() { fvm_phase_next_ = true; }
+    void evolution(const mkNode* /*nptr*/) { fvm_phase_next_ = true; }
     // conservative Allen-Cahn interface (MaterKit/inc/mkPhaseField.h:39-40); the case file
     // must select <solver><phaseModel> allenCahn </phaseModel>
     void relaxation_AllenCahn() { allen_cahn(); }
@@ -146,6 +187,12 @@ public:
     void reset_flags() {}  // flags are re-classified inside every phase step
 
 private:
+    void aniso_form(int kind, const char* name) {
+        if (p_.aniso_kind != kind)
+            throw mkt::mkLogicException(std::string("relaxation_anisotropy_") + name +
+                                        ": the case file selects another anisotropy form");
+        advance(MKG_PHASE);
+    }
     void allen_cahn() {
         if (p_.pf_model != MKG_PF_ALLEN_CAHN)
             throw mkt::mkLogicException(

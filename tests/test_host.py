@@ -6,6 +6,7 @@ import ctypes as C
 import json
 import math
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -215,3 +216,84 @@ def test_reference_two_phase_case_parses():
     assert list(m.force) == [1e-4, 0.0, 0.0] and info["seeds"] == 2
     w, _ = mkg.mp_params_from_xml(REF / "BoltzTechs - wetting.xml")
     assert (w.nx, w.ny, w.contact_angle, w.wet_faces) == (128, 64, 90.0, 0)
+
+
+DRIVER_CALLS = r"""
+#include <string>
+#include "mkshim.h"
+// Call forms of the reference's drivers (MaterKit/MaterKit.cpp run_* loops), compiled and
+// linked only: nothing here runs without a device.
+static void loops(mkFluidsMRT& flow, mkFluidsUser& user, mkFluidsMultiphase& two,
+                  const std::string& xml, int loops) {
+    flow.mkFluidsMRT::initialize(xml);
+    flow.mkFluidsMRT::initialize_geometry();
+    flow.initialize_geometry_Poiseuille();
+    flow.mkFluidsLBM::propagate();
+    flow.mkFluidsLBM::relaxation();
+    flow.mkFluidsMRT::propagate_PBB();
+    flow.mkFluidsMRT::relaxation();
+    flow.mkFluidsMRT::bdyVelocity();
+    flow.mkPhaseField::propagate();
+    flow.mkPhaseField::relaxation_anisotropy_thermal();
+    flow.mkPhaseField::relaxation_anisotropy_thermal(flow);
+    flow.mkPhaseField::relaxation_anisotropy_solute(loops);
+    flow.mkPhaseField::relaxation_anisotropy_cbc();
+    flow.mkPhaseField::relaxation_anisotropy_hcp();
+    flow.mkPhaseField::relaxation_AllenCahn();
+    flow.mkPhaseField::evolution_anisotropy();
+    flow.mkPhaseField::evolution(flow.node);
+    flow.mkThermalField::propagate();
+    flow.mkThermalField::relaxation();
+    flow.mkThermalField::relaxation(flow);
+    flow.update_velocity(flow);
+    flow.mkThermalField::bdy_flowin();
+    flow.mkThermalField::bdy_RBC();
+    flow.mkSolutalField::propagate();
+    flow.mkSolutalField::relaxation();
+    flow.mkSolutalField::evolution(flow.node);
+    flow.mkSolutalField::evolution_ATC1(flow.node);
+    flow.mkSolutalField::evolution_ATC2(flow.node);
+    flow.mkSolutalField::updateDCs(flow.node);
+    flow.mkSolutalField::bdy_LBflowin();
+    flow.check_variables(loops, 'x');
+    (void)flow.Thermal_NusseltRBC();
+    flow.write_vtkfile(loops);
+    user.initialize(xml);
+    user.initialize_geometry();
+    user.propagate();
+    user.relaxation();
+    two.initialize(xml);
+    two.initialize_geometry();
+    two.initialize_wetting(xml);
+    two.propagate();
+    two.relaxation();
+    two.write_vtkfile(loops);
+}
+
+int main(int argc, char** argv) {
+    if (argc > 99) {
+        mkSpace space(argv[1]);
+        mkFluidsMRT flow(space);
+        mkFluidsUser user(space);
+        mkFluidsMultiphase two(space);
+        loops(flow, user, two, argv[1], argc);
+    }
+    return 0;
+}
+"""
+
+
+def test_reference_driver_call_forms_compile_against_the_shim(tmp_path):
+    """The reference drivers' qualified calls, overloads and flow.node arguments compile and
+    link against the shim (include/mkg.h + shim/mkshim.h, libmkshim.so)."""
+    pkg = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd"
+    if not (pkg / "libmkshim.so").exists():
+        pytest.skip("shim not built")
+    src = tmp_path / "driver.cpp"
+    src.write_text(DRIVER_CALLS)
+    exe = tmp_path / "driver"
+    res = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{ROOT / 'include'}",
+                          f"-I{pkg / 'shim'}", "-o", str(exe), str(src), f"-L{pkg}", "-lmkshim",
+                          "-lmkg", f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    assert subprocess.run([str(exe)], timeout=60).returncode == 0
