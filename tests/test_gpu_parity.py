@@ -215,6 +215,17 @@ def test_fp64_z_stage_walls_match_oracle(case):
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+def test_fp64_z_stage_with_two_rows_per_block_matches_oracle(case, bcs):
+    """nz = 128: a 256-thread block holds two z rows of two waves each."""
+    errs, fd, fo, dev, orc = _run(case, bcs, 5, walls=bool(case.get("flow_q")), dims=(3, 2, 128))
+    dev.close()
+    assert np.array_equal(fd, fo), "flags differ"
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
 def test_z_stage_agrees_with_direct_o_steps(case, monkeypatch):
     """The z stage changes only how populations move: FP32 results with and without it agree
     to FP32 rounding (the two kernels are compiled separately, so the compiler may contract
