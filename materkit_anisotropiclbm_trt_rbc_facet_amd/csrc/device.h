@@ -393,10 +393,13 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, con
             }
         }
     }
-    zm[tid] = act ? m : 0u;
+    // without faces and walls, with every thread updating, every z population is sent
+    if constexpr (FACES) zm[tid] = act ? m : 0u;
 }
 
-template <int Q, class R>
+// FACES false: the block has no face cells, no walls and no thread past the last cell, so
+// every sender sent every z population and the masks are not read.
+template <int Q, class R, bool FACES = true>
 __device__ inline void zpush_finish(const LatArgs& L, uint32_t S, const Cell& c,
                                     const StepArgs& a, R (*zv)[256], const uint32_t* zm,
                                     uint32_t tid) {
@@ -408,7 +411,7 @@ __device__ inline void zpush_finish(const LatArgs& L, uint32_t S, const Cell& c,
         if (cz(i) != 0) {
             // sender: the cell (x, y, z - c_z) of this thread's row
             const uint32_t s = rb + ((uint32_t)(c.z - cz(i)) & zm1);
-            if ((zm[s] >> zslot<Q>(i)) & 1u)
+            if (!FACES || ((zm[s] >> zslot<Q>(i)) & 1u))
                 stp(slot_at(base + (size_t)i * S, nbr(c, cx(i), cy(i), 0)), zv[zslot<Q>(i)][s]);
         }
     }

@@ -127,7 +127,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
     const uint32_t tid = threadIdx.x;
     // Blocks (z stage) or waves (otherwise) none of whose cells lies on a non-periodic face,
-    // in a domain without wall cells, take the body without face rules (FACES false).
+    // in a domain without wall cells, take the body without face rules (FACES false); a z
+    // stage block must also lie wholly before the last cell, so that every thread updates.
     bool slow = a.has_walls != 0;
     if (!slow) {
         const uint32_t t = thread_cell(a);
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         const uint32_t np = (uint32_t)(a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask);
         const int face = (e.bits & np) != 0u;
         if constexpr (Z)
-            slow = __syncthreads_or(face) != 0;
+            slow = __syncthreads_or(face || t >= a.N) != 0;
         else
             slow = __ballot(face) != 0ull;
     }
@@ -152,6 +153,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     const int8_t fl = a.flag[c.id];
     if constexpr (Z) {
         act = act && fl != kIsWalls;
+        if constexpr (!FACES) act = true;  // no walls, no thread past the last cell
     } else {
         if (fl == kIsWalls) return;
     }
@@ -389,11 +391,11 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     if constexpr (Z) {
         __syncthreads();  // B2
         if constexpr (FQ > 0)
-            if (a.f.adv) zpush_finish<FQ, R>(a.f, S, c, a, zwf, zmf, tid);
+            if (a.f.adv) zpush_finish<FQ, R, FACES>(a.f, S, c, a, zwf, zmf, tid);
         if constexpr (TH)
-            if (a.h.adv) zpush_finish<7, R>(a.h, S, c, a, zwh, zmh, tid);
+            if (a.h.adv) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
         if constexpr (SO)
-            if (a.c.adv) zpush_finish<7, R>(a.c, S, c, a, zwc, zmc, tid);
+            if (a.c.adv) zpush_finish<7, R, FACES>(a.c, S, c, a, zwc, zmc, tid);
         if constexpr (PQ > 0)
             if (a.g.adv) zpull_finish<PQ, R>(g, c, a, zrg, tid, own_g);
     }
@@ -419,7 +421,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         }
         if constexpr (Z) {
             __syncthreads();  // B3
-            if (a.g.adv) zpush_finish<PQ, R>(a.g, S, c, a, zrf, zmg, tid);
+            if (a.g.adv) zpush_finish<PQ, R, FACES>(a.g, S, c, a, zrf, zmg, tid);
         }
     }
     };
