@@ -119,6 +119,23 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
     __shared__ R zr[ZQ][256], zw[ZQ][256], zrh[ZT][256], zwh[ZT][256];
     __shared__ uint32_t zm[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1];
     const uint32_t tid = threadIdx.x;
+    // Blocks (z stage) or waves none of whose cells lies on a non-periodic face, in a domain
+    // without wall cells (and, with the z stage, wholly before the last cell), take the body
+    // without face rules (FACES false), as the fused step does.
+    bool slow = m.a.has_walls != 0;
+    if (!slow) {
+        const uint32_t t = thread_cell(m.a);
+        Cell e;
+        locate_at(m.a, t < m.a.N ? t : m.a.N - 1u, e);
+        const uint32_t np = (uint32_t)(L.npmask | (TH ? m.a.h.npmask : 0));
+        const int face = (e.bits & np) != 0u;
+        if constexpr (Z)
+            slow = __syncthreads_or(face || t >= m.a.N) != 0;
+        else
+            slow = __ballot(face) != 0ull;
+    }
+    auto body = [&](auto faces_tag) {
+    constexpr bool FACES = decltype(faces_tag)::value;
     Cell c;
     bool act = true;
     if constexpr (Z) {
@@ -129,6 +146,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
     const bool wall = m.a.flag[c.id] == kIsWalls;
     if constexpr (Z) {
         act = act && !wall;
+        if constexpr (!FACES) act = true;  // no walls, no thread past the last cell
     } else {
         if (wall) return;
     }
@@ -152,16 +170,16 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
     R g[19];
     uint32_t own = 0;
     if constexpr (Z)
-        own = zpull_issue<19, R>(g, L, S, c, a, zr, tid);
+        own = zpull_issue<19, R, FACES>(g, L, S, c, a, zr, tid);
     else
-        read_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
+        read_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
     R h[TH ? 7 : 1];
     uint32_t own_h = 0;
     if constexpr (TH) {
         if constexpr (Z)
-            own_h = zpull_issue<7, R>(h, a.h, S, c, a, zrh, tid);
+            own_h = zpull_issue<7, R, FACES>(h, a.h, S, c, a, zrh, tid);
         else
-            read_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
     }
     if constexpr (Z) {
         __syncthreads();
@@ -183,9 +201,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
             h[i] = h[i] - omh * (h[i] - w * sh) + w * srh;
         }
         if constexpr (Z)
-            zpush_stage<7, R>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act, FaceState<R>());
+            zpush_stage<7, R, FACES>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+                                     FaceState<R>());
         else
-            write_pops<7, ODD, R>(h, a.h, S, c, a, kKindScalar);
+            write_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
     } else {
         if (m.thermal && m.growth != 0.0) {
             R hh[7];
@@ -216,13 +235,18 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         g[i] = g[i] - om * (g[i] - geq) + w * (src + zd * cn);
     }
     if constexpr (Z) {
-        zpush_stage<19, R>(g, L, S, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
+        zpush_stage<19, R, FACES>(g, L, S, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
         __syncthreads();
-        zpush_finish<19, R>(L, S, c, a, zw, zm, tid);
-        if constexpr (TH) zpush_finish<7, R>(a.h, S, c, a, zwh, zmh, tid);
+        zpush_finish<19, R, FACES>(L, S, c, a, zw, zm, tid);
+        if constexpr (TH) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
     } else {
-        write_pops<19, ODD, R>(g, L, S, c, a, kKindScalar);
+        write_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
     }
+    };
+    if (slow)
+        body(std::true_type{});
+    else
+        body(std::false_type{});
 }
 
 // Equilibrium start in the own-slot layout: phase populations from phi_j (doubles, n x N) with
