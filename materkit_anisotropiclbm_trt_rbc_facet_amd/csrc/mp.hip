@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -149,11 +150,40 @@ __device__ inline void mp_gradient(const R (&P)[19], R (&g)[3]) {
     }
 }
 
-template <class R, bool ODD>
+// ZS: O step with the z stage of the fused step (the block holds whole z rows): three block
+// barriers, B1 after the flow pull loads, B2 after the flow push stage and the Cahn-Hilliard
+// pull loads, B3 after the Cahn-Hilliard push stage, which reuses the flow pull region (last
+// read before B2).  Blocks (z stage) or waves without face cells take the body without face
+// rules (FACES false).
+template <class R, bool ODD, bool ZS = false>
 __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
-    Cell c;
-    if (!locate(m.a, c)) return;
+    constexpr bool Z = ODD && ZS;
+    constexpr int ZQ = Z ? nzdirs<19>() : 1;
+    __shared__ R zrf[ZQ][256], zwf[ZQ][256], zrh[ZQ][256];
+    __shared__ uint32_t zmf[Z ? 256 : 1], zmh[Z ? 256 : 1];
     const StepArgs& a = m.a;
+    const uint32_t tid = threadIdx.x;
+    bool slow;
+    {
+        const uint32_t t = thread_cell(a);
+        Cell e;
+        locate_at(a, t < a.N ? t : a.N - 1u, e);
+        const int face = (e.bits & (uint32_t)(a.f.npmask | a.g.npmask)) != 0u;
+        if constexpr (Z)
+            slow = __syncthreads_or(face || t >= a.N) != 0;
+        else
+            slow = __ballot(face) != 0ull;
+    }
+    auto body = [&](auto faces_tag) {
+    constexpr bool FACES = decltype(faces_tag)::value;
+    Cell c;
+    bool act = true;
+    if constexpr (Z) {
+        act = locate_clamped(a, c);
+        if constexpr (!FACES) act = true;  // no thread past the last cell
+    } else {
+        if (!locate(a, c)) return;
+    }
     const uint32_t S = a.S;
     R P[19];
     mp_stencil<R>(m, c, (const R*)m.phi, P);
@@ -162,7 +192,14 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     const R ph = P[0];
     const R muc = *slot_at((const R*)m.mu, c.id);
     R f[19];
-    read_pops<19, ODD, R>(f, a.f, S, c, a, kKindFlow);
+    uint32_t own_f = 0;
+    if constexpr (Z) {
+        own_f = zpull_issue<19, R, FACES>(f, a.f, S, c, a, zrf, tid);
+        __syncthreads();  // B1
+        zpull_finish<19, R>(f, c, a, zrf, tid, own_f);
+    } else {
+        read_pops<19, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow);
+    }
     R rho, u[3], p, F[3], gr[3];
     mp_macro<R>(m, f, ph, muc, g, rho, u, p, F, gr);
     const R pc = ph < (R)0 ? (R)0 : (ph > (R)1 ? (R)1 : ph);
@@ -182,7 +219,10 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
         f[i] = f[i] - itf * (f[i] - feq) + Fi;
     }
     FaceState<R> fs;
-    write_pops<19, ODD, R>(f, a.f, S, c, a, kKindFlow, fs);
+    if constexpr (Z)
+        zpush_stage<19, R, FACES>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+    else
+        write_pops<19, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow, fs);
 
     // Cahn-Hilliard lattice
     R* pu = (R*)m.phiu;
@@ -192,10 +232,18 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
         R* q = slot_at(pu + (size_t)d * S, c.id);
         const R v = ph * u[d];
         dpu[d] = v - *q;
-        *q = v;
+        if (act) *q = v;
     }
     R h[19];
-    read_pops<19, ODD, R>(h, a.g, S, c, a, kKindScalar);
+    uint32_t own_h = 0;
+    if constexpr (Z) {
+        own_h = zpull_issue<19, R, FACES>(h, a.g, S, c, a, zrh, tid);
+        __syncthreads();  // B2
+        zpush_finish<19, R, FACES>(a.f, S, c, a, zwf, zmf, tid);
+        zpull_finish<19, R>(h, c, a, zrh, tid, own_h);
+    } else {
+        read_pops<19, ODD, R, FACES>(h, a.g, S, c, a, kKindScalar);
+    }
     const R ith = (R)(1.0 / m.tau_h), kh = (R)(1.0 - 0.5 / m.tau_h);
     const R em = (R)m.eta * muc;
 #pragma unroll
@@ -206,7 +254,19 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
         const R heq = i == 0 ? ph + (w - (R)1) * em : w * (em + (R)3 * ph * cu);
         h[i] = h[i] - ith * (h[i] - heq) + kh * w * (R)3 * cd;
     }
-    write_pops<19, ODD, R>(h, a.g, S, c, a, kKindScalar);
+    if constexpr (Z) {
+        zpush_stage<19, R, FACES>(h, a.g, S, c, a, kKindScalar, zrf, zmh, tid, act,
+                                  FaceState<R>());
+        __syncthreads();  // B3
+        zpush_finish<19, R, FACES>(a.g, S, c, a, zrf, zmh, tid);
+    } else {
+        write_pops<19, ODD, R, FACES>(h, a.g, S, c, a, kKindScalar);
+    }
+    };
+    if (slow)
+        body(std::true_type{});
+    else
+        body(std::false_type{});
 }
 
 // p and u (3 x N cell-interleaved, doubles) of the current pre-collision state (phi and mu
@@ -286,6 +346,7 @@ struct mkg_mp_ctx {
     double* partials = nullptr;
     double* red_out = nullptr;
     int lay = 0;
+    int zstage = 0;  // O steps take the z shift through LDS (blocks hold whole z rows)
 };
 
 namespace {
@@ -423,6 +484,8 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
     h->S = ((h->N + 127u) & ~127u) + 1024u;
     h->F = 2u * ((uint32_t)p->ny * p->nz + (uint32_t)p->nx * p->nz + (uint32_t)p->nx * p->ny);
     h->nblk = (h->N + 255u) / 256u;
+    h->zstage = (p->nz >= 64 && 256 % p->nz == 0) ? 1 : 0;
+    if (const char* env = std::getenv("MKG_ZSTAGE")) h->zstage = h->zstage && std::atoi(env) != 0;
     auto bad = [&](hipError_t e, const char* what) {
         mp_destroy_impl(h);
         return pfail(std::string(what) + ": " + hipGetErrorString(e));
@@ -492,11 +555,13 @@ int mkg_mp_step(mkg_mp_handle h, int nsteps) {
         const int odd = h->lay;
         if (h->fp32) {
             launch_phi_mu<float>(m, odd, h->stream);
-            if (odd) k_mp_step<float, true><<<grid, block, 0, h->stream>>>(m);
+            if (odd && h->zstage) k_mp_step<float, true, true><<<grid, block, 0, h->stream>>>(m);
+            else if (odd) k_mp_step<float, true><<<grid, block, 0, h->stream>>>(m);
             else k_mp_step<float, false><<<grid, block, 0, h->stream>>>(m);
         } else {
             launch_phi_mu<double>(m, odd, h->stream);
-            if (odd) k_mp_step<double, true><<<grid, block, 0, h->stream>>>(m);
+            if (odd && h->zstage) k_mp_step<double, true, true><<<grid, block, 0, h->stream>>>(m);
+            else if (odd) k_mp_step<double, true><<<grid, block, 0, h->stream>>>(m);
             else k_mp_step<double, false><<<grid, block, 0, h->stream>>>(m);
         }
         h->lay ^= 1;

@@ -28,6 +28,10 @@ CASES = {
     # viscosity from mu = rho nu linear or harmonic in phi
     "visc_mu_linear": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P], visc=1),
     "visc_mu_harmonic": dict(dims=(10, 12, 8), bc=[P, P, BB, BB, P, P], visc=2),
+    # nz = 64: O steps take the z shift through LDS (the block holds whole z rows)
+    "z64_periodic": dict(dims=(5, 3, 64), bc=[P] * 6),  # last block part-filled
+    "z64_wet_walls": dict(dims=(4, 3, 64), bc=[P, P, BB, BB, BB, BB], wet=0b110000, angle=70.0,
+                          center=(1.5, 1.0, 3.0)),
 }
 
 
@@ -173,3 +177,22 @@ def test_wetting_drop_on_device_settles_at_the_contact_angle(theta):
     angle, _ = contact_angle(got32.reshape(100, 40))
     assert abs(angle - theta) < 3.0, angle
     assert abs(angle - contact_angle(ref.reshape(100, 40))[0]) < 0.05
+
+
+@pytest.mark.parametrize("name", ["z64_periodic", "z64_wet_walls"])
+def test_two_phase_z_stage_agrees_with_direct_o_steps(name, monkeypatch):
+    """The z stage changes only how populations move: FP32 results with and without it agree
+    to FP32 rounding."""
+    m = make(name, mkg.PREC_FP32)
+    phi, u, p = start(m, center=CASES[name].get("center"))
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MKG_ZSTAGE", flag)
+        dev = mkg.MpSolver(m)
+        dev.init(phi, u, p)
+        dev.step(7)
+        out.append((dev.get(mkg.MP_POP_FLOW), dev.get(mkg.MP_POP_PHASE)))
+        dev.close()
+    for k in range(2):
+        scale = np.abs(out[1][k]).max()
+        assert np.abs(out[0][k] - out[1][k]).max() <= 1e-6 * scale
