@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
 
 // Minimum waves per SIMD requested from the register allocator for the fused kernel.  FP32:
 // four (at most 128 VGPRs).  Left alone, the allocator gives the z-staged O step 136 VGPRs and
-// three waves; with four it spills 12 bytes per lane and the step runs 7 % faster at 256^3.
+// three waves; with four it spills (72 bytes of scratch per lane, about 6 B per cell of extra
+// HBM writes at 256^3) and the step runs 7 % faster.
 // FP64 kernels need about 230 VGPRs (two waves); a bound there only adds spills.
 #ifndef MKG_STEP_WAVES_F32
 #define MKG_STEP_WAVES_F32 4
