@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--flow-adjoint-steps", type=int, default=100)
     ap.add_argument("--no-mpf", action="store_true")
     ap.add_argument("--no-fvm", action="store_true")
     ap.add_argument("--no-mp", action="store_true")
@@ -214,6 +215,44 @@ def fvm_cost(size: int, steps: int) -> dict:
             "mlups": round(N / dt / 1e6, 1), "ms_per_step": round(dt * 1e3, 4),
             "alg_bytes_per_cell": bytes_cell, "step_gbs": round(gbs, 1),
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def flow_adjoint_cost(steps: int) -> dict:
+    """Flow adjoint (mkg_flow_adjoint_run) on the duct of the Poiseuille case at 128 x 64 x 64
+    (D3Q19 BGK, FP32, body force, bounce walls): ratio = (adjoint run - plain forward) / plain
+    forward.  The adjoint run tapes every step's populations in FP64 and reverses in FP64."""
+    import torch
+
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+
+    case = cases.load("C1", precision=mkg.PREC_FP32)
+    p = case.params
+    p.nx, p.ny, p.nz = 128, 64, 64
+
+    def fresh():
+        dev = mkg.Solver(p)
+        dev.init_equilibrium()
+        dev.step(2, mkg.FLOW)
+        return dev
+
+    dev = fresh()
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev.step(steps, mkg.FLOW)
+    dev.sync()
+    fwd = time.perf_counter() - t0
+    target = dev.get(mkg.F_VEL) * 1.01
+    dev.close()
+    dev = fresh()
+    dev.sync()
+    t0 = time.perf_counter()
+    r = dev.flow_adjoint_run(steps, target)
+    adj = time.perf_counter() - t0
+    dev.close()
+    return {"workload": "flow adjoint, duct 128x64x64, D3Q19 BGK, FP32 forward, FP64 tape",
+            "steps": steps, "forward_s": round(fwd, 4), "adjoint_run_s": round(adj, 4),
+            "ratio": round((adj - fwd) / fwd, 3), "loss": r["loss"]}
 
 
 def mp_cost(size: int, steps: int) -> dict:
@@ -413,6 +452,7 @@ def main():
         adj = adjoint_cost(a.adjoint_steps)
         out["adjoint_over_forward"] = adj["ratio"]
         out["adjoint"] = adj
+        out["flow_adjoint"] = flow_adjoint_cost(a.flow_adjoint_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         out["cpu_baseline"]["other_solvers"] = cpu_other_solvers(a.cpu_seconds)

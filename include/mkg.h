@@ -255,6 +255,19 @@ int mkg_enable_timing(mkg_handle h, int on);
 int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* phi_target, double* loss,
                     double* dL_deps4, double* dL_dkappa, double* times);
 
+/* Discrete adjoint of nsteps flow-only steps (no phase, thermal or solute lattice) from the
+ * current state: the generalised moment collision (BGK, TRT, MRT) with Guo forcing and
+ * streaming with periodic, bounce (any wall velocity) or outflow faces and wall cells
+ * (the adjoints of mkFluidsLBM::propagate / relaxation and the MRT overrides,
+ * MaterKit/inc/mkFluidsLBM.h:47-51, mkFluidsMRT.h:42-46; no counterpart in the reference).
+ * Loss L = 1/2 sum over non-wall cells |u_N - u_target|^2 (u_target: 3 x N, cell-interleaved;
+ * u the Guo half-force velocity), reduced in FP64.  grad[7] = dL/d(force x, y, z) and
+ * dL/d(s_nu, s_bulk, s_odd, s_even), each rate taken as independent (a BGK gradient with
+ * respect to its single rate is the sum of the four).  The handle is left at step nsteps.
+ * The forward keeps every step's populations in FP64 (8 Q N bytes per step). */
+int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, double* loss,
+                         double* grad);
+
 /* ---- Multi-phase field ---------------------------------------------------------------
  * n order parameters phi_j in [-1, 1], each with its own D3Q19 population lattice, plus an
  * optional D3Q7 thermal lattice: the per-grain phase lattices (gVec, phiVec) of

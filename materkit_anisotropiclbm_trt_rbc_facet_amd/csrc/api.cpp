@@ -819,6 +819,87 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
 
 
 // ---------------------------------------------------------------- discrete adjoint
+int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, double* loss,
+                         double* grad) {
+    if (!h || !u_target || !loss || !grad) return fail("null argument");
+    const int Q = h->q[LF];
+    if (!Q || h->q[LG] || h->q[LH] || h->q[LC])
+        return fail("the flow adjoint covers the flow lattice alone (no phase, thermal or solute)");
+    if (h->p.flow_incomp) return fail("the flow adjoint covers the compressible equilibrium");
+    for (int f = 0; f < 6; f++) {
+        const int b = h->p.bc_f[f];
+        if (b != MKG_BC_PERIODIC && b != MKG_BC_BOUNCE && b != MKG_BC_OUTFLOW)
+            return fail("the flow adjoint takes periodic, bounce or outflow flow faces");
+    }
+    if (nsteps < 1) return fail("nsteps must be >= 1");
+    const size_t N = h->N, QN = (size_t)Q * N;
+    {
+        std::vector<int8_t> fl(N);
+        HIPCHK(hipMemcpy(fl.data(), h->flag, N, hipMemcpyDeviceToHost));
+        for (size_t x = 0; x < N; x++)
+            if (fl[x] != MKG_IS_FLUID && fl[x] != MKG_IS_WALLS)
+                return fail("the flow adjoint takes fluid and wall cells only");
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const size_t need = ((size_t)(nsteps + 1) * QN + 2 * QN + 3 * N) * 8 + (size_t)8 * h->nblk * 8;
+    if (need > free_b / 10 * 9) return fail("the flow adjoint tape does not fit in device memory");
+    double *tape = nullptr, *lam[2] = {nullptr, nullptr}, *ut = nullptr, *part = nullptr,
+           *out = nullptr;
+    auto release = [&]() {
+        double* bufs[] = {tape, lam[0], lam[1], ut, part, out};
+        for (double* b : bufs)
+            if (b) (void)hipFree(b);
+    };
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&tape, (size_t)(nsteps + 1) * QN * 8)) != hipSuccess ||
+        (e = hipMalloc(&lam[0], QN * 8)) != hipSuccess || (e = hipMalloc(&lam[1], QN * 8)) != hipSuccess ||
+        (e = hipMalloc(&ut, 3 * N * 8)) != hipSuccess ||
+        (e = hipMalloc(&part, (size_t)8 * h->nblk * 8)) != hipSuccess ||
+        (e = hipMalloc(&out, 8 * 8)) != hipSuccess) {
+        release();
+        return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    int rc = 0;
+    do {
+        if ((e = hipMemcpyAsync(ut, u_target, 3 * N * 8, hipMemcpyHostToDevice, h->stream)) ||
+            (e = hipMemsetAsync(part, 0, (size_t)8 * h->nblk * 8, h->stream))) {
+            rc = fail(std::string("hip: ") + hipGetErrorString(e));
+            break;
+        }
+        // forward, keeping each level's canonical populations
+        for (int n = 0; n <= nsteps && rc == 0; n++) {
+            StepArgs a = make_args(h);
+            launch_canonical(Q, h->fp32, a, a.f, 0, tape + (size_t)n * QN, h->stream);
+            if (n < nsteps && mkg_step(h, 1, MKG_FLOW)) rc = -1;
+        }
+        if (rc) break;
+        StepArgs a = make_args(h);
+        launch_fadj_loss(Q, a, tape + (size_t)nsteps * QN, ut, lam[0], part, h->stream);
+        int cur = 0;
+        for (int n = nsteps - 1; n >= 0; n--) {
+            launch_fadj_step(Q, a, tape + (size_t)n * QN, lam[cur], lam[cur ^ 1], part, h->stream);
+            cur ^= 1;
+        }
+        for (int k = 0; k < 8; k++) launch_sum_partials(part + (size_t)k * h->nblk, h->nblk, out + k, h->stream);
+        if ((e = hipGetLastError()) != hipSuccess) {
+            rc = fail(std::string("flow adjoint kernels: ") + hipGetErrorString(e));
+            break;
+        }
+        double hv[8];
+        if ((e = hipMemcpyAsync(hv, out, 8 * 8, hipMemcpyDeviceToHost, h->stream)) ||
+            (e = hipStreamSynchronize(h->stream))) {
+            rc = fail(std::string("hip: ") + hipGetErrorString(e));
+            break;
+        }
+        *loss = hv[0];
+        for (int k = 0; k < 7; k++) grad[k] = hv[1 + k];
+    } while (false);
+    (void)hipStreamSynchronize(h->stream);
+    release();
+    return rc;
+}
+
 int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, double* loss,
                     double* dL_deps4, double* dL_dkappa, double* times) {
     if (!h || !target || !loss || !dL_deps4) return fail("null argument");

@@ -196,6 +196,7 @@ def lib() -> C.CDLL:
         "mkg_last_timing": (C.c_int, [vp, vp]),
         "mkg_enable_timing": (C.c_int, [vp, C.c_int]),
         "mkg_adjoint_run": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]),
+        "mkg_flow_adjoint_run": (C.c_int, [vp, C.c_int, vp, vp, vp]),
         "mkg_mpf_params_size": (C.c_size_t, []),
         "mkg_mpf_params_default": (C.c_int, [P(MpfParams)]),
         "mkg_mpf_create": (C.c_int, [P(MpfParams), P(vp)]),
@@ -406,6 +407,19 @@ class Solver:
         return {"loss": loss.value, "dL_deps4": deps.value, "dL_dkappa": dk,
                 "forward_ms": times[0], "recompute_ms": times[1], "reverse_ms": times[2],
                 "total_ms": times[3]}
+
+    def flow_adjoint_run(self, nsteps: int, u_target) -> dict:
+        """Forward nsteps flow-only steps, FP64 loss 1/2 sum |u_N - u_target|^2 over non-wall
+        cells and its gradient with respect to the body force and the four collision rates."""
+        t = np.ascontiguousarray(u_target, dtype=np.float64).reshape(-1)
+        if t.size != 3 * self.N:
+            raise MkgError("u_target must have 3 N values")
+        loss = C.c_double()
+        grad = np.zeros(7)
+        _check(lib().mkg_flow_adjoint_run(self._h, int(nsteps), _ptr(t), C.byref(loss),
+                                          _ptr(grad)))
+        return {"loss": loss.value, "dL_dforce": grad[:3].copy(),
+                "dL_drates": dict(zip(("s_nu", "s_bulk", "s_odd", "s_even"), grad[3:]))}
 
     def phi_now(self) -> np.ndarray:
         """phi = sum_i g_i of the current pre-collision phase populations."""

@@ -1435,3 +1435,165 @@ void orc_mp_init(const orc_mp_params* m, const double* phi0, const double* u0, c
     free(phi);
     free(mu);
 }
+
+/* ------------------------------------------------------------------ flow adjoint */
+/* Discrete adjoint of the flow-only step with the generalised moment collision
+ * (ORC_OP_MRT form: momentum, traceless second order at s_nu, trace at s_bulk, odd and even
+ * ghosts at s_odd / s_even; BGK and TRT are its special cases) and Guo forcing, for
+ * L = 1/2 sum over non-wall cells |u_N - u*|^2 with u = (sum c f + F/2)/rho after nsteps.
+ * Faces: periodic, bounce (any wall velocity: a constant term) or outflow (zero gradient);
+ * wall cells.  grad[7] = dL/dF (3), dL/d(s_nu, s_bulk, s_odd, s_even).
+ *
+ * Reverse of one step, per non-wall cell y:
+ *  - streaming transposed: post_i(y) was read by the cell y + c_i in direction i (no face
+ *    crossed, y + c_i not a wall), or by y itself in direction inv(i) (bounce face or wall
+ *    at y + c_i), and also by y in direction i when y - c_i lies behind an outflow face;
+ *  - collision f* = feq + A (f - feq) + C G, A = sum_k (1 - r_k) P_k, C = sum_k (1 - r_k/2)
+ *    P_k over the five parts; the projectors are w-orthogonal, P^T = W^-1 P W;
+ *    lambda_f = A^T l + J_feq^T (I - A)^T l + J_G^T C^T l through rho = sum f and
+ *    u = (j + F/2)/rho; dL/dr_k = -l . P_k (f - feq + G/2). */
+int orc_flow_adjoint(const orc_params* p, const int8_t* flag, const double* f0, int nsteps,
+                     const double* u_target, double* loss, double* grad) {
+    int q = p->flow_q;
+    if ((q != 19 && q != 27) || p->phase_q || p->thermal || p->solute || p->flow_incomp)
+        return -1;
+    for (int k = 0; k < 6; k++)
+        if (p->bc_f[k] != 0 && p->bc_f[k] != 1 && p->bc_f[k] != 3) return -1;
+    long N = NCELL(p);
+    for (long x = 0; x < N; x++)
+        if (flag[x] != 0 && flag[x] != 3) return -1; /* fluid and wall cells only */
+    lat_t L = make_lat(q);
+    mrt_t* M = (q == 19) ? &MRT19 : &MRT27;
+    build_mrt(q, M);
+    /* forward with a tape of the pre-collision populations */
+    double* tape = malloc((size_t)(nsteps + 1) * q * N * sizeof(double));
+    orc_state s;
+    memset(&s, 0, sizeof(s));
+    s.f = malloc((size_t)q * N * sizeof(double));
+    s.flag = malloc((size_t)N);
+    memcpy(s.flag, flag, (size_t)N);
+    memcpy(s.f, f0, (size_t)q * N * sizeof(double));
+    for (int n = 0; n < nsteps; n++) {
+        memcpy(tape + (size_t)n * q * N, s.f, (size_t)q * N * sizeof(double));
+        orc_step(p, &s, 1, ORC_OP_MRT, 1);
+    }
+    memcpy(tape + (size_t)nsteps * q * N, s.f, (size_t)q * N * sizeof(double));
+    free(s.f);
+    free(s.flag);
+    for (int k = 0; k < 7; k++) grad[k] = 0;
+    const double* F = p->force;
+    /* loss and its adjoint */
+    double* lam = calloc((size_t)q * N, sizeof(double));
+    double* lst = calloc((size_t)q * N, sizeof(double));
+    double Ls = 0;
+    const double* fN = tape + (size_t)nsteps * q * N;
+    for (long x = 0; x < N; x++) {
+        if (flag[x] == 3) continue;
+        double rho = 0, j[3] = {0, 0, 0};
+        for (int i = 0; i < q; i++) {
+            double v = fN[i * N + x];
+            rho += v;
+            for (int d = 0; d < 3; d++) j[d] += E27[i][d] * v;
+        }
+        double u[3], du[3];
+        for (int d = 0; d < 3; d++) {
+            u[d] = (j[d] + 0.5 * F[d]) / rho;
+            du[d] = u[d] - u_target[3 * x + d];
+            Ls += 0.5 * du[d] * du[d];
+            grad[d] += du[d] / (2.0 * rho);
+        }
+        for (int i = 0; i < q; i++) {
+            double s2 = 0;
+            for (int d = 0; d < 3; d++) s2 += du[d] * (E27[i][d] - u[d]) / rho;
+            lam[i * N + x] = s2;
+        }
+    }
+    *loss = Ls;
+    const double* Ps[5] = {M->mom, M->shear, M->bulk, M->godd, M->geven};
+    const double rates[5] = {1.0, p->s_nu, p->s_bulk, p->s_odd, p->s_even};
+    for (int n = nsteps - 1; n >= 0; n--) {
+        const double* fn = tape + (size_t)n * q * N;
+        /* streaming transposed: lst = adjoint of the post-collision populations */
+        for (int x = 0; x < p->nx; x++)
+            for (int y = 0; y < p->ny; y++)
+                for (int z = 0; z < p->nz; z++) {
+                    long id = lin(p, x, y, z);
+                    if (flag[id] == 3) continue;
+                    for (int i = 0; i < q; i++) {
+                        int X = x + E27[i][0], Y = y + E27[i][1], Z = z + E27[i][2];
+                        int face = resolve(p, p->bc_f, &X, &Y, &Z);
+                        int k = inv_dir(i);
+                        double v = 0;
+                        if (face < 0) {
+                            long t = lin(p, X, Y, Z);
+                            v = flag[t] == 3 ? lam[k * N + id] : lam[i * N + t];
+                        } else if (p->bc_f[face] == 1) {
+                            v = lam[k * N + id];
+                        }
+                        int Xb = x - E27[i][0], Yb = y - E27[i][1], Zb = z - E27[i][2];
+                        int fb = resolve(p, p->bc_f, &Xb, &Yb, &Zb);
+                        if (fb >= 0 && p->bc_f[fb] == 3) v += lam[i * N + id];
+                        lst[i * N + id] = v;
+                    }
+                }
+        /* collision transposed */
+        for (long x = 0; x < N; x++) {
+            if (flag[x] == 3) {
+                for (int i = 0; i < q; i++) lam[i * N + x] = 0;
+                continue;
+            }
+            double f[27], l[27], lw[27], feq[27], G[27], ne[27], t[27];
+            double rho = 0, j[3] = {0, 0, 0};
+            for (int i = 0; i < q; i++) {
+                f[i] = fn[i * N + x];
+                l[i] = lst[i * N + x];
+                lw[i] = L.w[i] * l[i];
+                rho += f[i];
+                for (int d = 0; d < 3; d++) j[d] += E27[i][d] * f[i];
+            }
+            double u[3];
+            for (int d = 0; d < 3; d++) u[d] = (j[d] + 0.5 * F[d]) / rho;
+            orc_equilibrium(q, rho, u, feq);
+            guo_term(q, u, F, G);
+            for (int i = 0; i < q; i++) ne[i] = f[i] - feq[i];
+            double ATl[27] = {0}, CTl[27] = {0};
+            for (int k = 0; k < 5; k++) {
+                /* P^T l = W^-1 P (W l) */
+                matvec(q, Ps[k], lw, t);
+                for (int i = 0; i < q; i++) {
+                    ATl[i] += (1.0 - rates[k]) * t[i] / L.w[i];
+                    CTl[i] += (1.0 - 0.5 * rates[k]) * t[i] / L.w[i];
+                }
+                if (k > 0) {
+                    double pn[27], pg[27], s1 = 0;
+                    matvec(q, Ps[k], ne, pn);
+                    matvec(q, Ps[k], G, pg);
+                    for (int i = 0; i < q; i++) s1 += l[i] * (pn[i] + 0.5 * pg[i]);
+                    grad[2 + k] -= s1;
+                }
+            }
+            double rb = 0, ub[3] = {0, 0, 0};
+            for (int i = 0; i < q; i++) {
+                double mu_i = l[i] - ATl[i];  /* (I - A)^T l */
+                double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
+                double cF = E27[i][0] * F[0] + E27[i][1] * F[1] + E27[i][2] * F[2];
+                rb += mu_i * feq[i] / rho;
+                for (int d = 0; d < 3; d++) {
+                    ub[d] += mu_i * L.w[i] * rho * (3.0 * E27[i][d] + 9.0 * cu * E27[i][d] - 3.0 * u[d]);
+                    ub[d] += CTl[i] * L.w[i] * (-3.0 * F[d] + 9.0 * E27[i][d] * cF);
+                    grad[d] += CTl[i] * L.w[i] * (3.0 * (E27[i][d] - u[d]) + 9.0 * cu * E27[i][d]);
+                }
+            }
+            for (int d = 0; d < 3; d++) grad[d] += ub[d] / (2.0 * rho);
+            for (int i = 0; i < q; i++) {
+                double s2 = 0;
+                for (int d = 0; d < 3; d++) s2 += ub[d] * (E27[i][d] - u[d]) / rho;
+                lam[i * N + x] = ATl[i] + rb + s2;
+            }
+        }
+    }
+    free(lam);
+    free(lst);
+    free(tape);
+    return 0;
+}

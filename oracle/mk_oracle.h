@@ -152,6 +152,13 @@ int orc_mp_step(const orc_mp_params* m, double* f, double* h, double* phiu, int 
 void orc_mp_macro(const orc_mp_params* m, const double* f, const double* h, double* phi,
                   double* mu, double* p, double* u3);
 
+/* Discrete adjoint of nsteps flow-only steps (generalised moment collision, Guo forcing;
+ * faces periodic, bounce or outflow; wall cells) from the pre-collision populations f0
+ * (q x N): L = 1/2 sum over non-wall cells |u_N - u_target|^2; grad[7] = dL/dF (3),
+ * dL/d(s_nu, s_bulk, s_odd, s_even).  Returns -1 for an unsupported configuration. */
+int orc_flow_adjoint(const orc_params* p, const int8_t* flag, const double* f0, int nsteps,
+                     const double* u_target, double* loss, double* grad);
+
 /* Lattice tables (for tests). */
 int orc_lattice(int q, int* e /* q*3 */, double* w, int* inv, double* cs2);
 

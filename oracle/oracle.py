@@ -73,6 +73,8 @@ def lib() -> ct.CDLL:
         L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, ct.c_int]
         L.orc_mpf_step.restype = ct.c_int
+        L.orc_flow_adjoint.argtypes = [vp, vp, vp, ct.c_int, vp, vp, vp]
+        L.orc_flow_adjoint.restype = ct.c_int
         _lib = L
     return _lib
 
@@ -328,3 +330,19 @@ class MpState:
         lib().orc_mp_macro(ct.byref(self.params), _p(self.f), _p(self.h), _p(phi), _p(mu),
                            _p(p), _p(u))
         return {"phi": phi, "mu": mu, "p": p, "u": u.reshape(N, 3)}
+
+
+def flow_adjoint(params, flag, f0, nsteps: int, u_target):
+    """Discrete adjoint of nsteps flow-only steps (orc_flow_adjoint): returns the loss
+    1/2 sum |u_N - u_target|^2 over non-wall cells and the gradient with respect to
+    (F_x, F_y, F_z, s_nu, s_bulk, s_odd, s_even)."""
+    fl = np.ascontiguousarray(flag, dtype=np.int8)
+    f = np.ascontiguousarray(f0, dtype=np.float64)
+    ut = np.ascontiguousarray(u_target, dtype=np.float64).reshape(-1)
+    loss = ct.c_double()
+    grad = np.zeros(7)
+    rc = lib().orc_flow_adjoint(ct.byref(params), _p(fl), _p(f), int(nsteps), _p(ut),
+                                ct.byref(loss), _p(grad))
+    if rc != 0:
+        raise RuntimeError("oracle rejected the flow-adjoint configuration")
+    return loss.value, grad

@@ -147,3 +147,57 @@ def wetting_drop(theta, prec=None, nx=100, ny=40, R=12.0, W=4.0):
     X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
     r = np.sqrt((X - (nx - 1) / 2) ** 2 + (Y + 0.5) ** 2)
     return m, (0.5 - 0.5 * np.tanh(2 * (r - R) / W)).reshape(-1)
+
+
+def flow_adjoint_case(q=19, op="mrt", prec=mkg.PREC_FP64, dims=(7, 6, 5), seed=0, walls=True):
+    """A flow-only case for the adjoint: bounce x- face, outflow x+, periodic y, bounce z faces
+    (the z- wall moving at 0.01 along x), random near-equilibrium populations, wall cells, and
+    a random velocity target."""
+    p = mkg.default_params(nx=dims[0], ny=dims[1], nz=dims[2], flow_q=q, precision=prec)
+    om = 1.0 / 0.8
+    p.s_nu = om
+    if op == "bgk":
+        p.s_bulk = p.s_odd = p.s_even = om
+    elif op == "trt":
+        p.s_bulk = p.s_even = om
+        p.s_odd = 1.0 / ((3.0 / 16.0) / (1.0 / om - 0.5) + 0.5)
+    else:
+        p.s_bulk, p.s_odd, p.s_even = 1.19, 1.2, 1.4
+    p.force[0], p.force[1], p.force[2] = 1e-5, -2e-6, 3e-6
+    for k, v in enumerate([mkg.BC_BOUNCE, mkg.BC_OUTFLOW, mkg.BC_PERIODIC, mkg.BC_PERIODIC,
+                           mkg.BC_BOUNCE, mkg.BC_BOUNCE]):
+        p.bc_f[k] = v
+    p.u_face[4][0] = 0.01
+    st = random_state(p, seed=seed, with_walls=walls)
+    N = int(p.nx) * int(p.ny) * int(p.nz)
+    ut = 0.01 * np.random.default_rng(seed + 100).standard_normal(3 * N)
+    return p, st, ut
+
+
+def oracle_flow_loss(p, st, nsteps, ut):
+    """1/2 sum over non-wall cells |u_N - u_target|^2 after nsteps oracle flow steps (the
+    generalised moment collision)."""
+    s = O.State.empty(p)
+    s.f[:] = st["f"]
+    s.flag[:] = st["flag"]
+    s.step(nsteps, 1, O.OP_MRT)
+    N = s.N
+    f = s.f.reshape(int(p.flow_q), N)
+    e, w, inv, cs2 = O.lattice(int(p.flow_q))
+    rho = f.sum(0)
+    u = ((e.T.astype(float) @ f).T + 0.5 * np.array(list(p.force))) / rho[:, None]
+    m = st["flag"] != mkg.IS_WALLS
+    return 0.5 * ((u - ut.reshape(N, 3))[m] ** 2).sum()
+
+
+FLOW_ADJ_PARAMS = ("Fx", "Fy", "Fz", "s_nu", "s_bulk", "s_odd", "s_even")
+
+
+def perturbed(p, k, h):
+    """A copy of p with the k-th flow-adjoint parameter moved by h."""
+    pp = p.copy()
+    if k < 3:
+        pp.force[k] = p.force[k] + h
+    else:
+        setattr(pp, FLOW_ADJ_PARAMS[k], getattr(p, FLOW_ADJ_PARAMS[k]) + h)
+    return pp
