@@ -875,10 +875,11 @@ int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, doubl
         }
         if (rc) break;
         StepArgs a = make_args(h);
-        launch_fadj_loss(Q, a, tape + (size_t)nsteps * QN, ut, lam[0], part, h->stream);
+        launch_fadj_loss(Q, h->fp32, a, tape + (size_t)nsteps * QN, ut, lam[0], part, h->stream);
         int cur = 0;
         for (int n = nsteps - 1; n >= 0; n--) {
-            launch_fadj_step(Q, a, tape + (size_t)n * QN, lam[cur], lam[cur ^ 1], part, h->stream);
+            launch_fadj_step(Q, h->fp32, a, tape + (size_t)n * QN, lam[cur], lam[cur ^ 1], part,
+                             h->stream);
             cur ^= 1;
         }
         for (int k = 0; k < 8; k++) launch_sum_partials(part + (size_t)k * h->nblk, h->nblk, out + k, h->stream);

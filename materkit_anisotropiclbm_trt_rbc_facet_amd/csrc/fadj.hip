@@ -4,7 +4,8 @@
 // and even ghosts at s_odd / s_even -- BGK and TRT are its special cases) with Guo forcing,
 // then streaming with periodic, bounce (any wall velocity) and outflow faces and wall cells.
 // Loss L = 1/2 sum over non-wall cells |u_N - u*|^2, u = (sum c f + F/2)/rho.  The forward
-// populations come from a tape of canonical FP64 values, and the reverse runs in FP64.
+// populations come from a tape of canonical FP64 values; the reverse runs in the forward's
+// precision (float adjoints for FP32 storage) with FP64 block sums.
 //
 // Reverse of one step, per non-wall cell y (lambda = adjoint of the pre-collision populations
 // of the next level, l = adjoint of the post-collision populations of this level):
@@ -23,47 +24,44 @@
 
 namespace mkg {
 
-// Parts of a Q-vector v in the w-orthogonal decomposition: momentum, traceless second order,
-// trace, odd and even ghosts (the mass part is implied: v minus all of them).
-template <int Q>
-__device__ inline void fadj_parts(const double (&v)[Q], double (&mo)[Q], double (&sh)[Q],
-                                  double (&bk)[Q], double (&go)[Q], double (&ge)[Q]) {
-    double m0 = 0, j[3] = {0, 0, 0}, P[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const double x = v[i];
-        const double X = cx(i), Y = cy(i), Z = cz(i);
+// Moments of a Q-vector that fix its w-orthogonal parts: mass m0, momentum j, the traceless
+// second moment T (xx, yy, zz, xy, xz, yz) and the trace moment tr = sum (c^2 - 3 cs^2) v.
+template <class R>
+struct Mom {
+    R m0 = 0, j0 = 0, j1 = 0, j2 = 0, T0 = 0, T1 = 0, T2 = 0, T3 = 0, T4 = 0, T5 = 0, tr = 0;
+    __device__ inline void add(int i, R x) {
+        const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
         m0 += x;
-        j[0] += X * x;
-        j[1] += Y * x;
-        j[2] += Z * x;
-        P[0] += X * X * x;
-        P[1] += Y * Y * x;
-        P[2] += Z * Z * x;
-        P[3] += X * Y * x;
-        P[4] += X * Z * x;
-        P[5] += Y * Z * x;
+        j0 += X * x;
+        j1 += Y * x;
+        j2 += Z * x;
+        T0 += X * X * x;
+        T1 += Y * Y * x;
+        T2 += Z * Z * x;
+        T3 += X * Y * x;
+        T4 += X * Z * x;
+        T5 += Y * Z * x;
     }
-    const double trace = P[0] + P[1] + P[2];
-    const double tr = trace - m0;  // sum (c^2 - 3 cs^2) v
-    const double t3 = trace / 3.0;
-    const double T0 = P[0] - t3, T1 = P[1] - t3, T2 = P[2] - t3;
-    double g[Q];
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        const double w = wgt<Q>(i);
-        const double X = cx(i), Y = cy(i), Z = cz(i);
-        mo[i] = 3.0 * w * (X * j[0] + Y * j[1] + Z * j[2]);
-        sh[i] = 4.5 * w * (X * X * T0 + Y * Y * T1 + Z * Z * T2 + 2.0 * X * Y * P[3] +
-                           2.0 * X * Z * P[4] + 2.0 * Y * Z * P[5]);
-        bk[i] = 1.5 * w * (double)(csq(i) - 1) * tr;
-        g[i] = v[i] - w * m0 - mo[i] - sh[i] - bk[i];
+    // second moments -> traceless part and trace moment
+    __device__ inline void finish() {
+        const R trace = T0 + T1 + T2, t3 = trace / (R)3;
+        T0 -= t3;
+        T1 -= t3;
+        T2 -= t3;
+        tr = trace - m0;
     }
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-        go[i] = 0.5 * (g[i] - g[inv(i)]);
-        ge[i] = 0.5 * (g[i] + g[inv(i)]);
-    }
+};
+// Parts of direction i of the vector with moments m and component v: momentum, traceless
+// second order, trace, and the remainder (ghost).
+template <int Q, class R>
+__device__ inline void part_of(int i, const Mom<R>& m, R v, R& mo, R& sh, R& bk, R& gh) {
+    const R w = (R)wgt<Q>(i);
+    const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
+    mo = (R)3 * w * (X * m.j0 + Y * m.j1 + Z * m.j2);
+    sh = (R)4.5 * w * (X * X * m.T0 + Y * Y * m.T1 + Z * Z * m.T2 + (R)2 * X * Y * m.T3 +
+                       (R)2 * X * Z * m.T4 + (R)2 * Y * Z * m.T5);
+    bk = (R)1.5 * w * (R)(csq(i) - 1) * m.tr;
+    gh = v - w * m.m0 - mo - sh - bk;
 }
 
 // Deterministic block sum of NV values per thread (wave shuffles, then the four waves in
@@ -87,9 +85,9 @@ __device__ inline void fadj_block_add(double (&v)[NV], double* part, uint32_t nb
 }
 
 // Loss and the adjoint of the final populations; partials 0 (loss) and 1..3 (dL/dF).
-template <int Q>
+template <int Q, class R>
 __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const double* fN,
-                                                   const double* ut, double* lam, double* part) {
+                                                   const double* ut, R* lam, double* part) {
     const uint32_t N = a.N;
     Cell c;
     const bool in = locate_clamped(a, c);
@@ -116,7 +114,7 @@ __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const doubl
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const double s = du[0] * (cx(i) - u[0]) + du[1] * (cy(i) - u[1]) + du[2] * (cz(i) - u[2]);
-            lam[(size_t)i * N + c.id] = s / rho;
+            lam[(size_t)i * N + c.id] = (R)(s / rho);
         }
     }
     fadj_block_add<4>(v, part, a.nblk);
@@ -124,10 +122,16 @@ __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const doubl
 
 // One reverse step: lam_in = adjoint of level n+1, fn = tape of level n; partials 1..3
 // (dL/dF) and 4..7 (dL/d s_nu, s_bulk, s_odd, s_even) accumulate.
-template <int Q>
-__global__ void __launch_bounds__(256) k_fadj_step(const StepArgs a, const double* fn,
-                                                   const double* lam_in, double* lam_out,
-                                                   double* part) {
+// Minimum waves per SIMD asked of the register allocator for the FP32 reverse step: two
+// (240 VGPRs, no spills) measured faster than three or four, which spill (76 and 232 bytes
+// of scratch per lane), at 128 x 64 x 64.
+#ifndef MKG_FADJ_WAVES_F32
+#define MKG_FADJ_WAVES_F32 2
+#endif
+
+template <int Q, class R>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_FADJ_WAVES_F32 : 1) k_fadj_step(const StepArgs a, const double* fn,
+                                                   const R* lam_in, R* lam_out, double* part) {
     const uint32_t N = a.N;
     Cell c;
     const bool in = locate_clamped(a, c);
@@ -135,12 +139,12 @@ __global__ void __launch_bounds__(256) k_fadj_step(const StepArgs a, const doubl
     double gsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (act) {
         const uint32_t np = (uint32_t)a.f.npmask & c.bits;
-        double l[Q];
+        R l[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int X = cx(i), Y = cy(i), Z = cz(i);
             const uint32_t hp = np & faces_of(X, Y, Z);
-            double v = 0;
+            R v = 0;
             if (hp == 0u) {
                 const uint32_t t = nbr(c, X, Y, Z);
                 v = a.flag[t] == kIsWalls ? lam_in[(size_t)inv(i) * N + c.id]
@@ -153,93 +157,131 @@ __global__ void __launch_bounds__(256) k_fadj_step(const StepArgs a, const doubl
                 v += lam_in[(size_t)i * N + c.id];
             l[i] = v;
         }
-        double f[Q], rho = 0, j[3] = {0, 0, 0};
+        R f[Q], rho = 0, j[3] = {0, 0, 0};
 #pragma unroll
         for (int i = 0; i < Q; i++) {
-            f[i] = fn[(size_t)i * N + c.id];
+            f[i] = (R)fn[(size_t)i * N + c.id];
             rho += f[i];
             j[0] += cx(i) * f[i];
             j[1] += cy(i) * f[i];
             j[2] += cz(i) * f[i];
         }
-        const double F0 = a.force[0], F1 = a.force[1], F2 = a.force[2];
-        const double u0 = (j[0] + 0.5 * F0) / rho, u1 = (j[1] + 0.5 * F1) / rho,
-                     u2 = (j[2] + 0.5 * F2) / rho;
-        const double uu = u0 * u0 + u1 * u1 + u2 * u2, uF = u0 * F0 + u1 * F1 + u2 * F2;
-        const double s_nu = a.s_nu, s_b = a.s_bulk, s_o = a.s_odd, s_e = a.s_even;
-        double feq[Q], d[Q], lw[Q];
+        const R F0 = (R)a.force[0], F1 = (R)a.force[1], F2 = (R)a.force[2];
+        const R ir = (R)1 / rho;
+        const R u0 = (j[0] + (R)0.5 * F0) * ir, u1 = (j[1] + (R)0.5 * F1) * ir,
+                u2 = (j[2] + (R)0.5 * F2) * ir;
+        const R uu = u0 * u0 + u1 * u1 + u2 * u2, uF = u0 * F0 + u1 * F1 + u2 * F2;
+        const R s_nu = (R)a.s_nu, s_b = (R)a.s_bulk, s_o = (R)a.s_odd, s_e = (R)a.s_even;
+        auto feq_of = [&](int i) {
+            const R cu = cx(i) * u0 + cy(i) * u1 + cz(i) * u2;
+            return (R)wgt<Q>(i) * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu);
+        };
+        // d = f - feq + G/2 (for the rates; f is dead afterwards)
+        auto d_of = [&](int i) {
+            const R cu = cx(i) * u0 + cy(i) * u1 + cz(i) * u2;
+            const R cF = cx(i) * F0 + cy(i) * F1 + cz(i) * F2;
+            return f[i] - feq_of(i) + (R)0.5 * (R)wgt<Q>(i) * ((R)3 * (cF - uF) + (R)9 * cu * cF);
+        };
+        Mom<R> md, ml;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
-            const double w = wgt<Q>(i);
-            const double cu = cx(i) * u0 + cy(i) * u1 + cz(i) * u2;
-            const double cF = cx(i) * F0 + cy(i) * F1 + cz(i) * F2;
-            feq[i] = w * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-            const double G = w * (3.0 * (cF - uF) + 9.0 * cu * cF);
-            d[i] = f[i] - feq[i] + 0.5 * G;
-            lw[i] = w * l[i];
+            md.add(i, d_of(i));
+            ml.add(i, (R)wgt<Q>(i) * l[i]);
         }
-        double mo[Q], sh[Q], bk[Q], go[Q], ge[Q];
+        md.finish();
+        ml.finish();
         // rates: dL/dr_k = -l . P_k (f - feq + G/2)
-        fadj_parts<Q>(d, mo, sh, bk, go, ge);
+        R r4 = 0, r5 = 0, r6 = 0, r7 = 0;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
-            gsum[4] -= l[i] * sh[i];
-            gsum[5] -= l[i] * bk[i];
-            gsum[6] -= l[i] * go[i];
-            gsum[7] -= l[i] * ge[i];
+            R mo, sh, bk, gi, mk, sk, bkk, gk;
+            part_of<Q, R>(i, md, d_of(i), mo, sh, bk, gi);
+            part_of<Q, R>(inv(i), md, d_of(inv(i)), mk, sk, bkk, gk);
+            r4 -= l[i] * sh;
+            r5 -= l[i] * bk;
+            r6 -= l[i] * (R)0.5 * (gi - gk);
+            r7 -= l[i] * (R)0.5 * (gi + gk);
         }
-        // A^T l and C^T l through P^T = W^-1 P W
-        fadj_parts<Q>(lw, mo, sh, bk, go, ge);
-        double rb = 0, ub0 = 0, ub1 = 0, ub2 = 0;
-        double ATl[Q];
+        gsum[4] = (double)r4;
+        gsum[5] = (double)r5;
+        gsum[6] = (double)r6;
+        gsum[7] = (double)r7;
+        // A^T l and C^T l of direction i through P^T = W^-1 P W
+        auto atl_ctl = [&](int i, R& ATl, R& CTl) {
+            R mo, sh, bk, gi, mk, sk, bkk, gk;
+            part_of<Q, R>(i, ml, (R)wgt<Q>(i) * l[i], mo, sh, bk, gi);
+            part_of<Q, R>(inv(i), ml, (R)wgt<Q>(inv(i)) * l[inv(i)], mk, sk, bkk, gk);
+            const R go = (R)0.5 * (gi - gk), ge = (R)0.5 * (gi + gk);
+            const R iw = (R)(1.0 / wgt<Q>(i));
+            ATl = (((R)1 - s_nu) * sh + ((R)1 - s_b) * bk + ((R)1 - s_o) * go + ((R)1 - s_e) * ge) * iw;
+            CTl = ((R)0.5 * mo + ((R)1 - (R)0.5 * s_nu) * sh + ((R)1 - (R)0.5 * s_b) * bk +
+                   ((R)1 - (R)0.5 * s_o) * go + ((R)1 - (R)0.5 * s_e) * ge) * iw;
+        };
+        R rb = 0, ub0 = 0, ub1 = 0, ub2 = 0, q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
-            const double w = wgt<Q>(i);
-            const double iw = 1.0 / w;
-            ATl[i] = ((1.0 - s_nu) * sh[i] + (1.0 - s_b) * bk[i] + (1.0 - s_o) * go[i] +
-                      (1.0 - s_e) * ge[i]) * iw;
-            const double CTl = (0.5 * mo[i] + (1.0 - 0.5 * s_nu) * sh[i] + (1.0 - 0.5 * s_b) * bk[i] +
-                                (1.0 - 0.5 * s_o) * go[i] + (1.0 - 0.5 * s_e) * ge[i]) * iw;
-            const double mu = l[i] - ATl[i];
-            const double X = cx(i), Y = cy(i), Z = cz(i);
-            const double cu = X * u0 + Y * u1 + Z * u2;
-            const double cF = X * F0 + Y * F1 + Z * F2;
-            rb += mu * feq[i] / rho;
-            const double a1 = mu * w * rho, a2 = CTl * w;
-            ub0 += a1 * (3.0 * X + 9.0 * cu * X - 3.0 * u0) + a2 * (-3.0 * F0 + 9.0 * X * cF);
-            ub1 += a1 * (3.0 * Y + 9.0 * cu * Y - 3.0 * u1) + a2 * (-3.0 * F1 + 9.0 * Y * cF);
-            ub2 += a1 * (3.0 * Z + 9.0 * cu * Z - 3.0 * u2) + a2 * (-3.0 * F2 + 9.0 * Z * cF);
-            gsum[1] += a2 * (3.0 * (X - u0) + 9.0 * cu * X);
-            gsum[2] += a2 * (3.0 * (Y - u1) + 9.0 * cu * Y);
-            gsum[3] += a2 * (3.0 * (Z - u2) + 9.0 * cu * Z);
+            R ATl, CTl;
+            atl_ctl(i, ATl, CTl);
+            // the final adjoint's A^T l part now, the rest once rb and ubar are known
+            lam_out[(size_t)i * N + c.id] = ATl;
+            const R w = (R)wgt<Q>(i);
+            const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
+            const R cu = X * u0 + Y * u1 + Z * u2;
+            const R cF = X * F0 + Y * F1 + Z * F2;
+            const R mu = l[i] - ATl;
+            rb += mu * feq_of(i) * ir;
+            const R a1 = mu * w * rho, a2 = CTl * w;
+            ub0 += a1 * ((R)3 * X + (R)9 * cu * X - (R)3 * u0) + a2 * ((R)-3 * F0 + (R)9 * X * cF);
+            ub1 += a1 * ((R)3 * Y + (R)9 * cu * Y - (R)3 * u1) + a2 * ((R)-3 * F1 + (R)9 * Y * cF);
+            ub2 += a1 * ((R)3 * Z + (R)9 * cu * Z - (R)3 * u2) + a2 * ((R)-3 * F2 + (R)9 * Z * cF);
+            q1 += a2 * ((R)3 * (X - u0) + (R)9 * cu * X);
+            q2 += a2 * ((R)3 * (Y - u1) + (R)9 * cu * Y);
+            q3 += a2 * ((R)3 * (Z - u2) + (R)9 * cu * Z);
         }
-        gsum[1] += ub0 / (2.0 * rho);
-        gsum[2] += ub1 / (2.0 * rho);
-        gsum[3] += ub2 / (2.0 * rho);
+        gsum[1] = (double)(q1 + (R)0.5 * ub0 * ir);
+        gsum[2] = (double)(q2 + (R)0.5 * ub1 * ir);
+        gsum[3] = (double)(q3 + (R)0.5 * ub2 * ir);
 #pragma unroll
-        for (int i = 0; i < Q; i++)
-            lam_out[(size_t)i * N + c.id] =
-                ATl[i] + rb + (ub0 * (cx(i) - u0) + ub1 * (cy(i) - u1) + ub2 * (cz(i) - u2)) / rho;
+        for (int i = 0; i < Q; i++) {
+            R* o = &lam_out[(size_t)i * N + c.id];
+            *o = *o + rb + (ub0 * (cx(i) - u0) + ub1 * (cy(i) - u1) + ub2 * (cz(i) - u2)) * ir;
+        }
     } else if (in) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) lam_out[(size_t)i * N + c.id] = 0.0;
+        for (int i = 0; i < Q; i++) lam_out[(size_t)i * N + c.id] = (R)0;
     }
     fadj_block_add<8>(gsum, part, a.nblk);
 }
 
-int launch_fadj_loss(int q, const StepArgs& a, const double* fN, const double* ut, double* lam,
-                     double* part, hipStream_t s) {
-    if (q == 19) k_fadj_loss<19><<<a.nblk, 256, 0, s>>>(a, fN, ut, lam, part);
-    else if (q == 27) k_fadj_loss<27><<<a.nblk, 256, 0, s>>>(a, fN, ut, lam, part);
+// The reverse runs in the forward's precision (fp32: float adjoints and arithmetic), with
+// the FP64 tape and FP64 block sums.
+template <class R>
+static int fadj_loss_t(int q, const StepArgs& a, const double* fN, const double* ut, void* lam,
+                       double* part, hipStream_t s) {
+    if (q == 19) k_fadj_loss<19, R><<<a.nblk, 256, 0, s>>>(a, fN, ut, (R*)lam, part);
+    else if (q == 27) k_fadj_loss<27, R><<<a.nblk, 256, 0, s>>>(a, fN, ut, (R*)lam, part);
     else return -1;
     return 0;
 }
-int launch_fadj_step(int q, const StepArgs& a, const double* fn, const double* lam_in,
-                     double* lam_out, double* part, hipStream_t s) {
-    if (q == 19) k_fadj_step<19><<<a.nblk, 256, 0, s>>>(a, fn, lam_in, lam_out, part);
-    else if (q == 27) k_fadj_step<27><<<a.nblk, 256, 0, s>>>(a, fn, lam_in, lam_out, part);
+template <class R>
+static int fadj_step_t(int q, const StepArgs& a, const double* fn, const void* lam_in,
+                       void* lam_out, double* part, hipStream_t s) {
+    if (q == 19)
+        k_fadj_step<19, R><<<a.nblk, 256, 0, s>>>(a, fn, (const R*)lam_in, (R*)lam_out, part);
+    else if (q == 27)
+        k_fadj_step<27, R><<<a.nblk, 256, 0, s>>>(a, fn, (const R*)lam_in, (R*)lam_out, part);
     else return -1;
     return 0;
+}
+int launch_fadj_loss(int q, int fp32, const StepArgs& a, const double* fN, const double* ut,
+                     void* lam, double* part, hipStream_t s) {
+    return fp32 ? fadj_loss_t<float>(q, a, fN, ut, lam, part, s)
+                : fadj_loss_t<double>(q, a, fN, ut, lam, part, s);
+}
+int launch_fadj_step(int q, int fp32, const StepArgs& a, const double* fn, const void* lam_in,
+                     void* lam_out, double* part, hipStream_t s) {
+    return fp32 ? fadj_step_t<float>(q, a, fn, lam_in, lam_out, part, s)
+                : fadj_step_t<double>(q, a, fn, lam_in, lam_out, part, s);
 }
 
 }  // namespace mkg

@@ -120,9 +120,10 @@ int launch_sum_partials(const double* part, uint32_t n, double* out, hipStream_t
 
 // Flow adjoint (fadj.hip): loss and final adjoint from the canonical FP64 populations fN
 // (partials 0..3 of an 8 x nblk array), and one reverse step over the tape fn (partials 1..7).
-int launch_fadj_loss(int q, const StepArgs& a, const double* fN, const double* ut, double* lam,
-                     double* part, hipStream_t s);
-int launch_fadj_step(int q, const StepArgs& a, const double* fn, const double* lam_in,
-                     double* lam_out, double* part, hipStream_t s);
+// The adjoint buffers hold the storage type (float when fp32).
+int launch_fadj_loss(int q, int fp32, const StepArgs& a, const double* fN, const double* ut,
+                     void* lam, double* part, hipStream_t s);
+int launch_fadj_step(int q, int fp32, const StepArgs& a, const double* fn, const void* lam_in,
+                     void* lam_out, double* part, hipStream_t s);
 
 }  // namespace mkg
