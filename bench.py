@@ -19,6 +19,11 @@ import sys
 import time
 from pathlib import Path
 
+# The CPU baselines run the OpenMP oracle with its threads pinned to neighbouring cores; the
+# OpenMP runtime reads these once, when the first library that uses it loads.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent
@@ -290,6 +295,21 @@ def mp_cost(size: int, steps: int) -> dict:
             "step_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def host_cpu() -> dict:
+    """The host the CPU baselines ran on: CPU model, CPUs this process may use, binding."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "cpus_available": len(os.sched_getaffinity(0)),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
 def cpu_baseline(seconds: float):
     """The reference-structured FP64 CPU path (oracle/mk_aos.c: AoS node records of the
     reference's 3640 B layout, one OpenMP sweep per operator) on a 256 x 256 x 16 slab of the
@@ -316,7 +336,7 @@ def cpu_baseline(seconds: float):
         n += 1
     dt = time.perf_counter() - t0
     return {"value": round(N * n_aos / dt_aos / 1e6, 3), "unit": "MLUPS", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
             "sample": f"reference-structured FP64 path (AoS 3640 B nodes, a sweep per "
                       f"operator), fused flow+thermal+phase step, {p.nx}x{p.ny}x{p.nz} slab of "
                       f"the 256^3 workload, {n_aos} steps in {dt_aos:.1f} s",
