@@ -1,0 +1,92 @@
+"""Size-independent properties of the multi-phase field and two-phase flow solvers at the
+256^3 size the benchmark runs (z-staged O steps, face-free blocks), where the oracle is too
+slow to follow: the sums the models conserve stay put, every value stays finite, and the O
+steps through LDS agree with the direct O steps to FP32 rounding at 128^3."""
+import numpy as np
+import pytest
+
+from materkit_anisotropiclbm_trt_rbc_facet_amd import cases, mkg
+
+pytestmark = pytest.mark.gpu
+
+
+def mpf_256(size=256):
+    m = mkg.default_mpf_params(nx=size, ny=size, nz=size, n=2, precision=mkg.PREC_FP32,
+                               thermal=1, alpha=0.1, latent_T=0.5)
+    m.u[1][0] = 0.02
+    m.kappa[0], m.kappa[1] = 0.0, 1.0
+    c = size / 2.0
+    phi = cases.mpf_seed_fields((size, size, size), [[c, c, c], [c, c - 16, c]], [8, 8], m.wdx)
+    return m, phi
+
+
+def test_multi_phase_field_at_256_conserves_and_stays_finite():
+    """Order parameter 0 has no interaction (kappa_0 = 0) and no growth: its sum is conserved
+    (periodic faces) to FP32 rounding over 60 steps; order parameter 1 moves at 0.02."""
+    print("mpf 256^3", flush=True)
+    m, phi = mpf_256()
+    dev = mkg.MpfSolver(m)
+    dev.init(phi)
+    s0 = dev.sums()
+    dev.step(60)
+    s1 = dev.sums()
+    ph = dev.get(mkg.MPF_PHI)
+    dev.close()
+    assert np.isfinite(ph).all()
+    N = 256 ** 3
+    assert abs(s1[0] - s0[0]) < 1e-5 * N  # FP32 storage: 1e-5 of the cell count
+    assert np.abs(ph).max() <= 1.0 + 1e-3
+
+
+def test_two_phase_flow_at_256_conserves_phi_and_stays_finite():
+    """A drop under gravity in a periodic 256^3 box: sum phi is conserved to FP32 rounding
+    over 60 steps, velocities stay finite and small."""
+    print("two-phase 256^3", flush=True)
+    size = 256
+    m = mkg.mp_params(nx=size, ny=size, nz=size, precision=mkg.PREC_FP32, rho_l=3.0, rho_g=1.0,
+                      nu_l=0.1, nu_g=0.05, sigma=0.005, width=4.0, mobility=0.05, tau_h=1.0)
+    m.gravity[2] = -1e-6
+    x = np.arange(size) - size / 2.0
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    phi = (0.5 + 0.5 * np.tanh(2 * (size / 4.0 - r) / 4.0)).reshape(-1)
+    del r
+    dev = mkg.MpSolver(m)
+    dev.init(phi)
+    s0 = dev.phi_sum()
+    dev.step(60)
+    s1 = dev.phi_sum()
+    u = dev.get(mkg.MP_VEL)
+    dev.close()
+    assert abs(s1 - s0) < 1e-5 * phi.size  # FP32 storage
+    assert np.isfinite(u).all() and np.abs(u).max() < 1e-2
+
+
+def test_z_stage_matches_direct_o_steps_at_128(monkeypatch):
+    """The O steps through LDS and the direct O steps of both solvers agree to FP32 rounding
+    after 9 steps at 128^3."""
+    size = 128
+    m, phi = mpf_256(size)
+    out = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MKG_ZSTAGE", flag)
+        dev = mkg.MpfSolver(m)
+        dev.init(phi)
+        dev.step(9)
+        out.append(dev.get(mkg.MPF_PHI))
+        dev.close()
+    assert np.abs(out[0] - out[1]).max() < 1e-5
+    mp = mkg.mp_params(nx=size, ny=size, nz=size, precision=mkg.PREC_FP32, rho_l=3.0,
+                       rho_g=1.0, nu_l=0.1, nu_g=0.05, sigma=0.005, width=4.0, mobility=0.05,
+                       tau_h=1.0)
+    x = np.arange(size) - size / 2.0
+    r = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    phi2 = (0.5 + 0.5 * np.tanh(2 * (size / 4.0 - r) / 4.0)).reshape(-1)
+    res = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("MKG_ZSTAGE", flag)
+        dev = mkg.MpSolver(mp)
+        dev.init(phi2)
+        dev.step(9)
+        res.append(dev.get(mkg.MP_PHI))
+        dev.close()
+    assert np.abs(res[0] - res[1]).max() < 1e-5
