@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -69,6 +70,137 @@ __global__ void __launch_bounds__(256) k_fvm_phase(const StepArgs a, const R* __
     const R S = phase_source<R>(P, U, an, (R)a.lambda, (R)a.pf_dt, (R)a.pf_wdx2, M, false, as);
     const R lap = P[22] + P[4] + P[16] + P[10] + P[14] + P[12] - (R)6 * p0;
     *slot_at(phin, c.id) = p0 + S + (R)(a.pf_dt * a.pf_wdx2) * lap;
+}
+
+// x cells per thread of the marching phase kernel
+constexpr uint32_t kFvmXC = 8;
+
+// Face flux J.n of the face of cell P (27-point layout) on side `side` of axis `ax`: the normal
+// difference and the transverse central differences averaged over the two cells, through
+// face_flux (the same arithmetic as phase_source).
+template <class R>
+__device__ inline R fvm_face(const R (&P)[27], int ax, int side, const Aniso<R>& an) {
+#define PP(dx, dy, dz) P[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
+    const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
+    R g[3];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        if (b == ax) {
+            g[b] = (R)side * (PP(o0, o1, o2) - PP(0, 0, 0));
+        } else {
+            const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+            g[b] = (R)0.25 * (PP(e0, e1, e2) - PP(-e0, -e1, -e2) + PP(o0 + e0, o1 + e1, o2 + e2) -
+                              PP(o0 - e0, o1 - e1, o2 - e2));
+        }
+    }
+#undef PP
+    const R M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    return face_flux<R>(g[0], g[1], g[2], ax, an, M, false);
+}
+
+// Marching form of k_fvm_phase for blocks that hold whole z rows (256 % nz == 0, ny a multiple
+// of the rows per block): each thread walks XC cells along x.  A face flux is evaluated once
+// for both of its cells where they share a thread or a block: the x face carries over in a
+// register, the z face passes through LDS (double-buffered, one barrier per plane); y faces
+// and the first x face of a walk are evaluated per cell, as are z faces at a non-periodic z
+// boundary.  The phi stencil rolls plane by plane (9 loads per cell instead of 19).  The
+// cell part (gradient, anisotropy, source) is phase_source's.
+template <class R, int XC>
+__global__ void __launch_bounds__(256) k_fvm_phase_march(const StepArgs a, const R* __restrict__ phi,
+                                                         const R* __restrict__ cc,
+                                                         R* __restrict__ phin) {
+    __shared__ R jz[2][256];
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = 256u / nz;                 // z rows per block
+    const uint32_t ygroups = ny / rows;
+    const uint32_t yg = blockIdx.x % ygroups, xc = blockIdx.x / ygroups;
+    const uint32_t y = yg * rows + tid / nz, z = tid % nz;
+    const uint32_t x0 = xc * XC;
+    const uint32_t xend = x0 + XC < nx ? x0 + XC : nx;
+    const uint32_t np = (uint32_t)a.g.npmask;
+    const bool zper = (np & (ZLO | ZHI)) == 0u;
+    const uint32_t rb = tid - z;  // first thread of this z row
+    // neighbour offsets in y and z (periodic faces wrap, others replicate the edge)
+    const uint32_t ylo = y == 0 ? ((np & YLO) ? 0u : ny - 1) : y - 1;
+    const uint32_t yhi = y == ny - 1 ? ((np & YHI) ? y : 0u) : y + 1;
+    const uint32_t zlo = z == 0 ? ((np & ZLO) ? 0u : nz - 1) : z - 1;
+    const uint32_t zhi = z == nz - 1 ? ((np & ZHI) ? z : 0u) : z + 1;
+    const uint32_t yo[3] = {ylo * nz, y * nz, yhi * nz}, zo[3] = {zlo, z, zhi};
+    const uint32_t pl = ny * nz;
+    auto xidx = [&](int xx) -> uint32_t {
+        if (xx < 0) return (np & XLO) ? 0u : nx - 1;
+        if (xx >= (int)nx) return (np & XHI) ? nx - 1 : 0u;
+        return (uint32_t)xx;
+    };
+    R Q[3][9];  // planes x-1, x, x+1: [(dy+1)*3 + (dz+1)]
+    auto load_plane = [&](R (&q)[9], int xx) {
+        const uint32_t b = xidx(xx) * pl;
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) q[dy * 3 + dz] = *slot_at(phi, b + yo[dy] + zo[dz]);
+    };
+    load_plane(Q[0], (int)x0 - 1);
+    load_plane(Q[1], (int)x0);
+    const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
+    R jx_carry = 0;
+    int buf = 0;
+    for (uint32_t x = x0; x < xend; x++) {
+        load_plane(Q[2], (int)x + 1);
+        R P[27];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            P[k] = Q[0][k];
+            P[9 + k] = Q[1][k];
+            P[18 + k] = Q[2][k];
+        }
+        const uint32_t id = x * pl + y * nz + z;
+        const R jzp = fvm_face<R>(P, 2, 1, an);
+        jz[buf][tid] = jzp;
+        const R jxp = fvm_face<R>(P, 0, 1, an);
+        const R jxm = x == x0 ? fvm_face<R>(P, 0, -1, an) : jx_carry;
+        const R jyp = fvm_face<R>(P, 1, 1, an);
+        const R jym = fvm_face<R>(P, 1, -1, an);
+        __syncthreads();
+        R jzm;
+        if (z == 0 && !zper)
+            jzm = fvm_face<R>(P, 2, -1, an);
+        else
+            jzm = jz[buf][rb + (z == 0 ? nz - 1 : z - 1)];
+        const R divJ = jxp - jxm + jyp - jym + jzp - jzm;
+        const R p0 = P[13];
+        const R lap = P[22] + P[4] + P[16] + P[10] + P[14] + P[12] - (R)6 * p0;
+        const R U = *slot_at(cc, id) / fvm_A<R>(a, p0);
+        // cell part of phase_source: LB-weighted cell gradient, anisotropy, source
+        R gr0 = 0, gr1 = 0, gr2 = 0;
+#pragma unroll
+        for (int i = 1; i < 19; i++) {
+            const R v = (R)(3.0 * wgt<19>(i)) * P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)];
+            if (cx(i)) gr0 += (R)cx(i) * v;
+            if (cy(i)) gr1 += (R)cy(i) * v;
+            if (cz(i)) gr2 += (R)cz(i) * v;
+        }
+        R as = 1;
+        const R gg = gr0 * gr0 + gr1 * gr1 + gr2 * gr2;
+        if (gg > (R)1e-30) {
+            const R gn = sqrt(gg);
+            R D0, D1, D2;
+            as = aniso_eval<R>(an, gr0 / gn, gr1 / gn, gr2 / gn, D0, D1, D2);
+        }
+        const R d = (R)1 - p0 * p0;
+        const R Qs = (p0 - (R)a.lambda * U * d) * d;
+        const R a2 = as * as;
+        const R S = (R)a.pf_dt / a2 * ((divJ - a2 * lap) * (R)a.pf_wdx2 + Qs);
+        *slot_at(phin, id) = p0 + S + (R)(a.pf_dt * a.pf_wdx2) * lap;
+        jx_carry = jxp;
+        buf ^= 1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            Q[0][k] = Q[1][k];
+            Q[1][k] = Q[2][k];
+        }
+    }
 }
 
 template <class R>
@@ -147,6 +279,7 @@ struct mkg_fvm_ctx {
     void* phi[2] = {nullptr, nullptr};
     void* c[2] = {nullptr, nullptr};
     int cur = 0;  // buffer of the current level
+    uint32_t march_blk = 0;   // blocks of the marching phase kernel (0: per-cell kernel)
     int8_t* flag = nullptr;   // all fluid: the reductions read flags
     double* scratch = nullptr;
     double* partials = nullptr;
@@ -255,6 +388,13 @@ int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
     h->N = (uint32_t)n;
     h->S = ((h->N + 127u) & ~127u) + 1024u;
     h->nblk = (h->N + 255u) / 256u;
+    // the marching phase kernel needs whole z rows per block
+    if (256u % (uint32_t)p->nz == 0u && (uint32_t)p->ny % (256u / (uint32_t)p->nz) == 0u) {
+        const uint32_t rows = 256u / (uint32_t)p->nz;
+        h->march_blk = (((uint32_t)p->nx + kFvmXC - 1) / kFvmXC) * ((uint32_t)p->ny / rows);
+    }
+    if (const char* env = std::getenv("MKG_FVM_MARCH"))
+        if (std::atoi(env) == 0) h->march_blk = 0;
     auto bad = [&](hipError_t e, const char* what) {
         fvm_destroy_impl(h);
         return ffail(std::string(what) + ": " + hipGetErrorString(e));
@@ -304,14 +444,20 @@ int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
         // with the phase held fixed both kernels see phi' = phi
         void* phin = evolve_phase ? h->phi[n] : h->phi[o];
         if (h->fp32) {
-            if (evolve_phase)
+            if (evolve_phase && h->march_blk)
+                k_fvm_phase_march<float, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
+                    a, (const float*)h->phi[o], (const float*)h->c[o], (float*)phin);
+            else if (evolve_phase)
                 k_fvm_phase<float><<<grid, block, 0, h->stream>>>(
                     a, (const float*)h->phi[o], (const float*)h->c[o], (float*)phin);
             k_fvm_solute<float><<<grid, block, 0, h->stream>>>(
                 a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
                 (float*)h->c[n]);
         } else {
-            if (evolve_phase)
+            if (evolve_phase && h->march_blk)
+                k_fvm_phase_march<double, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
+                    a, (const double*)h->phi[o], (const double*)h->c[o], (double*)phin);
+            else if (evolve_phase)
                 k_fvm_phase<double><<<grid, block, 0, h->stream>>>(
                     a, (const double*)h->phi[o], (const double*)h->c[o], (double*)phin);
             k_fvm_solute<double><<<grid, block, 0, h->stream>>>(
