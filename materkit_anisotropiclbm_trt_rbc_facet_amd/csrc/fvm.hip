@@ -11,6 +11,8 @@
 //                 J_out = -D q_f (U_nb - U) + s j_at,a; j_at = -a_t W/dx (1 + (1-k) U_f) dphi_f
 //                 g/|g| on the face (g: normal difference, transverse means of phi)
 // Both read the old fields and write the other buffer of a pair; c - phi/2 is conserved.
+// When a block can hold whole z rows, the marching forms (k_fvm_phase_march,
+// k_fvm_solute_march) evaluate each face once for both of its cells where they can.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -266,6 +268,165 @@ __global__ void __launch_bounds__(256) k_fvm_solute(const StepArgs a, const R* _
     *slot_at(cn, c.id) = *slot_at(cc, c.id) - out + (R)0.5 * d0;
 }
 
+// Outward solute flux of a cell through its interior face on side s of axis ax (the
+// k_fvm_solute form): diffusion -D q_f (U_n - U_0) and the anti-trapping current.
+template <class R>
+__device__ inline R sol_face(const StepArgs& a, const R (&P)[27], int ax, int s, R U0, R q0,
+                             R d0, R Un, R pn, R dn) {
+#define PP(dx, dy, dz) P[((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1)]
+    const R D = (R)a.D_l, atw = (R)a.at_w, km = (R)1 - (R)a.k_part;
+    const R p0 = PP(0, 0, 0);
+    const R qf = (R)0.5 * (q0 + fvm_q<R>(a, pn));
+    R J = -D * qf * (Un - U0);
+    if (atw != (R)0) {
+        const int o0 = ax == 0 ? s : 0, o1 = ax == 1 ? s : 0, o2 = ax == 2 ? s : 0;
+        R g[3];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            if (b == ax) {
+                g[b] = (R)s * (pn - p0);
+            } else {
+                const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                g[b] = (R)0.25 * (PP(e0, e1, e2) - PP(-e0, -e1, -e2) + PP(o0 + e0, o1 + e1, o2 + e2) -
+                                  PP(o0 - e0, o1 - e1, o2 - e2));
+            }
+        }
+        const R gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        if (gg > (R)1e-20) {
+            const R Uf = (R)0.5 * (U0 + Un);
+            const R df = (R)0.5 * (d0 + dn);
+            J += (R)s * (-atw * ((R)1 + km * Uf) * df * g[ax] / sqrt(gg));
+        }
+    }
+#undef PP
+    return J;
+}
+
+// Outward flux through a non-periodic solute face: Dirichlet fixes U at the face, the others
+// carry none.
+template <class R>
+__device__ inline R sol_wall(const StepArgs& a, int face, R U0, R q0) {
+    if (pick6(face, a.c.bc) == kBcDirichlet)
+        return (R)-2 * (R)a.D_l * q0 * ((R)pick6(face, a.c.fval) - U0);
+    return (R)0;
+}
+
+// Marching form of k_fvm_solute (same block shape as k_fvm_phase_march): the outward flux of
+// a face is the negative of its neighbour's, so each thread evaluates its high x, y, z faces,
+// takes the low x face from its previous cell and the low z face from LDS; y faces, the first
+// low x face of a walk and the faces on non-periodic boundaries are evaluated per cell.
+template <class R, int XC>
+__global__ void __launch_bounds__(256) k_fvm_solute_march(const StepArgs a, const R* __restrict__ phi,
+                                                          const R* __restrict__ phin,
+                                                          const R* __restrict__ cc,
+                                                          R* __restrict__ cn) {
+    __shared__ R jz[2][256];
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = 256u / nz;
+    const uint32_t ygroups = ny / rows;
+    const uint32_t yg = blockIdx.x % ygroups, xc = blockIdx.x / ygroups;
+    const uint32_t y = yg * rows + tid / nz, z = tid % nz;
+    const uint32_t x0 = xc * XC;
+    const uint32_t xend = x0 + XC < nx ? x0 + XC : nx;
+    const uint32_t npg = (uint32_t)a.g.npmask, npc = (uint32_t)a.c.npmask;
+    const uint32_t rb = tid - z;
+    // phi stencil offsets (phase faces: periodic wrap, others replicate the edge)
+    const uint32_t ylo = y == 0 ? ((npg & YLO) ? 0u : ny - 1) : y - 1;
+    const uint32_t yhi = y == ny - 1 ? ((npg & YHI) ? y : 0u) : y + 1;
+    const uint32_t zlo = z == 0 ? ((npg & ZLO) ? 0u : nz - 1) : z - 1;
+    const uint32_t zhi = z == nz - 1 ? ((npg & ZHI) ? z : 0u) : z + 1;
+    const uint32_t yo[3] = {ylo * nz, y * nz, yhi * nz}, zo[3] = {zlo, z, zhi};
+    const uint32_t pl = ny * nz;
+    auto xidx = [&](int xx) -> uint32_t {
+        if (xx < 0) return (npg & XLO) ? 0u : nx - 1;
+        if (xx >= (int)nx) return (npg & XHI) ? nx - 1 : 0u;
+        return (uint32_t)xx;
+    };
+    // solute neighbours (periodic axes are the same for both fields)
+    const uint32_t yw_lo = y == 0 ? ny - 1 : y - 1, yw_hi = y == ny - 1 ? 0u : y + 1;
+    const uint32_t zw_hi = z == nz - 1 ? 0u : z + 1;
+    const bool ylo_wall = y == 0 && (npc & YLO), yhi_wall = y == ny - 1 && (npc & YHI);
+    const bool zlo_wall = z == 0 && (npc & ZLO), zhi_wall = z == nz - 1 && (npc & ZHI);
+    R Q[3][9];
+    auto load_plane = [&](R (&q)[9], int xx) {
+        const uint32_t b = xidx(xx) * pl;
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dz = 0; dz < 3; dz++) q[dy * 3 + dz] = *slot_at(phi, b + yo[dy] + zo[dz]);
+    };
+    load_plane(Q[0], (int)x0 - 1);
+    load_plane(Q[1], (int)x0);
+    const uint32_t col = y * nz + z;
+    R c0 = *slot_at(cc, x0 * pl + col), pn0 = *slot_at(phin, x0 * pl + col);
+    R jx_carry = 0;
+    int buf = 0;
+    for (uint32_t x = x0; x < xend; x++) {
+        load_plane(Q[2], (int)x + 1);
+        R P[27];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            P[k] = Q[0][k];
+            P[9 + k] = Q[1][k];
+            P[18 + k] = Q[2][k];
+        }
+        const uint32_t id = x * pl + col;
+        const R p0 = P[13];
+        const R U0 = c0 / fvm_A<R>(a, p0);
+        const R q0 = fvm_q<R>(a, p0);
+        const R d0 = pn0 - p0;
+        auto interior = [&](int ax, int s, uint32_t nb, R pn) {
+            const R Un = *slot_at(cc, nb) / fvm_A<R>(a, pn);
+            return sol_face<R>(a, P, ax, s, U0, q0, d0, Un, pn, *slot_at(phin, nb) - pn);
+        };
+        // x+ face (and the next cell's values)
+        const bool xhi_wall = x == nx - 1 && (npc & XHI);
+        const uint32_t xn = x == nx - 1 ? 0u : x + 1;
+        const R c1 = *slot_at(cc, xn * pl + col), pn1 = *slot_at(phin, xn * pl + col);
+        R jxp;
+        if (xhi_wall) {
+            jxp = sol_wall<R>(a, 1, U0, q0);
+        } else {
+            const R pn = P[22];
+            const R Un = c1 / fvm_A<R>(a, pn);
+            jxp = sol_face<R>(a, P, 0, 1, U0, q0, d0, Un, pn, pn1 - pn);
+        }
+        // z+ face through LDS (the z neighbour's low face)
+        const R jzp = zhi_wall ? sol_wall<R>(a, 5, U0, q0)
+                               : interior(2, 1, x * pl + y * nz + zw_hi, P[14]);
+        jz[buf][tid] = jzp;
+        // x- face: the previous cell's x+ face, or evaluated at the start of the walk
+        R jxm;
+        if (x != x0) {
+            jxm = -jx_carry;
+        } else if (x == 0 && (npc & XLO)) {
+            jxm = sol_wall<R>(a, 0, U0, q0);
+        } else {
+            jxm = interior(0, -1, (x == 0 ? nx - 1 : x - 1) * pl + col, P[4]);
+        }
+        const R jyp = yhi_wall ? sol_wall<R>(a, 3, U0, q0) : interior(1, 1, x * pl + yw_hi * nz + z, P[16]);
+        const R jym = ylo_wall ? sol_wall<R>(a, 2, U0, q0) : interior(1, -1, x * pl + yw_lo * nz + z, P[10]);
+        __syncthreads();
+        R jzm;
+        if (zlo_wall)
+            jzm = sol_wall<R>(a, 4, U0, q0);
+        else
+            jzm = -jz[buf][rb + (z == 0 ? nz - 1 : z - 1)];
+        const R out = jxp + jxm + jyp + jym + jzp + jzm;
+        *slot_at(cn, id) = c0 - out + (R)0.5 * d0;
+        jx_carry = jxp;
+        c0 = c1;
+        pn0 = pn1;
+        buf ^= 1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            Q[0][k] = Q[1][k];
+            Q[1][k] = Q[2][k];
+        }
+    }
+}
+
 }  // namespace mkg
 
 using namespace mkg;
@@ -279,7 +440,7 @@ struct mkg_fvm_ctx {
     void* phi[2] = {nullptr, nullptr};
     void* c[2] = {nullptr, nullptr};
     int cur = 0;  // buffer of the current level
-    uint32_t march_blk = 0;   // blocks of the marching phase kernel (0: per-cell kernel)
+    uint32_t march_blk = 0;   // blocks of the marching kernels (0: the per-cell kernels)
     int8_t* flag = nullptr;   // all fluid: the reductions read flags
     double* scratch = nullptr;
     double* partials = nullptr;
@@ -388,7 +549,7 @@ int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
     h->N = (uint32_t)n;
     h->S = ((h->N + 127u) & ~127u) + 1024u;
     h->nblk = (h->N + 255u) / 256u;
-    // the marching phase kernel needs whole z rows per block
+    // the marching kernels need whole z rows per block
     if (256u % (uint32_t)p->nz == 0u && (uint32_t)p->ny % (256u / (uint32_t)p->nz) == 0u) {
         const uint32_t rows = 256u / (uint32_t)p->nz;
         h->march_blk = (((uint32_t)p->nx + kFvmXC - 1) / kFvmXC) * ((uint32_t)p->ny / rows);
@@ -450,9 +611,14 @@ int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
             else if (evolve_phase)
                 k_fvm_phase<float><<<grid, block, 0, h->stream>>>(
                     a, (const float*)h->phi[o], (const float*)h->c[o], (float*)phin);
-            k_fvm_solute<float><<<grid, block, 0, h->stream>>>(
-                a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
-                (float*)h->c[n]);
+            if (h->march_blk)
+                k_fvm_solute_march<float, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
+                    a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
+                    (float*)h->c[n]);
+            else
+                k_fvm_solute<float><<<grid, block, 0, h->stream>>>(
+                    a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
+                    (float*)h->c[n]);
         } else {
             if (evolve_phase && h->march_blk)
                 k_fvm_phase_march<double, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
@@ -460,9 +626,14 @@ int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
             else if (evolve_phase)
                 k_fvm_phase<double><<<grid, block, 0, h->stream>>>(
                     a, (const double*)h->phi[o], (const double*)h->c[o], (double*)phin);
-            k_fvm_solute<double><<<grid, block, 0, h->stream>>>(
-                a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
-                (double*)h->c[n]);
+            if (h->march_blk)
+                k_fvm_solute_march<double, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
+                    a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
+                    (double*)h->c[n]);
+            else
+                k_fvm_solute<double><<<grid, block, 0, h->stream>>>(
+                    a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
+                    (double*)h->c[n]);
         }
         if (!evolve_phase)
             FCHK(hipMemcpyAsync(h->phi[n], h->phi[o], (size_t)h->N * h->esz,

@@ -94,7 +94,7 @@ def test_fvm_rejects_unsupported_parameters():
 @pytest.mark.parametrize("prec,tol", [(mkg.PREC_FP64, 1e-13), (mkg.PREC_FP32, 1e-6)],
                          ids=["fp64", "fp32"])
 @pytest.mark.parametrize("name", ["march_periodic", "march_faces"])
-def test_fvm_marching_phase_kernel_agrees_with_per_cell_kernel(name, prec, tol, monkeypatch):
+def test_fvm_marching_kernels_agree_with_per_cell_kernels(name, prec, tol, monkeypatch):
     """A face flux evaluated once for both cells equals the two per-cell evaluations up to the
     order of the transverse sums: the fields agree to rounding after 6 steps."""
     p = make(name, prec)
