@@ -5,7 +5,7 @@
 // order, plus a fused loop.  The case file and the loop are chosen on the command line
 // (the reference hard-codes choice = 8, MaterKit.cpp:50).
 //
-//   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk]
+//   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk | plt]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
 //         solutal_dendrite | solutal_only | rbc | allen_cahn | multi_phase_field |
 //         multiphase | wetting | fused
@@ -124,6 +124,8 @@ int main(int argc, char** argv) {
         if (argc > 3) maxLoops = std::atoi(argv[3]);
         if (argc > 4) maxFrame = std::atoi(argv[4]);
         const bool vtk = argc > 5 && std::strcmp(argv[5], "vtk") == 0;
+        // "plt": Tecplot frames (every cell) and the mid-z plane as VTK and Tecplot
+        const bool plt = argc > 5 && std::strcmp(argv[5], "plt") == 0;
         if (maxFrame < 1) maxFrame = 1;
         if (loop == "multi_phase_field") return run_multi_phase_field(xml, maxLoops, maxFrame, vtk);
         if (loop == "multiphase") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk);
@@ -144,6 +146,11 @@ int main(int argc, char** argv) {
             if (loops % every == 0) {
                 frame_info(flow, frame, loops);
                 if (vtk) flow.write_vtkfile(loops);
+                if (plt) {
+                    flow.write_pltfile(frame);
+                    flow.write_pltfile_2D(loops, 'z');
+                    flow.write_vtkfile_2D(loops);
+                }
                 frame++;
             }
             if (loops == maxLoops) break;

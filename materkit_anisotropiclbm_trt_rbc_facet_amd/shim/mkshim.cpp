@@ -233,6 +233,100 @@ void mkFluidsLBM::check_variables(int loops, const char nPlane) {
                     s[0] / cnt, s[1] / cnt, s[2] / cnt);
 }
 
+void mkFluidsLBM::read_parameters(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_params_from_xml(xmlfname.c_str(), &p_, run_info_)) throw_io("read_parameters");
+}
+
+// Fields of the frame writers: name and values (N each; velocity as three components).
+static std::vector<std::pair<std::string, std::vector<double>>> frame_fields(
+    const mkFluidsLBM& f, const mkg_params& p) {
+    std::vector<std::pair<std::string, std::vector<double>>> out;
+    const size_t N = (size_t)p.nx * p.ny * p.nz;
+    if (p.flow_q) {
+        std::vector<double> r = f.field(MKG_F_RHO);
+        for (double& v : r) v -= 1.0;
+        out.emplace_back("density_difference", std::move(r));
+    }
+    if (p.phase_q) out.emplace_back("order_parameter", f.field(MKG_F_PHI));
+    if (p.solute) out.emplace_back("X0_solute", f.field(MKG_F_CONC));
+    if (p.thermal) out.emplace_back("temperature", f.field(MKG_F_TEMP));
+    if (p.flow_q) {
+        std::vector<double> u = f.field(MKG_F_VEL), c[3];
+        for (int k = 0; k < 3; k++) {
+            c[k].resize(N);
+            for (size_t i = 0; i < N; i++) c[k][i] = u[3 * i + k];
+        }
+        out.emplace_back("u", std::move(c[0]));
+        out.emplace_back("v", std::move(c[1]));
+        out.emplace_back("w", std::move(c[2]));
+    }
+    return out;
+}
+
+// Tecplot ASCII zone over the box [lo, hi) of cells, x fastest.
+static void write_plt(const std::string& path, const mkg_params& p,
+                      const std::vector<std::pair<std::string, std::vector<double>>>& fields,
+                      const int (&lo)[3], const int (&hi)[3]) {
+    std::ofstream o(path);
+    if (!o) throw mkt::mkIOException("cannot write " + path);
+    o << "TITLE = \"fluid_state\"\nVARIABLES = \"x\", \"y\", \"z\"";
+    for (const auto& f : fields) o << ", \"" << f.first << "\"";
+    o << "\nZONE I=" << hi[0] - lo[0] << ", J=" << hi[1] - lo[1] << ", K=" << hi[2] - lo[2]
+      << ", F=POINT\n" << std::setprecision(9);
+    for (int z = lo[2]; z < hi[2]; z++)
+        for (int y = lo[1]; y < hi[1]; y++)
+            for (int x = lo[0]; x < hi[0]; x++) {
+                const size_t id = ((size_t)x * p.ny + y) * p.nz + z;
+                o << x + 0.5 << " " << y + 0.5 << " " << z + 0.5;
+                for (const auto& f : fields) o << " " << f.second[id];
+                o << "\n";
+            }
+}
+
+void mkFluidsLBM::write_pltfile(int frame) {
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "plt", ec);
+    const int lo[3] = {0, 0, 0}, hi[3] = {p_.nx, p_.ny, p_.nz};
+    write_plt(workspace_ + "plt/fluid_" + std::to_string(frame) + ".plt", p_, frame_fields(*this, p_),
+              lo, hi);
+}
+
+void mkFluidsLBM::write_pltfile_2D(int loops, char nPlane) {
+    const int ax = nPlane == 'x' ? 0 : (nPlane == 'y' ? 1 : (nPlane == 'z' ? 2 : -1));
+    if (ax < 0) throw mkt::mkLogicException("write_pltfile_2D: nPlane is 'x', 'y' or 'z'");
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "plt", ec);
+    int lo[3] = {0, 0, 0}, hi[3] = {p_.nx, p_.ny, p_.nz};
+    lo[ax] = hi[ax] / 2;
+    hi[ax] = lo[ax] + 1;
+    write_plt(workspace_ + "plt/fluid2D_" + std::string(1, nPlane) + "_t" + std::to_string(loops) +
+                  ".plt",
+              p_, frame_fields(*this, p_), lo, hi);
+}
+
+void mkFluidsLBM::write_vtkfile_2D(int loops) {
+    const int Mx = p_.nx, My = p_.ny, z0 = p_.nz / 2;
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "vtk", ec);
+    const std::string fn = workspace_ + "vtk/fluid2D_t" + std::to_string(loops) + ".vtk";
+    std::ofstream o(fn);
+    if (!o) throw mkt::mkIOException("cannot write " + fn);
+    o << "# vtk DataFile Version 3.0\nfluid_state\nASCII\nDATASET RECTILINEAR_GRID\n";
+    o << "DIMENSIONS " << Mx << " " << My << " 1\n";
+    o << "X_COORDINATES " << Mx << " float\n";
+    for (int i = 0; i < Mx; i++) o << i + 0.5 << " ";
+    o << "\nY_COORDINATES " << My << " float\n";
+    for (int i = 0; i < My; i++) o << i + 0.5 << " ";
+    o << "\nZ_COORDINATES 1 float\n" << z0 + 0.5 << "\n";
+    o << "POINT_DATA " << (size_t)Mx * My << "\n" << std::setprecision(9);
+    for (const auto& f : frame_fields(*this, p_)) {
+        o << "SCALARS " << f.first << " float 1\nLOOKUP_TABLE default\n";
+        for (int y = 0; y < My; y++)
+            for (int x = 0; x < Mx; x++) o << f.second[((size_t)x * My + y) * p_.nz + z0] << "\n";
+    }
+}
+
 void mkFluidsLBM::write_vtkfile(int loops) {
     const int Mx = p_.nx, My = p_.ny, Mz = p_.nz;
     const size_t N = (size_t)Mx * My * Mz;

@@ -77,6 +77,12 @@ public:
     virtual void boundary_walls() {}
     virtual void bdyPeriod() {}
     virtual void bdyVelocity() {}
+    virtual void bdyCouette() {}  // moving walls are the case file's face velocities
+    // walls whose velocity varies along the wall have no counterpart (uniform face velocities)
+    virtual void bdyRotation() { throw mkt::mkLogicException("bdyRotation: no rotating walls"); }
+    virtual void bdySpiral() { throw mkt::mkLogicException("bdySpiral: no spiral walls"); }
+    // parameters of a case file without creating the device state (initialize does both)
+    virtual void read_parameters(const std::string& xmlfname);
 
     virtual double Thermal_NusseltRBC();
     virtual double Solutal_NusseltRBC();
@@ -84,6 +90,12 @@ public:
     // legacy VTK frame (RECTILINEAR_GRID, cell centres x+0.5, x fastest), file
     // <workspace>vtk/fluid_t<loops>.vtk as read by the reference's SaveContourBatch.py:41
     virtual void write_vtkfile(int loops);
+    // the mid-z plane as a legacy VTK frame, <workspace>vtk/fluid2D_t<loops>.vtk
+    virtual void write_vtkfile_2D(int loops);
+    // Tecplot ASCII (POINT) frames of the same fields: every cell, or the mid plane normal to
+    // nPlane ('x', 'y', 'z'); <workspace>plt/fluid_<frame>.plt, fluid2D_<nPlane>_t<loops>.plt
+    virtual void write_pltfile(int frame);
+    virtual void write_pltfile_2D(int loops, char nPlane);
 
     // fused steps of every field
     void step(int nsteps = 1);

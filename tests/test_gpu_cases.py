@@ -423,3 +423,33 @@ def test_reference_api_driver_runs_the_solutal_loops(tmp_path):
     f.close()
     # the frames hold 9 significant digits
     assert np.abs(c - c1).max() < 1e-7
+
+
+def test_reference_api_driver_writes_tecplot_and_plane_frames(tmp_path):
+    """mkrun with 'plt': write_pltfile (every cell), write_pltfile_2D and write_vtkfile_2D
+    (mid-z plane) hold the same values as the full VTK frame of the same state."""
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    xml = tmp_path / "c3small.xml"
+    src = (ROOT / "cases" / "C3_free_dendrite.xml").read_text()
+    xml.write_text(src.replace("256   256   256", "16 12 10").replace("127.5 127.5 127.5",
+                                                                      "7.5 5.5 4.5"))
+    for opt in ("vtk", "plt"):
+        res = subprocess.run([str(exe), str(xml), "thermal_dendrite", "20", "1", opt],
+                             capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert res.returncode == 0, res.stderr
+    full = (tmp_path / "vtk" / "fluid_t20.vtk").read_text().split("\n")
+    k = full.index("SCALARS order_parameter float 1") + 2
+    phi = np.array([float(v) for v in full[k:k + 16 * 12 * 10]]).reshape(10, 12, 16)  # z, y, x
+    plane = (tmp_path / "vtk" / "fluid2D_t20.vtk").read_text().split("\n")
+    k = plane.index("SCALARS order_parameter float 1") + 2
+    mid = np.array([float(v) for v in plane[k:k + 16 * 12]]).reshape(12, 16)
+    np.testing.assert_allclose(mid, phi[5], atol=1e-8)
+    plt = (tmp_path / "plt" / "fluid_1.plt").read_text().split("\n")
+    assert plt[1].startswith('VARIABLES = "x", "y", "z", "order_parameter", "temperature"')
+    assert plt[2].startswith("ZONE I=16, J=12, K=10")
+    vals = np.array([[float(v) for v in line.split()] for line in plt[3:3 + 16 * 12 * 10]])
+    np.testing.assert_allclose(vals[:, 3].reshape(10, 12, 16), phi, atol=1e-8)
+    p2 = (tmp_path / "plt" / "fluid2D_z_t20.plt").read_text().split("\n")
+    assert p2[2].startswith("ZONE I=16, J=12, K=1")
+    v2 = np.array([[float(v) for v in line.split()] for line in p2[3:3 + 16 * 12]])
+    np.testing.assert_allclose(v2[:, 3].reshape(12, 16), phi[5], atol=1e-8)

@@ -258,6 +258,14 @@ static void loops(mkFluidsMRT& flow, mkFluidsUser& user, mkFluidsMultiphase& two
     flow.check_variables(loops, 'x');
     (void)flow.Thermal_NusseltRBC();
     flow.write_vtkfile(loops);
+    flow.write_vtkfile_2D(loops);
+    flow.write_pltfile(loops);
+    flow.write_pltfile_2D(loops, 'y');
+    flow.read_parameters(xml);
+    flow.bdyPeriod();
+    flow.bdyCouette();
+    flow.boundary();
+    flow.boundary_walls();
     user.initialize(xml);
     user.initialize_geometry();
     user.propagate();
