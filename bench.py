@@ -23,6 +23,8 @@ from pathlib import Path
 # OpenMP runtime reads these once, when the first library that uses it loads.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# CPUs this process may use, read before the GPU runtime pins the main thread
+HOST_CPUS = len(os.sched_getaffinity(0))
 
 import numpy as np
 
@@ -305,7 +307,7 @@ def host_cpu() -> dict:
                 break
     except OSError:
         pass
-    return {"model": model, "cpus_available": len(os.sched_getaffinity(0)),
+    return {"model": model, "cpus_available": HOST_CPUS,
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "omp_places": os.environ.get("OMP_PLACES")}
 
