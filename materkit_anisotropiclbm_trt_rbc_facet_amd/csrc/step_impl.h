@@ -80,9 +80,9 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
 }
 
 // Minimum waves per SIMD requested from the register allocator for the fused kernel.  FP32:
-// four (at most 128 VGPRs).  Left alone, the allocator gives the z-staged O step 136 VGPRs and
-// three waves; with four it spills (72 bytes of scratch per lane, about 6 B per cell of extra
-// HBM writes at 256^3) and the step runs 7 % faster.
+// four (at most 128 VGPRs).  Left alone, the allocator gives the z-staged O step holding both
+// bodies 136 VGPRs and three waves; with four it spills (72 bytes of scratch per lane) and the
+// step runs 7 % faster.  The all-periodic variant (PER) needs 115 VGPRs and does not spill.
 // FP64 kernels need about 230 VGPRs (two waves); a bound there only adds spills.
 #ifndef MKG_STEP_WAVES_F32
 #define MKG_STEP_WAVES_F32 4
@@ -104,6 +104,9 @@ constexpr int zslots_of() {
     return Q == 19 ? nzdirs<19>() : (Q == 27 ? nzdirs<27>() : (Q == 7 ? nzdirs<7>() : 1));
 }
 
+// PER: every face of every lattice periodic, no wall cells, and (with ZS) no thread past the
+// last cell: the kernel holds only the body without face rules, which the register allocator
+// then fits in 128 VGPRs with fewer spills than the kernel holding both bodies.
 // ZS: O step with the z stage (only instantiated with ODD; the block must hold whole z rows).
 // With it every thread stays to the end (barriers), and act marks the threads that update.
 // The z stage runs in three block barriers:
@@ -112,7 +115,7 @@ constexpr int zslots_of() {
 //       issued after the phase source, so that the stencil values are dead by then)
 //   B3  after the phase push stage, which reuses the flow pull region (last read before B2)
 // Every other stage region is written once and read once, on either side of one barrier.
-template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false>
+template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     k_step(const StepArgs a) {
     constexpr bool Z = ODD && ZS;
@@ -130,8 +133,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     // Blocks (z stage) or waves (otherwise) none of whose cells lies on a non-periodic face,
     // in a domain without wall cells, take the body without face rules (FACES false); a z
     // stage block must also lie wholly before the last cell, so that every thread updates.
-    bool slow = a.has_walls != 0;
-    if (!slow) {
+    bool slow = !PER && a.has_walls != 0;
+    if (!PER && !slow) {
         const uint32_t t = thread_cell(a);
         Cell e;
         locate_at(a, t < a.N ? t : a.N - 1u, e);
@@ -426,7 +429,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         }
     }
     };
-    if (slow)
+    if constexpr (PER)
+        body(std::false_type{});
+    else if (slow)
         body(std::true_type{});
     else
         body(std::false_type{});
@@ -436,12 +441,23 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
 template <class R, int FQ, int PQ, bool TH, bool SO>
 inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
-    if (odd && a.zstage)
-        k_step<R, FQ, PQ, TH, SO, true, true><<<grid, block, 0, s>>>(a);
-    else if (odd)
-        k_step<R, FQ, PQ, TH, SO, true><<<grid, block, 0, s>>>(a);
-    else
-        k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);
+    const bool per = !a.has_walls && (a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask) == 0;
+    if (odd && a.zstage) {
+        if (per && a.N == a.nblk * 256u)
+            k_step<R, FQ, PQ, TH, SO, true, true, true><<<grid, block, 0, s>>>(a);
+        else
+            k_step<R, FQ, PQ, TH, SO, true, true><<<grid, block, 0, s>>>(a);
+    } else if (odd) {
+        if (per)
+            k_step<R, FQ, PQ, TH, SO, true, false, true><<<grid, block, 0, s>>>(a);
+        else
+            k_step<R, FQ, PQ, TH, SO, true><<<grid, block, 0, s>>>(a);
+    } else {
+        if (per)
+            k_step<R, FQ, PQ, TH, SO, false, false, true><<<grid, block, 0, s>>>(a);
+        else
+            k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);
+    }
 }
 
 template <class R, int PQ>
