@@ -225,6 +225,21 @@ def test_fp64_z_stage_with_two_rows_per_block_matches_oracle(case, bcs):
         assert e < 1e-12, f"{k}: rel err {e:.3e}"
 
 
+# All faces periodic, no wall cells and a grid of whole blocks (4 x 4 x 64 = 1024 cells):
+# every launch takes the kernel variant compiled with only the face-free body, E and O steps
+# alike (the z-staged O step included).
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("prec", [mkg.PREC_FP64, mkg.PREC_FP32], ids=["fp64", "fp32"])
+def test_all_periodic_variant_matches_oracle(case, prec):
+    errs, fd, fo, dev, orc = _run(case, "periodic", 6, prec=prec, dims=(4, 4, 64))
+    dev.close()
+    if prec == mkg.PREC_FP64:  # FP32 flags may flip where phi sits on a threshold
+        assert np.array_equal(fd, fo), "flags differ"
+    tol = 1e-12 if prec == mkg.PREC_FP64 else 1e-5
+    for k, e in errs.items():
+        assert e < tol, f"{k}: rel err {e:.3e}"
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
 def test_z_stage_agrees_with_direct_o_steps(case, monkeypatch):
     """The z stage changes only how populations move: FP32 results with and without it agree
