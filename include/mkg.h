@@ -15,9 +15,10 @@
  *                            readParameters (MaterKit/inc/mkFunction.h:62)
  *   mkg_create            <- mkFluidsMRT(const mkSpace&) + node[] allocation
  *                            (MaterKit/inc/mkFluidsMRT.h:37, mkFluidsLBM.h:95)
- *   mkg_init_geometry     <- initialize_geometry / initialize_geometry_Poiseuille /
- *                            initialize_seeds (mkFluidsMRT.h:40, mkFlowField.h:33,
- *                            mkMultiphaseField.h:52, MaterKit/src/mkFluidsUser.cpp:44-66)
+ *   mkg_init_seeds,       <- initialize_geometry / initialize_geometry_Poiseuille /
+ *   mkg_init_equilibrium,    initialize_seeds (mkFluidsMRT.h:40, mkFlowField.h:33,
+ *   mkg_set_field(FLAG)      mkMultiphaseField.h:52, MaterKit/src/mkFluidsUser.cpp:44-66):
+ *                            seed spheres, equilibrium start, wall cells
  *   mkg_step (mask)       <- per-step call sequence propagate()/propagate_PBB()/relaxation()
  *                            of every field (MaterKit/MaterKit.cpp:277-280, 500-518,
  *                            775-781; mkFluidsLBM.h:47-49; mkPhaseField.h:41-48;

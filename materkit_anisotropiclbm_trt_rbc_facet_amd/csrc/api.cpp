@@ -791,8 +791,13 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
         std::fclose(fp);
         return fail("checkpoint header does not match this handle");
     }
+    if (p.nx != h->p.nx || p.ny != h->p.ny || p.nz != h->p.nz) {
+        std::fclose(fp);
+        return fail("checkpoint grid does not match this handle");
+    }
     if (p.flow_q != h->p.flow_q || p.phase_q != h->p.phase_q || p.thermal != h->p.thermal ||
-        p.solute != h->p.solute || (p.has_kappa != 0) != (h->kappa != nullptr)) {
+        p.solute != h->p.solute || p.precision != h->p.precision ||
+        (p.has_kappa != 0) != (h->kappa != nullptr)) {
         std::fclose(fp);
         return fail("checkpoint lattices do not match this handle");
     }
@@ -805,10 +810,18 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
     };
     for (int L = 0; L < 4 && ok; L++) ok = load(h->pop[L], h->pop_elems(L) * h->esz);
     if (ok && h->q[LG]) ok = load(h->phi[0], h->S * h->esz) && load(h->phi[1], h->S * h->esz);
-    if (ok) ok = load(h->flag, h->S) && load(h->grain, (size_t)h->S * 2);
+    // the flags decide which kernel bodies run (wall cells need the body with face rules)
+    bool walls = false;
+    if (ok) {
+        ok = load(h->flag, h->S);
+        for (uint32_t x = 0; ok && x < h->N; x++)
+            if ((int8_t)buf[x] == MKG_IS_WALLS) walls = true;
+    }
+    if (ok) ok = load(h->grain, (size_t)h->S * 2);
     if (ok && h->kappa) ok = load(h->kappa, h->S * h->esz);
     std::fclose(fp);
     if (!ok) return fail("checkpoint body truncated");
+    h->has_walls = walls ? 1 : 0;
     h->p = p;
     std::memcpy(h->lay, lay, sizeof(lay));
     std::memcpy(h->steps, steps, sizeof(steps));
@@ -921,7 +934,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     // ---- device storage
     struct Ck {
         int lay_g, lay_h;
-        int64_t pf;
+        int64_t pf, steps_g, steps_h;
     };
     std::vector<Ck> ck(tape ? 1 : nseg + 1);
     const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH);
@@ -978,7 +991,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         HIPCHK(hipMemcpyAsync(base + PG * e, h->pop[LH], PH * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + (PG + PH) * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + (PG + PH + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
-        ck[k] = {h->lay[LG], h->lay[LH], h->pf_count};
+        ck[k] = {h->lay[LG], h->lay[LH], h->pf_count, h->steps[LG], h->steps[LH]};
         return 0;
     };
     auto load_ck = [&](int k) -> int {
@@ -990,6 +1003,8 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         h->lay[LG] = ck[k].lay_g;
         h->lay[LH] = ck[k].lay_h;
         h->pf_count = ck[k].pf;
+        h->steps[LG] = ck[k].steps_g;
+        h->steps[LH] = ck[k].steps_h;
         return 0;
     };
     hipEvent_t ev[4];

@@ -34,6 +34,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Planted mistakes.  The default build has ORC_MUTANT 0.  The CPU tests build copies of this
+ * file with ORC_MUTANT set to one of the values below (oracle.planted) and check that the pin
+ * written for that term fails on the copy:
+ *   1  D3Q7 thermal advective equilibrium with cs^2 = 1/3 instead of 1/4
+ *   2  latent-heat source with its sign flipped
+ *   3  latent-heat source with twice its factor
+ *   4  Boussinesq force with twice its factor
+ *   5  Boussinesq force with its sign flipped
+ *   6  D3Q7 solute advective equilibrium with cs^2 = 1/3
+ *   7  solute rejection source with its sign flipped */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------ lattices */
 /* Direction order: rest, 6 faces (+x,-x,+y,-y,+z,-z), 12 edges, 8 corners; opposite pairs
  * are (2k-1, 2k).  x is the slowest axis of the linear index (x*ny + y)*nz + z. */
@@ -472,6 +486,22 @@ typedef struct {
     double *rho, *u, *T, *C, *U, *ns;
 } macro_t;
 
+/* Body force density on the flow: the constant force plus the Boussinesq buoyancy
+ * rho0 * buoy * (T - T_ref), buoy = -beta_T g (mk_Gravity, mk_beta_T,
+ * MaterKit/inc/mkFluidsLBM.h:123-131). */
+static void body_force(const orc_params* p, double T, double* F) {
+    double b = p->rho0 * (T - p->T_ref);
+#if ORC_MUTANT == 4
+    b *= 2.0;
+#elif ORC_MUTANT == 5
+    b = -b;
+#endif
+    for (int a = 0; a < 3; a++) {
+        F[a] = p->force[a];
+        if (p->thermal) F[a] += p->buoy[a] * b;
+    }
+}
+
 static void compute_macro(const orc_params* p, const orc_state* s, macro_t* m) {
     long N = NCELL(p);
     lat_t Lf = make_lat(p->flow_q ? p->flow_q : 19);
@@ -504,10 +534,7 @@ static void compute_macro(const orc_params* p, const orc_state* s, macro_t* m) {
                 for (int a = 0; a < 3; a++) j[a] += E27[i][a] * fi;
             }
             double F[3];
-            for (int a = 0; a < 3; a++) {
-                F[a] = p->force[a];
-                if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (m->T[x] - p->T_ref);
-            }
+            body_force(p, m->T[x], F);
             m->rho[x] = rho;
             for (int a = 0; a < 3; a++) m->u[3 * x + a] = (j[a] + 0.5 * F[a]) / flow_den(p, rho);
         } else {
@@ -653,10 +680,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             double fin[27], fo[27], F[3];
             if (s->flag[x] == 3) continue;
             for (int i = 0; i < q; i++) fin[i] = s->f[i * N + x];
-            for (int a = 0; a < 3; a++) {
-                F[a] = p->force[a];
-                if (p->thermal) F[a] += p->rho0 * p->buoy[a] * (m.T[x] - p->T_ref);
-            }
+            body_force(p, m.T[x], F);
             orc_collide_cell(p, op, fin, F, fo, m.ns[x]);
             for (int i = 0; i < q; i++) post[i * N + x] = fo[i];
         }
@@ -746,11 +770,23 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
             double T = m.T[x];
             const double* u = uadv + 3 * x;
+            /* heq = w T (1 + c.u / cs^2) with the D3Q7 cs^2 = 1/4 (sum_i w_i c_i c_i = I/4),
+             * whose first moment is T u; latent source w_i L dphi, L = latent_T
+             * (mkThermalField::relaxation(flow), MaterKit/inc/mkThermalField.h:33-36;
+             * mkNode::temperature_collision, MaterKit/inc/mkNode.h:156) */
+            double cs2_adv_T = L.cs2, latT = p->latent_T;
+#if ORC_MUTANT == 1
+            cs2_adv_T = 1.0 / 3.0;
+#elif ORC_MUTANT == 2
+            latT = -latT;
+#elif ORC_MUTANT == 3
+            latT = 2.0 * latT;
+#endif
             for (int i = 0; i < 7; i++) {
                 double cu = E27[i][0] * u[0] + E27[i][1] * u[1] + E27[i][2] * u[2];
-                double heq = L.w[i] * T * (1.0 + cu / L.cs2);
+                double heq = L.w[i] * T * (1.0 + cu / cs2_adv_T);
                 double hi = s->h[i * N + x];
-                post[i * N + x] = hi - om * (hi - heq) + L.w[i] * p->latent_T * dphi;
+                post[i * N + x] = hi - om * (hi - heq) + L.w[i] * latT * dphi;
             }
         }
         stream(p, s, 7, p->bc_h, 1, p->T_face, NULL, NULL, NULL, post, s->h, fresh);
@@ -802,11 +838,17 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
                 double jat = -atw * (1.0 + (1.0 - p->k_part) * m.U[x]) * dphi / sqrt(gg);
                 for (int a = 0; a < 3; a++) j[a] += jat * gphi[a];
             }
+            double cs2_adv_C = L.cs2, latC = p->latent_C;
+#if ORC_MUTANT == 6
+            cs2_adv_C = 1.0 / 3.0;
+#elif ORC_MUTANT == 7
+            latC = -latC;
+#endif
             for (int i = 0; i < 7; i++) {
                 double cj = E27[i][0] * j[0] + E27[i][1] * j[1] + E27[i][2] * j[2];
-                double ceq = L.w[i] * (C + cj / L.cs2);
+                double ceq = L.w[i] * (C + cj / cs2_adv_C);
                 double ci = s->c[i * N + x];
-                post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * p->latent_C * dphi;
+                post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * latC * dphi;
             }
         }
         /* solute Dirichlet faces fix U: c_w = A(phi) U_w */

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import subprocess
+from contextlib import contextmanager
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -38,45 +39,81 @@ def lib() -> ct.CDLL:
     global _lib
     if _lib is None:
         build()
-        L = ct.CDLL(str(LIB))
-        vp = ct.c_void_p
-        L.orc_params_size.restype = ct.c_size_t
-        L.orc_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.c_int]
-        L.orc_step.restype = ct.c_int
-        L.orc_aos_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.POINTER(ct.c_double)]
-        L.orc_aos_step.restype = ct.c_int
-        L.orc_moments.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.orc_init_equilibrium.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.orc_init_seeds.argtypes = [vp, vp, ct.c_int, vp, vp]
-        L.orc_equilibrium.argtypes = [ct.c_int, ct.c_double, vp, vp]
-        L.orc_equilibrium_he.argtypes = [ct.c_int, ct.c_double, ct.c_double, vp, vp]
-        L.orc_collide_cell.argtypes = [vp, ct.c_int, vp, vp, vp, ct.c_double]
-        L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
-        L.orc_face_flux.argtypes = [vp, vp, vp, ct.c_int]
-        L.orc_face_flux.restype = ct.c_double
-        L.orc_aniso.argtypes = [vp, vp, vp]
-        L.orc_aniso.restype = ct.c_double
-        L.orc_lattice.argtypes = [ct.c_int, vp, vp, vp, vp]
-        L.orc_lattice.restype = ct.c_int
-        L.orc_splitmix64.argtypes = [ct.POINTER(ct.c_uint64)]
-        L.orc_splitmix64.restype = ct.c_uint64
-        L.orc_uniform.argtypes = [ct.POINTER(ct.c_uint64)]
-        L.orc_uniform.restype = ct.c_double
-        L.orc_cell_uniform.argtypes = [ct.c_uint64, ct.c_uint64]
-        L.orc_cell_uniform.restype = ct.c_double
-        L.orc_mp_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.orc_mp_step.argtypes = [vp, vp, vp, vp, ct.c_int]
-        L.orc_mp_step.restype = ct.c_int
-        L.orc_mp_macro.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
-        L.orc_fvm_step.restype = ct.c_int
-        L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-        L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, ct.c_int]
-        L.orc_mpf_step.restype = ct.c_int
-        L.orc_flow_adjoint.argtypes = [vp, vp, vp, ct.c_int, vp, vp, vp]
-        L.orc_flow_adjoint.restype = ct.c_int
-        _lib = L
+        _lib = _bind(ct.CDLL(str(LIB)))
     return _lib
+
+
+# Planted mistakes of mk_oracle.c (ORC_MUTANT values; the file lists what each one changes).
+MUTANTS = {"thermal_cs2": 1, "latent_sign": 2, "latent_factor": 3, "buoyancy_factor": 4,
+           "buoyancy_sign": 5, "solute_cs2": 6, "rejection_sign": 7}
+_mutant_libs: dict = {}
+
+
+def _build_mutant(name: str) -> ct.CDLL:
+    if name not in _mutant_libs:
+        import tempfile
+
+        out = Path(tempfile.mkdtemp(prefix="orc_mutant_")) / f"liborc_{name}.so"
+        cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared",
+               f"-DORC_MUTANT={MUTANTS[name]}", "-o", str(out), str(HERE / "mk_oracle.c"),
+               str(HERE / "mk_aos.c"), "-lm"]
+        subprocess.run(cmd, check=True)
+        _mutant_libs[name] = _bind(ct.CDLL(str(out)))
+    return _mutant_libs[name]
+
+
+@contextmanager
+def planted(name: str):
+    """Run the oracle with one planted mistake (a copy built with ORC_MUTANT set) inside the
+    block: used by the tests to show that a pin fails on the mistake it is meant to catch."""
+    global _lib
+    saved = lib()
+    _lib = _build_mutant(name)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _bind(L: ct.CDLL) -> ct.CDLL:
+    """Declare the argument and result types of the oracle's entry points."""
+    vp = ct.c_void_p
+    L.orc_params_size.restype = ct.c_size_t
+    L.orc_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.c_int]
+    L.orc_step.restype = ct.c_int
+    L.orc_aos_step.argtypes = [vp, vp, ct.c_int, ct.c_int, ct.POINTER(ct.c_double)]
+    L.orc_aos_step.restype = ct.c_int
+    L.orc_moments.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_init_equilibrium.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_init_seeds.argtypes = [vp, vp, ct.c_int, vp, vp]
+    L.orc_equilibrium.argtypes = [ct.c_int, ct.c_double, vp, vp]
+    L.orc_equilibrium_he.argtypes = [ct.c_int, ct.c_double, ct.c_double, vp, vp]
+    L.orc_collide_cell.argtypes = [vp, ct.c_int, vp, vp, vp, ct.c_double]
+    L.orc_phase_source.argtypes = [vp, vp, ct.c_double, ct.c_int, vp, vp]
+    L.orc_face_flux.argtypes = [vp, vp, vp, ct.c_int]
+    L.orc_face_flux.restype = ct.c_double
+    L.orc_aniso.argtypes = [vp, vp, vp]
+    L.orc_aniso.restype = ct.c_double
+    L.orc_lattice.argtypes = [ct.c_int, vp, vp, vp, vp]
+    L.orc_lattice.restype = ct.c_int
+    L.orc_splitmix64.argtypes = [ct.POINTER(ct.c_uint64)]
+    L.orc_splitmix64.restype = ct.c_uint64
+    L.orc_uniform.argtypes = [ct.POINTER(ct.c_uint64)]
+    L.orc_uniform.restype = ct.c_double
+    L.orc_cell_uniform.argtypes = [ct.c_uint64, ct.c_uint64]
+    L.orc_cell_uniform.restype = ct.c_double
+    L.orc_mp_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_mp_step.argtypes = [vp, vp, vp, vp, ct.c_int]
+    L.orc_mp_step.restype = ct.c_int
+    L.orc_mp_macro.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
+    L.orc_fvm_step.restype = ct.c_int
+    L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, ct.c_int]
+    L.orc_mpf_step.restype = ct.c_int
+    L.orc_flow_adjoint.argtypes = [vp, vp, vp, ct.c_int, vp, vp, vp]
+    L.orc_flow_adjoint.restype = ct.c_int
+    return L
 
 
 class _State(ct.Structure):

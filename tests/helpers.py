@@ -89,8 +89,18 @@ def load_device(params, st) -> mkg.Solver:
     return d
 
 
+def _pop_deviation(p, key, pops):
+    """Populations minus the rest part w_i * ref that FP32 storage subtracts."""
+    q = {"f": int(p.flow_q), "g": int(p.phase_q), "h": 7, "c": 7}[key]
+    ref = {"f": p.ref_f, "g": p.ref_g, "h": p.ref_h, "c": p.ref_c}[key]
+    e, w, inv, cs2 = lat(q)
+    return pops.reshape(q, -1) - (w * ref)[:, None]
+
+
 def compare_pops(dev: mkg.Solver, orc: O.State):
-    """Max relative error per present lattice (normalised by the max magnitude)."""
+    """Max error per present lattice, relative to the largest deviation |p_i - w_i ref| of the
+    oracle's populations (the part FP32 storage keeps; the rest part w_i ref is exact), and
+    the max error of phi relative to its range."""
     out = {}
     for k, fid in (("f", mkg.F_POP_FLOW), ("g", mkg.F_POP_PHASE), ("h", mkg.F_POP_THERMAL),
                    ("c", mkg.F_POP_SOLUTE)):
@@ -98,11 +108,44 @@ def compare_pops(dev: mkg.Solver, orc: O.State):
         if ref is None:
             continue
         got = dev.get(fid)
-        scale = max(np.abs(ref).max(), 1e-300)
+        scale = max(np.abs(_pop_deviation(orc.params, k, ref)).max(), 1e-300)
         out[k] = float(np.abs(got - ref).max() / scale)
     if orc.phi is not None:
         got = dev.get(mkg.F_PHI)
-        out["phi"] = float(np.abs(got - orc.phi).max())
+        rng = max(float(orc.phi.max() - orc.phi.min()), 1e-300)
+        out["phi"] = float(np.abs(got - orc.phi).max() / rng)
+    return out
+
+
+def field_errors(dev: mkg.Solver, orc: O.State, mask=None):
+    """Max and RMS error of every present macroscopic field (rho, u, T, c, phi) of the device
+    against the oracle, each relative to the oracle field's range (max - min; for u the
+    largest speed).  mask (optional, N bools) restricts the cells compared."""
+    p = orc.params
+    m = orc.moments()
+    out = {}
+    pairs = []
+    if p.flow_q:
+        pairs.append(("rho", dev.get(mkg.F_RHO), m["rho"]))
+        pairs.append(("u", dev.get(mkg.F_VEL).reshape(-1, 3), m["vel"].reshape(-1, 3)))
+    if p.thermal:
+        pairs.append(("T", dev.get(mkg.F_TEMP), m["T"]))
+    if p.solute:
+        pairs.append(("c", dev.get(mkg.F_CONC), m["C"]))
+    if p.phase_q:
+        pairs.append(("phi", dev.get(mkg.F_PHI), orc.phi))
+    for name, got, ref in pairs:
+        if mask is not None:
+            got, ref = got[mask], ref[mask]
+        d = got - ref
+        if name == "u":
+            scale = float(np.sqrt((ref ** 2).sum(1)).max())
+            d = np.sqrt((d ** 2).sum(1))
+        else:
+            scale = float(ref.max() - ref.min())
+        scale = max(scale, 1e-300)
+        out[name] = {"max": float(np.abs(d).max() / scale),
+                     "rms": float(np.sqrt((d ** 2).mean()) / scale), "scale": scale}
     return out
 
 

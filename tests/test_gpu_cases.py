@@ -207,6 +207,39 @@ def test_checkpoint_resume_is_bit_exact(tmp_path):
         np.testing.assert_array_equal(x, y)
 
 
+def test_checkpoint_resume_with_walls_and_faces_is_bit_exact(tmp_path):
+    """A state with wall cells and non-periodic faces resumes bit-exactly in a fresh handle
+    (which has no walls until the checkpoint's flags arrive), and a checkpoint of another grid
+    with the same cell count is refused."""
+    from helpers import load_device, random_state
+    from test_gpu_parity import make_params
+
+    p = make_params(flow_q=19, phase_q=19, thermal=1, op="trt", bcs="mixed",
+                    prec=mkg.PREC_FP32, dims=(6, 4, 64))
+    st = random_state(p, seed=11, with_walls=True)
+    assert (st["flag"] == mkg.IS_WALLS).any()
+    a = load_device(p, st)
+    a.step(5, mkg.ALL)
+    path = tmp_path / "ck_walls.bin"
+    a.checkpoint_save(path)
+    a.step(6, mkg.ALL)
+    ref = [a.get(f) for f in (mkg.F_POP_FLOW, mkg.F_POP_PHASE, mkg.F_POP_THERMAL, mkg.F_FLAG)]
+    a.close()
+    b = mkg.Solver(p)
+    b.checkpoint_load(path)
+    b.step(6, mkg.ALL)
+    got = [b.get(f) for f in (mkg.F_POP_FLOW, mkg.F_POP_PHASE, mkg.F_POP_THERMAL, mkg.F_FLAG)]
+    b.close()
+    for x, y in zip(ref, got):
+        np.testing.assert_array_equal(x, y)
+    q = p.copy()
+    q.nx, q.ny = 4, 6
+    c = mkg.Solver(q)
+    with pytest.raises(mkg.MkgError):
+        c.checkpoint_load(path)
+    c.close()
+
+
 def test_runs_are_deterministic():
     outs = []
     for _ in range(2):

@@ -90,3 +90,77 @@ def test_z_stage_matches_direct_o_steps_at_128(monkeypatch):
         res.append(dev.get(mkg.MP_PHI))
         dev.close()
     assert np.abs(res[0] - res[1]).max() < 1e-5
+
+
+def _oracle_from_device(p, d):
+    """An oracle state holding the device's current state (canonical populations, both phase
+    buffers, flags)."""
+    from oracle import oracle as O
+
+    s = O.State.empty(p)
+    s.f[:] = d.get(mkg.F_POP_FLOW)
+    s.g[:] = d.get(mkg.F_POP_PHASE)
+    s.h[:] = d.get(mkg.F_POP_THERMAL)
+    s.phi[:] = d.get(mkg.F_PHI)
+    s.phi_prev[:] = d.get(mkg.F_PHI_PREV)
+    s.flag[:] = d.get(mkg.F_FLAG)
+    return s
+
+
+def test_benchmark_step_at_256_matches_oracle():
+    """The benchmark workload itself (D3Q19 TRT flow with Boussinesq buoyancy, D3Q7 thermal,
+    D3Q19 anisotropic phase field, 256^3, FP32, all faces periodic: the face-free kernel
+    variant with the z-staged O step, one z row per block).  After 40 device steps from the
+    seed, the state is copied into the FP64 oracle and both take two more steps (one O step,
+    one E step); populations agree to 1e-5 of their deviations, flags exactly."""
+    from helpers import compare_pops, field_errors
+    from oracle import oracle as O
+
+    print("fused 256^3 vs oracle", flush=True)
+    case = cases.load("fused256")
+    p = case.params
+    assert p.nz == 256 and p.precision == mkg.PREC_FP32
+    d = mkg.Solver(p)
+    cases.setup(d, case)
+    d.step(41, mkg.ALL)  # odd: the next device step is an O step
+    s = _oracle_from_device(p, d)
+    d.step(2, mkg.ALL)
+    s.step(2, 15, O.OP_TRT)
+    errs = compare_pops(d, s)
+    fe = field_errors(d, s)
+    flags_equal = np.array_equal(d.get(mkg.F_FLAG), s.flag)
+    d.close()
+    print(f"errs {errs} fields {fe}", flush=True)
+    assert flags_equal
+    for k, e in errs.items():
+        assert e < 1e-5, f"{k}: {e:.3e}"
+
+
+def test_benchmark_step_at_256_conserves_mass_and_energy():
+    """64 steps of the benchmark workload: sum rho is conserved (periodic, mass-conserving
+    collisions and partial bounce-back), sum (T - L phi) too (latent heat L dphi), and
+    every population stays finite."""
+    print("fused 256^3 conservation", flush=True)
+    case = cases.load("fused256")
+    p = case.params
+    d = mkg.Solver(p)
+    cases.setup(d, case)
+    d.step(1, mkg.ALL)
+    m0 = d.reduce(mkg.R_MASS)[0]
+    e0 = d.reduce(mkg.R_ENERGY)[0] - p.latent_T * d.reduce(mkg.R_PHI)[0]
+    p0 = d.reduce(mkg.R_PHI)[0]
+    d.step(64, mkg.ALL)
+    m1 = d.reduce(mkg.R_MASS)[0]
+    e1 = d.reduce(mkg.R_ENERGY)[0] - p.latent_T * d.reduce(mkg.R_PHI)[0]
+    p1 = d.reduce(mkg.R_PHI)[0]
+    bad = d.reduce(mkg.R_NONFINITE)[0]
+    umax = d.reduce(mkg.R_UMAX)[0]
+    d.close()
+    N = 256 ** 3
+    print(f"mass {m0!r} -> {m1!r}, T - L phi {e0!r} -> {e1!r}, phi {p0!r} -> {p1!r}, "
+          f"umax {umax:.3e}", flush=True)
+    assert bad == 0
+    assert p1 > p0 + 100.0  # the seed grows
+    assert abs(m1 - m0) < 1e-8 * N
+    assert abs(e1 - e0) < 1e-6 * N
+    assert 0 < umax < 0.05

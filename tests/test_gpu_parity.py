@@ -240,6 +240,42 @@ def test_all_periodic_variant_matches_oracle(case, prec):
         assert e < tol, f"{k}: rel err {e:.3e}"
 
 
+# The benchmark's launch shape: nz = 256, one z row per 256-thread block, z-staged O steps.
+# All-periodic grids take the variant compiled with only the face-free body (PER), which the
+# 256^3 benchmark times; mixed faces take the kernel holding both bodies.
+NZ256 = (4, 4, 256)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+@pytest.mark.parametrize("prec", [mkg.PREC_FP64, mkg.PREC_FP32], ids=["fp64", "fp32"])
+def test_nz256_matches_oracle(case, bcs, prec):
+    errs, fd, fo, dev, orc = _run(case, bcs, 6, prec=prec, dims=NZ256)
+    dev.close()
+    if prec == mkg.PREC_FP64:
+        assert np.array_equal(fd, fo), "flags differ"
+    tol = 1e-12 if prec == mkg.PREC_FP64 else 1e-5
+    for k, e in errs.items():
+        assert e < tol, f"{k}: rel err {e:.3e}"
+
+
+FUSED_CASES = [c for c in CASES if c.get("flow_q") and c.get("phase_q")]
+
+
+@pytest.mark.parametrize("case", FUSED_CASES, ids=[c["id"] for c in FUSED_CASES])
+@pytest.mark.parametrize("bcs", ["periodic", "mixed"])
+@pytest.mark.parametrize("prec", [mkg.PREC_FP64, mkg.PREC_FP32], ids=["fp64", "fp32"])
+def test_nz256_odd_grid_with_walls_matches_oracle(case, bcs, prec):
+    """2 x 3 x 256 with wall cells, 7 steps (ends in the shifted layout)."""
+    errs, fd, fo, dev, orc = _run(case, bcs, 7, prec=prec, walls=True, dims=(2, 3, 256))
+    dev.close()
+    if prec == mkg.PREC_FP64:
+        assert np.array_equal(fd, fo), "flags differ"
+    tol = 1e-12 if prec == mkg.PREC_FP64 else 1e-5
+    for k, e in errs.items():
+        assert e < tol, f"{k}: rel err {e:.3e}"
+
+
 @pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
 def test_z_stage_agrees_with_direct_o_steps(case, monkeypatch):
     """The z stage changes only how populations move: FP32 results with and without it agree
