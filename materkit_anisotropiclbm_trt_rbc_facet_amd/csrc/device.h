@@ -623,11 +623,12 @@ __device__ inline R phase_source(const R (&P)[27], R upf, const Aniso<R>& an, R 
             lap += (R)side * g[ax];
         }
     }
-    // isotropic LB-weighted cell gradient (D3Q19 weights over cs^2 = 1/3)
+    // isotropic LB-weighted cell gradient (D3Q19 weights over cs^2 = 1/3), summed over
+    // differences to the centre value (no cancellation of O(1) terms in the bulk)
     R gr0 = 0, gr1 = 0, gr2 = 0;
 #pragma unroll
     for (int i = 1; i < 19; i++) {
-        const R v = (R)(3.0 * wgt<19>(i)) * PP(cx(i), cy(i), cz(i));
+        const R v = (R)(3.0 * wgt<19>(i)) * (PP(cx(i), cy(i), cz(i)) - PP(0, 0, 0));
         if (cx(i)) gr0 += (R)cx(i) * v;
         if (cy(i)) gr1 += (R)cy(i) * v;
         if (cz(i)) gr2 += (R)cz(i) * v;

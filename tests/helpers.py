@@ -44,7 +44,9 @@ def random_state(params, seed=0, with_walls=False, phi_mode="seeds"):
         e, w, inv, cs2 = lat(p.phase_q)
         g = w[None, :] * phi[:, None] + 1e-3 * w[None, :] * rng.standard_normal((N, p.phase_q))
         st["g"] = np.ascontiguousarray(g.T).reshape(-1)
-        st["phi_prev"] = phi + 1e-3 * rng.standard_normal(N)
+        # dphi/dt is carried by the interface ((1 - phi^2) weight), as at a moving front: the
+        # bulk of each phase changes only by rounding
+        st["phi_prev"] = phi + 1e-3 * (1 - phi ** 2) * rng.standard_normal(N)
         st["phi"] = phi
     if p.thermal:
         e, w, inv, cs2 = lat(7)
@@ -97,24 +99,58 @@ def _pop_deviation(p, key, pops):
     return pops.reshape(q, -1) - (w * ref)[:, None]
 
 
-def compare_pops(dev: mkg.Solver, orc: O.State):
+def compare_pops(dev: mkg.Solver, orc: O.State, cells=None):
     """Max error per present lattice, relative to the largest deviation |p_i - w_i ref| of the
     oracle's populations (the part FP32 storage keeps; the rest part w_i ref is exact), and
-    the max error of phi relative to its range."""
+    the max error of phi relative to its range.  cells (optional, N bools): the cells
+    compared."""
     out = {}
     for k, fid in (("f", mkg.F_POP_FLOW), ("g", mkg.F_POP_PHASE), ("h", mkg.F_POP_THERMAL),
                    ("c", mkg.F_POP_SOLUTE)):
         ref = getattr(orc, k)
         if ref is None:
             continue
-        got = dev.get(fid)
+        q = ref.size // orc.N
+        got = dev.get(fid).reshape(q, -1)
+        ref = ref.reshape(q, -1)
         scale = max(np.abs(_pop_deviation(orc.params, k, ref)).max(), 1e-300)
-        out[k] = float(np.abs(got - ref).max() / scale)
+        d = np.abs(got - ref)
+        if cells is not None:
+            d = d[:, cells]
+        out[k] = float(d.max() / scale)
     if orc.phi is not None:
         got = dev.get(mkg.F_PHI)
         rng = max(float(orc.phi.max() - orc.phi.min()), 1e-300)
-        out["phi"] = float(np.abs(got - orc.phi).max() / rng)
+        d = np.abs(got - orc.phi)
+        if cells is not None:
+            d = d[cells]
+        out["phi"] = float(d.max() / rng)
     return out
+
+
+def step_pair(dev: mkg.Solver, orc: O.State, nsteps: int, op: int, mask: int = 15):
+    """Advance device and oracle nsteps steps, one at a time, and return the cells outside the
+    light cone of every flag that differs between them.  In FP32 a phase value within rounding
+    of the classification threshold can flip a flag (interface / solid), which changes the
+    solid fraction the flow sees; its effect spreads by one cell per step.  Returns
+    (trusted cells, number of flips)."""
+    p = orc.params
+    dims = (int(p.nx), int(p.ny), int(p.nz))
+    untrusted = np.zeros(dims, dtype=bool)
+    flips = 0
+    for k in range(1, nsteps + 1):
+        dev.step(1, mask)
+        orc.step(1, mask, op)
+        diff = (dev.get(mkg.F_FLAG) != orc.flag).reshape(dims)
+        if diff.any():
+            flips += int(diff.sum())
+            r = nsteps - k + 1
+            for x, y, z in zip(*np.nonzero(diff)):
+                ix = np.arange(x - r, x + r + 1) % dims[0]
+                iy = np.arange(y - r, y + r + 1) % dims[1]
+                iz = np.arange(z - r, z + r + 1) % dims[2]
+                untrusted[np.ix_(ix, iy, iz)] = True
+    return ~untrusted.reshape(-1), flips
 
 
 def field_errors(dev: mkg.Solver, orc: O.State, mask=None):

@@ -47,9 +47,11 @@ def test_config1_full_size_matches_oracle():
     o.init_equilibrium()
     d.step(100, mkg.ALL)
     o.step(100, 1, O.OP_BGK)
-    err = compare_pops(d, o)
+    got = d.get(mkg.F_POP_FLOW)
     d.close()
-    assert err["f"] < 1e-13
+    # the device stores deviations f - w rho0 (about 1e-5 here); the oracle holds full values
+    # (about 0.3, ulp 5e-17), so the comparison is on the full values' scale
+    assert np.abs(got - o.f).max() < 1e-13 * np.abs(o.f).max()
 
 
 def test_config2_rbc_fp32_field_error_and_convection():

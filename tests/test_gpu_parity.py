@@ -7,7 +7,7 @@ running an odd and an even number of steps.
 import numpy as np
 import pytest
 
-from helpers import compare_pops, load_device, load_oracle, random_state
+from helpers import compare_pops, load_device, load_oracle, random_state, step_pair
 from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
 from oracle import oracle as O
 
@@ -114,9 +114,19 @@ def _run(case, bcs, nsteps, prec=mkg.PREC_FP64, walls=False, seed=1, dims=(10, 9
     try:
         print(f"[parity] {case['id']} bcs={bcs} steps={nsteps} prec={prec} walls={walls} "
               f"dims={dims}", flush=True)
-        dev.step(nsteps, mkg.ALL)
-        orc.step(nsteps, 15, OPS[op])
-        errs = compare_pops(dev, orc)
+        if prec == mkg.PREC_FP32:
+            # FP32 flags may flip where phi sits within rounding of the threshold: compare
+            # outside the light cones of such cells (at most 5 % of the grid)
+            cells, flips = step_pair(dev, orc, nsteps, OPS[op])
+            if flips:
+                print(f"[parity] {flips} flag flips, {int((~cells).sum())} cells excluded",
+                      flush=True)
+            assert (~cells).mean() < 0.05
+            errs = compare_pops(dev, orc, cells)
+        else:
+            dev.step(nsteps, mkg.ALL)
+            orc.step(nsteps, 15, OPS[op])
+            errs = compare_pops(dev, orc)
         flags_dev = dev.get(mkg.F_FLAG)
         return errs, flags_dev, orc.flag.copy(), dev, orc
     finally:
