@@ -40,9 +40,16 @@ def random_state(params, seed=0, with_walls=False, phi_mode="seeds"):
         X, Y, Z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
         r = np.sqrt((X - nx / 2.0) ** 2 + (Y - ny / 2.2) ** 2 + (Z - nz / 1.9) ** 2)
         phi = -np.tanh((r - min(nx, ny, nz) / 4.0) / 2.0).reshape(-1)
-        phi = np.clip(phi + 0.02 * rng.standard_normal(N), -1, 1)
+        # noise carried by the interface, as in the fields the solver produces (|grad phi|
+        # grows with 1 - phi^2): noise in a bulk phase would put local extrema of phi there,
+        # where the anisotropy and anti-trapping directions grad phi / |grad phi| are
+        # undefined while 1 - phi^2 is not zero
+        phi = np.clip(phi + 0.02 * (1 - phi ** 2) * rng.standard_normal(N), -1, 1)
         e, w, inv, cs2 = lat(p.phase_q)
-        g = w[None, :] * phi[:, None] + 1e-3 * w[None, :] * rng.standard_normal((N, p.phase_q))
+        # non-equilibrium part carried by the interface: a bulk phase at rest sits at its
+        # equilibrium w_i phi
+        g = w[None, :] * phi[:, None] + 1e-3 * (1 - phi ** 2)[:, None] * w[None, :] * \
+            rng.standard_normal((N, p.phase_q))
         st["g"] = np.ascontiguousarray(g.T).reshape(-1)
         # dphi/dt is carried by the interface ((1 - phi^2) weight), as at a moving front: the
         # bulk of each phase changes only by rounding
@@ -142,8 +149,8 @@ def step_pair(dev: mkg.Solver, orc: O.State, nsteps: int, op: int, mask: int = 1
         dev.step(1, mask)
         orc.step(1, mask, op)
         diff = (dev.get(mkg.F_FLAG) != orc.flag).reshape(dims)
-        if diff.any():
-            flips += int(diff.sum())
+        flips += int(diff.sum())
+        if diff.any() and p.flow_q:  # flags act on the flow only (solid fraction)
             r = nsteps - k + 1
             for x, y, z in zip(*np.nonzero(diff)):
                 ix = np.arange(x - r, x + r + 1) % dims[0]
