@@ -116,12 +116,12 @@ def _run(case, bcs, nsteps, prec=mkg.PREC_FP64, walls=False, seed=1, dims=(10, 9
               f"dims={dims}", flush=True)
         if prec == mkg.PREC_FP32:
             # FP32 flags may flip where phi sits within rounding of the threshold: compare
-            # outside the light cones of such cells (at most 5 % of the grid)
+            # outside the light cones of such cells (at most 10 % of the grid)
             cells, flips = step_pair(dev, orc, nsteps, OPS[op])
             if flips:
                 print(f"[parity] {flips} flag flips, {int((~cells).sum())} cells excluded",
                       flush=True)
-            assert (~cells).mean() < 0.05
+            assert (~cells).mean() < 0.1
             errs = compare_pops(dev, orc, cells)
         else:
             dev.step(nsteps, mkg.ALL)
