@@ -236,6 +236,15 @@ int mkg_set_field(mkg_handle h, int field, const void* host, int dtype, size_t c
 
 int mkg_reduce(mkg_handle h, int which, double* out);
 
+/* The same reductions (same values, bit for bit) without any device allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The kernels read
+ * the cell values straight from the population storage and keep their per-block partials in
+ * one fixed device-resident array per GPU (4 MiB, held until the process ends).  Grids of
+ * more than 2^25 cells (33,554,432) are rejected with an error.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (other threads, other handles' streams) race. */
+int mkg_reduce_noalloc(mkg_handle h, int which, double* out);
+
 int mkg_checkpoint_save(mkg_handle h, const char* path);
 int mkg_checkpoint_load(mkg_handle h, const char* path);
 

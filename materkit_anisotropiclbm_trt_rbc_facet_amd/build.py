@@ -32,8 +32,8 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "adjoint.hip", "fadj.hip", "mpf.hip",
-           "fvm.hip", "mp.hip",
+SOURCES = ["step_f32.hip", "step_f64.hip", "aux.hip", "reduce_static.hip", "adjoint.hip",
+           "fadj.hip", "mpf.hip", "fvm.hip", "mp.hip",
            "api.cpp",
            "case.cpp"]
 

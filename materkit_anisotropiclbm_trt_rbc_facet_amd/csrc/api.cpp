@@ -321,6 +321,35 @@ int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid) {
     return 0;
 }
 
+// Fields a reduction reads (flow, thermal, solute, phase) and the lattices it needs.
+int reduce_needs(const mkg_ctx* h, int which, int* need) {
+    if (which < 0 || which > MKG_R_TIP) return fail("unknown reduction");
+    need[0] = which == MKG_R_MASS || which == MKG_R_NUSSELT || which == MKG_R_MOMENTUM ||
+              which == MKG_R_UMAX;
+    need[1] = which == MKG_R_ENERGY || which == MKG_R_NUSSELT;
+    need[2] = which == MKG_R_SOLUTE;
+    need[3] = which == MKG_R_SOLID || which == MKG_R_PHI || which == MKG_R_TIP;
+    if (need[0] && !h->q[LF]) return fail("reduction needs the flow lattice");
+    if (need[1] && !h->q[LH]) return fail("reduction needs the thermal lattice");
+    if (need[2] && !h->q[LC]) return fail("reduction needs the solute lattice");
+    if (need[3] && !h->q[LG]) return fail("reduction needs the phase lattice");
+    return 0;
+}
+
+// Host-side scaling of the reduced values: the solid fraction is a mean, the Nusselt number
+// 1 + <u_z T> H / (alpha dT) (z-vertical Rayleigh-Benard, walls at the z faces).
+void finish_reduce(const mkg_ctx* h, int which, double* v, double* out) {
+    const double N = (double)h->N;
+    if (which == MKG_R_SOLID) v[0] /= N;
+    if (which == MKG_R_NUSSELT) {
+        const double dT = h->p.T_face[4] - h->p.T_face[5];
+        const double H = h->p.nz;
+        v[0] = 1.0 + (v[0] / N) * H / (h->p.alpha_l * dT);
+    }
+    const int nout = (which == MKG_R_MOMENTUM || which == MKG_R_TIP) ? 3 : 1;
+    for (int k = 0; k < nout; k++) out[k] = v[k];
+}
+
 int step_once(mkg_ctx* h, const int* adv, hipEvent_t* ev, void*) {
     return step_once_impl(h, adv, ev, [](mkg_ctx*) { return 0; });
 }
@@ -386,6 +415,10 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
     };
     if (hipGetDevice(&h->dev) != hipSuccess) return cleanup(fail("hipGetDevice failed"));
     if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail("hipStreamCreate failed"));
+    {
+        std::string err;
+        if (noalloc_prepare(h->dev, &err)) return cleanup(fail(err));
+    }
     for (int L = 0; L < 4; L++)
         if (alloc_zero(&h->pop[L], h->pop_elems(L) * h->esz, h->stream)) return cleanup(-1);
     if (p->phase_q) {
@@ -686,7 +719,8 @@ int mkg_set_field(mkg_handle h, int field, const void* host, int dtype, size_t c
 
 int mkg_reduce(mkg_handle h, int which, double* out) {
     if (!h || !out) return fail("null argument");
-    if (which < 0 || which > MKG_R_TIP) return fail("unknown reduction");
+    int need[4];
+    if (reduce_needs(h, which, need)) return -1;
     if (ensure_scratch(h)) return -1;
     const size_t N = h->N;
     if (which == MKG_R_NONFINITE) {
@@ -709,15 +743,7 @@ int mkg_reduce(mkg_handle h, int which, double* out) {
     double* T = rho + 4 * N;
     double* C = rho + 5 * N;
     double* phi = rho + 6 * N;
-    bool need_flow = which == MKG_R_MASS || which == MKG_R_NUSSELT || which == MKG_R_MOMENTUM ||
-                     which == MKG_R_UMAX;
-    bool need_T = which == MKG_R_ENERGY || which == MKG_R_NUSSELT;
-    bool need_C = which == MKG_R_SOLUTE;
-    bool need_phi = which == MKG_R_SOLID || which == MKG_R_PHI || which == MKG_R_TIP;
-    if (need_flow && !h->q[LF]) return fail("reduction needs the flow lattice");
-    if (need_T && !h->q[LH]) return fail("reduction needs the thermal lattice");
-    if (need_C && !h->q[LC]) return fail("reduction needs the solute lattice");
-    if (need_phi && !h->q[LG]) return fail("reduction needs the phase lattice");
+    const bool need_flow = need[0], need_T = need[1], need_C = need[2], need_phi = need[3];
     StepArgs a = make_args(h);
     if (need_flow || need_T || need_C) {
         launch_macro(h->q[LF], h->fp32, a, need_flow ? rho : nullptr, need_flow ? u : nullptr,
@@ -734,13 +760,35 @@ int mkg_reduce(mkg_handle h, int which, double* out) {
     double v[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(v, h->red_out, 4 * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (which == MKG_R_SOLID) v[0] /= (double)N;
-    if (which == MKG_R_NUSSELT) {
-        const double dT = h->p.T_face[4] - h->p.T_face[5];
-        const double H = h->p.nz;
-        v[0] = 1.0 + (v[0] / (double)N) * H / (h->p.alpha_l * dT);
+    finish_reduce(h, which, v, out);
+    return 0;
+}
+
+int mkg_reduce_noalloc(mkg_handle h, int which, double* out) {
+    if (!h || !out) return fail("null argument");
+    int need[4];
+    if (reduce_needs(h, which, need)) return -1;
+    StepArgs a = make_args(h);
+    void* pops[4];
+    size_t sizes[4];
+    int npops = 0;
+    for (int L = 0; L < 4; L++)
+        if (h->pop[L]) {
+            pops[npops] = h->pop[L];
+            sizes[npops++] = (size_t)h->q[L] * h->S;
+        }
+    // cell values needed: flow (with the scalars), the scalars alone, or the phase field
+    const int macro = need[0] ? 2 : ((need[1] || need[2]) ? 1 : 0);
+    double v[4] = {0, 0, 0, 0};
+    std::string err;
+    if (reduce_noalloc(h->dev, h->q[LF], h->fp32, a, which, macro, pops, sizes, npops,
+                       h->stream, v, &err))
+        return fail("mkg_reduce_noalloc: " + err);
+    if (which == MKG_R_NONFINITE) {
+        out[0] = v[0];
+        return 0;
     }
-    for (int k = 0; k < nout; k++) out[k] = v[k];
+    finish_reduce(h, which, v, out);
     return 0;
 }
 

@@ -1,6 +1,6 @@
 // Auxiliary kernels: layout-independent population read/write, macroscopic fields,
 // initial states, seed geometry and the FP64 diagnostic reductions.
-#include "device.h"
+#include "cellred.h"
 
 namespace mkg {
 
@@ -76,66 +76,18 @@ int launch_store_pops(int q, int fp32, const StepArgs& a, const LatArgs& L, cons
 }
 
 // ------------------------------------------------------------------ macroscopic fields
-template <class R, int Q>
-__device__ inline void read_any(R* p, const LatArgs& L, const StepArgs& a, const Cell& c,
-                                int kind, bool wall) {
-    if (wall)
-        read_pops<Q, false, R>(p, L, a.S, c, a, kind);
-    else
-        read_pops_rt<Q, R>(p, L, a.S, c, a, kind);
-}
-
 template <class R, int FQ>
 __global__ void __launch_bounds__(256) k_macro(const StepArgs a, double* rho, double* u,
                                                double* T, double* C) {
     Cell c;
     if (!locate(a, c)) return;
     const bool wall = a.flag[c.id] == kIsWalls;
-    double Tv = 0;
-    if (a.h.pop) {
-        R h[7];
-        read_any<R, 7>(h, a.h, a, c, kKindScalar, wall);
-        R s = 0;
-#pragma unroll
-        for (int i = 0; i < 7; i++) s += h[i];
-        Tv = a.h.ref + (double)s;
-        if (T) T[c.id] = Tv;
-    }
-    if (a.c.pop && C) {
-        R p[7];
-        read_any<R, 7>(p, a.c, a, c, kKindScalar, wall);
-        R s = 0;
-#pragma unroll
-        for (int i = 0; i < 7; i++) s += p[i];
-        C[c.id] = a.c.ref + (double)s;
-    }
-    if constexpr (FQ > 0) {
-        R f[FQ];
-        read_any<R, FQ>(f, a.f, a, c, kKindFlow, wall);
-        R dr = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int i = 0; i < FQ; i++) {
-            dr += f[i];
-            if (cx(i)) jx += (R)cx(i) * f[i];
-            if (cy(i)) jy += (R)cy(i) * f[i];
-            if (cz(i)) jz += (R)cz(i) * f[i];
-        }
-        const R rh = (R)a.f.ref + dr;
-        double F[3] = {a.force[0], a.force[1], a.force[2]};
-        if (a.h.pop)
-            for (int k = 0; k < 3; k++) F[k] += a.rho0 * a.buoy[k] * (Tv - a.T_ref);
-        if (rho) rho[c.id] = (double)rh;
-        if (u) {
-            const double den = a.incomp ? a.rho0 : (double)rh;
-            u[3 * (size_t)c.id + 0] = ((double)jx + 0.5 * F[0]) / den;
-            u[3 * (size_t)c.id + 1] = ((double)jy + 0.5 * F[1]) / den;
-            u[3 * (size_t)c.id + 2] = ((double)jz + 0.5 * F[2]) / den;
-        }
-    } else if (u) {
-        u[3 * (size_t)c.id + 0] = 0;
-        u[3 * (size_t)c.id + 1] = 0;
-        u[3 * (size_t)c.id + 2] = 0;
-    }
+    const CellMacro m = cell_macro<R, FQ>(a, c, wall, C != nullptr);
+    if (T && a.h.pop) T[c.id] = m.T;
+    if (C && a.c.pop) C[c.id] = m.C;
+    if (rho && FQ > 0) rho[c.id] = m.rho;
+    if (u)
+        for (int k = 0; k < 3; k++) u[3 * (size_t)c.id + k] = m.u[k];
 }
 
 template <class R>
@@ -314,121 +266,38 @@ int launch_init_seeds(int fp32, int pq, const StepArgs& a, int n, const double* 
 }
 
 // ------------------------------------------------------------------ reductions
-// Wavefront (64-lane) butterfly, then LDS across the four wavefronts of a block; per-block
-// partials are combined by a single block in fixed order, so results are deterministic.
-constexpr int kRedMax = 4;
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ inline void block_reduce(double* v, int nv, bool is_max, double* out_block) {
-    __shared__ double sm[4][kRedMax];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int k = 0; k < nv; k++) {
-        double r = is_max ? wave_max(v[k]) : wave_sum(v[k]);
-        if (lane == 0) sm[wv][k] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < nv; k++) {
-            double r = sm[0][k];
-            for (int w = 1; w < 4; w++) r = is_max ? fmax(r, sm[w][k]) : r + sm[w][k];
-            out_block[k] = r;
-        }
-    }
-}
-
 __global__ void __launch_bounds__(256) k_reduce(const StepArgs a, int which, const double* rho,
                                                 const double* u, const double* T,
                                                 const double* C, const double* phi,
                                                 double* partials) {
     double v[kRedMax] = {0, 0, 0, 0};
     uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    bool is_max = (which == 7 || which == 9);
+    const bool is_max = reduce_is_max(which);
     if (is_max)
         for (int k = 0; k < kRedMax; k++) v[k] = -1e300;
     if (t < a.N && a.flag[t] != kIsWalls) {
-        switch (which) {
-            case 0: v[0] = rho[t]; break;
-            case 1: v[0] = T[t]; break;
-            case 2: v[0] = C[t]; break;
-            case 3: {
-                double fs = 0.5 * (1.0 + phi[t]);
-                v[0] = fs < 0 ? 0 : (fs > 1 ? 1 : fs);
-                break;
-            }
-            case 4: v[0] = phi[t]; break;
-            case 5: v[0] = u[3 * (size_t)t + 2] * T[t]; break;
-            case 6:
-                for (int k = 0; k < 3; k++) v[k] = rho[t] * u[3 * (size_t)t + k];
-                break;
-            case 7: {
-                double ux = u[3 * (size_t)t], uy = u[3 * (size_t)t + 1], uz = u[3 * (size_t)t + 2];
-                v[0] = sqrt(ux * ux + uy * uy + uz * uz);
-                break;
-            }
-            case 8: v[0] = 0; break;
-            case 9: {
-                // phi = 0 crossings between a cell with phi >= 0 and its +x / +y / +z
-                // neighbour with phi < 0, linearly interpolated; the largest per axis is the
-                // extreme point of the phi = 0 surface along that axis (the tip)
-                const double p0 = phi[t];
-                if (p0 >= 0.0) {
-                    const uint32_t z = t % a.nz, r = t / a.nz, y = r % a.ny, x = r / a.ny;
-                    const uint32_t c3[3] = {x, y, z};
-                    const uint32_t n3[3] = {(uint32_t)a.nx, (uint32_t)a.ny, (uint32_t)a.nz};
-                    const uint32_t st[3] = {(uint32_t)a.ny * (uint32_t)a.nz, (uint32_t)a.nz, 1u};
-                    for (int k = 0; k < 3; k++) {
-                        if (c3[k] + 1u < n3[k]) {
-                            const double p1 = phi[t + st[k]];
-                            if (p1 < 0.0)
-                                v[k] = c3[k] + p0 / (p0 - p1) - 0.5 * (n3[k] - 1.0);
-                        }
-                    }
-                }
-                break;
-            }
-        }
+        const double zero3[3] = {0, 0, 0};
+        reduce_value(a, which, t, rho ? rho[t] : 0.0, u ? u + 3 * (size_t)t : zero3,
+                     T ? T[t] : 0.0,
+                     C ? C[t] : 0.0, phi ? phi[t] : 0.0, [&](uint32_t k) { return phi[k]; }, v);
     }
     block_reduce(v, kRedMax, is_max, partials + (size_t)blockIdx.x * kRedMax);
 }
 
 __global__ void __launch_bounds__(256) k_reduce_final(const double* partials, uint32_t nb,
                                                       int is_max, double* out) {
-    double v[kRedMax];
-    for (int k = 0; k < kRedMax; k++) v[k] = is_max ? -1e300 : 0.0;
-    // fixed assignment of partials to threads and fixed combination order
-    for (uint32_t b = threadIdx.x; b < nb; b += 256)
-        for (int k = 0; k < kRedMax; k++) {
-            double p = partials[(size_t)b * kRedMax + k];
-            v[k] = is_max ? fmax(v[k], p) : v[k] + p;
-        }
-    block_reduce(v, kRedMax, is_max != 0, out);
+    reduce_partials(partials, nb, is_max != 0, out);
 }
 
-// Non-finite population count over one lattice buffer (storage type).
 template <class R>
 __global__ void __launch_bounds__(256) k_nonfinite(const R* p, size_t n, double* partials) {
-    double v[kRedMax] = {0, 0, 0, 0};
-    size_t t = (size_t)blockIdx.x * 256u + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * 256u;
-    for (size_t i = t; i < n; i += stride)
-        if (!isfinite((double)p[i])) v[0] += 1.0;
-    block_reduce(v, kRedMax, false, partials + (size_t)blockIdx.x * kRedMax);
+    nonfinite_partial<R>(p, n, partials);
 }
 
 int launch_reduce(int which, const StepArgs& a, const double* rho, const double* u,
                   const double* T, const double* C, const double* phi, double* partials,
                   double* out, int* nout, hipStream_t s) {
-    int is_max = (which == 7 || which == 9);
+    int is_max = reduce_is_max(which) ? 1 : 0;
     k_reduce<<<grid_of(a), 256, 0, s>>>(a, which, rho, u, T, C, phi, partials);
     k_reduce_final<<<1, 256, 0, s>>>(partials, a.nblk, is_max, out);
     *nout = (which == 6 || which == 9) ? 3 : 1;

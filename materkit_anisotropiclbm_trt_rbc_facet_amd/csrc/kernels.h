@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace mkg {
 
 // One population lattice as seen by a kernel.
@@ -125,5 +127,14 @@ int launch_fadj_loss(int q, int fp32, const StepArgs& a, const double* fN, const
                      void* lam, double* part, hipStream_t s);
 int launch_fadj_step(int q, int fp32, const StepArgs& a, const double* fn, const void* lam_in,
                      void* lam_out, double* part, hipStream_t s);
+
+// Reductions without allocation (reduce_static.hip): per-block partials and the result live
+// in a fixed device-resident array.  noalloc_prepare looks its address up on device dev (once
+// per device); reduce_noalloc runs reduction `which` and copies out[0..3] to host.  macro:
+// 0 the phase field only, 1 the scalar lattices, 2 flow and scalars.
+int noalloc_prepare(int dev, std::string* err);
+int reduce_noalloc(int dev, int fq, int fp32, const StepArgs& a, int which, int macro,
+                   void* const* pops, const size_t* sizes, int npops, hipStream_t s,
+                   double* host, std::string* err);
 
 }  // namespace mkg

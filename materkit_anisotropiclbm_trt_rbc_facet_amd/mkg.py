@@ -191,6 +191,7 @@ def lib() -> C.CDLL:
         "mkg_get_field": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_size_t]),
         "mkg_set_field": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_size_t]),
         "mkg_reduce": (C.c_int, [vp, C.c_int, vp]),
+        "mkg_reduce_noalloc": (C.c_int, [vp, C.c_int, vp]),
         "mkg_checkpoint_save": (C.c_int, [vp, C.c_char_p]),
         "mkg_checkpoint_load": (C.c_int, [vp, C.c_char_p]),
         "mkg_last_timing": (C.c_int, [vp, vp]),
@@ -375,6 +376,12 @@ class Solver:
     def reduce(self, which: int) -> np.ndarray:
         out = np.zeros(4)
         _check(lib().mkg_reduce(self._h, which, _ptr(out)))
+        return out[:3] if which in (R_MOMENTUM, R_TIP) else out[:1]
+
+    def reduce_noalloc(self, which: int) -> np.ndarray:
+        """mkg_reduce_noalloc: the same reduction with no device allocation."""
+        out = np.zeros(4)
+        _check(lib().mkg_reduce_noalloc(self._h, which, _ptr(out)))
         return out[:3] if which in (R_MOMENTUM, R_TIP) else out[:1]
 
     def checkpoint_save(self, path) -> None:
