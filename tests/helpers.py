@@ -163,7 +163,9 @@ def step_pair(dev: mkg.Solver, orc: O.State, nsteps: int, op: int, mask: int = 1
 def field_errors(dev: mkg.Solver, orc: O.State, mask=None):
     """Max and RMS error of every present macroscopic field (rho, u, T, c, phi) of the device
     against the oracle, each relative to the oracle field's range (max - min; for u the
-    largest speed).  mask (optional, N bools) restricts the cells compared."""
+    largest speed; for rho the reference density rho0, since the density of a nearly
+    incompressible flow varies by far less than its rounding).  mask (optional, N bools)
+    restricts the cells compared."""
     p = orc.params
     m = orc.moments()
     out = {}
@@ -184,6 +186,8 @@ def field_errors(dev: mkg.Solver, orc: O.State, mask=None):
         if name == "u":
             scale = float(np.sqrt((ref ** 2).sum(1)).max())
             d = np.sqrt((d ** 2).sum(1))
+        elif name == "rho":
+            scale = float(p.rho0)
         else:
             scale = float(ref.max() - ref.min())
         scale = max(scale, 1e-300)
