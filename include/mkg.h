@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define MKG_ABI_VERSION 3
+#define MKG_ABI_VERSION 4
 
 /* ---- node flags: identical codes to MaterKit/inc/mkGlobalDefs.h:140-145 ---- */
 #define MKG_IS_OUTLET (-2)
@@ -68,6 +68,14 @@ extern "C" {
 #define MKG_BC_OUTFLOW 3   /* unknown incoming populations copied from the cell itself */
 #define MKG_BC_PRESSURE 4  /* flow only: anti-bounce-back to the equilibrium at density rho0 and
                               the cell's velocity (pressure outlet) */
+#define MKG_BC_VELOCITY 5  /* flow only: velocity face by non-equilibrium extrapolation
+                              (mkFluidsUser::boundary_vel, MaterKit/src/mkFluidsUser.cpp:
+                              455-505; bdyVelocity, MaterKit/inc/mkFluidsLBM.h:63): after the
+                              collision, each cell of the face layer takes the post-collision
+                              populations of its inward neighbour n plus f_eq(rho_n, u_b) -
+                              f_eq(rho_n, u_n), u_n = sum c f*_n / rho_n, u_b the face velocity;
+                              populations entering the face layer from outside are its own
+                              (zero gradient).  Two velocity faces must not share an edge. */
 
 /* ---- anisotropy forms a_s(n), n the unit normal in the crystal frame ---- */
 #define MKG_ANISO_CUBIC 0 /* 1 - 3 eps4 + 4 eps4 sum n^4 (Karma-Rappel), eps4 */
@@ -166,6 +174,12 @@ typedef struct mkg_params {
     /* deviation references for FP32 storage */
     double ref_f, ref_g, ref_h, ref_c;
     double rot[MKG_MAX_GRAINS][9]; /* row-major lab<-crystal rotation per grain */
+    /* rotating walls (bdyRotation / bdySpiral, MaterKit/inc/mkFluidsLBM.h:64-65; angVel_0,
+       :180): the velocity of a bounce or velocity face at the wall point r is
+       u_face + the tangential part of wall_omega x (r - wall_center), lattice units, with
+       r = the cell centre moved half a cell through the face */
+    double wall_omega[3];
+    double wall_center[3];
 } mkg_params;
 
 /* Phase-field model of the phase lattice.

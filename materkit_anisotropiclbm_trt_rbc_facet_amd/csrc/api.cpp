@@ -65,6 +65,7 @@ struct mkg_ctx {
     void* kappa = nullptr;
     double* rot = nullptr;
     double* scratch = nullptr;  // 27*N doubles, allocated on first use
+    void* nee = nullptr;        // Q x F storage-type values of the velocity faces
     double* partials = nullptr;
     double* red_out = nullptr;
     int has_walls = 0;
@@ -179,7 +180,29 @@ StepArgs make_args(const mkg_ctx* h) {
     a.omega_g = 1.0 / (0.5 + p.pf_dt * p.pf_wdx2 / cs2g);
     a.pf_model = p.pf_model;
     a.ac_zc = p.pf_wdx2 > 0 ? (1.0 - 0.5 * a.omega_g) / (std::sqrt(2.0) * std::sqrt(p.pf_wdx2)) : 0.0;
+    for (int k = 0; k < 3; k++) {
+        a.wall_omega[k] = p.wall_omega[k];
+        a.wall_center[k] = p.wall_center[k];
+    }
+    a.wall_rot = (p.wall_omega[0] != 0.0 || p.wall_omega[1] != 0.0 || p.wall_omega[2] != 0.0) ? 1 : 0;
+    a.nee_mask = 0;
+    for (int f = 0; f < 6; f++)
+        if (p.bc_f[f] == MKG_BC_VELOCITY) a.nee_mask |= 1 << f;
+    a.nee = h->nee;
     return a;
+}
+
+bool has_velocity_faces(const mkg_params& p) {
+    for (int f = 0; f < 6; f++)
+        if (p.bc_f[f] == MKG_BC_VELOCITY) return true;
+    return false;
+}
+
+// Buffer of the velocity faces' replaced populations (Q x F), when the flow has such faces.
+int ensure_nee(mkg_ctx* h, const mkg_params& p) {
+    if (h->nee || !p.flow_q || !has_velocity_faces(p)) return 0;
+    HIPCHK(hipMalloc(&h->nee, (size_t)p.flow_q * h->F * h->esz));
+    return 0;
 }
 
 int kind_of(int L) { return L == LF ? 0 : 1; }
@@ -231,10 +254,22 @@ int validate(const mkg_params* p) {
     const int32_t* bcs[4] = {p->bc_f, p->bc_g, p->bc_h, p->bc_c};
     for (int L = 0; L < 4; L++)
         for (int f = 0; f < 6; f++)
-            if (bcs[L][f] < 0 || bcs[L][f] > 4) return fail("face boundary code out of range");
+            if (bcs[L][f] < 0 || bcs[L][f] > 5) return fail("face boundary code out of range");
     for (int L = 1; L < 4; L++)
         for (int f = 0; f < 6; f++)
-            if (bcs[L][f] == MKG_BC_PRESSURE) return fail("pressure faces are for the flow lattice");
+            if (bcs[L][f] == MKG_BC_PRESSURE || bcs[L][f] == MKG_BC_VELOCITY)
+                return fail("pressure and velocity faces are for the flow lattice");
+    {
+        const int dims[3] = {p->nx, p->ny, p->nz};
+        for (int f = 0; f < 6; f++) {
+            if (p->bc_f[f] != MKG_BC_VELOCITY) continue;
+            if (dims[f / 2] < 3)
+                return fail("a velocity face needs at least three cells along its normal");
+            for (int g = 0; g < 6; g++)
+                if (g / 2 != f / 2 && p->bc_f[g] == MKG_BC_VELOCITY)
+                    return fail("velocity faces must not share an edge");
+        }
+    }
     for (int a = 0; a < 3; a++) {
         // a face pair must be periodic on both sides or on neither
         for (int L = 0; L < 4; L++)
@@ -310,6 +345,8 @@ int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid) {
     a.g.adv = adv[LG];
     a.h.adv = adv[LH];
     a.c.adv = adv[LC];
+    if (adv[LF] && a.nee_mask && launch_nee(k, h->fp32, a, h->stream))
+        return fail("velocity-face kernel unavailable");
     if (launch_step(k, h->fp32, lay0, a, h->stream))
         return fail("no kernel instantiation for this lattice combination");
     if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
@@ -439,6 +476,7 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
         hipStreamSynchronize(h->stream);
         hipFree(tmp);
     }
+    if (ensure_nee(h, *p)) return cleanup(-1);
     if (alloc_zero((void**)&h->rot, sizeof(p->rot), h->stream)) return cleanup(-1);
     if (hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream) !=
         hipSuccess)
@@ -463,6 +501,7 @@ int mkg_destroy(mkg_handle h) {
     hipFree(h->kappa);
     hipFree(h->rot);
     hipFree(h->scratch);
+    hipFree(h->nee);
     hipFree(h->partials);
     hipFree(h->red_out);
     for (auto e : h->events) hipEventDestroy(e);
@@ -491,6 +530,7 @@ int mkg_set_params(mkg_handle h, const mkg_params* p) {
     // bring every lattice to the own-slot layout before the new parameters apply
     for (int L = 0; L < 4; L++)
         if (normalize(h, L)) return -1;
+    if (ensure_nee(h, *p)) return -1;
     h->p = *p;
     HIPCHK(hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));

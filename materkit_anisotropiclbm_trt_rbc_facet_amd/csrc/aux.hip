@@ -8,6 +8,12 @@ int launch_step_f32(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
 int launch_step_f64(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
 int launch_phase_moments_f32(int pq, int odd, const StepArgs& a, void* out, hipStream_t s);
 int launch_phase_moments_f64(int pq, int odd, const StepArgs& a, void* out, hipStream_t s);
+int launch_nee_f32(const Combo& k, const StepArgs& a, hipStream_t s);
+int launch_nee_f64(const Combo& k, const StepArgs& a, hipStream_t s);
+
+int launch_nee(const Combo& k, int fp32, const StepArgs& a, hipStream_t s) {
+    return fp32 ? launch_nee_f32(k, a, s) : launch_nee_f64(k, a, s);
+}
 
 int launch_step(const Combo& k, int fp32, int odd, const StepArgs& a, hipStream_t s) {
     return fp32 ? launch_step_f32(k, odd, a, s) : launch_step_f64(k, odd, a, s);

@@ -168,7 +168,8 @@ bool load_case(const char* path, Case& c) {
 
 int bc_code(const std::string& w) {
     if (w == "periodic") return MKG_BC_PERIODIC;
-    if (w == "bounce" || w == "wall" || w == "noflux" || w == "velocity") return MKG_BC_BOUNCE;
+    if (w == "bounce" || w == "wall" || w == "noflux") return MKG_BC_BOUNCE;
+    if (w == "velocity") return MKG_BC_VELOCITY;
     if (w == "dirichlet") return MKG_BC_DIRICHLET;
     if (w == "outflow") return MKG_BC_OUTFLOW;
     if (w == "pressure") return MKG_BC_PRESSURE;
@@ -421,6 +422,18 @@ int mkg_params_from_xml(const char* path, mkg_params* p, int32_t* run_info) {
     auto uin = c.nums("solver/inflow");
     if (uin.size() == 3)
         for (int k = 0; k < 3; k++) p->u_face[0][k] = uin[k];
+    // rotating walls (bdyRotation / bdySpiral): angular velocity and the point on the axis
+    auto wr = c.nums("solver/wallRotation");
+    if (wr.size() == 3)
+        for (int k = 0; k < 3; k++) p->wall_omega[k] = wr[k];
+    auto wc = c.nums("solver/wallCenter");
+    if (wc.size() == 3) {
+        for (int k = 0; k < 3; k++) p->wall_center[k] = wc[k];
+    } else {
+        p->wall_center[0] = 0.5 * (p->nx - 1);
+        p->wall_center[1] = 0.5 * (p->ny - 1);
+        p->wall_center[2] = 0.5 * (p->nz - 1);
+    }
 
     // ---- seeds / grains
     int nseed = 0;

@@ -114,6 +114,32 @@ __device__ inline double face_cu(int f, int X, int Y, int Z, const double (*uf)[
     return r;
 }
 
+// Rotating-wall part of the wall velocity of face f next to cell c: the tangential part of
+// wall_omega x (r - wall_center), r the wall point half a cell through the face
+// (bdyRotation / bdySpiral, MaterKit/inc/mkFluidsLBM.h:64-65).
+__device__ inline void wall_rotation(const StepArgs& a, int f, const Cell& c, double* t) {
+    const int ax = f >> 1;
+    const double sg = (f & 1) ? 0.5 : -0.5;
+    const double r0 = c.x + (ax == 0 ? sg : 0.0) - a.wall_center[0];
+    const double r1 = c.y + (ax == 1 ? sg : 0.0) - a.wall_center[1];
+    const double r2 = c.z + (ax == 2 ? sg : 0.0) - a.wall_center[2];
+    const double* w = a.wall_omega;
+    t[0] = ax == 0 ? 0.0 : w[1] * r2 - w[2] * r1;
+    t[1] = ax == 1 ? 0.0 : w[2] * r0 - w[0] * r2;
+    t[2] = ax == 2 ? 0.0 : w[0] * r1 - w[1] * r0;
+}
+
+// c_i . u_w for the wall of face f next to cell c: the face velocity plus the rotating part.
+__device__ inline double wall_cu(const StepArgs& a, int f, const Cell& c, int X, int Y, int Z) {
+    double cu = face_cu(f, X, Y, Z, a.u_face);
+    if (a.wall_rot) {
+        double t[3];
+        wall_rotation(a, f, c, t);
+        cu += X * t[0] + Y * t[1] + Z * t[2];
+    }
+    return cu;
+}
+
 __device__ inline uint32_t nbr(const Cell& c, int dx, int dy, int dz) {
     return c.ox[dx + 1] + c.oy[dy + 1] + c.oz[dz + 1];
 }
@@ -192,9 +218,10 @@ struct FaceState {
 //              velocity u: 2 w rho0 (1 + 4.5 (c.u)^2 - 1.5 u.u) - f*_inv(i) (flow outlet)
 template <int Q, class R>
 __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const StepArgs& a,
-                            int kind, const FaceState<R>& fs) {
+                            int kind, const FaceState<R>& fs, const Cell& c) {
     const int t = pick6(f, L.bc);
-    if (t == kBcOutflow) return p[i];
+    // outflow, and the layer of a velocity face, which continues itself
+    if (t == kBcOutflow || (t == kBcVelocity && kind == kKindFlow)) return p[i];
     if (t == kBcPressure) {
         const R cu = cx(i) * fs.ux + cy(i) * fs.uy + cz(i) * fs.uz;
         const R uu = fs.ux * fs.ux + fs.uy * fs.uy + fs.uz * fs.uz;
@@ -205,7 +232,7 @@ __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const St
         return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)pick6(f, L.fval) - (R)L.ref);
     R v = p[inv(i)];
     if (kind == kKindFlow) {
-        const double cu = face_cu(f, cx(i), cy(i), cz(i), a.u_face);
+        const double cu = wall_cu(a, f, c, cx(i), cy(i), cz(i));
         v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu) * fs.wscale;
     }
     return v;
@@ -262,7 +289,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
                 if (hit) {
                     const int f = __builtin_ctz(hit);
                     stp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, f)],
-                        face_in<Q, R>(p, i, f, L, a, kind, fs));
+                        face_in<Q, R>(p, i, f, L, a, kind, fs, c));
                 }
             }
         }
@@ -280,7 +307,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
             } else {
                 const int face = __builtin_ctz(hit);
                 stp(slot_at(base + (size_t)inv(i) * S, c.id),
-                    face_in<Q, R>(p, inv(i), face, L, a, kind, fs));
+                    face_in<Q, R>(p, inv(i), face, L, a, kind, fs, c));
             }
         }
     }
@@ -381,7 +408,7 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, con
         if (hit) {
             if (act)
                 stp(slot_at(base + (size_t)inv(i) * S, c.id),
-                    face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs));
+                    face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs, c));
         } else {
             const uint32_t tgt = nbr(c, dx, dy, dz);
             if (FACES && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {

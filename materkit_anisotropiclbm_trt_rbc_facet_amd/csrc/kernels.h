@@ -47,6 +47,13 @@ struct StepArgs {
     int32_t aniso_kind, pf_model;
     double aniso[2];
     double ac_zc;  // Allen-Cahn counter-term scale (1 - omega_g/2) / (sqrt 2 W0/dx)
+    // rotating walls: wall velocity u_face + tangential part of wall_omega x (r - wall_center)
+    int32_t wall_rot;  // wall_omega != 0
+    int32_t nee_mask;  // bit f: face f of the flow lattice is a velocity face
+    double wall_omega[3], wall_center[3];
+    // velocity faces: the replaced post-collision populations of each face cell (storage type,
+    // Q x F, slot i of face cell k at i * F + k), written by the pre-pass of the step
+    void* nee;
 };
 
 // Selector of a fused-step instantiation: presence of each lattice.
@@ -56,6 +63,9 @@ struct Combo {
 
 // Returns 0 on success, -1 when the combination has no instantiation.
 int launch_step(const Combo& k, int fp32, int odd, const StepArgs& a, hipStream_t s);
+// Pre-pass of a step with velocity faces: the post-collision populations every face cell of a
+// velocity face takes (non-equilibrium extrapolation from its inward neighbour), into a.nee.
+int launch_nee(const Combo& k, int fp32, const StepArgs& a, hipStream_t s);
 int launch_phase_moments(int pq, int fp32, int odd, const StepArgs& a, void* phi_out,
                          hipStream_t s);
 

@@ -49,6 +49,7 @@ constexpr int kIsOutlet = -2, kIsInlet = -1, kIsFluid = 0, kIsInter = 1, kIsSoli
               kIsWalls = 3;
 
 // Face boundary codes (faces x-, x+, y-, y+, z-, z+).
-constexpr int kBcPeriodic = 0, kBcBounce = 1, kBcDirichlet = 2, kBcOutflow = 3, kBcPressure = 4;
+constexpr int kBcPeriodic = 0, kBcBounce = 1, kBcDirichlet = 2, kBcOutflow = 3, kBcPressure = 4,
+              kBcVelocity = 5;
 
 }  // namespace mkg

@@ -14,6 +14,17 @@ int launch_step_f64(const Combo& k, int odd, const StepArgs& a, hipStream_t s) {
     return -1;
 }
 
+#define MKG_NEE_CASE(FQ, PQ, TH, SO)                                      \
+    if (k.fq == FQ && k.pq == PQ && k.th == TH && k.so == SO) {           \
+        run_nee<double, FQ, PQ, (bool)TH>(a, s);                              \
+        return 0;                                                         \
+    }
+
+int launch_nee_f64(const Combo& k, const StepArgs& a, hipStream_t s) {
+    MKG_COMBOS(MKG_NEE_CASE)
+    return -1;
+}
+
 int launch_phase_moments_f64(int pq, int odd, const StepArgs& a, void* out, hipStream_t s) {
     if (pq == 19) {
         run_phase_moments<double, 19>(odd, a, out, s);

@@ -62,6 +62,142 @@ __device__ inline void stencil_gradient(const R (&P)[27], R (&gr)[3]) {
     }
 }
 
+// Velocity faces (non-equilibrium extrapolation, mkFluidsUser::boundary_vel,
+// MaterKit/src/mkFluidsUser.cpp:455-505): a face cell of a velocity face replaces its
+// post-collision populations by the ones the pre-pass k_nee formed from its inward neighbour
+// (unless that neighbour is a wall cell), before they stream.
+template <int Q, class R>
+__device__ inline void nee_replace(R* f, const StepArgs& a, const Cell& c) {
+    const uint32_t hit = c.bits & (uint32_t)a.nee_mask;
+    if (!hit) return;
+    const int fc = __builtin_ctz(hit);
+    const int ax = fc >> 1, sg = (fc & 1) ? -1 : 1;
+    const uint32_t n = nbr(c, ax == 0 ? sg : 0, ax == 1 ? sg : 0, ax == 2 ? sg : 0);
+    if (a.flag[n] == kIsWalls) return;
+    const R* __restrict__ nb = (const R*)a.nee;
+    const uint32_t k = face_slot(a, c, fc);
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = nb[(size_t)i * a.F + k];
+}
+
+// Pre-pass of a step with velocity faces, one thread per face-buffer index: for a cell b of
+// a velocity face and its inward neighbour n, the collision of n exactly as k_step forms it
+// (moments, body force with buoyancy, solid fraction, generalised moment relaxation), then
+// f*_b = f*_n + f_eq(rho_n, u_b) - f_eq(rho_n, u_n) with u_n = sum c f*_n / rho_n (:485-500)
+// and u_b the face velocity; the equilibrium is the case's (compressible or He-Luo).
+template <class R, int FQ, bool TH, bool PH>
+__global__ void __launch_bounds__(256) k_nee(const StepArgs a) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k >= a.F) return;
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+    const uint32_t A = ny * nz, B = nx * nz, C = nx * ny;
+    int fc;
+    uint32_t x, y, z;
+    if (k < 2u * A) {  // the inverse of face_slot
+        fc = (int)(k / A);
+        const uint32_t r = k % A;
+        y = r / nz;
+        z = r % nz;
+        x = (fc & 1) ? nx - 1u : 0u;
+    } else if (k < 2u * (A + B)) {
+        const uint32_t q = k - 2u * A;
+        fc = 2 + (int)(q / B);
+        const uint32_t r = q % B;
+        x = r / nz;
+        z = r % nz;
+        y = (fc & 1) ? ny - 1u : 0u;
+    } else {
+        const uint32_t q = k - 2u * (A + B);
+        fc = 4 + (int)(q / C);
+        const uint32_t r = q % C;
+        x = r / ny;
+        y = r % ny;
+        z = (fc & 1) ? nz - 1u : 0u;
+    }
+    if (!((a.nee_mask >> fc) & 1)) return;
+    Cell cb, cn;
+    locate_at(a, (x * ny + y) * nz + z, cb);
+    const int ax = fc >> 1, sg = (fc & 1) ? -1 : 1;
+    locate_at(a, nbr(cb, ax == 0 ? sg : 0, ax == 1 ? sg : 0, ax == 2 ? sg : 0), cn);
+    const int8_t fl = a.flag[cn.id];
+    if (a.flag[cb.id] == kIsWalls || fl == kIsWalls) return;
+    const uint32_t S = a.S;
+    R ns = 0;
+    if constexpr (PH) {
+        const R phi = *slot_at((const R*)a.phi_cur, cn.id);
+        if (fl == kIsSolid)
+            ns = (R)1;
+        else if (fl == kIsInter)
+            ns = clamp01((R)0.5 * ((R)1 + phi));
+    }
+    R T = 0;
+    if constexpr (TH) {
+        R h[7];
+        read_pops_rt<7, R>(h, a.h, S, cn, a, kKindScalar);
+        R sh = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) sh += h[i];
+        T = (R)a.h.ref + sh;
+    }
+    R f[FQ];
+    read_pops_rt<FQ, R>(f, a.f, S, cn, a, kKindFlow);
+    R dr = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < FQ; i++) {
+        dr += f[i];
+        if (cx(i)) jx += (R)cx(i) * f[i];
+        if (cy(i)) jy += (R)cy(i) * f[i];
+        if (cz(i)) jz += (R)cz(i) * f[i];
+    }
+    const R rho = (R)a.f.ref + dr;
+    R Fx = (R)a.force[0], Fy = (R)a.force[1], Fz = (R)a.force[2];
+    if constexpr (TH) {
+        const R dT = T - (R)a.T_ref;
+        Fx += (R)(a.rho0 * a.buoy[0]) * dT;
+        Fy += (R)(a.rho0 * a.buoy[1]) * dT;
+        Fz += (R)(a.rho0 * a.buoy[2]) * dT;
+    }
+    const R den = a.incomp ? (R)a.rho0 : rho;
+    const R ux = (jx + (R)0.5 * Fx) / den, uy = (jy + (R)0.5 * Fy) / den,
+            uz = (jz + (R)0.5 * Fz) / den;
+    flow_collide<FQ, R>(f, dr, den, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk, (R)a.s_odd,
+                        (R)a.s_even, ns);
+    // the extrapolation: rho_n and u_n of the post-collision populations
+    R pr = 0, px = 0, py = 0, pz = 0;
+#pragma unroll
+    for (int i = 0; i < FQ; i++) {
+        pr += f[i];
+        if (cx(i)) px += (R)cx(i) * f[i];
+        if (cy(i)) py += (R)cy(i) * f[i];
+        if (cz(i)) pz += (R)cz(i) * f[i];
+    }
+    const R rn = (R)a.f.ref + pr;
+    const R vx = px / rn, vy = py / rn, vz = pz / rn;
+    // face velocity (selected without a runtime index into the argument block)
+    double ub[3] = {0, 0, 0};
+#pragma unroll
+    for (int f6 = 0; f6 < 6; f6++)
+        if (fc == f6)
+            for (int d = 0; d < 3; d++) ub[d] = a.u_face[f6][d];
+    if (a.wall_rot) {
+        double t[3];
+        wall_rotation(a, fc, cb, t);
+        for (int d = 0; d < 3; d++) ub[d] += t[d];
+    }
+    const R bx = (R)ub[0], by = (R)ub[1], bz = (R)ub[2];
+    const R de = a.incomp ? (R)a.rho0 : rn;  // density of the velocity terms
+    const R uu_b = bx * bx + by * by + bz * bz, uu_n = vx * vx + vy * vy + vz * vz;
+    R* __restrict__ out = (R*)a.nee;
+#pragma unroll
+    for (int i = 0; i < FQ; i++) {
+        const R cb_ = cx(i) * bx + cy(i) * by + cz(i) * bz;
+        const R cn_ = cx(i) * vx + cy(i) * vy + cz(i) * vz;
+        const R d = (R)wgt<FQ>(i) * de *
+                    ((R)3 * (cb_ - cn_) + (R)4.5 * (cb_ * cb_ - cn_ * cn_) - (R)1.5 * (uu_b - uu_n));
+        out[(size_t)i * a.F + k] = f[i] + d;
+    }
+}
+
 template <class R, int PQ, bool ODD>
 __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __restrict__ phi_out) {
     Cell c;
@@ -259,6 +395,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         if (a.f.adv) {
             flow_collide<FQ, R>(f, dr, den, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
                                 (R)a.s_odd, (R)a.s_even, ns);
+            if constexpr (FACES) nee_replace<FQ, R>(f, a, c);
             FaceState<R> fs;
             fs.wscale = (R)1 - ns;
             fs.ux = ux;
@@ -461,6 +598,14 @@ inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
             k_step<R, FQ, PQ, TH, SO, false, false, true><<<grid, block, 0, s>>>(a);
         else
             k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);
+    }
+}
+
+template <class R, int FQ, int PQ, bool TH>
+inline void run_nee(const StepArgs& a, hipStream_t s) {
+    if constexpr (FQ > 0) {
+        dim3 grid((a.F + 255u) / 256u), block(256);
+        k_nee<R, FQ, TH, (PQ > 0)><<<grid, block, 0, s>>>(a);
     }
 }
 

@@ -19,7 +19,7 @@ MAX_GRAINS = 64
 IS_OUTLET, IS_INLET, IS_FLUID, IS_INTER, IS_SOLID, IS_WALLS = -2, -1, 0, 1, 2, 3
 OP_BGK, OP_TRT, OP_MRT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
-BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW, BC_PRESSURE = 0, 1, 2, 3, 4
+BC_PERIODIC, BC_BOUNCE, BC_DIRICHLET, BC_OUTFLOW, BC_PRESSURE, BC_VELOCITY = 0, 1, 2, 3, 4, 5
 ANISO_CUBIC, ANISO_CBC, ANISO_HCP, ANISO_FACET = 0, 1, 2, 3
 PF_KARMA_RAPPEL, PF_ALLEN_CAHN = 0, 1
 FLOW, PHASE, THERMAL, SOLUTE = 1, 2, 4, 8
@@ -51,6 +51,7 @@ PARAM_FIELDS = [
     ("ref_f", C.c_double), ("ref_g", C.c_double), ("ref_h", C.c_double),
     ("ref_c", C.c_double),
     ("rot", (C.c_double * 9) * MAX_GRAINS),
+    ("wall_omega", C.c_double * 3), ("wall_center", C.c_double * 3),
 ]
 
 

@@ -78,9 +78,10 @@ public:
     virtual void bdyPeriod() {}
     virtual void bdyVelocity() {}
     virtual void bdyCouette() {}  // moving walls are the case file's face velocities
-    // walls whose velocity varies along the wall have no counterpart (uniform face velocities)
-    virtual void bdyRotation() { throw mkt::mkLogicException("bdyRotation: no rotating walls"); }
-    virtual void bdySpiral() { throw mkt::mkLogicException("bdySpiral: no spiral walls"); }
+    // rotating / spiral walls are the case file's solver/wallRotation (+ the face velocities
+    // for the axial part of a spiral), applied by the face rules in every step
+    virtual void bdyRotation() {}
+    virtual void bdySpiral() {}
     // parameters of a case file without creating the device state (initialize does both)
     virtual void read_parameters(const std::string& xmlfname);
 

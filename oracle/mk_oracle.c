@@ -43,7 +43,8 @@
  *   4  Boussinesq force with twice its factor
  *   5  Boussinesq force with its sign flipped
  *   6  D3Q7 solute advective equilibrium with cs^2 = 1/3
- *   7  solute rejection source with its sign flipped */
+ *   7  solute rejection source with its sign flipped
+ *   8  velocity face with f_eq(rho_n, u_n) - f_eq(rho_n, u_b) (the difference reversed) */
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
 #endif
@@ -577,6 +578,21 @@ void orc_moments(const orc_params* p, const orc_state* s, double* rho, double* v
  * 1 - n_s, so that a solid cell on an inlet receives no mass.  dscale (scalar, may be NULL =
  * 1): per-cell factor of the Dirichlet value (A(phi) for the solute, whose faces fix U).
  * vel (flow): the cells' fluid velocities, for pressure faces (code 4). */
+/* Velocity of the wall of face f next to cell (x, y, z): u_face[f] plus the tangential part
+ * of wall_omega x (r - wall_center), r the wall point half a cell through the face
+ * (rotating walls, bdyRotation / bdySpiral, MaterKit/inc/mkFluidsLBM.h:64-65). */
+static void wall_velocity(const orc_params* p, int f, int x, int y, int z, double* uw) {
+    double nrm[3] = {0, 0, 0};
+    nrm[f / 2] = (f & 1) ? 1.0 : -1.0;
+    double r[3] = {x + 0.5 * nrm[0] - p->wall_center[0], y + 0.5 * nrm[1] - p->wall_center[1],
+                   z + 0.5 * nrm[2] - p->wall_center[2]};
+    const double* w = p->wall_omega;
+    double rot[3] = {w[1] * r[2] - w[2] * r[1], w[2] * r[0] - w[0] * r[2],
+                     w[0] * r[1] - w[1] * r[0]};
+    double rn = rot[0] * nrm[0] + rot[1] * nrm[1] + rot[2] * nrm[2];
+    for (int a = 0; a < 3; a++) uw[a] = p->u_face[f][a] + rot[a] - rn * nrm[a];
+}
+
 static void stream(const orc_params* p, const orc_state* s, int q, const int* bc, int kind,
                    const double* fvals, const double* wscale, const double* dscale,
                    const double* vel, const double* post, const double* old, double* out) {
@@ -598,7 +614,8 @@ static void stream(const orc_params* p, const orc_state* s, int q, const int* bc
                     double v;
                     if (face >= 0) {
                         int t = bc[face];
-                        if (t == 3) {
+                        if (t == 3 || (t == 5 && kind == 0)) {
+                            /* outflow; the layer of a velocity face continues itself */
                             v = post[i * N + id];
                         } else if (t == 4 && kind == 0) {
                             /* pressure outlet: anti-bounce-back to the equilibrium at rho0 and
@@ -614,7 +631,8 @@ static void stream(const orc_params* p, const orc_state* s, int q, const int* bc
                         } else {
                             v = post[k * N + id];
                             if (kind == 0) {
-                                const double* uw = p->u_face[face];
+                                double uw[3];
+                                wall_velocity(p, face, x, y, z, uw);
                                 double cu = E27[i][0] * uw[0] + E27[i][1] * uw[1] + E27[i][2] * uw[2];
                                 v += 6.0 * L.w[i] * p->rho0 * cu * (wscale ? wscale[id] : 1.0);
                             }
@@ -635,6 +653,53 @@ static int8_t classify(const orc_params* p, int8_t old, double phi) {
     if (phi >= p->phi_s) return 2;
     if (phi > -p->phi_s) return 1;
     return 0;
+}
+
+/* Velocity faces by non-equilibrium extrapolation (mkFluidsUser::boundary_vel,
+ * MaterKit/src/mkFluidsUser.cpp:455-505): every non-wall cell b of the face layer takes the
+ * post-collision populations of its inward neighbour n (unless n is a wall cell) plus
+ * f_eq(rho_n, u_b) - f_eq(rho_n, u_n), with rho_n = sum f*_n, u_n = sum c f*_n / rho_n (:485-500)
+ * and u_b the face velocity.  Velocity faces share no edge, so no n is itself replaced. */
+static void nee_faces(const orc_params* p, orc_state* s, int q, double* post) {
+    long N = NCELL(p);
+    int dims[3] = {p->nx, p->ny, p->nz};
+    for (int f = 0; f < 6; f++) {
+        if (p->bc_f[f] != 5) continue;
+        int ax = f / 2, side = f & 1;
+        int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+        for (int u1 = 0; u1 < dims[a1]; u1++)
+            for (int u2 = 0; u2 < dims[a2]; u2++) {
+                int cb[3], cn[3];
+                cb[ax] = side ? dims[ax] - 1 : 0;
+                cb[a1] = u1;
+                cb[a2] = u2;
+                for (int k = 0; k < 3; k++) cn[k] = cb[k];
+                cn[ax] += side ? -1 : 1;
+                long b = lin(p, cb[0], cb[1], cb[2]), n = lin(p, cn[0], cn[1], cn[2]);
+                if (s->flag[b] == 3 || s->flag[n] == 3) continue;
+                double rho = 0, j[3] = {0, 0, 0};
+                for (int i = 0; i < q; i++) {
+                    double v = post[i * N + n];
+                    rho += v;
+                    for (int d = 0; d < 3; d++) j[d] += E27[i][d] * v;
+                }
+                double un[3] = {j[0] / rho, j[1] / rho, j[2] / rho}, ub[3];
+                wall_velocity(p, f, cb[0], cb[1], cb[2], ub);
+                double eb[27], en[27];
+                if (p->flow_incomp) {
+                    orc_equilibrium_he(q, rho, p->rho0, ub, eb);
+                    orc_equilibrium_he(q, rho, p->rho0, un, en);
+                } else {
+                    orc_equilibrium(q, rho, ub, eb);
+                    orc_equilibrium(q, rho, un, en);
+                }
+#if ORC_MUTANT == 8
+                for (int i = 0; i < q; i++) post[i * N + b] = post[i * N + n] - eb[i] + en[i];
+#else
+                for (int i = 0; i < q; i++) post[i * N + b] = post[i * N + n] + eb[i] - en[i];
+#endif
+            }
+    }
 }
 
 /* ------------------------------------------------------------------ one step */
@@ -684,6 +749,7 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
             orc_collide_cell(p, op, fin, F, fo, m.ns[x]);
             for (int i = 0; i < q; i++) post[i * N + x] = fo[i];
         }
+        nee_faces(p, s, q, post);
         double* lf = (double*)malloc((size_t)N * sizeof(double));
         for (long x = 0; x < N; x++) lf[x] = 1.0 - m.ns[x];
         stream(p, s, q, p->bc_f, 0, NULL, lf, NULL, m.u, post, s->f, fresh);
@@ -871,6 +937,15 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
 int orc_step(const orc_params* p, orc_state* s, int mask, int op, int nsteps) {
     if (p->flow_q && p->flow_q != 19 && p->flow_q != 27) return -1;
     if (p->phase_q && p->phase_q != 7 && p->phase_q != 19) return -1;
+    for (int f = 0; f < 6; f++) {
+        if (p->bc_g[f] == 5 || p->bc_h[f] == 5 || p->bc_c[f] == 5) return -1;
+        /* velocity faces share no edge and stand at least three cells apart */
+        if (p->bc_f[f] != 5) continue;
+        int dims[3] = {p->nx, p->ny, p->nz};
+        if (dims[f / 2] < 3) return -1;
+        for (int g = 0; g < 6; g++)
+            if (g / 2 != f / 2 && p->bc_f[g] == 5) return -1;
+    }
     if (p->flow_q == 19) build_mrt(19, &MRT19);
     if (p->flow_q == 27) build_mrt(27, &MRT27);
     for (int n = 0; n < nsteps; n++) one_step(p, s, mask, op);

@@ -41,6 +41,8 @@ typedef struct orc_params {
     double aniso[2];
     double ref_f, ref_g, ref_h, ref_c;
     double rot[ORC_MAX_GRAINS][9];
+    double wall_omega[3];
+    double wall_center[3];
 } orc_params;
 
 /* Collision operator selector for the flow lattice (the oracle keeps three separate

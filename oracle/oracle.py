@@ -45,7 +45,7 @@ def lib() -> ct.CDLL:
 
 # Planted mistakes of mk_oracle.c (ORC_MUTANT values; the file lists what each one changes).
 MUTANTS = {"thermal_cs2": 1, "latent_sign": 2, "latent_factor": 3, "buoyancy_factor": 4,
-           "buoyancy_sign": 5, "solute_cs2": 6, "rejection_sign": 7}
+           "buoyancy_sign": 5, "solute_cs2": 6, "rejection_sign": 7, "velocity_face_sign": 8}
 _mutant_libs: dict = {}
 
 

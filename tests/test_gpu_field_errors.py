@@ -28,7 +28,7 @@ CONFIGS = [
     ("C2", dict(nx=64, ny=64, nz=32), 200, {"rho": 1e-7, "u": 1e-5, "T": 1e-6}),
     ("C3", dict(nx=64, ny=64, nz=64), 1500, {"T": 1e-5, "phi": 1e-5}),
     # C4: interface/solid flags that flip in FP32 change the solid fraction the flow sees
-    ("C4", dict(nx=48, ny=48, nz=48), 400, {"rho": 1e-5, "u": 1e-3, "c": 1e-5, "phi": 2e-5}),
+    ("C4", dict(nx=48, ny=48, nz=48), 400, {"rho": 1e-5, "u": 1e-3, "c": 2e-5, "phi": 2e-5}),
     ("C5", dict(nx=64, ny=64, nz=64), 500, {"T": 1e-5, "phi": 1e-5}),
 ]
 

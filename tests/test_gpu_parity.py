@@ -13,8 +13,8 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-P, BB, DI, OUT, PR = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_OUTFLOW,
-                      mkg.BC_PRESSURE)
+P, BB, DI, OUT, PR, VE = (mkg.BC_PERIODIC, mkg.BC_BOUNCE, mkg.BC_DIRICHLET, mkg.BC_OUTFLOW,
+                          mkg.BC_PRESSURE, mkg.BC_VELOCITY)
 
 
 def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodic",
@@ -63,6 +63,15 @@ def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodi
               T_face=[0.3, 0, 0, 0, -0.2, 0], C_face=[-0.4, 0, 0, 0, 0, 0])
         p.u_face[0][0] = 0.02
         p.u_face[5][0], p.u_face[5][1] = 0.01, 0.005
+    if bcs == "velocity":
+        # velocity inlet (non-equilibrium extrapolation) and pressure outlet along x, walls
+        # along z rotating about a tilted axis through the domain centre
+        p.set(bc_f=[VE, PR, P, P, BB, BB], bc_g=[OUT, OUT, P, P, BB, BB],
+              bc_h=[DI, OUT, P, P, DI, BB], bc_c=[DI, OUT, P, P, BB, BB],
+              T_face=[0.3, 0, 0, 0, -0.2, 0], C_face=[-0.4, 0, 0, 0, 0, 0],
+              wall_omega=[0.002, -0.001, 0.0015],
+              wall_center=[(dims[0] - 1) / 2.0, (dims[1] - 1) / 2.0, (dims[2] - 1) / 2.0])
+        p.u_face[0][0], p.u_face[0][1], p.u_face[0][2] = 0.02, 0.003, -0.001
     return p
 
 
@@ -278,6 +287,25 @@ FUSED_CASES = [c for c in CASES if c.get("flow_q") and c.get("phase_q")]
 def test_nz256_odd_grid_with_walls_matches_oracle(case, bcs, prec):
     """2 x 3 x 256 with wall cells, 7 steps (ends in the shifted layout)."""
     errs, fd, fo, dev, orc = _run(case, bcs, 7, prec=prec, walls=True, dims=(2, 3, 256))
+    dev.close()
+    if prec == mkg.PREC_FP64:
+        assert np.array_equal(fd, fo), "flags differ"
+    tol = 1e-12 if prec == mkg.PREC_FP64 else 1e-5
+    for k, e in errs.items():
+        assert e < tol, f"{k}: rel err {e:.3e}"
+
+
+FLOW_CASES = [c for c in CASES if c.get("flow_q")]
+
+
+@pytest.mark.parametrize("case", FLOW_CASES, ids=[c["id"] for c in FLOW_CASES])
+@pytest.mark.parametrize("dims", [(10, 9, 8), ZDIMS, NZ256], ids=["10x9x8", "5x3x64", "4x4x256"])
+@pytest.mark.parametrize("prec", [mkg.PREC_FP64, mkg.PREC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("nsteps", [5, 6])
+def test_velocity_faces_and_rotating_walls_match_oracle(case, dims, prec, nsteps):
+    """Velocity inlet by non-equilibrium extrapolation (mkFluidsUser::boundary_vel), pressure
+    outlet, rotating walls; both AA layouts, with wall cells."""
+    errs, fd, fo, dev, orc = _run(case, "velocity", nsteps, prec=prec, walls=True, dims=dims)
     dev.close()
     if prec == mkg.PREC_FP64:
         assert np.array_equal(fd, fo), "flags differ"

@@ -30,8 +30,8 @@ def test_library_exports_every_declared_entry_point():
     missing = decl - exported
     assert not missing, f"declared but not exported: {sorted(missing)}"
     L = mkg.lib()
-    assert L.mkg_abi_version() == 3
-    assert L.mkg_params_size() == 5264
+    assert L.mkg_abi_version() == 4
+    assert L.mkg_params_size() == 5312
 
 
 def test_params_layout_matches_oracle():
