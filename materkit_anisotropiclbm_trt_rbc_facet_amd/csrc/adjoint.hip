@@ -30,6 +30,10 @@ namespace mkg {
 //   dda_kl = |G|^2 d2a/dG_k dG_l = 16 e [3 n_k^2 d_kl - 4 n_k^3 n_l - 4 n_l^3 n_k - s4 d_kl
 //                                         + 6 s4 n_k n_l]
 //   dJ_k/de = |G| (2 a ae n_k + ae da_k + a da_k / e),  ae = -3 + 4 s4.
+// Only the product H v with v = the lab axis in crystal coordinates is needed; it is formed
+// from the dot products n.v, da.v and n^3.v (about a third of the work of the full matrix):
+//   (H v)_k = a^2 v_k + 2a (n_k da.v + da_k n.v) + da_k da.v
+//             + 16 e a (3 n_k^2 v_k - 4 n_k^3 n.v - 4 n_k n^3.v - s4 v_k + 6 s4 n_k n.v).
 // Returns J_axis (lab) and fills dJ/dg (lab, 3) and dJ/de.
 template <class R>
 __device__ inline R face_flux_d(const R (&g)[3], int axis, R eps4, const R (&M)[9], bool rot,
@@ -50,38 +54,32 @@ __device__ inline R face_flux_d(const R (&g)[3], int axis, R eps4, const R (&M)[
         return g[axis];
     }
     const R r = sqrt(r2);
-    R n[3] = {G[0] / r, G[1] / r, G[2] / r};
-    const R s4 = pow4(n[0]) + pow4(n[1]) + pow4(n[2]);
+    const R ir = (R)1 / r;
+    const R n[3] = {G[0] * ir, G[1] * ir, G[2] * ir};
+    const R n3[3] = {n[0] * n[0] * n[0], n[1] * n[1] * n[1], n[2] * n[2] * n[2]};
+    const R s4 = n3[0] * n[0] + n3[1] * n[1] + n3[2] * n[2];
     const R a = (R)1 - (R)3 * eps4 + (R)4 * eps4 * s4;
     const R ae = (R)-3 + (R)4 * s4;
-    R da1[3], da[3], J[3], Je[3];
-    for (int k = 0; k < 3; k++) {
-        da1[k] = (R)16 * (n[k] * n[k] * n[k] - s4 * n[k]);
-        da[k] = eps4 * da1[k];
-        J[k] = r * (a * a * n[k] + a * da[k]);
-        Je[k] = r * ((R)2 * a * ae * n[k] + ae * da[k] + a * da1[k]);
-    }
     // v = row `axis` of M (lab axis in crystal coordinates)
     R v[3];
     for (int k = 0; k < 3; k++) v[k] = rot ? M[axis * 3 + k] : (R)(k == axis);
+    R da1[3], da[3];
+    R nv = 0, dav = 0, n3v = 0, Jl = 0, Jel = 0;
+    for (int k = 0; k < 3; k++) {
+        da1[k] = (R)16 * (n3[k] - s4 * n[k]);
+        da[k] = eps4 * da1[k];
+        nv += n[k] * v[k];
+        dav += da[k] * v[k];
+        n3v += n3[k] * v[k];
+        Jl += v[k] * (a * a * n[k] + a * da[k]);
+        Jel += v[k] * ((R)2 * a * ae * n[k] + ae * da[k] + a * da1[k]);
+    }
+    const R a2 = a * a, c16 = (R)16 * eps4 * a;
     R w[3];
-    for (int k = 0; k < 3; k++) {
-        R s = 0;
-        for (int l = 0; l < 3; l++) {
-            const R dkl = (R)(k == l);
-            const R dda = (R)16 * eps4 *
-                          ((R)3 * n[k] * n[k] * dkl - (R)4 * n[k] * n[k] * n[k] * n[l] -
-                           (R)4 * n[l] * n[l] * n[l] * n[k] - s4 * dkl + (R)6 * s4 * n[k] * n[l]);
-            const R H = a * a * dkl + (R)2 * a * (n[k] * da[l] + n[l] * da[k]) + da[k] * da[l] + a * dda;
-            s += H * v[l];
-        }
-        w[k] = s;
-    }
-    R Jl = 0, Jel = 0;
-    for (int k = 0; k < 3; k++) {
-        Jl += v[k] * J[k];
-        Jel += v[k] * Je[k];
-    }
+    for (int k = 0; k < 3; k++)
+        w[k] = a2 * v[k] + (R)2 * a * (n[k] * dav + da[k] * nv) + da[k] * dav +
+               c16 * ((R)3 * n[k] * n[k] * v[k] - (R)4 * n3[k] * nv - (R)4 * n[k] * n3v -
+                      s4 * v[k] + (R)6 * s4 * n[k] * nv);
     if (rot) {
         for (int b = 0; b < 3; b++) dJdg[b] = M[b * 3 + 0] * w[0] + M[b * 3 + 1] * w[1] + M[b * 3 + 2] * w[2];
     } else {
@@ -89,8 +87,8 @@ __device__ inline R face_flux_d(const R (&g)[3], int axis, R eps4, const R (&M)[
         dJdg[1] = w[1];
         dJdg[2] = w[2];
     }
-    dJde = Jel;
-    return Jl;
+    dJde = r * Jel;
+    return r * Jl;
 }
 
 // Cell anisotropy a(grad) with da/dgrad (lab) and da/de.
@@ -130,36 +128,13 @@ __device__ inline R cell_as_d(const R (&gr)[3], R eps4, const R (&M)[9], bool ro
 #define PIX(dx, dy, dz) (((dx) + 1) * 9 + ((dy) + 1) * 3 + ((dz) + 1))
 
 // Reverse of phase_source: given Sh = dL/dS, accumulates dL/dP (27, corners unused) and
-// returns dL/du; adds dL/deps4 to *eh.
+// returns dL/du; adds dL/deps4 to *eh.  The cell anisotropy a comes first (the face terms of
+// the reverse scale with 1/a^2), then one pass over the six faces forms each face flux with its
+// derivatives and scatters its adjoint at once; the term through a, which needs div J, closes.
 template <class R>
 __device__ inline R phase_source_adj(const R (&P)[27], R upf, R eps4, R lambda, R pf_dt,
                                      R pf_wdx2, const R (&M)[9], bool rot, R Sh, R (&Ph)[27],
                                      R& eh) {
-    // ---- forward values needed by the reverse
-    R gf[6][3], dJ[6][3], dJe[6];
-    R divJ = 0;
-#pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-#pragma unroll
-        for (int sd = 0; sd < 2; sd++) {
-            const int f = 2 * ax + sd;
-            const int side = sd == 0 ? 1 : -1;
-            const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
-#pragma unroll
-            for (int b = 0; b < 3; b++) {
-                if (b == ax) {
-                    gf[f][b] = (R)side * (P[PIX(o0, o1, o2)] - P[PIX(0, 0, 0)]);
-                } else {
-                    const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
-                    gf[f][b] = (R)0.25 * (P[PIX(e0, e1, e2)] - P[PIX(-e0, -e1, -e2)] +
-                                          P[PIX(o0 + e0, o1 + e1, o2 + e2)] -
-                                          P[PIX(o0 - e0, o1 - e1, o2 - e2)]);
-                }
-            }
-            const R J = face_flux_d<R>(gf[f], ax, eps4, M, rot, dJ[f], dJe[f]);
-            divJ += (R)side * J;
-        }
-    }
     R gr[3] = {0, 0, 0};
 #pragma unroll
     for (int i = 1; i < 19; i++) {
@@ -170,30 +145,38 @@ __device__ inline R phase_source_adj(const R (&P)[27], R upf, R eps4, R lambda, 
     }
     R dadg[3], dade;
     const R a = cell_as_d<R>(gr, eps4, M, rot, dadg, dade);
-    const R ph = P[PIX(0, 0, 0)];
-    const R d = (R)1 - ph * ph;
-    const R Qs = (ph - lambda * upf * d) * d;
     const R a2 = a * a;
-    // ---- reverse
     const R Jh = Sh * pf_dt * pf_wdx2 / a2;          // dL/d(divJ)
     const R lh = -Sh * pf_dt * pf_wdx2;              // dL/d(lap)
-    const R Qh = Sh * pf_dt / a2;                    // dL/dQ
-    const R ah = Sh * (R)-2 * pf_dt * (pf_wdx2 * divJ + Qs) / (a2 * a);
-    R e_acc = ah * dade;
+    R divJ = 0, e_acc = 0;
 #pragma unroll
     for (int k = 0; k < 27; k++) Ph[k] = 0;
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
 #pragma unroll
         for (int sd = 0; sd < 2; sd++) {
-            const int f = 2 * ax + sd;
             const int side = sd == 0 ? 1 : -1;
             const int o0 = ax == 0 ? side : 0, o1 = ax == 1 ? side : 0, o2 = ax == 2 ? side : 0;
+            R gf[3];
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b == ax) {
+                    gf[b] = (R)side * (P[PIX(o0, o1, o2)] - P[PIX(0, 0, 0)]);
+                } else {
+                    const int e0 = b == 0, e1 = b == 1, e2 = b == 2;
+                    gf[b] = (R)0.25 * (P[PIX(e0, e1, e2)] - P[PIX(-e0, -e1, -e2)] +
+                                       P[PIX(o0 + e0, o1 + e1, o2 + e2)] -
+                                       P[PIX(o0 - e0, o1 - e1, o2 - e2)]);
+                }
+            }
+            R dJ[3], dJe;
+            const R J = face_flux_d<R>(gf, ax, eps4, M, rot, dJ, dJe);
+            divJ += (R)side * J;
             const R Jf = (R)side * Jh;
-            e_acc += Jf * dJe[f];
+            e_acc += Jf * dJe;
             R gh[3];
 #pragma unroll
-            for (int b = 0; b < 3; b++) gh[b] = Jf * dJ[f][b];
+            for (int b = 0; b < 3; b++) gh[b] = Jf * dJ[b];
             gh[ax] += (R)side * lh;
 #pragma unroll
             for (int b = 0; b < 3; b++) {
@@ -211,6 +194,12 @@ __device__ inline R phase_source_adj(const R (&P)[27], R upf, R eps4, R lambda, 
             }
         }
     }
+    const R ph = P[PIX(0, 0, 0)];
+    const R d = (R)1 - ph * ph;
+    const R Qs = (ph - lambda * upf * d) * d;
+    const R Qh = Sh * pf_dt / a2;                    // dL/dQ
+    const R ah = Sh * (R)-2 * pf_dt * (pf_wdx2 * divJ + Qs) / (a2 * a);
+    e_acc += ah * dade;
     const R grh0 = ah * dadg[0], grh1 = ah * dadg[1], grh2 = ah * dadg[2];
 #pragma unroll
     for (int i = 1; i < 19; i++) {
@@ -371,6 +360,14 @@ __global__ void __launch_bounds__(256) k_adj_stencil(const StepArgs a, const R* 
     const uint32_t N = a.N;
     const uint32_t np = (uint32_t)a.g.npmask;
     R s = 0;
+    if ((c.bits & np) == 0u) {
+        // away from non-periodic faces every direction has its one regular source
+#pragma unroll
+        for (int k = 0; k < 19; k++)
+            s += ldp(slot_at(P + (size_t)k * N, nbr(c, -cx(k), -cy(k), -cz(k))));
+        nout[c.id] = s;
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 19; k++) {
         // per axis: the regular source (x - d, invalid across a non-periodic face) and, on a

@@ -317,7 +317,7 @@ int copy_in(double* ddst, const void* host, int dtype, size_t count, hipStream_t
 // before the phase moment pass, between the passes, after the fused pass.  mid (optional)
 // runs between the two passes (used by the adjoint to save phi^n and the thermal state).
 template <class Mid>
-int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid) {
+int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid, void* tape = nullptr) {
     Combo k{h->q[LF], h->q[LG], h->q[LH] ? 1 : 0, h->q[LC] ? 1 : 0};
     int lay0 = -1;
     bool mixed = false;
@@ -345,6 +345,7 @@ int step_once_impl(mkg_ctx* h, const int* adv, hipEvent_t* ev, Mid mid) {
     a.g.adv = adv[LG];
     a.h.adv = adv[LH];
     a.c.adv = adv[LC];
+    a.tape = tape;
     if (adv[LF] && a.nee_mask && launch_nee(k, h->fp32, a, h->stream))
         return fail("velocity-face kernel unavailable");
     if (launch_step(k, h->fp32, lay0, a, h->stream))
@@ -1027,7 +1028,9 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     std::vector<Ck> ck(tape ? 1 : nseg + 1);
     const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH);
     const size_t ck_elems = PG + PH + 2 * S;
-    const size_t seg_stride = S + PH;  // phi^n and the h buffer of one step
+    // tape of one step, written by the forward kernel: phi^n, then the 7 pre-collision thermal
+    // populations in own-slot order (stride S)
+    const size_t seg_stride = 8 * S;
     const size_t seg_elems = (size_t)ckpt * seg_stride;
     char *ckbuf = nullptr, *segbuf = nullptr, *work = nullptr;
     double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr;
@@ -1100,19 +1103,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     auto cleanup_ev = [&]() {
         for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
     };
-    std::vector<int> seg_lay(ckpt);
-    // per-step values of a forward step: phi^n and the thermal AA buffer before the step
+    // a forward step that writes its tape slot: phi^n and the thermal populations before it
     auto stepper = [&](int n, int n0) -> int {
         char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
-        seg_lay[n - n0] = h->lay[LH];
-        auto mid = [&](mkg_ctx* hh) -> int {
-            HIPCHK(hipMemcpyAsync(slot, hh->phi[hh->pf_count & 1], S * e,
-                                  hipMemcpyDeviceToDevice, hh->stream));
-            HIPCHK(hipMemcpyAsync(slot + S * e, hh->pop[LH], PH * e, hipMemcpyDeviceToDevice,
-                                  hh->stream));
-            return 0;
-        };
-        return step_once_impl(h, adv, nullptr, mid);
+        return step_once_impl(h, adv, nullptr, [](mkg_ctx*) { return 0; }, slot);
     };
     ADJCHK(hipEventRecord(ev[0], h->stream));
     // ---- forward pass: checkpoints, or the tape
@@ -1157,7 +1151,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
             j.P = Pb; j.kap_grad = kap; j.phin = slot;
             j.hsave = a.h;
             j.hsave.pop = slot + S * e;
-            j.hsave.lay = seg_lay[n - n0];
+            j.hsave.lay = 0;
             j.eps_part = eps_part;
             launch_adj_collide(h->fp32, a, j, h->stream);
             launch_adj_stencil(h->fp32, a, Pb, nout, h->stream);

@@ -54,6 +54,9 @@ struct StepArgs {
     // velocity faces: the replaced post-collision populations of each face cell (storage type,
     // Q x F, slot i of face cell k at i * F + k), written by the pre-pass of the step
     void* nee;
+    // adjoint tape of the step (phase field + thermal model): phi^n at tape[cell] and the
+    // pre-collision thermal populations at tape[(1 + k) * S + cell]; null: no tape
+    void* tape;
 };
 
 // Selector of a fused-step instantiation: presence of each lattice.

@@ -255,7 +255,10 @@ constexpr int zslots_of() {
 //       issued after the phase source, so that the stencil values are dead by then)
 //   B3  after the phase push stage, which reuses the flow pull region (last read before B2)
 // Every other stage region is written once and read once, on either side of one barrier.
-template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false>
+// TAPE: the step also writes the adjoint tape (phi^n and the pre-collision thermal
+// populations of every cell; instantiated for the phase-field + thermal model only).
+template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false,
+          bool TAPE = false>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     k_step(const StepArgs a) {
     constexpr bool Z = ODD && ZS;
@@ -420,6 +423,14 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             if (a.has_kappa) alpha *= ((const R*)a.kappa)[c.id];
             const R om = (R)1 / ((R)0.5 + alpha / (R)0.25);
             const R src = (R)a.latent_T * dphi;
+            if constexpr (TAPE) {
+                if (act) {
+                    R* __restrict__ tp = (R*)a.tape;
+                    *slot_at(tp, c.id) = phi;
+#pragma unroll
+                    for (int i = 0; i < 7; i++) *slot_at(tp + (size_t)(1 + i) * S, c.id) = h[i];
+                }
+            }
 #pragma unroll
             for (int i = 0; i < 7; i++) {
                 const R w = (R)wgt<7>(i);
@@ -583,6 +594,29 @@ template <class R, int FQ, int PQ, bool TH, bool SO>
 inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
     const bool per = !a.has_walls && (a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask) == 0;
+    if constexpr (FQ == 0 && PQ == 19 && TH && !SO) {
+        // the adjoint's forward steps (the phase-field + thermal model), with the kernel
+        // variant a plain step would take, so that the recompute repeats the forward exactly
+        if (a.tape) {
+            if (odd && a.zstage) {
+                if (per && a.N == a.nblk * 256u)
+                    k_step<R, FQ, PQ, TH, SO, true, true, true, true><<<grid, block, 0, s>>>(a);
+                else
+                    k_step<R, FQ, PQ, TH, SO, true, true, false, true><<<grid, block, 0, s>>>(a);
+            } else if (odd) {
+                if (per)
+                    k_step<R, FQ, PQ, TH, SO, true, false, true, true><<<grid, block, 0, s>>>(a);
+                else
+                    k_step<R, FQ, PQ, TH, SO, true, false, false, true><<<grid, block, 0, s>>>(a);
+            } else {
+                if (per)
+                    k_step<R, FQ, PQ, TH, SO, false, false, true, true><<<grid, block, 0, s>>>(a);
+                else
+                    k_step<R, FQ, PQ, TH, SO, false, false, false, true><<<grid, block, 0, s>>>(a);
+            }
+            return;
+        }
+    }
     if (odd && a.zstage) {
         if (per && a.N == a.nblk * 256u)
             k_step<R, FQ, PQ, TH, SO, true, true, true><<<grid, block, 0, s>>>(a);
