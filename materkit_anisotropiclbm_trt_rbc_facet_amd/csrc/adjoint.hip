@@ -216,7 +216,9 @@ __device__ inline R phase_source_adj(const R (&P)[27], R upf, R eps4, R lambda, 
 // ---------------------------------------------------------------- adjoint streaming
 // dL/d(post_i(y)) from the adjoint of the next level's pre-collision populations (canonical
 // layout, adj_j(z) = A[j*N + z] + cellwide[z]); reverses the pull rules of the forward step.
-template <int Q, class R>
+// FACES false: the cell touches no non-periodic face of the lattice, so every population was
+// pulled by its regular neighbour.
+template <int Q, class R, bool FACES = true>
 __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __restrict__ cw,
                                   const LatArgs& L, const Cell& c, const StepArgs& a, int kind,
                                   uint32_t N) {
@@ -225,32 +227,36 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
         const int dx = cx(i), dy = cy(i), dz = cz(i);
         R v = 0;
         // (a)/(b): where post_i(y) is pulled to
-        uint32_t hit = c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask;
+        uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
         if (hit == 0u) {
             const uint32_t t = nbr(c, dx, dy, dz);
-            v = ldp(slot_at(A + (size_t)i * N, t)) + (cw ? cw[t] : (R)0);
+            v = ldp(slot_at(A + (size_t)i * N, t)) + (cw ? *slot_at(cw, t) : (R)0);
         } else {
             const int face = __builtin_ctz(hit);
             const int tb = pick6(face, L.bc);
-            const R own = *slot_at(A + (size_t)inv(i) * N, c.id) + (cw ? cw[c.id] : (R)0);
+            const R own = *slot_at(A + (size_t)inv(i) * N, c.id) + (cw ? *slot_at(cw, c.id) : (R)0);
             if (tb == kBcOutflow) v = 0;
             else if (tb == kBcDirichlet && kind == kKindScalar) v = -own;
             else v = own;
         }
         // (c): zero-gradient outflow copies post_i(y) into new_i(y)
-        uint32_t hin = c.bits & faces_of(-dx, -dy, -dz) & (uint32_t)L.npmask;
+        uint32_t hin = FACES ? c.bits & faces_of(-dx, -dy, -dz) & (uint32_t)L.npmask : 0u;
         if (hin != 0u) {
             const int face = __builtin_ctz(hin);
-            if (pick6(face, L.bc) == kBcOutflow) v += *slot_at(A + (size_t)i * N, c.id) + (cw ? cw[c.id] : (R)0);
+            if (pick6(face, L.bc) == kBcOutflow)
+                v += *slot_at(A + (size_t)i * N, c.id) + (cw ? *slot_at(cw, c.id) : (R)0);
         }
         out[i] = v;
     }
 }
 
 // ---------------------------------------------------------------- kernels
-// FP32: held to four waves per SIMD (at most 128 VGPRs; the allocator alone takes 132 and three)
+// Minimum waves per SIMD of the FP32 reverse kernel (the register bound follows from it).
+#ifndef MKG_ADJ_WAVES_F32
+#define MKG_ADJ_WAVES_F32 4
+#endif
 template <class R>
-__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
     k_adj_collide(const StepArgs a, const AdjArgs j) {
     Cell c;
     const bool valid = locate(a, c);
@@ -260,8 +266,13 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
     double eps_loc = 0;
     if (valid && a.flag[id] != kIsWalls) {
         R gs[19], hs[7];
-        adj_gather<19, R>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
-        adj_gather<7, R>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+        if ((c.bits & (uint32_t)(a.g.npmask | a.h.npmask)) == 0u) {
+            adj_gather<19, R, false>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
+            adj_gather<7, R, false>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+        } else {
+            adj_gather<19, R>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
+            adj_gather<7, R>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+        }
         // forward values at level n
         R h[7];
         read_pops_rt<7, R>(h, j.hsave, a.S, c, a, kKindScalar);
@@ -285,7 +296,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         for (int k = 0; k < 27; k++) P[k] = 0;
 #pragma unroll
         for (int i = 0; i < 19; i++)
-            P[PIX(cx(i), cy(i), cz(i))] = ph[px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]];
+            P[PIX(cx(i), cy(i), cz(i))] = *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
         const R phi = P[PIX(0, 0, 0)];
         R Mrot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         bool rot = false;

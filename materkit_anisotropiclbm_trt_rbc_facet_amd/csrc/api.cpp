@@ -1035,8 +1035,22 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     char *ckbuf = nullptr, *segbuf = nullptr, *work = nullptr;
     double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr;
     int rc = 0;
+    // events: 0-1 forward, 2-4 reverse phase, 5-6 segment tape filled, 7-8 segment tape free;
+    // the reverse sweep runs on a stream of its own (default priority) when checkpointing
+    hipEvent_t ev[9] = {};
+    hipStream_t rs = h->stream;
     auto release = [&]() {
         hipStreamSynchronize(h->stream);
+        if (rs != h->stream) {
+            hipStreamSynchronize(rs);
+            hipStreamDestroy(rs);
+            rs = h->stream;
+        }
+        for (int k = 0; k < 9; k++)
+            if (ev[k]) {
+                hipEventDestroy(ev[k]);
+                ev[k] = nullptr;
+            }
         hipFree(ckbuf);
         hipFree(segbuf);
         hipFree(work);
@@ -1053,7 +1067,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         }                                                                        \
     } while (0)
     if (!tape) ADJCHK(hipMalloc((void**)&ckbuf, (nseg + 1) * ck_elems * e));
-    ADJCHK(hipMalloc((void**)&segbuf, seg_elems * e));
+    // two segment tapes when checkpointing: the recompute of the next (earlier) segment fills
+    // one while the reverse sweep of the current segment reads the other
+    const int nbuf = tape ? 1 : 2;
+    ADJCHK(hipMalloc((void**)&segbuf, nbuf * seg_elems * e));
     // work: Gin, Gout (19N), nin, nout (N), Hin, Hout (7N), pin, pout (N), P (19N), kap (N)
     const size_t work_elems = (2 * 19 + 2 + 2 * 7 + 2 + 19 + 1) * N;
     ADJCHK(hipMalloc((void**)&work, work_elems * e));
@@ -1098,28 +1115,35 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         h->steps[LH] = ck[k].steps_h;
         return 0;
     };
-    hipEvent_t ev[4];
-    for (int k = 0; k < 4; k++) ADJCHK(hipEventCreate(&ev[k]));
-    auto cleanup_ev = [&]() {
-        for (int k = 0; k < 4; k++) hipEventDestroy(ev[k]);
-    };
+    for (int k = 0; k < 9; k++) ADJCHK(hipEventCreate(&ev[k]));
+    if (!tape) ADJCHK(hipStreamCreate(&rs));
+    auto seg_base = [&](int sgi) { return segbuf + (size_t)(sgi % nbuf) * seg_elems * e; };
     // a forward step that writes its tape slot: phi^n and the thermal populations before it
-    auto stepper = [&](int n, int n0) -> int {
-        char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
+    auto stepper = [&](int n, int n0, char* base) -> int {
+        char* slot = base + (size_t)(n - n0) * seg_stride * e;
         return step_once_impl(h, adv, nullptr, [](mkg_ctx*) { return 0; }, slot);
+    };
+    // recompute of segment sgi from its checkpoint on the handle's stream, into its tape
+    auto recompute = [&](int sgi) -> int {
+        const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+        if (load_ck(sgi)) return -1;
+        for (int n = n0; n < n1; n++)
+            if (stepper(n, n0, seg_base(sgi))) return -1;
+        HIPCHK(hipEventRecord(ev[5 + sgi % 2], h->stream));
+        return 0;
     };
     ADJCHK(hipEventRecord(ev[0], h->stream));
     // ---- forward pass: checkpoints, or the tape
     for (int n = 0; n < nsteps; n++) {
         if (tape) {
-            if (stepper(n, 0)) { rc = -1; break; }
+            if (stepper(n, 0, segbuf)) { rc = -1; break; }
             continue;
         }
         if (n % ckpt == 0 && save_ck(n / ckpt)) { rc = -1; break; }
         if (step_once(h, adv, nullptr, nullptr)) { rc = -1; break; }
     }
     if (rc == 0 && !tape && save_ck(nseg)) rc = -1;  // final forward state
-    if (rc) { cleanup_ev(); release(); return -1; }
+    if (rc) { release(); return -1; }
     // ---- loss and seed of the reverse sweep (phi_N - target, cell-wide)
     {
         StepArgs a = make_args(h);
@@ -1127,65 +1151,64 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         ADJCHK(hipGetLastError());
     }
     ADJCHK(hipEventRecord(ev[1], h->stream));
-    float t_re = 0, t_rev = 0;
+    // ---- reverse sweep, segment by segment from the last.  With checkpoints, segment s - 1 is
+    // recomputed on the handle's stream while the reverse of segment s runs on rs; a segment
+    // tape is refilled only after the reverse that read it has finished.  The reverse kernels
+    // read geometry, faces, rates, kappa and flags, never the handle's populations.
+    const StepArgs ar = make_args(h);
+    ADJCHK(hipEventRecord(ev[2], h->stream));
+    if (!tape && recompute(nseg - 1)) rc = -1;
+    ADJCHK(hipEventRecord(ev[3], h->stream));
     for (int sgi = nseg - 1; sgi >= 0 && rc == 0; sgi--) {
         const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);
-        hipEvent_t ea, eb, ec;
-        hipEventCreate(&ea);
-        hipEventCreate(&eb);
-        hipEventCreate(&ec);
-        hipEventRecord(ea, h->stream);
         if (!tape) {
-            if (load_ck(sgi)) { rc = -1; break; }
-            for (int n = n0; n < n1; n++)
-                if (stepper(n, n0)) { rc = -1; break; }
+            ADJCHK(hipStreamWaitEvent(rs, ev[5 + sgi % 2], 0));
+            if (sgi > 0) {
+                if (sgi + 1 <= nseg - 1) ADJCHK(hipStreamWaitEvent(h->stream, ev[7 + (sgi - 1) % 2], 0));
+                if (recompute(sgi - 1)) { rc = -1; break; }
+            }
         }
-        hipEventRecord(eb, h->stream);
-        for (int n = n1 - 1; n >= n0 && rc == 0; n--) {
-            char* slot = segbuf + (size_t)(n - n0) * seg_stride * e;
-            StepArgs a = make_args(h);
+        char* base = seg_base(sgi);
+        for (int n = n1 - 1; n >= n0; n--) {
+            char* slot = base + (size_t)(n - n0) * seg_stride * e;
             AdjArgs j;
             std::memset(&j, 0, sizeof(j));
             j.gin = Gin; j.nin = nin; j.hin = Hin; j.pin = pin;
             j.gout = Gout; j.hout = Hout; j.pout = pout;
             j.P = Pb; j.kap_grad = kap; j.phin = slot;
-            j.hsave = a.h;
+            j.hsave = ar.h;
             j.hsave.pop = slot + S * e;
             j.hsave.lay = 0;
             j.eps_part = eps_part;
-            launch_adj_collide(h->fp32, a, j, h->stream);
-            launch_adj_stencil(h->fp32, a, Pb, nout, h->stream);
+            launch_adj_collide(h->fp32, ar, j, rs);
+            launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
             std::swap(Gin, Gout);
             std::swap(nin, nout);
             std::swap(Hin, Hout);
             std::swap(pin, pout);
         }
-        hipEventRecord(ec, h->stream);
-        hipEventSynchronize(ec);
-        float x = 0, y = 0;
-        hipEventElapsedTime(&x, ea, eb);
-        hipEventElapsedTime(&y, eb, ec);
-        t_re += x;
-        t_rev += y;
-        hipEventDestroy(ea);
-        hipEventDestroy(eb);
-        hipEventDestroy(ec);
+        if (!tape) ADJCHK(hipEventRecord(ev[7 + sgi % 2], rs));
         if (hipGetLastError() != hipSuccess) rc = fail("adjoint kernel launch failed");
     }
-    if (rc) { cleanup_ev(); release(); return -1; }
+    if (rc) { release(); return -1; }
+    ADJCHK(hipEventRecord(ev[4], rs));
     // ---- outputs
-    launch_sum_partials(eps_part, h->nblk, dscal + 1, h->stream);
+    launch_sum_partials(eps_part, h->nblk, dscal + 1, rs);
     ADJCHK(hipGetLastError());
     double sc[2];
-    ADJCHK(hipMemcpyAsync(sc, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ADJCHK(hipMemcpyAsync(sc, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, rs));
     if (dL_dkappa) {
-        if (ensure_scratch(h)) { cleanup_ev(); release(); return -1; }
-        launch_to_double(h->fp32, kap, h->scratch, h->N, h->stream);
+        if (ensure_scratch(h)) { release(); return -1; }
+        launch_to_double(h->fp32, kap, h->scratch, h->N, rs);
         ADJCHK(hipMemcpyAsync(dL_dkappa, h->scratch, N * sizeof(double), hipMemcpyDeviceToHost,
-                              h->stream));
+                              rs));
     }
+    ADJCHK(hipStreamSynchronize(rs));
+    float t_re = 0, t_rev = 0;
+    ADJCHK(hipEventElapsedTime(&t_re, ev[2], ev[3]));
+    ADJCHK(hipEventElapsedTime(&t_rev, ev[3], ev[4]));
     // leave the handle at the final forward state (the tape never moved it)
-    if (!tape && load_ck(nseg)) { cleanup_ev(); release(); return -1; }
+    if (!tape && load_ck(nseg)) { release(); return -1; }
     ADJCHK(hipStreamSynchronize(h->stream));
     float t_fwd = 0;
     hipEventElapsedTime(&t_fwd, ev[0], ev[1]);
@@ -1197,7 +1220,6 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         times[2] = t_rev;
         times[3] = t_fwd + t_re + t_rev;
     }
-    cleanup_ev();
     release();
 #undef ADJCHK
     return 0;

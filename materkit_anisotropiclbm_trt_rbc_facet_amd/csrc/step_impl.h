@@ -426,9 +426,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             if constexpr (TAPE) {
                 if (act) {
                     R* __restrict__ tp = (R*)a.tape;
-                    *slot_at(tp, c.id) = phi;
+                    stp(slot_at(tp, c.id), phi);
 #pragma unroll
-                    for (int i = 0; i < 7; i++) *slot_at(tp + (size_t)(1 + i) * S, c.id) = h[i];
+                    for (int i = 0; i < 7; i++) stp(slot_at(tp + (size_t)(1 + i) * S, c.id), h[i]);
                 }
             }
 #pragma unroll
