@@ -268,14 +268,18 @@ int mkg_last_timing(mkg_handle h, double* out);
 int mkg_enable_timing(mkg_handle h, int on);
 
 /* Discrete adjoint (no counterpart in the reference) of nsteps fused phase-field (D3Q19) +
- * thermal (D3Q7) steps from the current state: a forward pass with checkpoints every ckpt
- * steps, the FP64 loss L = 1/2 sum (phi_N - phi_target)^2 with phi_N = sum_i g_i after the
- * last step, then the reverse sweep with per-segment recompute.  ckpt = 0 keeps a tape
- * instead: the forward pass stores what the reverse sweep reads for every step (phi^n and the
- * thermal populations, 8 values per cell and step) and nothing is recomputed.  Outputs L,
- * dL/deps4 and (optional, needs has_kappa) dL/dkappa per cell (N doubles); times (optional,
- * 4 doubles): forward ms, recompute ms, reverse ms, total.  The handle is left at the final
- * forward state. */
+ * thermal (D3Q7) steps from the current state, with or without a D3Q19 flow (the coupled step
+ * of run_flow_thermal_dendrite, MaterKit/MaterKit.cpp:500-518: the melt advects both scalars,
+ * feels the Boussinesq buoyancy and sees the solid through partial bounce-back; compressible
+ * equilibrium, periodic / bounce / outflow flow faces): a forward pass with checkpoints every
+ * ckpt steps, the FP64 loss L = 1/2 sum (phi_N - phi_target)^2 with phi_N = sum_i g_i after
+ * the last step, then the reverse sweep with per-segment recompute (segment s - 1 recomputed
+ * on a second stream while segment s is reversed).  ckpt = 0 keeps a tape instead: the
+ * forward pass stores what the reverse sweep reads for every step (phi^n and the thermal
+ * populations, 8 values per cell and step, plus the flow populations in FP64) and nothing is
+ * recomputed.  Outputs L, dL/deps4 and (optional, needs has_kappa) dL/dkappa per cell (N
+ * doubles); times (optional, 4 doubles): forward ms, recompute ms not overlapped with the
+ * reverse, reverse phase ms, total.  The handle is left at the final forward state. */
 int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* phi_target, double* loss,
                     double* dL_deps4, double* dL_dkappa, double* times);
 

@@ -19,6 +19,7 @@
 //                  receives; it is folded in when the next (earlier) step gathers.
 // Adjoint populations live in plain two-lattice canonical buffers (Q x N).
 #include "device.h"
+#include "fadj.h"
 
 namespace mkg {
 
@@ -255,7 +256,13 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
 #ifndef MKG_ADJ_WAVES_F32
 #define MKG_ADJ_WAVES_F32 4
 #endif
-template <class R>
+// FLOW: the coupled model.  The scalars are advected with u~ = (1 - n_s) u, u the flow
+// velocity at level n and n_s the solid fraction of partial bounce-back:
+//   h_eq,k = w_k T (1 + c_k.u~ / cs_h^2),  g_eq,i = w_i (phi + (phi + 1) c_i.u~ / cs_g^2),
+// so their reverse adds w_k (c_k.u~) / cs^2 terms to dL/dT and dL/dphi and emits dL/du~
+// (j.ut) for the flow kernel; the buoyancy part of dL/dT arrives as a cell-wide thermal
+// adjoint (j.hcw_in) with the gathered populations.
+template <class R, bool FLOW>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
     k_adj_collide(const StepArgs a, const AdjArgs j) {
     Cell c;
@@ -266,12 +273,13 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
     double eps_loc = 0;
     if (valid && a.flag[id] != kIsWalls) {
         R gs[19], hs[7];
+        const R* hcw = FLOW ? (const R*)j.hcw_in : nullptr;
         if ((c.bits & (uint32_t)(a.g.npmask | a.h.npmask)) == 0u) {
             adj_gather<19, R, false>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
-            adj_gather<7, R, false>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+            adj_gather<7, R, false>(hs, (const R*)j.hin, hcw, a.h, c, a, kKindScalar, N);
         } else {
             adj_gather<19, R>(gs, (const R*)j.gin, (const R*)j.nin, a.g, c, a, kKindScalar, N);
-            adj_gather<7, R>(hs, (const R*)j.hin, nullptr, a.h, c, a, kKindScalar, N);
+            adj_gather<7, R>(hs, (const R*)j.hin, hcw, a.h, c, a, kKindScalar, N);
         }
         // forward values at level n
         R h[7];
@@ -317,14 +325,63 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
         const R om = (R)1 / ((R)0.5 + kap * alpha0 / (R)0.25);
         const R dom_dphi = -om * om * kap * dalpha / (R)0.25;
         const R dom_dkap = -om * om * alpha0 / (R)0.25;
-        // adjoint sums
-        R A = 0, B = 0, Om = 0;
+        // advection velocity of the scalars at level n (coupled model): u from the flow tape
+        // with the body force and buoyancy, times the liquid fraction 1 - n_s
+        R v0 = 0, v1 = 0, v2 = 0;
+        if constexpr (FLOW) {
+            R rho = 0, j0 = 0, j1 = 0, j2 = 0;
 #pragma unroll
-        for (int i = 0; i < 19; i++) A += (R)wgt<19>(i) * gs[i];
+            for (int i = 0; i < 19; i++) {
+                const R fi = (R)j.ftape[(size_t)i * N + id];
+                rho += fi;
+                if (cx(i)) j0 += (R)cx(i) * fi;
+                if (cy(i)) j1 += (R)cy(i) * fi;
+                if (cz(i)) j2 += (R)cz(i) * fi;
+            }
+            const R dT = T - (R)a.T_ref;
+            const R F0 = (R)a.force[0] + (R)(a.rho0 * a.buoy[0]) * dT;
+            const R F1 = (R)a.force[1] + (R)(a.rho0 * a.buoy[1]) * dT;
+            const R F2 = (R)a.force[2] + (R)(a.rho0 * a.buoy[2]) * dT;
+            const int8_t fl = a.flag[id];
+            R ns = 0;
+            if (fl == kIsSolid) ns = (R)1;
+            else if (fl == kIsInter) ns = fs;
+            const R q = ((R)1 - ns) / rho;
+            v0 = (j0 + (R)0.5 * F0) * q;
+            v1 = (j1 + (R)0.5 * F1) * q;
+            v2 = (j2 + (R)0.5 * F2) * q;
+        }
+        // adjoint sums; with advection, B and A carry the (c.u~)/cs^2 part of the equilibria
+        R A = 0, B = 0, Om = 0, Au = 0;
+        R ut0 = 0, ut1 = 0, ut2 = 0;  // dL/du~ of the two scalar equilibria
+#pragma unroll
+        for (int i = 0; i < 19; i++) {
+            const R wl = (R)wgt<19>(i) * gs[i];
+            A += wl;
+            if constexpr (FLOW) {
+                const R cv = cx(i) * v0 + cy(i) * v1 + cz(i) * v2;
+                Au += (R)3 * cv * wl;
+                if (cx(i)) ut0 += (R)(3 * cx(i)) * wl;
+                if (cy(i)) ut1 += (R)(3 * cy(i)) * wl;
+                if (cz(i)) ut2 += (R)(3 * cz(i)) * wl;
+            }
+        }
+        R uh0 = 0, uh1 = 0, uh2 = 0;
 #pragma unroll
         for (int k = 0; k < 7; k++) {
-            B += (R)wgt<7>(k) * hs[k];
-            Om -= (h[k] - (R)wgt<7>(k) * sh) * hs[k];
+            const R wl = (R)wgt<7>(k) * hs[k];
+            R heq = (R)wgt<7>(k) * sh;
+            if constexpr (FLOW) {
+                const R cv = cx(k) * v0 + cy(k) * v1 + cz(k) * v2;
+                heq += (R)wgt<7>(k) * T * (R)4 * cv;
+                B += wl * ((R)1 + (R)4 * cv);
+                if (cx(k)) uh0 += (R)(4 * cx(k)) * wl;
+                if (cy(k)) uh1 += (R)(4 * cy(k)) * wl;
+                if (cz(k)) uh2 += (R)(4 * cz(k)) * wl;
+            } else {
+                B += wl;
+            }
+            Om -= (h[k] - heq) * hs[k];
         }
         R Ph[27];
         R eh = 0;
@@ -337,12 +394,27 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
         for (int k = 0; k < 7; k++) stp(slot_at(ho + (size_t)k * N, id), ((R)1 - om) * hs[k] + Th);
         const R L = (R)a.latent_T;
         const R pin = ((const R*)j.pin)[id];
-        const R phl = (R)a.omega_g * A + L * B + pin + Om * dom_dphi;
+        // the latent source w_k L dphi has no advective part: its adjoint is L sum_k w_k hs_k
+        R Bl = B;
+        if constexpr (FLOW) {
+            Bl = 0;
+#pragma unroll
+            for (int k = 0; k < 7; k++) Bl += (R)wgt<7>(k) * hs[k];
+        }
+        const R phl = (R)a.omega_g * (A + Au) + L * Bl + pin + Om * dom_dphi;
+        if constexpr (FLOW) {
+            // dL/du~ = omega_g sum_i w_i gs_i (phi + 1) 3 c_i + omega_h sum_k w_k hs_k T 4 c_k
+            R* __restrict__ ut = (R*)j.ut;
+            const R pg = (R)a.omega_g * (phi + (R)1), ph_ = om * T;
+            ut[id] = pg * ut0 + ph_ * uh0;
+            ut[(size_t)N + id] = pg * ut1 + ph_ * uh1;
+            ut[2 * (size_t)N + id] = pg * ut2 + ph_ * uh2;
+        }
         R* __restrict__ go = (R*)j.gout;
         const R og = (R)1 - (R)a.omega_g;
 #pragma unroll
         for (int i = 0; i < 19; i++) stp(slot_at(go + (size_t)i * N, id), og * gs[i] + phl);
-        ((R*)j.pout)[id] = -L * B;
+        ((R*)j.pout)[id] = -L * Bl;
         if (a.has_kappa) ((R*)j.kap_grad)[id] += Om * dom_dkap;
         R* __restrict__ Pb = (R*)j.P;
 #pragma unroll
@@ -455,10 +527,137 @@ __global__ void k_sum_partials(const double* part, uint32_t n, double* out) {
     if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
 }
 
+// Flow reverse of the coupled step (after k_adj_collide<R, true> and k_adj_stencil), per
+// non-wall cell y.  Forward: rho, u = (j + F/2)/rho with F = force + rho0 buoy (T - T_ref);
+// post = the generalised moment collision with Guo forcing; out_i = (1 - n_s) post_i
+// + n_s f_inv(i) (partial bounce-back); out streams with the face rules of the flow lattice;
+// the scalars are advected with u~ = (1 - n_s) u.  Reverse:
+//   l = the streamed flow adjoint of level n+1 (the gather of the flow-only reverse),
+//   dL/df = collide_reverse((1 - n_s) l; velocity adjoint (1 - n_s) dL/du~) + n_s l_inv,
+//   dL/dn_s = sum_i l_i (f_inv(i) - post_i) - u . dL/du~ (+ the moving-wall terms of bounce
+//             faces, which scale with 1 - n_s),
+//   dL/dT (buoyancy) = rho0 buoy . dL/dF   -> the cell-wide thermal adjoint of level n,
+//   dL/dphi += dL/dn_s / 2 in interface cells (n_s = (1 + phi)/2 there) -> the cell-wide phase
+//   adjoint of level n (added to what k_adj_stencil wrote).
+template <class R>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1)
+    k_cadj_flow(const StepArgs a, const AdjArgs j) {
+    constexpr int Q = 19;
+    const uint32_t N = a.N;
+    Cell c;
+    if (!locate(a, c)) return;
+    const uint32_t id = c.id;
+    const R* __restrict__ lam_in = (const R*)j.fin;
+    R* __restrict__ lam_out = (R*)j.fout;
+    if (a.flag[id] == kIsWalls) {
+        for (int i = 0; i < Q; i++) lam_out[(size_t)i * N + id] = 0;
+        ((R*)j.hcw_out)[id] = 0;
+        return;
+    }
+    // l_i(y) = lambda_i(y + c_i), reflected at bounce faces and walls, plus the outflow copy.
+    // A population returned by a bounce face carries the moving-wall term
+    // 6 w rho0 c_inv(i).u_w (1 - n_s): its n_s adjoint collects in dns_wall.
+    const uint32_t np = (uint32_t)a.f.npmask & c.bits;
+    R l[Q];
+    R dns_wall = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        const int X = cx(i), Y = cy(i), Z = cz(i);
+        const uint32_t hp = np & faces_of(X, Y, Z);
+        R v = 0;
+        if (hp == 0u) {
+            const uint32_t t = nbr(c, X, Y, Z);
+            v = a.flag[t] == kIsWalls ? *slot_at(lam_in + (size_t)inv(i) * N, id)
+                                      : *slot_at(lam_in + (size_t)i * N, t);
+        } else if (pick6(__builtin_ctz(hp), a.f.bc) == kBcBounce) {
+            v = *slot_at(lam_in + (size_t)inv(i) * N, id);
+            const double cu = wall_cu(a, __builtin_ctz(hp), c, X, Y, Z);
+            dns_wall += v * (R)(6.0 * wgt<Q>(i) * a.rho0 * cu);
+        }
+        const uint32_t hb = np & faces_of(-X, -Y, -Z);
+        if (hb != 0u && pick6(__builtin_ctz(hb), a.f.bc) == kBcOutflow)
+            v += *slot_at(lam_in + (size_t)i * N, id);
+        l[i] = v;
+    }
+    // forward values at level n
+    R f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = (R)j.ftape[(size_t)i * N + id];
+    R h[7];
+    read_pops_rt<7, R>(h, j.hsave, a.S, c, a, kKindScalar);
+    R sh = 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) sh += h[k];
+    const R T = (R)a.h.ref + sh;
+    const R phi = *slot_at((const R*)j.phin, id);
+    const int8_t fl = a.flag[id];
+    R ns = 0, dns_dphi = 0;
+    if (fl == kIsSolid) {
+        ns = (R)1;
+    } else if (fl == kIsInter) {
+        const R q = (R)0.5 * ((R)1 + phi);
+        ns = q < (R)0 ? (R)0 : (q > (R)1 ? (R)1 : q);
+        dns_dphi = (q > (R)0 && q < (R)1) ? (R)0.5 : (R)0;
+    }
+    const R dT = T - (R)a.T_ref;
+    const R F[3] = {(R)a.force[0] + (R)(a.rho0 * a.buoy[0]) * dT,
+                    (R)a.force[1] + (R)(a.rho0 * a.buoy[1]) * dT,
+                    (R)a.force[2] + (R)(a.rho0 * a.buoy[2]) * dT};
+    R rho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        rho += f[i];
+        if (cx(i)) jx += (R)cx(i) * f[i];
+        if (cy(i)) jy += (R)cy(i) * f[i];
+        if (cz(i)) jz += (R)cz(i) * f[i];
+    }
+    const R ux = (jx + (R)0.5 * F[0]) / rho, uy = (jy + (R)0.5 * F[1]) / rho,
+            uz = (jz + (R)0.5 * F[2]) / rho;
+    // post-collision populations without the partial bounce-back (full values: reference 0)
+    R post[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) post[i] = f[i];
+    flow_collide<Q, R>(post, rho, rho, ux, uy, uz, F[0], F[1], F[2], (R)a.s_nu, (R)a.s_bulk,
+                       (R)a.s_odd, (R)a.s_even, (R)0);
+    const R keep = (R)1 - ns;
+    R dns = dns_wall, lp[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        dns += l[i] * (f[inv(i)] - post[i]);
+        lp[i] = keep * l[i];
+    }
+    const R* __restrict__ ut = (const R*)j.ut;
+    const R w0 = *slot_at(ut, id), w1 = *slot_at(ut + (size_t)N, id), w2 = *slot_at(ut + 2 * (size_t)N, id);
+    dns -= ux * w0 + uy * w1 + uz * w2;
+    const R ub_ext[3] = {keep * w0, keep * w1, keep * w2};
+    R dF[3], drate[4];
+    collide_reverse<Q, R, false>(a, f, F, lp, ub_ext, lam_out, N, id, dF, drate);
+    // partial bounce-back: out_i takes n_s f_inv(i)
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        R* o = &lam_out[(size_t)i * N + id];
+        *o = *o + ns * l[inv(i)];
+    }
+    ((R*)j.hcw_out)[id] = (R)a.rho0 * ((R)a.buoy[0] * dF[0] + (R)a.buoy[1] * dF[1] +
+                                       (R)a.buoy[2] * dF[2]);
+    R* nacc = (R*)j.nacc;
+    nacc[id] = nacc[id] + dns_dphi * dns;
+}
+
 // ---------------------------------------------------------------- launchers
 int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
-    if (fp32) k_adj_collide<float><<<a.nblk, 256, 0, s>>>(a, j);
-    else k_adj_collide<double><<<a.nblk, 256, 0, s>>>(a, j);
+    if (fp32) k_adj_collide<float, false><<<a.nblk, 256, 0, s>>>(a, j);
+    else k_adj_collide<double, false><<<a.nblk, 256, 0, s>>>(a, j);
+    return 0;
+}
+int launch_adj_collide_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
+    if (fp32) k_adj_collide<float, true><<<a.nblk, 256, 0, s>>>(a, j);
+    else k_adj_collide<double, true><<<a.nblk, 256, 0, s>>>(a, j);
+    return 0;
+}
+int launch_cadj_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
+    if (fp32) k_cadj_flow<float><<<a.nblk, 256, 0, s>>>(a, j);
+    else k_cadj_flow<double><<<a.nblk, 256, 0, s>>>(a, j);
     return 0;
 }
 int launch_adj_stencil(int fp32, const StepArgs& a, const void* P, void* nout, hipStream_t s) {

@@ -1006,8 +1006,19 @@ int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, doubl
 int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, double* loss,
                     double* dL_deps4, double* dL_dkappa, double* times) {
     if (!h || !target || !loss || !dL_deps4) return fail("null argument");
-    if (h->q[LF] || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
-        return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model only");
+    if ((h->q[LF] != 0 && h->q[LF] != 19) || h->q[LC] || h->q[LG] != 19 || h->q[LH] != 7)
+        return fail("the adjoint covers the D3Q19 phase field + D3Q7 thermal model, with or "
+                    "without a D3Q19 flow");
+    // coupled model: the flow advects both scalars and feels the buoyancy
+    const bool flow = h->q[LF] == 19;
+    if (flow) {
+        if (h->p.flow_incomp) return fail("the coupled adjoint covers the compressible equilibrium");
+        for (int f = 0; f < 6; f++) {
+            const int b = h->p.bc_f[f];
+            if (b != MKG_BC_PERIODIC && b != MKG_BC_BOUNCE && b != MKG_BC_OUTFLOW)
+                return fail("the coupled adjoint takes periodic, bounce or outflow flow faces");
+        }
+    }
     if (h->has_walls) return fail("the adjoint does not support wall cells");
     if (h->p.aniso_kind != MKG_ANISO_CUBIC || h->p.pf_model != MKG_PF_KARMA_RAPPEL)
         return fail("the adjoint covers the cubic (Karma-Rappel) anisotropy only");
@@ -1019,21 +1030,23 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     const bool tape = ckpt == 0;
     if (tape) ckpt = nsteps;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
-    const int adv[4] = {0, 1, 1, 0};
+    const int adv[4] = {flow ? 1 : 0, 1, 1, 0};
     // ---- device storage
     struct Ck {
-        int lay_g, lay_h;
-        int64_t pf, steps_g, steps_h;
+        int lay_g, lay_h, lay_f;
+        int64_t pf, steps_g, steps_h, steps_f;
     };
     std::vector<Ck> ck(tape ? 1 : nseg + 1);
-    const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH);
-    const size_t ck_elems = PG + PH + 2 * S;
+    const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH), PF = flow ? h->pop_elems(LF) : 0;
+    const size_t ck_elems = PG + PH + 2 * S + PF;
+    // flow tape of one step (coupled model): the canonical pre-collision populations, FP64
+    const size_t fseg_stride = flow ? (size_t)19 * N : 0;
     // tape of one step, written by the forward kernel: phi^n, then the 7 pre-collision thermal
     // populations in own-slot order (stride S)
     const size_t seg_stride = 8 * S;
     const size_t seg_elems = (size_t)ckpt * seg_stride;
     char *ckbuf = nullptr, *segbuf = nullptr, *work = nullptr;
-    double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr;
+    double *dtarget = nullptr, *dpart = nullptr, *dscal = nullptr, *fseg = nullptr;
     int rc = 0;
     // events: 0-1 forward, 2-4 reverse phase, 5-6 segment tape filled, 7-8 segment tape free;
     // the reverse sweep runs on a stream of its own (default priority) when checkpointing
@@ -1053,6 +1066,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
             }
         hipFree(ckbuf);
         hipFree(segbuf);
+        hipFree(fseg);
         hipFree(work);
         hipFree(dtarget);
         hipFree(dpart);
@@ -1071,8 +1085,11 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     // one while the reverse sweep of the current segment reads the other
     const int nbuf = tape ? 1 : 2;
     ADJCHK(hipMalloc((void**)&segbuf, nbuf * seg_elems * e));
-    // work: Gin, Gout (19N), nin, nout (N), Hin, Hout (7N), pin, pout (N), P (19N), kap (N)
-    const size_t work_elems = (2 * 19 + 2 + 2 * 7 + 2 + 19 + 1) * N;
+    if (flow) ADJCHK(hipMalloc((void**)&fseg, nbuf * (size_t)ckpt * fseg_stride * sizeof(double)));
+    // work: Gin, Gout (19N), nin, nout (N), Hin, Hout (7N), pin, pout (N), P (19N), kap (N);
+    // coupled: Fin, Fout (19N), u~ adjoint (3N), thermal cell-wide in / out (N)
+    const size_t work_elems =
+        ((2 * 19 + 2 + 2 * 7 + 2 + 19 + 1) + (flow ? 2 * 19 + 3 + 2 : 0)) * N;
     ADJCHK(hipMalloc((void**)&work, work_elems * e));
     ADJCHK(hipMemsetAsync(work, 0, work_elems * e, h->stream));
     ADJCHK(hipMalloc((void**)&dtarget, N * sizeof(double)));
@@ -1090,6 +1107,11 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     char* pout = pin + N * e;
     char* Pb = pout + N * e;
     char* kap = Pb + 19 * N * e;
+    char* Fin = kap + N * e;
+    char* Fout = Fin + 19 * N * e;
+    char* Ub = Fout + 19 * N * e;
+    char* Cin = Ub + 3 * N * e;
+    char* Cout = Cin + N * e;
     double* eps_part = dpart;
     double* loss_part = dpart + h->nblk;
 
@@ -1099,7 +1121,11 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         HIPCHK(hipMemcpyAsync(base + PG * e, h->pop[LH], PH * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + (PG + PH) * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + (PG + PH + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
-        ck[k] = {h->lay[LG], h->lay[LH], h->pf_count, h->steps[LG], h->steps[LH]};
+        if (flow)
+            HIPCHK(hipMemcpyAsync(base + (PG + PH + 2 * S) * e, h->pop[LF], PF * e,
+                                  hipMemcpyDeviceToDevice, h->stream));
+        ck[k] = {h->lay[LG], h->lay[LH], h->lay[LF], h->pf_count, h->steps[LG], h->steps[LH],
+                 h->steps[LF]};
         return 0;
     };
     auto load_ck = [&](int k) -> int {
@@ -1108,19 +1134,34 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         HIPCHK(hipMemcpyAsync(h->pop[LH], base + PG * e, PH * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->phi[0], base + (PG + PH) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->phi[1], base + (PG + PH + S) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        if (flow)
+            HIPCHK(hipMemcpyAsync(h->pop[LF], base + (PG + PH + 2 * S) * e, PF * e,
+                                  hipMemcpyDeviceToDevice, h->stream));
         h->lay[LG] = ck[k].lay_g;
         h->lay[LH] = ck[k].lay_h;
         h->pf_count = ck[k].pf;
         h->steps[LG] = ck[k].steps_g;
         h->steps[LH] = ck[k].steps_h;
+        if (flow) {
+            h->lay[LF] = ck[k].lay_f;
+            h->steps[LF] = ck[k].steps_f;
+        }
         return 0;
     };
     for (int k = 0; k < 9; k++) ADJCHK(hipEventCreate(&ev[k]));
     if (!tape) ADJCHK(hipStreamCreate(&rs));
     auto seg_base = [&](int sgi) { return segbuf + (size_t)(sgi % nbuf) * seg_elems * e; };
+    auto fseg_base = [&](int sgi) { return fseg + (size_t)(sgi % nbuf) * ckpt * fseg_stride; };
     // a forward step that writes its tape slot: phi^n and the thermal populations before it
-    auto stepper = [&](int n, int n0, char* base) -> int {
+    // (by the step kernel), and with a flow the canonical flow populations before it
+    auto stepper = [&](int n, int n0, char* base, double* fbase) -> int {
         char* slot = base + (size_t)(n - n0) * seg_stride * e;
+        if (flow) {
+            StepArgs a = make_args(h);
+            if (launch_canonical(19, h->fp32, a, a.f, 0, fbase + (size_t)(n - n0) * fseg_stride,
+                                 h->stream))
+                return fail("flow tape kernel unavailable");
+        }
         return step_once_impl(h, adv, nullptr, [](mkg_ctx*) { return 0; }, slot);
     };
     // recompute of segment sgi from its checkpoint on the handle's stream, into its tape
@@ -1128,7 +1169,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);
         if (load_ck(sgi)) return -1;
         for (int n = n0; n < n1; n++)
-            if (stepper(n, n0, seg_base(sgi))) return -1;
+            if (stepper(n, n0, seg_base(sgi), fseg_base(sgi))) return -1;
         HIPCHK(hipEventRecord(ev[5 + sgi % 2], h->stream));
         return 0;
     };
@@ -1136,7 +1177,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     // ---- forward pass: checkpoints, or the tape
     for (int n = 0; n < nsteps; n++) {
         if (tape) {
-            if (stepper(n, 0, segbuf)) { rc = -1; break; }
+            if (stepper(n, 0, segbuf, fseg)) { rc = -1; break; }
             continue;
         }
         if (n % ckpt == 0 && save_ck(n / ckpt)) { rc = -1; break; }
@@ -1180,8 +1221,20 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
             j.hsave.pop = slot + S * e;
             j.hsave.lay = 0;
             j.eps_part = eps_part;
-            launch_adj_collide(h->fp32, ar, j, rs);
-            launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
+            if (flow) {
+                // scalars and phase with advection, stencil transpose, then the flow
+                j.ftape = fseg_base(sgi) + (size_t)(n - n0) * fseg_stride;
+                j.fin = Fin; j.fout = Fout; j.ut = Ub;
+                j.hcw_in = Cin; j.hcw_out = Cout; j.nacc = nout;
+                launch_adj_collide_flow(h->fp32, ar, j, rs);
+                launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
+                launch_cadj_flow(h->fp32, ar, j, rs);
+                std::swap(Fin, Fout);
+                std::swap(Cin, Cout);
+            } else {
+                launch_adj_collide(h->fp32, ar, j, rs);
+                launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
+            }
             std::swap(Gin, Gout);
             std::swap(nin, nout);
             std::swap(Hin, Hout);

@@ -21,68 +21,9 @@
 
 #include "kernels.h"
 #include "device.h"
+#include "fadj.h"
 
 namespace mkg {
-
-// Moments of a Q-vector that fix its w-orthogonal parts: mass m0, momentum j, the traceless
-// second moment T (xx, yy, zz, xy, xz, yz) and the trace moment tr = sum (c^2 - 3 cs^2) v.
-template <class R>
-struct Mom {
-    R m0 = 0, j0 = 0, j1 = 0, j2 = 0, T0 = 0, T1 = 0, T2 = 0, T3 = 0, T4 = 0, T5 = 0, tr = 0;
-    __device__ inline void add(int i, R x) {
-        const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
-        m0 += x;
-        j0 += X * x;
-        j1 += Y * x;
-        j2 += Z * x;
-        T0 += X * X * x;
-        T1 += Y * Y * x;
-        T2 += Z * Z * x;
-        T3 += X * Y * x;
-        T4 += X * Z * x;
-        T5 += Y * Z * x;
-    }
-    // second moments -> traceless part and trace moment
-    __device__ inline void finish() {
-        const R trace = T0 + T1 + T2, t3 = trace / (R)3;
-        T0 -= t3;
-        T1 -= t3;
-        T2 -= t3;
-        tr = trace - m0;
-    }
-};
-// Parts of direction i of the vector with moments m and component v: momentum, traceless
-// second order, trace, and the remainder (ghost).
-template <int Q, class R>
-__device__ inline void part_of(int i, const Mom<R>& m, R v, R& mo, R& sh, R& bk, R& gh) {
-    const R w = (R)wgt<Q>(i);
-    const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
-    mo = (R)3 * w * (X * m.j0 + Y * m.j1 + Z * m.j2);
-    sh = (R)4.5 * w * (X * X * m.T0 + Y * Y * m.T1 + Z * Z * m.T2 + (R)2 * X * Y * m.T3 +
-                       (R)2 * X * Z * m.T4 + (R)2 * Y * Z * m.T5);
-    bk = (R)1.5 * w * (R)(csq(i) - 1) * m.tr;
-    gh = v - w * m.m0 - mo - sh - bk;
-}
-
-// Deterministic block sum of NV values per thread (wave shuffles, then the four waves in
-// order); thread 0 adds the totals to part[k * nblk + block].
-template <int NV>
-__device__ inline void fadj_block_add(double (&v)[NV], double* part, uint32_t nblk) {
-    __shared__ double red[NV][4];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double x = v[k];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            part[(size_t)k * nblk + blockIdx.x] += red[k][0] + red[k][1] + red[k][2] + red[k][3];
-    }
-}
 
 // Loss and the adjoint of the final populations; partials 0 (loss) and 1..3 (dL/dF).
 template <int Q, class R>
@@ -157,95 +98,15 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_FADJ_WAVES_F32 : 1) 
                 v += lam_in[(size_t)i * N + c.id];
             l[i] = v;
         }
-        R f[Q], rho = 0, j[3] = {0, 0, 0};
+        R f[Q];
 #pragma unroll
-        for (int i = 0; i < Q; i++) {
-            f[i] = (R)fn[(size_t)i * N + c.id];
-            rho += f[i];
-            j[0] += cx(i) * f[i];
-            j[1] += cy(i) * f[i];
-            j[2] += cz(i) * f[i];
-        }
-        const R F0 = (R)a.force[0], F1 = (R)a.force[1], F2 = (R)a.force[2];
-        const R ir = (R)1 / rho;
-        const R u0 = (j[0] + (R)0.5 * F0) * ir, u1 = (j[1] + (R)0.5 * F1) * ir,
-                u2 = (j[2] + (R)0.5 * F2) * ir;
-        const R uu = u0 * u0 + u1 * u1 + u2 * u2, uF = u0 * F0 + u1 * F1 + u2 * F2;
-        const R s_nu = (R)a.s_nu, s_b = (R)a.s_bulk, s_o = (R)a.s_odd, s_e = (R)a.s_even;
-        auto feq_of = [&](int i) {
-            const R cu = cx(i) * u0 + cy(i) * u1 + cz(i) * u2;
-            return (R)wgt<Q>(i) * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * uu);
-        };
-        // d = f - feq + G/2 (for the rates; f is dead afterwards)
-        auto d_of = [&](int i) {
-            const R cu = cx(i) * u0 + cy(i) * u1 + cz(i) * u2;
-            const R cF = cx(i) * F0 + cy(i) * F1 + cz(i) * F2;
-            return f[i] - feq_of(i) + (R)0.5 * (R)wgt<Q>(i) * ((R)3 * (cF - uF) + (R)9 * cu * cF);
-        };
-        Mom<R> md, ml;
-#pragma unroll
-        for (int i = 0; i < Q; i++) {
-            md.add(i, d_of(i));
-            ml.add(i, (R)wgt<Q>(i) * l[i]);
-        }
-        md.finish();
-        ml.finish();
-        // rates: dL/dr_k = -l . P_k (f - feq + G/2)
-        R r4 = 0, r5 = 0, r6 = 0, r7 = 0;
-#pragma unroll
-        for (int i = 0; i < Q; i++) {
-            R mo, sh, bk, gi, mk, sk, bkk, gk;
-            part_of<Q, R>(i, md, d_of(i), mo, sh, bk, gi);
-            part_of<Q, R>(inv(i), md, d_of(inv(i)), mk, sk, bkk, gk);
-            r4 -= l[i] * sh;
-            r5 -= l[i] * bk;
-            r6 -= l[i] * (R)0.5 * (gi - gk);
-            r7 -= l[i] * (R)0.5 * (gi + gk);
-        }
-        gsum[4] = (double)r4;
-        gsum[5] = (double)r5;
-        gsum[6] = (double)r6;
-        gsum[7] = (double)r7;
-        // A^T l and C^T l of direction i through P^T = W^-1 P W
-        auto atl_ctl = [&](int i, R& ATl, R& CTl) {
-            R mo, sh, bk, gi, mk, sk, bkk, gk;
-            part_of<Q, R>(i, ml, (R)wgt<Q>(i) * l[i], mo, sh, bk, gi);
-            part_of<Q, R>(inv(i), ml, (R)wgt<Q>(inv(i)) * l[inv(i)], mk, sk, bkk, gk);
-            const R go = (R)0.5 * (gi - gk), ge = (R)0.5 * (gi + gk);
-            const R iw = (R)(1.0 / wgt<Q>(i));
-            ATl = (((R)1 - s_nu) * sh + ((R)1 - s_b) * bk + ((R)1 - s_o) * go + ((R)1 - s_e) * ge) * iw;
-            CTl = ((R)0.5 * mo + ((R)1 - (R)0.5 * s_nu) * sh + ((R)1 - (R)0.5 * s_b) * bk +
-                   ((R)1 - (R)0.5 * s_o) * go + ((R)1 - (R)0.5 * s_e) * ge) * iw;
-        };
-        R rb = 0, ub0 = 0, ub1 = 0, ub2 = 0, q1 = 0, q2 = 0, q3 = 0;
-#pragma unroll
-        for (int i = 0; i < Q; i++) {
-            R ATl, CTl;
-            atl_ctl(i, ATl, CTl);
-            // the final adjoint's A^T l part now, the rest once rb and ubar are known
-            lam_out[(size_t)i * N + c.id] = ATl;
-            const R w = (R)wgt<Q>(i);
-            const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
-            const R cu = X * u0 + Y * u1 + Z * u2;
-            const R cF = X * F0 + Y * F1 + Z * F2;
-            const R mu = l[i] - ATl;
-            rb += mu * feq_of(i) * ir;
-            const R a1 = mu * w * rho, a2 = CTl * w;
-            ub0 += a1 * ((R)3 * X + (R)9 * cu * X - (R)3 * u0) + a2 * ((R)-3 * F0 + (R)9 * X * cF);
-            ub1 += a1 * ((R)3 * Y + (R)9 * cu * Y - (R)3 * u1) + a2 * ((R)-3 * F1 + (R)9 * Y * cF);
-            ub2 += a1 * ((R)3 * Z + (R)9 * cu * Z - (R)3 * u2) + a2 * ((R)-3 * F2 + (R)9 * Z * cF);
-            q1 += a2 * ((R)3 * (X - u0) + (R)9 * cu * X);
-            q2 += a2 * ((R)3 * (Y - u1) + (R)9 * cu * Y);
-            q3 += a2 * ((R)3 * (Z - u2) + (R)9 * cu * Z);
-        }
-        gsum[1] = (double)(q1 + (R)0.5 * ub0 * ir);
-        gsum[2] = (double)(q2 + (R)0.5 * ub1 * ir);
-        gsum[3] = (double)(q3 + (R)0.5 * ub2 * ir);
-#pragma unroll
-        for (int i = 0; i < Q; i++) {
-            R* o = &lam_out[(size_t)i * N + c.id];
-            *o = *o + rb + (ub0 * (cx(i) - u0) + ub1 * (cy(i) - u1) + ub2 * (cz(i) - u2)) * ir;
-        }
+        for (int i = 0; i < Q; i++) f[i] = (R)fn[(size_t)i * N + c.id];
+        const R F[3] = {(R)a.force[0], (R)a.force[1], (R)a.force[2]};
+        const R ub_ext[3] = {0, 0, 0};
+        R dF[3], drate[4];
+        collide_reverse<Q, R, true>(a, f, F, l, ub_ext, lam_out, N, c.id, dF, drate);
+        for (int k = 0; k < 3; k++) gsum[1 + k] = (double)dF[k];
+        for (int k = 0; k < 4; k++) gsum[4 + k] = (double)drate[k];
     } else if (in) {
 #pragma unroll
         for (int i = 0; i < Q; i++) lam_out[(size_t)i * N + c.id] = (R)0;

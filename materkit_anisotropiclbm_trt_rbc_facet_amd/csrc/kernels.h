@@ -125,9 +125,21 @@ struct AdjArgs {
     const void* phin;    // N: phi^n of the forward step being reversed
     LatArgs hsave;       // forward thermal populations at level n (raw AA buffer + layout)
     double* eps_part;    // per-block FP64 partials of dL/deps4 (accumulated)
+    // coupled model (flow + thermal + phase):
+    const double* ftape; // Q x N canonical flow populations at level n (full values, FP64)
+    const void* fin;     // Q x N adjoint of the flow populations at level n+1
+    void* fout;          // Q x N adjoint of the flow populations at level n
+    void* ut;            // 3 x N dL/d(velocity the scalars are advected with), collide -> flow
+    const void* hcw_in;  // N cell-wide thermal adjoint of level n+1 (the buoyancy term)
+    void* hcw_out;       // N cell-wide thermal adjoint of level n
+    void* nacc;          // N cell-wide phase adjoint of level n: the flow kernel adds its part
 };
 
 int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
+// Coupled model: the scalar/phase reverse with the advection velocity (flow 1), and the flow
+// reverse with partial bounce-back, buoyancy and the scalars' velocity adjoint.
+int launch_adj_collide_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
+int launch_cadj_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
 int launch_adj_stencil(int fp32, const StepArgs& a, const void* P, void* nout, hipStream_t s);
 int launch_adj_loss(int fp32, const StepArgs& a, const double* target, void* nseed, double* part,
                     double* out, hipStream_t s);

@@ -256,7 +256,8 @@ constexpr int zslots_of() {
 //   B3  after the phase push stage, which reuses the flow pull region (last read before B2)
 // Every other stage region is written once and read once, on either side of one barrier.
 // TAPE: the step also writes the adjoint tape (phi^n and the pre-collision thermal
-// populations of every cell; instantiated for the phase-field + thermal model only).
+// populations of every cell; instantiated for the phase-field + thermal model, with or
+// without the D3Q19 flow).
 template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false,
           bool TAPE = false>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
@@ -594,7 +595,7 @@ template <class R, int FQ, int PQ, bool TH, bool SO>
 inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
     const bool per = !a.has_walls && (a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask) == 0;
-    if constexpr (FQ == 0 && PQ == 19 && TH && !SO) {
+    if constexpr ((FQ == 0 || FQ == 19) && PQ == 19 && TH && !SO) {
         // the adjoint's forward steps (the phase-field + thermal model), with the kernel
         // variant a plain step would take, so that the recompute repeats the forward exactly
         if (a.tape) {
