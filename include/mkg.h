@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define MKG_ABI_VERSION 4
+#define MKG_ABI_VERSION 5
 
 /* ---- node flags: identical codes to MaterKit/inc/mkGlobalDefs.h:140-145 ---- */
 #define MKG_IS_OUTLET (-2)
@@ -236,6 +236,17 @@ int mkg_init_seeds(mkg_handle h, int n, const double* centers, const double* rad
  * unit quaternions (Shoemake), both drawn from a splitmix64 stream of `seed`. */
 int mkg_random_seeds(uint64_t seed, int n, double radius, int nx, int ny, int nz,
                      double* centers, double* quats);
+/* Anisotropy of one crystal at an interface normal, the query of mkPhaseField::anisotropy /
+ * anisotropy_facet / anisotropy_cbc / anisotropy_hcp (MaterKit/inc/mkPhaseField.h:61-64) and
+ * of the per-grain mkMultiphaseField versions (MaterKit/inc/mkMultiphaseField.h:31-34), with
+ * the forms of the kernels: kind MKG_ANISO_*, eps4 and par[2] its parameters, rot the
+ * row-major lab <- crystal rotation (NULL: identity), normal the lab-frame normal (any length).
+ * Returns a_s(n) and n_v = R a_s (D - n (n.D)), D = d a_s / d n in the crystal frame, so that
+ * the flux of the phase equation is J = |grad phi| (a_s^2 n + n_v).  Returns 1 (a_s = 1,
+ * n_v = 0) for a zero normal, 0 otherwise. */
+int mkg_aniso_eval(int kind, double eps4, const double* par, const double* rot,
+                   const double* normal, double* a_s, double* n_v);
+
 /* Quaternion (w,x,y,z) to row-major rotation matrix. */
 int mkg_quat_to_rot(const double* q, double* R);
 
@@ -305,7 +316,16 @@ int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, doubl
  * follows a conservative Allen-Cahn LB (Chai et al. 2018, the getAddedPhasePDFs term) with
  * its own advection velocity and the pairwise interaction term of the template; the combined
  * field phi = (n - 1) + sum_j phi_j is kept beside them.  One handle owns its device memory
- * and one HIP stream. */
+ * and one HIP stream.
+ *
+ * MKG_MPF_KARMA_RAPPEL makes every order parameter an anisotropic Karma-Rappel grain of its
+ * own form (MKG_ANISO_*), strength and orientation, driven by T or by the solute's U, with the
+ * pairwise interaction on the same time scale tau0 a_s^2 -- the per-grain anisotropy and the
+ * thermal / solutal growth of mkMultiphaseField (anisotropy, anisotropy_facet, anisotropy_cbc,
+ * anisotropy_hcp, relaxation_anisotropy_thermal, relaxation_solute, relaxation_thermal;
+ * MaterKit/inc/mkMultiphaseField.h:31-45).  One grain is the single-grain phase field of
+ * mkg_step exactly.  The optional D3Q7 solute lattice follows the single-grain solute equation
+ * (c = A(phi) U, anti-trapping current) with the combined phi. */
 #define MKG_MPF_MAX 4
 typedef struct mkg_mpf_params {
     int32_t nx, ny, nz;
@@ -324,7 +344,20 @@ typedef struct mkg_mpf_params {
     double alpha, latent_T, T_ref;
     double T_face[6];
     double phi_face[6];         /* Dirichlet value of the phase lattices */
+    int32_t model;              /* MKG_MPF_TEMPLATE or MKG_MPF_KARMA_RAPPEL */
+    int32_t coupling;           /* field driving Karma-Rappel grains: 0 none, 1 T, 2 U */
+    int32_t aniso_kind[MKG_MPF_MAX];  /* MKG_ANISO_* form of each grain */
+    double eps4[MKG_MPF_MAX];         /* cubic strength of each grain (MKG_ANISO_CUBIC) */
+    double aniso[MKG_MPF_MAX][2];     /* parameters of the other forms, per grain */
+    double rot[MKG_MPF_MAX][9];       /* row-major lab <- crystal rotation of each grain */
+    int32_t solute;             /* 1: D3Q7 solute lattice (Karma-Rappel grains) */
+    int32_t bc_c[6];            /* face codes of the solute lattice */
+    double D_l, D_s, k_part, q_min, latent_C, at_coef;  /* as in mkg_params */
+    double C_face[6];           /* Dirichlet U of the solute faces */
 } mkg_mpf_params;
+
+#define MKG_MPF_TEMPLATE 0      /* conservative Allen-Cahn order parameters (the template) */
+#define MKG_MPF_KARMA_RAPPEL 1  /* anisotropic grains, see above */
 
 typedef struct mkg_mpf_ctx* mkg_mpf_handle;
 
@@ -333,13 +366,17 @@ typedef struct mkg_mpf_ctx* mkg_mpf_handle;
 #define MKG_MPF_PHI_PREV 2  /* (n + 1) x N: the pass before */
 #define MKG_MPF_THERMAL 3   /* 7 x N thermal populations (doubles) */
 #define MKG_MPF_FLAG 4      /* N int8: 0 fluid, 3 wall (half-way bounce-back) */
+#define MKG_MPF_SOLUTE 5    /* 7 x N solute populations (doubles) */
 
 size_t mkg_mpf_params_size(void);
 int mkg_mpf_params_default(mkg_mpf_params* m);
 int mkg_mpf_create(const mkg_mpf_params* m, mkg_mpf_handle* out);
 int mkg_mpf_destroy(mkg_mpf_handle h);
-/* Equilibrium start: phi n x N order parameters, T N (NULL: T_ref), both doubles. */
+/* Equilibrium start: phi n x N order parameters, T N (NULL: T_ref), both doubles; the
+ * solute lattice (if any) starts at c = 0. */
 int mkg_mpf_init(mkg_mpf_handle h, const double* phi, const double* T);
+/* The same with the solute c (N doubles, NULL: 0). */
+int mkg_mpf_init_fields(mkg_mpf_handle h, const double* phi, const double* T, const double* C);
 /* nsteps steps: phase moment pass, then collide-stream of every lattice (fused). */
 int mkg_mpf_step(mkg_mpf_handle h, int nsteps);
 int mkg_mpf_sync(mkg_mpf_handle h);

@@ -1,8 +1,14 @@
 // Auxiliary kernels: layout-independent population read/write, macroscopic fields,
 // initial states, seed geometry and the FP64 diagnostic reductions.
+#include <cmath>
+#include <string>
+
+#include "../../include/mkg.h"
 #include "cellred.h"
 
 namespace mkg {
+
+void set_error(const std::string& msg);
 
 int launch_step_f32(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
 int launch_step_f64(const Combo& k, int odd, const StepArgs& a, hipStream_t s);
@@ -320,3 +326,38 @@ int launch_nonfinite(int fp32, const void* p, size_t n, double* partials, double
 }
 
 }  // namespace mkg
+
+// Host evaluation of the anisotropy of one crystal (mkPhaseField / mkMultiphaseField
+// anisotropy, anisotropy_facet, anisotropy_cbc, anisotropy_hcp): the forms of the kernels.
+extern "C" int mkg_aniso_eval(int kind, double eps4, const double* par, const double* rot,
+                              const double* normal, double* a_s, double* n_v) {
+    if (!normal || !a_s || !n_v) {
+        mkg::set_error("mkg_aniso_eval: normal, a_s and n_v must not be null");
+        return -1;
+    }
+    if (kind < MKG_ANISO_CUBIC || kind > MKG_ANISO_FACET) {
+        mkg::set_error("mkg_aniso_eval: kind must be one of MKG_ANISO_*");
+        return -1;
+    }
+    const double g2 = normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2];
+    if (!(g2 > 1e-30)) {  // no interface direction: isotropic
+        *a_s = 1.0;
+        n_v[0] = n_v[1] = n_v[2] = 0.0;
+        return 1;
+    }
+    const double gn = std::sqrt(g2);
+    const double l[3] = {normal[0] / gn, normal[1] / gn, normal[2] / gn};
+    double c[3];  // crystal frame: R^T l
+    for (int k = 0; k < 3; k++)
+        c[k] = rot ? rot[k] * l[0] + rot[3 + k] * l[1] + rot[6 + k] * l[2] : l[k];
+    const mkg::Aniso<double> an{kind, eps4, par ? par[0] : 0.0, par ? par[1] : 0.0};
+    double D[3];
+    const double as = mkg::aniso_eval<double>(an, c[0], c[1], c[2], D[0], D[1], D[2]);
+    const double nD = c[0] * D[0] + c[1] * D[1] + c[2] * D[2];
+    double t[3];
+    for (int k = 0; k < 3; k++) t[k] = as * (D[k] - c[k] * nD);
+    for (int k = 0; k < 3; k++)
+        n_v[k] = rot ? rot[3 * k] * t[0] + rot[3 * k + 1] * t[1] + rot[3 * k + 2] * t[2] : t[k];
+    *a_s = as;
+    return 0;
+}

@@ -540,6 +540,23 @@ __device__ inline void flow_collide(R* f, R dr, R rho, R ux, R uy, R uz, R Fx, R
     }
 }
 
+// Isotropic LB-weighted gradient of the stencil (D3Q19 weights over cs^2 = 1/3), summed over
+// differences to the centre value (sum_i c_i w_i = 0): in the bulk, where phi is near +-1, the
+// terms are then small instead of cancelling O(1) values, which keeps the FP32 gradient (and
+// the direction of the anti-trapping current taken from it) accurate.
+template <class R>
+__device__ inline void stencil_gradient(const R (&P)[27], R (&gr)[3]) {
+    gr[0] = gr[1] = gr[2] = 0;
+    const R p0 = P[13];
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+        const R v = (R)(3.0 * wgt<19>(i)) * (P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] - p0);
+        if (cx(i)) gr[0] += (R)cx(i) * v;
+        if (cy(i)) gr[1] += (R)cy(i) * v;
+        if (cz(i)) gr[2] += (R)cz(i) * v;
+    }
+}
+
 // ------------------------------------------------------------------ phase-field source
 template <class R>
 __device__ inline R pow4(R v) {
@@ -558,7 +575,8 @@ struct Aniso {
     R e4, p0, p1;
 };
 template <class R>
-__device__ inline R aniso_eval(const Aniso<R>& an, R n0, R n1, R n2, R& d0, R& d1, R& d2) {
+__host__ __device__ inline R aniso_eval(const Aniso<R>& an, R n0, R n1, R n2, R& d0, R& d1,
+                                        R& d2) {
     const R q0 = n0 * n0, q1 = n1 * n1, q2 = n2 * n2;
     if (an.kind == 1) {  // two cubic harmonics
         const R Q = q0 * q0 + q1 * q1 + q2 * q2, S = q0 * q1 * q2;

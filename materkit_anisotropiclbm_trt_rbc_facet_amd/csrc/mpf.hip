@@ -47,6 +47,14 @@ struct MpfArgs {
     double omega_g;  // 1 / (dt / wdx^-2 / cs^2 + 1/2)
     double zc;       // (1 - omega_g / 2) / (sqrt 2 W0/dx): scale of the Allen-Cahn counter-term
     double dt, omega_h, latent_T, T_ref;
+    // Karma-Rappel grains and the solute lattice
+    int32_t coupling;  // 0 none, 1 T, 2 U
+    int32_t need_T, need_U;  // launches other than the last read T / U from the populations
+    int32_t akind[MKG_MPF_MAX];
+    int32_t rotated[MKG_MPF_MAX];  // 0: identity rotation
+    double e4[MKG_MPF_MAX], ap[MKG_MPF_MAX][2], rot[MKG_MPF_MAX][9];
+    double wdx2;
+    double D_l, D_s, k_part, q_min, latent_C, at_w;
 };
 
 template <class R, bool ODD>
@@ -102,22 +110,60 @@ __device__ inline void mpf_gradient(const StepArgs& a, const Cell& c, const R* p
     }
 }
 
-// Collide-stream of one order parameter j (one launch per order parameter; the thermal
-// lattice rides on the last one).  The populations of order parameter j are updated in place,
-// which no other launch of the step reads; T for the growth term comes from the thermal
-// populations, which only the last launch updates.  O steps take the z shift through LDS when
-// a block holds whole z rows (ZS): one barrier after the pull loads, one after the push stage.
+template <class R>
+__device__ inline R clamp_pm1(R v) {
+    return v < (R)-1 ? (R)-1 : (v > (R)1 ? (R)1 : v);
+}
+
+// The 3 x 3 x 3 neighbourhood of order parameter plane ph around the cell (the 19 points of
+// D3Q19; corners unused): periodic faces wrap, other faces replicate the edge value.
+template <class R>
+__device__ inline void mpf_stencil(const StepArgs& a, const Cell& c, const R* ph, R (&P)[27]) {
+    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
+    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
+    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & XLO) px[0] = px[1];
+    if (np & XHI) px[2] = px[1];
+    if (np & YLO) py[0] = py[1];
+    if (np & YHI) py[2] = py[1];
+    if (np & ZLO) pz[0] = pz[1];
+    if (np & ZHI) pz[2] = pz[1];
+#pragma unroll
+    for (int k = 0; k < 27; k++) P[k] = 0;
+#pragma unroll
+    for (int i = 0; i < 19; i++)
+        P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
+            *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+}
+
+#ifndef MKG_MPF_KR_WAVES_F32
+#define MKG_MPF_KR_WAVES_F32 4
+#endif
+
+// Collide-stream of one order parameter j (one launch per order parameter; the thermal and
+// solute lattices ride on the last one).  The populations of order parameter j are updated in
+// place, which no other launch of the step reads; T and U for the growth terms come from the
+// thermal and solute populations, which only the last launch updates.  O steps take the z
+// shift through LDS when a block holds whole z rows (ZS): on the last launch one barrier after
+// the scalar pull loads, then one after the phase pull loads and one after the phase push
+// stage.  The last launch collides the scalar lattices first, so that their populations are
+// dead before the phase stencil is loaded.
+// KR: Karma-Rappel grain (the single-grain phase source with the grain's anisotropy and
+// rotation, interaction on the time scale tau0 a_s^2), else the template's conservative
+// Allen-Cahn order parameter.
 // Launched per order parameter, the kernel holds one lattice in registers (four waves per
 // SIMD); the earlier single launch over all order parameters took 168-232 VGPRs.
-template <class R, bool TH, bool ODD, bool ZS = false>
-__global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
-    k_mpf_one(const MpfArgs m, const LatArgs L, const int j, const double ux_, const double uy_,
-              const double uz_, const double kappa_j) {
+template <class R, bool TH, bool SO, bool KR, bool ODD, bool ZS = false>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F32 : 4) : 1)
+    k_mpf_one(const MpfArgs m, const LatArgs L, const int j) {
     constexpr bool Z = ODD && ZS;
     constexpr int ZQ = Z ? nzdirs<19>() : 1;
     constexpr int ZT = (Z && TH) ? nzdirs<7>() : 1;
-    __shared__ R zr[ZQ][256], zw[ZQ][256], zrh[ZT][256], zwh[ZT][256];
-    __shared__ uint32_t zm[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1];
+    constexpr int ZC = (Z && SO) ? nzdirs<7>() : 1;
+    __shared__ R zr[ZQ][256], zw[ZQ][256], zrh[ZT][256], zwh[ZT][256], zrc[ZC][256],
+        zwc[ZC][256];
+    __shared__ uint32_t zm[Z ? 256 : 1], zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
     const uint32_t tid = threadIdx.x;
     // Blocks (z stage) or waves none of whose cells lies on a non-periodic face, in a domain
     // without wall cells (and, with the z stage, wholly before the last cell), take the body
@@ -127,7 +173,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         const uint32_t t = thread_cell(m.a);
         Cell e;
         locate_at(m.a, t < m.a.N ? t : m.a.N - 1u, e);
-        const uint32_t np = (uint32_t)(L.npmask | (TH ? m.a.h.npmask : 0));
+        const uint32_t np = (uint32_t)(L.npmask | (TH ? m.a.h.npmask : 0) |
+                                       (SO ? m.a.c.npmask : 0));
         const int face = (e.bits & np) != 0u;
         if constexpr (Z)
             slow = __syncthreads_or(face || t >= m.a.N) != 0;
@@ -155,46 +202,39 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
     const R* __restrict__ cur = (const R*)a.phi_cur;
     const R* __restrict__ prev = (const R*)a.phi_prev;
     const R pj = *slot_at(cur + (size_t)j * S, c.id);
-    R P = 0;
-    for (int k = 0; k < m.n; k++) {
-        if (k != j) {
-            const R hk = (R)0.5 * *slot_at(cur + (size_t)k * S, c.id) + (R)0.5;
-            P += hk * hk;
-        }
-    }
-    P *= (R)(-2.0 * kappa_j) * ((R)0.5 * pj + (R)0.5);
-    R s[3];
-    mpf_gradient<R>(a, c, cur + (size_t)j * S, s);
+    // combined phi (moment pass of this step) and its change, for the scalar lattices
+    const R pc = *slot_at(cur + (size_t)m.n * S, c.id);
+    const R dpc = pc - *slot_at(prev + (size_t)m.n * S, c.id);
 
-    // loads: order parameter j, and the thermal populations on the last launch
-    R g[19];
-    uint32_t own = 0;
-    if constexpr (Z)
-        own = zpull_issue<19, R, FACES>(g, L, S, c, a, zr, tid);
-    else
-        read_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
+    // ---- scalar lattices (the last launch): pull loads, moments, collide, push stage
     R h[TH ? 7 : 1];
-    uint32_t own_h = 0;
+    R cc[SO ? 7 : 1];
+    uint32_t own_h = 0, own_c = 0;
     if constexpr (TH) {
         if constexpr (Z)
             own_h = zpull_issue<7, R, FACES>(h, a.h, S, c, a, zrh, tid);
         else
             read_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
     }
-    if constexpr (Z) {
-        __syncthreads();
-        zpull_finish<19, R>(g, c, a, zr, tid, own);
-        if constexpr (TH) zpull_finish<7, R>(h, c, a, zrh, tid, own_h);
+    if constexpr (SO) {
+        if constexpr (Z)
+            own_c = zpull_issue<7, R, FACES>(cc, a.c, S, c, a, zrc, tid);
+        else
+            read_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar);
     }
-    R T = (R)m.T_ref;
+    if constexpr (Z && (TH || SO)) {
+        __syncthreads();
+        if constexpr (TH) zpull_finish<7, R>(h, c, a, zrh, tid, own_h);
+        if constexpr (SO) zpull_finish<7, R>(cc, c, a, zrc, tid, own_c);
+    }
+    R T = (R)m.T_ref, U = 0;
     if constexpr (TH) {
         // thermal collide-stream (mkThermalField::propagate / relaxation of run_multiPhaseField)
         R sh = 0;
 #pragma unroll
         for (int i = 0; i < 7; i++) sh += h[i];
         T = (R)a.h.ref + sh;
-        const R dphi = *slot_at(cur + (size_t)m.n * S, c.id) - *slot_at(prev + (size_t)m.n * S, c.id);
-        const R omh = (R)m.omega_h, srh = (R)m.latent_T * dphi;
+        const R omh = (R)m.omega_h, srh = (R)m.latent_T * dpc;
 #pragma unroll
         for (int i = 0; i < 7; i++) {
             const R w = (R)wgt<7>(i);
@@ -206,7 +246,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         else
             write_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
     } else {
-        if (m.thermal && m.growth != 0.0) {
+        if (m.need_T) {
             R hh[7];
             read_pops_rt<7, R>(hh, a.h, S, c, a, kKindScalar);
             R sh = 0;
@@ -215,15 +255,113 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
             T = (R)a.h.ref + sh;
         }
     }
-    const R sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + (R)m.eps;
-    const R n0 = s[0] / sn, n1 = s[1] / sn, n2 = s[2] / sn;
-    const R d = (R)1 - pj * pj;
-    const R Q = (R)m.growth * (pj - (R)m.lambda * T * d) * d;
-    const R src = (R)m.dt * (P + Q);
-    const R ux = (R)ux_, uy = (R)uy_, uz = (R)uz_;
+    // the solute sees the combined phi clamped to [-1, 1] (overlapping grains exceed 1, where
+    // A(phi) would vanish)
+    const R ps = clamp_pm1(pc);
+    const R A = (R)0.5 * (((R)1 + (R)m.k_part) - ((R)1 - (R)m.k_part) * ps);
+    if constexpr (SO) {
+        // solute collide-stream: the single-grain solute equation with the clamped phi
+        R sc = 0;
+#pragma unroll
+        for (int i = 0; i < 7; i++) sc += cc[i];
+        const R C = (R)a.c.ref + sc;
+        U = C / A;
+        const R dps = ps - clamp_pm1(*slot_at(prev + (size_t)m.n * S, c.id));
+        R Pc[27];
+        mpf_stencil<R>(a, c, cur + (size_t)m.n * S, Pc);
+#pragma unroll
+        for (int k = 0; k < 27; k++) Pc[k] = clamp_pm1(Pc[k]);
+        R gphi[3];
+        stencil_gradient<R>(Pc, gphi);
+        R q = (R)0.5 * ((R)1 - ps);
+        if (m.D_l > 0) q += (R)(m.D_s / m.D_l) * (R)0.5 * ((R)1 + ps);
+        if (q < (R)m.q_min) q = (R)m.q_min;
+        const R Dc = (R)m.D_l * q / A;
+        const R om = (R)1 / ((R)0.5 + Dc / (R)0.25);
+        const R vfac = (R)(-0.5 * (1.0 - m.k_part)) * (R)m.D_l * q / (A * A);
+        R jx = C * vfac * gphi[0], jy = C * vfac * gphi[1], jz = C * vfac * gphi[2];
+        const R gg = gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2];
+        if (m.at_w != 0.0 && gg > (R)1e-20) {
+            const R jat = -(R)m.at_w * ((R)1 + ((R)1 - (R)m.k_part) * U) * dps / sqrt(gg);
+            jx += jat * gphi[0];
+            jy += jat * gphi[1];
+            jz += jat * gphi[2];
+        }
+        const R src = (R)m.latent_C * dps;
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+            const R w = (R)wgt<7>(i);
+            const R cj = cx(i) * jx + cy(i) * jy + cz(i) * jz;
+            const R ceq = w * (sc + cj / (R)0.25);
+            cc[i] = cc[i] - om * (cc[i] - ceq) + w * src;
+        }
+        FaceState<R> fs;
+        fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
+        if constexpr (Z)
+            zpush_stage<7, R, FACES>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+        else
+            write_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar, fs);
+    } else {
+        if (m.need_U) {
+            R c7[7];
+            read_pops_rt<7, R>(c7, a.c, S, c, a, kKindScalar);
+            R sc = 0;
+#pragma unroll
+            for (int i = 0; i < 7; i++) sc += c7[i];
+            U = ((R)a.c.ref + sc) / A;
+        }
+    }
+
+    // ---- order parameter j: interaction, source, then its populations
+    R P = 0;
+    for (int k = 0; k < m.n; k++) {
+        if (k != j) {
+            const R hk = (R)0.5 * *slot_at(cur + (size_t)k * S, c.id) + (R)0.5;
+            P += hk * hk;
+        }
+    }
+    P *= (R)(-2.0 * m.kappa[j]) * ((R)0.5 * pj + (R)0.5);
+    R src, zd, n0 = 0, n1 = 0, n2 = 0;
+    if constexpr (KR) {
+        R Pj[27];
+        mpf_stencil<R>(a, c, cur + (size_t)j * S, Pj);
+        R Mrot[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) Mrot[k] = (R)m.rot[j][k];
+        const Aniso<R> an{m.akind[j], (R)m.e4[j], (R)m.ap[j][0], (R)m.ap[j][1]};
+        const R upf = m.coupling == 1 ? T : (m.coupling == 2 ? U : (R)0);
+        R as;
+        const R Sg = phase_source<R>(Pj, upf, an, (R)m.lambda, (R)m.dt, (R)m.wdx2, Mrot,
+                                     m.rotated[j] != 0, as);
+        src = Sg + (R)m.dt * P / (as * as);
+        zd = 0;
+    } else {
+        R s[3];
+        mpf_gradient<R>(a, c, cur + (size_t)j * S, s);
+        const R sn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + (R)m.eps;
+        n0 = s[0] / sn;
+        n1 = s[1] / sn;
+        n2 = s[2] / sn;
+        const R d = (R)1 - pj * pj;
+        const R Q = (R)m.growth * (pj - (R)m.lambda * T * d) * d;
+        src = (R)m.dt * (P + Q);
+        zd = (R)m.zc * d;
+    }
+    R g[19];
+    uint32_t own = 0;
+    if constexpr (Z)
+        own = zpull_issue<19, R, FACES>(g, L, S, c, a, zr, tid);
+    else
+        read_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
+    if constexpr (Z) {
+        __syncthreads();
+        zpull_finish<19, R>(g, c, a, zr, tid, own);
+        if constexpr (TH) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
+        if constexpr (SO) zpush_finish<7, R, FACES>(a.c, S, c, a, zwc, zmc, tid);
+    }
+    const R ux = (R)m.u[j][0], uy = (R)m.u[j][1], uz = (R)m.u[j][2];
     const R uu = ux * ux + uy * uy + uz * uz;
     const R dp = pj - (R)L.ref;
-    const R zd = (R)m.zc * d;
     const R om = (R)m.omega_g;
 #pragma unroll
     for (int i = 0; i < 19; i++) {
@@ -238,7 +376,6 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
         zpush_stage<19, R, FACES>(g, L, S, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
         __syncthreads();
         zpush_finish<19, R, FACES>(L, S, c, a, zw, zm, tid);
-        if constexpr (TH) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
     } else {
         write_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
     }
@@ -250,10 +387,12 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 4 : 1)
 }
 
 // Equilibrium start in the own-slot layout: phase populations from phi_j (doubles, n x N) with
-// each order parameter's velocity, both phase buffers, thermal populations from T (or T_ref).
+// each order parameter's velocity, both phase buffers, thermal populations from T (or T_ref),
+// solute populations from c (or 0).
 template <class R>
 __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double* phij,
-                                                  const double* Tin, R* phi0, R* phi1) {
+                                                  const double* Tin, const double* Cin,
+                                                  R* phi0, R* phi1) {
     Cell c;
     if (!locate(m.a, c)) return;
     const uint32_t S = m.a.S, N = m.a.N;
@@ -283,6 +422,13 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
         for (int i = 0; i < 7; i++)
             *slot_at(hb + (size_t)i * S, c.id) = (R)(wgt<7>(i) * (t - m.a.h.ref));
     }
+    if (m.a.c.pop) {
+        const double cv = Cin ? Cin[c.id] : 0.0;
+        R* cb = (R*)m.a.c.pop;
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+            *slot_at(cb + (size_t)i * S, c.id) = (R)(wgt<7>(i) * (cv - m.a.c.ref));
+    }
 }
 
 template <class R>
@@ -308,6 +454,7 @@ struct mkg_mpf_ctx {
     hipStream_t stream = nullptr;
     void* g = nullptr;        // n lattices of 19 (S + F) elements
     void* h = nullptr;        // 7 (S + F) elements
+    void* c = nullptr;        // 7 (S + F) elements (solute)
     void* phi[2] = {nullptr, nullptr};  // (n + 1) S elements each
     int8_t* flag = nullptr;
     double* scratch = nullptr;  // 19 N doubles
@@ -370,6 +517,7 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
         if (j < p.n) m.gl[j].pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
     }
     if (h->h) fill(a.h, h->h, p.bc_h, p.T_ref, p.T_face);
+    if (h->c) fill(a.c, h->c, p.bc_c, 0.0, p.C_face);
     a.phi_cur = h->phi[h->pf_count & 1];
     a.phi_prev = h->phi[(h->pf_count + 1) & 1];
     m.lat = h->lat_elems();
@@ -388,6 +536,29 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
     m.omega_h = 1.0 / (0.5 + p.alpha / 0.25);
     m.latent_T = p.latent_T;
     m.T_ref = p.T_ref;
+    const bool kr = p.model == MKG_MPF_KARMA_RAPPEL;
+    m.coupling = kr ? p.coupling : 0;
+    m.need_T = h->h && (kr ? p.coupling == 1 : p.growth != 0.0);
+    m.need_U = h->c && kr && p.coupling == 2;
+    for (int j = 0; j < MKG_MPF_MAX; j++) {
+        m.akind[j] = p.aniso_kind[j];
+        m.e4[j] = p.eps4[j];
+        m.ap[j][0] = p.aniso[j][0];
+        m.ap[j][1] = p.aniso[j][1];
+        int ident = 1;
+        for (int k = 0; k < 9; k++) {
+            m.rot[j][k] = p.rot[j][k];
+            ident &= p.rot[j][k] == ((k % 4 == 0) ? 1.0 : 0.0);
+        }
+        m.rotated[j] = !ident;
+    }
+    m.wdx2 = p.wdx * p.wdx;
+    m.D_l = p.D_l;
+    m.D_s = p.D_s;
+    m.k_part = h->c ? p.k_part : 1.0;
+    m.q_min = p.q_min;
+    m.latent_C = p.latent_C;
+    m.at_w = p.at_coef * p.wdx;
     return m;
 }
 
@@ -402,9 +573,10 @@ int mpf_normalize(mkg_mpf_ctx* h) {
             launch_store_pops(19, h->fp32, m.a, L, h->scratch, h->stream))
             return mfail("normalize: lattice unavailable");
     }
-    if (h->h) {
-        if (launch_canonical(7, h->fp32, m.a, m.a.h, 1, h->scratch, h->stream) ||
-            launch_store_pops(7, h->fp32, m.a, m.a.h, h->scratch, h->stream))
+    for (const LatArgs* S7 : {&m.a.h, &m.a.c}) {
+        if (!S7->pop) continue;
+        if (launch_canonical(7, h->fp32, m.a, *S7, 1, h->scratch, h->stream) ||
+            launch_store_pops(7, h->fp32, m.a, *S7, h->scratch, h->stream))
             return mfail("normalize: lattice unavailable");
     }
     MCHK(hipGetLastError());
@@ -412,30 +584,42 @@ int mpf_normalize(mkg_mpf_ctx* h) {
     return 0;
 }
 
-template <class R, bool TH>
+template <class R, bool TH, bool SO, bool KR>
 void launch_mpf_one(const MpfArgs& m, int j, int odd, int zs, hipStream_t s) {
     dim3 grid(m.a.nblk), block(256);
-    const double* u = m.u[j];
     if (odd && zs)
-        k_mpf_one<R, TH, true, true><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+        k_mpf_one<R, TH, SO, KR, true, true><<<grid, block, 0, s>>>(m, m.gl[j], j);
     else if (odd)
-        k_mpf_one<R, TH, true><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+        k_mpf_one<R, TH, SO, KR, true><<<grid, block, 0, s>>>(m, m.gl[j], j);
     else
-        k_mpf_one<R, TH, false><<<grid, block, 0, s>>>(m, m.gl[j], j, u[0], u[1], u[2], m.kappa[j]);
+        k_mpf_one<R, TH, SO, KR, false><<<grid, block, 0, s>>>(m, m.gl[j], j);
 }
-// Collide-stream of every lattice: one launch per order parameter, thermal on the last.
+// Collide-stream of every lattice: one launch per order parameter, the scalar lattices on the
+// last.  The solute lattice exists only with Karma-Rappel grains.
 template <class R>
-void launch_mpf_step_kernel(const MpfArgs& m, int odd, int th, hipStream_t s) {
+void launch_mpf_step_kernel(const MpfArgs& m, int odd, int th, int so, int kr, hipStream_t s) {
+    const int zs = m.a.zstage;
     for (int j = 0; j < m.n; j++) {
-        if (th && j == m.n - 1) launch_mpf_one<R, true>(m, j, odd, m.a.zstage, s);
-        else launch_mpf_one<R, false>(m, j, odd, m.a.zstage, s);
+        const bool last = j == m.n - 1;
+        if (!kr) {
+            if (th && last) launch_mpf_one<R, true, false, false>(m, j, odd, zs, s);
+            else launch_mpf_one<R, false, false, false>(m, j, odd, zs, s);
+        } else if (last && th && so) {
+            launch_mpf_one<R, true, true, true>(m, j, odd, zs, s);
+        } else if (last && th) {
+            launch_mpf_one<R, true, false, true>(m, j, odd, zs, s);
+        } else if (last && so) {
+            launch_mpf_one<R, false, true, true>(m, j, odd, zs, s);
+        } else {
+            launch_mpf_one<R, false, false, true>(m, j, odd, zs, s);
+        }
     }
 }
 
 void mpf_destroy_impl(mkg_mpf_ctx* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->g, h->h, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
+    void* bufs[] = {h->g, h->h, h->c, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
                     h->red_out};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -462,6 +646,11 @@ int mkg_mpf_params_default(mkg_mpf_params* m) {
     m->alpha = 0.1;
     m->latent_T = 0.5;
     for (int f = 0; f < 6; f++) m->phi_face[f] = -1.0;
+    for (int j = 0; j < MKG_MPF_MAX; j++)
+        for (int k = 0; k < 3; k++) m->rot[j][4 * k] = 1.0;
+    m->k_part = 1.0;
+    m->q_min = 1e-3;
+    m->latent_C = 0.5;
     return 0;
 }
 
@@ -480,15 +669,42 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     if ((n + 65536.0) * esz * (p->n + 1) >= 4294967296.0)
         return mfail("grid too large: the phase buffer (cells x (n+1) x element size) must stay below 4 GiB");
     for (int f = 0; f < 6; f++) {
-        const int32_t codes[2] = {p->bc[f], p->bc_h[f]};
+        const int32_t codes[3] = {p->bc[f], p->bc_h[f], p->bc_c[f]};
         for (int32_t v : codes)
             if (v < MKG_BC_PERIODIC || v > MKG_BC_OUTFLOW)
                 return mfail("face codes of the multi-phase field are periodic, bounce, Dirichlet or outflow");
     }
     for (int a = 0; a < 3; a++)
         if ((p->bc[2 * a] == 0) != (p->bc[2 * a + 1] == 0) ||
-            (p->bc_h[2 * a] == 0) != (p->bc_h[2 * a + 1] == 0))
+            (p->bc_h[2 * a] == 0) != (p->bc_h[2 * a + 1] == 0) ||
+            (p->bc_c[2 * a] == 0) != (p->bc_c[2 * a + 1] == 0))
             return mfail("a periodic axis needs periodic codes on both faces");
+    if (p->model != MKG_MPF_TEMPLATE && p->model != MKG_MPF_KARMA_RAPPEL)
+        return mfail("model must be MKG_MPF_TEMPLATE or MKG_MPF_KARMA_RAPPEL");
+    if (p->solute && p->model != MKG_MPF_KARMA_RAPPEL)
+        return mfail("the solute lattice drives Karma-Rappel grains (model MKG_MPF_KARMA_RAPPEL)");
+    if (p->model == MKG_MPF_KARMA_RAPPEL) {
+        if (p->coupling < 0 || p->coupling > 2) return mfail("coupling must be 0, 1 (T) or 2 (U)");
+        if (p->coupling == 1 && !p->thermal) return mfail("coupling 1 needs the thermal lattice");
+        if (p->coupling == 2 && !p->solute) return mfail("coupling 2 needs the solute lattice");
+        for (int j = 0; j < p->n; j++) {
+            if (p->aniso_kind[j] < MKG_ANISO_CUBIC || p->aniso_kind[j] > MKG_ANISO_FACET)
+                return mfail("aniso_kind of every grain must be one of MKG_ANISO_*");
+            double err = 0;  // rotations: R^T R = I
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) {
+                    double d = 0;
+                    for (int k = 0; k < 3; k++) d += p->rot[j][3 * k + r] * p->rot[j][3 * k + q];
+                    err = std::fmax(err, std::fabs(d - (r == q ? 1.0 : 0.0)));
+                }
+            if (!(err < 1e-9)) return mfail("rot of every grain must be an orthogonal matrix");
+        }
+    }
+    if (p->solute) {
+        if (!(p->D_l >= 0) || !(p->D_s >= 0)) return mfail("D_l and D_s must be >= 0");
+        if (!(p->k_part > 0) || !(p->k_part <= 1)) return mfail("k_part must lie in (0, 1]");
+        if (!(p->q_min > 0)) return mfail("q_min must be > 0");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return mfail("no HIP device available");
     mkg_mpf_ctx* h = new mkg_mpf_ctx();
@@ -512,10 +728,14 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     const size_t gb = (size_t)p->n * h->lat_elems() * h->esz;
     if ((e = hipMalloc(&h->g, gb)) != hipSuccess) return bad(e, "hipMalloc");
     if ((e = hipMemsetAsync(h->g, 0, gb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    const size_t sb = (size_t)7 * ((size_t)h->S + h->F) * h->esz;
     if (p->thermal) {
-        const size_t hb = (size_t)7 * ((size_t)h->S + h->F) * h->esz;
-        if ((e = hipMalloc(&h->h, hb)) != hipSuccess) return bad(e, "hipMalloc");
-        if ((e = hipMemsetAsync(h->h, 0, hb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+        if ((e = hipMalloc(&h->h, sb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(h->h, 0, sb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    }
+    if (p->solute) {
+        if ((e = hipMalloc(&h->c, sb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(h->c, 0, sb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
     }
     const size_t pb = (size_t)(p->n + 1) * h->S * h->esz;
     for (int b = 0; b < 2; b++) {
@@ -540,11 +760,16 @@ int mkg_mpf_destroy(mkg_mpf_handle h) {
 }
 
 int mkg_mpf_init(mkg_mpf_handle h, const double* phi, const double* T) {
+    return mkg_mpf_init_fields(h, phi, T, nullptr);
+}
+
+int mkg_mpf_init_fields(mkg_mpf_handle h, const double* phi, const double* T, const double* C) {
     if (!h) return mfail("null handle");
     if (!phi) return mfail("phi is null");
     const size_t N = h->N;
     double* dphi = nullptr;
     double* dT = nullptr;
+    double* dC = nullptr;
     MCHK(hipMalloc((void**)&dphi, (size_t)h->p.n * N * sizeof(double)));
     hipError_t e = hipMemcpyAsync(dphi, phi, (size_t)h->p.n * N * sizeof(double),
                                   hipMemcpyHostToDevice, h->stream);
@@ -553,22 +778,29 @@ int mkg_mpf_init(mkg_mpf_handle h, const double* phi, const double* T) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(dT, T, N * sizeof(double), hipMemcpyHostToDevice, h->stream);
     }
+    if (e == hipSuccess && C && h->c) {
+        e = hipMalloc((void**)&dC, N * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dC, C, N * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    }
     if (e == hipSuccess) {
         h->lay = 0;
         h->pf_count = 0;
         MpfArgs m = mpf_args(h);
         dim3 grid(h->nblk), block(256);
         if (h->fp32)
-            k_mpf_init<float><<<grid, block, 0, h->stream>>>(m, dphi, dT, (float*)h->phi[0],
+            k_mpf_init<float><<<grid, block, 0, h->stream>>>(m, dphi, dT, dC, (float*)h->phi[0],
                                                              (float*)h->phi[1]);
         else
-            k_mpf_init<double><<<grid, block, 0, h->stream>>>(m, dphi, dT, (double*)h->phi[0],
+            k_mpf_init<double><<<grid, block, 0, h->stream>>>(m, dphi, dT, dC,
+                                                              (double*)h->phi[0],
                                                               (double*)h->phi[1]);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(dphi);
     if (dT) (void)hipFree(dT);
+    if (dC) (void)hipFree(dC);
     if (e != hipSuccess) return mfail(std::string("mkg_mpf_init: ") + hipGetErrorString(e));
     return 0;
 }
@@ -582,8 +814,10 @@ int mkg_mpf_step(mkg_mpf_handle h, int nsteps) {
         const int odd = h->lay;
         if (h->fp32) mpf_launch_moments<float>(m, odd, h->phi[h->pf_count & 1], h->stream);
         else mpf_launch_moments<double>(m, odd, h->phi[h->pf_count & 1], h->stream);
-        if (h->fp32) launch_mpf_step_kernel<float>(m, odd, h->h != nullptr, h->stream);
-        else launch_mpf_step_kernel<double>(m, odd, h->h != nullptr, h->stream);
+        const int th = h->h != nullptr, so = h->c != nullptr;
+        const int kr = h->p.model == MKG_MPF_KARMA_RAPPEL;
+        if (h->fp32) launch_mpf_step_kernel<float>(m, odd, th, so, kr, h->stream);
+        else launch_mpf_step_kernel<double>(m, odd, th, so, kr, h->stream);
         h->lay ^= 1;
     }
     MCHK(hipGetLastError());
@@ -602,13 +836,14 @@ int mkg_mpf_get(mkg_mpf_handle h, int which, void* host, size_t count) {
     const size_t N = h->N;
     const int n = h->p.n;
     MpfArgs m = mpf_args(h);
-    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL) {
+    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL || which == MKG_MPF_SOLUTE) {
         const int nl = which == MKG_MPF_POPS ? n : 1;
         const int q = which == MKG_MPF_POPS ? 19 : 7;
         if (which == MKG_MPF_THERMAL && !h->h) return mfail("thermal lattice not present");
+        if (which == MKG_MPF_SOLUTE && !h->c) return mfail("solute lattice not present");
         if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
         for (int j = 0; j < nl; j++) {
-            LatArgs L = which == MKG_MPF_POPS ? m.a.g : m.a.h;
+            LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
             if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
             if (launch_canonical(q, h->fp32, m.a, L, 1, h->scratch, h->stream))
                 return mfail("lattice unavailable");
@@ -644,15 +879,16 @@ int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count) {
     if (!host) return mfail("host buffer is null");
     const size_t N = h->N;
     const int n = h->p.n;
-    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL) {
+    if (which == MKG_MPF_POPS || which == MKG_MPF_THERMAL || which == MKG_MPF_SOLUTE) {
         const int nl = which == MKG_MPF_POPS ? n : 1;
         const int q = which == MKG_MPF_POPS ? 19 : 7;
         if (which == MKG_MPF_THERMAL && !h->h) return mfail("thermal lattice not present");
+        if (which == MKG_MPF_SOLUTE && !h->c) return mfail("solute lattice not present");
         if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
         if (mpf_normalize(h)) return -1;
         MpfArgs m = mpf_args(h);
         for (int j = 0; j < nl; j++) {
-            LatArgs L = which == MKG_MPF_POPS ? m.a.g : m.a.h;
+            LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
             if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
             MCHK(hipMemcpyAsync(h->scratch, (const double*)host + (size_t)j * q * N,
                                 (size_t)q * N * sizeof(double), hipMemcpyHostToDevice, h->stream));

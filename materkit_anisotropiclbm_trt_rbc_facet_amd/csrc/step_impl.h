@@ -45,23 +45,6 @@ __device__ inline void load_phase_stencil(const StepArgs& a, const Cell& c, R (&
             *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
 }
 
-// Isotropic LB-weighted gradient of the stencil (D3Q19 weights over cs^2 = 1/3), summed over
-// differences to the centre value (sum_i c_i w_i = 0): in the bulk, where phi is near +-1, the
-// terms are then small instead of cancelling O(1) values, which keeps the FP32 gradient (and
-// the direction of the anti-trapping current taken from it) accurate.
-template <class R>
-__device__ inline void stencil_gradient(const R (&P)[27], R (&gr)[3]) {
-    gr[0] = gr[1] = gr[2] = 0;
-    const R p0 = P[13];
-#pragma unroll
-    for (int i = 1; i < 19; i++) {
-        const R v = (R)(3.0 * wgt<19>(i)) * (P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] - p0);
-        if (cx(i)) gr[0] += (R)cx(i) * v;
-        if (cy(i)) gr[1] += (R)cy(i) * v;
-        if (cz(i)) gr[2] += (R)cz(i) * v;
-    }
-}
-
 // Velocity faces (non-equilibrium extrapolation, mkFluidsUser::boundary_vel,
 // MaterKit/src/mkFluidsUser.cpp:455-505): a face cell of a velocity face replaces its
 // post-collision populations by the ones the pre-pass k_nee formed from its inward neighbour

@@ -94,7 +94,8 @@ FVM_PHI, FVM_CONC, FVM_U = 0, 1, 2
 MP_POP_FLOW, MP_POP_PHASE, MP_PHI, MP_MU, MP_PRESSURE, MP_VEL, MP_PHIU = range(7)
 MP_VISC_NU_LINEAR, MP_VISC_MU_LINEAR, MP_VISC_MU_HARMONIC = range(3)
 MPF_MAX = 4
-MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG = range(5)
+MPF_POPS, MPF_PHI, MPF_PHI_PREV, MPF_THERMAL, MPF_FLAG, MPF_SOLUTE = range(6)
+MPF_TEMPLATE, MPF_KARMA_RAPPEL = 0, 1
 
 MPF_FIELDS = [
     ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n", C.c_int32),
@@ -105,6 +106,12 @@ MPF_FIELDS = [
     ("growth", C.c_double), ("lambda_", C.c_double), ("eps", C.c_double),
     ("alpha", C.c_double), ("latent_T", C.c_double), ("T_ref", C.c_double),
     ("T_face", C.c_double * 6), ("phi_face", C.c_double * 6),
+    ("model", C.c_int32), ("coupling", C.c_int32), ("aniso_kind", C.c_int32 * MPF_MAX),
+    ("eps4", C.c_double * MPF_MAX), ("aniso", (C.c_double * 2) * MPF_MAX),
+    ("rot", (C.c_double * 9) * MPF_MAX),
+    ("solute", C.c_int32), ("bc_c", C.c_int32 * 6),
+    ("D_l", C.c_double), ("D_s", C.c_double), ("k_part", C.c_double), ("q_min", C.c_double),
+    ("latent_C", C.c_double), ("at_coef", C.c_double), ("C_face", C.c_double * 6),
 ]
 
 
@@ -204,6 +211,8 @@ def lib() -> C.CDLL:
         "mkg_mpf_create": (C.c_int, [P(MpfParams), P(vp)]),
         "mkg_mpf_destroy": (C.c_int, [vp]),
         "mkg_mpf_init": (C.c_int, [vp, vp, vp]),
+        "mkg_mpf_init_fields": (C.c_int, [vp, vp, vp, vp]),
+        "mkg_aniso_eval": (C.c_int, [C.c_int, C.c_double, vp, vp, vp, vp, vp]),
         "mkg_mpf_step": (C.c_int, [vp, C.c_int]),
         "mkg_mpf_sync": (C.c_int, [vp]),
         "mkg_mpf_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
@@ -282,6 +291,20 @@ def random_seeds(seed: int, n: int, radius: float, dims) -> tuple[np.ndarray, np
     _check(lib().mkg_random_seeds(seed, n, radius, int(dims[0]), int(dims[1]), int(dims[2]),
                                   _ptr(centers), _ptr(quats)))
     return centers, quats
+
+
+def aniso_eval(kind: int, normal, eps4: float = 0.0, par=(0.0, 0.0), rot=None):
+    """a_s and n_v = R a_s (D - n (n.D)) of one crystal at the lab-frame normal (the
+    anisotropy query of mkPhaseField / mkMultiphaseField)."""
+    nrm = np.ascontiguousarray(normal, dtype=np.float64)
+    pp = np.ascontiguousarray(par, dtype=np.float64)
+    R = None if rot is None else np.ascontiguousarray(rot, dtype=np.float64).reshape(-1)
+    a = C.c_double()
+    nv = np.zeros(3)
+    rc = lib().mkg_aniso_eval(int(kind), float(eps4), _ptr(pp), _ptr(R), _ptr(nrm),
+                              C.byref(a), _ptr(nv))
+    _check(0 if rc == 1 else rc)  # 1: zero normal, isotropic
+    return a.value, nv
 
 
 def quat_to_rot(q) -> np.ndarray:
@@ -472,12 +495,19 @@ class MpfSolver:
     def __exit__(self, *a):
         self.close()
 
-    def init(self, phi, T=None) -> None:
+    def init(self, phi, T=None, C=None) -> None:
+        """Equilibrium start from phi (n x N), T and the solute c (N each, None: T_ref, 0)."""
         ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
         if ph.size != self.n * self.N:
             raise MkgError("phi must hold n x N values")
         t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
-        _check(lib().mkg_mpf_init(self._h, _ptr(ph), _ptr(t)))
+        if C is None:
+            _check(lib().mkg_mpf_init(self._h, _ptr(ph), _ptr(t)))
+            return
+        c = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
+        if c.size != self.N or (t is not None and t.size != self.N):
+            raise MkgError("T and C must hold N values")
+        _check(lib().mkg_mpf_init_fields(self._h, _ptr(ph), _ptr(t), _ptr(c)))
 
     def step(self, nsteps: int = 1) -> None:
         _check(lib().mkg_mpf_step(self._h, int(nsteps)))
@@ -488,7 +518,7 @@ class MpfSolver:
     def _count(self, which: int) -> int:
         return {MPF_POPS: self.n * 19 * self.N, MPF_PHI: (self.n + 1) * self.N,
                 MPF_PHI_PREV: (self.n + 1) * self.N, MPF_THERMAL: 7 * self.N,
-                MPF_FLAG: self.N}[which]
+                MPF_FLAG: self.N, MPF_SOLUTE: 7 * self.N}[which]
 
     def get(self, which: int) -> np.ndarray:
         n = self._count(which)

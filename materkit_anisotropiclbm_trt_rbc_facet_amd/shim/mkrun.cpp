@@ -8,9 +8,11 @@
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk | plt]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
 //         solutal_dendrite | solutal_only | rbc | allen_cahn | multi_phase_field |
-//         multiphase | wetting | fused
+//         grains | multiphase | wetting | fused
 // (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
-// run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; multiphase: run_flow_multiphase,
+// run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; grains: the same loop on the
+// anisotropic multi-grain class mkMultiphaseField with relaxation_anisotropy_thermal or
+// relaxation_solute, as the case drives the grains; multiphase: run_flow_multiphase,
 // MaterKit.cpp:290-331, and wetting: run_flow_multiphase_wetting, :333-375, on
 // mkFluidsMultiphase)
 #include <algorithm>
@@ -73,6 +75,46 @@ static int run_multi_phase_field(const std::string& xml, int maxLoops, int maxFr
     return 0;
 }
 
+// The multi-phase loop on the anisotropic multi-grain class: per frame the solid cells
+// (phi_j > 0) of every grain
+static int run_grains(const std::string& xml, int maxLoops, int maxFrame, bool vtk) {
+    mkSpace space(xml);
+    mkMultiphaseField flow(space);
+    flow.initialize(xml);
+    flow.initialize_geometry();
+    std::printf("%20s= %d \n%20s= %d \n\nframe \t\t\t loops\n", "maxLoops ", maxLoops,
+                "maxFrame ", maxFrame);
+    const int every = std::max(1, maxLoops / maxFrame);
+    const mkg_mpf_params& m = flow.mpf_params();
+    const size_t N = (size_t)m.nx * m.ny * m.nz;
+    int frame = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int loops = 0; loops <= maxLoops; loops++) {
+        if (loops % every == 0) {
+            const std::vector<double> ph = flow.order_parameters();
+            std::printf("%05d\t%18d", frame, loops);
+            for (int j = 0; j < m.n; j++) {
+                size_t solid = 0;
+                for (size_t k = 0; k < N; k++) solid += ph[(size_t)j * N + k] > 0.0;
+                std::printf("\tsolid_%d %zu", j, solid);
+            }
+            std::printf("\n");
+            if (vtk) flow.write_vtkfile(loops);
+            frame++;
+        }
+        if (loops == maxLoops) break;
+        flow.propagate();
+        if (m.coupling == 2)
+            flow.relaxation_solute();
+        else
+            flow.relaxation_anisotropy_thermal();
+    }
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("wall %.3f s, %.1f MLUPS (including frame output)\n", s,
+                (double)N * maxLoops / s / 1e6);
+    return 0;
+}
+
 // run_flow_multiphase (MaterKit/MaterKit.cpp:290-331) on the two-phase class
 static int run_flow_multiphase(const std::string& xml, int maxLoops, int maxFrame, bool vtk,
                                bool wetting = false) {
@@ -128,6 +170,7 @@ int main(int argc, char** argv) {
         const bool plt = argc > 5 && std::strcmp(argv[5], "plt") == 0;
         if (maxFrame < 1) maxFrame = 1;
         if (loop == "multi_phase_field") return run_multi_phase_field(xml, maxLoops, maxFrame, vtk);
+        if (loop == "grains") return run_grains(xml, maxLoops, maxFrame, vtk);
         if (loop == "multiphase") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk);
         if (loop == "wetting") return run_flow_multiphase(xml, maxLoops, maxFrame, vtk, true);
         mkSpace space(xml);

@@ -380,6 +380,181 @@ void mkFluidsLBM::write_vtkfile(int loops) {
     }
 }
 
+// ------------------------------------------------------------------ anisotropic multi-grain field
+mkMultiphaseField::~mkMultiphaseField() {
+    if (m_) mkg_mpf_destroy(m_);
+}
+
+void mkMultiphaseField::initialize(const std::string& xmlfname) {
+    xml_ = xmlfname;
+    if (mkg_params_from_xml(xmlfname.c_str(), &p_, run_info_)) throw_io("initialize");
+    seed_number = run_info_[2];
+    phase_number = seed_number;
+    if (phase_number < 1 || phase_number > MKG_MPF_MAX)
+        throw mkt::mkLogicException("the multi-grain phase field needs 1.." +
+                                    std::to_string(MKG_MPF_MAX) + " seeds (one grain each)");
+    std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS), R(9 * MKG_MAX_GRAINS);
+    if (mkg_seeds_from_xml(xmlfname.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), R.data()) < 0)
+        throw_io("seeds");
+    vec_center.assign(c.begin(), c.begin() + 3 * phase_number);
+    vec_angle.assign(R.begin(), R.begin() + 9 * phase_number);  // as rotation matrices
+    check(mkg_mpf_params_default(&mp_));
+    mp_.nx = p_.nx;
+    mp_.ny = p_.ny;
+    mp_.nz = p_.nz;
+    mp_.n = phase_number;
+    mp_.precision = p_.precision;
+    mp_.model = MKG_MPF_KARMA_RAPPEL;
+    mp_.thermal = p_.thermal;
+    mp_.solute = p_.solute;
+    mp_.coupling = p_.pf_coupling;
+    mp_.dt = p_.pf_dt;
+    mp_.wdx = std::sqrt(p_.pf_wdx2);
+    mp_.lambda = p_.lambda;
+    mp_.alpha = p_.alpha_l;
+    mp_.latent_T = p_.latent_T;
+    mp_.T_ref = p_.T_ref;
+    mp_.D_l = p_.D_l;
+    mp_.D_s = p_.D_s;
+    mp_.k_part = p_.k_part;
+    mp_.q_min = p_.q_min;
+    mp_.latent_C = p_.latent_C;
+    mp_.at_coef = p_.at_coef;
+    for (int f = 0; f < 6; f++) {
+        mp_.bc[f] = p_.bc_g[f];
+        mp_.bc_h[f] = p_.bc_h[f];
+        mp_.bc_c[f] = p_.bc_c[f];
+        mp_.T_face[f] = p_.T_face[f];
+        mp_.C_face[f] = p_.C_face[f];
+    }
+    for (int j = 0; j < phase_number; j++) {
+        mp_.u[j][0] = 0.0;  // grains do not move
+        mp_.kappa[j] = 1.0;
+        mp_.aniso_kind[j] = p_.aniso_kind;
+        mp_.eps4[j] = p_.eps4;
+        mp_.aniso[j][0] = p_.aniso[0];
+        mp_.aniso[j][1] = p_.aniso[1];
+        for (int k = 0; k < 9; k++) mp_.rot[j][k] = R[9 * j + k];
+    }
+    if (m_) mkg_mpf_destroy(m_);
+    m_ = nullptr;
+    check(mkg_mpf_create(&mp_, &m_));
+}
+
+void mkMultiphaseField::initialize_geometry() {
+    if (!m_) throw mkt::mkLogicException("initialize_geometry before initialize");
+    // phi_j = -tanh((|x - c_j| - r_j) / (sqrt 2 W0/dx)); T = the case's initial value;
+    // c = A(phi) U0 of the combined phi, U0 = the case's initial supersaturation variable
+    std::vector<double> c(3 * MKG_MAX_GRAINS), r(MKG_MAX_GRAINS);
+    if (mkg_seeds_from_xml(xml_.c_str(), MKG_MAX_GRAINS, c.data(), r.data(), nullptr) < 0)
+        throw_io("seeds");
+    const size_t N = (size_t)mp_.nx * mp_.ny * mp_.nz;
+    const int n = mp_.n;
+    std::vector<double> phi((size_t)n * N), C(N);
+    const double width = std::sqrt(2.0) * mp_.wdx;
+    for (int j = 0; j < n; j++)
+        for (int x = 0; x < mp_.nx; x++)
+            for (int y = 0; y < mp_.ny; y++)
+                for (int z = 0; z < mp_.nz; z++) {
+                    const double dx = x - c[3 * j], dy = y - c[3 * j + 1], dz = z - c[3 * j + 2];
+                    const double d = std::sqrt(dx * dx + dy * dy + dz * dz) - r[j];
+                    phi[(size_t)j * N + ((size_t)x * mp_.ny + y) * mp_.nz + z] = -std::tanh(d / width);
+                }
+    for (size_t i = 0; i < N; i++) {
+        double comb = n - 1.0;
+        for (int j = 0; j < n; j++) comb += phi[(size_t)j * N + i];
+        comb = std::fmin(1.0, std::fmax(-1.0, comb));
+        C[i] = 0.5 * ((1 + mp_.k_part) - (1 - mp_.k_part) * comb) * p_.ref_c;
+    }
+    check(mkg_mpf_init_fields(m_, phi.data(), nullptr, mp_.solute ? C.data() : nullptr));
+}
+
+void mkMultiphaseField::advance(int mask) {
+    if (!m_) throw mkt::mkLogicException("relaxation before initialize");
+    if (mask & (MKG_PHASE | MKG_FLOW)) check(mkg_mpf_step(m_, 1));
+    pending_ &= ~mask;
+}
+
+void mkMultiphaseField::driven(int coupling, const char* name) {
+    if (!m_) throw mkt::mkLogicException(std::string(name) + " before initialize");
+    if (mp_.coupling != coupling)
+        throw mkt::mkLogicException(std::string(name) + ": the case drives the grains by " +
+                                    (mp_.coupling == 1 ? "T" : (mp_.coupling == 2 ? "U" : "nothing")));
+    advance(MKG_PHASE);
+}
+
+void mkMultiphaseField::relaxation_AllenCahn() {
+    throw mkt::mkLogicException(
+        "mkMultiphaseField::relaxation_AllenCahn is not provided: the Allen-Cahn order "
+        "parameters are mkFluidsUser's (mkg_mpf model MKG_MPF_TEMPLATE)");
+}
+
+void mkMultiphaseField::evolution_anisotropy() {
+    throw mkt::mkLogicException(
+        "mkMultiphaseField::evolution_anisotropy (finite volumes) is not provided: the grains "
+        "evolve on the lattice (relaxation_anisotropy_thermal / relaxation_solute)");
+}
+
+bool mkMultiphaseField::grain_query(int kind, int id, mkVector3d& N_v, mkReal& A_s,
+                                    const mkVector3d& normal) {
+    if (id < 1 || id > mp_.n)
+        throw mkt::mkLogicException("anisotropy: grain id must be 1.." + std::to_string(mp_.n));
+    const int j = id - 1;
+    const int rc = mkg_aniso_eval(kind, mp_.eps4[j], mp_.aniso[j], mp_.rot[j], normal.ptr(),
+                                  &A_s, N_v.ptr());
+    if (rc < 0) throw mkt::mkLogicException(mkg_last_error());
+    return rc == 0;
+}
+
+std::vector<double> mkMultiphaseField::order_parameters() const {
+    const size_t N = (size_t)mp_.nx * mp_.ny * mp_.nz;
+    std::vector<double> v((size_t)(mp_.n + 1) * N);
+    check(mkg_mpf_get(m_, MKG_MPF_PHI, v.data(), v.size()));
+    return v;
+}
+
+void mkMultiphaseField::write_vtkfile(int loops) {
+    const int Mx = mp_.nx, My = mp_.ny, Mz = mp_.nz;
+    const size_t N = (size_t)Mx * My * Mz;
+    std::ostringstream fn;
+    std::error_code ec;
+    std::filesystem::create_directories(workspace_ + "vtk", ec);
+    fn << workspace_ << "vtk/grains_t" << loops << ".vtk";
+    std::ofstream o(fn.str());
+    if (!o) throw mkt::mkIOException("cannot write " + fn.str());
+    o << "# vtk DataFile Version 3.0\ngrains\nASCII\nDATASET RECTILINEAR_GRID\n";
+    o << "DIMENSIONS " << Mx << " " << My << " " << Mz << "\n";
+    auto coords = [&](const char* nm, int m) {
+        o << nm << " " << m << " float\n";
+        for (int i = 0; i < m; i++) o << i + 0.5 << " ";
+        o << "\n";
+    };
+    coords("X_COORDINATES", Mx);
+    coords("Y_COORDINATES", My);
+    coords("Z_COORDINATES", Mz);
+    o << "POINT_DATA " << N << "\n" << std::setprecision(9);
+    const std::vector<double> ph = order_parameters();
+    auto scalar = [&](const std::string& nm, const double* v) {
+        o << "SCALARS " << nm << " float 1\nLOOKUP_TABLE default\n";
+        for (int z = 0; z < Mz; z++)
+            for (int y = 0; y < My; y++)
+                for (int x = 0; x < Mx; x++) o << v[((size_t)x * My + y) * Mz + z] << "\n";
+    };
+    for (int j = 0; j < mp_.n; j++) scalar("phi_" + std::to_string(j), ph.data() + (size_t)j * N);
+    scalar("phi", ph.data() + (size_t)mp_.n * N);
+    // grain id: the order parameter with the largest phi where any exceeds 0, else 0
+    std::vector<double> gid(N, 0.0);
+    for (size_t i = 0; i < N; i++) {
+        double best = 0.0;
+        for (int j = 0; j < mp_.n; j++)
+            if (ph[(size_t)j * N + i] > best) {
+                best = ph[(size_t)j * N + i];
+                gid[i] = j + 1;
+            }
+    }
+    scalar("grain", gid.data());
+}
+
 // ------------------------------------------------------------------ multi-phase field
 mkFluidsUser::~mkFluidsUser() {
     if (m_) mkg_mpf_destroy(m_);

@@ -15,6 +15,7 @@
 // mkt::mkLogicException (everything else), as the reference's exception classes
 // (MaterKit/inc/mkRunTimeDiagnostics.h:71-117).
 #pragma once
+#include <cmath>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -29,6 +30,37 @@ struct mkLogicException : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 }  // namespace mkt
+
+typedef double mkReal;  // the double-precision build of MaterKit/inc/mkGlobalDefs.h:175-180
+
+// Three-vector of the reference (MaterKit/inc/mkVectors.h:123-221): the members the
+// anisotropy queries and drivers use.
+class mkVector3d {
+public:
+    mkVector3d() = default;
+    mkVector3d(mkReal x, mkReal y, mkReal z) : v_{x, y, z} {}
+    void set(mkReal x, mkReal y, mkReal z) {
+        v_[0] = x;
+        v_[1] = y;
+        v_[2] = z;
+    }
+    mkReal& operator[](int i) { return v_[i]; }
+    mkReal operator[](int i) const { return v_[i]; }
+    mkReal* ptr() { return v_; }
+    const mkReal* ptr() const { return v_; }
+    mkReal x() const { return v_[0]; }
+    mkReal y() const { return v_[1]; }
+    mkReal z() const { return v_[2]; }
+    mkReal operator*(const mkVector3d& r) const { return v_[0] * r[0] + v_[1] * r[1] + v_[2] * r[2]; }
+    mkVector3d operator*(mkReal a) const { return {v_[0] * a, v_[1] * a, v_[2] * a}; }
+    mkVector3d operator+(const mkVector3d& r) const { return {v_[0] + r[0], v_[1] + r[1], v_[2] + r[2]}; }
+    mkVector3d operator-(const mkVector3d& r) const { return {v_[0] - r[0], v_[1] - r[1], v_[2] - r[2]}; }
+    mkReal length_sq() const { return *this * *this; }
+    mkReal length() const { return std::sqrt(length_sq()); }
+
+private:
+    mkReal v_[3] = {0, 0, 0};
+};
 
 // Grid of a case file (MaterKit/inc/mkSpace.h:29-73).
 class mkSpace {
@@ -198,8 +230,35 @@ public:
     void relaxation_AllenCahn() { allen_cahn(); }
     void relaxation_AllenCahn_growth_thermal() { allen_cahn(); }
     void reset_flags() {}  // flags are re-classified inside every phase step
+    // Anisotropy of grain id (1-based, the node grain ids; 0 or no grains: unrotated) at the
+    // interface normal (MaterKit/inc/mkPhaseField.h:61-64): A_s = a_s(n) and
+    // N_v = R a_s (D - n (n.D)) of the case's strengths, in the form the name gives
+    // (anisotropy: the cubic form).  x, y, z are not used: the grain id names the crystal.
+    // False for a zero normal (A_s = 1, N_v = 0).
+    bool anisotropy(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                    const int& x, const int& y, const int& z) {
+        return aniso_query(MKG_ANISO_CUBIC, id, N_v, A_s, normal);
+    }
+    bool anisotropy_facet(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                          const int& x, const int& y, const int& z) {
+        return aniso_query(MKG_ANISO_FACET, id, N_v, A_s, normal);
+    }
+    bool anisotropy_cbc(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                        const int& x, const int& y, const int& z) {
+        return aniso_query(MKG_ANISO_CBC, id, N_v, A_s, normal);
+    }
+    bool anisotropy_hcp(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                        const int& x, const int& y, const int& z) {
+        return aniso_query(MKG_ANISO_HCP, id, N_v, A_s, normal);
+    }
 
 private:
+    bool aniso_query(int kind, int id, mkVector3d& N_v, mkReal& A_s, const mkVector3d& normal) {
+        const double* R = (id > 0 && id <= p_.ngrains) ? p_.rot[id - 1] : nullptr;
+        const int rc = mkg_aniso_eval(kind, p_.eps4, p_.aniso, R, normal.ptr(), &A_s, N_v.ptr());
+        if (rc < 0) throw mkt::mkLogicException(mkg_last_error());
+        return rc == 0;
+    }
     void aniso_form(int kind, const char* name) {
         if (p_.aniso_kind != kind)
             throw mkt::mkLogicException(std::string("relaxation_anisotropy_") + name +
@@ -255,6 +314,70 @@ protected:
     void advance(int mask) override;
 
 private:
+    mkg_mpf_handle m_ = nullptr;
+    mkg_mpf_params mp_{};
+};
+
+// Anisotropic multi-grain phase field (MaterKit/inc/mkMultiphaseField.h:23-58) on an mkg_mpf
+// handle with Karma-Rappel grains: one order parameter per seed of the case file (at most
+// MKG_MPF_MAX), each with the case's anisotropy form and strengths (<phasechange>
+// <anisotropy>, <solver><anisotropyForm>) and the seed's own orientation (the ZYX Euler
+// angles of <seed><angle>, one triple per seed; a seed without one is unrotated), driven by
+// the case's thermal
+// lattice (relaxation_anisotropy_thermal, relaxation_thermal) or solute lattice
+// (relaxation_solute, for a case with <solver><solute> and no thermal lattice or
+// <solver><coupling> solute).  Each relaxation call advances every lattice by one step; the
+// driving field is the case's, and a call naming the other one throws.  The finite-volume
+// evolution_anisotropy and the Allen-Cahn relaxation of this class are not provided (they
+// throw): the Allen-Cahn grains are mkFluidsUser's.
+class mkMultiphaseField : public mkFluidsMRT {
+public:
+    mkMultiphaseField() = default;
+    explicit mkMultiphaseField(const mkSpace& space) : mkFluidsLBM(space), mkFluidsMRT(space) {}
+    ~mkMultiphaseField() override;
+    void initialize(const std::string& xmlfname) override;
+    void initialize_geometry() override;
+    void propagate() override { pending_ |= MKG_PHASE; }
+    void relaxation() override { advance(MKG_PHASE); }
+    void relaxation_anisotropy_thermal() { driven(1, "relaxation_anisotropy_thermal"); }
+    void relaxation_thermal() { driven(1, "relaxation_thermal"); }
+    void relaxation_solute() { driven(2, "relaxation_solute"); }
+    void relaxation_AllenCahn();
+    void evolution_anisotropy();
+    void evolution(const mkNode* /*nptr*/) { evolution_anisotropy(); }
+    // Anisotropy of order parameter id - 1 (MaterKit/inc/mkMultiphaseField.h:31-34): its
+    // rotation, the case's strengths, the form the name gives.  False for a zero normal.
+    bool anisotropy(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                    const int& x, const int& y, const int& z) {
+        return grain_query(MKG_ANISO_CUBIC, id, N_v, A_s, normal);
+    }
+    bool anisotropy_facet(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                          const int& x, const int& y, const int& z) {
+        return grain_query(MKG_ANISO_FACET, id, N_v, A_s, normal);
+    }
+    bool anisotropy_cbc(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                        const int& x, const int& y, const int& z) {
+        return grain_query(MKG_ANISO_CBC, id, N_v, A_s, normal);
+    }
+    bool anisotropy_hcp(const int& id, mkVector3d& N_v, mkReal& A_s, mkVector3d& normal,
+                        const int& x, const int& y, const int& z) {
+        return grain_query(MKG_ANISO_HCP, id, N_v, A_s, normal);
+    }
+    void write_vtkfile(int loops) override;
+    mkg_mpf_handle mpf() const { return m_; }
+    const mkg_mpf_params& mpf_params() const { return mp_; }
+    // phi_j (n x N) and the combined phi (N) at the last moment pass
+    std::vector<double> order_parameters() const;
+
+protected:
+    void advance(int mask) override;
+    int seed_number = 0;   // seeds of the case file
+    int phase_number = 0;  // order parameters (one per seed)
+    std::vector<double> vec_angle, vec_center;
+
+private:
+    void driven(int coupling, const char* name);
+    bool grain_query(int kind, int id, mkVector3d& N_v, mkReal& A_s, const mkVector3d& normal);
     mkg_mpf_handle m_ = nullptr;
     mkg_mpf_params mp_{};
 };

@@ -702,6 +702,88 @@ static void nee_faces(const orc_params* p, orc_state* s, int q, double* post) {
     }
 }
 
+/* Solute collision (mkSolutalField::propagate + relaxation) of every non-wall cell into post
+ * (7 x N): dc/dt + div(c u) = div(D q grad U), U = c/A(phi), written as
+ * dc/dt + div(c (u + V)) = div(D_c grad c) with D_c = D q / A and V = D q grad A / A^2.
+ * phi_f / phi_prev: the phase field that drives the solute (NULL: liquid everywhere, no phase
+ * terms); cpop: the 7 x N populations; Cm, Um: the cell values of c and U; uadv: 3 x N
+ * advection velocity or NULL; per: the periodic axes of the phase-field gradient. */
+static void solute_collide(const orc_params* p, const int8_t* flag, const double* phi_f,
+                           const double* phi_prev, const double* cpop, const double* Cm,
+                           const double* Um, const double* uadv, const int* per,
+                           double* post) {
+    long N = NCELL(p);
+    lat_t L = make_lat(7);
+    lat_t L19 = make_lat(19);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int X = 0; X < p->nx; X++)
+        for (int Y = 0; Y < p->ny; Y++)
+            for (int Z = 0; Z < p->nz; Z++) {
+                long x = lin(p, X, Y, Z);
+                if (flag[x] == 3) continue;
+                double phi = phi_f ? phi_f[x] : -1.0;
+                double gphi[3] = {0, 0, 0};
+                if (phi_f) {
+                    for (int i = 1; i < 19; i++) {
+                        int XX = clampwrap(X + E27[i][0], p->nx, per[0]);
+                        int YY = clampwrap(Y + E27[i][1], p->ny, per[1]);
+                        int ZZ = clampwrap(Z + E27[i][2], p->nz, per[2]);
+                        double v = 3.0 * L19.w[i] * phi_f[lin(p, XX, YY, ZZ)];
+                        for (int a = 0; a < 3; a++) gphi[a] += E27[i][a] * v;
+                    }
+                }
+                double q = 0.5 * (1.0 - phi);
+                if (p->D_l > 0) q += (p->D_s / p->D_l) * 0.5 * (1.0 + phi);
+                if (q < p->q_min) q = p->q_min;
+                double A = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
+                double Dc = p->D_l * q / A;
+                double om = 1.0 / (0.5 + Dc / L.cs2);
+                double vfac = -0.5 * (1.0 - p->k_part) * p->D_l * q / (A * A);
+                double dphi = phi_f ? (phi_f[x] - phi_prev[x]) : 0.0;
+                double C = Cm[x];
+                const double zero3[3] = {0, 0, 0};
+                const double* u = uadv ? uadv + 3 * x : zero3;
+                /* equilibrium flux c (u + V) + j_at, the anti-trapping current (Karma 2001)
+                 * j_at = -a_t W (1 + (1-k) U) dphi/dt n, n = grad phi / |grad phi|, which
+                 * moves solute from the growing solid to the liquid (mkSolutalField::ATcurrent,
+                 * MaterKit/inc/mkSolutalField.h) */
+                double j[3];
+                for (int a = 0; a < 3; a++) j[a] = C * (u[a] + vfac * gphi[a]);
+                double gg = gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2];
+                double atw = p->at_coef * sqrt(p->pf_wdx2 > 0 ? p->pf_wdx2 : 0.0);
+                if (phi_f && atw != 0.0 && gg > 1e-20) {
+                    double jat = -atw * (1.0 + (1.0 - p->k_part) * Um[x]) * dphi / sqrt(gg);
+                    for (int a = 0; a < 3; a++) j[a] += jat * gphi[a];
+                }
+                double cs2_adv_C = L.cs2, latC = p->latent_C;
+#if ORC_MUTANT == 6
+                cs2_adv_C = 1.0 / 3.0;
+#elif ORC_MUTANT == 7
+                latC = -latC;
+#endif
+                for (int i = 0; i < 7; i++) {
+                    double cj = E27[i][0] * j[0] + E27[i][1] * j[1] + E27[i][2] * j[2];
+                    double ceq = L.w[i] * (C + cj / cs2_adv_C);
+                    double ci = cpop[i * N + x];
+                    post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * latC * dphi;
+                }
+            }
+}
+
+/* Streaming of the solute lattice; its Dirichlet faces fix U: c_w = A(phi) U_w. */
+static void solute_stream(const orc_params* p, const orc_state* s, const double* phi_f,
+                          const double* post, double* c, double* fresh) {
+    long N = NCELL(p);
+    double* Ac = (double*)malloc((size_t)N * sizeof(double));
+    for (long x = 0; x < N; x++) {
+        double ph = phi_f ? phi_f[x] : -1.0;
+        Ac[x] = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * ph);
+    }
+    stream(p, s, 7, p->bc_c, 1, p->C_face, NULL, Ac, NULL, post, c, fresh);
+    free(Ac);
+    memcpy(c, fresh, (size_t)7 * N * sizeof(double));
+}
+
 /* ------------------------------------------------------------------ one step */
 static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
     long N = NCELL(p);
@@ -859,73 +941,11 @@ static void one_step(const orc_params* p, orc_state* s, int mask, int op) {
         memcpy(s->h, fresh, (size_t)7 * N * sizeof(double));
     }
 
-    /* solute: dc/dt + div(c u) = div(D q grad U), U = c/A(phi), written as
-     * dc/dt + div(c (u + V)) = div(D_c grad c) with D_c = D q / A and V = D q grad A / A^2 */
     if (adv_c) {
-        lat_t L = make_lat(7);
-        lat_t L19 = make_lat(19);
         int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int X = 0; X < p->nx; X++)
-            for (int Y = 0; Y < p->ny; Y++)
-                for (int Z = 0; Z < p->nz; Z++) {
-            long x = lin(p, X, Y, Z);
-            if (s->flag[x] == 3) continue;
-            double phi = p->phase_q ? s->phi[x] : -1.0;
-            double gphi[3] = {0, 0, 0};
-            if (p->phase_q) {
-                for (int i = 1; i < 19; i++) {
-                    int XX = clampwrap(X + E27[i][0], p->nx, per[0]);
-                    int YY = clampwrap(Y + E27[i][1], p->ny, per[1]);
-                    int ZZ = clampwrap(Z + E27[i][2], p->nz, per[2]);
-                    double v = 3.0 * L19.w[i] * s->phi[lin(p, XX, YY, ZZ)];
-                    for (int a = 0; a < 3; a++) gphi[a] += E27[i][a] * v;
-                }
-            }
-            double q = 0.5 * (1.0 - phi);
-            if (p->D_l > 0) q += (p->D_s / p->D_l) * 0.5 * (1.0 + phi);
-            if (q < p->q_min) q = p->q_min;
-            double A = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
-            double Dc = p->D_l * q / A;
-            double om = 1.0 / (0.5 + Dc / L.cs2);
-            double vfac = -0.5 * (1.0 - p->k_part) * p->D_l * q / (A * A);
-            double dphi = p->phase_q ? (s->phi[x] - s->phi_prev[x]) : 0.0;
-            double C = m.C[x];
-            const double* u = uadv + 3 * x;
-            /* equilibrium flux c (u + V) + j_at, the anti-trapping current (Karma 2001)
-             * j_at = -a_t W (1 + (1-k) U) dphi/dt n, n = grad phi / |grad phi|, which moves
-             * solute from the growing solid to the liquid (mkSolutalField::ATcurrent,
-             * MaterKit/inc/mkSolutalField.h) */
-            double j[3];
-            for (int a = 0; a < 3; a++) j[a] = C * (u[a] + vfac * gphi[a]);
-            double gg = gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2];
-            double atw = p->at_coef * sqrt(p->pf_wdx2 > 0 ? p->pf_wdx2 : 0.0);
-            if (p->phase_q && atw != 0.0 && gg > 1e-20) {
-                double jat = -atw * (1.0 + (1.0 - p->k_part) * m.U[x]) * dphi / sqrt(gg);
-                for (int a = 0; a < 3; a++) j[a] += jat * gphi[a];
-            }
-            double cs2_adv_C = L.cs2, latC = p->latent_C;
-#if ORC_MUTANT == 6
-            cs2_adv_C = 1.0 / 3.0;
-#elif ORC_MUTANT == 7
-            latC = -latC;
-#endif
-            for (int i = 0; i < 7; i++) {
-                double cj = E27[i][0] * j[0] + E27[i][1] * j[1] + E27[i][2] * j[2];
-                double ceq = L.w[i] * (C + cj / cs2_adv_C);
-                double ci = s->c[i * N + x];
-                post[i * N + x] = ci - om * (ci - ceq) + L.w[i] * latC * dphi;
-            }
-        }
-        /* solute Dirichlet faces fix U: c_w = A(phi) U_w */
-        double* Ac = (double*)malloc((size_t)N * sizeof(double));
-        for (long x = 0; x < N; x++) {
-            double ph = p->phase_q ? s->phi[x] : -1.0;
-            Ac[x] = 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * ph);
-        }
-        stream(p, s, 7, p->bc_c, 1, p->C_face, NULL, Ac, NULL, post, s->c, fresh);
-        free(Ac);
-        memcpy(s->c, fresh, (size_t)7 * N * sizeof(double));
+        const double* ph = p->phase_q ? s->phi : NULL;
+        solute_collide(p, s->flag, ph, s->phi_prev, s->c, m.C, m.U, uadv, per, post);
+        solute_stream(p, s, ph, post, s->c, fresh);
     }
 
     free(uadv);
@@ -1045,7 +1065,19 @@ void orc_init_seeds(const orc_params* p, orc_state* s, int n, const double* cent
  * combined phi at every step (:171-172), so it feeds nothing.
  * Optional D3Q7 thermal lattice (run_multiPhaseField's mkThermalField calls, :566-567): BGK
  * diffusion with alpha, no advection, latent source w_i L dphi of the combined phi; T enters
- * Q_j. */
+ * Q_j.
+ *
+ * Karma-Rappel grains (model 1; mkMultiphaseField::relaxation_anisotropy_thermal,
+ * relaxation_solute, relaxation_thermal with anisotropy / _facet / _cbc / _hcp per grain,
+ * MaterKit/inc/mkMultiphaseField.h:31-45, whose bodies are not public): each order parameter
+ * is the anisotropic phase field of the single-grain solver (orc_phase_source) with the
+ * grain's own form, strength and rotation, driven by u = T (coupling 1) or U (coupling 2), and
+ * the pairwise interaction joins the source on the same time scale tau0 a_s^2:
+ *        g* = g - omega_g (g - g_eq) + w_i [S_j + dt P_j / a_j^2]
+ * without the Allen-Cahn counter-term.  With one grain this is the single-grain phase field
+ * exactly.  Optional D3Q7 solute lattice (solute 1): the single-grain solute equation
+ * (solute_collide) driven by the combined phi clamped to [-1, 1] (grains that overlap take it
+ * above 1, where A(phi) = ((1 + k) - (1 - k) phi) / 2 would vanish), U = c / A(phi). */
 static orc_params mpf_geom(const orc_mpf_params* m, const int* bc) {
     orc_params p;
     memset(&p, 0, sizeof(p));
@@ -1057,10 +1089,10 @@ static orc_params mpf_geom(const orc_mpf_params* m, const int* bc) {
     return p;
 }
 
-void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, double* g,
-                  double* phi, double* phi_prev, double* h) {
+void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, const double* C,
+                  double* g, double* phi, double* phi_prev, double* h, double* c) {
     long N = (long)m->nx * m->ny * m->nz;
-    lat_t L = make_lat(19);
+    lat_t L7 = make_lat(7);
     for (long x = 0; x < N; x++) {
         double comb = m->n - 1.0;
         for (int j = 0; j < m->n; j++) {
@@ -1074,23 +1106,63 @@ void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, 
         }
         phi[(long)m->n * N + x] = phi_prev[(long)m->n * N + x] = comb;
         if (h) {
-            lat_t L7 = make_lat(7);
             double t = T ? T[x] : m->T_ref;
             for (int i = 0; i < 7; i++) h[i * N + x] = L7.w[i] * t;
         }
+        if (c) {
+            double cc = C ? C[x] : 0.0;
+            for (int i = 0; i < 7; i++) c[i * N + x] = L7.w[i] * cc;
+        }
     }
-    (void)L;
+}
+
+/* The single-grain phase-field parameters of grain j (form, strength, rotation). */
+static orc_params mpf_grain(const orc_mpf_params* m, int j) {
+    orc_params p = mpf_geom(m, m->bc);
+    p.phase_q = 19;
+    p.aniso_kind = m->aniso_kind[j];
+    p.eps4 = m->eps4[j];
+    p.aniso[0] = m->aniso[j][0];
+    p.aniso[1] = m->aniso[j][1];
+    p.ngrains = 1;
+    memcpy(p.rot[0], m->rot[j], sizeof(p.rot[0]));
+    p.lambda = m->lambda;
+    p.pf_dt = m->dt;
+    p.pf_wdx2 = m->wdx * m->wdx;
+    return p;
+}
+
+/* The single-grain solute parameters of the solute lattice. */
+static orc_params mpf_solute(const orc_mpf_params* m) {
+    orc_params p = mpf_geom(m, m->bc);
+    p.phase_q = 19;
+    p.solute = 1;
+    for (int f = 0; f < 6; f++) {
+        p.bc_c[f] = m->bc_c[f];
+        p.C_face[f] = m->C_face[f];
+    }
+    p.D_l = m->D_l;
+    p.D_s = m->D_s;
+    p.k_part = m->k_part;
+    p.q_min = m->q_min;
+    p.latent_C = m->latent_C;
+    p.at_coef = m->at_coef;
+    p.pf_wdx2 = m->wdx * m->wdx;
+    return p;
 }
 
 int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_prev, double* h,
-                 int8_t* flag, int nsteps) {
+                 double* c, int8_t* flag, int nsteps) {
     if (m->n < 1 || m->n > ORC_MPF_MAX) return -1;
+    if (m->model != 0 && m->model != 1) return -1;
+    if ((m->thermal != 0) != (h != NULL) || (m->solute != 0) != (c != NULL)) return -1;
     long N = (long)m->nx * m->ny * m->nz;
     int n = m->n;
     lat_t L = make_lat(19);
     lat_t L7 = make_lat(7);
     orc_params pg = mpf_geom(m, m->bc);
     orc_params ph7 = mpf_geom(m, m->bc_h);
+    orc_params ps = mpf_solute(m);
     int8_t* fl = flag;
     int8_t* own = NULL;
     if (!fl) {
@@ -1119,7 +1191,7 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
             }
             phi[(long)n * N + x] = comb;
         }
-        /* thermal moments at this level (T enters the growth term) */
+        /* thermal and solute moments at this level (T or U drives the grains) */
         double* Tc = NULL;
         if (h) {
             Tc = malloc((size_t)N * sizeof(double));
@@ -1129,15 +1201,65 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
                 Tc[x] = t;
             }
         }
+        /* the solute sees the combined phi clamped to [-1, 1] (overlapping grains exceed 1) */
+        double *Cc = NULL, *Uc = NULL, *ps_now = NULL, *ps_prev = NULL;
+        if (c) {
+            Cc = malloc((size_t)N * sizeof(double));
+            Uc = malloc((size_t)N * sizeof(double));
+            ps_now = malloc((size_t)N * sizeof(double));
+            ps_prev = malloc((size_t)N * sizeof(double));
+            for (long x = 0; x < N; x++) {
+                ps_now[x] = fmin(1.0, fmax(-1.0, phi[(long)n * N + x]));
+                ps_prev[x] = fmin(1.0, fmax(-1.0, phi_prev[(long)n * N + x]));
+                double cs = 0;
+                for (int i = 0; i < 7; i++) cs += c[i * N + x];
+                double A = 0.5 * ((1.0 + m->k_part) - (1.0 - m->k_part) * ps_now[x]);
+                Cc[x] = cs;
+                Uc[x] = cs / A;
+            }
+        }
         /* 2. collision and 3. streaming, one order parameter at a time */
         for (int j = 0; j < n; j++) {
             const double* u = m->u[j];
+            orc_params pj = mpf_grain(m, j);
 #pragma omp parallel for collapse(2) schedule(static)
             for (int X = 0; X < m->nx; X++)
                 for (int Y = 0; Y < m->ny; Y++)
                     for (int Z = 0; Z < m->nz; Z++) {
                         long x = lin(&pg, X, Y, Z);
                         if (fl[x] == 3) continue;
+                        double pj_ = phi[j * N + x];
+                        double P = 0;
+                        for (int k = 0; k < n; k++)
+                            if (k != j) {
+                                double hk = 0.5 * phi[k * N + x] + 0.5;
+                                P += hk * hk;
+                            }
+                        P *= -2.0 * m->kappa[j] * (0.5 * pj_ + 0.5);
+                        double feq[27];
+                        orc_equilibrium(19, pj_, u, feq);
+                        if (m->model == 1) {
+                            double phi27[27];
+                            for (int dx = -1; dx <= 1; dx++)
+                                for (int dy = -1; dy <= 1; dy++)
+                                    for (int dz = -1; dz <= 1; dz++) {
+                                        int XX = clampwrap(X + dx, m->nx, per[0]);
+                                        int YY = clampwrap(Y + dy, m->ny, per[1]);
+                                        int ZZ = clampwrap(Z + dz, m->nz, per[2]);
+                                        P27(dx, dy, dz) = phi[j * N + lin(&pg, XX, YY, ZZ)];
+                                    }
+                            double upf = 0;
+                            if (m->coupling == 1 && Tc) upf = Tc[x];
+                            if (m->coupling == 2 && Uc) upf = Uc[x];
+                            double S, a;
+                            orc_phase_source(&pj, phi27, upf, 1, &S, &a);
+                            double src = S + m->dt * P / (a * a);
+                            for (int i = 0; i < 19; i++) {
+                                double gi = g[((long)j * 19 + i) * N + x];
+                                post[i * N + x] = gi - omg * (gi - feq[i]) + L.w[i] * src;
+                            }
+                            continue;
+                        }
                         double s3[3] = {0, 0, 0};
                         for (int i = 1; i < 19; i++) {
                             int XX = clampwrap(X + E27[i][0], m->nx, per[0]);
@@ -1148,19 +1270,9 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
                         }
                         double sn = sqrt(s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]) + m->eps;
                         double nv[3] = {s3[0] / sn, s3[1] / sn, s3[2] / sn};
-                        double pj = phi[j * N + x];
-                        double P = 0;
-                        for (int k = 0; k < n; k++)
-                            if (k != j) {
-                                double hk = 0.5 * phi[k * N + x] + 0.5;
-                                P += hk * hk;
-                            }
-                        P *= -2.0 * m->kappa[j] * (0.5 * pj + 0.5);
-                        double d = 1.0 - pj * pj;
+                        double d = 1.0 - pj_ * pj_;
                         double T = Tc ? Tc[x] : m->T_ref;
-                        double Q = m->growth * (pj - m->lambda * T * d) * d;
-                        double feq[27];
-                        orc_equilibrium(19, pj, u, feq);
+                        double Q = m->growth * (pj_ - m->lambda * T * d) * d;
                         for (int i = 0; i < 19; i++) {
                             double cn = E27[i][0] * nv[0] + E27[i][1] * nv[1] + E27[i][2] * nv[2];
                             double Zc = (1.0 - 0.5 * omg) * L.w[i] * cn * d * lam;
@@ -1188,6 +1300,15 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
             stream(&ph7, &st, 7, m->bc_h, 1, m->T_face, NULL, NULL, NULL, post, h, fresh);
             memcpy(h, fresh, (size_t)7 * N * sizeof(double));
             free(Tc);
+        }
+        /* solute, driven by the clamped combined phi */
+        if (c) {
+            solute_collide(&ps, fl, ps_now, ps_prev, c, Cc, Uc, NULL, per, post);
+            solute_stream(&ps, &st, ps_now, post, c, fresh);
+            free(Cc);
+            free(Uc);
+            free(ps_now);
+            free(ps_prev);
         }
     }
     free(post);

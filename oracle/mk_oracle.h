@@ -99,9 +99,10 @@ double orc_aniso(const orc_params* p, const double* n, double* D);
  * outside that scope. */
 int orc_aos_step(const orc_params* p, orc_state* s, int op, int nsteps, double* seconds);
 
-/* Multi-phase field (mkFluidsUser's phase_number order parameters): n conservative
- * Allen-Cahn order parameters on D3Q19 with pairwise interaction, optional D3Q7 thermal
- * lattice.  Same layout as the product's mkg_mpf_params. */
+/* Multi-phase field (mkFluidsUser's phase_number order parameters): n order parameters on
+ * D3Q19 with pairwise interaction -- conservative Allen-Cahn (the template) or Karma-Rappel
+ * grains, each with its own anisotropy and orientation (mkMultiphaseField) -- and optional
+ * D3Q7 thermal and solute lattices.  Same layout as the product's mkg_mpf_params. */
 #define ORC_MPF_MAX 4
 typedef struct orc_mpf_params {
     int32_t nx, ny, nz, n;
@@ -114,14 +115,25 @@ typedef struct orc_mpf_params {
     double alpha, latent_T, T_ref;  /* thermal lattice */
     double T_face[6];
     double phi_face[6];             /* Dirichlet value of the phase lattices */
+    int32_t model;                  /* 0 template (Allen-Cahn), 1 Karma-Rappel grains */
+    int32_t coupling;               /* field driving the grains: 0 none, 1 T, 2 U */
+    int32_t aniso_kind[ORC_MPF_MAX];
+    double eps4[ORC_MPF_MAX];
+    double aniso[ORC_MPF_MAX][2];
+    double rot[ORC_MPF_MAX][9];     /* lab <- crystal rotation of each grain, row-major */
+    int32_t solute;                 /* D3Q7 solute lattice driven by the combined phi */
+    int32_t bc_c[6];
+    double D_l, D_s, k_part, q_min, latent_C, at_coef;
+    double C_face[6];
 } orc_mpf_params;
 
 /* g: n x 19 x N populations; phi, phi_prev: (n + 1) x N (phi_j, then the combined phi);
- * h: 7 x N or NULL; flag: N or NULL (no walls). */
-void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, double* g,
-                  double* phi, double* phi_prev, double* h);
+ * h, c: 7 x N or NULL; T, C: the initial temperature and solute c (NULL: T_ref, 0);
+ * flag: N or NULL (no walls). */
+void orc_mpf_init(const orc_mpf_params* m, const double* phij, const double* T, const double* C,
+                  double* g, double* phi, double* phi_prev, double* h, double* c);
 int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_prev, double* h,
-                 int8_t* flag, int nsteps);
+                 double* c, int8_t* flag, int nsteps);
 
 /* Explicit finite-volume phase field (evolution_anisotropy) and solute with anti-trapping
  * current (evolution_ATC1/2): phi and c (= A(phi) U) are N doubles each, advanced in place;

@@ -108,8 +108,8 @@ def _bind(L: ct.CDLL) -> ct.CDLL:
     L.orc_mp_macro.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
     L.orc_fvm_step.restype = ct.c_int
-    L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp]
-    L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, ct.c_int]
+    L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, ct.c_int]
     L.orc_mpf_step.restype = ct.c_int
     L.orc_flow_adjoint.argtypes = [vp, vp, vp, ct.c_int, vp, vp, vp]
     L.orc_flow_adjoint.restype = ct.c_int
@@ -295,32 +295,37 @@ def cell_uniform(seed: int, cells) -> np.ndarray:
 @dataclass
 class MpfState:
     """Oracle state of the multi-phase field (params is an ``MpfParams``-layout struct):
-    g n x 19 x N, phi / phi_prev (n + 1) x N, h 7 x N (thermal), flag N."""
+    g n x 19 x N, phi / phi_prev (n + 1) x N, h 7 x N (thermal), c 7 x N (solute), flag N."""
 
     params: object
     g: np.ndarray = None
     phi: np.ndarray = None
     phi_prev: np.ndarray = None
     h: np.ndarray | None = None
+    c: np.ndarray | None = None
     flag: np.ndarray = None
 
     @classmethod
-    def start(cls, params, phij, T=None) -> "MpfState":
+    def start(cls, params, phij, T=None, C=None) -> "MpfState":
         N = int(params.nx) * int(params.ny) * int(params.nz)
         n = int(params.n)
         s = cls(params=params, g=np.zeros(n * 19 * N), phi=np.zeros((n + 1) * N),
                 phi_prev=np.zeros((n + 1) * N), flag=np.zeros(N, dtype=np.int8))
         if params.thermal:
             s.h = np.zeros(7 * N)
+        if params.solute:
+            s.c = np.zeros(7 * N)
         ph = np.ascontiguousarray(phij, dtype=np.float64).reshape(-1)
         t = None if T is None else np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
-        lib().orc_mpf_init(ct.byref(params), _p(ph), _p(t), _p(s.g), _p(s.phi),
-                           _p(s.phi_prev), _p(s.h))
+        cc = None if C is None else np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
+        lib().orc_mpf_init(ct.byref(params), _p(ph), _p(t), _p(cc), _p(s.g), _p(s.phi),
+                           _p(s.phi_prev), _p(s.h), _p(s.c))
         return s
 
     def step(self, nsteps: int = 1) -> None:
         rc = lib().orc_mpf_step(ct.byref(self.params), _p(self.g), _p(self.phi),
-                                _p(self.phi_prev), _p(self.h), _p(self.flag), int(nsteps))
+                                _p(self.phi_prev), _p(self.h), _p(self.c), _p(self.flag),
+                                int(nsteps))
         if rc != 0:
             raise RuntimeError("oracle rejected the multi-phase parameter block")
 

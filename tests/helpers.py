@@ -291,3 +291,89 @@ def perturbed(p, k, h):
     else:
         setattr(pp, FLOW_ADJ_PARAMS[k], getattr(p, FLOW_ADJ_PARAMS[k]) + h)
     return pp
+
+
+# ---- Karma-Rappel grains of the multi-phase field
+
+def quat_rot(q) -> np.ndarray:
+    """Row-major rotation matrix of the unit quaternion q = (w, x, y, z)."""
+    w, x, y, z = np.asarray(q, dtype=float) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+GRAIN_FORMS = {  # aniso_kind, eps4, the two parameters of the other forms
+    "cubic": (mkg.ANISO_CUBIC, 0.05, (0.0, 0.0)),
+    "cbc": (mkg.ANISO_CBC, 0.0, (0.04, -0.01)),
+    "hcp": (mkg.ANISO_HCP, 0.0, (0.03, -0.02)),
+    "facet": (mkg.ANISO_FACET, 0.0, (0.05, 0.2)),
+}
+
+
+def grain_params(dims, n, forms, rots, coupling=1, prec=mkg.PREC_FP64, kappa=1.0, **kw):
+    """Multi-phase parameters of n Karma-Rappel grains (forms: names of GRAIN_FORMS, rots: 3x3
+    rotations), periodic in x and y, bounce-back phase faces and Dirichlet T / U faces in z;
+    coupling 1: thermal lattice (latent heat), 2: solute lattice (anti-trapping)."""
+    m = mkg.default_mpf_params(nx=dims[0], ny=dims[1], nz=dims[2], n=n, precision=prec,
+                               model=mkg.MPF_KARMA_RAPPEL, coupling=coupling, dt=0.008, wdx=2.5,
+                               lambda_=3.19, alpha=0.1, latent_T=0.5, T_ref=-0.55)
+    m.set(bc=[0, 0, 0, 0, 1, 1])
+    for j in range(n):
+        m.u[j][0] = 0.0
+        m.kappa[j] = kappa
+        kind, e4, ap = GRAIN_FORMS[forms[j]]
+        m.aniso_kind[j] = kind
+        m.eps4[j] = e4
+        m.aniso[j][0], m.aniso[j][1] = ap
+        R = np.asarray(rots[j], dtype=float).reshape(-1)
+        for k in range(9):
+            m.rot[j][k] = R[k]
+    if coupling == 1:
+        m.thermal = 1
+        m.set(bc_h=[0, 0, 0, 0, 2, 2], T_face=[0, 0, 0, 0, -0.5, -0.6])
+    if coupling == 2:
+        m.solute = 1
+        m.set(bc_c=[0, 0, 0, 0, 2, 2], C_face=[0, 0, 0, 0, -0.3, -0.35])
+        m.D_l, m.D_s, m.k_part, m.q_min = 1.0, 1e-4, 0.15, 1e-3
+        m.latent_C, m.at_coef = 0.4, 1.0 / (2.0 * math.sqrt(2.0))
+    m.set(**kw)
+    return m
+
+
+def grain_fields(m, centers, radii, seed=4):
+    """Seeds of the grains, a noisy T (coupling 1) or c = A(phi) U (coupling 2) around the
+    face values."""
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import cases
+    dims = (int(m.nx), int(m.ny), int(m.nz))
+    N = int(np.prod(dims))
+    phi = cases.mpf_seed_fields(dims, centers, radii, float(m.wdx))
+    rng = np.random.default_rng(seed)
+    T = C = None
+    if m.thermal:
+        T = -0.55 + 0.01 * rng.standard_normal(N)
+    if m.solute:
+        comb = np.clip((m.n - 1) + phi.sum(0), -1.0, 1.0)
+        A = 0.5 * ((1 + m.k_part) - (1 - m.k_part) * comb)
+        C = A * (-0.3 + 0.01 * rng.standard_normal(N))
+    return phi, T, C
+
+
+def rot_z(deg):
+    t = np.radians(deg)
+    return np.array([[np.cos(t), -np.sin(t), 0.0], [np.sin(t), np.cos(t), 0.0], [0.0, 0.0, 1.0]])
+
+
+def six_fold_orientation(ph, center):
+    """Orientation (degrees, in [0, 60)) of the six-fold growth of a 2-D grain: the phase of
+    sum over its solid cells of exp(6 i theta) about the seed, divided by 6."""
+    nx, ny = ph.shape
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    th = np.arctan2(Y - center[1], X - center[0])
+    M = (np.exp(6j * th) * (ph > 0)).sum()
+    return float(np.degrees(np.angle(M)) / 6.0) % 60.0
+
+
+def angle_gap(a, b):
+    d = abs(a - b) % 60.0
+    return min(d, 60.0 - d)

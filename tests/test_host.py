@@ -30,7 +30,7 @@ def test_library_exports_every_declared_entry_point():
     missing = decl - exported
     assert not missing, f"declared but not exported: {sorted(missing)}"
     L = mkg.lib()
-    assert L.mkg_abi_version() == 4
+    assert L.mkg_abi_version() == 5
     assert L.mkg_params_size() == 5312
 
 
@@ -224,7 +224,7 @@ DRIVER_CALLS = r"""
 // Call forms of the reference's drivers (MaterKit/MaterKit.cpp run_* loops), compiled and
 // linked only: nothing here runs without a device.
 static void loops(mkFluidsMRT& flow, mkFluidsUser& user, mkFluidsMultiphase& two,
-                  const std::string& xml, int loops) {
+                  mkMultiphaseField& grains, const std::string& xml, int loops) {
     flow.mkFluidsMRT::initialize(xml);
     flow.mkFluidsMRT::initialize_geometry();
     flow.initialize_geometry_Poiseuille();
@@ -276,6 +276,25 @@ static void loops(mkFluidsMRT& flow, mkFluidsUser& user, mkFluidsMultiphase& two
     two.propagate();
     two.relaxation();
     two.write_vtkfile(loops);
+    grains.initialize(xml);
+    grains.initialize_geometry();
+    grains.propagate();
+    grains.relaxation();
+    grains.relaxation_anisotropy_thermal();
+    grains.relaxation_solute();
+    grains.relaxation_thermal();
+    grains.relaxation_AllenCahn();
+    grains.evolution_anisotropy();
+    grains.evolution(flow.node);
+    mkVector3d N_v, normal(1.0, 0.5, 0.25);
+    mkReal A_s = 1.0;
+    int x = 0, y = 0, z = 0;
+    (void)grains.anisotropy(1, N_v, A_s, normal, x, y, z);
+    (void)grains.anisotropy_facet(1, N_v, A_s, normal, x, y, z);
+    (void)grains.anisotropy_cbc(1, N_v, A_s, normal, x, y, z);
+    (void)grains.anisotropy_hcp(1, N_v, A_s, normal, x, y, z);
+    (void)flow.anisotropy(1, N_v, A_s, normal, x, y, z);
+    grains.write_vtkfile(loops);
 }
 
 int main(int argc, char** argv) {
@@ -284,11 +303,37 @@ int main(int argc, char** argv) {
         mkFluidsMRT flow(space);
         mkFluidsUser user(space);
         mkFluidsMultiphase two(space);
-        loops(flow, user, two, argv[1], argc);
+        mkMultiphaseField grains(space);
+        loops(flow, user, two, grains, argv[1], argc);
     }
     return 0;
 }
 """
+
+
+@pytest.mark.parametrize("form", ["cubic", "cbc", "hcp", "facet"])
+def test_anisotropy_query_matches_oracle_face_flux(form):
+    """mkg_aniso_eval (the anisotropy / anisotropy_facet / _cbc / _hcp query) gives a_s and
+    n_v with J = |g| (a_s^2 n + n_v) equal to the oracle's face flux, for rotated crystals."""
+    kinds = {"cubic": (mkg.ANISO_CUBIC, 0.05, (0.0, 0.0)), "cbc": (mkg.ANISO_CBC, 0.0, (0.04, -0.01)),
+             "hcp": (mkg.ANISO_HCP, 0.0, (0.03, -0.02)), "facet": (mkg.ANISO_FACET, 0.0, (0.05, 0.2))}
+    kind, e4, par = kinds[form]
+    p = mkg.default_params(eps4=e4, aniso_kind=kind)
+    p.aniso[0], p.aniso[1] = par
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        q = rng.standard_normal(4)
+        R = mkg.quat_to_rot(q / np.linalg.norm(q))
+        g = rng.standard_normal(3)
+        a, nv = mkg.aniso_eval(kind, g, e4, par, R)
+        n = g / np.linalg.norm(g)
+        J = np.linalg.norm(g) * (a * a * n + nv)
+        ref = np.array([O.face_flux(p, g, ax, R) for ax in range(3)])
+        np.testing.assert_allclose(J, ref, rtol=1e-13, atol=1e-14)
+        a_un, _ = mkg.aniso_eval(kind, R.T @ g, e4, par)  # the crystal-frame normal, no rotation
+        assert abs(a - a_un) < 1e-14
+    a0, nv0 = mkg.aniso_eval(kind, np.zeros(3), e4, par)
+    assert a0 == 1.0 and not nv0.any()
 
 
 def test_reference_driver_call_forms_compile_against_the_shim(tmp_path):
