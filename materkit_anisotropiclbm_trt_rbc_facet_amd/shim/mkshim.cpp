@@ -380,6 +380,40 @@ void mkFluidsLBM::write_vtkfile(int loops) {
     }
 }
 
+// ------------------------------------------------------------------ driver utilities
+mkCPUTimer* timer = nullptr;
+namespace mkt {
+std::string paraXmlFileName;
+}
+const std::string workspace = "./";
+
+mkCPUTimer::mkCPUTimer(const std::string label) : label_(label), t0_(std::chrono::steady_clock::now()) {}
+
+mkCPUTimer::~mkCPUTimer() { std::printf("%s: %.3f s\n", label_.c_str(), GetTime(0)); }
+
+double mkCPUTimer::GetTime(int /*flag*/) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+double mkCPUTimer::GetSystemTime() {
+    return std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch())
+        .count();
+}
+
+extern "C" int make_datafolder() {
+    std::error_code ec;
+    std::filesystem::create_directories(workspace + "vtk", ec);
+    return ec ? -1 : 0;
+}
+
+extern "C" int make_numthreads() { return 1; }
+
+void mkFluidsMultiphase::update_geometry(mkFluidsMRT& /*dendrite*/) {
+    throw mkt::mkLogicException(
+        "mkFluidsMultiphase::update_geometry (two-phase flow around a growing dendrite) is not "
+        "provided");
+}
+
 // ------------------------------------------------------------------ anisotropic multi-grain field
 mkMultiphaseField::~mkMultiphaseField() {
     if (m_) mkg_mpf_destroy(m_);

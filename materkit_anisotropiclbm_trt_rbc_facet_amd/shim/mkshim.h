@@ -15,6 +15,7 @@
 // mkt::mkLogicException (everything else), as the reference's exception classes
 // (MaterKit/inc/mkRunTimeDiagnostics.h:71-117).
 #pragma once
+#include <chrono>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -404,6 +405,9 @@ public:
     void propagate() override { pending_ |= MKG_FLOW; }
     void relaxation() override { advance(MKG_FLOW); }
     void relaxation_phi() {}  // the Cahn-Hilliard lattice steps with the flow
+    // The solidifying two-phase coupling of run_flow_multiphase_dendrite
+    // (MaterKit/inc/mkFluidsMultiphase.h:37) is not provided: it throws.
+    void update_geometry(mkFluidsMRT& dendrite);
     void write_vtkfile(int loops) override;
     mkg_mp_handle mp() const { return m_; }
     const mkg_mp_params& mp_params() const { return mp_; }
@@ -418,3 +422,32 @@ private:
 
 // readParameters of the reference (MaterKit/inc/mkFunction.h:62): loops and frames.
 void readParameters(int& maxLoops, int& maxFrame, const std::string& xmlFileName);
+
+// Driver utilities of the reference that its run_* loops use: the wall-clock timer
+// (MaterKit/inc/mkCPUTimer.h:36-55, the global `timer` of AiLBFlow.h:83), which prints its
+// label and the elapsed seconds when deleted; the case-file name (mkt::paraXmlFileName,
+// mkFunction.h:132); the output folder prefix (workspace, mkGlobalDefs.h:86: here the current
+// directory); and main's set-up calls (mkFunction.h:58-59): make_datafolder creates the vtk
+// folder under the workspace, make_numthreads returns 1 (the device runs the steps; no host
+// threads).
+class mkCPUTimer {
+public:
+    mkCPUTimer() : mkCPUTimer("timer") {}
+    explicit mkCPUTimer(const std::string label);
+    ~mkCPUTimer();
+    double GetTime(int flag);  // seconds since construction (flag unused)
+    double GetSystemTime();    // seconds since the epoch
+
+private:
+    std::string label_;
+    std::chrono::steady_clock::time_point t0_;
+};
+extern mkCPUTimer* timer;
+namespace mkt {
+extern std::string paraXmlFileName;
+}
+extern const std::string workspace;
+extern "C" {
+int make_datafolder();
+int make_numthreads();
+}

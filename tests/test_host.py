@@ -350,3 +350,24 @@ def test_reference_driver_call_forms_compile_against_the_shim(tmp_path):
                           "-lmkg", f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
     assert res.returncode == 0, res.stderr
     assert subprocess.run([str(exe)], timeout=60).returncode == 0
+
+
+REF_DRIVER = Path("/root/reference/MaterKit/MaterKit.cpp")
+
+
+@pytest.mark.skipif(not REF_DRIVER.exists(), reason="reference tree absent")
+def test_reference_driver_compiles_unchanged_against_the_shim(tmp_path):
+    """The reference's own driver, MaterKit/MaterKit.cpp, compiled in place and unchanged:
+    its includes resolve to the forwarding headers of shim/compat, and it links against
+    libmkshim.so and libmkg.so.  (It is not run: its main reads the reference's case files.)"""
+    pkg = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd"
+    if not (pkg / "libmkshim.so").exists():
+        pytest.skip("shim not built")
+    exe = tmp_path / "materkit"
+    res = subprocess.run(["g++", "-std=c++17", f"-I{pkg / 'shim' / 'compat'}",
+                          f"-I{pkg / 'shim'}", f"-I{ROOT / 'include'}", "-o", str(exe),
+                          str(REF_DRIVER), f"-L{pkg}", "-lmkshim", "-lmkg",
+                          f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    errors = [l for l in res.stderr.splitlines() if "error" in l]
+    assert res.returncode == 0, "\n".join(errors[:20])
+    assert exe.exists()
