@@ -88,6 +88,8 @@ __device__ inline void locate_at(const StepArgs& a, uint32_t t, Cell& c) {
              (y + 1 == ny ? YHI : 0u) | (z == 0 ? ZLO : 0u) | (z + 1 == nz ? ZHI : 0u);
 }
 
+constexpr uint32_t kAllFaces = 0x3Fu;
+
 // Faces crossed when stepping from a cell by (dx, dy, dz).
 __host__ __device__ constexpr uint32_t faces_of(int dx, int dy, int dz) {
     return (dx < 0 ? XLO : dx > 0 ? XHI : 0u) | (dy < 0 ? YLO : dy > 0 ? YHI : 0u) |
@@ -142,6 +144,32 @@ __device__ inline double wall_cu(const StepArgs& a, int f, const Cell& c, int X,
 
 __device__ inline uint32_t nbr(const Cell& c, int dx, int dy, int dz) {
     return c.ox[dx + 1] + c.oy[dy + 1] + c.oz[dz + 1];
+}
+
+// Neighbourhood of a cell for the gradient stencils: the offsets of coordinates -1, 0, +1
+// along each axis, with the cell's own offset in place of the neighbour across each face in
+// np (the edge value replicated at non-periodic faces).  Held as named scalars: arrays copied
+// from the Cell record and patched at run time made the compiler keep the record in scratch
+// memory.
+struct Nbh {
+    uint32_t x0, x1, x2, y0, y1, y2, z0, z1, z2;
+    __device__ uint32_t at(int dx, int dy, int dz) const {
+        return (dx < 0 ? x0 : (dx == 0 ? x1 : x2)) + (dy < 0 ? y0 : (dy == 0 ? y1 : y2)) +
+               (dz < 0 ? z0 : (dz == 0 ? z1 : z2));
+    }
+};
+__device__ inline Nbh replicated_nbh(const Cell& c, uint32_t np) {
+    Nbh o;
+    o.x1 = c.ox[1];
+    o.y1 = c.oy[1];
+    o.z1 = c.oz[1];
+    o.x0 = (np & XLO) ? o.x1 : c.ox[0];
+    o.x2 = (np & XHI) ? o.x1 : c.ox[2];
+    o.y0 = (np & YLO) ? o.y1 : c.oy[0];
+    o.y2 = (np & YHI) ? o.y1 : c.oy[2];
+    o.z0 = (np & ZLO) ? o.z1 : c.oz[0];
+    o.z2 = (np & ZHI) ? o.z1 : c.oz[2];
+    return o;
 }
 
 // ------------------------------------------------------------------ population access
@@ -246,9 +274,14 @@ __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const St
 //   - crossing out in an E step, the E step stores the returning population in the face
 //     buffer, which the face cell alone reads in the next O step.  Its own slots cannot hold
 //     it: the interior neighbours overwrite them during the O step.
-template <int Q, bool ODD, class R, bool FACES = true>
+// FACES: the cell may lie on a non-periodic face or next to a wall cell.  FM (with FACES)
+// limits the face checks to the faces in the mask at compile time: a block whose only face
+// cells lie on z faces, in a domain without wall cells, takes FM = ZLO | ZHI and carries no
+// x / y face or wall logic.
+template <int Q, bool ODD, class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                  const StepArgs& a, int kind) {
+    constexpr bool WALLS = FACES && FM == kAllFaces;
     const R* __restrict__ base = (const R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
@@ -258,10 +291,10 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
-            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
+            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & FM & (uint32_t)L.npmask : 0u;
             if (hit == 0u) {
                 uint32_t src = nbr(c, dx, dy, dz);
-                if (FACES && a.has_walls && i != 0 && a.flag[src] == kIsWalls)
+                if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls)
                     p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
                 else
                     p[i] = ldp(slot_at(base + (size_t)inv(i) * S, src));
@@ -273,19 +306,20 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
     }
 }
 
-template <int Q, bool ODD, class R, bool FACES = true>
+template <int Q, bool ODD, class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                   const StepArgs& a, int kind,
                                   const FaceState<R>& fs = FaceState<R>()) {
+    constexpr bool WALLS = FACES && FM == kAllFaces;
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
         for (int i = 0; i < Q; i++) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
-        if (FACES && (c.bits & (uint32_t)L.npmask)) {
+        if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
 #pragma unroll
             for (int i = 1; i < Q; i++) {
                 const uint32_t hit =
-                    c.bits & faces_of(-cx(i), -cy(i), -cz(i)) & (uint32_t)L.npmask;
+                    c.bits & faces_of(-cx(i), -cy(i), -cz(i)) & FM & (uint32_t)L.npmask;
                 if (hit) {
                     const int f = __builtin_ctz(hit);
                     stp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, f)],
@@ -297,10 +331,10 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int dx = cx(i), dy = cy(i), dz = cz(i);
-            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
+            uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & FM & (uint32_t)L.npmask : 0u;
             if (hit == 0u) {
                 uint32_t tgt = nbr(c, dx, dy, dz);
-                if (FACES && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
+                if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
                     stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
                 else
                     stp(slot_at(base + (size_t)i * S, tgt), p[i]);
@@ -348,9 +382,10 @@ __host__ __device__ constexpr int nzdirs() {
 // stage region must not be written again before every thread has passed a barrier that
 // follows its last read.  Global memory needs no ordering: each (cell, slot) is loaded and
 // stored by one thread.
-template <int Q, class R, bool FACES = true>
+template <int Q, class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                        const StepArgs& a, R (*zv)[256], uint32_t tid) {
+    constexpr bool WALLS = FACES && FM == kAllFaces;
     constexpr int NZ = nzdirs<Q>();
     static_assert(NZ <= 32, "z mask too small");
     const R* __restrict__ base = (const R*)L.pop;
@@ -363,13 +398,14 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
-        const uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
+        const uint32_t hit =
+            FACES ? c.bits & faces_of(dx, dy, dz) & FM & (uint32_t)L.npmask : 0u;
         if (hit) {
             p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, __builtin_ctz(hit))]);
             if (dz != 0) own |= 1u << zslot<Q>(i);
         } else {
             const uint32_t src = nbr(c, dx, dy, dz);
-            if (FACES && a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
+            if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
                 p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
                 if (dz != 0) own |= 1u << zslot<Q>(i);
             } else if (dz == 0) {
@@ -393,17 +429,19 @@ __device__ inline void zpull_finish(R* p, const Cell& c, const StepArgs& a, R (*
             p[i] = zv[zslot<Q>(i)][rb + ((uint32_t)(c.z - cz(i)) & zm1)];
 }
 
-template <int Q, class R, bool FACES = true>
+template <int Q, class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
                                    const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
                                    uint32_t tid, bool act, const FaceState<R>& fs) {
+    constexpr bool WALLS = FACES && FM == kAllFaces;
     static_assert(nzdirs<Q>() <= 32, "z mask too small");
     R* __restrict__ base = (R*)L.pop;
     uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int dx = cx(i), dy = cy(i), dz = cz(i);
-        const uint32_t hit = FACES ? c.bits & faces_of(dx, dy, dz) & (uint32_t)L.npmask : 0u;
+        const uint32_t hit =
+            FACES ? c.bits & faces_of(dx, dy, dz) & FM & (uint32_t)L.npmask : 0u;
         if (dz != 0) zv[zslot<Q>(i)][tid] = p[i];
         if (hit) {
             if (act)
@@ -411,7 +449,7 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, con
                     face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs, c));
         } else {
             const uint32_t tgt = nbr(c, dx, dy, dz);
-            if (FACES && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
+            if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
                 if (act) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
             } else if (dz == 0) {
                 if (act) stp(slot_at(base + (size_t)i * S, tgt), p[i]);

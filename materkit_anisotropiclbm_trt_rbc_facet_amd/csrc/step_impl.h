@@ -26,23 +26,13 @@ __device__ inline R clamp01(R v) {
 
 // 19-point phase stencil around the cell (periodic faces wrap, other faces replicate the
 // edge value), index (dx+1)*9 + (dy+1)*3 + (dz+1).
-template <class R, bool FACES = true>
+template <class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline void load_phase_stencil(const StepArgs& a, const Cell& c, R (&P)[27]) {
     const R* __restrict__ ph = (const R*)a.phi_cur;
-    uint32_t px[3] = {c.ox[0], c.ox[1], c.ox[2]};
-    uint32_t py[3] = {c.oy[0], c.oy[1], c.oy[2]};
-    uint32_t pz[3] = {c.oz[0], c.oz[1], c.oz[2]};
-    const uint32_t np = FACES ? (uint32_t)a.g.npmask & c.bits : 0u;
-    if (np & XLO) px[0] = px[1];
-    if (np & XHI) px[2] = px[1];
-    if (np & YLO) py[0] = py[1];
-    if (np & YHI) py[2] = py[1];
-    if (np & ZLO) pz[0] = pz[1];
-    if (np & ZHI) pz[2] = pz[1];
+    const Nbh o = replicated_nbh(c, FACES ? (uint32_t)a.g.npmask & c.bits & FM : 0u);
 #pragma unroll
     for (int i = 0; i < 19; i++)
-        P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] =
-            *slot_at(ph, px[cx(i) + 1] + py[cy(i) + 1] + pz[cz(i) + 1]);
+        P[(cx(i) + 1) * 9 + (cy(i) + 1) * 3 + (cz(i) + 1)] = *slot_at(ph, o.at(cx(i), cy(i), cz(i)));
 }
 
 // Velocity faces (non-equilibrium extrapolation, mkFluidsUser::boundary_vel,
@@ -54,8 +44,14 @@ __device__ inline void nee_replace(R* f, const StepArgs& a, const Cell& c) {
     const uint32_t hit = c.bits & (uint32_t)a.nee_mask;
     if (!hit) return;
     const int fc = __builtin_ctz(hit);
-    const int ax = fc >> 1, sg = (fc & 1) ? -1 : 1;
-    const uint32_t n = nbr(c, ax == 0 ? sg : 0, ax == 1 ? sg : 0, ax == 2 ? sg : 0);
+    // the inward neighbour, from constant offsets only: a runtime index into the cell's
+    // offset arrays would move the whole Cell record to scratch memory
+    const uint32_t n = fc == 0   ? nbr(c, 1, 0, 0)
+                       : fc == 1 ? nbr(c, -1, 0, 0)
+                       : fc == 2 ? nbr(c, 0, 1, 0)
+                       : fc == 3 ? nbr(c, 0, -1, 0)
+                       : fc == 4 ? nbr(c, 0, 0, 1)
+                                 : nbr(c, 0, 0, -1);
     if (a.flag[n] == kIsWalls) return;
     const R* __restrict__ nb = (const R*)a.nee;
     const uint32_t k = face_slot(a, c, fc);
@@ -210,6 +206,9 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
 #ifndef MKG_STEP_WAVES_F32
 #define MKG_STEP_WAVES_F32 4
 #endif
+#ifndef MKG_STEP_WAVES_F32_ZFACES
+#define MKG_STEP_WAVES_F32_ZFACES MKG_STEP_WAVES_F32
+#endif
 
 // LDS slots of the z stage: flow pull region (reused by the phase push), flow push region,
 // phase pull region.
@@ -242,8 +241,11 @@ constexpr int zslots_of() {
 // populations of every cell; instantiated for the phase-field + thermal model, with or
 // without the D3Q19 flow).
 template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false,
-          bool TAPE = false>
-__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
+          bool TAPE = false, uint32_t SFM = kAllFaces>
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
+                                                             ? MKG_STEP_WAVES_F32_ZFACES
+                                                             : MKG_STEP_WAVES_F32)
+                                                      : 1)
     k_step(const StepArgs a) {
     constexpr bool Z = ODD && ZS;
     constexpr int ZF = Z ? zstage_slots<FQ, PQ>() : 1;
@@ -258,8 +260,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         zmh[(Z && TH) ? 256 : 1], zmc[(Z && SO) ? 256 : 1];
     const uint32_t tid = threadIdx.x;
     // Blocks (z stage) or waves (otherwise) none of whose cells lies on a non-periodic face,
-    // in a domain without wall cells, take the body without face rules (FACES false); a z
-    // stage block must also lie wholly before the last cell, so that every thread updates.
+    // in a domain without wall cells, take the body without face rules (FM = 0); a z stage
+    // block must also lie wholly before the last cell, so that every thread updates.  The
+    // others take the body with the face rules of SFM (every face, or only the z faces in a
+    // domain whose only non-periodic faces are z faces, with no wall cells or velocity faces).
     bool slow = !PER && a.has_walls != 0;
     if (!PER && !slow) {
         const uint32_t t = thread_cell(a);
@@ -273,7 +277,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             slow = __ballot(face) != 0ull;
     }
     auto body = [&](auto faces_tag) {
-    constexpr bool FACES = decltype(faces_tag)::value;
+    constexpr uint32_t FM = decltype(faces_tag)::value;
+    constexpr bool FACES = FM != 0u;
     Cell c;
     bool act = true;
     if constexpr (Z) {
@@ -309,25 +314,25 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         if (!a.h.adv)
             read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
         else if constexpr (Z)
-            own_h = zpull_issue<7, R, FACES>(h, a.h, S, c, a, zrh, tid);
+            own_h = zpull_issue<7, R, FACES, FM>(h, a.h, S, c, a, zrh, tid);
         else
-            read_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES, FM>(h, a.h, S, c, a, kKindScalar);
     }
     if constexpr (SO) {
         if (!a.c.adv)
             read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
         else if constexpr (Z)
-            own_c = zpull_issue<7, R, FACES>(cc, a.c, S, c, a, zrc, tid);
+            own_c = zpull_issue<7, R, FACES, FM>(cc, a.c, S, c, a, zrc, tid);
         else
-            read_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar);
     }
     if constexpr (FQ > 0) {
         if (!a.f.adv)
             read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
         else if constexpr (Z)
-            own_f = zpull_issue<FQ, R, FACES>(f, a.f, S, c, a, zrf, tid);
+            own_f = zpull_issue<FQ, R, FACES, FM>(f, a.f, S, c, a, zrf, tid);
         else
-            read_pops<FQ, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow);
+            read_pops<FQ, ODD, R, FACES, FM>(f, a.f, S, c, a, kKindFlow);
     }
     if constexpr (Z) {
         __syncthreads();  // B1
@@ -382,16 +387,16 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
         if (a.f.adv) {
             flow_collide<FQ, R>(f, dr, den, ux, uy, uz, Fx, Fy, Fz, (R)a.s_nu, (R)a.s_bulk,
                                 (R)a.s_odd, (R)a.s_even, ns);
-            if constexpr (FACES) nee_replace<FQ, R>(f, a, c);
+            if constexpr (FM == kAllFaces) nee_replace<FQ, R>(f, a, c);
             FaceState<R> fs;
             fs.wscale = (R)1 - ns;
             fs.ux = ux;
             fs.uy = uy;
             fs.uz = uz;
             if constexpr (Z)
-                zpush_stage<FQ, R, FACES>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+                zpush_stage<FQ, R, FACES, FM>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
             else
-                write_pops<FQ, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow, fs);
+                write_pops<FQ, ODD, R, FACES, FM>(f, a.f, S, c, a, kKindFlow, fs);
         }
         const R om = (R)1 - ns;
         ux *= om;
@@ -423,10 +428,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                 h[i] = h[i] - om * (h[i] - heq) + w * src;
             }
             if constexpr (Z)
-                zpush_stage<7, R, FACES>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+                zpush_stage<7, R, FACES, FM>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
                                   FaceState<R>());
             else
-                write_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
+                write_pops<7, ODD, R, FACES, FM>(h, a.h, S, c, a, kKindScalar);
         }
     }
 
@@ -434,7 +439,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     //      c = A(phi) U obeys dc/dt + div(c u) = div(D q grad U), written as an
     //      advection-diffusion of c with D_c = D q / A and the drift V = D q grad A / A^2
     R P[27];
-    if constexpr (SO && PQ > 0) load_phase_stencil<R, FACES>(a, c, P);
+    if constexpr (SO && PQ > 0) load_phase_stencil<R, FACES, FM>(a, c, P);
     if constexpr (SO) {
         if (a.c.adv) {
             R gphi[3] = {0, 0, 0};
@@ -471,9 +476,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             FaceState<R> fs;
             fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
             if constexpr (Z)
-                zpush_stage<7, R, FACES>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+                zpush_stage<7, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
             else
-                write_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar, fs);
+                write_pops<7, ODD, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar, fs);
         }
     }
 
@@ -485,7 +490,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     uint32_t own_g = 0;
     if constexpr (PQ > 0) {
         if (a.g.adv) {
-            if constexpr (!SO) load_phase_stencil<R, FACES>(a, c, P);
+            if constexpr (!SO) load_phase_stencil<R, FACES, FM>(a, c, P);
             R upf = 0;
             if (a.pf_coupling == 1) upf = T;
             if (a.pf_coupling == 2) upf = U;
@@ -523,9 +528,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
             // the populations are read after the source so that the stencil values are dead
             // (loaded with the other lattices, they measured 13 % slower at 256^3)
             if constexpr (Z)
-                own_g = zpull_issue<PQ, R, FACES>(g, a.g, S, c, a, zrg, tid);
+                own_g = zpull_issue<PQ, R, FACES, FM>(g, a.g, S, c, a, zrg, tid);
             else
-                read_pops<PQ, ODD, R, FACES>(g, a.g, S, c, a, kKindScalar);
+                read_pops<PQ, ODD, R, FACES, FM>(g, a.g, S, c, a, kKindScalar);
         }
     }
     if constexpr (Z) {
@@ -554,10 +559,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
                 g[i] = g[i] - om * (g[i] - geq) + w * (Sg + acz * cn);
             }
             if constexpr (Z)
-                zpush_stage<PQ, R, FACES>(g, a.g, S, c, a, kKindScalar, zrf, zmg, tid, act,
+                zpush_stage<PQ, R, FACES, FM>(g, a.g, S, c, a, kKindScalar, zrf, zmg, tid, act,
                                    FaceState<R>());
             else
-                write_pops<PQ, ODD, R, FACES>(g, a.g, S, c, a, kKindScalar);
+                write_pops<PQ, ODD, R, FACES, FM>(g, a.g, S, c, a, kKindScalar);
         }
         if constexpr (Z) {
             __syncthreads();  // B3
@@ -566,11 +571,11 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_STEP_WAVES_F32 : 1)
     }
     };
     if constexpr (PER)
-        body(std::false_type{});
+        body(std::integral_constant<uint32_t, 0u>{});
     else if (slow)
-        body(std::true_type{});
+        body(std::integral_constant<uint32_t, SFM>{});
     else
-        body(std::false_type{});
+        body(std::integral_constant<uint32_t, 0u>{});
 }
 
 // Host-side launch helpers for one storage type.
@@ -601,19 +606,30 @@ inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
             return;
         }
     }
+    // only z faces non-periodic, no wall cells, no velocity faces: the face body checks the
+    // z faces alone
+    const uint32_t np = (uint32_t)(a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask);
+    const bool zonly = !per && !a.has_walls && !a.nee_mask && (np & ~(uint32_t)(ZLO | ZHI)) == 0u;
+    constexpr uint32_t ZF = ZLO | ZHI;
     if (odd && a.zstage) {
         if (per && a.N == a.nblk * 256u)
             k_step<R, FQ, PQ, TH, SO, true, true, true><<<grid, block, 0, s>>>(a);
+        else if (zonly)
+            k_step<R, FQ, PQ, TH, SO, true, true, false, false, ZF><<<grid, block, 0, s>>>(a);
         else
             k_step<R, FQ, PQ, TH, SO, true, true><<<grid, block, 0, s>>>(a);
     } else if (odd) {
         if (per)
             k_step<R, FQ, PQ, TH, SO, true, false, true><<<grid, block, 0, s>>>(a);
+        else if (zonly)
+            k_step<R, FQ, PQ, TH, SO, true, false, false, false, ZF><<<grid, block, 0, s>>>(a);
         else
             k_step<R, FQ, PQ, TH, SO, true><<<grid, block, 0, s>>>(a);
     } else {
         if (per)
             k_step<R, FQ, PQ, TH, SO, false, false, true><<<grid, block, 0, s>>>(a);
+        else if (zonly)
+            k_step<R, FQ, PQ, TH, SO, false, false, false, false, ZF><<<grid, block, 0, s>>>(a);
         else
             k_step<R, FQ, PQ, TH, SO, false><<<grid, block, 0, s>>>(a);
     }

@@ -194,6 +194,81 @@ std::vector<double> mkFluidsLBM::field(int id) const {
     return v;
 }
 
+mkNode* mkFluidsLBM::fetch_nodes() {
+    if (!h_) throw mkt::mkLogicException("fetch_nodes before initialize (or on a class without "
+                                         "the lattice handle)");
+    fvm_flush();
+    const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
+    nodes_.resize(N);
+    auto pops = [&](int id, int q, mkPDFs mkNode::*member) {
+        if (!q) return;
+        std::vector<double> v((size_t)q * N);
+        check(mkg_get_field(h_, id, v.data(), MKG_DT_F64, v.size()));
+        for (size_t k = 0; k < N; k++) {
+            mkPDFs& d = nodes_[k].*member;
+            d.resize(q);
+            for (int i = 0; i < q; i++) d[i] = v[(size_t)i * N + k];
+        }
+    };
+    pops(MKG_F_POP_FLOW, p_.flow_q, &mkNode::f_hlp);
+    pops(MKG_F_POP_PHASE, p_.phase_q, &mkNode::g_hlp);
+    pops(MKG_F_POP_THERMAL, p_.thermal ? 7 : 0, &mkNode::h_hlp);
+    pops(MKG_F_POP_SOLUTE, p_.solute ? 7 : 0, &mkNode::x_hlp);
+    auto scalar = [&](int id, bool present, auto set) {
+        if (!present) return;
+        std::vector<double> v(N);
+        check(mkg_get_field(h_, id, v.data(), MKG_DT_F64, N));
+        for (size_t k = 0; k < N; k++) set(nodes_[k], v[k]);
+    };
+    scalar(MKG_F_RHO, p_.flow_q != 0, [](mkNode& n, double v) { n.density = v; });
+    scalar(MKG_F_TEMP, p_.thermal != 0, [](mkNode& n, double v) { n.temperature = v; });
+    scalar(MKG_F_CONC, p_.solute != 0, [](mkNode& n, double v) { n.X_solutes.assign(1, v); });
+    if (p_.phase_q) {
+        std::vector<double> now(N), prev(N);
+        check(mkg_get_field(h_, MKG_F_PHI, now.data(), MKG_DT_F64, N));
+        check(mkg_get_field(h_, MKG_F_PHI_PREV, prev.data(), MKG_DT_F64, N));
+        for (size_t k = 0; k < N; k++) {
+            nodes_[k].phi = now[k];
+            nodes_[k].d_phi = now[k] - prev[k];
+        }
+    }
+    if (p_.flow_q) {
+        std::vector<double> u(3 * N);
+        check(mkg_get_field(h_, MKG_F_VEL, u.data(), MKG_DT_F64, u.size()));
+        for (size_t k = 0; k < N; k++) nodes_[k].vel.set(u[3 * k], u[3 * k + 1], u[3 * k + 2]);
+    }
+    std::vector<int8_t> fl(N);
+    check(mkg_get_field(h_, MKG_F_FLAG, fl.data(), MKG_DT_I8, N));
+    std::vector<int16_t> gr(N);
+    check(mkg_get_field(h_, MKG_F_GRAIN, gr.data(), MKG_DT_I16, N));
+    for (size_t k = 0; k < N; k++) {
+        nodes_[k].flag = fl[k];
+        nodes_[k].grain_id = gr[k];
+    }
+    node = nodes_.data();
+    return node;
+}
+
+void mkFluidsLBM::store_nodes() {
+    if (!h_ || nodes_.empty()) throw mkt::mkLogicException("store_nodes before fetch_nodes");
+    fvm_flush();
+    const size_t N = nodes_.size();
+    auto pops = [&](int id, int q, mkPDFs mkNode::*member) {
+        if (!q) return;
+        std::vector<double> v((size_t)q * N);
+        for (size_t k = 0; k < N; k++)
+            for (int i = 0; i < q; i++) v[(size_t)i * N + k] = (nodes_[k].*member)[i];
+        check(mkg_set_field(h_, id, v.data(), MKG_DT_F64, v.size()));
+    };
+    pops(MKG_F_POP_FLOW, p_.flow_q, &mkNode::f_hlp);
+    pops(MKG_F_POP_PHASE, p_.phase_q, &mkNode::g_hlp);
+    pops(MKG_F_POP_THERMAL, p_.thermal ? 7 : 0, &mkNode::h_hlp);
+    pops(MKG_F_POP_SOLUTE, p_.solute ? 7 : 0, &mkNode::x_hlp);
+    std::vector<int8_t> fl(N);
+    for (size_t k = 0; k < N; k++) fl[k] = (int8_t)nodes_[k].flag;
+    check(mkg_set_field(h_, MKG_F_FLAG, fl.data(), MKG_DT_I8, N));
+}
+
 double mkFluidsLBM::Thermal_NusseltRBC() {
     fvm_flush();
     double v[4];

@@ -80,10 +80,42 @@ private:
     std::string xml_;
 };
 
-// The reference's node record (MaterKit/inc/mkNode.h:134-198).  Node state lives on the device
-// in structure-of-arrays form, so the type is incomplete here: a driver can pass flow.node to
-// the finite-volume calls (evolution_ATC2(flow.node)), but cannot read node[index].
-struct mkNode;
+// Populations of one node (MaterKit/inc/mkPDFs.h:22-123): up to 27 values, size() of them
+// used (the lattice's Q).
+class mkPDFs {
+public:
+    mkReal& operator[](int i) { return v_[i]; }
+    mkReal operator[](int i) const { return v_[i]; }
+    int size() const { return q_; }
+    void resize(int q) { q_ = q; }
+    mkReal density() const {
+        mkReal s = 0;
+        for (int i = 0; i < q_; i++) s += v_[i];
+        return s;
+    }
+
+private:
+    mkReal v_[27] = {};
+    int q_ = 0;
+};
+
+// The reference's node record (MaterKit/inc/mkNode.h:134-198) as a host mirror of the device
+// state: the pre-collision populations of every lattice at the current time level (f_hlp,
+// g_hlp, h_hlp, x_hlp; canonical order of include/mkg.h's lattice tables, full values), the
+// macroscopic fields (density, vel, phi, d_phi, temperature, X_solutes[0] = the conserved
+// solute c), flag and grain_id.  The state itself lives on the device in structure-of-arrays
+// form; mkFluidsLBM::fetch_nodes() materialises this mirror, store_nodes() writes the
+// populations and flags back.
+struct mkNode {
+    mkPDFs f_hlp, g_hlp, h_hlp, x_hlp;
+    mkVector3d vel;
+    double density = 0, phi = 0, d_phi = 0;
+    mkReal temperature = 0;
+    std::vector<mkReal> X_solutes;
+    int flag = 0;
+    int grain_id = 0;
+    int getGrainId() const { return grain_id; }
+};
 
 class mkFluidsLBM {
 public:
@@ -141,8 +173,14 @@ public:
     const mkg_params& params() const { return p_; }
     std::vector<double> field(int id) const;
     void set_workspace(const std::string& dir) { workspace_ = dir; }
-    // node array of the reference (MaterKit/inc/mkFluidsLBM.h:95); never dereferenced here
+    // Node array of the reference (MaterKit/inc/mkFluidsLBM.h:95), materialised lazily:
+    // null until fetch_nodes(), which copies the device state into a host mirror of
+    // Mx*My*Mz records (index (x*My + y)*Mz + z) and points node at it; a later fetch_nodes()
+    // refreshes it (device steps do not).  store_nodes() writes the mirror's populations and
+    // flags back to the device (the macroscopic fields follow from the populations).
     mkNode* node = nullptr;
+    mkNode* fetch_nodes();
+    void store_nodes();
 
 protected:
     virtual void advance(int mask);
@@ -163,6 +201,7 @@ protected:
     std::string workspace_ = "./";
     mkg_handle h_ = nullptr;
     mkg_params p_{};
+    std::vector<mkNode> nodes_;
     int32_t run_info_[4] = {0, 0, 0, 0};
     int pending_ = 0;
 };

@@ -548,3 +548,82 @@ def test_reference_api_driver_grows_thermal_grains(tmp_path):
     first, last = ([int(l.split(f"solid_{j}")[1].split()[0]) for j in range(3)]
                    for l in (rows[0], rows[-1]))
     assert all(b > 1.5 * a for a, b in zip(first, last)), (first, last)
+
+
+NODE_MIRROR = r"""
+#include <cmath>
+#include <cstdio>
+#include "mkshim.h"
+// Host node mirror: fetch after device steps, check it against the device's own moments,
+// write modified populations and flags back, fetch again.
+int main(int argc, char** argv) {
+    mkSpace space(argv[1]);
+    mkFluidsMRT flow(space);
+    flow.mkFluidsMRT::initialize(argv[1]);
+    flow.mkFluidsMRT::initialize_geometry();
+    flow.step(3);
+    if (flow.node) return 10;  // lazily materialised: nothing before fetch_nodes
+    mkNode* node = flow.fetch_nodes();
+    const mkg_params& p = flow.params();
+    const int N = p.nx * p.ny * p.nz;
+    // phi is the last moment pass (the phase field the step collided with, as the reference's
+    // node.phi after relaxation); T and rho are the moments of the current populations
+    const std::vector<double> phi = flow.field(MKG_F_PHI);
+    double e_phi = 0, e_T = 0, e_rho = 0;
+    for (int k = 0; k < N; k++) {
+        e_phi = std::fmax(e_phi, std::fabs(phi[k] - node[k].phi));
+        e_T = std::fmax(e_T, std::fabs(node[k].h_hlp.density() - node[k].temperature));
+        if (p.flow_q) e_rho = std::fmax(e_rho, std::fabs(node[k].f_hlp.density() - node[k].density));
+    }
+    std::printf("moments %.3e %.3e %.3e\n", e_phi, e_T, e_rho);
+    if (e_phi > 1e-12 || e_T > 1e-12 || e_rho > 1e-12) return 11;
+    // perturb: a wall block, shifted populations
+    for (int k = 0; k < N; k++) {
+        node[k].g_hlp[0] += 1e-3;
+        node[k].h_hlp[1] -= 2e-3;
+        if (p.flow_q) node[k].f_hlp[2] += 1e-4;
+    }
+    node[(2 * p.ny + 3) * p.nz + 4].flag = MKG_IS_WALLS;
+    flow.store_nodes();
+    std::vector<mkNode> mine(node, node + N);
+    mkNode* back = flow.fetch_nodes();
+    double e = 0;
+    for (int k = 0; k < N; k++) {
+        for (int i = 0; i < p.phase_q; i++) e = std::fmax(e, std::fabs(back[k].g_hlp[i] - mine[k].g_hlp[i]));
+        for (int i = 0; i < 7; i++) e = std::fmax(e, std::fabs(back[k].h_hlp[i] - mine[k].h_hlp[i]));
+        for (int i = 0; i < p.flow_q; i++) e = std::fmax(e, std::fabs(back[k].f_hlp[i] - mine[k].f_hlp[i]));
+        if (back[k].flag != mine[k].flag) return 12;
+    }
+    std::printf("round trip %.3e\n", e);
+    if (e > 1e-15) return 13;
+    flow.step(2);
+    std::printf("ok\n");
+    return 0;
+}
+"""
+
+
+def test_node_mirror_round_trip(tmp_path):
+    """The host node mirror (node[] of mkFluidsLBM, materialised by fetch_nodes): after device
+    steps its populations sum to the device's rho and T and phi is the device's last moment
+    pass; populations and flags written through store_nodes come back unchanged."""
+    pkg = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd"
+    xml = (ROOT / "cases" / "C3_free_dendrite.xml").read_text()
+    xml = xml.replace("<MeshXYZ> 256   256   256 </MeshXYZ>", "<MeshXYZ> 24 20 12 </MeshXYZ>")
+    xml = xml.replace("<center> 127.5 127.5 127.5 </center>", "<center> 11.5 9.5 5.5 </center>")
+    xml = xml.replace("<radius> 8 </radius>", "<radius> 4 </radius>")
+    xml = xml.replace("<precision> fp32 </precision>", "<precision> fp64 </precision>")
+    xml = xml.replace("<flowField> false </flowField>", "<flowField> true </flowField>")
+    case = tmp_path / "small.xml"
+    case.write_text(xml)
+    src = tmp_path / "mirror.cpp"
+    src.write_text(NODE_MIRROR)
+    exe = tmp_path / "mirror"
+    res = subprocess.run(["g++", "-std=c++17", f"-I{ROOT / 'include'}", f"-I{pkg / 'shim'}",
+                          "-o", str(exe), str(src), f"-L{pkg}", "-lmkshim", "-lmkg",
+                          f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    run = subprocess.run([str(exe), str(case)], capture_output=True, text=True, timeout=120)
+    print(run.stdout, flush=True)
+    assert run.returncode == 0, (run.returncode, run.stdout, run.stderr)
+    assert "ok" in run.stdout

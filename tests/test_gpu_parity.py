@@ -63,6 +63,13 @@ def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodi
               T_face=[0.3, 0, 0, 0, -0.2, 0], C_face=[-0.4, 0, 0, 0, 0, 0])
         p.u_face[0][0] = 0.02
         p.u_face[5][0], p.u_face[5][1] = 0.01, 0.005
+    if bcs == "zwalls":
+        # only the z faces non-periodic, no wall cells: the launches take the kernel variant
+        # whose face body checks the z faces alone
+        p.set(bc_f=[P, P, P, P, BB, BB], bc_g=[P, P, P, P, BB, OUT], bc_h=[P, P, P, P, DI, DI],
+              bc_c=[P, P, P, P, DI, BB], T_face=[0, 0, 0, 0, 0.3, -0.2],
+              C_face=[0, 0, 0, 0, -0.4, 0])
+        p.u_face[5][0], p.u_face[5][1] = 0.01, -0.004
     if bcs == "velocity":
         # velocity inlet (non-equilibrium extrapolation) and pressure outlet along x, walls
         # along z rotating about a tilted axis through the domain centre
@@ -336,3 +343,25 @@ def test_z_stage_agrees_with_direct_o_steps(case, monkeypatch):
     for f in out[0]:
         scale = np.abs(out[1][f]).max()
         assert np.abs(out[0][f] - out[1][f]).max() <= 1e-6 * scale, f
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("dims", [(10, 9, 8), (5, 3, 64), (2, 3, 256)],
+                         ids=["nz8", "nz64", "nz256"])
+@pytest.mark.parametrize("nsteps", [5, 6])
+def test_fp64_z_faces_only_match_oracle(case, dims, nsteps):
+    """Non-periodic z faces only (moving top wall, Dirichlet T and c): the kernel variant with
+    the z face rules alone, without and with the z stage (one z row per block at nz = 256)."""
+    errs, fd, fo, dev, orc = _run(case, "zwalls", nsteps, dims=dims)
+    dev.close()
+    assert np.array_equal(fd, fo), "flags differ"
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+def test_fp32_z_faces_only_close_to_oracle(case):
+    errs, fd, fo, dev, orc = _run(case, "zwalls", 6, prec=mkg.PREC_FP32, dims=(4, 4, 256))
+    dev.close()
+    for k, e in errs.items():
+        assert e < 1e-5, f"{k}: rel err {e:.3e}"
