@@ -61,43 +61,17 @@ def parse():
     ap.add_argument("--no-mp", action="store_true")
     ap.add_argument("--mpf-steps", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run three plain 1 GiB device copies (the reference kernel for "
+                    help="first run one plain 1 GiB device copy (k_copy4, the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     return ap.parse_args()
 
 
 def copy_calibration():
-    import torch
+    """One plain 1 GiB device copy by the library's copy kernel (k_copy4), the reference that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    from materkit_anisotropiclbm_trt_rbc_facet_amd import mkg
 
-    x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    del x, y
-
-
-def copy_bandwidth(gib: float = 1.0, reps: int = 10) -> float:
-    """Device copy rate (GB/s of read + write) of torch's copy kernel on a gib-GiB buffer,
-    timed with events on the stream the copies run on: the streaming rate a kernel can reach
-    in practice, beside the 8 TB/s of the data sheet."""
-    import torch
-
-    n = int(gib * 2 ** 30) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        y.copy_(x)
-    e1.record(s)
-    e1.synchronize()
-    sec = e0.elapsed_time(e1) * 1e-3 / reps
-    del x, y
-    return 2 * n * 4 / sec / 1e9
+    mkg.stream_copy_rate(1 << 30, 1)
 
 
 def run_device(prec, size, steps, warmup, faces=False):
@@ -526,7 +500,7 @@ def main():
         out["cpu_baseline"]["other_solvers"] = cpu_other_solvers(a.cpu_seconds)
     # the whole step on the benchmark's own definition: b32 algorithmic bytes per cell over the
     # step time (k_step plus the phase-moment pass, which re-reads the phase populations)
-    copy_gbs = copy_bandwidth()
+    copy_gbs = mkg.stream_copy_rate(1 << 30, 10)
     step_s = wall32 / a.steps
     out["whole_step"] = {
         "frac": round(b32 * N / step_s / 1e9 / HBM_PEAK_GBS, 4),

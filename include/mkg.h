@@ -247,6 +247,12 @@ int mkg_random_seeds(uint64_t seed, int n, double radius, int nx, int ny, int nz
 int mkg_aniso_eval(int kind, double eps4, const double* par, const double* rot,
                    const double* normal, double* a_s, double* n_v);
 
+/* Streaming rate of the device in GB/s (read + write) of a plain copy kernel over two
+ * buffers of `bytes` each (allocated and freed by the call), averaged over `reps` launches:
+ * the practical ceiling against which the bandwidth-bound kernels are read.  No reference
+ * counterpart (a measurement aid). */
+int mkg_stream_copy_rate(size_t bytes, int reps, double* gbs);
+
 /* Quaternion (w,x,y,z) to row-major rotation matrix. */
 int mkg_quat_to_rot(const double* q, double* R);
 

@@ -361,3 +361,53 @@ extern "C" int mkg_aniso_eval(int kind, double eps4, const double* par, const do
     *a_s = as;
     return 0;
 }
+
+// Streaming rate of the device: a copy kernel (one four-float vector per lane, nontemporal
+// loads and stores, one launch over the whole buffer), timed with HIP events on a stream of
+// its own.  The reference against which a bandwidth-bound kernel's rate is read.
+namespace {
+typedef float copy_f4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy4(const copy_f4* __restrict__ src,
+                                               copy_f4* __restrict__ dst, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+}
+}  // namespace
+
+extern "C" int mkg_stream_copy_rate(size_t bytes, int reps, double* gbs) {
+    if (!gbs || bytes < 16 || reps < 1) {
+        mkg::set_error("mkg_stream_copy_rate: bytes >= 16, reps >= 1 and gbs required");
+        return -1;
+    }
+    const size_t n4 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, n4 * 16);
+    if (e == hipSuccess) e = hipMalloc(&b, n4 * 16);
+    if (e == hipSuccess) e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, n4 * 16, s);
+    float ms = 0;
+    if (e == hipSuccess) {
+        const dim3 grid((unsigned)((n4 + 255) / 256)), block(256);
+        k_copy4<<<grid, block, 0, s>>>((const copy_f4*)a, (copy_f4*)b, n4);  // warm-up
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps; r++) k_copy4<<<grid, block, 0, s>>>((const copy_f4*)a, (copy_f4*)b, n4);
+        hipEventRecord(e1, s);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) {
+        mkg::set_error(std::string("mkg_stream_copy_rate: ") + hipGetErrorString(e));
+        return -1;
+    }
+    *gbs = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
+    return 0;
+}

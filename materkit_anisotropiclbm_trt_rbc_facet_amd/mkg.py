@@ -213,6 +213,7 @@ def lib() -> C.CDLL:
         "mkg_mpf_init": (C.c_int, [vp, vp, vp]),
         "mkg_mpf_init_fields": (C.c_int, [vp, vp, vp, vp]),
         "mkg_aniso_eval": (C.c_int, [C.c_int, C.c_double, vp, vp, vp, vp, vp]),
+        "mkg_stream_copy_rate": (C.c_int, [C.c_size_t, C.c_int, vp]),
         "mkg_mpf_step": (C.c_int, [vp, C.c_int]),
         "mkg_mpf_sync": (C.c_int, [vp]),
         "mkg_mpf_get": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
@@ -305,6 +306,13 @@ def aniso_eval(kind: int, normal, eps4: float = 0.0, par=(0.0, 0.0), rot=None):
                               C.byref(a), _ptr(nv))
     _check(0 if rc == 1 else rc)  # 1: zero normal, isotropic
     return a.value, nv
+
+
+def stream_copy_rate(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """GB/s (read + write) of the library's plain copy kernel on the current device."""
+    out = C.c_double()
+    _check(lib().mkg_stream_copy_rate(int(nbytes), int(reps), C.byref(out)))
+    return out.value
 
 
 def quat_to_rot(q) -> np.ndarray:
