@@ -11,12 +11,15 @@
 // The reverse of one step is two launches:
 //   k_adj_collide  gathers the streamed adjoints, reverses both collisions and the source
 //                  (reverse mode through the face fluxes and the cell anisotropy, with the
-//                  exact Hessian of the Karma-Rappel energy), writes the adjoint populations,
-//                  the per-neighbour stencil contributions P_k, and accumulates dL/dkappa and
-//                  the FP64 block partials of dL/deps4;
-//   k_adj_stencil  gathers P_k back to the phi values they came from (transpose of the
-//                  stencil), giving the part of phi-adjoint that every population of the cell
-//                  receives; it is folded in when the next (earlier) step gathers.
+//                  exact Hessian of the Karma-Rappel energy), writes the adjoint populations
+//                  and the stencil contributions, and accumulates dL/dkappa and the FP64 block
+//                  partials of dL/deps4;
+//   k_adj_qsum /   the transpose of the stencil: the part of the phi adjoint that every
+//   k_adj_stencil  population of the cell receives, folded in when the next (earlier) step
+//                  gathers.  When a block holds whole z rows the contributions leave
+//                  k_adj_collide as nine column sums per cell (36 B in FP32) and k_adj_qsum
+//                  adds them; otherwise as 19 per-direction values (76 B) that k_adj_stencil
+//                  gathers with the face clamps.
 // Adjoint populations live in plain two-lattice canonical buffers (Q x N).
 #include "device.h"
 #include "fadj.h"
@@ -251,6 +254,92 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
     }
 }
 
+// ---------------------------------------------------------------- grouped stencil transpose
+// Ph[d] (d a 19-point stencil offset) is the adjoint of phi read by cell x at
+// clampwrap(x + d): a non-periodic face of the phase lattice replicates the cell's own value.
+// The contributions are first moved to the offsets they were actually read from (a clamped
+// axis folds d onto 0), then summed per (dx, dy) column: the dz = +-1 parts go to the cells at
+// z -+ 1 of the same row through LDS (a non-periodic z face has folded them away, so the
+// wrap-around reads zero there), and the column sum lands at the target row (x+dx, y+dy) in
+// array Q[(dx+1)*3 + (dy+1)].  Every (array, cell) is written by exactly one source cell per
+// step (the one at cell - (dx, dy), with periodic wrap), so no array needs clearing; the phi
+// adjoint of a cell is the sum of its nine array values (k_adj_qsum).  Every thread of the
+// block calls this (one barrier); act: the thread holds a live cell.
+template <class R>
+__device__ inline void group_stencil_adjoint(const StepArgs& a, const Cell& c, bool act,
+                                             R (&Ph)[27], R (*zq)[256], R* __restrict__ Q) {
+    const uint32_t np = (uint32_t)a.g.npmask & c.bits;
+    if (np & (XLO | XHI)) {
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                if (np & XLO) {
+                    Ph[PIX(0, dy, dz)] += Ph[PIX(-1, dy, dz)];
+                    Ph[PIX(-1, dy, dz)] = 0;
+                }
+                if (np & XHI) {
+                    Ph[PIX(0, dy, dz)] += Ph[PIX(1, dy, dz)];
+                    Ph[PIX(1, dy, dz)] = 0;
+                }
+            }
+    }
+    if (np & (YLO | YHI)) {
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+            for (int dz = -1; dz <= 1; dz++) {
+                if (np & YLO) {
+                    Ph[PIX(dx, 0, dz)] += Ph[PIX(dx, -1, dz)];
+                    Ph[PIX(dx, -1, dz)] = 0;
+                }
+                if (np & YHI) {
+                    Ph[PIX(dx, 0, dz)] += Ph[PIX(dx, 1, dz)];
+                    Ph[PIX(dx, 1, dz)] = 0;
+                }
+            }
+    }
+    if (np & (ZLO | ZHI)) {
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++) {
+                if (np & ZLO) {
+                    Ph[PIX(dx, dy, 0)] += Ph[PIX(dx, dy, -1)];
+                    Ph[PIX(dx, dy, -1)] = 0;
+                }
+                if (np & ZHI) {
+                    Ph[PIX(dx, dy, 0)] += Ph[PIX(dx, dy, 1)];
+                    Ph[PIX(dx, dy, 1)] = 0;
+                }
+            }
+    }
+    // the columns with dz != 0 parts: (0,0), (-1,0), (1,0), (0,-1), (0,1)
+    const uint32_t tid = threadIdx.x;
+    constexpr int CX[5] = {0, -1, 1, 0, 0}, CY[5] = {0, 0, 0, -1, 1};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        zq[2 * k][tid] = Ph[PIX(CX[k], CY[k], 1)];       // to z + 1
+        zq[2 * k + 1][tid] = Ph[PIX(CX[k], CY[k], -1)];  // to z - 1
+    }
+    __syncthreads();
+    if (!act) return;
+    const uint32_t zm1 = (uint32_t)a.nz - 1u;
+    const uint32_t rb = tid - (tid & zm1);
+    const uint32_t below = rb + ((uint32_t)(c.z - 1) & zm1), above = rb + ((uint32_t)(c.z + 1) & zm1);
+    const size_t N = a.N;
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++) {
+            R v = Ph[PIX(dx, dy, 0)];
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+                if (CX[k] == dx && CY[k] == dy) v += zq[2 * k][below] + zq[2 * k + 1][above];
+            stp(slot_at(Q + (size_t)((dx + 1) * 3 + (dy + 1)) * N, nbr(c, dx, dy, 0)), v);
+        }
+}
+
 // ---------------------------------------------------------------- kernels
 // Minimum waves per SIMD of the FP32 reverse kernel (the register bound follows from it).
 #ifndef MKG_ADJ_WAVES_F32
@@ -262,15 +351,23 @@ __device__ inline void adj_gather(R* out, const R* __restrict__ A, const R* __re
 // so their reverse adds w_k (c_k.u~) / cs^2 terms to dL/dT and dL/dphi and emits dL/du~
 // (j.ut) for the flow kernel; the buoyancy part of dL/dT arrives as a cell-wide thermal
 // adjoint (j.hcw_in) with the gathered populations.
-template <class R, bool FLOW>
+// GQ: the block holds whole z rows.  The stencil contributions of a cell are summed per
+// (dx, dy) column of the stencil into nine row sums, the z shifts taken through LDS, and
+// written to nine N-arrays at the target rows (see group_stencil_adjoint); otherwise the 19
+// contributions are written per direction for k_adj_stencil.
+template <class R, bool FLOW, bool GQ>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
     k_adj_collide(const StepArgs a, const AdjArgs j) {
     Cell c;
-    const bool valid = locate(a, c);
+    const bool valid = GQ ? locate_clamped(a, c) : locate(a, c);
     const uint32_t N = a.N;
     const uint32_t id = c.id;
     __shared__ double red[4];
+    __shared__ R zq[GQ ? 10 : 1][256];
     double eps_loc = 0;
+    R Ph[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) Ph[k] = 0;
     if (valid && a.flag[id] != kIsWalls) {
         R gs[19], hs[7];
         const R* hcw = FLOW ? (const R*)j.hcw_in : nullptr;
@@ -283,7 +380,7 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
         }
         // forward values at level n
         R h[7];
-        read_pops_rt<7, R>(h, j.hsave, a.S, c, a, kKindScalar);
+        for (int k = 0; k < 7; k++) h[k] = *slot_at((const R*)j.htape + (size_t)k * a.S, c.id);
         R sh = 0;
 #pragma unroll
         for (int k = 0; k < 7; k++) sh += h[k];
@@ -383,7 +480,6 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
             }
             Om -= (h[k] - heq) * hs[k];
         }
-        R Ph[27];
         R eh = 0;
         const R upf = a.pf_coupling == 1 ? T : (R)0;
         const R uh = phase_source_adj<R>(P, upf, (R)a.eps4, (R)a.lambda, (R)a.pf_dt,
@@ -416,11 +512,15 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
         for (int i = 0; i < 19; i++) stp(slot_at(go + (size_t)i * N, id), og * gs[i] + phl);
         ((R*)j.pout)[id] = -L * Bl;
         if (a.has_kappa) ((R*)j.kap_grad)[id] += Om * dom_dkap;
-        R* __restrict__ Pb = (R*)j.P;
+        if constexpr (!GQ) {
+            R* __restrict__ Pb = (R*)j.P;
 #pragma unroll
-        for (int i = 0; i < 19; i++) stp(slot_at(Pb + (size_t)i * N, id), Ph[PIX(cx(i), cy(i), cz(i))]);
+            for (int i = 0; i < 19; i++)
+                stp(slot_at(Pb + (size_t)i * N, id), Ph[PIX(cx(i), cy(i), cz(i))]);
+        }
         eps_loc = (double)eh;
     }
+    if constexpr (GQ) group_stencil_adjoint<R>(a, c, valid, Ph, zq, (R*)j.P);
     // FP64 block sum of dL/deps4 (wavefront butterfly + LDS), accumulated per block
     double v = eps_loc;
 #pragma unroll
@@ -488,6 +588,19 @@ __global__ void __launch_bounds__(256) k_adj_stencil(const StepArgs a, const R* 
     nout[c.id] = s;
 }
 
+// The phi adjoint of every cell from the nine column sums of group_stencil_adjoint.
+template <class R>
+__global__ void __launch_bounds__(256) k_adj_qsum(const StepArgs a, const R* __restrict__ Q,
+                                                  R* __restrict__ nout) {
+    const uint32_t t = thread_cell(a);
+    if (t >= a.N) return;
+    const size_t N = a.N;
+    R s = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s += ldp(slot_at(Q + (size_t)k * N, t));
+    nout[t] = s;
+}
+
 // Loss L = 1/2 sum (phi - target)^2 (FP64 block partials) and the seed of the reverse sweep:
 // the cell-wide phase adjoint phi - target.
 template <class R>
@@ -498,7 +611,7 @@ __global__ void __launch_bounds__(256) k_adj_loss(const StepArgs a, const double
     double v = 0;
     if (locate(a, c) && a.flag[c.id] != kIsWalls) {
         R g[19];
-        read_pops_rt<19, R>(g, a.g, a.S, c, a, kKindScalar);
+        read_pops_rt<19, R>(g, a.g, c, a, kKindScalar);
         R s = 0;
 #pragma unroll
         for (int i = 0; i < 19; i++) s += g[i];
@@ -584,7 +697,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1)
 #pragma unroll
     for (int i = 0; i < Q; i++) f[i] = (R)j.ftape[(size_t)i * N + id];
     R h[7];
-    read_pops_rt<7, R>(h, j.hsave, a.S, c, a, kKindScalar);
+    // the tape holds the pre-collision thermal populations slot by slot (stride S)
+    for (int k = 0; k < 7; k++) h[k] = *slot_at((const R*)j.htape + (size_t)k * a.S, c.id);
     R sh = 0;
 #pragma unroll
     for (int k = 0; k < 7; k++) sh += h[k];
@@ -645,14 +759,31 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1)
 }
 
 // ---------------------------------------------------------------- launchers
-int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
-    if (fp32) k_adj_collide<float, false><<<a.nblk, 256, 0, s>>>(a, j);
-    else k_adj_collide<double, false><<<a.nblk, 256, 0, s>>>(a, j);
+int launch_adj_collide(int fp32, int grouped, const StepArgs& a, const AdjArgs& j,
+                       hipStream_t s) {
+    if (grouped) {
+        if (fp32) k_adj_collide<float, false, true><<<a.nblk, 256, 0, s>>>(a, j);
+        else k_adj_collide<double, false, true><<<a.nblk, 256, 0, s>>>(a, j);
+    } else {
+        if (fp32) k_adj_collide<float, false, false><<<a.nblk, 256, 0, s>>>(a, j);
+        else k_adj_collide<double, false, false><<<a.nblk, 256, 0, s>>>(a, j);
+    }
     return 0;
 }
-int launch_adj_collide_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {
-    if (fp32) k_adj_collide<float, true><<<a.nblk, 256, 0, s>>>(a, j);
-    else k_adj_collide<double, true><<<a.nblk, 256, 0, s>>>(a, j);
+int launch_adj_collide_flow(int fp32, int grouped, const StepArgs& a, const AdjArgs& j,
+                            hipStream_t s) {
+    if (grouped) {
+        if (fp32) k_adj_collide<float, true, true><<<a.nblk, 256, 0, s>>>(a, j);
+        else k_adj_collide<double, true, true><<<a.nblk, 256, 0, s>>>(a, j);
+    } else {
+        if (fp32) k_adj_collide<float, true, false><<<a.nblk, 256, 0, s>>>(a, j);
+        else k_adj_collide<double, true, false><<<a.nblk, 256, 0, s>>>(a, j);
+    }
+    return 0;
+}
+int launch_adj_qsum(int fp32, const StepArgs& a, const void* Q, void* nout, hipStream_t s) {
+    if (fp32) k_adj_qsum<float><<<a.nblk, 256, 0, s>>>(a, (const float*)Q, (float*)nout);
+    else k_adj_qsum<double><<<a.nblk, 256, 0, s>>>(a, (const double*)Q, (double*)nout);
     return 0;
 }
 int launch_cadj_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s) {

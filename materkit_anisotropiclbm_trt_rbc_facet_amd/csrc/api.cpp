@@ -38,6 +38,13 @@ enum { LF = 0, LG = 1, LH = 2, LC = 3 };
 // boxes): pad 0 gives 0.689 of HBM peak, 1024 gives 0.701-0.707 (256: 0.700, 512: 0.692,
 // 2048: 0.696, 4096: 0.693).
 constexpr uint32_t kSlotPad = 1024;
+// Cells per page of the paged population layout (2^kPageShift).  A 45-slot in-place update at
+// 256^3 (FP32) streams at 6.6 TB/s with pages of 2048-16384 cells holding every slot of every
+// lattice, 5.7-5.8 TB/s with one array per slot at any pad, 6.3 with pages of 1024 cells and
+// 6.3 with 2048-cell pages per lattice.  The fused step at 256^3 measured fastest with pages of
+// 8192 cells (k_step 1.011 ms; 1024: 1.033, 2048: 1.027, 4096: 1.020, 16384: 1.025; one array
+// per slot: 1.074).
+constexpr uint32_t kPageShift = 13;
 
 }  // namespace
 
@@ -54,7 +61,15 @@ struct mkg_ctx {
     int zstage = 0;  // O steps take the z shift through LDS (blocks hold whole z rows)
     int dev = 0;
     hipStream_t stream = nullptr;
+    // Every lattice's populations and face buffers in one allocation (popbuf, pop_total
+    // elements); pop[L] is slot 0 of lattice L and face[L] its face buffer.  Paged or
+    // slot-major addressing (pg_*, sl_stride; StepArgs).
+    void* popbuf = nullptr;
+    size_t pop_total = 0;
+    int paged = 0;
+    uint32_t pg_shift = 31, pg_mask = 0x7FFFFFFFu, pg_stride = 0, sl_stride = 0;
     void* pop[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* face[4] = {nullptr, nullptr, nullptr, nullptr};
     int q[4] = {0, 0, 0, 0};
     int lay[4] = {0, 0, 0, 0};
     int64_t steps[4] = {0, 0, 0, 0};
@@ -72,8 +87,6 @@ struct mkg_ctx {
     int timing = 0;
     std::vector<hipEvent_t> events;
     double t_last[4] = {0, 0, 0, 0};
-    // elements of one lattice's allocation: Q slots of S plus Q face-buffer slots of F
-    size_t pop_elems(int L) const { return (size_t)q[L] * ((size_t)S + F); }
 };
 
 namespace {
@@ -109,6 +122,7 @@ void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const doub
               LatArgs& out) {
     std::memset(&out, 0, sizeof(out));
     out.pop = h->pop[L];
+    out.face = h->face[L];
     int np = 0;
     for (int f = 0; f < 6; f++) {
         out.bc[f] = bc[f];
@@ -130,6 +144,10 @@ StepArgs make_args(const mkg_ctx* h) {
     a.nz = p.nz;
     a.N = h->N;
     a.S = h->S;
+    a.pg_shift = h->pg_shift;
+    a.pg_mask = h->pg_mask;
+    a.pg_stride = h->pg_stride;
+    a.sl_stride = h->sl_stride;
     a.nblk = h->nblk;
     a.F = h->F;
     a.zstage = h->zstage;
@@ -202,6 +220,53 @@ bool has_velocity_faces(const mkg_params& p) {
 int ensure_nee(mkg_ctx* h, const mkg_params& p) {
     if (h->nee || !p.flow_q || !has_velocity_faces(p)) return 0;
     HIPCHK(hipMalloc(&h->nee, (size_t)p.flow_q * h->F * h->esz));
+    return 0;
+}
+
+// The population block of a handle.  Paged when the block stays below 4 GiB (the kernels
+// address a lattice with 32-bit byte offsets from its slot 0), slot-major otherwise or when
+// MKG_PAGES=0 (MKG_PAGE_SHIFT sets the page size).  Face buffers follow the populations,
+// lattice by lattice.
+int alloc_pops(mkg_ctx* h) {
+    uint32_t shift = kPageShift;
+    if (const char* env = std::getenv("MKG_PAGE_SHIFT")) {
+        const int v = std::atoi(env);
+        if (v >= 8 && v <= 16) shift = (uint32_t)v;
+    }
+    const uint32_t B = 1u << shift;
+    size_t qt = 0;
+    for (int L = 0; L < 4; L++) qt += (size_t)h->q[L];
+    if (qt == 0) return 0;
+    const size_t npages = ((size_t)h->N + B - 1) / B;
+    const size_t paged_total = npages * qt * B + qt * h->F;
+    bool paged = paged_total * h->esz < ((size_t)1 << 32);
+    if (const char* env = std::getenv("MKG_PAGES")) paged = paged && std::atoi(env) != 0;
+    size_t slots, off = 0, foff;
+    if (paged) {
+        h->paged = 1;
+        h->pg_shift = shift;
+        h->pg_mask = B - 1u;
+        h->pg_stride = (uint32_t)(qt * B);
+        h->sl_stride = B;
+        slots = npages * qt * B;
+    } else {
+        h->paged = 0;
+        h->pg_shift = 31;
+        h->pg_mask = 0x7FFFFFFFu;
+        h->pg_stride = 0;
+        h->sl_stride = h->S;
+        slots = qt * h->S;
+    }
+    h->pop_total = slots + qt * h->F;
+    if (alloc_zero(&h->popbuf, h->pop_total * h->esz, h->stream)) return -1;
+    foff = slots;
+    for (int L = 0; L < 4; L++) {
+        if (!h->q[L]) continue;
+        h->pop[L] = (char*)h->popbuf + off * h->sl_stride * h->esz;
+        h->face[L] = (char*)h->popbuf + foff * h->esz;
+        off += (size_t)h->q[L];
+        foff += (size_t)h->q[L] * h->F;
+    }
     return 0;
 }
 
@@ -457,8 +522,7 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
         std::string err;
         if (noalloc_prepare(h->dev, &err)) return cleanup(fail(err));
     }
-    for (int L = 0; L < 4; L++)
-        if (alloc_zero(&h->pop[L], h->pop_elems(L) * h->esz, h->stream)) return cleanup(-1);
+    if (alloc_pops(h)) return cleanup(-1);
     if (p->phase_q) {
         for (int b = 0; b < 2; b++)
             if (alloc_zero(&h->phi[b], (size_t)h->S * h->esz, h->stream)) return cleanup(-1);
@@ -494,7 +558,7 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
 int mkg_destroy(mkg_handle h) {
     if (!h) return 0;
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (int L = 0; L < 4; L++) hipFree(h->pop[L]);
+    hipFree(h->popbuf);
     hipFree(h->phi[0]);
     hipFree(h->phi[1]);
     hipFree(h->flag);
@@ -766,15 +830,11 @@ int mkg_reduce(mkg_handle h, int which, double* out) {
     const size_t N = h->N;
     if (which == MKG_R_NONFINITE) {
         double total = 0;
-        for (int L = 0; L < 4; L++) {
-            if (!h->pop[L]) continue;
-            launch_nonfinite(h->fp32, h->pop[L], (size_t)h->q[L] * h->S, h->partials, h->red_out,
-                             h->stream);
+        if (h->popbuf) {
+            launch_nonfinite(h->fp32, h->popbuf, h->pop_total, h->partials, h->red_out, h->stream);
             HIPCHK(hipGetLastError());
-            double v = 0;
-            HIPCHK(hipMemcpyAsync(&v, h->red_out, 8, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipMemcpyAsync(&total, h->red_out, 8, hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
-            total += v;
         }
         out[0] = total;
         return 0;
@@ -813,11 +873,10 @@ int mkg_reduce_noalloc(mkg_handle h, int which, double* out) {
     void* pops[4];
     size_t sizes[4];
     int npops = 0;
-    for (int L = 0; L < 4; L++)
-        if (h->pop[L]) {
-            pops[npops] = h->pop[L];
-            sizes[npops++] = (size_t)h->q[L] * h->S;
-        }
+    if (h->popbuf) {
+        pops[npops] = h->popbuf;
+        sizes[npops++] = h->pop_total;
+    }
     // cell values needed: flow (with the scalars), the scalars alone, or the phase field
     const int macro = need[0] ? 2 : ((need[1] || need[2]) ? 1 : 0);
     double v[4] = {0, 0, 0, 0};
@@ -834,7 +893,7 @@ int mkg_reduce_noalloc(mkg_handle h, int which, double* out) {
 }
 
 // ---------------------------------------------------------------- checkpoints
-static const char kMagic[8] = {'M', 'K', 'G', 'C', 'K', 'P', 'T', '1'};
+static const char kMagic[8] = {'M', 'K', 'G', 'C', 'K', 'P', 'T', '2'};
 
 int mkg_checkpoint_save(mkg_handle h, const char* path) {
     if (!h || !path) return fail("null argument");
@@ -842,7 +901,9 @@ int mkg_checkpoint_save(mkg_handle h, const char* path) {
     if (!fp) return fail(std::string("cannot open ") + path);
     HIPCHK(hipStreamSynchronize(h->stream));
     auto wr = [&](const void* p, size_t n) { return std::fwrite(p, 1, n, fp) == n; };
-    uint64_t hdr[4] = {MKG_ABI_VERSION, sizeof(mkg_params), h->N, h->esz};
+    // the population block is stored as laid out in memory: the header names its layout
+    uint64_t hdr[6] = {MKG_ABI_VERSION, sizeof(mkg_params), h->N, h->esz, h->pop_total,
+                       ((uint64_t)h->paged << 32) | h->pg_shift};
     bool ok = wr(kMagic, 8) && wr(hdr, sizeof(hdr)) && wr(&h->p, sizeof(h->p)) &&
               wr(h->lay, sizeof(h->lay)) && wr(h->steps, sizeof(h->steps)) &&
               wr(&h->pf_count, sizeof(h->pf_count));
@@ -853,7 +914,7 @@ int mkg_checkpoint_save(mkg_handle h, const char* path) {
         if (hipMemcpy(buf.data(), dptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return false;
         return wr(buf.data(), bytes);
     };
-    for (int L = 0; L < 4 && ok; L++) ok = dump(h->pop[L], h->pop_elems(L) * h->esz);
+    if (ok) ok = dump(h->popbuf, h->pop_total * h->esz);
     if (ok && h->q[LG]) ok = dump(h->phi[0], h->S * h->esz) && dump(h->phi[1], h->S * h->esz);
     if (ok) ok = dump(h->flag, h->S) && dump(h->grain, (size_t)h->S * 2);
     if (ok && h->kappa) ok = dump(h->kappa, h->S * h->esz);
@@ -868,13 +929,14 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
     if (!fp) return fail(std::string("cannot open ") + path);
     auto rd = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
     char magic[8];
-    uint64_t hdr[4];
+    uint64_t hdr[6];
     mkg_params p;
     int lay[4];
     int64_t steps[4], pfc;
     bool ok = rd(magic, 8) && std::memcmp(magic, kMagic, 8) == 0 && rd(hdr, sizeof(hdr)) &&
               hdr[0] == MKG_ABI_VERSION && hdr[1] == sizeof(mkg_params) && hdr[2] == h->N &&
-              hdr[3] == h->esz && rd(&p, sizeof(p)) && rd(lay, sizeof(lay)) &&
+              hdr[3] == h->esz && hdr[4] == h->pop_total &&
+              hdr[5] == (((uint64_t)h->paged << 32) | h->pg_shift) && rd(&p, sizeof(p)) && rd(lay, sizeof(lay)) &&
               rd(steps, sizeof(steps)) && rd(&pfc, sizeof(pfc));
     if (!ok) {
         std::fclose(fp);
@@ -897,7 +959,7 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
         if (!rd(buf.data(), bytes)) return false;
         return hipMemcpy(dptr, buf.data(), bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    for (int L = 0; L < 4 && ok; L++) ok = load(h->pop[L], h->pop_elems(L) * h->esz);
+    if (ok) ok = load(h->popbuf, h->pop_total * h->esz);
     if (ok && h->q[LG]) ok = load(h->phi[0], h->S * h->esz) && load(h->phi[1], h->S * h->esz);
     // the flags decide which kernel bodies run (wall cells need the body with face rules)
     bool walls = false;
@@ -1037,8 +1099,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         int64_t pf, steps_g, steps_h, steps_f;
     };
     std::vector<Ck> ck(tape ? 1 : nseg + 1);
-    const size_t PG = h->pop_elems(LG), PH = h->pop_elems(LH), PF = flow ? h->pop_elems(LF) : 0;
-    const size_t ck_elems = PG + PH + 2 * S + PF;
+    // a checkpoint: every population of the handle (the phase, thermal and, in the coupled
+    // model, flow lattices) and both phase buffers
+    const size_t PP = h->pop_total;
+    const size_t ck_elems = PP + 2 * S;
     // flow tape of one step (coupled model): the canonical pre-collision populations, FP64
     const size_t fseg_stride = flow ? (size_t)19 * N : 0;
     // tape of one step, written by the forward kernel: phi^n, then the 7 pre-collision thermal
@@ -1117,26 +1181,18 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
 
     auto save_ck = [&](int k) -> int {
         char* base = ckbuf + (size_t)k * ck_elems * e;
-        HIPCHK(hipMemcpyAsync(base, h->pop[LG], PG * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + PG * e, h->pop[LH], PH * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + (PG + PH) * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(base + (PG + PH + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
-        if (flow)
-            HIPCHK(hipMemcpyAsync(base + (PG + PH + 2 * S) * e, h->pop[LF], PF * e,
-                                  hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base, h->popbuf, PP * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + PP * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + (PP + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
         ck[k] = {h->lay[LG], h->lay[LH], h->lay[LF], h->pf_count, h->steps[LG], h->steps[LH],
                  h->steps[LF]};
         return 0;
     };
     auto load_ck = [&](int k) -> int {
         char* base = ckbuf + (size_t)k * ck_elems * e;
-        HIPCHK(hipMemcpyAsync(h->pop[LG], base, PG * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->pop[LH], base + PG * e, PH * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->phi[0], base + (PG + PH) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(h->phi[1], base + (PG + PH + S) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
-        if (flow)
-            HIPCHK(hipMemcpyAsync(h->pop[LF], base + (PG + PH + 2 * S) * e, PF * e,
-                                  hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->popbuf, base, PP * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[0], base + PP * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->phi[1], base + (PP + S) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
         h->lay[LG] = ck[k].lay_g;
         h->lay[LH] = ck[k].lay_h;
         h->pf_count = ck[k].pf;
@@ -1197,6 +1253,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     // tape is refilled only after the reverse that read it has finished.  The reverse kernels
     // read geometry, faces, rates, kappa and flags, never the handle's populations.
     const StepArgs ar = make_args(h);
+    // stencil transpose by column sums when a block holds whole z rows (MKG_ADJ_GROUPED=0
+    // selects the per-direction form)
+    int grouped = h->zstage;
+    if (const char* env = std::getenv("MKG_ADJ_GROUPED")) grouped = grouped && std::atoi(env) != 0;
     ADJCHK(hipEventRecord(ev[2], h->stream));
     if (!tape && recompute(nseg - 1)) rc = -1;
     ADJCHK(hipEventRecord(ev[3], h->stream));
@@ -1217,23 +1277,23 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
             j.gin = Gin; j.nin = nin; j.hin = Hin; j.pin = pin;
             j.gout = Gout; j.hout = Hout; j.pout = pout;
             j.P = Pb; j.kap_grad = kap; j.phin = slot;
-            j.hsave = ar.h;
-            j.hsave.pop = slot + S * e;
-            j.hsave.lay = 0;
+            j.htape = slot + S * e;
             j.eps_part = eps_part;
             if (flow) {
                 // scalars and phase with advection, stencil transpose, then the flow
                 j.ftape = fseg_base(sgi) + (size_t)(n - n0) * fseg_stride;
                 j.fin = Fin; j.fout = Fout; j.ut = Ub;
                 j.hcw_in = Cin; j.hcw_out = Cout; j.nacc = nout;
-                launch_adj_collide_flow(h->fp32, ar, j, rs);
-                launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
+                launch_adj_collide_flow(h->fp32, grouped, ar, j, rs);
+                if (grouped) launch_adj_qsum(h->fp32, ar, Pb, nout, rs);
+                else launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
                 launch_cadj_flow(h->fp32, ar, j, rs);
                 std::swap(Fin, Fout);
                 std::swap(Cin, Cout);
             } else {
-                launch_adj_collide(h->fp32, ar, j, rs);
-                launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
+                launch_adj_collide(h->fp32, grouped, ar, j, rs);
+                if (grouped) launch_adj_qsum(h->fp32, ar, Pb, nout, rs);
+                else launch_adj_stencil(h->fp32, ar, Pb, nout, rs);
             }
             std::swap(Gin, Gout);
             std::swap(nin, nout);

@@ -16,9 +16,9 @@ template <class R, int Q>
 __device__ inline void read_any(R* p, const LatArgs& L, const StepArgs& a, const Cell& c,
                                 int kind, bool wall) {
     if (wall)
-        read_pops<Q, false, R>(p, L, a.S, c, a, kind);
+        read_pops<Q, false, R>(p, L, c, a, kind);
     else
-        read_pops_rt<Q, R>(p, L, a.S, c, a, kind);
+        read_pops_rt<Q, R>(p, L, c, a, kind);
 }
 
 // Macroscopic values of one cell from its populations: T = sum h, C = sum c, rho = sum f and

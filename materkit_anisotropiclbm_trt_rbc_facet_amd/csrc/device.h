@@ -215,6 +215,34 @@ __device__ inline const R* slot_at(const R* slot, uint32_t idx) {
     return (const R*)((const char*)slot + idx * (uint32_t)sizeof(R));
 }
 
+// Population addressing (StepArgs): the element offset of cell c within a slot, and slot i
+// of a lattice.  In the slot-major layout pcell(c) = c.
+__device__ inline uint32_t pcell(const StepArgs& a, uint32_t c) {
+    return (c >> a.pg_shift) * a.pg_stride + (c & a.pg_mask);
+}
+template <class R>
+__device__ inline R* pslot(R* base, int i, const StepArgs& a) {
+    return base + (size_t)i * a.sl_stride;
+}
+template <class R>
+__device__ inline const R* pslot(const R* base, int i, const StepArgs& a) {
+    return base + (size_t)i * a.sl_stride;
+}
+// Population i of cell c.
+template <class R>
+__device__ inline R* pop_at(R* base, int i, uint32_t c, const StepArgs& a) {
+    return slot_at(pslot(base, i, a), pcell(a, c));
+}
+template <class R>
+__device__ inline const R* pop_at(const R* base, int i, uint32_t c, const StepArgs& a) {
+    return slot_at(pslot(base, i, a), pcell(a, c));
+}
+// Face-buffer value of slot i for face cell k.
+template <class R>
+__device__ inline R* face_at(const LatArgs& L, int i, uint32_t F, uint32_t k) {
+    return (R*)L.face + (size_t)i * F + k;
+}
+
 // Position of a face cell in the face buffer: faces x-, x+ (ny*nz cells each, index
 // y*nz + z), then y-, y+ (x*nz + z), then z-, z+ (x*ny + y).
 __device__ inline uint32_t face_slot(const StepArgs& a, const Cell& c, int f) {
@@ -279,13 +307,13 @@ __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const St
 // cells lie on z faces, in a domain without wall cells, takes FM = ZLO | ZHI and carries no
 // x / y face or wall logic.
 template <int Q, bool ODD, class R, bool FACES = true, uint32_t FM = kAllFaces>
-__device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline void read_pops(R* p, const LatArgs& L, const Cell& c,
                                  const StepArgs& a, int kind) {
     constexpr bool WALLS = FACES && FM == kAllFaces;
     const R* __restrict__ base = (const R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
+        for (int i = 0; i < Q; i++) p[i] = ldp(pop_at(base, i, c.id, a));
     } else {
         (void)kind;
 #pragma unroll
@@ -295,26 +323,26 @@ __device__ inline void read_pops(R* p, const LatArgs& L, uint32_t S, const Cell&
             if (hit == 0u) {
                 uint32_t src = nbr(c, dx, dy, dz);
                 if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls)
-                    p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
+                    p[i] = ldp(pop_at(base, i, c.id, a));
                 else
-                    p[i] = ldp(slot_at(base + (size_t)inv(i) * S, src));
+                    p[i] = ldp(pop_at(base, inv(i), src, a));
             } else {
                 const int face = __builtin_ctz(hit);
-                p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, face)]);
+                p[i] = ldp(face_at<R>(L, i, a.F, face_slot(a, c, face)));
             }
         }
     }
 }
 
 template <int Q, bool ODD, class R, bool FACES = true, uint32_t FM = kAllFaces>
-__device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline void write_pops(const R* p, const LatArgs& L, const Cell& c,
                                   const StepArgs& a, int kind,
                                   const FaceState<R>& fs = FaceState<R>()) {
     constexpr bool WALLS = FACES && FM == kAllFaces;
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
+        for (int i = 0; i < Q; i++) stp(pop_at(base, inv(i), c.id, a), p[i]);
         if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
 #pragma unroll
             for (int i = 1; i < Q; i++) {
@@ -322,7 +350,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
                     c.bits & faces_of(-cx(i), -cy(i), -cz(i)) & FM & (uint32_t)L.npmask;
                 if (hit) {
                     const int f = __builtin_ctz(hit);
-                    stp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, f)],
+                    stp(face_at<R>(L, i, a.F, face_slot(a, c, f)),
                         face_in<Q, R>(p, i, f, L, a, kind, fs, c));
                 }
             }
@@ -335,12 +363,12 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, uint32_t S, cons
             if (hit == 0u) {
                 uint32_t tgt = nbr(c, dx, dy, dz);
                 if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
-                    stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
+                    stp(pop_at(base, inv(i), c.id, a), p[i]);
                 else
-                    stp(slot_at(base + (size_t)i * S, tgt), p[i]);
+                    stp(pop_at(base, i, tgt, a), p[i]);
             } else {
                 const int face = __builtin_ctz(hit);
-                stp(slot_at(base + (size_t)inv(i) * S, c.id),
+                stp(pop_at(base, inv(i), c.id, a),
                     face_in<Q, R>(p, inv(i), face, L, a, kind, fs, c));
             }
         }
@@ -383,7 +411,7 @@ __host__ __device__ constexpr int nzdirs() {
 // follows its last read.  Global memory needs no ordering: each (cell, slot) is loaded and
 // stored by one thread.
 template <int Q, class R, bool FACES = true, uint32_t FM = kAllFaces>
-__device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, const Cell& c,
                                        const StepArgs& a, R (*zv)[256], uint32_t tid) {
     constexpr bool WALLS = FACES && FM == kAllFaces;
     constexpr int NZ = nzdirs<Q>();
@@ -393,7 +421,7 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const
 #pragma unroll
     for (int i = 1; i < Q; i++)
         if (cz(i) != 0)
-            t[zslot<Q>(i)] = ldp(slot_at(base + (size_t)inv(i) * S, nbr(c, -cx(i), -cy(i), 0)));
+            t[zslot<Q>(i)] = ldp(pop_at(base, inv(i), nbr(c, -cx(i), -cy(i), 0), a));
     uint32_t own = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -401,15 +429,15 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, uint32_t S, const
         const uint32_t hit =
             FACES ? c.bits & faces_of(dx, dy, dz) & FM & (uint32_t)L.npmask : 0u;
         if (hit) {
-            p[i] = ldp(&base[(size_t)Q * S + (size_t)i * a.F + face_slot(a, c, __builtin_ctz(hit))]);
+            p[i] = ldp(face_at<R>(L, i, a.F, face_slot(a, c, __builtin_ctz(hit))));
             if (dz != 0) own |= 1u << zslot<Q>(i);
         } else {
             const uint32_t src = nbr(c, dx, dy, dz);
             if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
-                p[i] = ldp(slot_at(base + (size_t)i * S, c.id));
+                p[i] = ldp(pop_at(base, i, c.id, a));
                 if (dz != 0) own |= 1u << zslot<Q>(i);
             } else if (dz == 0) {
-                p[i] = ldp(slot_at(base + (size_t)inv(i) * S, src));
+                p[i] = ldp(pop_at(base, inv(i), src, a));
             }
         }
     }
@@ -430,7 +458,7 @@ __device__ inline void zpull_finish(R* p, const Cell& c, const StepArgs& a, R (*
 }
 
 template <int Q, class R, bool FACES = true, uint32_t FM = kAllFaces>
-__device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
                                    const StepArgs& a, int kind, R (*zv)[256], uint32_t* zm,
                                    uint32_t tid, bool act, const FaceState<R>& fs) {
     constexpr bool WALLS = FACES && FM == kAllFaces;
@@ -445,14 +473,14 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, con
         if (dz != 0) zv[zslot<Q>(i)][tid] = p[i];
         if (hit) {
             if (act)
-                stp(slot_at(base + (size_t)inv(i) * S, c.id),
+                stp(pop_at(base, inv(i), c.id, a),
                     face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs, c));
         } else {
             const uint32_t tgt = nbr(c, dx, dy, dz);
             if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
-                if (act) stp(slot_at(base + (size_t)inv(i) * S, c.id), p[i]);
+                if (act) stp(pop_at(base, inv(i), c.id, a), p[i]);
             } else if (dz == 0) {
-                if (act) stp(slot_at(base + (size_t)i * S, tgt), p[i]);
+                if (act) stp(pop_at(base, i, tgt, a), p[i]);
             } else {
                 m |= 1u << zslot<Q>(i);
             }
@@ -465,7 +493,7 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, uint32_t S, con
 // FACES false: the block has no face cells, no walls and no thread past the last cell, so
 // every sender sent every z population and the masks are not read.
 template <int Q, class R, bool FACES = true>
-__device__ inline void zpush_finish(const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline void zpush_finish(const LatArgs& L, const Cell& c,
                                     const StepArgs& a, R (*zv)[256], const uint32_t* zm,
                                     uint32_t tid) {
     R* __restrict__ base = (R*)L.pop;
@@ -477,19 +505,19 @@ __device__ inline void zpush_finish(const LatArgs& L, uint32_t S, const Cell& c,
             // sender: the cell (x, y, z - c_z) of this thread's row
             const uint32_t s = rb + ((uint32_t)(c.z - cz(i)) & zm1);
             if (!FACES || ((zm[s] >> zslot<Q>(i)) & 1u))
-                stp(slot_at(base + (size_t)i * S, nbr(c, cx(i), cy(i), 0)), zv[zslot<Q>(i)][s]);
+                stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a), zv[zslot<Q>(i)][s]);
         }
     }
 }
 
 // Read with a runtime layout (lattices used only for coupling in this launch).
 template <int Q, class R>
-__device__ inline void read_pops_rt(R* p, const LatArgs& L, uint32_t S, const Cell& c,
+__device__ inline void read_pops_rt(R* p, const LatArgs& L, const Cell& c,
                                     const StepArgs& a, int kind) {
     if (L.lay)
-        read_pops<Q, true, R>(p, L, S, c, a, kind);
+        read_pops<Q, true, R>(p, L, c, a, kind);
     else
-        read_pops<Q, false, R>(p, L, S, c, a, kind);
+        read_pops<Q, false, R>(p, L, c, a, kind);
 }
 
 // ------------------------------------------------------------------ flow collision

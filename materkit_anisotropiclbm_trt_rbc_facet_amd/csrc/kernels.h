@@ -10,8 +10,10 @@ namespace mkg {
 
 // One population lattice as seen by a kernel.
 struct LatArgs {
-    void* pop;        // Q * (S + F) elements of the storage type: slot i at pop + i*S, then
-                      // the face buffer, slot i at pop + Q*S + i*F (see face_slot)
+    void* pop;        // slot 0 of the lattice; cell c of slot i at pop + i * sl_stride
+                      // + pcell(c) (StepArgs: population addressing)
+    void* face;       // face buffer, Q x F elements: slot i of face cell k at face + i*F + k
+                      // (see face_slot)
     int32_t bc[6];    // face codes x-, x+, y-, y+, z-, z+
     int32_t npmask;   // bit f set when face f is not periodic
     int32_t lay;      // current AA layout: 0 = own slots hold f_i, 1 = shifted
@@ -24,7 +26,14 @@ struct LatArgs {
 struct StepArgs {
     int32_t nx, ny, nz;
     uint32_t N;       // number of cells
-    uint32_t S;       // slot stride (N rounded up)
+    uint32_t S;       // stride of the per-cell arrays (phi, flags, kappa): N rounded up
+    // Population addressing: cell c of slot i of a lattice is at
+    //   L.pop + i * sl_stride + (c >> pg_shift) * pg_stride + (c & pg_mask).
+    // Paged layout: pages of 2^pg_shift cells; a page holds every slot of every lattice of
+    // the handle, slot by slot (sl_stride = page cells, pg_stride = all slots x page cells).
+    // Slot-major layout: one array per slot (pg_shift = 31, pg_mask = 2^31 - 1,
+    // sl_stride = S).
+    uint32_t pg_shift, pg_mask, pg_stride, sl_stride;
     uint32_t nblk;    // blocks in the launch (XCD-aware remap when divisible by 8)
     uint32_t F;       // face-buffer stride: cells on the six faces, 2 (ny nz + nx nz + nx ny)
     int32_t has_walls, has_kappa, ngrains, pf_coupling;
@@ -58,6 +67,19 @@ struct StepArgs {
     // pre-collision thermal populations at tape[(1 + k) * S + cell]; null: no tape
     void* tape;
 };
+
+// Slot-major population addressing: one array per slot, stride S (pcell(c) = c).
+inline void slot_major(StepArgs& a) {
+    a.pg_shift = 31;
+    a.pg_mask = 0x7FFFFFFFu;
+    a.pg_stride = 0;
+    a.sl_stride = a.S;
+}
+// A slot-major lattice allocation: Q slots of S elements, then its Q x F face buffer.
+inline void slot_major_lattice(LatArgs& L, void* pop, int q, uint32_t S, size_t esz) {
+    L.pop = pop;
+    L.face = pop ? (char*)pop + (size_t)q * S * esz : nullptr;
+}
 
 // Selector of a fused-step instantiation: presence of each lattice.
 struct Combo {
@@ -120,10 +142,10 @@ struct AdjArgs {
     void* gout;
     void* hout;
     void* pout;
-    void* P;             // 19 x N stencil contributions
+    void* P;             // stencil contributions: 19 x N per direction, or 9 x N column sums
     void* kap_grad;      // N accumulated dL/dkappa (storage type)
     const void* phin;    // N: phi^n of the forward step being reversed
-    LatArgs hsave;       // forward thermal populations at level n (raw AA buffer + layout)
+    const void* htape;   // pre-collision thermal populations of level n: slot k at htape + k*S
     double* eps_part;    // per-block FP64 partials of dL/deps4 (accumulated)
     // coupled model (flow + thermal + phase):
     const double* ftape; // Q x N canonical flow populations at level n (full values, FP64)
@@ -135,12 +157,18 @@ struct AdjArgs {
     void* nacc;          // N cell-wide phase adjoint of level n: the flow kernel adds its part
 };
 
-int launch_adj_collide(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
+// grouped: the launch's blocks hold whole z rows; the stencil contributions leave as nine
+// column sums per cell for launch_adj_qsum (else 19 per-direction values for
+// launch_adj_stencil).
+int launch_adj_collide(int fp32, int grouped, const StepArgs& a, const AdjArgs& j,
+                       hipStream_t s);
 // Coupled model: the scalar/phase reverse with the advection velocity (flow 1), and the flow
 // reverse with partial bounce-back, buoyancy and the scalars' velocity adjoint.
-int launch_adj_collide_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
+int launch_adj_collide_flow(int fp32, int grouped, const StepArgs& a, const AdjArgs& j,
+                            hipStream_t s);
 int launch_cadj_flow(int fp32, const StepArgs& a, const AdjArgs& j, hipStream_t s);
 int launch_adj_stencil(int fp32, const StepArgs& a, const void* P, void* nout, hipStream_t s);
+int launch_adj_qsum(int fp32, const StepArgs& a, const void* Q, void* nout, hipStream_t s);
 int launch_adj_loss(int fp32, const StepArgs& a, const double* target, void* nseed, double* part,
                     double* out, hipStream_t s);
 int launch_sum_partials(const double* part, uint32_t n, double* out, hipStream_t s);

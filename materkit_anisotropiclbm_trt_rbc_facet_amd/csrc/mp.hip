@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_mp_phi(const MpArgs m) {
     Cell c;
     if (!locate(m.a, c)) return;
     R h[19];
-    read_pops<19, ODD, R>(h, m.a.g, m.a.S, c, m.a, kKindScalar);
+    read_pops<19, ODD, R>(h, m.a.g, c, m.a, kKindScalar);
     R s = 0;
 #pragma unroll
     for (int i = 0; i < 19; i++) s += h[i];
@@ -194,11 +194,11 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     R f[19];
     uint32_t own_f = 0;
     if constexpr (Z) {
-        own_f = zpull_issue<19, R, FACES>(f, a.f, S, c, a, zrf, tid);
+        own_f = zpull_issue<19, R, FACES>(f, a.f, c, a, zrf, tid);
         __syncthreads();  // B1
         zpull_finish<19, R>(f, c, a, zrf, tid, own_f);
     } else {
-        read_pops<19, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow);
+        read_pops<19, ODD, R, FACES>(f, a.f, c, a, kKindFlow);
     }
     R rho, u[3], p, F[3], gr[3];
     mp_macro<R>(m, f, ph, muc, g, rho, u, p, F, gr);
@@ -220,9 +220,9 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     }
     FaceState<R> fs;
     if constexpr (Z)
-        zpush_stage<19, R, FACES>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+        zpush_stage<19, R, FACES>(f, a.f, c, a, kKindFlow, zwf, zmf, tid, act, fs);
     else
-        write_pops<19, ODD, R, FACES>(f, a.f, S, c, a, kKindFlow, fs);
+        write_pops<19, ODD, R, FACES>(f, a.f, c, a, kKindFlow, fs);
 
     // Cahn-Hilliard lattice
     R* pu = (R*)m.phiu;
@@ -237,12 +237,12 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
     R h[19];
     uint32_t own_h = 0;
     if constexpr (Z) {
-        own_h = zpull_issue<19, R, FACES>(h, a.g, S, c, a, zrh, tid);
+        own_h = zpull_issue<19, R, FACES>(h, a.g, c, a, zrh, tid);
         __syncthreads();  // B2
-        zpush_finish<19, R, FACES>(a.f, S, c, a, zwf, zmf, tid);
+        zpush_finish<19, R, FACES>(a.f, c, a, zwf, zmf, tid);
         zpull_finish<19, R>(h, c, a, zrh, tid, own_h);
     } else {
-        read_pops<19, ODD, R, FACES>(h, a.g, S, c, a, kKindScalar);
+        read_pops<19, ODD, R, FACES>(h, a.g, c, a, kKindScalar);
     }
     const R ith = (R)(1.0 / m.tau_h), kh = (R)(1.0 - 0.5 / m.tau_h);
     const R em = (R)m.eta * muc;
@@ -255,12 +255,12 @@ __global__ void __launch_bounds__(256) k_mp_step(const MpArgs m) {
         h[i] = h[i] - ith * (h[i] - heq) + kh * w * (R)3 * cd;
     }
     if constexpr (Z) {
-        zpush_stage<19, R, FACES>(h, a.g, S, c, a, kKindScalar, zrf, zmh, tid, act,
+        zpush_stage<19, R, FACES>(h, a.g, c, a, kKindScalar, zrf, zmh, tid, act,
                                   FaceState<R>());
         __syncthreads();  // B3
-        zpush_finish<19, R, FACES>(a.g, S, c, a, zrf, zmh, tid);
+        zpush_finish<19, R, FACES>(a.g, c, a, zrf, zmh, tid);
     } else {
-        write_pops<19, ODD, R, FACES>(h, a.g, S, c, a, kKindScalar);
+        write_pops<19, ODD, R, FACES>(h, a.g, c, a, kKindScalar);
     }
     };
     if (slow)
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) k_mp_macro(const MpArgs m, double* p_out,
     R g[3];
     mp_gradient<R>(P, g);
     R f[19];
-    read_pops_rt<19, R>(f, m.a.f, m.a.S, c, m.a, kKindFlow);
+    read_pops_rt<19, R>(f, m.a.f, c, m.a, kKindFlow);
     R rho, u[3], p, F[3], gr[3];
     mp_macro<R>(m, f, P[0], *slot_at((const R*)m.mu, c.id), g, rho, u, p, F, gr);
     p_out[c.id] = (double)p;
@@ -384,8 +384,9 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     a.f.npmask = a.g.npmask = np;
     a.f.lay = a.g.lay = h->lay;
     a.f.adv = a.g.adv = 1;
-    a.f.pop = h->f;
-    a.g.pop = h->h;
+    slot_major(a);
+    slot_major_lattice(a.f, h->f, 19, h->S, h->esz);
+    slot_major_lattice(a.g, h->h, 19, h->S, h->esz);
     m.rho_l = p.rho_l;
     m.rho_g = p.rho_g;
     m.nu_l = p.nu_l;

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_mpf_moments(const MpfArgs m, R* __restr
         if (j < m.n) {
             const LatArgs& L = m.gl[j];  // j is unrolled: a fixed offset, no copy
             R g[19];
-            read_pops<19, ODD, R>(g, L, S, c, m.a, kKindScalar);
+            read_pops<19, ODD, R>(g, L, c, m.a, kKindScalar);
             R s = 0;
 #pragma unroll
             for (int i = 0; i < 19; i++) s += g[i];
@@ -212,15 +212,15 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F
     uint32_t own_h = 0, own_c = 0;
     if constexpr (TH) {
         if constexpr (Z)
-            own_h = zpull_issue<7, R, FACES>(h, a.h, S, c, a, zrh, tid);
+            own_h = zpull_issue<7, R, FACES>(h, a.h, c, a, zrh, tid);
         else
-            read_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES>(h, a.h, c, a, kKindScalar);
     }
     if constexpr (SO) {
         if constexpr (Z)
-            own_c = zpull_issue<7, R, FACES>(cc, a.c, S, c, a, zrc, tid);
+            own_c = zpull_issue<7, R, FACES>(cc, a.c, c, a, zrc, tid);
         else
-            read_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES>(cc, a.c, c, a, kKindScalar);
     }
     if constexpr (Z && (TH || SO)) {
         __syncthreads();
@@ -241,14 +241,14 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F
             h[i] = h[i] - omh * (h[i] - w * sh) + w * srh;
         }
         if constexpr (Z)
-            zpush_stage<7, R, FACES>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+            zpush_stage<7, R, FACES>(h, a.h, c, a, kKindScalar, zwh, zmh, tid, act,
                                      FaceState<R>());
         else
-            write_pops<7, ODD, R, FACES>(h, a.h, S, c, a, kKindScalar);
+            write_pops<7, ODD, R, FACES>(h, a.h, c, a, kKindScalar);
     } else {
         if (m.need_T) {
             R hh[7];
-            read_pops_rt<7, R>(hh, a.h, S, c, a, kKindScalar);
+            read_pops_rt<7, R>(hh, a.h, c, a, kKindScalar);
             R sh = 0;
 #pragma unroll
             for (int i = 0; i < 7; i++) sh += hh[i];
@@ -298,13 +298,13 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F
         FaceState<R> fs;
         fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
         if constexpr (Z)
-            zpush_stage<7, R, FACES>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+            zpush_stage<7, R, FACES>(cc, a.c, c, a, kKindScalar, zwc, zmc, tid, act, fs);
         else
-            write_pops<7, ODD, R, FACES>(cc, a.c, S, c, a, kKindScalar, fs);
+            write_pops<7, ODD, R, FACES>(cc, a.c, c, a, kKindScalar, fs);
     } else {
         if (m.need_U) {
             R c7[7];
-            read_pops_rt<7, R>(c7, a.c, S, c, a, kKindScalar);
+            read_pops_rt<7, R>(c7, a.c, c, a, kKindScalar);
             R sc = 0;
 #pragma unroll
             for (int i = 0; i < 7; i++) sc += c7[i];
@@ -350,14 +350,14 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F
     R g[19];
     uint32_t own = 0;
     if constexpr (Z)
-        own = zpull_issue<19, R, FACES>(g, L, S, c, a, zr, tid);
+        own = zpull_issue<19, R, FACES>(g, L, c, a, zr, tid);
     else
-        read_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
+        read_pops<19, ODD, R, FACES>(g, L, c, a, kKindScalar);
     if constexpr (Z) {
         __syncthreads();
         zpull_finish<19, R>(g, c, a, zr, tid, own);
-        if constexpr (TH) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
-        if constexpr (SO) zpush_finish<7, R, FACES>(a.c, S, c, a, zwc, zmc, tid);
+        if constexpr (TH) zpush_finish<7, R, FACES>(a.h, c, a, zwh, zmh, tid);
+        if constexpr (SO) zpush_finish<7, R, FACES>(a.c, c, a, zwc, zmc, tid);
     }
     const R ux = (R)m.u[j][0], uy = (R)m.u[j][1], uz = (R)m.u[j][2];
     const R uu = ux * ux + uy * uy + uz * uz;
@@ -373,11 +373,11 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? (KR ? MKG_MPF_KR_WAVES_F
         g[i] = g[i] - om * (g[i] - geq) + w * (src + zd * cn);
     }
     if constexpr (Z) {
-        zpush_stage<19, R, FACES>(g, L, S, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
+        zpush_stage<19, R, FACES>(g, L, c, a, kKindScalar, zw, zm, tid, act, FaceState<R>());
         __syncthreads();
-        zpush_finish<19, R, FACES>(L, S, c, a, zw, zm, tid);
+        zpush_finish<19, R, FACES>(L, c, a, zw, zm, tid);
     } else {
-        write_pops<19, ODD, R, FACES>(g, L, S, c, a, kKindScalar);
+        write_pops<19, ODD, R, FACES>(g, L, c, a, kKindScalar);
     }
     };
     if (slow)
@@ -498,8 +498,10 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
     a.zstage = h->zstage;
     a.flag = h->flag;
     a.rho0 = 1.0;
-    auto fill = [&](LatArgs& L, void* pop, const int32_t* bc, double ref, const double* fval) {
-        L.pop = pop;
+    slot_major(a);
+    auto fill = [&](LatArgs& L, void* pop, int q, const int32_t* bc, double ref,
+                    const double* fval) {
+        slot_major_lattice(L, pop, q, h->S, h->esz);
         int np = 0;
         for (int f = 0; f < 6; f++) {
             L.bc[f] = bc[f];
@@ -511,13 +513,13 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
         L.adv = 1;
         L.ref = ref;
     };
-    fill(a.g, h->g, p.bc, -1.0, p.phi_face);
+    fill(a.g, h->g, 19, p.bc, -1.0, p.phi_face);
     for (int j = 0; j < MKG_MPF_MAX; j++) {
         m.gl[j] = a.g;
-        if (j < p.n) m.gl[j].pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+        if (j < p.n) slot_major_lattice(m.gl[j], (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
     }
-    if (h->h) fill(a.h, h->h, p.bc_h, p.T_ref, p.T_face);
-    if (h->c) fill(a.c, h->c, p.bc_c, 0.0, p.C_face);
+    if (h->h) fill(a.h, h->h, 7, p.bc_h, p.T_ref, p.T_face);
+    if (h->c) fill(a.c, h->c, 7, p.bc_c, 0.0, p.C_face);
     a.phi_cur = h->phi[h->pf_count & 1];
     a.phi_prev = h->phi[(h->pf_count + 1) & 1];
     m.lat = h->lat_elems();
@@ -568,7 +570,7 @@ int mpf_normalize(mkg_mpf_ctx* h) {
     MpfArgs m = mpf_args(h);
     for (int j = 0; j < h->p.n; j++) {
         LatArgs L = m.a.g;
-        L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+        slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
         if (launch_canonical(19, h->fp32, m.a, L, 1, h->scratch, h->stream) ||
             launch_store_pops(19, h->fp32, m.a, L, h->scratch, h->stream))
             return mfail("normalize: lattice unavailable");
@@ -844,7 +846,8 @@ int mkg_mpf_get(mkg_mpf_handle h, int which, void* host, size_t count) {
         if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
         for (int j = 0; j < nl; j++) {
             LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
-            if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+            if (which == MKG_MPF_POPS)
+                slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
             if (launch_canonical(q, h->fp32, m.a, L, 1, h->scratch, h->stream))
                 return mfail("lattice unavailable");
             MCHK(hipMemcpyAsync((double*)host + (size_t)j * q * N, h->scratch,
@@ -889,7 +892,8 @@ int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count) {
         MpfArgs m = mpf_args(h);
         for (int j = 0; j < nl; j++) {
             LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
-            if (which == MKG_MPF_POPS) L.pop = (char*)h->g + (size_t)j * h->lat_elems() * h->esz;
+            if (which == MKG_MPF_POPS)
+                slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
             MCHK(hipMemcpyAsync(h->scratch, (const double*)host + (size_t)j * q * N,
                                 (size_t)q * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
             if (launch_store_pops(q, h->fp32, m.a, L, h->scratch, h->stream))

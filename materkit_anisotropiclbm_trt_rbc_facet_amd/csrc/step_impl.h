@@ -100,7 +100,6 @@ __global__ void __launch_bounds__(256) k_nee(const StepArgs a) {
     locate_at(a, nbr(cb, ax == 0 ? sg : 0, ax == 1 ? sg : 0, ax == 2 ? sg : 0), cn);
     const int8_t fl = a.flag[cn.id];
     if (a.flag[cb.id] == kIsWalls || fl == kIsWalls) return;
-    const uint32_t S = a.S;
     R ns = 0;
     if constexpr (PH) {
         const R phi = *slot_at((const R*)a.phi_cur, cn.id);
@@ -112,14 +111,14 @@ __global__ void __launch_bounds__(256) k_nee(const StepArgs a) {
     R T = 0;
     if constexpr (TH) {
         R h[7];
-        read_pops_rt<7, R>(h, a.h, S, cn, a, kKindScalar);
+        read_pops_rt<7, R>(h, a.h, cn, a, kKindScalar);
         R sh = 0;
 #pragma unroll
         for (int i = 0; i < 7; i++) sh += h[i];
         T = (R)a.h.ref + sh;
     }
     R f[FQ];
-    read_pops_rt<FQ, R>(f, a.f, S, cn, a, kKindFlow);
+    read_pops_rt<FQ, R>(f, a.f, cn, a, kKindFlow);
     R dr = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int i = 0; i < FQ; i++) {
@@ -177,18 +176,34 @@ __global__ void __launch_bounds__(256) k_nee(const StepArgs a) {
     }
 }
 
-template <class R, int PQ, bool ODD>
+// ZS: O step with the z stage (the block holds whole z rows): the c_z != 0 populations are
+// loaded at the cell's own z and shifted through LDS, as in k_step.  PER: every face periodic
+// and no wall cells (no face rules).
+template <class R, int PQ, bool ODD, bool ZS = false, bool PER = false>
 __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __restrict__ phi_out) {
+    constexpr bool Z = ODD && ZS;
+    __shared__ R zv[Z ? nzdirs<PQ>() : 1][256];
     Cell c;
-    if (!locate(a, c)) return;
+    bool act = true;
+    if constexpr (Z) {
+        act = locate_clamped(a, c);
+    } else {
+        if (!locate(a, c)) return;
+    }
     const int8_t fl = a.flag[c.id];
+    R g[PQ];
+    if constexpr (Z) {
+        const uint32_t own = zpull_issue<PQ, R, !PER>(g, a.g, c, a, zv, threadIdx.x);
+        __syncthreads();
+        zpull_finish<PQ, R>(g, c, a, zv, threadIdx.x, own);
+        if (!act) return;
+    }
     if (fl == kIsWalls) {
         // wall cells keep their phase value: carry it into the buffer being written
         phi_out[c.id] = ((const R*)a.phi_prev)[c.id];
         return;
     }
-    R g[PQ];
-    read_pops<PQ, ODD, R>(g, a.g, a.S, c, a, kKindScalar);
+    if constexpr (!Z) read_pops<PQ, ODD, R, !PER>(g, a.g, c, a, kKindScalar);
     R s = 0;
 #pragma unroll
     for (int i = 0; i < PQ; i++) s += g[i];
@@ -312,27 +327,27 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
     uint32_t own_h = 0, own_c = 0, own_f = 0;
     if constexpr (TH) {
         if (!a.h.adv)
-            read_pops_rt<7, R>(h, a.h, S, c, a, kKindScalar);
+            read_pops_rt<7, R>(h, a.h, c, a, kKindScalar);
         else if constexpr (Z)
-            own_h = zpull_issue<7, R, FACES, FM>(h, a.h, S, c, a, zrh, tid);
+            own_h = zpull_issue<7, R, FACES, FM>(h, a.h, c, a, zrh, tid);
         else
-            read_pops<7, ODD, R, FACES, FM>(h, a.h, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES, FM>(h, a.h, c, a, kKindScalar);
     }
     if constexpr (SO) {
         if (!a.c.adv)
-            read_pops_rt<7, R>(cc, a.c, S, c, a, kKindScalar);
+            read_pops_rt<7, R>(cc, a.c, c, a, kKindScalar);
         else if constexpr (Z)
-            own_c = zpull_issue<7, R, FACES, FM>(cc, a.c, S, c, a, zrc, tid);
+            own_c = zpull_issue<7, R, FACES, FM>(cc, a.c, c, a, zrc, tid);
         else
-            read_pops<7, ODD, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar);
+            read_pops<7, ODD, R, FACES, FM>(cc, a.c, c, a, kKindScalar);
     }
     if constexpr (FQ > 0) {
         if (!a.f.adv)
-            read_pops_rt<FQ, R>(f, a.f, S, c, a, kKindFlow);
+            read_pops_rt<FQ, R>(f, a.f, c, a, kKindFlow);
         else if constexpr (Z)
-            own_f = zpull_issue<FQ, R, FACES, FM>(f, a.f, S, c, a, zrf, tid);
+            own_f = zpull_issue<FQ, R, FACES, FM>(f, a.f, c, a, zrf, tid);
         else
-            read_pops<FQ, ODD, R, FACES, FM>(f, a.f, S, c, a, kKindFlow);
+            read_pops<FQ, ODD, R, FACES, FM>(f, a.f, c, a, kKindFlow);
     }
     if constexpr (Z) {
         __syncthreads();  // B1
@@ -394,9 +409,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
             fs.uy = uy;
             fs.uz = uz;
             if constexpr (Z)
-                zpush_stage<FQ, R, FACES, FM>(f, a.f, S, c, a, kKindFlow, zwf, zmf, tid, act, fs);
+                zpush_stage<FQ, R, FACES, FM>(f, a.f, c, a, kKindFlow, zwf, zmf, tid, act, fs);
             else
-                write_pops<FQ, ODD, R, FACES, FM>(f, a.f, S, c, a, kKindFlow, fs);
+                write_pops<FQ, ODD, R, FACES, FM>(f, a.f, c, a, kKindFlow, fs);
         }
         const R om = (R)1 - ns;
         ux *= om;
@@ -428,10 +443,10 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
                 h[i] = h[i] - om * (h[i] - heq) + w * src;
             }
             if constexpr (Z)
-                zpush_stage<7, R, FACES, FM>(h, a.h, S, c, a, kKindScalar, zwh, zmh, tid, act,
+                zpush_stage<7, R, FACES, FM>(h, a.h, c, a, kKindScalar, zwh, zmh, tid, act,
                                   FaceState<R>());
             else
-                write_pops<7, ODD, R, FACES, FM>(h, a.h, S, c, a, kKindScalar);
+                write_pops<7, ODD, R, FACES, FM>(h, a.h, c, a, kKindScalar);
         }
     }
 
@@ -476,9 +491,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
             FaceState<R> fs;
             fs.dscale = A;  // Dirichlet faces fix U: c_w = A(phi) U_w
             if constexpr (Z)
-                zpush_stage<7, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar, zwc, zmc, tid, act, fs);
+                zpush_stage<7, R, FACES, FM>(cc, a.c, c, a, kKindScalar, zwc, zmc, tid, act, fs);
             else
-                write_pops<7, ODD, R, FACES, FM>(cc, a.c, S, c, a, kKindScalar, fs);
+                write_pops<7, ODD, R, FACES, FM>(cc, a.c, c, a, kKindScalar, fs);
         }
     }
 
@@ -528,19 +543,19 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
             // the populations are read after the source so that the stencil values are dead
             // (loaded with the other lattices, they measured 13 % slower at 256^3)
             if constexpr (Z)
-                own_g = zpull_issue<PQ, R, FACES, FM>(g, a.g, S, c, a, zrg, tid);
+                own_g = zpull_issue<PQ, R, FACES, FM>(g, a.g, c, a, zrg, tid);
             else
-                read_pops<PQ, ODD, R, FACES, FM>(g, a.g, S, c, a, kKindScalar);
+                read_pops<PQ, ODD, R, FACES, FM>(g, a.g, c, a, kKindScalar);
         }
     }
     if constexpr (Z) {
         __syncthreads();  // B2
         if constexpr (FQ > 0)
-            if (a.f.adv) zpush_finish<FQ, R, FACES>(a.f, S, c, a, zwf, zmf, tid);
+            if (a.f.adv) zpush_finish<FQ, R, FACES>(a.f, c, a, zwf, zmf, tid);
         if constexpr (TH)
-            if (a.h.adv) zpush_finish<7, R, FACES>(a.h, S, c, a, zwh, zmh, tid);
+            if (a.h.adv) zpush_finish<7, R, FACES>(a.h, c, a, zwh, zmh, tid);
         if constexpr (SO)
-            if (a.c.adv) zpush_finish<7, R, FACES>(a.c, S, c, a, zwc, zmc, tid);
+            if (a.c.adv) zpush_finish<7, R, FACES>(a.c, c, a, zwc, zmc, tid);
         if constexpr (PQ > 0)
             if (a.g.adv) zpull_finish<PQ, R>(g, c, a, zrg, tid, own_g);
     }
@@ -559,14 +574,14 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
                 g[i] = g[i] - om * (g[i] - geq) + w * (Sg + acz * cn);
             }
             if constexpr (Z)
-                zpush_stage<PQ, R, FACES, FM>(g, a.g, S, c, a, kKindScalar, zrf, zmg, tid, act,
+                zpush_stage<PQ, R, FACES, FM>(g, a.g, c, a, kKindScalar, zrf, zmg, tid, act,
                                    FaceState<R>());
             else
-                write_pops<PQ, ODD, R, FACES, FM>(g, a.g, S, c, a, kKindScalar);
+                write_pops<PQ, ODD, R, FACES, FM>(g, a.g, c, a, kKindScalar);
         }
         if constexpr (Z) {
             __syncthreads();  // B3
-            if (a.g.adv) zpush_finish<PQ, R, FACES>(a.g, S, c, a, zrf, zmg, tid);
+            if (a.g.adv) zpush_finish<PQ, R, FACES>(a.g, c, a, zrf, zmg, tid);
         }
     }
     };
@@ -646,10 +661,20 @@ inline void run_nee(const StepArgs& a, hipStream_t s) {
 template <class R, int PQ>
 inline void run_phase_moments(int odd, const StepArgs& a, void* phi_out, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
-    if (odd)
-        k_phase_moments<R, PQ, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
-    else
+    const bool per = !a.has_walls && a.g.npmask == 0;
+    if (odd && a.zstage) {
+        if (per)
+            k_phase_moments<R, PQ, true, true, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+        else
+            k_phase_moments<R, PQ, true, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+    } else if (odd) {
+        if (per)
+            k_phase_moments<R, PQ, true, false, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+        else
+            k_phase_moments<R, PQ, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+    } else {
         k_phase_moments<R, PQ, false><<<grid, block, 0, s>>>(a, (R*)phi_out);
+    }
 }
 
 // Lattice combinations with an instantiation: (flow Q, phase Q, thermal, solute).
