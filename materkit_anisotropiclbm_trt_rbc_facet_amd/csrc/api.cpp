@@ -50,6 +50,44 @@ constexpr uint32_t kPageShift = 13;
 
 namespace mkg {
 void set_error(const std::string& msg) { g_err = msg; }
+
+PopBlock plan_pop_block(const int* q, int nlat, uint32_t N, uint32_t S, uint32_t F, size_t esz) {
+    PopBlock b;
+    uint32_t shift = kPageShift;
+    if (const char* env = std::getenv("MKG_PAGE_SHIFT")) {
+        const int v = std::atoi(env);
+        if (v >= 8 && v <= 16) shift = (uint32_t)v;
+    }
+    const uint32_t B = 1u << shift;
+    size_t qt = 0;
+    for (int k = 0; k < nlat; k++) qt += (size_t)q[k];
+    if (qt == 0) return b;
+    const size_t npages = ((size_t)N + B - 1) / B;
+    const size_t paged_total = npages * qt * B + qt * F;
+    bool paged = paged_total * esz < ((size_t)1 << 32);
+    if (const char* env = std::getenv("MKG_PAGES")) paged = paged && std::atoi(env) != 0;
+    size_t slots;
+    if (paged) {
+        b.paged = 1;
+        b.pg_shift = shift;
+        b.pg_mask = B - 1u;
+        b.pg_stride = (uint32_t)(qt * B);
+        b.sl_stride = B;
+        slots = npages * qt * B;
+    } else {
+        b.sl_stride = S;
+        slots = qt * S;
+    }
+    b.total = slots + qt * F;
+    size_t off = 0, foff = slots;
+    for (int k = 0; k < nlat; k++) {
+        b.slot0[k] = off * b.sl_stride;
+        b.face0[k] = foff;
+        off += (size_t)q[k];
+        foff += (size_t)q[k] * F;
+    }
+    return b;
+}
 }  // namespace mkg
 
 struct mkg_ctx {
@@ -223,49 +261,20 @@ int ensure_nee(mkg_ctx* h, const mkg_params& p) {
     return 0;
 }
 
-// The population block of a handle.  Paged when the block stays below 4 GiB (the kernels
-// address a lattice with 32-bit byte offsets from its slot 0), slot-major otherwise or when
-// MKG_PAGES=0 (MKG_PAGE_SHIFT sets the page size).  Face buffers follow the populations,
-// lattice by lattice.
 int alloc_pops(mkg_ctx* h) {
-    uint32_t shift = kPageShift;
-    if (const char* env = std::getenv("MKG_PAGE_SHIFT")) {
-        const int v = std::atoi(env);
-        if (v >= 8 && v <= 16) shift = (uint32_t)v;
-    }
-    const uint32_t B = 1u << shift;
-    size_t qt = 0;
-    for (int L = 0; L < 4; L++) qt += (size_t)h->q[L];
-    if (qt == 0) return 0;
-    const size_t npages = ((size_t)h->N + B - 1) / B;
-    const size_t paged_total = npages * qt * B + qt * h->F;
-    bool paged = paged_total * h->esz < ((size_t)1 << 32);
-    if (const char* env = std::getenv("MKG_PAGES")) paged = paged && std::atoi(env) != 0;
-    size_t slots, off = 0, foff;
-    if (paged) {
-        h->paged = 1;
-        h->pg_shift = shift;
-        h->pg_mask = B - 1u;
-        h->pg_stride = (uint32_t)(qt * B);
-        h->sl_stride = B;
-        slots = npages * qt * B;
-    } else {
-        h->paged = 0;
-        h->pg_shift = 31;
-        h->pg_mask = 0x7FFFFFFFu;
-        h->pg_stride = 0;
-        h->sl_stride = h->S;
-        slots = qt * h->S;
-    }
-    h->pop_total = slots + qt * h->F;
+    const PopBlock b = plan_pop_block(h->q, 4, h->N, h->S, h->F, h->esz);
+    if (b.total == 0) return 0;
+    h->paged = b.paged;
+    h->pg_shift = b.pg_shift;
+    h->pg_mask = b.pg_mask;
+    h->pg_stride = b.pg_stride;
+    h->sl_stride = b.sl_stride;
+    h->pop_total = b.total;
     if (alloc_zero(&h->popbuf, h->pop_total * h->esz, h->stream)) return -1;
-    foff = slots;
     for (int L = 0; L < 4; L++) {
         if (!h->q[L]) continue;
-        h->pop[L] = (char*)h->popbuf + off * h->sl_stride * h->esz;
-        h->face[L] = (char*)h->popbuf + foff * h->esz;
-        off += (size_t)h->q[L];
-        foff += (size_t)h->q[L] * h->F;
+        h->pop[L] = (char*)h->popbuf + b.slot0[L] * h->esz;
+        h->face[L] = (char*)h->popbuf + b.face0[L] * h->esz;
     }
     return 0;
 }

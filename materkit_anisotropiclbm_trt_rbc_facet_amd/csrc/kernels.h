@@ -68,18 +68,32 @@ struct StepArgs {
     void* tape;
 };
 
-// Slot-major population addressing: one array per slot, stride S (pcell(c) = c).
-inline void slot_major(StepArgs& a) {
-    a.pg_shift = 31;
-    a.pg_mask = 0x7FFFFFFFu;
-    a.pg_stride = 0;
-    a.sl_stride = a.S;
-}
-// A slot-major lattice allocation: Q slots of S elements, then its Q x F face buffer.
-inline void slot_major_lattice(LatArgs& L, void* pop, int q, uint32_t S, size_t esz) {
-    L.pop = pop;
-    L.face = pop ? (char*)pop + (size_t)q * S * esz : nullptr;
-}
+// Population block of a handle: the slots of every lattice, then every lattice's Q x F face
+// buffer, in one allocation of `total` elements.  Paged (pages of 2^pg_shift cells, each
+// holding every slot of every lattice) when the block stays below 4 GiB, so that the kernels
+// can address a lattice with 32-bit byte offsets from its slot 0; slot-major otherwise, or when
+// MKG_PAGES=0.  MKG_PAGE_SHIFT sets the page size (default 2^13 cells).
+constexpr int kMaxLattices = 8;
+struct PopBlock {
+    int paged = 0;
+    uint32_t pg_shift = 31, pg_mask = 0x7FFFFFFFu, pg_stride = 0, sl_stride = 0;
+    size_t total = 0;                   // elements
+    size_t slot0[kMaxLattices] = {};    // element offset of slot 0 of lattice k
+    size_t face0[kMaxLattices] = {};    // element offset of the face buffer of lattice k
+    void apply(StepArgs& a) const {
+        a.pg_shift = pg_shift;
+        a.pg_mask = pg_mask;
+        a.pg_stride = pg_stride;
+        a.sl_stride = sl_stride;
+    }
+    // lattice k of a block allocated at base (element size esz)
+    void lattice(LatArgs& L, void* base, int k, size_t esz) const {
+        L.pop = (char*)base + slot0[k] * esz;
+        L.face = (char*)base + face0[k] * esz;
+    }
+};
+// q[k]: slots of lattice k (0: absent); N cells, S per-cell array stride, F face cells.
+PopBlock plan_pop_block(const int* q, int nlat, uint32_t N, uint32_t S, uint32_t F, size_t esz);
 
 // Selector of a fused-step instantiation: presence of each lattice.
 struct Combo {

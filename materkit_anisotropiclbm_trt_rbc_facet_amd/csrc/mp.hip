@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) k_mp_init_h(const MpArgs m, const double*
     if (!locate(m.a, c)) return;
     R* hb = (R*)m.a.g.pop;
 #pragma unroll
-    for (int i = 0; i < 19; i++) *slot_at(hb + (size_t)i * m.a.S, c.id) = (R)(wgt<19>(i) * phi0[c.id]);
+    for (int i = 0; i < 19; i++) *pop_at(hb, i, c.id, m.a) = (R)(wgt<19>(i) * phi0[c.id]);
 }
 
 // Equilibria of both lattices from phi0, u0 (3N or null), p0 (N or null) and mu of phi0.
@@ -318,8 +318,8 @@ __global__ void __launch_bounds__(256) k_mp_init_eq(const MpArgs m, const double
         const double w = wgt<19>(i);
         const double cu = cx(i) * u[0] + cy(i) * u[1] + cz(i) * u[2];
         const double si = w * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-        *slot_at(fb + (size_t)i * S, c.id) = (R)((i == 0 ? w - 1.0 : w) * 3.0 * p + rho * si);
-        *slot_at(hb + (size_t)i * S, c.id) =
+        *pop_at(fb, i, c.id, m.a) = (R)((i == 0 ? w - 1.0 : w) * 3.0 * p + rho * si);
+        *pop_at(hb, i, c.id, m.a) =
             (R)(i == 0 ? ph + (w - 1.0) * em : w * (em + 3.0 * ph * cu));
     }
     R* pu = (R*)m.phiu;
@@ -336,8 +336,10 @@ struct mkg_mp_ctx {
     size_t esz = 8;
     uint32_t N = 0, S = 0, F = 0, nblk = 0;
     hipStream_t stream = nullptr;
-    void* f = nullptr;
-    void* h = nullptr;
+    // the velocity-pressure lattice f (block lattice 0) and the Cahn-Hilliard lattice h
+    // (lattice 1) in one population block
+    void* pops = nullptr;
+    PopBlock pb;
     void* phi = nullptr;
     void* mu = nullptr;
     void* phiu = nullptr;
@@ -384,9 +386,9 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     a.f.npmask = a.g.npmask = np;
     a.f.lay = a.g.lay = h->lay;
     a.f.adv = a.g.adv = 1;
-    slot_major(a);
-    slot_major_lattice(a.f, h->f, 19, h->S, h->esz);
-    slot_major_lattice(a.g, h->h, 19, h->S, h->esz);
+    h->pb.apply(a);
+    h->pb.lattice(a.f, h->pops, 0, h->esz);
+    h->pb.lattice(a.g, h->pops, 1, h->esz);
     m.rho_l = p.rho_l;
     m.rho_g = p.rho_g;
     m.nu_l = p.nu_l;
@@ -419,7 +421,7 @@ void launch_phi_mu(const MpArgs& m, int odd, hipStream_t s) {
 void mp_destroy_impl(mkg_mp_ctx* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->f, h->h, h->phi, h->mu, h->phiu, h->flag, h->scratch, h->partials,
+    void* bufs[] = {h->pops, h->phi, h->mu, h->phiu, h->flag, h->scratch, h->partials,
                     h->red_out};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -493,12 +495,11 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
     };
     hipError_t e;
     if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad(e, "hipStreamCreate");
-    const size_t lb = (size_t)19 * ((size_t)h->S + h->F) * h->esz;
-    void** bufs[] = {&h->f, &h->h};
-    for (void** b : bufs) {
-        if ((e = hipMalloc(b, lb)) != hipSuccess) return bad(e, "hipMalloc");
-        if ((e = hipMemsetAsync(*b, 0, lb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
-    }
+    const int q[2] = {19, 19};
+    h->pb = plan_pop_block(q, 2, h->N, h->S, h->F, h->esz);
+    const size_t lb = h->pb.total * h->esz;
+    if ((e = hipMalloc(&h->pops, lb)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = hipMemsetAsync(h->pops, 0, lb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
     if ((e = hipMalloc(&h->phi, (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
     if ((e = hipMalloc(&h->mu, (size_t)h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");
     if ((e = hipMalloc(&h->phiu, (size_t)3 * h->S * h->esz)) != hipSuccess) return bad(e, "hipMalloc");

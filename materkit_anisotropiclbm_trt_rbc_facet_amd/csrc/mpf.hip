@@ -39,7 +39,6 @@ void set_error(const std::string& msg);
 struct MpfArgs {
     StepArgs a;    // geometry, flags and walls; a.g = lattice of order parameter 0, a.h thermal
     LatArgs gl[MKG_MPF_MAX];  // lattice of each order parameter (a.g with its own base)
-    uint64_t lat;  // elements from one order parameter's lattice to the next: 19 (S + F)
     int32_t n, thermal;
     double u[MKG_MPF_MAX][3];
     double kappa[MKG_MPF_MAX];
@@ -399,7 +398,7 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
     double comb = m.n - 1.0;
     for (int j = 0; j < m.n; j++) {
         const double pj = phij[(size_t)j * N + c.id];
-        R* base = (R*)m.a.g.pop + (size_t)j * m.lat;
+        R* base = (R*)m.gl[j].pop;
         const double ux = m.u[j][0], uy = m.u[j][1], uz = m.u[j][2];
         const double uu = ux * ux + uy * uy + uz * uz;
 #pragma unroll
@@ -407,7 +406,7 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
             const double w = wgt<19>(i);
             const double cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
             const double geq = w * pj * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-            *slot_at(base + (size_t)i * S, c.id) = (R)(geq - w * m.a.g.ref);
+            *pop_at(base, i, c.id, m.a) = (R)(geq - w * m.a.g.ref);
         }
         *slot_at(phi0 + (size_t)j * S, c.id) = (R)pj;
         *slot_at(phi1 + (size_t)j * S, c.id) = (R)pj;
@@ -420,14 +419,14 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
         R* hb = (R*)m.a.h.pop;
 #pragma unroll
         for (int i = 0; i < 7; i++)
-            *slot_at(hb + (size_t)i * S, c.id) = (R)(wgt<7>(i) * (t - m.a.h.ref));
+            *pop_at(hb, i, c.id, m.a) = (R)(wgt<7>(i) * (t - m.a.h.ref));
     }
     if (m.a.c.pop) {
         const double cv = Cin ? Cin[c.id] : 0.0;
         R* cb = (R*)m.a.c.pop;
 #pragma unroll
         for (int i = 0; i < 7; i++)
-            *slot_at(cb + (size_t)i * S, c.id) = (R)(wgt<7>(i) * (cv - m.a.c.ref));
+            *pop_at(cb, i, c.id, m.a) = (R)(wgt<7>(i) * (cv - m.a.c.ref));
     }
 }
 
@@ -452,9 +451,13 @@ struct mkg_mpf_ctx {
     size_t esz = 8;
     uint32_t N = 0, S = 0, F = 0, nblk = 0;
     hipStream_t stream = nullptr;
-    void* g = nullptr;        // n lattices of 19 (S + F) elements
-    void* h = nullptr;        // 7 (S + F) elements
-    void* c = nullptr;        // 7 (S + F) elements (solute)
+    // population block: the n order-parameter lattices (block lattices 0..n-1), the thermal
+    // (n) and the solute (n + 1) lattice; h / c point at the thermal / solute slot 0 (null when
+    // absent)
+    void* pops = nullptr;
+    PopBlock pb;
+    void* h = nullptr;
+    void* c = nullptr;
     void* phi[2] = {nullptr, nullptr};  // (n + 1) S elements each
     int8_t* flag = nullptr;
     double* scratch = nullptr;  // 19 N doubles
@@ -464,7 +467,8 @@ struct mkg_mpf_ctx {
     int zstage = 0;           // O steps take the z shift through LDS
     int64_t pf_count = 0;
     int has_walls = 0;
-    size_t lat_elems() const { return (size_t)19 * ((size_t)S + F); }
+    // order parameter j's lattice descriptor (base, face buffer) on top of a filled one
+    void lattice(LatArgs& L, int j) const { pb.lattice(L, pops, j, esz); }
 };
 
 namespace {
@@ -498,10 +502,9 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
     a.zstage = h->zstage;
     a.flag = h->flag;
     a.rho0 = 1.0;
-    slot_major(a);
-    auto fill = [&](LatArgs& L, void* pop, int q, const int32_t* bc, double ref,
-                    const double* fval) {
-        slot_major_lattice(L, pop, q, h->S, h->esz);
+    h->pb.apply(a);
+    auto fill = [&](LatArgs& L, int k, const int32_t* bc, double ref, const double* fval) {
+        h->pb.lattice(L, h->pops, k, h->esz);
         int np = 0;
         for (int f = 0; f < 6; f++) {
             L.bc[f] = bc[f];
@@ -513,16 +516,15 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
         L.adv = 1;
         L.ref = ref;
     };
-    fill(a.g, h->g, 19, p.bc, -1.0, p.phi_face);
+    fill(a.g, 0, p.bc, -1.0, p.phi_face);
     for (int j = 0; j < MKG_MPF_MAX; j++) {
         m.gl[j] = a.g;
-        if (j < p.n) slot_major_lattice(m.gl[j], (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
+        if (j < p.n) h->lattice(m.gl[j], j);
     }
-    if (h->h) fill(a.h, h->h, 7, p.bc_h, p.T_ref, p.T_face);
-    if (h->c) fill(a.c, h->c, 7, p.bc_c, 0.0, p.C_face);
+    if (h->h) fill(a.h, p.n, p.bc_h, p.T_ref, p.T_face);
+    if (h->c) fill(a.c, p.n + 1, p.bc_c, 0.0, p.C_face);
     a.phi_cur = h->phi[h->pf_count & 1];
     a.phi_prev = h->phi[(h->pf_count + 1) & 1];
-    m.lat = h->lat_elems();
     m.n = p.n;
     m.thermal = p.thermal ? 1 : 0;
     for (int j = 0; j < MKG_MPF_MAX; j++) {
@@ -570,7 +572,7 @@ int mpf_normalize(mkg_mpf_ctx* h) {
     MpfArgs m = mpf_args(h);
     for (int j = 0; j < h->p.n; j++) {
         LatArgs L = m.a.g;
-        slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
+        h->lattice(L, j);
         if (launch_canonical(19, h->fp32, m.a, L, 1, h->scratch, h->stream) ||
             launch_store_pops(19, h->fp32, m.a, L, h->scratch, h->stream))
             return mfail("normalize: lattice unavailable");
@@ -621,7 +623,7 @@ void launch_mpf_step_kernel(const MpfArgs& m, int odd, int th, int so, int kr, h
 void mpf_destroy_impl(mkg_mpf_ctx* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->g, h->h, h->c, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
+    void* bufs[] = {h->pops, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
                     h->red_out};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -727,17 +729,17 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     };
     hipError_t e;
     if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad(e, "hipStreamCreate");
-    const size_t gb = (size_t)p->n * h->lat_elems() * h->esz;
-    if ((e = hipMalloc(&h->g, gb)) != hipSuccess) return bad(e, "hipMalloc");
-    if ((e = hipMemsetAsync(h->g, 0, gb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
-    const size_t sb = (size_t)7 * ((size_t)h->S + h->F) * h->esz;
-    if (p->thermal) {
-        if ((e = hipMalloc(&h->h, sb)) != hipSuccess) return bad(e, "hipMalloc");
-        if ((e = hipMemsetAsync(h->h, 0, sb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
-    }
-    if (p->solute) {
-        if ((e = hipMalloc(&h->c, sb)) != hipSuccess) return bad(e, "hipMalloc");
-        if ((e = hipMemsetAsync(h->c, 0, sb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+    {
+        int q[kMaxLattices] = {};
+        for (int j = 0; j < p->n; j++) q[j] = 19;
+        q[p->n] = p->thermal ? 7 : 0;
+        q[p->n + 1] = p->solute ? 7 : 0;
+        h->pb = plan_pop_block(q, p->n + 2, h->N, h->S, h->F, h->esz);
+        const size_t gb = h->pb.total * h->esz;
+        if ((e = hipMalloc(&h->pops, gb)) != hipSuccess) return bad(e, "hipMalloc");
+        if ((e = hipMemsetAsync(h->pops, 0, gb, h->stream)) != hipSuccess) return bad(e, "hipMemsetAsync");
+        if (p->thermal) h->h = (char*)h->pops + h->pb.slot0[p->n] * h->esz;
+        if (p->solute) h->c = (char*)h->pops + h->pb.slot0[p->n + 1] * h->esz;
     }
     const size_t pb = (size_t)(p->n + 1) * h->S * h->esz;
     for (int b = 0; b < 2; b++) {
@@ -846,8 +848,7 @@ int mkg_mpf_get(mkg_mpf_handle h, int which, void* host, size_t count) {
         if (count != (size_t)nl * q * N) return mfail("count does not match the field size");
         for (int j = 0; j < nl; j++) {
             LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
-            if (which == MKG_MPF_POPS)
-                slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
+            if (which == MKG_MPF_POPS) h->lattice(L, j);
             if (launch_canonical(q, h->fp32, m.a, L, 1, h->scratch, h->stream))
                 return mfail("lattice unavailable");
             MCHK(hipMemcpyAsync((double*)host + (size_t)j * q * N, h->scratch,
@@ -892,8 +893,7 @@ int mkg_mpf_set(mkg_mpf_handle h, int which, const void* host, size_t count) {
         MpfArgs m = mpf_args(h);
         for (int j = 0; j < nl; j++) {
             LatArgs L = which == MKG_MPF_POPS ? m.a.g : (which == MKG_MPF_THERMAL ? m.a.h : m.a.c);
-            if (which == MKG_MPF_POPS)
-                slot_major_lattice(L, (char*)h->g + (size_t)j * h->lat_elems() * h->esz, 19, h->S, h->esz);
+            if (which == MKG_MPF_POPS) h->lattice(L, j);
             MCHK(hipMemcpyAsync(h->scratch, (const double*)host + (size_t)j * q * N,
                                 (size_t)q * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
             if (launch_store_pops(q, h->fp32, m.a, L, h->scratch, h->stream))
