@@ -402,11 +402,18 @@ int mkg_mpf_sums(mkg_mpf_handle h, double* out);
  * diffusive flux D q(phi) grad U and the anti-trapping current.  The handle takes the phase and
  * solute parameters of an mkg_params block (pf_dt, pf_wdx2, eps4, lambda, aniso_*, D_l, D_s,
  * k_part, at_coef, q_min; faces bc_g: periodic or bounce = zero flux; bc_c: periodic, bounce or
- * Dirichlet with C_face as the face value of U); ngrains must be 0. */
+ * Dirichlet with C_face as the face value of U); ngrains must be 0.
+ * With thermal = 1 the potential is the temperature (mkSolutalField::evolution_thermal,
+ * MaterKit/inc/mkSolutalField.h:47-48, "potential is temperature"): the phase source takes
+ * u = T and T' = T - sum_faces J_out + L_T (phi' - phi) with J_out = -alpha_f (T_nb - T),
+ * alpha_f the mean of alpha(phi) = alpha_l (1 - f_s) + alpha_s f_s of the two cells (faces
+ * bc_h, Dirichlet values T_face; 0 < alpha <= 1/6); fields MKG_FVM_PHI and MKG_FVM_TEMP, and
+ * mkg_fvm_sums gives the sums of phi and T. */
 typedef struct mkg_fvm_ctx* mkg_fvm_handle;
 #define MKG_FVM_PHI 0   /* N, phase field */
 #define MKG_FVM_CONC 1  /* N, c = A(phi) U */
 #define MKG_FVM_U 2     /* N, U = c / A(phi) (get only) */
+#define MKG_FVM_TEMP 3  /* N, T: the potential of a thermal handle */
 
 int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out);
 int mkg_fvm_destroy(mkg_fvm_handle h);
@@ -416,7 +423,7 @@ int mkg_fvm_set(mkg_fvm_handle h, int which, const double* host, size_t count);
 int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase);
 int mkg_fvm_sync(mkg_fvm_handle h);
 int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count);
-/* FP64 sums of phi and of c over every cell (out[0], out[1]). */
+/* FP64 sums of phi and of c (or T) over every cell (out[0], out[1]). */
 int mkg_fvm_sums(mkg_fvm_handle h, double* out);
 
 /* ---- Two-phase flow ----------------------------------------------------------------------

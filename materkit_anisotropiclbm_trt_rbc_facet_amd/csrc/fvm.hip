@@ -11,6 +11,10 @@
 //                 J_out = -D q_f (U_nb - U) + s j_at,a; j_at = -a_t W/dx (1 + (1-k) U_f) dphi_f
 //                 g/|g| on the face (g: normal difference, transverse means of phi)
 // Both read the old fields and write the other buffer of a pair; c - phi/2 is conserved.
+// With the temperature as the potential (mkSolutalField::evolution_thermal,
+// MaterKit/inc/mkSolutalField.h:47-48) the phase source takes u = T and the second launch is
+//   k_fvm_thermal T' = T - sum_faces J_out + L_T (phi' - phi), J_out = -alpha_f (T_nb - T),
+//                 alpha_f the mean of the two cells' alpha(phi); T - L_T phi is conserved.
 // When a block can hold whole z rows, the marching forms (k_fvm_phase_march,
 // k_fvm_solute_march) evaluate each face once for both of its cells where they can.
 #include <hip/hip_runtime.h>
@@ -36,6 +40,20 @@ __device__ inline R fvm_q(const StepArgs& a, R phi) {
 template <class R>
 __device__ inline R fvm_A(const StepArgs& a, R phi) {
     return (R)0.5 * (((R)1 + (R)a.k_part) - ((R)1 - (R)a.k_part) * phi);
+}
+// The potential u that drives the phase source: U = c / A(phi) of the solute, or T itself
+// (pf_coupling 1).
+template <class R>
+__device__ inline R fvm_u(const StepArgs& a, R v, R phi) {
+    return a.pf_coupling == 1 ? v : v / fvm_A<R>(a, phi);
+}
+// Thermal diffusivity of a cell: alpha_l (1 - f_s) + alpha_s f_s, f_s = (1 + phi)/2 in [0, 1]
+// (the thermal lattice's alpha(phi)).
+template <class R>
+__device__ inline R fvm_alpha(const StepArgs& a, R phi) {
+    R fs = (R)0.5 * ((R)1 + phi);
+    fs = fs < (R)0 ? (R)0 : (fs > (R)1 ? (R)1 : fs);
+    return (R)a.alpha_l * ((R)1 - fs) + (R)a.alpha_s * fs;
 }
 
 // 19-point stencil of phi (periodic phase faces wrap, others replicate the edge).
@@ -65,7 +83,7 @@ __global__ void __launch_bounds__(256) k_fvm_phase(const StepArgs a, const R* __
     R P[27];
     fvm_stencil<R>(a, c, phi, P);
     const R p0 = P[13];
-    const R U = *slot_at(cc, c.id) / fvm_A<R>(a, p0);
+    const R U = fvm_u<R>(a, *slot_at(cc, c.id), p0);
     const R M[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     const Aniso<R> an{a.aniso_kind, (R)a.eps4, (R)a.aniso[0], (R)a.aniso[1]};
     R as;
@@ -178,7 +196,7 @@ __global__ void __launch_bounds__(256) k_fvm_phase_march(const StepArgs a, const
         const R divJ = jxp - jxm + jyp - jym + jzp - jzm;
         const R p0 = P[13];
         const R lap = P[22] + P[4] + P[16] + P[10] + P[14] + P[12] - (R)6 * p0;
-        const R U = *slot_at(cc, id) / fvm_A<R>(a, p0);
+        const R U = fvm_u<R>(a, *slot_at(cc, id), p0);
         // cell part of phase_source: LB-weighted cell gradient, anisotropy, source
         R gr0 = 0, gr1 = 0, gr2 = 0;
 #pragma unroll
@@ -208,6 +226,40 @@ __global__ void __launch_bounds__(256) k_fvm_phase_march(const StepArgs a, const
             Q[1][k] = Q[2][k];
         }
     }
+}
+
+// Heat equation with the latent source on the faces of the solute update (one cell per
+// thread): Dirichlet thermal faces fix T at the face (J_out = -2 alpha (T_w - T)), bounce
+// faces carry no flux.
+template <class R>
+__global__ void __launch_bounds__(256) k_fvm_thermal(const StepArgs a, const R* __restrict__ phi,
+                                                     const R* __restrict__ phin,
+                                                     const R* __restrict__ T,
+                                                     R* __restrict__ Tn) {
+    Cell c;
+    if (!locate(a, c)) return;
+    const R p0 = *slot_at(phi, c.id);
+    const R T0 = *slot_at(T, c.id);
+    const R a0 = fvm_alpha<R>(a, p0);
+    const uint32_t nph = (uint32_t)a.h.npmask & c.bits;
+    R out = 0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+#pragma unroll
+        for (int sd = 0; sd < 2; sd++) {
+            const int s = sd == 0 ? 1 : -1;
+            const int face = 2 * ax + (s > 0 ? 1 : 0);
+            if (nph & (1u << face)) {
+                if (pick6(face, a.h.bc) == kBcDirichlet)
+                    out += (R)-2 * a0 * ((R)pick6(face, a.h.fval) - T0);
+                continue;
+            }
+            const uint32_t nb = nbr(c, ax == 0 ? s : 0, ax == 1 ? s : 0, ax == 2 ? s : 0);
+            const R af = (R)0.5 * (a0 + fvm_alpha<R>(a, *slot_at(phi, nb)));
+            out += -af * (*slot_at(T, nb) - T0);
+        }
+    }
+    *slot_at(Tn, c.id) = T0 - out + (R)a.latent_T * (*slot_at(phin, c.id) - p0);
 }
 
 template <class R>
@@ -443,7 +495,8 @@ struct mkg_fvm_ctx {
     uint32_t N = 0, S = 0, nblk = 0;
     hipStream_t stream = nullptr;
     void* phi[2] = {nullptr, nullptr};
-    void* c[2] = {nullptr, nullptr};
+    void* c[2] = {nullptr, nullptr};  // c = A(phi) U, or T when thermal
+    int thermal = 0;  // the potential is the temperature (evolution_thermal)
     int cur = 0;  // buffer of the current level
     uint32_t march_blk = 0;   // blocks of the marching kernels (0: the per-cell kernels)
     int8_t* flag = nullptr;   // all fluid: the reductions read flags
@@ -486,6 +539,17 @@ StepArgs fvm_args(const mkg_fvm_ctx* h) {
     }
     a.g.npmask = npg;
     a.c.npmask = npc;
+    int nph = 0;
+    for (int f = 0; f < 6; f++) {
+        a.h.bc[f] = p.bc_h[f];
+        a.h.fval[f] = p.T_face[f];
+        if (p.bc_h[f] != MKG_BC_PERIODIC) nph |= 1 << f;
+    }
+    a.h.npmask = nph;
+    a.alpha_l = p.alpha_l;
+    a.alpha_s = p.alpha_s;
+    a.latent_T = p.latent_T;
+    a.pf_coupling = h->thermal ? 1 : 2;
     a.D_l = p.D_l;
     a.D_s = p.D_s;
     a.k_part = p.k_part;
@@ -529,19 +593,25 @@ int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
     if (p->aniso_kind == MKG_ANISO_FACET && !(p->aniso[1] > 0))
         return ffail("the facet anisotropy needs a rounding nu > 0");
     if (!(p->k_part > 0) || !(p->q_min > 0)) return ffail("k_part and q_min must be > 0");
+    // the potential: the temperature (thermal = 1, evolution_thermal) or the solute
+    const int32_t* bcp = p->thermal ? p->bc_h : p->bc_c;
     for (int f = 0; f < 6; f++) {
         if (p->bc_g[f] != MKG_BC_PERIODIC && p->bc_g[f] != MKG_BC_BOUNCE)
             return ffail("finite-volume phase faces are periodic or bounce (zero flux)");
-        if (p->bc_c[f] != MKG_BC_PERIODIC && p->bc_c[f] != MKG_BC_BOUNCE &&
-            p->bc_c[f] != MKG_BC_DIRICHLET)
-            return ffail("finite-volume solute faces are periodic, bounce (zero flux) or Dirichlet");
+        if (bcp[f] != MKG_BC_PERIODIC && bcp[f] != MKG_BC_BOUNCE && bcp[f] != MKG_BC_DIRICHLET)
+            return ffail("finite-volume solute / thermal faces are periodic, bounce (zero flux) "
+                         "or Dirichlet");
     }
     for (int a = 0; a < 3; a++) {
-        const bool pg = p->bc_g[2 * a] == 0, pc = p->bc_c[2 * a] == 0;
-        if (pg != (p->bc_g[2 * a + 1] == 0) || pc != (p->bc_c[2 * a + 1] == 0))
+        const bool pg = p->bc_g[2 * a] == 0, pc = bcp[2 * a] == 0;
+        if (pg != (p->bc_g[2 * a + 1] == 0) || pc != (bcp[2 * a + 1] == 0))
             return ffail("a periodic axis needs periodic codes on both faces");
-        if (pg != pc) return ffail("the phase and solute fields must be periodic on the same axes");
+        if (pg != pc)
+            return ffail("the phase and solute / thermal fields must be periodic on the same axes");
     }
+    if (p->thermal && (!(p->alpha_l > 0) || !(p->alpha_s > 0) || p->alpha_l > 1.0 / 6.0 ||
+                       p->alpha_s > 1.0 / 6.0))
+        return ffail("the explicit heat equation needs 0 < alpha_l, alpha_s <= 1/6");
     const double n = (double)p->nx * p->ny * p->nz;
     const double esz = p->precision == MKG_PREC_FP32 ? 4.0 : 8.0;
     if ((n + 65536.0) * esz >= 4294967296.0) return ffail("grid too large for 32-bit offsets");
@@ -549,6 +619,7 @@ int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ffail("no HIP device available");
     mkg_fvm_ctx* h = new mkg_fvm_ctx();
     h->p = *p;
+    h->thermal = p->thermal ? 1 : 0;
     h->fp32 = p->precision == MKG_PREC_FP32;
     h->esz = h->fp32 ? 4 : 8;
     h->N = (uint32_t)n;
@@ -591,7 +662,9 @@ int mkg_fvm_destroy(mkg_fvm_handle h) {
 int mkg_fvm_set(mkg_fvm_handle h, int which, const double* host, size_t count) {
     if (!h) return ffail("null handle");
     if (!host) return ffail("host buffer is null");
-    if (which != MKG_FVM_PHI && which != MKG_FVM_CONC) return ffail("set takes MKG_FVM_PHI or MKG_FVM_CONC");
+    if (which != MKG_FVM_PHI && which != (h->thermal ? MKG_FVM_TEMP : MKG_FVM_CONC))
+        return ffail(h->thermal ? "set takes MKG_FVM_PHI or MKG_FVM_TEMP (thermal handle)"
+                                : "set takes MKG_FVM_PHI or MKG_FVM_CONC");
     if (count != h->N) return ffail("count does not match the field size");
     void* dst = which == MKG_FVM_PHI ? h->phi[h->cur] : h->c[h->cur];
     FCHK(hipMemcpyAsync(h->scratch, host, h->N * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -616,7 +689,11 @@ int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
             else if (evolve_phase)
                 k_fvm_phase<float><<<grid, block, 0, h->stream>>>(
                     a, (const float*)h->phi[o], (const float*)h->c[o], (float*)phin);
-            if (h->march_blk)
+            if (h->thermal)
+                k_fvm_thermal<float><<<grid, block, 0, h->stream>>>(
+                    a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
+                    (float*)h->c[n]);
+            else if (h->march_blk)
                 k_fvm_solute_march<float, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
                     a, (const float*)h->phi[o], (const float*)phin, (const float*)h->c[o],
                     (float*)h->c[n]);
@@ -631,7 +708,11 @@ int mkg_fvm_step(mkg_fvm_handle h, int nsteps, int evolve_phase) {
             else if (evolve_phase)
                 k_fvm_phase<double><<<grid, block, 0, h->stream>>>(
                     a, (const double*)h->phi[o], (const double*)h->c[o], (double*)phin);
-            if (h->march_blk)
+            if (h->thermal)
+                k_fvm_thermal<double><<<grid, block, 0, h->stream>>>(
+                    a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
+                    (double*)h->c[n]);
+            else if (h->march_blk)
                 k_fvm_solute_march<double, kFvmXC><<<h->march_blk, 256, 0, h->stream>>>(
                     a, (const double*)h->phi[o], (const double*)phin, (const double*)h->c[o],
                     (double*)h->c[n]);
@@ -659,14 +740,15 @@ int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count) {
     if (!h) return ffail("null handle");
     if (!host) return ffail("host buffer is null");
     if (count != h->N) return ffail("count does not match the field size");
-    if (which == MKG_FVM_PHI || which == MKG_FVM_CONC) {
+    const int pot = h->thermal ? MKG_FVM_TEMP : MKG_FVM_CONC;
+    if (which == MKG_FVM_PHI || which == pot) {
         const void* src = which == MKG_FVM_PHI ? h->phi[h->cur] : h->c[h->cur];
         launch_to_double(h->fp32, src, h->scratch, h->N, h->stream);
         FCHK(hipMemcpyAsync(host, h->scratch, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         FCHK(hipStreamSynchronize(h->stream));
         return 0;
     }
-    if (which == MKG_FVM_U) {
+    if (which == MKG_FVM_U && !h->thermal) {
         if (mkg_fvm_get(h, MKG_FVM_CONC, host, count)) return -1;
         double* ph = new double[count];
         const int rc = mkg_fvm_get(h, MKG_FVM_PHI, ph, count);
@@ -677,7 +759,8 @@ int mkg_fvm_get(mkg_fvm_handle h, int which, double* host, size_t count) {
         delete[] ph;
         return rc;
     }
-    return ffail("unknown finite-volume field id");
+    return ffail(h->thermal ? "a thermal handle has MKG_FVM_PHI and MKG_FVM_TEMP"
+                            : "a solute handle has MKG_FVM_PHI, MKG_FVM_CONC and MKG_FVM_U");
 }
 
 int mkg_fvm_sums(mkg_fvm_handle h, double* out) {

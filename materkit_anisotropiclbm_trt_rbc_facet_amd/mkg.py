@@ -90,7 +90,7 @@ class MkgParams(C.Structure):
         return int(self.nx) * int(self.ny) * int(self.nz)
 
 
-FVM_PHI, FVM_CONC, FVM_U = 0, 1, 2
+FVM_PHI, FVM_CONC, FVM_U, FVM_TEMP = 0, 1, 2, 3
 MP_POP_FLOW, MP_POP_PHASE, MP_PHI, MP_MU, MP_PRESSURE, MP_VEL, MP_PHIU = range(7)
 MP_VISC_NU_LINEAR, MP_VISC_MU_LINEAR, MP_VISC_MU_HARMONIC = range(3)
 MPF_MAX = 4
@@ -547,7 +547,8 @@ class MpfSolver:
 
 class FvmSolver:
     """Explicit finite-volume phase field + solute with anti-trapping current on the device
-    (wraps an ``mkg_fvm_handle``; evolution_anisotropy / evolution_ATC1-2)."""
+    (wraps an ``mkg_fvm_handle``; evolution_anisotropy / evolution_ATC1-2, or with
+    ``thermal = 1`` evolution_thermal: the phase driven by T and the heat equation)."""
 
     def __init__(self, params: MkgParams):
         self.params = params.copy()

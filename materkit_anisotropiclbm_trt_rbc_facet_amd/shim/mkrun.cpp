@@ -7,14 +7,15 @@
 //
 //   mkrun <case.xml> <loop> [maxLoops] [maxFrame] [vtk | plt]
 //   loop: poiseuille | thermal_dendrite | flow_thermal_dendrite | flow_solutal_dendrite |
-//         solutal_dendrite | solutal_only | rbc | allen_cahn | multi_phase_field |
-//         grains | multiphase | wetting | fused
+//         solutal_dendrite | solutal_only | thermal_fvm | rbc | allen_cahn |
+//         multi_phase_field | grains | multiphase | wetting | fused
 // (allen_cahn: run_flow_AllenCahn, MaterKit.cpp:186-242; multi_phase_field:
 // run_multiPhaseField, MaterKit.cpp:529-576, on mkFluidsUser; grains: the same loop on the
 // anisotropic multi-grain class mkMultiphaseField with relaxation_anisotropy_thermal or
 // relaxation_solute, as the case drives the grains; multiphase: run_flow_multiphase,
 // MaterKit.cpp:290-331, and wetting: run_flow_multiphase_wetting, :333-375, on
-// mkFluidsMultiphase)
+// mkFluidsMultiphase; thermal_fvm: evolution_anisotropy and evolution_thermal, the
+// finite-volume phase field with the temperature as its potential)
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -230,6 +231,10 @@ int main(int argc, char** argv) {
                 flow.mkSolutalField::relaxation();
             } else if (loop == "solutal_only") {
                 flow.mkSolutalField::evolution_ATC2(flow.node);
+            } else if (loop == "thermal_fvm") {
+                // finite-volume dendrite: the phase field and the temperature as the potential
+                flow.mkPhaseField::evolution_anisotropy();
+                flow.mkSolutalField::evolution_thermal(flow.node);
             } else if (loop == "rbc") {
                 flow.mkFluidsMRT::propagate();
                 flow.mkFluidsMRT::relaxation();

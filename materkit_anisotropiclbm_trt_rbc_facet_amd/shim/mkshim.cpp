@@ -64,41 +64,47 @@ mkFluidsLBM::~mkFluidsLBM() {
     if (h_) mkg_destroy(h_);
 }
 
-void mkFluidsLBM::fvm_evolve(bool anti_trapping) {
+void mkFluidsLBM::fvm_evolve(int kind) {
     if (!h_) throw mkt::mkLogicException("evolution before initialize");
-    if (!p_.phase_q || !p_.solute)
-        throw mkt::mkLogicException("the finite-volume updates need the phase and solute fields");
-    const int k = anti_trapping ? 1 : 0;
+    const bool thermal = kind == kFvmThermal;
+    if (!p_.phase_q || !(thermal ? p_.thermal : p_.solute))
+        throw mkt::mkLogicException(thermal ? "evolution_thermal needs the phase and thermal fields"
+                                            : "the finite-volume updates need the phase and "
+                                              "solute fields");
     const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
-    if (fvm_live_ != k) {
-        std::vector<double> phi(N), c(N);
-        if (fvm_live_ >= 0) {  // the other handle holds the newest values
+    if (fvm_live_ != kind) {
+        const int fid = thermal ? MKG_FVM_TEMP : MKG_FVM_CONC;
+        std::vector<double> phi(N), v(N);
+        if (fvm_live_ >= 0 && (fvm_live_ == kFvmThermal) == thermal) {
+            // the other solute handle holds the newest phi and c
             check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_PHI, phi.data(), N));
-            check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_CONC, c.data(), N));
+            check(mkg_fvm_get(fvm_[fvm_live_], fid, v.data(), N));
         } else {
+            fvm_flush();  // a handle of the other potential hands its fields to the lattices
             check(mkg_get_field(h_, MKG_F_PHI, phi.data(), MKG_DT_F64, N));
-            check(mkg_get_field(h_, MKG_F_CONC, c.data(), MKG_DT_F64, N));
+            check(mkg_get_field(h_, thermal ? MKG_F_TEMP : MKG_F_CONC, v.data(), MKG_DT_F64, N));
         }
-        if (!fvm_[k]) {
+        if (!fvm_[kind]) {
             mkg_params q = p_;
             q.ngrains = 0;
-            if (!anti_trapping) q.at_coef = 0.0;
-            check(mkg_fvm_create(&q, &fvm_[k]));
+            q.thermal = thermal ? 1 : 0;
+            if (kind != kFvmSoluteAT) q.at_coef = 0.0;
+            check(mkg_fvm_create(&q, &fvm_[kind]));
         }
-        check(mkg_fvm_set(fvm_[k], MKG_FVM_PHI, phi.data(), N));
-        check(mkg_fvm_set(fvm_[k], MKG_FVM_CONC, c.data(), N));
-        fvm_live_ = k;
+        check(mkg_fvm_set(fvm_[kind], MKG_FVM_PHI, phi.data(), N));
+        check(mkg_fvm_set(fvm_[kind], fid, v.data(), N));
+        fvm_live_ = kind;
     }
-    check(mkg_fvm_step(fvm_[k], 1, fvm_phase_next_ ? 1 : 0));
+    check(mkg_fvm_step(fvm_[kind], 1, fvm_phase_next_ ? 1 : 0));
     fvm_phase_next_ = false;
 }
 
 void mkFluidsLBM::fvm_flush() const {
     if (fvm_live_ < 0) return;
+    const bool thermal = fvm_live_ == kFvmThermal;
     const size_t N = (size_t)p_.nx * p_.ny * p_.nz;
-    std::vector<double> phi(N), c(N), rho, u, T;
+    std::vector<double> phi(N), rho, u, T, c;
     check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_PHI, phi.data(), N));
-    check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_CONC, c.data(), N));
     if (p_.flow_q) {
         rho.resize(N);
         u.resize(3 * N);
@@ -107,10 +113,17 @@ void mkFluidsLBM::fvm_flush() const {
     }
     if (p_.thermal) {
         T.resize(N);
-        check(mkg_get_field(h_, MKG_F_TEMP, T.data(), MKG_DT_F64, N));
+        if (thermal) check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_TEMP, T.data(), N));
+        else check(mkg_get_field(h_, MKG_F_TEMP, T.data(), MKG_DT_F64, N));
+    }
+    if (p_.solute) {
+        c.resize(N);
+        if (!thermal) check(mkg_fvm_get(fvm_[fvm_live_], MKG_FVM_CONC, c.data(), N));
+        else check(mkg_get_field(h_, MKG_F_CONC, c.data(), MKG_DT_F64, N));
     }
     check(mkg_init_equilibrium(h_, p_.flow_q ? rho.data() : nullptr, p_.flow_q ? u.data() : nullptr,
-                               p_.thermal ? T.data() : nullptr, c.data(), phi.data()));
+                               p_.thermal ? T.data() : nullptr, p_.solute ? c.data() : nullptr,
+                               phi.data()));
     fvm_live_ = -1;
 }
 

@@ -186,14 +186,17 @@ protected:
     virtual void advance(int mask);
     void check(int rc) const;
     // Finite-volume operators (mkPhaseField::evolution_anisotropy, mkSolutalField::evolution*):
-    // an mkg_fvm handle per anti-trapping choice, created at the first call from the case's
-    // parameters (without grain rotations).  It takes phi and c from the lattices when it
-    // starts and keeps the newest values while only finite-volume calls follow; the next
-    // lattice step or field read hands them back (the lattices restart at the equilibrium of
-    // the current fields).  A phase call marks the next solute call to advance phi with c.
-    void fvm_evolve(bool anti_trapping);
+    // an mkg_fvm handle per kind (solute without / with the anti-trapping current, or the
+    // temperature as the potential), created at the first call from the case's parameters
+    // (without grain rotations).  It takes phi and its potential (c or T) from the lattices
+    // when it starts and keeps the newest values while only finite-volume calls follow; the
+    // next lattice step or field read hands them back (the lattices restart at the
+    // equilibrium of the current fields).  A phase call marks the next potential call to
+    // advance phi with it.
+    enum { kFvmSolute = 0, kFvmSoluteAT = 1, kFvmThermal = 2 };
+    void fvm_evolve(int kind);
     void fvm_flush() const;
-    mkg_fvm_handle fvm_[2] = {nullptr, nullptr};
+    mkg_fvm_handle fvm_[3] = {nullptr, nullptr, nullptr};
     mutable int fvm_live_ = -1;  // handle holding the newest phi and c; -1: the lattices
     bool fvm_phase_next_ = false;
     mkSpace space_;
@@ -237,10 +240,14 @@ public:
     void relaxation() override { advance(MKG_SOLUTE); }
     void relaxation_anisotropy() { advance(MKG_SOLUTE); }
     // finite-volume solute updates (MaterKit/inc/mkSolutalField.h:41-51): with the
-    // anti-trapping current (ATC1 / ATC2) or without it (evolution)
-    void evolution(const mkNode* /*nptr*/) { fvm_evolve(false); }
-    void evolution_ATC1(const mkNode* /*nptr*/) { fvm_evolve(true); }
-    void evolution_ATC2(const mkNode* /*nptr*/) { fvm_evolve(true); }
+    // anti-trapping current (ATC1 / ATC2) or without it (evolution; evolution_old, the
+    // earlier entry of the same update); evolution_thermal takes the temperature as the
+    // potential ("potential is temperature", :47) and needs the thermal lattice
+    void evolution(const mkNode* /*nptr*/) { fvm_evolve(kFvmSolute); }
+    void evolution_old(const mkNode* /*nptr*/) { fvm_evolve(kFvmSolute); }
+    void evolution_ATC1(const mkNode* /*nptr*/) { fvm_evolve(kFvmSoluteAT); }
+    void evolution_ATC2(const mkNode* /*nptr*/) { fvm_evolve(kFvmSoluteAT); }
+    void evolution_thermal(const mkNode* /*nptr*/) { fvm_evolve(kFvmThermal); }
     void updateDCs(const mkNode* /*nptr*/) {}  // D(phi) is evaluated inside every solute update
     void bdy_constX() {}
     void bdy_RBC() {}
@@ -325,6 +332,8 @@ public:
     void propagate() override { pending_ |= MKG_FLOW; }
     void propagate_PBB() override { pending_ |= MKG_FLOW; }
     void relaxation() override { advance(MKG_FLOW); }
+    // the node-array form (MaterKit/inc/mkFluidsMRT.h:46): the node array lives on the device
+    void relaxation(const mkNode* /*nptr*/) { advance(MKG_FLOW); }
     void initialize(const std::string& xmlfname) override { mkFluidsLBM::initialize(xmlfname); }
     void initialize_geometry() override { mkFluidsLBM::initialize_geometry(); }
 };

@@ -1344,6 +1344,37 @@ static double fvm_A(const orc_params* p, double phi) {
     return 0.5 * ((1.0 + p->k_part) - (1.0 - p->k_part) * phi);
 }
 
+/* The phase pass of one finite-volume step: phin = phi + S + dt W^2/dx^2 lap7 phi with the
+ * lattice solver's source S driven by u[x] (evolve_phase 0: phin = phi). */
+static void fvm_phase_pass(const orc_params* p, const double* phi, const double* u, double* phin,
+                           int evolve_phase) {
+    int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int X = 0; X < p->nx; X++)
+        for (int Y = 0; Y < p->ny; Y++)
+            for (int Z = 0; Z < p->nz; Z++) {
+                long id = lin(p, X, Y, Z);
+                if (!evolve_phase) {
+                    phin[id] = phi[id];
+                    continue;
+                }
+                double phi27[27];
+                for (int dx = -1; dx <= 1; dx++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dz = -1; dz <= 1; dz++) {
+                            int XX = clampwrap(X + dx, p->nx, per[0]);
+                            int YY = clampwrap(Y + dy, p->ny, per[1]);
+                            int ZZ = clampwrap(Z + dz, p->nz, per[2]);
+                            P27(dx, dy, dz) = phi[lin(p, XX, YY, ZZ)];
+                        }
+                double S, a;
+                orc_phase_source(p, phi27, u[id], 0, &S, &a);
+                double lap = P27(1, 0, 0) + P27(-1, 0, 0) + P27(0, 1, 0) + P27(0, -1, 0) +
+                             P27(0, 0, 1) + P27(0, 0, -1) - 6.0 * P27(0, 0, 0);
+                phin[id] = phi[id] + S + p->pf_dt * p->pf_wdx2 * lap;
+            }
+}
+
 int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int evolve_phase) {
     long N = NCELL(p);
     int per[3] = {p->bc_g[0] == 0, p->bc_g[2] == 0, p->bc_g[4] == 0};
@@ -1355,31 +1386,7 @@ int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int ev
     int dims[3] = {p->nx, p->ny, p->nz};
     for (int step = 0; step < nsteps; step++) {
         for (long x = 0; x < N; x++) U[x] = c[x] / fvm_A(p, phi[x]);
-        /* phase */
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int X = 0; X < p->nx; X++)
-            for (int Y = 0; Y < p->ny; Y++)
-                for (int Z = 0; Z < p->nz; Z++) {
-                    long id = lin(p, X, Y, Z);
-                    if (!evolve_phase) {
-                        phin[id] = phi[id];
-                        continue;
-                    }
-                    double phi27[27];
-                    for (int dx = -1; dx <= 1; dx++)
-                        for (int dy = -1; dy <= 1; dy++)
-                            for (int dz = -1; dz <= 1; dz++) {
-                                int XX = clampwrap(X + dx, p->nx, per[0]);
-                                int YY = clampwrap(Y + dy, p->ny, per[1]);
-                                int ZZ = clampwrap(Z + dz, p->nz, per[2]);
-                                P27(dx, dy, dz) = phi[lin(p, XX, YY, ZZ)];
-                            }
-                    double S, a;
-                    orc_phase_source(p, phi27, U[id], 0, &S, &a);
-                    double lap = P27(1, 0, 0) + P27(-1, 0, 0) + P27(0, 1, 0) + P27(0, -1, 0) +
-                                 P27(0, 0, 1) + P27(0, 0, -1) - 6.0 * P27(0, 0, 0);
-                    phin[id] = phi[id] + S + p->pf_dt * p->pf_wdx2 * lap;
-                }
+        fvm_phase_pass(p, phi, U, phin, evolve_phase);
         /* solute */
 #pragma omp parallel for collapse(2) schedule(static)
         for (int X = 0; X < p->nx; X++)
@@ -1442,6 +1449,66 @@ int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int ev
     free(phin);
     free(cn);
     free(U);
+    return 0;
+}
+
+/* Finite-volume phase field driven by the temperature (mkSolutalField::evolution_thermal,
+ * MaterKit/inc/mkSolutalField.h:47-48: "potential is temperature"): the phase pass of
+ * orc_fvm_step with u = T, then the heat equation with the latent source on the same faces,
+ *   T' = T - sum_faces J_out + L_T (phi' - phi),   J_out = -alpha_f (T_nb - T),
+ *   alpha_f = (alpha(phi) + alpha(phi_nb))/2,  alpha = alpha_l (1 - f_s) + alpha_s f_s,
+ *   f_s = (1 + phi)/2 clamped to [0, 1]  (the thermal lattice's alpha(phi));
+ *   faces bc_h: periodic; bounce = zero flux; Dirichlet fixes T at the face:
+ *   J_out = -2 alpha(phi) (T_w - T).
+ * T - L_T phi is conserved exactly with periodic and zero-flux faces. */
+int orc_fvm_thermal_step(const orc_params* p, double* phi, double* T, int nsteps,
+                         int evolve_phase) {
+    long N = NCELL(p);
+    int pert[3] = {p->bc_h[0] == 0, p->bc_h[2] == 0, p->bc_h[4] == 0};
+    double* phin = malloc((size_t)N * sizeof(double));
+    double* Tn = malloc((size_t)N * sizeof(double));
+    int dims[3] = {p->nx, p->ny, p->nz};
+    for (int step = 0; step < nsteps; step++) {
+        fvm_phase_pass(p, phi, T, phin, evolve_phase);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int X = 0; X < p->nx; X++)
+            for (int Y = 0; Y < p->ny; Y++)
+                for (int Z = 0; Z < p->nz; Z++) {
+                    long id = lin(p, X, Y, Z);
+                    double a0;
+                    {
+                        double fs = 0.5 * (1.0 + phi[id]);
+                        fs = fs < 0 ? 0 : (fs > 1 ? 1 : fs);
+                        a0 = p->alpha_l * (1.0 - fs) + p->alpha_s * fs;
+                    }
+                    double out = 0;
+                    for (int ax = 0; ax < 3; ax++)
+                        for (int s = 1; s >= -1; s -= 2) {
+                            int Q3[3] = {X, Y, Z};
+                            Q3[ax] += s;
+                            int face = 2 * ax + (s > 0 ? 1 : 0);
+                            if (Q3[ax] < 0 || Q3[ax] >= dims[ax]) {
+                                if (pert[ax]) {
+                                    Q3[ax] = (Q3[ax] + dims[ax]) % dims[ax];
+                                } else {
+                                    if (p->bc_h[face] == 2) /* Dirichlet: T fixed at the face */
+                                        out += -2.0 * a0 * (p->T_face[face] - T[id]);
+                                    continue;
+                                }
+                            }
+                            long nb = lin(p, Q3[0], Q3[1], Q3[2]);
+                            double fs = 0.5 * (1.0 + phi[nb]);
+                            fs = fs < 0 ? 0 : (fs > 1 ? 1 : fs);
+                            double an = p->alpha_l * (1.0 - fs) + p->alpha_s * fs;
+                            out += -0.5 * (a0 + an) * (T[nb] - T[id]);
+                        }
+                    Tn[id] = T[id] - out + p->latent_T * (phin[id] - phi[id]);
+                }
+        memcpy(phi, phin, (size_t)N * sizeof(double));
+        memcpy(T, Tn, (size_t)N * sizeof(double));
+    }
+    free(phin);
+    free(Tn);
     return 0;
 }
 

@@ -142,6 +142,13 @@ int orc_mpf_step(const orc_mpf_params* m, double* g, double* phi, double* phi_pr
  * C_face as U values). */
 int orc_fvm_step(const orc_params* p, double* phi, double* c, int nsteps, int evolve_phase);
 
+/* Finite-volume phase field driven by the temperature, and the heat equation with the latent
+ * source (evolution_thermal): phi and T are N doubles each, advanced in place.  Uses the
+ * phase parameters as orc_fvm_step does, with u = T, and alpha_l, alpha_s, latent_T, bc_h,
+ * T_face. */
+int orc_fvm_thermal_step(const orc_params* p, double* phi, double* T, int nsteps,
+                         int evolve_phase);
+
 /* Two-phase flow (mkFluidsMultiphase; Liang et al. 2014 phase-field model): Cahn-Hilliard
  * lattice h and velocity-pressure flow lattice f, both D3Q19.  Same layout as the product's
  * mkg_mp_params. */

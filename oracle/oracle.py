@@ -108,6 +108,8 @@ def _bind(L: ct.CDLL) -> ct.CDLL:
     L.orc_mp_macro.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.orc_fvm_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
     L.orc_fvm_step.restype = ct.c_int
+    L.orc_fvm_thermal_step.argtypes = [vp, vp, vp, ct.c_int, ct.c_int]
+    L.orc_fvm_thermal_step.restype = ct.c_int
     L.orc_mpf_init.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.orc_mpf_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, ct.c_int]
     L.orc_mpf_step.restype = ct.c_int
@@ -335,6 +337,17 @@ def fvm_step(params, phi, c, nsteps: int = 1, evolve_phase: bool = True) -> None
     assert phi.dtype == np.float64 and c.dtype == np.float64
     assert phi.flags.c_contiguous and c.flags.c_contiguous
     rc = lib().orc_fvm_step(ct.byref(params), _p(phi), _p(c), int(nsteps), 1 if evolve_phase else 0)
+    if rc != 0:
+        raise RuntimeError("oracle rejected the finite-volume parameter block")
+
+
+def fvm_thermal_step(params, phi, T, nsteps: int = 1, evolve_phase: bool = True) -> None:
+    """Explicit finite-volume phase field driven by T and the heat equation with the latent
+    source (in place on float64 arrays of N values)."""
+    assert phi.dtype == np.float64 and T.dtype == np.float64
+    assert phi.flags.c_contiguous and T.flags.c_contiguous
+    rc = lib().orc_fvm_thermal_step(ct.byref(params), _p(phi), _p(T), int(nsteps),
+                                    1 if evolve_phase else 0)
     if rc != 0:
         raise RuntimeError("oracle rejected the finite-volume parameter block")
 

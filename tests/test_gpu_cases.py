@@ -517,6 +517,40 @@ def test_reference_api_driver_runs_the_solutal_loops(tmp_path):
     assert np.abs(c - c1).max() < 1e-7
 
 
+def test_reference_api_driver_runs_the_thermal_finite_volume_loop(tmp_path):
+    """mkrun thermal_fvm (evolution_anisotropy + evolution_thermal, the finite-volume phase
+    field with the temperature as its potential) grows a 2-D dendrite seed in the undercooled
+    melt, conserves T - L_T phi, and its last frame equals the finite-volume solver run from
+    the driver's first frame."""
+    exe = ROOT / "materkit_anisotropiclbm_trt_rbc_facet_amd" / "mkrun"
+    xml = tmp_path / "thermal_fvm.xml"
+    xml.write_text((ROOT / "cases" / "C3_free_dendrite.xml").read_text()
+                   .replace("<MeshXYZ> 256   256   256 </MeshXYZ>", "<MeshXYZ> 64 64 1 </MeshXYZ>")
+                   .replace("<center> 127.5 127.5 127.5 </center>", "<center> 31.5 31.5 0 </center>")
+                   .replace("<precision> fp32 </precision>", "<precision> fp64 </precision>"))
+    res = subprocess.run([str(exe), str(xml), "thermal_fvm", "400", "1", "vtk"],
+                         capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert res.returncode == 0, res.stderr
+    vtk = tmp_path / "vtk"
+    phi0 = _vtk_scalar(vtk / "fluid_t0.vtk", "order_parameter", 64, 64)
+    T0 = _vtk_scalar(vtk / "fluid_t0.vtk", "temperature", 64, 64)
+    phi1 = _vtk_scalar(vtk / "fluid_t400.vtk", "order_parameter", 64, 64)
+    T1 = _vtk_scalar(vtk / "fluid_t400.vtk", "temperature", 64, 64)
+    assert phi1.sum() > phi0.sum() + 20.0  # the seed grows
+    p, _ = mkg.params_from_xml(xml)
+    L = p.latent_T
+    # periodic faces: T - L phi is conserved (to the frames' 9 significant digits)
+    assert abs((T1 - L * phi1).sum() - (T0 - L * phi0).sum()) < 1e-5 * T0.size
+    p.ngrains = 0
+    f = mkg.FvmSolver(p)
+    f.set(mkg.FVM_PHI, phi0)
+    f.set(mkg.FVM_TEMP, T0)
+    f.step(400, evolve_phase=True)
+    phi, T = f.get(mkg.FVM_PHI), f.get(mkg.FVM_TEMP)
+    f.close()
+    assert np.abs(phi - phi1).max() < 1e-6 and np.abs(T - T1).max() < 1e-6
+
+
 def test_reference_api_driver_writes_tecplot_and_plane_frames(tmp_path):
     """mkrun with 'plt': write_pltfile (every cell), write_pltfile_2D and write_vtkfile_2D
     (mid-z plane) hold the same values as the full VTK frame of the same state."""

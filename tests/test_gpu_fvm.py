@@ -110,3 +110,59 @@ def test_fvm_marching_kernels_agree_with_per_cell_kernels(name, prec, tol, monke
         dev.close()
     assert np.abs(out[0][0] - out[1][0]).max() <= tol
     assert np.abs(out[0][1] - out[1][1]).max() <= tol
+
+
+# ---- temperature as the potential (mkSolutalField::evolution_thermal,
+#      MaterKit/inc/mkSolutalField.h:47-48)
+
+TCASES = {
+    "periodic": dict(dims=(14, 12, 10), bc_g=[P] * 6, bc_h=[P] * 6),
+    "dirichlet": dict(dims=(12, 11, 9), bc_g=[BB, BB, P, P, BB, BB], bc_h=[DI, BB, P, P, BB, DI]),
+    "march_faces": dict(dims=(10, 4, 64), bc_g=[BB] * 6, bc_h=[DI, BB, BB, BB, BB, DI]),
+}
+
+
+def make_thermal(name, prec):
+    cfg = TCASES[name]
+    d = cfg["dims"]
+    p = mkg.default_params(nx=d[0], ny=d[1], nz=d[2], phase_q=19, thermal=1, pf_coupling=1,
+                           eps4=0.05, pf_dt=0.008, pf_wdx2=6.25, alpha_l=0.12, alpha_s=0.07,
+                           latent_T=0.5, phi_s=0.99, precision=prec, **{"lambda": 3.19})
+    p.set(bc_g=cfg["bc_g"], bc_h=cfg["bc_h"])
+    p.T_face[0], p.T_face[5] = -0.4, -0.7
+    return p
+
+
+@pytest.mark.parametrize("evolve", [True, False], ids=["evolve", "thermal_only"])
+@pytest.mark.parametrize("prec,tol", [(mkg.PREC_FP64, 1e-12), (mkg.PREC_FP32, 1e-5)],
+                         ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", sorted(TCASES))
+def test_fvm_thermal_matches_oracle(name, prec, tol, evolve):
+    p = make_thermal(name, prec)
+    phi, _ = start(p)
+    T = np.ascontiguousarray(-0.55 + 0.03 * np.random.default_rng(8).standard_normal(phi.size))
+    print(f"fvm thermal {name} prec {prec} evolve {evolve}", flush=True)
+    dev = mkg.FvmSolver(p)
+    dev.set(mkg.FVM_PHI, phi)
+    dev.set(mkg.FVM_TEMP, T)
+    dev.step(7, evolve)
+    O.fvm_thermal_step(make_thermal(name, mkg.PREC_FP64), phi, T, 7, evolve)
+    gp, gT = dev.get(mkg.FVM_PHI), dev.get(mkg.FVM_TEMP)
+    s = dev.sums()
+    dev.close()
+    assert np.abs(gp - phi).max() < tol * np.abs(phi).max()
+    assert np.abs(gT - T).max() < tol * np.abs(T).max()
+    np.testing.assert_allclose(s, [phi.sum(), T.sum()], rtol=max(tol, 1e-12) * 10, atol=1e-6)
+
+
+def test_fvm_thermal_handle_fields_and_limits():
+    p = make_thermal("periodic", mkg.PREC_FP64)
+    dev = mkg.FvmSolver(p)
+    with pytest.raises(mkg.MkgError, match="MKG_FVM_TEMP"):
+        dev.set(mkg.FVM_CONC, np.zeros(dev.N))
+    with pytest.raises(mkg.MkgError, match="thermal handle"):
+        dev.get(mkg.FVM_U)
+    dev.close()
+    p.alpha_l = 0.2
+    with pytest.raises(mkg.MkgError, match="1/6"):
+        mkg.FvmSolver(p)

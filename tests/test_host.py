@@ -253,6 +253,9 @@ static void loops(mkFluidsMRT& flow, mkFluidsUser& user, mkFluidsMultiphase& two
     flow.mkSolutalField::evolution(flow.node);
     flow.mkSolutalField::evolution_ATC1(flow.node);
     flow.mkSolutalField::evolution_ATC2(flow.node);
+    flow.mkSolutalField::evolution_old(flow.node);
+    flow.mkSolutalField::evolution_thermal(flow.node);
+    flow.mkFluidsMRT::relaxation(flow.node);
     flow.mkSolutalField::updateDCs(flow.node);
     flow.mkSolutalField::bdy_LBflowin();
     flow.check_variables(loops, 'x');
