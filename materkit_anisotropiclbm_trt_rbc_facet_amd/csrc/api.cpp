@@ -34,9 +34,10 @@ int fail(const std::string& msg) {
     } while (0)
 
 enum { LF = 0, LG = 1, LH = 2, LC = 3 };
-// Elements added to every slot stride.  Measured on the fused 256^3 FP32 step (k_step, two
-// boxes): pad 0 gives 0.689 of HBM peak, 1024 gives 0.701-0.707 (256: 0.700, 512: 0.692,
-// 2048: 0.696, 4096: 0.693).
+// Elements added to the stride S of the per-cell arrays, which is also the slot stride of the
+// slot-major population layout.  Measured on the fused 256^3 FP32 step with one array per
+// slot (k_step, two boxes): pad 0 gives 0.689 of HBM peak, 1024 gives 0.701-0.707 (256:
+// 0.700, 512: 0.692, 2048: 0.696, 4096: 0.693).
 constexpr uint32_t kSlotPad = 1024;
 // Cells per page of the paged population layout (2^kPageShift).  A 45-slot in-place update at
 // 256^3 (FP32) streams at 6.6 TB/s with pages of 2048-16384 cells holding every slot of every
@@ -503,9 +504,10 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
     h->fp32 = p->precision == MKG_PREC_FP32;
     h->esz = h->fp32 ? 4 : 8;
     h->N = (uint32_t)((size_t)p->nx * p->ny * p->nz);
-    // Slot stride: the cell count rounded up to 128 elements plus a pad, so that the slots of
-    // one lattice do not all start at the same offset modulo a large power of two (which would
-    // send the Q loads of a cell to the same memory channel).  MKG_SLOT_PAD overrides the pad.
+    // S: the cell count rounded up to 128 elements plus a pad, the stride of the per-cell
+    // arrays and of the slot-major population layout, so that its slots do not all start at
+    // the same offset modulo a large power of two (which would send the Q loads of a cell to
+    // the same memory channel).  MKG_SLOT_PAD overrides the pad.
     {
         uint32_t pad = kSlotPad;
         if (const char* env = std::getenv("MKG_SLOT_PAD")) pad = (uint32_t)std::strtoul(env, nullptr, 10);
