@@ -64,15 +64,20 @@ PopBlock plan_pop_block(const int* q, int nlat, uint32_t N, uint32_t S, uint32_t
     for (int k = 0; k < nlat; k++) qt += (size_t)q[k];
     if (qt == 0) return b;
     const size_t npages = ((size_t)N + B - 1) / B;
-    const size_t paged_total = npages * qt * B + qt * F;
-    bool paged = paged_total * esz < ((size_t)1 << 32);
+    // FP32 kernels address every lattice through one shared page stride (pages hold every
+    // lattice); FP64 kernels take a page stride per lattice (pages hold one lattice)
+    const bool shared = esz == 4;
+    size_t qmax = 0;
+    for (int k = 0; k < nlat; k++) qmax = std::max(qmax, (size_t)q[k]);
+    const size_t span = npages * (shared ? qt : qmax) * B + qt * F;  // largest offset range
+    bool paged = span * esz < ((size_t)1 << 32);
     if (const char* env = std::getenv("MKG_PAGES")) paged = paged && std::atoi(env) != 0;
     size_t slots;
     if (paged) {
         b.paged = 1;
         b.pg_shift = shift;
         b.pg_mask = B - 1u;
-        b.pg_stride = (uint32_t)(qt * B);
+        b.pg_stride = shared ? (uint32_t)(qt * B) : 0u;
         b.sl_stride = B;
         slots = npages * qt * B;
     } else {
@@ -82,7 +87,14 @@ PopBlock plan_pop_block(const int* q, int nlat, uint32_t N, uint32_t S, uint32_t
     b.total = slots + qt * F;
     size_t off = 0, foff = slots;
     for (int k = 0; k < nlat; k++) {
-        b.slot0[k] = off * b.sl_stride;
+        if (paged && !shared) {
+            // lattice k: npages pages of its q[k] slots, from element off * B on
+            b.slot0[k] = off * npages * B;
+            b.lat_stride[k] = (uint32_t)((size_t)q[k] * B);
+        } else {
+            b.slot0[k] = off * b.sl_stride;
+            b.lat_stride[k] = b.pg_stride;
+        }
         b.face0[k] = foff;
         off += (size_t)q[k];
         foff += (size_t)q[k] * F;
@@ -109,6 +121,7 @@ struct mkg_ctx {
     uint32_t pg_shift = 31, pg_mask = 0x7FFFFFFFu, pg_stride = 0, sl_stride = 0;
     void* pop[4] = {nullptr, nullptr, nullptr, nullptr};
     void* face[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t lstride[4] = {0, 0, 0, 0};  // page stride of each lattice (FP64 kernels)
     int q[4] = {0, 0, 0, 0};
     int lay[4] = {0, 0, 0, 0};
     int64_t steps[4] = {0, 0, 0, 0};
@@ -162,6 +175,7 @@ void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const doub
     std::memset(&out, 0, sizeof(out));
     out.pop = h->pop[L];
     out.face = h->face[L];
+    out.pg_stride = h->lstride[L];
     int np = 0;
     for (int f = 0; f < 6; f++) {
         out.bc[f] = bc[f];
@@ -276,6 +290,7 @@ int alloc_pops(mkg_ctx* h) {
         if (!h->q[L]) continue;
         h->pop[L] = (char*)h->popbuf + b.slot0[L] * h->esz;
         h->face[L] = (char*)h->popbuf + b.face0[L] * h->esz;
+        h->lstride[L] = b.lat_stride[L];
     }
     return 0;
 }

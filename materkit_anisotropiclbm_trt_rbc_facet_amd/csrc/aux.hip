@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_store(const StepArgs a, const LatArgs L
     R* base = (R*)L.pop;
 #pragma unroll
     for (int i = 0; i < Q; i++)
-        *pop_at(base, i, c.id, a) = (R)(in[(size_t)i * a.N + c.id] - wgt<Q>(i) * L.ref);
+        *pop_at(base, i, c.id, a, L) = (R)(in[(size_t)i * a.N + c.id] - wgt<Q>(i) * L.ref);
 }
 
 template <class R>
@@ -161,13 +161,13 @@ __global__ void __launch_bounds__(256) k_init_eq(const StepArgs a, const double*
         const double t = T ? T[id] : a.T_ref;
         R* h = (R*)a.h.pop;
 #pragma unroll
-        for (int i = 0; i < 7; i++) *pop_at(h, i, id, a) = (R)(wgt<7>(i) * (t - a.h.ref));
+        for (int i = 0; i < 7; i++) *pop_at(h, i, id, a, a.h) = (R)(wgt<7>(i) * (t - a.h.ref));
     }
     if (a.c.pop) {
         const double cv = C ? C[id] : 0.0;
         R* p = (R*)a.c.pop;
 #pragma unroll
-        for (int i = 0; i < 7; i++) *pop_at(p, i, id, a) = (R)(wgt<7>(i) * (cv - a.c.ref));
+        for (int i = 0; i < 7; i++) *pop_at(p, i, id, a, a.c) = (R)(wgt<7>(i) * (cv - a.c.ref));
     }
 }
 
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_init_flow(const StepArgs a, const doubl
         const double feq =
             a.incomp ? wgt<Q>(i) * (r + a.rho0 * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu))
                      : wgt<Q>(i) * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-        *pop_at(f, i, id, a) = (R)(feq - wgt<Q>(i) * a.f.ref);
+        *pop_at(f, i, id, a, a.f) = (R)(feq - wgt<Q>(i) * a.f.ref);
     }
 }
 
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_init_phase(const StepArgs a, const doub
     R* g = (R*)a.g.pop;
     const double ph = phi[c.id];
 #pragma unroll
-    for (int i = 0; i < Q; i++) *pop_at(g, i, c.id, a) = (R)(wgt<Q>(i) * (ph - a.g.ref));
+    for (int i = 0; i < Q; i++) *pop_at(g, i, c.id, a, a.g) = (R)(wgt<Q>(i) * (ph - a.g.ref));
 }
 
 template <class R>

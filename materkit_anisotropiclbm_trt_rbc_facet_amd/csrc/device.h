@@ -215,10 +215,13 @@ __device__ inline const R* slot_at(const R* slot, uint32_t idx) {
     return (const R*)((const char*)slot + idx * (uint32_t)sizeof(R));
 }
 
-// Population addressing (StepArgs): the element offset of cell c within a slot, and slot i
-// of a lattice.  In the slot-major layout pcell(c) = c.
-__device__ inline uint32_t pcell(const StepArgs& a, uint32_t c) {
-    return (c >> a.pg_shift) * a.pg_stride + (c & a.pg_mask);
+// Population addressing (StepArgs, LatArgs): the element offset of cell c within a slot of
+// lattice L, and slot i of a lattice.  In the slot-major layout pcell(c) = c.  FP32 pages hold
+// every lattice (one stride for all, uniform across the lattices of a kernel); FP64 pages hold
+// one lattice each.
+template <class R>
+__device__ inline uint32_t pcell(const StepArgs& a, const LatArgs& L, uint32_t c) {
+    return (c >> a.pg_shift) * (sizeof(R) == 4 ? a.pg_stride : L.pg_stride) + (c & a.pg_mask);
 }
 template <class R>
 __device__ inline R* pslot(R* base, int i, const StepArgs& a) {
@@ -228,14 +231,15 @@ template <class R>
 __device__ inline const R* pslot(const R* base, int i, const StepArgs& a) {
     return base + (size_t)i * a.sl_stride;
 }
-// Population i of cell c.
+// Population i of cell c of lattice L (base: L's slot 0).
 template <class R>
-__device__ inline R* pop_at(R* base, int i, uint32_t c, const StepArgs& a) {
-    return slot_at(pslot(base, i, a), pcell(a, c));
+__device__ inline R* pop_at(R* base, int i, uint32_t c, const StepArgs& a, const LatArgs& L) {
+    return slot_at(pslot(base, i, a), pcell<R>(a, L, c));
 }
 template <class R>
-__device__ inline const R* pop_at(const R* base, int i, uint32_t c, const StepArgs& a) {
-    return slot_at(pslot(base, i, a), pcell(a, c));
+__device__ inline const R* pop_at(const R* base, int i, uint32_t c, const StepArgs& a,
+                                  const LatArgs& L) {
+    return slot_at(pslot(base, i, a), pcell<R>(a, L, c));
 }
 // Face-buffer value of slot i for face cell k.
 template <class R>
@@ -313,7 +317,7 @@ __device__ inline void read_pops(R* p, const LatArgs& L, const Cell& c,
     const R* __restrict__ base = (const R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) p[i] = ldp(pop_at(base, i, c.id, a));
+        for (int i = 0; i < Q; i++) p[i] = ldp(pop_at(base, i, c.id, a, L));
     } else {
         (void)kind;
 #pragma unroll
@@ -323,9 +327,9 @@ __device__ inline void read_pops(R* p, const LatArgs& L, const Cell& c,
             if (hit == 0u) {
                 uint32_t src = nbr(c, dx, dy, dz);
                 if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls)
-                    p[i] = ldp(pop_at(base, i, c.id, a));
+                    p[i] = ldp(pop_at(base, i, c.id, a, L));
                 else
-                    p[i] = ldp(pop_at(base, inv(i), src, a));
+                    p[i] = ldp(pop_at(base, inv(i), src, a, L));
             } else {
                 const int face = __builtin_ctz(hit);
                 p[i] = ldp(face_at<R>(L, i, a.F, face_slot(a, c, face)));
@@ -342,7 +346,7 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, const Cell& c,
     R* __restrict__ base = (R*)L.pop;
     if constexpr (!ODD) {
 #pragma unroll
-        for (int i = 0; i < Q; i++) stp(pop_at(base, inv(i), c.id, a), p[i]);
+        for (int i = 0; i < Q; i++) stp(pop_at(base, inv(i), c.id, a, L), p[i]);
         if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
 #pragma unroll
             for (int i = 1; i < Q; i++) {
@@ -363,12 +367,12 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, const Cell& c,
             if (hit == 0u) {
                 uint32_t tgt = nbr(c, dx, dy, dz);
                 if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls)
-                    stp(pop_at(base, inv(i), c.id, a), p[i]);
+                    stp(pop_at(base, inv(i), c.id, a, L), p[i]);
                 else
-                    stp(pop_at(base, i, tgt, a), p[i]);
+                    stp(pop_at(base, i, tgt, a, L), p[i]);
             } else {
                 const int face = __builtin_ctz(hit);
-                stp(pop_at(base, inv(i), c.id, a),
+                stp(pop_at(base, inv(i), c.id, a, L),
                     face_in<Q, R>(p, inv(i), face, L, a, kind, fs, c));
             }
         }
@@ -421,7 +425,7 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, const Cell& c,
 #pragma unroll
     for (int i = 1; i < Q; i++)
         if (cz(i) != 0)
-            t[zslot<Q>(i)] = ldp(pop_at(base, inv(i), nbr(c, -cx(i), -cy(i), 0), a));
+            t[zslot<Q>(i)] = ldp(pop_at(base, inv(i), nbr(c, -cx(i), -cy(i), 0), a, L));
     uint32_t own = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -434,10 +438,10 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, const Cell& c,
         } else {
             const uint32_t src = nbr(c, dx, dy, dz);
             if (WALLS && a.has_walls && i != 0 && a.flag[src] == kIsWalls) {
-                p[i] = ldp(pop_at(base, i, c.id, a));
+                p[i] = ldp(pop_at(base, i, c.id, a, L));
                 if (dz != 0) own |= 1u << zslot<Q>(i);
             } else if (dz == 0) {
-                p[i] = ldp(pop_at(base, inv(i), src, a));
+                p[i] = ldp(pop_at(base, inv(i), src, a, L));
             }
         }
     }
@@ -473,14 +477,14 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
         if (dz != 0) zv[zslot<Q>(i)][tid] = p[i];
         if (hit) {
             if (act)
-                stp(pop_at(base, inv(i), c.id, a),
+                stp(pop_at(base, inv(i), c.id, a, L),
                     face_in<Q, R>(p, inv(i), __builtin_ctz(hit), L, a, kind, fs, c));
         } else {
             const uint32_t tgt = nbr(c, dx, dy, dz);
             if (WALLS && a.has_walls && i != 0 && a.flag[tgt] == kIsWalls) {
-                if (act) stp(pop_at(base, inv(i), c.id, a), p[i]);
+                if (act) stp(pop_at(base, inv(i), c.id, a, L), p[i]);
             } else if (dz == 0) {
-                if (act) stp(pop_at(base, i, tgt, a), p[i]);
+                if (act) stp(pop_at(base, i, tgt, a, L), p[i]);
             } else {
                 m |= 1u << zslot<Q>(i);
             }
@@ -505,7 +509,7 @@ __device__ inline void zpush_finish(const LatArgs& L, const Cell& c,
             // sender: the cell (x, y, z - c_z) of this thread's row
             const uint32_t s = rb + ((uint32_t)(c.z - cz(i)) & zm1);
             if (!FACES || ((zm[s] >> zslot<Q>(i)) & 1u))
-                stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a), zv[zslot<Q>(i)][s]);
+                stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a, L), zv[zslot<Q>(i)][s]);
         }
     }
 }

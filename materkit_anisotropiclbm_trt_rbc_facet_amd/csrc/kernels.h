@@ -18,7 +18,8 @@ struct LatArgs {
     int32_t npmask;   // bit f set when face f is not periodic
     int32_t lay;      // current AA layout: 0 = own slots hold f_i, 1 = shifted
     int32_t adv;      // advance this lattice in this launch
-    int32_t pad;
+    uint32_t pg_stride;  // elements from one page of this lattice to the next (FP64 kernels;
+                         // the FP32 kernels take StepArgs::pg_stride, every lattice sharing pages)
     double ref;       // deviation reference: stored value = f_i - w_i * ref
     double fval[6];   // Dirichlet face values (scalar lattices)
 };
@@ -28,11 +29,14 @@ struct StepArgs {
     uint32_t N;       // number of cells
     uint32_t S;       // stride of the per-cell arrays (phi, flags, kappa): N rounded up
     // Population addressing: cell c of slot i of a lattice is at
-    //   L.pop + i * sl_stride + (c >> pg_shift) * pg_stride + (c & pg_mask).
-    // Paged layout: pages of 2^pg_shift cells; a page holds every slot of every lattice of
-    // the handle, slot by slot (sl_stride = page cells, pg_stride = all slots x page cells).
+    //   L.pop + i * sl_stride + (c >> pg_shift) * stride + (c & pg_mask),
+    // stride = pg_stride in FP32 kernels, L.pg_stride in FP64 kernels.
+    // Paged layout: pages of 2^pg_shift cells (sl_stride = page cells).  FP32: a page holds
+    // every slot of every lattice of the handle, slot by slot (pg_stride = all slots x page
+    // cells).  FP64: each lattice has pages of its own slots (L.pg_stride = its slots x page
+    // cells), so that every lattice stays within 4 GiB of 32-bit byte offsets.
     // Slot-major layout: one array per slot (pg_shift = 31, pg_mask = 2^31 - 1,
-    // sl_stride = S).
+    // sl_stride = S, strides 0).
     uint32_t pg_shift, pg_mask, pg_stride, sl_stride;
     uint32_t nblk;    // blocks in the launch (XCD-aware remap when divisible by 8)
     uint32_t F;       // face-buffer stride: cells on the six faces, 2 (ny nz + nx nz + nx ny)
@@ -69,10 +73,11 @@ struct StepArgs {
 };
 
 // Population block of a handle: the slots of every lattice, then every lattice's Q x F face
-// buffer, in one allocation of `total` elements.  Paged (pages of 2^pg_shift cells, each
-// holding every slot of every lattice) when the block stays below 4 GiB, so that the kernels
-// can address a lattice with 32-bit byte offsets from its slot 0; slot-major otherwise, or when
-// MKG_PAGES=0.  MKG_PAGE_SHIFT sets the page size (default 2^13 cells).
+// buffer, in one allocation of `total` elements.  Paged (pages of 2^pg_shift cells) when the
+// kernels can address every lattice with 32-bit byte offsets from its slot 0: FP32 pages hold
+// every slot of every lattice (the block below 4 GiB), FP64 pages the slots of one lattice
+// (each lattice below 4 GiB); slot-major otherwise, or when MKG_PAGES=0.  MKG_PAGE_SHIFT sets
+// the page size (default 2^13 cells).
 constexpr int kMaxLattices = 8;
 struct PopBlock {
     int paged = 0;
@@ -80,6 +85,7 @@ struct PopBlock {
     size_t total = 0;                   // elements
     size_t slot0[kMaxLattices] = {};    // element offset of slot 0 of lattice k
     size_t face0[kMaxLattices] = {};    // element offset of the face buffer of lattice k
+    uint32_t lat_stride[kMaxLattices] = {};  // page stride of lattice k (LatArgs::pg_stride)
     void apply(StepArgs& a) const {
         a.pg_shift = pg_shift;
         a.pg_mask = pg_mask;
@@ -90,6 +96,7 @@ struct PopBlock {
     void lattice(LatArgs& L, void* base, int k, size_t esz) const {
         L.pop = (char*)base + slot0[k] * esz;
         L.face = (char*)base + face0[k] * esz;
+        L.pg_stride = lat_stride[k];
     }
 };
 // q[k]: slots of lattice k (0: absent); N cells, S per-cell array stride, F face cells.

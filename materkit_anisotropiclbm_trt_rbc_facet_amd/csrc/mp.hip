@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256) k_mp_init_h(const MpArgs m, const double*
     if (!locate(m.a, c)) return;
     R* hb = (R*)m.a.g.pop;
 #pragma unroll
-    for (int i = 0; i < 19; i++) *pop_at(hb, i, c.id, m.a) = (R)(wgt<19>(i) * phi0[c.id]);
+    for (int i = 0; i < 19; i++) *pop_at(hb, i, c.id, m.a, m.a.g) = (R)(wgt<19>(i) * phi0[c.id]);
 }
 
 // Equilibria of both lattices from phi0, u0 (3N or null), p0 (N or null) and mu of phi0.
@@ -318,8 +318,8 @@ __global__ void __launch_bounds__(256) k_mp_init_eq(const MpArgs m, const double
         const double w = wgt<19>(i);
         const double cu = cx(i) * u[0] + cy(i) * u[1] + cz(i) * u[2];
         const double si = w * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-        *pop_at(fb, i, c.id, m.a) = (R)((i == 0 ? w - 1.0 : w) * 3.0 * p + rho * si);
-        *pop_at(hb, i, c.id, m.a) =
+        *pop_at(fb, i, c.id, m.a, m.a.f) = (R)((i == 0 ? w - 1.0 : w) * 3.0 * p + rho * si);
+        *pop_at(hb, i, c.id, m.a, m.a.g) =
             (R)(i == 0 ? ph + (w - 1.0) * em : w * (em + 3.0 * ph * cu));
     }
     R* pu = (R*)m.phiu;

@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
             const double w = wgt<19>(i);
             const double cu = cx(i) * ux + cy(i) * uy + cz(i) * uz;
             const double geq = w * pj * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-            *pop_at(base, i, c.id, m.a) = (R)(geq - w * m.a.g.ref);
+            *pop_at(base, i, c.id, m.a, m.gl[j]) = (R)(geq - w * m.a.g.ref);
         }
         *slot_at(phi0 + (size_t)j * S, c.id) = (R)pj;
         *slot_at(phi1 + (size_t)j * S, c.id) = (R)pj;
@@ -419,14 +419,14 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
         R* hb = (R*)m.a.h.pop;
 #pragma unroll
         for (int i = 0; i < 7; i++)
-            *pop_at(hb, i, c.id, m.a) = (R)(wgt<7>(i) * (t - m.a.h.ref));
+            *pop_at(hb, i, c.id, m.a, m.a.h) = (R)(wgt<7>(i) * (t - m.a.h.ref));
     }
     if (m.a.c.pop) {
         const double cv = Cin ? Cin[c.id] : 0.0;
         R* cb = (R*)m.a.c.pop;
 #pragma unroll
         for (int i = 0; i < 7; i++)
-            *pop_at(cb, i, c.id, m.a) = (R)(wgt<7>(i) * (cv - m.a.c.ref));
+            *pop_at(cb, i, c.id, m.a, m.a.c) = (R)(wgt<7>(i) * (cv - m.a.c.ref));
     }
 }
 
