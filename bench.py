@@ -267,7 +267,7 @@ def flow_adjoint_cost(steps: int) -> dict:
     r = dev.flow_adjoint_run(steps, target)
     adj = time.perf_counter() - t0
     dev.close()
-    return {"workload": "flow adjoint, duct 128x64x64, D3Q19 BGK, FP32 forward, FP64 tape",
+    return {"workload": "flow adjoint, duct 128x64x64, D3Q19 BGK, FP32 forward and reverse, tape written by the forward step",
             "steps": steps, "forward_s": round(fwd, 4), "adjoint_run_s": round(adj, 4),
             "ratio": round((adj - fwd) / fwd, 3), "loss": r["loss"]}
 

@@ -309,7 +309,8 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* phi_target
  * u the Guo half-force velocity), reduced in FP64.  grad[7] = dL/d(force x, y, z) and
  * dL/d(s_nu, s_bulk, s_odd, s_even), each rate taken as independent (a BGK gradient with
  * respect to its single rate is the sum of the four).  The handle is left at step nsteps.
- * The forward keeps every step's populations in FP64 (8 Q N bytes per step). */
+ * The forward step writes every step's populations to a tape (Q N values of the storage type
+ * per step). */
 int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, double* loss,
                          double* grad);
 

@@ -1032,17 +1032,23 @@ int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, doubl
     }
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    const size_t need = ((size_t)(nsteps + 1) * QN + 2 * QN + 3 * N) * 8 + (size_t)8 * h->nblk * 8;
+    // tape: the pre-collision populations of every level the step kernel wrote (storage type,
+    // Q slots of S); the final level in canonical FP64 for the loss
+    const size_t tstep = (size_t)Q * h->S * h->esz;
+    const size_t need = (size_t)nsteps * tstep + (QN + 2 * QN + 3 * N) * 8 + (size_t)8 * h->nblk * 8;
     if (need > free_b / 10 * 9) return fail("the flow adjoint tape does not fit in device memory");
-    double *tape = nullptr, *lam[2] = {nullptr, nullptr}, *ut = nullptr, *part = nullptr,
+    char* tape = nullptr;
+    double *fN = nullptr, *lam[2] = {nullptr, nullptr}, *ut = nullptr, *part = nullptr,
            *out = nullptr;
     auto release = [&]() {
-        double* bufs[] = {tape, lam[0], lam[1], ut, part, out};
+        if (tape) (void)hipFree(tape);
+        double* bufs[] = {fN, lam[0], lam[1], ut, part, out};
         for (double* b : bufs)
             if (b) (void)hipFree(b);
     };
     hipError_t e = hipSuccess;
-    if ((e = hipMalloc(&tape, (size_t)(nsteps + 1) * QN * 8)) != hipSuccess ||
+    if ((e = hipMalloc(&tape, (size_t)nsteps * tstep)) != hipSuccess ||
+        (e = hipMalloc(&fN, QN * 8)) != hipSuccess ||
         (e = hipMalloc(&lam[0], QN * 8)) != hipSuccess || (e = hipMalloc(&lam[1], QN * 8)) != hipSuccess ||
         (e = hipMalloc(&ut, 3 * N * 8)) != hipSuccess ||
         (e = hipMalloc(&part, (size_t)8 * h->nblk * 8)) != hipSuccess ||
@@ -1057,19 +1063,20 @@ int mkg_flow_adjoint_run(mkg_handle h, int nsteps, const double* u_target, doubl
             rc = fail(std::string("hip: ") + hipGetErrorString(e));
             break;
         }
-        // forward, keeping each level's canonical populations
-        for (int n = 0; n <= nsteps && rc == 0; n++) {
-            StepArgs a = make_args(h);
-            launch_canonical(Q, h->fp32, a, a.f, 0, tape + (size_t)n * QN, h->stream);
-            if (n < nsteps && mkg_step(h, 1, MKG_FLOW)) rc = -1;
-        }
+        // forward: each step writes its level's pre-collision populations to the tape
+        const int adv[4] = {1, 0, 0, 0};
+        for (int n = 0; n < nsteps && rc == 0; n++)
+            if (step_once_impl(h, adv, nullptr, [](mkg_ctx*) { return 0; },
+                               tape + (size_t)n * tstep))
+                rc = -1;
         if (rc) break;
         StepArgs a = make_args(h);
-        launch_fadj_loss(Q, h->fp32, a, tape + (size_t)nsteps * QN, ut, lam[0], part, h->stream);
+        launch_canonical(Q, h->fp32, a, a.f, 0, fN, h->stream);
+        launch_fadj_loss(Q, h->fp32, a, fN, ut, lam[0], part, h->stream);
         int cur = 0;
         for (int n = nsteps - 1; n >= 0; n--) {
-            launch_fadj_step(Q, h->fp32, a, tape + (size_t)n * QN, lam[cur], lam[cur ^ 1], part,
-                             h->stream);
+            launch_fadj_step(Q, h->fp32, a, tape + (size_t)n * tstep, lam[cur], lam[cur ^ 1],
+                             part, h->stream);
             cur ^= 1;
         }
         for (int k = 0; k < 8; k++) launch_sum_partials(part + (size_t)k * h->nblk, h->nblk, out + k, h->stream);

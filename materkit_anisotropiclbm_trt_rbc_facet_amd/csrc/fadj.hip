@@ -4,8 +4,10 @@
 // and even ghosts at s_odd / s_even -- BGK and TRT are its special cases) with Guo forcing,
 // then streaming with periodic, bounce (any wall velocity) and outflow faces and wall cells.
 // Loss L = 1/2 sum over non-wall cells |u_N - u*|^2, u = (sum c f + F/2)/rho.  The forward
-// populations come from a tape of canonical FP64 values; the reverse runs in the forward's
-// precision (float adjoints for FP32 storage) with FP64 block sums.
+// populations of every level come from a tape that the forward step kernel writes itself
+// (its pre-collision populations, deviations in the storage type, slot i at tape + i*S); the
+// loss reads the final level in canonical FP64.  The reverse runs in the forward's precision
+// (float adjoints for FP32 storage) with FP64 block sums.
 //
 // Reverse of one step, per non-wall cell y (lambda = adjoint of the pre-collision populations
 // of the next level, l = adjoint of the post-collision populations of this level):
@@ -61,8 +63,8 @@ __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const doubl
     fadj_block_add<4>(v, part, a.nblk);
 }
 
-// One reverse step: lam_in = adjoint of level n+1, fn = tape of level n; partials 1..3
-// (dL/dF) and 4..7 (dL/d s_nu, s_bulk, s_odd, s_even) accumulate.
+// One reverse step: lam_in = adjoint of level n+1, tape = level n; partials 1..3 (dL/dF) and
+// 4..7 (dL/d s_nu, s_bulk, s_odd, s_even) accumulate.
 // Minimum waves per SIMD asked of the register allocator for the FP32 reverse step: two
 // (240 VGPRs, no spills) measured faster than three or four, which spill (76 and 232 bytes
 // of scratch per lane), at 128 x 64 x 64.
@@ -71,8 +73,9 @@ __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const doubl
 #endif
 
 template <int Q, class R>
-__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_FADJ_WAVES_F32 : 1) k_fadj_step(const StepArgs a, const double* fn,
-                                                   const R* lam_in, R* lam_out, double* part) {
+__global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_FADJ_WAVES_F32 : 1)
+    k_fadj_step(const StepArgs a, const R* __restrict__ tape, const R* lam_in, R* lam_out,
+                double* part) {
     const uint32_t N = a.N;
     Cell c;
     const bool in = locate_clamped(a, c);
@@ -100,7 +103,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_FADJ_WAVES_F32 : 1) 
         }
         R f[Q];
 #pragma unroll
-        for (int i = 0; i < Q; i++) f[i] = (R)fn[(size_t)i * N + c.id];
+        for (int i = 0; i < Q; i++)
+            f[i] = (R)(wgt<Q>(i) * a.f.ref) + *slot_at(tape + (size_t)i * a.S, c.id);
         const R F[3] = {(R)a.force[0], (R)a.force[1], (R)a.force[2]};
         const R ub_ext[3] = {0, 0, 0};
         R dF[3], drate[4];
@@ -125,12 +129,14 @@ static int fadj_loss_t(int q, const StepArgs& a, const double* fN, const double*
     return 0;
 }
 template <class R>
-static int fadj_step_t(int q, const StepArgs& a, const double* fn, const void* lam_in,
+static int fadj_step_t(int q, const StepArgs& a, const void* tape, const void* lam_in,
                        void* lam_out, double* part, hipStream_t s) {
     if (q == 19)
-        k_fadj_step<19, R><<<a.nblk, 256, 0, s>>>(a, fn, (const R*)lam_in, (R*)lam_out, part);
+        k_fadj_step<19, R><<<a.nblk, 256, 0, s>>>(a, (const R*)tape, (const R*)lam_in,
+                                                  (R*)lam_out, part);
     else if (q == 27)
-        k_fadj_step<27, R><<<a.nblk, 256, 0, s>>>(a, fn, (const R*)lam_in, (R*)lam_out, part);
+        k_fadj_step<27, R><<<a.nblk, 256, 0, s>>>(a, (const R*)tape, (const R*)lam_in,
+                                                  (R*)lam_out, part);
     else return -1;
     return 0;
 }
@@ -139,10 +145,10 @@ int launch_fadj_loss(int q, int fp32, const StepArgs& a, const double* fN, const
     return fp32 ? fadj_loss_t<float>(q, a, fN, ut, lam, part, s)
                 : fadj_loss_t<double>(q, a, fN, ut, lam, part, s);
 }
-int launch_fadj_step(int q, int fp32, const StepArgs& a, const double* fn, const void* lam_in,
+int launch_fadj_step(int q, int fp32, const StepArgs& a, const void* tape, const void* lam_in,
                      void* lam_out, double* part, hipStream_t s) {
-    return fp32 ? fadj_step_t<float>(q, a, fn, lam_in, lam_out, part, s)
-                : fadj_step_t<double>(q, a, fn, lam_in, lam_out, part, s);
+    return fp32 ? fadj_step_t<float>(q, a, tape, lam_in, lam_out, part, s)
+                : fadj_step_t<double>(q, a, tape, lam_in, lam_out, part, s);
 }
 
 }  // namespace mkg

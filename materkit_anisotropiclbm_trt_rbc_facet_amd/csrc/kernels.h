@@ -195,11 +195,12 @@ int launch_adj_loss(int fp32, const StepArgs& a, const double* target, void* nse
 int launch_sum_partials(const double* part, uint32_t n, double* out, hipStream_t s);
 
 // Flow adjoint (fadj.hip): loss and final adjoint from the canonical FP64 populations fN
-// (partials 0..3 of an 8 x nblk array), and one reverse step over the tape fn (partials 1..7).
-// The adjoint buffers hold the storage type (float when fp32).
+// (partials 0..3 of an 8 x nblk array), and one reverse step over the tape of its level (the
+// pre-collision populations the step kernel wrote: storage type, deviations, slot i at
+// tape + i*S; partials 1..7).  The adjoint buffers hold the storage type (float when fp32).
 int launch_fadj_loss(int q, int fp32, const StepArgs& a, const double* fN, const double* ut,
                      void* lam, double* part, hipStream_t s);
-int launch_fadj_step(int q, int fp32, const StepArgs& a, const double* fn, const void* lam_in,
+int launch_fadj_step(int q, int fp32, const StepArgs& a, const void* tape, const void* lam_in,
                      void* lam_out, double* part, hipStream_t s);
 
 // Reductions without allocation (reduce_static.hip): per-block partials and the result live

@@ -252,9 +252,9 @@ constexpr int zslots_of() {
 //       issued after the phase source, so that the stencil values are dead by then)
 //   B3  after the phase push stage, which reuses the flow pull region (last read before B2)
 // Every other stage region is written once and read once, on either side of one barrier.
-// TAPE: the step also writes the adjoint tape (phi^n and the pre-collision thermal
-// populations of every cell; instantiated for the phase-field + thermal model, with or
-// without the D3Q19 flow).
+// TAPE: the step also writes the adjoint tape: phi^n and the pre-collision thermal
+// populations of every cell (the phase-field + thermal model, with or without the D3Q19
+// flow), or the pre-collision flow populations (the flow alone), slot i at tape + i*S.
 template <class R, int FQ, int PQ, bool TH, bool SO, bool ODD, bool ZS = false, bool PER = false,
           bool TAPE = false, uint32_t SFM = kAllFaces>
 __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
@@ -378,6 +378,14 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
     // ---- flow
     R ux = 0, uy = 0, uz = 0;
     if constexpr (FQ > 0) {
+        if constexpr (TAPE && PQ == 0) {
+            // the flow adjoint's tape: the pre-collision populations (deviations) of level n
+            if (act) {
+                R* __restrict__ tp = (R*)a.tape;
+#pragma unroll
+                for (int i = 0; i < FQ; i++) stp(slot_at(tp + (size_t)i * S, c.id), f[i]);
+            }
+        }
         R dr = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < FQ; i++) {
@@ -598,9 +606,11 @@ template <class R, int FQ, int PQ, bool TH, bool SO>
 inline void run_step(int odd, const StepArgs& a, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
     const bool per = !a.has_walls && (a.f.npmask | a.g.npmask | a.h.npmask | a.c.npmask) == 0;
-    if constexpr ((FQ == 0 || FQ == 19) && PQ == 19 && TH && !SO) {
-        // the adjoint's forward steps (the phase-field + thermal model), with the kernel
-        // variant a plain step would take, so that the recompute repeats the forward exactly
+    // the adjoint's forward steps (the phase-field + thermal model with or without the flow,
+    // or the flow alone), with the kernel variant a plain step would take, so that a recompute
+    // repeats the forward exactly
+    if constexpr (((FQ == 0 || FQ == 19) && PQ == 19 && TH && !SO) ||
+                  (FQ > 0 && PQ == 0 && !TH && !SO)) {
         if (a.tape) {
             if (odd && a.zstage) {
                 if (per && a.N == a.nblk * 256u)
