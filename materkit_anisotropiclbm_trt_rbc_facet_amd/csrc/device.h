@@ -30,21 +30,23 @@ struct Cell {
     int x, y, z;
 };
 
-// Thread -> cell.  Blocks of 256 threads walk the linear index (z contiguous).  With
-// MKG_XCD_REMAP=1, consecutive hardware block ids (dealt round-robin to the eight XCDs) are
-// remapped so that each XCD works on one contiguous eighth of the domain.  That keeps the
-// neighbour rows of a block in its XCD's L2 but makes the eight XCDs stream eight distant
-// regions at once; measured at 256^3 the plain order is faster (5.29 vs 5.02 TB/s for the
-// fused step), so it is off by default.
+// Block order.  The hardware deals consecutive block ids round-robin to the eight XCDs, each
+// with its own L2.  With xcd_chunk = K > 0 (and nblk a multiple of 8K), remap_block gives
+// XCD x (block ids b with b mod 8 = x) runs of K consecutive logical blocks: its j-th block
+// takes logical block ((j / K) 8 + x) K + j mod K.  Neighbour rows of a stencil then sit in the
+// same XCD's L2 except at run ends, while the eight XCDs still sweep one window of the domain
+// together.  K = nblk / 8 would give each XCD one contiguous eighth of the domain.
+__device__ inline uint32_t remap_block(const StepArgs& a, uint32_t b, uint32_t nblk) {
+    const uint32_t K = (uint32_t)a.xcd_chunk;
+    if (K == 0u || nblk % (8u * K) != 0u) return b;
+    const uint32_t x = b & 7u, j = b >> 3;
+    return ((j / K) * 8u + x) * K + (j % K);
+}
+
+// Thread -> cell.  Blocks of 256 threads walk the linear index (z contiguous), in the block
+// order of remap_block.
 __device__ inline uint32_t thread_cell(const StepArgs& a) {
-    uint32_t b = blockIdx.x;
-#ifndef MKG_XCD_REMAP
-#define MKG_XCD_REMAP 0
-#endif
-#if MKG_XCD_REMAP
-    if ((a.nblk & 7u) == 0u) b = (b & 7u) * (a.nblk >> 3) + (b >> 3);
-#endif
-    return b * 256u + threadIdx.x;
+    return remap_block(a, blockIdx.x, a.nblk) * 256u + threadIdx.x;
 }
 
 __device__ inline void locate_at(const StepArgs& a, uint32_t t, Cell& c);

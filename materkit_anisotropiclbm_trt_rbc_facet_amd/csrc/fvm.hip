@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(256) k_fvm_phase_march(const StepArgs a, const
     const uint32_t tid = threadIdx.x;
     const uint32_t rows = 256u / nz;                 // z rows per block
     const uint32_t ygroups = ny / rows;
-    const uint32_t yg = blockIdx.x % ygroups, xc = blockIdx.x / ygroups;
+    const uint32_t bl = remap_block(a, blockIdx.x, gridDim.x);
+    const uint32_t yg = bl % ygroups, xc = bl / ygroups;
     const uint32_t y = yg * rows + tid / nz, z = tid % nz;
     const uint32_t x0 = xc * XC;
     const uint32_t xend = x0 + XC < nx ? x0 + XC : nx;
@@ -382,7 +383,8 @@ __global__ void __launch_bounds__(256) k_fvm_solute_march(const StepArgs a, cons
     const uint32_t tid = threadIdx.x;
     const uint32_t rows = 256u / nz;
     const uint32_t ygroups = ny / rows;
-    const uint32_t yg = blockIdx.x % ygroups, xc = blockIdx.x / ygroups;
+    const uint32_t bl = remap_block(a, blockIdx.x, gridDim.x);
+    const uint32_t yg = bl % ygroups, xc = bl / ygroups;
     const uint32_t y = yg * rows + tid / nz, z = tid % nz;
     const uint32_t x0 = xc * XC;
     const uint32_t xend = x0 + XC < nx ? x0 + XC : nx;
@@ -499,6 +501,7 @@ struct mkg_fvm_ctx {
     int thermal = 0;  // the potential is the temperature (evolution_thermal)
     int cur = 0;  // buffer of the current level
     uint32_t march_blk = 0;   // blocks of the marching kernels (0: the per-cell kernels)
+    int xcd_chunk = 0;        // block order of the launches, fixed at creation
     int8_t* flag = nullptr;   // all fluid: the reductions read flags
     double* scratch = nullptr;
     double* partials = nullptr;
@@ -529,6 +532,7 @@ StepArgs fvm_args(const mkg_fvm_ctx* h) {
     a.S = h->S;
     a.nblk = h->nblk;
     a.flag = h->flag;
+    a.xcd_chunk = h->xcd_chunk;
     int npg = 0, npc = 0;
     for (int f = 0; f < 6; f++) {
         a.g.bc[f] = p.bc_g[f];
@@ -625,6 +629,7 @@ int mkg_fvm_create(const mkg_params* p, mkg_fvm_handle* out) {
     h->N = (uint32_t)n;
     h->S = ((h->N + 127u) & ~127u) + 1024u;
     h->nblk = (h->N + 255u) / 256u;
+    h->xcd_chunk = xcd_chunk_setting(h->esz);
     // the marching kernels need whole z rows per block
     if (256u % (uint32_t)p->nz == 0u && (uint32_t)p->ny % (256u / (uint32_t)p->nz) == 0u) {
         const uint32_t rows = 256u / (uint32_t)p->nz;

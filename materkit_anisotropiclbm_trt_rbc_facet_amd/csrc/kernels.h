@@ -43,6 +43,7 @@ struct StepArgs {
     int32_t has_walls, has_kappa, ngrains, pf_coupling;
     int32_t zstage;   // O steps move the z populations through LDS (needs 256 % nz == 0)
     int32_t incomp;   // He-Luo incompressible equilibrium: rho0 in the u terms
+    int32_t xcd_chunk;  // block order (remap_block): runs of this many blocks per XCD, 0 off
     LatArgs f, g, h, c;
     const void* phi_cur;   // storage type, phase field at time level n
     const void* phi_prev;  // level n-1
@@ -101,6 +102,10 @@ struct PopBlock {
 };
 // q[k]: slots of lattice k (0: absent); N cells, S per-cell array stride, F face cells.
 PopBlock plan_pop_block(const int* q, int nlat, uint32_t N, uint32_t S, uint32_t F, size_t esz);
+
+// Block order of the launches (StepArgs::xcd_chunk) for elements of esz bytes:
+// MKG_XCD_CHUNK, else the default of that element size.
+int xcd_chunk_setting(size_t esz);
 
 // Selector of a fused-step instantiation: presence of each lattice.
 struct Combo {

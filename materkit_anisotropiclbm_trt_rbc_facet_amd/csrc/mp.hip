@@ -349,6 +349,7 @@ struct mkg_mp_ctx {
     double* red_out = nullptr;
     int lay = 0;
     int zstage = 0;  // O steps take the z shift through LDS (blocks hold whole z rows)
+    int xcd_chunk = 0;  // block order of the launches, fixed at creation
 };
 
 namespace {
@@ -377,6 +378,7 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     a.nblk = h->nblk;
     a.F = h->F;
     a.flag = h->flag;
+    a.xcd_chunk = h->xcd_chunk;
     a.rho0 = 0.0;  // no moving-wall term at bounce-back faces
     int np = 0;
     for (int f = 0; f < 6; f++) {
@@ -489,6 +491,7 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
     h->nblk = (h->N + 255u) / 256u;
     h->zstage = (p->nz >= 64 && 256 % p->nz == 0) ? 1 : 0;
     if (const char* env = std::getenv("MKG_ZSTAGE")) h->zstage = h->zstage && std::atoi(env) != 0;
+    h->xcd_chunk = xcd_chunk_setting(h->esz);
     auto bad = [&](hipError_t e, const char* what) {
         mp_destroy_impl(h);
         return pfail(std::string(what) + ": " + hipGetErrorString(e));

@@ -465,6 +465,7 @@ struct mkg_mpf_ctx {
     double* red_out = nullptr;
     int lay = 0;              // AA layout of every lattice
     int zstage = 0;           // O steps take the z shift through LDS
+    int xcd_chunk = 0;        // block order of the launches, fixed at creation
     int64_t pf_count = 0;
     int has_walls = 0;
     // order parameter j's lattice descriptor (base, face buffer) on top of a filled one
@@ -499,6 +500,7 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
     a.nblk = h->nblk;
     a.F = h->F;
     a.has_walls = h->has_walls;
+    a.xcd_chunk = h->xcd_chunk;
     a.zstage = h->zstage;
     a.flag = h->flag;
     a.rho0 = 1.0;
@@ -723,6 +725,7 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     // (nz = 64, 128 or 256); MKG_ZSTAGE=0 turns it off
     h->zstage = (p->nz >= 64 && 256 % p->nz == 0) ? 1 : 0;
     if (const char* env = std::getenv("MKG_ZSTAGE")) h->zstage = h->zstage && std::atoi(env) != 0;
+    h->xcd_chunk = xcd_chunk_setting(h->esz);
     auto bad = [&](hipError_t e, const char* what) {
         mpf_destroy_impl(h);
         return mfail(std::string(what) + ": " + hipGetErrorString(e));
