@@ -492,16 +492,22 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
             }
         }
     }
-    // without faces and walls, with every thread updating, every z population is sent
-    if constexpr (FACES) zm[tid] = act ? m : 0u;
+    // without faces and walls, with every thread updating, every z population is sent; with
+    // the z faces alone the receiver knows what was not sent (zpush_finish)
+    if constexpr (FACES && FM != (ZLO | ZHI)) zm[tid] = act ? m : 0u;
 }
 
 // FACES false: the block has no face cells, no walls and no thread past the last cell, so
 // every sender sent every z population and the masks are not read.
-template <int Q, class R, bool FACES = true>
+// FM = ZLO | ZHI (no wall cells, no x / y faces): a row's cells are live or not together, and
+// the only z population not sent is the one that crosses a non-periodic z face: the receiver
+// at z = 0 takes no c_z = +1 value (its sender, z = nz - 1, met the z+ face) and the receiver
+// at z = nz - 1 no c_z = -1 value (z- face); act is the receiver's own liveness.
+template <int Q, class R, bool FACES = true, uint32_t FM = kAllFaces>
 __device__ inline void zpush_finish(const LatArgs& L, const Cell& c,
                                     const StepArgs& a, R (*zv)[256], const uint32_t* zm,
-                                    uint32_t tid) {
+                                    uint32_t tid, bool act = true) {
+    constexpr bool ZROWS = FACES && FM == (ZLO | ZHI);
     R* __restrict__ base = (R*)L.pop;
     const uint32_t zm1 = (uint32_t)a.nz - 1u;
     const uint32_t rb = tid - (tid & zm1);
@@ -510,8 +516,15 @@ __device__ inline void zpush_finish(const LatArgs& L, const Cell& c,
         if (cz(i) != 0) {
             // sender: the cell (x, y, z - c_z) of this thread's row
             const uint32_t s = rb + ((uint32_t)(c.z - cz(i)) & zm1);
-            if (!FACES || ((zm[s] >> zslot<Q>(i)) & 1u))
-                stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a, L), zv[zslot<Q>(i)][s]);
+            bool send = true;
+            if constexpr (ZROWS) {
+                const uint32_t zr = cz(i) > 0 ? 0u : zm1;
+                const uint32_t face = cz(i) > 0 ? (uint32_t)ZHI : (uint32_t)ZLO;
+                send = act && !((uint32_t)c.z == zr && ((uint32_t)L.npmask & face));
+            } else if constexpr (FACES) {
+                send = (zm[s] >> zslot<Q>(i)) & 1u;
+            }
+            if (send) stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a, L), zv[zslot<Q>(i)][s]);
         }
     }
 }

@@ -279,8 +279,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
     // block must also lie wholly before the last cell, so that every thread updates.  The
     // others take the body with the face rules of SFM (every face, or only the z faces in a
     // domain whose only non-periodic faces are z faces, with no wall cells or velocity faces).
+    constexpr bool ZROWS = Z && SFM == (ZLO | ZHI);
     bool slow = !PER && a.has_walls != 0;
-    if (!PER && !slow) {
+    if (!PER && !ZROWS && !slow) {
         const uint32_t t = thread_cell(a);
         Cell e;
         locate_at(a, t < a.N ? t : a.N - 1u, e);
@@ -559,11 +560,11 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
     if constexpr (Z) {
         __syncthreads();  // B2
         if constexpr (FQ > 0)
-            if (a.f.adv) zpush_finish<FQ, R, FACES>(a.f, c, a, zwf, zmf, tid);
+            if (a.f.adv) zpush_finish<FQ, R, FACES, FM>(a.f, c, a, zwf, zmf, tid, act);
         if constexpr (TH)
-            if (a.h.adv) zpush_finish<7, R, FACES>(a.h, c, a, zwh, zmh, tid);
+            if (a.h.adv) zpush_finish<7, R, FACES, FM>(a.h, c, a, zwh, zmh, tid, act);
         if constexpr (SO)
-            if (a.c.adv) zpush_finish<7, R, FACES>(a.c, c, a, zwc, zmc, tid);
+            if (a.c.adv) zpush_finish<7, R, FACES, FM>(a.c, c, a, zwc, zmc, tid, act);
         if constexpr (PQ > 0)
             if (a.g.adv) zpull_finish<PQ, R>(g, c, a, zrg, tid, own_g);
     }
@@ -589,12 +590,16 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? ((ODD && ZS && !PER)
         }
         if constexpr (Z) {
             __syncthreads();  // B3
-            if (a.g.adv) zpush_finish<PQ, R, FACES>(a.g, c, a, zrf, zmg, tid);
+            if (a.g.adv) zpush_finish<PQ, R, FACES, FM>(a.g, c, a, zrf, zmg, tid, act);
         }
     }
     };
+    // With the z stage and only z faces, every z row holds a face cell, so every block takes
+    // the face body: the kernel holds it alone.
     if constexpr (PER)
         body(std::integral_constant<uint32_t, 0u>{});
+    else if (ZROWS)
+        body(std::integral_constant<uint32_t, SFM>{});
     else if (slow)
         body(std::integral_constant<uint32_t, SFM>{});
     else

@@ -70,6 +70,12 @@ def make_params(flow_q=0, phase_q=0, thermal=0, solute=0, op="bgk", bcs="periodi
               bc_c=[P, P, P, P, DI, BB], T_face=[0, 0, 0, 0, 0.3, -0.2],
               C_face=[0, 0, 0, 0, -0.4, 0])
         p.u_face[5][0], p.u_face[5][1] = 0.01, -0.004
+    if bcs == "zmixed":
+        # only z faces non-periodic, on some lattices: flow and solute walls in z, phase and
+        # temperature periodic in z
+        p.set(bc_f=[P, P, P, P, BB, BB], bc_g=[P] * 6, bc_h=[P] * 6,
+              bc_c=[P, P, P, P, BB, DI], C_face=[0, 0, 0, 0, 0, -0.3])
+        p.u_face[4][1] = 0.006
     if bcs == "velocity":
         # velocity inlet (non-equilibrium extrapolation) and pressure outlet along x, walls
         # along z rotating about a tilted axis through the domain centre
@@ -353,6 +359,18 @@ def test_fp64_z_faces_only_match_oracle(case, dims, nsteps):
     """Non-periodic z faces only (moving top wall, Dirichlet T and c): the kernel variant with
     the z face rules alone, without and with the z stage (one z row per block at nz = 256)."""
     errs, fd, fo, dev, orc = _run(case, "zwalls", nsteps, dims=dims)
+    dev.close()
+    assert np.array_equal(fd, fo), "flags differ"
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["id"] for c in CASES])
+@pytest.mark.parametrize("dims", [(5, 3, 64), (2, 3, 256)], ids=["nz64", "nz256"])
+def test_fp64_z_faces_on_some_lattices_match_oracle(case, dims):
+    """z walls on the flow and solute lattices, z-periodic phase and temperature: in the z
+    stage each lattice's own faces decide which z populations wrap along the row."""
+    errs, fd, fo, dev, orc = _run(case, "zmixed", 5, dims=dims)
     dev.close()
     assert np.array_equal(fd, fo), "flags differ"
     for k, e in errs.items():
