@@ -429,6 +429,36 @@ __device__ inline uint32_t zpull_issue(R* p, const LatArgs& L, const Cell& c,
         if (cz(i) != 0)
             t[zslot<Q>(i)] = ldp(pop_at(base, inv(i), nbr(c, -cx(i), -cy(i), 0), a, L));
     uint32_t own = 0;
+    if constexpr (FACES && FM == (ZLO | ZHI)) {
+        // z faces alone: every lane takes the face-free pull; the lane at z = 0 (z = nz - 1)
+        // then takes the populations arriving through a non-periodic z- (z+) face from the
+        // face buffer, one branch per face with the face known at compile time
+#pragma unroll
+        for (int i = 0; i < Q; i++)
+            if (cz(i) == 0) p[i] = ldp(pop_at(base, inv(i), nbr(c, -cx(i), -cy(i), 0), a, L));
+        const uint32_t npz = (uint32_t)L.npmask & (ZLO | ZHI);
+        if ((npz & ZLO) && c.z == 0) {
+            const uint32_t k = face_slot(a, c, 4);
+#pragma unroll
+            for (int i = 1; i < Q; i++)
+                if (cz(i) > 0) {
+                    p[i] = ldp(face_at<R>(L, i, a.F, k));
+                    own |= 1u << zslot<Q>(i);
+                }
+        }
+        if ((npz & ZHI) && c.z == a.nz - 1) {
+            const uint32_t k = face_slot(a, c, 5);
+#pragma unroll
+            for (int i = 1; i < Q; i++)
+                if (cz(i) < 0) {
+                    p[i] = ldp(face_at<R>(L, i, a.F, k));
+                    own |= 1u << zslot<Q>(i);
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < NZ; k++) zv[k][tid] = t[k];
+        return own;
+    }
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         const int dx = -cx(i), dy = -cy(i), dz = -cz(i);
@@ -470,6 +500,32 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
     constexpr bool WALLS = FACES && FM == kAllFaces;
     static_assert(nzdirs<Q>() <= 32, "z mask too small");
     R* __restrict__ base = (R*)L.pop;
+    if constexpr (FACES && FM == (ZLO | ZHI)) {
+        // z faces alone: every lane stages its z populations and stores the others directly;
+        // the lane at z = 0 (z = nz - 1) then returns the populations that leave through a
+        // non-periodic z- (z+) face into its own slots.  zpush_finish skips their wrap.
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            if (cz(i) != 0)
+                zv[zslot<Q>(i)][tid] = p[i];
+            else if (act)
+                stp(pop_at(base, i, nbr(c, cx(i), cy(i), 0), a, L), p[i]);
+        }
+        const uint32_t npz = (uint32_t)L.npmask & (ZLO | ZHI);
+        if ((npz & ZLO) && c.z == 0 && act) {
+#pragma unroll
+            for (int i = 1; i < Q; i++)
+                if (cz(i) < 0)
+                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), 4, L, a, kind, fs, c));
+        }
+        if ((npz & ZHI) && c.z == a.nz - 1 && act) {
+#pragma unroll
+            for (int i = 1; i < Q; i++)
+                if (cz(i) > 0)
+                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), 5, L, a, kind, fs, c));
+        }
+        return;
+    }
     uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
@@ -492,9 +548,8 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
             }
         }
     }
-    // without faces and walls, with every thread updating, every z population is sent; with
-    // the z faces alone the receiver knows what was not sent (zpush_finish)
-    if constexpr (FACES && FM != (ZLO | ZHI)) zm[tid] = act ? m : 0u;
+    // without faces and walls, with every thread updating, every z population is sent
+    if constexpr (FACES) zm[tid] = act ? m : 0u;
 }
 
 // FACES false: the block has no face cells, no walls and no thread past the last cell, so

@@ -178,8 +178,9 @@ __global__ void __launch_bounds__(256) k_nee(const StepArgs a) {
 
 // ZS: O step with the z stage (the block holds whole z rows): the c_z != 0 populations are
 // loaded at the cell's own z and shifted through LDS, as in k_step.  PER: every face periodic
-// and no wall cells (no face rules).
-template <class R, int PQ, bool ODD, bool ZS = false, bool PER = false>
+// and no wall cells (no face rules).  FM (with ZS): the faces the face rules check; ZLO | ZHI
+// when the phase lattice's only non-periodic faces are z faces and there are no wall cells.
+template <class R, int PQ, bool ODD, bool ZS = false, bool PER = false, uint32_t FM = kAllFaces>
 __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __restrict__ phi_out) {
     constexpr bool Z = ODD && ZS;
     __shared__ R zv[Z ? nzdirs<PQ>() : 1][256];
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(256) k_phase_moments(const StepArgs a, R* __re
     const int8_t fl = a.flag[c.id];
     R g[PQ];
     if constexpr (Z) {
-        const uint32_t own = zpull_issue<PQ, R, !PER>(g, a.g, c, a, zv, threadIdx.x);
+        const uint32_t own = zpull_issue<PQ, R, !PER, FM>(g, a.g, c, a, zv, threadIdx.x);
         __syncthreads();
         zpull_finish<PQ, R>(g, c, a, zv, threadIdx.x, own);
         if (!act) return;
@@ -677,9 +678,12 @@ template <class R, int PQ>
 inline void run_phase_moments(int odd, const StepArgs& a, void* phi_out, hipStream_t s) {
     dim3 grid(a.nblk), block(256);
     const bool per = !a.has_walls && a.g.npmask == 0;
+    const bool zonly = !a.has_walls && ((uint32_t)a.g.npmask & ~(uint32_t)(ZLO | ZHI)) == 0u;
     if (odd && a.zstage) {
         if (per)
             k_phase_moments<R, PQ, true, true, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
+        else if (zonly)
+            k_phase_moments<R, PQ, true, true, false, ZLO | ZHI><<<grid, block, 0, s>>>(a, (R*)phi_out);
         else
             k_phase_moments<R, PQ, true, true><<<grid, block, 0, s>>>(a, (R*)phi_out);
     } else if (odd) {
