@@ -56,6 +56,37 @@ constexpr int kXcdChunkF64 = 0;
 namespace mkg {
 void set_error(const std::string& msg) { g_err = msg; }
 
+FaceTab facetab_of(const StepArgs& a) {
+    FaceTab t;
+    std::memset(&t, 0, sizeof(t));
+    const LatArgs* lat[4] = {&a.f, &a.g, &a.h, &a.c};
+    for (int k = 0; k < 4; k++)
+        for (int f = 0; f < 6; f++) {
+            t.bc[k][f] = lat[k]->bc[f];
+            t.fval[k][f] = lat[k]->fval[f];
+            t.ref[k] = lat[k]->ref;
+        }
+    std::memcpy(t.u_face, a.u_face, sizeof(t.u_face));
+    t.rho0 = a.rho0;
+    t.wall_rot = a.wall_rot;
+    std::memcpy(t.wall_omega, a.wall_omega, sizeof(t.wall_omega));
+    std::memcpy(t.wall_center, a.wall_center, sizeof(t.wall_center));
+    return t;
+}
+
+hipError_t facetab_alloc(FaceTab** dev) {
+    hipError_t e = hipMalloc((void**)dev, sizeof(FaceTab));
+    if (e == hipSuccess) e = hipMemset(*dev, 0, sizeof(FaceTab));
+    return e;
+}
+
+hipError_t facetab_upload(FaceTab* dev, const StepArgs& a, hipStream_t s) {
+    const FaceTab t = facetab_of(a);
+    hipError_t e = hipMemcpyAsync(dev, &t, sizeof(t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
 int xcd_chunk_setting(size_t esz) {
     int v = esz == 4 ? kXcdChunkF32 : kXcdChunkF64;
     if (const char* env = std::getenv("MKG_XCD_CHUNK")) v = std::atoi(env);
@@ -142,6 +173,7 @@ struct mkg_ctx {
     int16_t* grain = nullptr;
     void* kappa = nullptr;
     double* rot = nullptr;
+    FaceTab* ftab = nullptr;    // boundary table of the face rules (uploaded with h->p)
     double* scratch = nullptr;  // 27*N doubles, allocated on first use
     void* nee = nullptr;        // Q x F storage-type values of the velocity faces
     double* partials = nullptr;
@@ -196,6 +228,7 @@ void fill_lat(const mkg_ctx* h, int L, const int32_t* bc, double ref, const doub
     out.npmask = np;
     out.lay = h->lay[L];
     out.adv = 0;
+    out.fid = L;
     out.ref = ref;
 }
 
@@ -231,6 +264,7 @@ StepArgs make_args(const mkg_ctx* h) {
     a.grain = h->grain;
     a.kappa = h->kappa;
     a.rot = h->rot;
+    a.ftab = h->ftab;
     a.s_nu = p.s_nu;
     a.s_bulk = p.s_bulk;
     a.s_odd = p.s_odd;
@@ -589,6 +623,9 @@ int mkg_create(const mkg_params* p, mkg_handle* out) {
                    h->stream))
         return cleanup(-1);
     if (alloc_zero((void**)&h->red_out, 16 * sizeof(double), h->stream)) return cleanup(-1);
+    if (facetab_alloc(&h->ftab) != hipSuccess) return cleanup(fail("hipMalloc failed"));
+    if (facetab_upload(h->ftab, make_args(h), h->stream) != hipSuccess)
+        return cleanup(fail("boundary table upload failed"));
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(fail("init sync failed"));
     *out = h;
     return 0;
@@ -604,6 +641,7 @@ int mkg_destroy(mkg_handle h) {
     hipFree(h->grain);
     hipFree(h->kappa);
     hipFree(h->rot);
+    hipFree(h->ftab);
     hipFree(h->scratch);
     hipFree(h->nee);
     hipFree(h->partials);
@@ -637,6 +675,7 @@ int mkg_set_params(mkg_handle h, const mkg_params* p) {
     if (ensure_nee(h, *p)) return -1;
     h->p = *p;
     HIPCHK(hipMemcpyAsync(h->rot, p->rot, sizeof(p->rot), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(facetab_upload(h->ftab, make_args(h), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -1017,6 +1056,7 @@ int mkg_checkpoint_load(mkg_handle h, const char* path) {
     std::memcpy(h->steps, steps, sizeof(steps));
     h->pf_count = pfc;
     HIPCHK(hipMemcpy(h->rot, p.rot, sizeof(p.rot), hipMemcpyHostToDevice));
+    HIPCHK(facetab_upload(h->ftab, make_args(h), h->stream));
     return 0;
 }
 

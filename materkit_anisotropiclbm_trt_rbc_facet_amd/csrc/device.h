@@ -121,7 +121,8 @@ __device__ inline double face_cu(int f, int X, int Y, int Z, const double (*uf)[
 // Rotating-wall part of the wall velocity of face f next to cell c: the tangential part of
 // wall_omega x (r - wall_center), r the wall point half a cell through the face
 // (bdyRotation / bdySpiral, MaterKit/inc/mkFluidsLBM.h:64-65).
-__device__ inline void wall_rotation(const StepArgs& a, int f, const Cell& c, double* t) {
+template <class W>
+__device__ inline void wall_rotation(const W& a, int f, const Cell& c, double* t) {
     const int ax = f >> 1;
     const double sg = (f & 1) ? 0.5 : -0.5;
     const double r0 = c.x + (ax == 0 ? sg : 0.0) - a.wall_center[0];
@@ -133,8 +134,10 @@ __device__ inline void wall_rotation(const StepArgs& a, int f, const Cell& c, do
     t[2] = ax == 2 ? 0.0 : w[0] * r1 - w[1] * r0;
 }
 
-// c_i . u_w for the wall of face f next to cell c: the face velocity plus the rotating part.
-__device__ inline double wall_cu(const StepArgs& a, int f, const Cell& c, int X, int Y, int Z) {
+// c_i . u_w for the wall of face f next to cell c: the face velocity plus the rotating part
+// (W: StepArgs or FaceTab, which hold the same wall fields).
+template <class W>
+__device__ inline double wall_cu(const W& a, int f, const Cell& c, int X, int Y, int Z) {
     double cu = face_cu(f, X, Y, Z, a.u_face);
     if (a.wall_rot) {
         double t[3];
@@ -278,24 +281,28 @@ struct FaceState {
 //   bounce     half-way bounce-back, plus 6 w rho0 c.u_w for a moving wall or inlet (flow)
 //   pressure   anti-bounce-back to the second-order equilibrium at rho0 and the cell's
 //              velocity u: 2 w rho0 (1 + 4.5 (c.u)^2 - 1.5 u.u) - f*_inv(i) (flow outlet)
+// The parameters come from the boundary table (a.ftab, entry L.fid), loaded where a face rule
+// runs.
 template <int Q, class R>
 __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const StepArgs& a,
                             int kind, const FaceState<R>& fs, const Cell& c) {
-    const int t = pick6(f, L.bc);
+    const FaceTab& T = *a.ftab;
+    const int t = T.bc[L.fid][f];
     // outflow, and the layer of a velocity face, which continues itself
     if (t == kBcOutflow || (t == kBcVelocity && kind == kKindFlow)) return p[i];
     if (t == kBcPressure) {
         const R cu = cx(i) * fs.ux + cy(i) * fs.uy + cz(i) * fs.uz;
         const R uu = fs.ux * fs.ux + fs.uy * fs.uy + fs.uz * fs.uz;
-        const R e = (R)a.rho0 * ((R)1 + (R)4.5 * cu * cu - (R)1.5 * uu);
-        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (e - (R)L.ref);
+        const R e = (R)T.rho0 * ((R)1 + (R)4.5 * cu * cu - (R)1.5 * uu);
+        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (e - (R)T.ref[L.fid]);
     }
     if (t == kBcDirichlet && kind == kKindScalar)
-        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)pick6(f, L.fval) - (R)L.ref);
+        return -p[inv(i)] +
+               (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)T.fval[L.fid][f] - (R)T.ref[L.fid]);
     R v = p[inv(i)];
     if (kind == kKindFlow) {
-        const double cu = wall_cu(a, f, c, cx(i), cy(i), cz(i));
-        v += (R)(6.0 * wgt<Q>(i) * a.rho0 * cu) * fs.wscale;
+        const double cu = wall_cu(T, f, c, cx(i), cy(i), cz(i));
+        v += (R)(6.0 * wgt<Q>(i) * T.rho0 * cu) * fs.wscale;
     }
     return v;
 }
@@ -349,7 +356,24 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, const Cell& c,
     if constexpr (!ODD) {
 #pragma unroll
         for (int i = 0; i < Q; i++) stp(pop_at(base, inv(i), c.id, a, L), p[i]);
-        if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
+        if constexpr (FACES && FM == (ZLO | ZHI)) {
+            // z faces alone: the lane at z = 0 (z = nz - 1) stores the populations that will
+            // enter through a non-periodic z- (z+) face, one branch per face with the face
+            // known at compile time
+            const uint32_t npz = (uint32_t)L.npmask & (ZLO | ZHI);
+            if ((npz & ZLO) && c.z == 0) {
+                const uint32_t k = face_slot(a, c, 4);
+#pragma unroll
+                for (int i = 1; i < Q; i++)
+                    if (cz(i) > 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, 4, L, a, kind, fs, c));
+            }
+            if ((npz & ZHI) && c.z == a.nz - 1) {
+                const uint32_t k = face_slot(a, c, 5);
+#pragma unroll
+                for (int i = 1; i < Q; i++)
+                    if (cz(i) < 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, 5, L, a, kind, fs, c));
+            }
+        } else if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
 #pragma unroll
             for (int i = 1; i < Q; i++) {
                 const uint32_t hit =

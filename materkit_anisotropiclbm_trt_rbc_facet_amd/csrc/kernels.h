@@ -20,8 +20,26 @@ struct LatArgs {
     int32_t adv;      // advance this lattice in this launch
     uint32_t pg_stride;  // elements from one page of this lattice to the next (FP64 kernels;
                          // the FP32 kernels take StepArgs::pg_stride, every lattice sharing pages)
+    int32_t fid;      // this lattice's entry of the boundary table (FaceTab): 0 flow, 1 phase
+                      // (every order parameter), 2 thermal, 3 solute
     double ref;       // deviation reference: stored value = f_i - w_i * ref
     double fval[6];   // Dirichlet face values (scalar lattices)
+};
+
+// Boundary parameters of the face rules (face_in) in device memory, a copy of the same fields
+// of a launch's StepArgs and lattices.  The face rules read them through StepArgs::ftab inside
+// the face branches.  Read as kernel arguments they are loaded at kernel entry and held in
+// scalar registers through the whole step, which spills scalar registers in the kernels with
+// face rules.
+struct FaceTab {
+    int32_t bc[4][6];     // by LatArgs::fid
+    int32_t wall_rot;
+    int32_t unused;
+    double fval[4][6];
+    double ref[4];
+    double u_face[6][3];
+    double rho0;
+    double wall_omega[3], wall_center[3];
 };
 
 struct StepArgs {
@@ -71,7 +89,16 @@ struct StepArgs {
     // adjoint tape of the step (phase field + thermal model): phi^n at tape[cell] and the
     // pre-collision thermal populations at tape[(1 + k) * S + cell]; null: no tape
     void* tape;
+    // device copy of the boundary parameters above (kernels with face rules)
+    const FaceTab* ftab;
 };
+
+// The boundary table of a launch's arguments (lattices f, g, h, c with fid 0..3).
+FaceTab facetab_of(const StepArgs& a);
+// A handle's device copy of its boundary table: allocated with the handle, uploaded where the
+// handle's parameters change (upload waits for the copy on stream s).
+hipError_t facetab_alloc(FaceTab** dev);
+hipError_t facetab_upload(FaceTab* dev, const StepArgs& a, hipStream_t s);
 
 // Population block of a handle: the slots of every lattice, then every lattice's Q x F face
 // buffer, in one allocation of `total` elements.  Paged (pages of 2^pg_shift cells) when the

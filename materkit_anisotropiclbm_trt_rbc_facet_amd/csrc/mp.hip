@@ -350,6 +350,7 @@ struct mkg_mp_ctx {
     int lay = 0;
     int zstage = 0;  // O steps take the z shift through LDS (blocks hold whole z rows)
     int xcd_chunk = 0;  // block order of the launches, fixed at creation
+    FaceTab* ftab = nullptr;  // boundary table of the face rules
 };
 
 namespace {
@@ -391,6 +392,9 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
     h->pb.apply(a);
     h->pb.lattice(a.f, h->pops, 0, h->esz);
     h->pb.lattice(a.g, h->pops, 1, h->esz);
+    a.f.fid = 0;
+    a.g.fid = 1;
+    a.ftab = h->ftab;
     m.rho_l = p.rho_l;
     m.rho_g = p.rho_g;
     m.nu_l = p.nu_l;
@@ -424,7 +428,7 @@ void mp_destroy_impl(mkg_mp_ctx* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->pops, h->phi, h->mu, h->phiu, h->flag, h->scratch, h->partials,
-                    h->red_out};
+                    h->red_out, h->ftab};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -513,6 +517,9 @@ int mkg_mp_create(const mkg_mp_params* p, mkg_mp_handle* out) {
     if ((e = hipMalloc((void**)&h->partials, (size_t)(h->nblk + 1) * 4 * sizeof(double))) != hipSuccess)
         return bad(e, "hipMalloc");
     if ((e = hipMalloc((void**)&h->red_out, 8 * sizeof(double))) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = facetab_alloc(&h->ftab)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = facetab_upload(h->ftab, mp_args(h).a, h->stream)) != hipSuccess)
+        return bad(e, "boundary table upload");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad(e, "hipStreamSynchronize");
     *out = h;
     return 0;

@@ -466,6 +466,7 @@ struct mkg_mpf_ctx {
     int lay = 0;              // AA layout of every lattice
     int zstage = 0;           // O steps take the z shift through LDS
     int xcd_chunk = 0;        // block order of the launches, fixed at creation
+    FaceTab* ftab = nullptr;  // boundary table of the face rules
     int64_t pf_count = 0;
     int has_walls = 0;
     // order parameter j's lattice descriptor (base, face buffer) on top of a filled one
@@ -519,12 +520,16 @@ MpfArgs mpf_args(const mkg_mpf_ctx* h) {
         L.ref = ref;
     };
     fill(a.g, 0, p.bc, -1.0, p.phi_face);
+    a.g.fid = 1;  // every order parameter has the phase lattice's faces
     for (int j = 0; j < MKG_MPF_MAX; j++) {
         m.gl[j] = a.g;
         if (j < p.n) h->lattice(m.gl[j], j);
     }
     if (h->h) fill(a.h, p.n, p.bc_h, p.T_ref, p.T_face);
     if (h->c) fill(a.c, p.n + 1, p.bc_c, 0.0, p.C_face);
+    a.h.fid = 2;
+    a.c.fid = 3;
+    a.ftab = h->ftab;
     a.phi_cur = h->phi[h->pf_count & 1];
     a.phi_prev = h->phi[(h->pf_count + 1) & 1];
     m.n = p.n;
@@ -626,7 +631,7 @@ void mpf_destroy_impl(mkg_mpf_ctx* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->pops, h->phi[0], h->phi[1], h->flag, h->scratch, h->partials,
-                    h->red_out};
+                    h->red_out, h->ftab};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -756,6 +761,9 @@ int mkg_mpf_create(const mkg_mpf_params* p, mkg_mpf_handle* out) {
     if ((e = hipMalloc((void**)&h->partials, (size_t)(h->nblk + 1) * 4 * sizeof(double))) != hipSuccess)
         return bad(e, "hipMalloc");
     if ((e = hipMalloc((void**)&h->red_out, 8 * sizeof(double))) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = facetab_alloc(&h->ftab)) != hipSuccess) return bad(e, "hipMalloc");
+    if ((e = facetab_upload(h->ftab, mpf_args(h).a, h->stream)) != hipSuccess)
+        return bad(e, "boundary table upload");
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad(e, "hipStreamSynchronize");
     *out = h;
     return 0;
