@@ -772,6 +772,34 @@ __host__ __device__ inline R aniso_eval(const Aniso<R>& an, R n0, R n1, R n2, R&
     return (R)1 - (R)3 * an.e4 + (R)4 * an.e4 * (q0 * q0 + q1 * q1 + q2 * q2);
 }
 
+// |v| and the unit vector v / |v| from s = |v|^2 > 1e-30 (a normal number).  FP32: one hardware
+// reciprocal square root (about 1 ulp), where the fast square root and divisions carry
+// range-scaling code; FP64: the exact square root and divisions.
+template <class R>
+__device__ inline R unit3(R s, R x, R y, R z, R& ux, R& uy, R& uz) {
+    if constexpr (sizeof(R) == 4) {
+        const R r = __builtin_amdgcn_rsqf(s);
+        ux = x * r;
+        uy = y * r;
+        uz = z * r;
+        return s * r;
+    } else {
+        const R n = sqrt(s);
+        ux = x / n;
+        uy = y / n;
+        uz = z / n;
+        return n;
+    }
+}
+// 1 / sqrt(s) for s > 1e-30 (FP32: the hardware reciprocal square root).
+template <class R>
+__device__ inline R inv_sqrt(R s) {
+    if constexpr (sizeof(R) == 4)
+        return __builtin_amdgcn_rsqf(s);
+    else
+        return (R)1 / sqrt(s);
+}
+
 // Normal component along `axis` of J = dE/dG, E = |G|^2 a_s(G/|G|)^2 / 2:
 //   J = a^2 G + |G| a (D - n (n.D)),  D = d a_s / d n,
 // evaluated from a lab-frame face gradient g (crystal frame through M when rot).
@@ -787,8 +815,8 @@ __device__ inline R face_flux(R g0, R g1, R g2, int axis, const Aniso<R>& an, co
     const R g2s = c0 * c0 + c1 * c1 + c2 * c2;
     const R ga = axis == 0 ? g0 : (axis == 1 ? g1 : g2);
     if (!(g2s > (R)1e-30)) return ga;
-    const R gn = sqrt(g2s);
-    const R n0 = c0 / gn, n1 = c1 / gn, n2 = c2 / gn;
+    R n0, n1, n2;
+    const R gn = unit3<R>(g2s, c0, c1, c2, n0, n1, n2);
     R D0, D1, D2;
     const R as = aniso_eval<R>(an, n0, n1, n2, D0, D1, D2);
     const R nD = n0 * D0 + n1 * D1 + n2 * D2;
@@ -851,9 +879,9 @@ __device__ inline R phase_source(const R (&P)[27], R upf, const Aniso<R>& an, R 
     R as = 1;
     const R gg = c0 * c0 + c1 * c1 + c2 * c2;
     if (gg > (R)1e-30) {
-        const R gn = sqrt(gg);
-        R D0, D1, D2;
-        as = aniso_eval<R>(an, c0 / gn, c1 / gn, c2 / gn, D0, D1, D2);
+        R n0, n1, n2, D0, D1, D2;
+        unit3<R>(gg, c0, c1, c2, n0, n1, n2);
+        as = aniso_eval<R>(an, n0, n1, n2, D0, D1, D2);
     }
     const R ph = PP(0, 0, 0);
     const R d = (R)1 - ph * ph;

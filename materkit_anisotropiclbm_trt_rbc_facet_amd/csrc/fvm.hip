@@ -210,9 +210,9 @@ __global__ void __launch_bounds__(256) k_fvm_phase_march(const StepArgs a, const
         R as = 1;
         const R gg = gr0 * gr0 + gr1 * gr1 + gr2 * gr2;
         if (gg > (R)1e-30) {
-            const R gn = sqrt(gg);
-            R D0, D1, D2;
-            as = aniso_eval<R>(an, gr0 / gn, gr1 / gn, gr2 / gn, D0, D1, D2);
+            R n0, n1, n2, D0, D1, D2;
+            unit3<R>(gg, gr0, gr1, gr2, n0, n1, n2);
+            as = aniso_eval<R>(an, n0, n1, n2, D0, D1, D2);
         }
         const R d = (R)1 - p0 * p0;
         const R Qs = (p0 - (R)a.lambda * U * d) * d;
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(256) k_fvm_solute(const StepArgs a, const R* _
                 if (gg > (R)1e-20) {
                     const R Uf = (R)0.5 * (U0 + Un);
                     const R df = (R)0.5 * (d0 + (*slot_at(phin, nb) - pn));
-                    const R ja = -atw * ((R)1 + km * Uf) * df * g[ax] / sqrt(gg);
+                    const R ja = -atw * ((R)1 + km * Uf) * df * g[ax] * inv_sqrt<R>(gg);
                     J += (R)s * ja;
                 }
             }
@@ -353,7 +353,7 @@ __device__ inline R sol_face(const StepArgs& a, const R (&P)[27], int ax, int s,
         if (gg > (R)1e-20) {
             const R Uf = (R)0.5 * (U0 + Un);
             const R df = (R)0.5 * (d0 + dn);
-            J += (R)s * (-atw * ((R)1 + km * Uf) * df * g[ax] / sqrt(gg));
+            J += (R)s * (-atw * ((R)1 + km * Uf) * df * g[ax] * inv_sqrt<R>(gg));
         }
     }
 #undef PP
