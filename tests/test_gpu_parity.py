@@ -383,3 +383,41 @@ def test_fp32_z_faces_only_close_to_oracle(case):
     dev.close()
     for k, e in errs.items():
         assert e < 1e-5, f"{k}: rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("bcs,walls", [("zwalls", False), ("mixed", True), ("velocity", False)])
+@pytest.mark.parametrize("first", [3, 4])
+def test_set_params_changes_face_values_mid_run(bcs, walls, first):
+    """Wall velocities and Dirichlet values changed through set_params act from the next step
+    on, as in the oracle run with the new parameters: the face rules read the handle's
+    boundary table, which set_params uploads again.  nz = 64 (z stage); after an odd and an
+    even number of steps."""
+    p = make_params(flow_q=19, phase_q=19, thermal=1, op="trt", bcs=bcs, prec=mkg.PREC_FP64,
+                    dims=ZDIMS)
+    st = random_state(p, seed=4, with_walls=walls)
+    orc = load_oracle(p, st)
+    dev = load_device(p, st)
+    print(f"[parity] set_params bcs={bcs} walls={walls} first={first}", flush=True)
+    dev.step(first, mkg.ALL)
+    orc.step(first, 15, O.OP_TRT)
+    p2 = p.copy()
+    p2.u_face[5][0], p2.u_face[5][1] = -0.015, 0.01
+    if bcs in ("mixed", "velocity"):
+        p2.u_face[0][0] = 0.005
+    if bcs == "velocity":
+        p2.wall_omega[0] = 0.5 * p.wall_omega[0] - 1e-4
+    p2.T_face[4] = 0.35
+    stale = load_oracle(p, st)  # the same run kept on the old parameters
+    stale.step(first + 5, 15, O.OP_TRT)
+    dev.set_params(p2)
+    orc.params = p2.copy()
+    dev.step(5, mkg.ALL)
+    orc.step(5, 15, O.OP_TRT)
+    # the change is visible: a device still on the old face values would fail by far
+    assert max(compare_pops(dev, stale).values()) > 1e-6
+    errs = compare_pops(dev, orc)
+    flags = dev.get(mkg.F_FLAG)
+    dev.close()
+    assert np.array_equal(flags, orc.flag)
+    for k, e in errs.items():
+        assert e < 1e-12, f"{k}: rel err {e:.3e}"
