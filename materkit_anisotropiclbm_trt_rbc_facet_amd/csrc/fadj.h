@@ -144,13 +144,17 @@ __device__ inline void collide_reverse(const StepArgs& a, const R* f, const R (&
         CTl = ((R)0.5 * mo + ((R)1 - (R)0.5 * s_nu) * sh + ((R)1 - (R)0.5 * s_b) * bk +
                ((R)1 - (R)0.5 * s_o) * go + ((R)1 - (R)0.5 * s_e) * ge) * iw;
     };
+    // A^T l waits in LDS (each thread reads back only its own slots) until rb and ubar are
+    // known: a store, load and store again of lam_out would make every direction wait for its
+    // own global store
+    __shared__ R atl[Q][256];
+    const uint32_t tid = threadIdx.x;
     R rb = 0, ub0 = ub_ext[0], ub1 = ub_ext[1], ub2 = ub_ext[2], q1 = 0, q2 = 0, q3 = 0;
 #pragma unroll
     for (int i = 0; i < Q; i++) {
         R ATl, CTl;
         atl_ctl(i, ATl, CTl);
-        // the final adjoint's A^T l part now, the rest once rb and ubar are known
-        lam_out[(size_t)i * N + id] = ATl;
+        atl[i][tid] = ATl;
         const R w = (R)wgt<Q>(i);
         const R X = (R)cx(i), Y = (R)cy(i), Z = (R)cz(i);
         const R cu = X * u0 + Y * u1 + Z * u2;
@@ -169,10 +173,9 @@ __device__ inline void collide_reverse(const StepArgs& a, const R* f, const R (&
     dF[1] = q2 + (R)0.5 * ub1 * ir;
     dF[2] = q3 + (R)0.5 * ub2 * ir;
 #pragma unroll
-    for (int i = 0; i < Q; i++) {
-        R* o = &lam_out[(size_t)i * N + id];
-        *o = *o + rb + (ub0 * (cx(i) - u0) + ub1 * (cy(i) - u1) + ub2 * (cz(i) - u2)) * ir;
-    }
+    for (int i = 0; i < Q; i++)
+        lam_out[(size_t)i * N + id] =
+            atl[i][tid] + rb + (ub0 * (cx(i) - u0) + ub1 * (cy(i) - u1) + ub2 * (cz(i) - u2)) * ir;
 }
 
 }  // namespace mkg

@@ -66,8 +66,7 @@ __global__ void __launch_bounds__(256) k_fadj_loss(const StepArgs a, const doubl
 // One reverse step: lam_in = adjoint of level n+1, tape = level n; partials 1..3 (dL/dF) and
 // 4..7 (dL/d s_nu, s_bulk, s_odd, s_even) accumulate.
 // Minimum waves per SIMD asked of the register allocator for the FP32 reverse step: two
-// (240 VGPRs, no spills) measured faster than three or four, which spill (76 and 232 bytes
-// of scratch per lane), at 128 x 64 x 64.
+// (197 VGPRs, no spills); three spill 76 bytes of scratch per lane.
 #ifndef MKG_FADJ_WAVES_F32
 #define MKG_FADJ_WAVES_F32 2
 #endif
