@@ -289,9 +289,10 @@ int mkg_enable_timing(mkg_handle h, int on);
  * of run_flow_thermal_dendrite, MaterKit/MaterKit.cpp:500-518: the melt advects both scalars,
  * feels the Boussinesq buoyancy and sees the solid through partial bounce-back; compressible
  * equilibrium, periodic / bounce / outflow flow faces): a forward pass with checkpoints every
- * ckpt steps, the FP64 loss L = 1/2 sum (phi_N - phi_target)^2 with phi_N = sum_i g_i after
- * the last step, then the reverse sweep with per-segment recompute (segment s - 1 recomputed
- * on a second stream while segment s is reversed).  ckpt = 0 keeps a tape instead: the
+ * ckpt steps (the last segment writes its tape as it steps), the FP64 loss
+ * L = 1/2 sum (phi_N - phi_target)^2 with phi_N = sum_i g_i after the last step, then the
+ * reverse sweep with per-segment recompute of the earlier segments (segment s - 1 recomputed
+ * while segment s is reversed on a second stream).  ckpt = 0 keeps a tape instead: the
  * forward pass stores what the reverse sweep reads for every step (phi^n and the thermal
  * populations, 8 values per cell and step, plus the flow populations in FP64) and nothing is
  * recomputed.  Outputs L, dL/deps4 and (optional, needs has_kappa) dL/dkappa per cell (N

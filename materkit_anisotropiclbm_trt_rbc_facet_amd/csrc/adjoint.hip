@@ -439,7 +439,9 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? MKG_ADJ_WAVES_F32 : 1)
             const R F0 = (R)a.force[0] + (R)(a.rho0 * a.buoy[0]) * dT;
             const R F1 = (R)a.force[1] + (R)(a.rho0 * a.buoy[1]) * dT;
             const R F2 = (R)a.force[2] + (R)(a.rho0 * a.buoy[2]) * dT;
-            const int8_t fl = a.flag[id];
+            // the flag of level n, from phi^n of the tape as the forward's moment pass set it
+            // (the flag array itself may hold another level while a segment is recomputed)
+            const int8_t fl = classify(a.flag[id], (double)phi, a.phi_s, a.pf_model);
             R ns = 0;
             if (fl == kIsSolid) ns = (R)1;
             else if (fl == kIsInter) ns = fs;
@@ -704,7 +706,8 @@ __global__ void __launch_bounds__(256, sizeof(R) == 4 ? 2 : 1)
     for (int k = 0; k < 7; k++) sh += h[k];
     const R T = (R)a.h.ref + sh;
     const R phi = *slot_at((const R*)j.phin, id);
-    const int8_t fl = a.flag[id];
+    // the flag of level n from phi^n, as in k_adj_collide
+    const int8_t fl = classify(a.flag[id], (double)phi, a.phi_s, a.pf_model);
     R ns = 0, dns_dphi = 0;
     if (fl == kIsSolid) {
         ns = (R)1;

@@ -1186,9 +1186,10 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     };
     std::vector<Ck> ck(tape ? 1 : nseg + 1);
     // a checkpoint: every population of the handle (the phase, thermal and, in the coupled
-    // model, flow lattices) and both phase buffers
+    // model, flow lattices), both phase buffers and the cell flags (S bytes, rounded up to
+    // whole elements)
     const size_t PP = h->pop_total;
-    const size_t ck_elems = PP + 2 * S;
+    const size_t ck_elems = PP + 2 * S + (S + e - 1) / e;
     // flow tape of one step (coupled model): the canonical pre-collision populations, FP64
     const size_t fseg_stride = flow ? (size_t)19 * N : 0;
     // tape of one step, written by the forward kernel: phi^n, then the 7 pre-collision thermal
@@ -1270,6 +1271,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         HIPCHK(hipMemcpyAsync(base, h->popbuf, PP * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + PP * e, h->phi[0], S * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(base + (PP + S) * e, h->phi[1], S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(base + (PP + 2 * S) * e, h->flag, S, hipMemcpyDeviceToDevice, h->stream));
         ck[k] = {h->lay[LG], h->lay[LH], h->lay[LF], h->pf_count, h->steps[LG], h->steps[LH],
                  h->steps[LF]};
         return 0;
@@ -1279,6 +1281,7 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         HIPCHK(hipMemcpyAsync(h->popbuf, base, PP * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->phi[0], base + PP * e, S * e, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->phi[1], base + (PP + S) * e, S * e, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->flag, base + (PP + 2 * S) * e, S, hipMemcpyDeviceToDevice, h->stream));
         h->lay[LG] = ck[k].lay_g;
         h->lay[LH] = ck[k].lay_h;
         h->pf_count = ck[k].pf;
@@ -1316,10 +1319,18 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
         return 0;
     };
     ADJCHK(hipEventRecord(ev[0], h->stream));
-    // ---- forward pass: checkpoints, or the tape
+    // ---- forward pass: checkpoints, or the tape.  With checkpoints, the last segment writes
+    // its tape as it steps (its recompute would repeat exactly these steps, and the reverse
+    // sweep starts with it), so it needs no checkpoint of its own; every earlier segment
+    // keeps a checkpoint and is recomputed.
+    const int last0 = (nseg - 1) * ckpt;
     for (int n = 0; n < nsteps; n++) {
         if (tape) {
             if (stepper(n, 0, segbuf, fseg)) { rc = -1; break; }
+            continue;
+        }
+        if (n >= last0) {
+            if (stepper(n, last0, seg_base(nseg - 1), fseg_base(nseg - 1))) { rc = -1; break; }
             continue;
         }
         if (n % ckpt == 0 && save_ck(n / ckpt)) { rc = -1; break; }
@@ -1343,8 +1354,12 @@ int mkg_adjoint_run(mkg_handle h, int nsteps, int ckpt, const double* target, do
     // selects the per-direction form)
     int grouped = h->zstage;
     if (const char* env = std::getenv("MKG_ADJ_GROUPED")) grouped = grouped && std::atoi(env) != 0;
+    // The last segment's tape came with the forward pass: nothing is recomputed before the
+    // reverse sweep starts (ev[2]..ev[3] keep the recompute slot of the timing report).  Its
+    // "tape filled" event is recorded here, after the loss kernel has written the seed that
+    // the reverse sweep reads first.
     ADJCHK(hipEventRecord(ev[2], h->stream));
-    if (!tape && recompute(nseg - 1)) rc = -1;
+    if (!tape) ADJCHK(hipEventRecord(ev[5 + (nseg - 1) % 2], h->stream));
     ADJCHK(hipEventRecord(ev[3], h->stream));
     for (int sgi = nseg - 1; sgi >= 0 && rc == 0; sgi--) {
         const int n0 = sgi * ckpt, n1 = std::min(nsteps, n0 + ckpt);

@@ -135,3 +135,32 @@ def test_coupled_fp32_adjoint_close_to_fp64(target):
     e2 = np.linalg.norm(r[1]["dL_dkappa"] - r[0]["dL_dkappa"]) / np.linalg.norm(r[0]["dL_dkappa"])
     print(f"[coupled adjoint] fp32 vs fp64: dL/deps4 {e1:.2e}, dL/dkappa {e2:.2e}", flush=True)
     assert e1 < 1e-3 and e2 < 1e-3
+
+
+def test_coupled_checkpoint_spacing_does_not_change_the_gradient(target):
+    """Checkpoints every 7, 30 or 100 steps, or the tape, give the same coupled gradient while
+    cells change class during the run (their partial bounce-back n_s changes): the reverse takes
+    each level's flags from phi^n of its tape, not from the flag array, which the recompute of
+    the previous segment rewrites meanwhile."""
+    p = params(bcs="walls")
+    p.phi_s = 0.5  # class thresholds inside the interface: its cells change class as it moves
+    kap = kappa_field(3)
+    nsteps = 120
+    outs, finals = [], []
+    for ck in (0, 7, 30, 100):
+        d = make_device(p, kap)
+        f0 = d.get(mkg.F_FLAG)
+        outs.append(d.adjoint_run(nsteps, ck, target))
+        # the handle is left at the final forward state, flags included
+        finals.append((d.get(mkg.F_FLAG), d.get(mkg.F_POP_FLOW)))
+        changed = int((finals[-1][0] != f0).sum())
+        d.close()
+        print(f"[coupled adjoint] ckpt={ck} cells that changed class: {changed}", flush=True)
+        assert changed > 0
+    for fl, fp in finals[1:]:
+        np.testing.assert_array_equal(fl, finals[0][0])
+        np.testing.assert_array_equal(fp, finals[0][1])
+    for o in outs[1:]:
+        assert abs(o["dL_deps4"] - outs[0]["dL_deps4"]) <= 1e-12 * abs(outs[0]["dL_deps4"])
+        assert np.abs(o["dL_dkappa"] - outs[0]["dL_dkappa"]).max() <= 1e-12 * np.abs(
+            outs[0]["dL_dkappa"]).max()
