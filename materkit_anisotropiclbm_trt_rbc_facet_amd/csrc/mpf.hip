@@ -56,31 +56,57 @@ struct MpfArgs {
     double D_l, D_s, k_part, q_min, latent_C, at_w;
 };
 
-template <class R, bool ODD>
+// ZS (with ODD): the block holds whole z rows, and the z-moving populations of every order
+// parameter are loaded at the cell's own z and shifted through LDS as in k_phase_moments (two
+// LDS regions in turn, one barrier per order parameter; every thread takes part).  PER: no
+// face rules (every face periodic, no wall cells).
+template <class R, bool ODD, bool ZS = false, bool PER = false>
 __global__ void __launch_bounds__(256) k_mpf_moments(const MpfArgs m, R* __restrict__ out) {
+    constexpr bool Z = ODD && ZS;
+    __shared__ R zv[Z ? 2 : 1][Z ? nzdirs<19>() : 1][256];
     Cell c;
-    if (!locate(m.a, c)) return;
+    bool act = true;
+    if constexpr (Z) {
+        act = locate_clamped(m.a, c);
+    } else {
+        if (!locate(m.a, c)) return;
+    }
     const uint32_t S = m.a.S;
     const R* __restrict__ prev = (const R*)m.a.phi_prev;
-    if (m.a.flag[c.id] == kIsWalls) {
+    const bool wall = m.a.flag[c.id] == kIsWalls;
+    if (!Z && wall) {
         for (int j = 0; j <= m.n; j++)
             *slot_at(out + (size_t)j * S, c.id) = *slot_at(prev + (size_t)j * S, c.id);
         return;
     }
+    const bool put = act && !wall;
     R comb = (R)(m.n - 1);
 #pragma unroll
     for (int j = 0; j < MKG_MPF_MAX; j++) {
         if (j < m.n) {
             const LatArgs& L = m.gl[j];  // j is unrolled: a fixed offset, no copy
             R g[19];
-            read_pops<19, ODD, R>(g, L, c, m.a, kKindScalar);
+            if constexpr (Z) {
+                const uint32_t own = zpull_issue<19, R, !PER>(g, L, c, m.a, zv[j & 1], threadIdx.x);
+                __syncthreads();
+                zpull_finish<19, R>(g, c, m.a, zv[j & 1], threadIdx.x, own);
+            } else {
+                read_pops<19, ODD, R, !PER>(g, L, c, m.a, kKindScalar);
+            }
             R s = 0;
 #pragma unroll
             for (int i = 0; i < 19; i++) s += g[i];
             const R ph = (R)L.ref + s;
-            *slot_at(out + (size_t)j * S, c.id) = ph;
+            if (put) *slot_at(out + (size_t)j * S, c.id) = ph;
             comb += ph;
         }
+    }
+    if (!act) return;
+    if (wall) {
+        // wall cells keep their order parameters: carry them into the buffer being written
+        for (int j = 0; j <= m.n; j++)
+            *slot_at(out + (size_t)j * S, c.id) = *slot_at(prev + (size_t)j * S, c.id);
+        return;
     }
     *slot_at(out + (size_t)m.n * S, c.id) = comb;
 }
@@ -433,7 +459,11 @@ __global__ void __launch_bounds__(256) k_mpf_init(const MpfArgs m, const double*
 template <class R>
 int mpf_launch_moments(const MpfArgs& m, int odd, void* phi_out, hipStream_t s) {
     dim3 grid(m.a.nblk), block(256);
-    if (odd) {
+    const bool per = !m.a.has_walls && m.a.g.npmask == 0;
+    if (odd && m.a.zstage) {
+        if (per) k_mpf_moments<R, true, true, true><<<grid, block, 0, s>>>(m, (R*)phi_out);
+        else k_mpf_moments<R, true, true><<<grid, block, 0, s>>>(m, (R*)phi_out);
+    } else if (odd) {
         k_mpf_moments<R, true><<<grid, block, 0, s>>>(m, (R*)phi_out);
     } else {
         k_mpf_moments<R, false><<<grid, block, 0, s>>>(m, (R*)phi_out);
