@@ -283,28 +283,58 @@ struct FaceState {
 //              velocity u: 2 w rho0 (1 + 4.5 (c.u)^2 - 1.5 u.u) - f*_inv(i) (flow outlet)
 // The parameters come from the boundary table (a.ftab, entry L.fid), loaded where a face rule
 // runs.
+// The parameters of one lattice's rule on one face, taken once for all the directions that
+// cross the face: the face code, the table values, the face velocity and the rotating part of
+// the wall velocity at the cell.
+struct FaceRule {
+    int t;
+    bool rotating;
+    double ref, fval, rho0;
+    double u[3], rot[3];
+};
+__device__ inline FaceRule face_rule(const StepArgs& a, const LatArgs& L, int f, const Cell& c) {
+    const FaceTab& T = *a.ftab;
+    FaceRule r;
+    r.t = T.bc[L.fid][f];
+    r.ref = T.ref[L.fid];
+    r.fval = T.fval[L.fid][f];
+    r.rho0 = T.rho0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.u[k] = T.u_face[f][k];
+    r.rotating = T.wall_rot != 0;
+    if (r.rotating)
+        wall_rotation(T, f, c, r.rot);
+    else
+        r.rot[0] = r.rot[1] = r.rot[2] = 0.0;
+    return r;
+}
+
+template <int Q, class R>
+__device__ inline R face_in(const R* p, int i, const FaceRule& r, int kind,
+                            const FaceState<R>& fs) {
+    // outflow, and the layer of a velocity face, which continues itself
+    if (r.t == kBcOutflow || (r.t == kBcVelocity && kind == kKindFlow)) return p[i];
+    if (r.t == kBcPressure) {
+        const R cu = cx(i) * fs.ux + cy(i) * fs.uy + cz(i) * fs.uz;
+        const R uu = fs.ux * fs.ux + fs.uy * fs.uy + fs.uz * fs.uz;
+        const R e = (R)r.rho0 * ((R)1 + (R)4.5 * cu * cu - (R)1.5 * uu);
+        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (e - (R)r.ref);
+    }
+    if (r.t == kBcDirichlet && kind == kKindScalar)
+        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)r.fval - (R)r.ref);
+    R v = p[inv(i)];
+    if (kind == kKindFlow) {
+        double cu = cx(i) * r.u[0] + cy(i) * r.u[1] + cz(i) * r.u[2];
+        if (r.rotating) cu += cx(i) * r.rot[0] + cy(i) * r.rot[1] + cz(i) * r.rot[2];
+        v += (R)(6.0 * wgt<Q>(i) * r.rho0 * cu) * fs.wscale;
+    }
+    return v;
+}
+
 template <int Q, class R>
 __device__ inline R face_in(const R* p, int i, int f, const LatArgs& L, const StepArgs& a,
                             int kind, const FaceState<R>& fs, const Cell& c) {
-    const FaceTab& T = *a.ftab;
-    const int t = T.bc[L.fid][f];
-    // outflow, and the layer of a velocity face, which continues itself
-    if (t == kBcOutflow || (t == kBcVelocity && kind == kKindFlow)) return p[i];
-    if (t == kBcPressure) {
-        const R cu = cx(i) * fs.ux + cy(i) * fs.uy + cz(i) * fs.uz;
-        const R uu = fs.ux * fs.ux + fs.uy * fs.uy + fs.uz * fs.uz;
-        const R e = (R)T.rho0 * ((R)1 + (R)4.5 * cu * cu - (R)1.5 * uu);
-        return -p[inv(i)] + (R)(2.0 * wgt<Q>(i)) * (e - (R)T.ref[L.fid]);
-    }
-    if (t == kBcDirichlet && kind == kKindScalar)
-        return -p[inv(i)] +
-               (R)(2.0 * wgt<Q>(i)) * (fs.dscale * (R)T.fval[L.fid][f] - (R)T.ref[L.fid]);
-    R v = p[inv(i)];
-    if (kind == kKindFlow) {
-        const double cu = wall_cu(T, f, c, cx(i), cy(i), cz(i));
-        v += (R)(6.0 * wgt<Q>(i) * T.rho0 * cu) * fs.wscale;
-    }
-    return v;
+    return face_in<Q, R>(p, i, face_rule(a, L, f, c), kind, fs);
 }
 
 // AA access.  E step (ODD false): read own slot i, write own slot inv(i).  O step: read slot
@@ -363,15 +393,17 @@ __device__ inline void write_pops(const R* p, const LatArgs& L, const Cell& c,
             const uint32_t npz = (uint32_t)L.npmask & (ZLO | ZHI);
             if ((npz & ZLO) && c.z == 0) {
                 const uint32_t k = face_slot(a, c, 4);
+                const FaceRule fr = face_rule(a, L, 4, c);
 #pragma unroll
                 for (int i = 1; i < Q; i++)
-                    if (cz(i) > 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, 4, L, a, kind, fs, c));
+                    if (cz(i) > 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, fr, kind, fs));
             }
             if ((npz & ZHI) && c.z == a.nz - 1) {
                 const uint32_t k = face_slot(a, c, 5);
+                const FaceRule fr = face_rule(a, L, 5, c);
 #pragma unroll
                 for (int i = 1; i < Q; i++)
-                    if (cz(i) < 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, 5, L, a, kind, fs, c));
+                    if (cz(i) < 0) stp(face_at<R>(L, i, a.F, k), face_in<Q, R>(p, i, fr, kind, fs));
             }
         } else if (FACES && (c.bits & FM & (uint32_t)L.npmask)) {
 #pragma unroll
@@ -537,16 +569,18 @@ __device__ inline void zpush_stage(const R* p, const LatArgs& L, const Cell& c,
         }
         const uint32_t npz = (uint32_t)L.npmask & (ZLO | ZHI);
         if ((npz & ZLO) && c.z == 0 && act) {
+            const FaceRule fr = face_rule(a, L, 4, c);
 #pragma unroll
             for (int i = 1; i < Q; i++)
                 if (cz(i) < 0)
-                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), 4, L, a, kind, fs, c));
+                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), fr, kind, fs));
         }
         if ((npz & ZHI) && c.z == a.nz - 1 && act) {
+            const FaceRule fr = face_rule(a, L, 5, c);
 #pragma unroll
             for (int i = 1; i < Q; i++)
                 if (cz(i) > 0)
-                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), 5, L, a, kind, fs, c));
+                    stp(pop_at(base, inv(i), c.id, a, L), face_in<Q, R>(p, inv(i), fr, kind, fs));
         }
         return;
     }
