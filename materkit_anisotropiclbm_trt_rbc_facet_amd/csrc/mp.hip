@@ -87,12 +87,25 @@ __device__ inline void mp_stencil(const MpArgs& m, const Cell& c, const R* v, R 
     }
 }
 
-template <class R, bool ODD>
+// ZS (with ODD): the block holds whole z rows, and the z-moving populations are loaded at the
+// cell's own z and shifted through LDS (every thread takes part in the barrier).  PER: no face
+// rules (every face of the lattice periodic, no wall cells).
+template <class R, bool ODD, bool ZS = false, bool PER = false>
 __global__ void __launch_bounds__(256) k_mp_phi(const MpArgs m) {
+    constexpr bool Z = ODD && ZS;
+    __shared__ R zv[Z ? nzdirs<19>() : 1][256];
     Cell c;
-    if (!locate(m.a, c)) return;
     R h[19];
-    read_pops<19, ODD, R>(h, m.a.g, c, m.a, kKindScalar);
+    if constexpr (Z) {
+        const bool act = locate_clamped(m.a, c);
+        const uint32_t own = zpull_issue<19, R, !PER>(h, m.a.g, c, m.a, zv, threadIdx.x);
+        __syncthreads();
+        zpull_finish<19, R>(h, c, m.a, zv, threadIdx.x, own);
+        if (!act) return;
+    } else {
+        if (!locate(m.a, c)) return;
+        read_pops<19, ODD, R>(h, m.a.g, c, m.a, kKindScalar);
+    }
     R s = 0;
 #pragma unroll
     for (int i = 0; i < 19; i++) s += h[i];
@@ -487,8 +500,15 @@ MpArgs mp_args(const mkg_mp_ctx* h) {
 template <class R>
 void launch_phi_mu(const MpArgs& m, int odd, hipStream_t s) {
     dim3 grid(m.a.nblk), block(256);
-    if (odd) k_mp_phi<R, true><<<grid, block, 0, s>>>(m);
-    else k_mp_phi<R, false><<<grid, block, 0, s>>>(m);
+    const bool per = !m.a.has_walls && m.a.g.npmask == 0;
+    if (odd && m.a.zstage) {
+        if (per) k_mp_phi<R, true, true, true><<<grid, block, 0, s>>>(m);
+        else k_mp_phi<R, true, true><<<grid, block, 0, s>>>(m);
+    } else if (odd) {
+        k_mp_phi<R, true><<<grid, block, 0, s>>>(m);
+    } else {
+        k_mp_phi<R, false><<<grid, block, 0, s>>>(m);
+    }
     if (m.a.zstage) k_mp_mu_z<R><<<grid, block, 0, s>>>(m);
     else k_mp_mu<R><<<grid, block, 0, s>>>(m);
 }
